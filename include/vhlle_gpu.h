@@ -1,0 +1,162 @@
+/*
+ * vhlle_gpu.h -- C-ABI of the MI355X-native vHLLE hydro step.
+ *
+ * The reference (yukarpenko/vhlle) exposes its hot path as a C++ class API:
+ * Hydro::performStep (src/hdo.cpp:1016-1134) acting on a Fluid of Cells
+ * (src/fld.h:10-88, src/cll.h:38-267) with an EoS (src/eos.h:21-38) and a
+ * TransportCoeff (src/trancoeff.h:6-45).  This header is the boundary BELOW
+ * that class API: plain C, POD structs, host pointers and sizes, integer
+ * status codes.  The C++ facade (vhlle_amd/host/vhlle.h: Fluid, Hydro, Cell, EoS,
+ * TransportCoeff) is written on top of it, and any other host language can
+ * bind it directly.
+ *
+ * State layout crossing the boundary: structure-of-arrays, component-major,
+ * arr[comp*N + cell], cell = ix + nx*(iy + ny*iz) -- the cell order of
+ * Fluid::getCell (src/fld.h:70-78).  pi-like fields hold the 10 independent
+ * components of the symmetric tensor in the order of index44 (src/cll.cpp:22-31).
+ *
+ * Every call is synchronous with respect to the host unless stated otherwise;
+ * the context owns one HIP stream.  Nothing here calls exit(): failures are
+ * returned as VH_* codes and the C++ facade maps them to the reference's exits.
+ */
+#ifndef VHLLE_GPU_H
+#define VHLLE_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+enum {
+  VH_OK = 0,
+  VH_ERR_ARG = -1,        /* bad argument (null pointer, size, mode) */
+  VH_ERR_HIP = -2,        /* a HIP runtime call failed */
+  VH_ERR_NODEVICE = -3,   /* no GPU visible */
+  VH_ERR_DTAU = -4        /* Hydro::setDtau would exit(1) (src/hdo.cpp:67-75) */
+};
+/* step status bits (vh_step / vh_run *status), the reference's exits: */
+enum {
+  VH_ST_NAN_FLUX = 1,     /* hlle_flux f_nb NaN -> exit(1)  (src/hdo.cpp:285-298) */
+  VH_ST_BIG_E = 2,        /* e > 1e10 -> exit(0)            (src/hdo.cpp:118-130) */
+  VH_ST_NAN_SOURCE = 4,   /* source_step k_nb NaN -> exit(1)(src/hdo.cpp:345-351) */
+  VH_ST_NAN_PI = 8        /* NSquant pi inf/nan -> exit(1)  (src/hdo.cpp:643-651) */
+};
+
+/* precision modes of vh_create */
+enum {
+  VH_PREC_FP64 = 0,       /* faithful FP64: matches the reference to rounding */
+  VH_PREC_MIXED = 1       /* FP32 fluxes/viscous sources, FP64 Q accumulation */
+};
+
+/* con2prim modes (vh_set_c2p_mode) */
+enum {
+  VH_C2P_REFERENCE = 0,   /* transformPV's bisection/Newton from v=0.5 */
+  VH_C2P_LEARNED = 1,     /* MLP initial guess + FP64 Newton, fallback to 0 */
+  VH_C2P_CLOSED = 2       /* closed-form conformal root + FP64 Newton polish */
+};
+
+/* Fluid geometry: the Fluid ctor arguments (src/fld.cpp:62-80).
+ * dx = (xmax-xmin)/(nx-1) etc.; nz = 1 gives dz = inf as in the reference. */
+typedef struct {
+  int nx, ny, nz;
+  double xmin, xmax, ymin, ymax, zmin, zmax;
+} vh_grid_t;
+
+/* EoS descriptor replacing the virtual EoS (src/eos.h:21-38).
+ * kind 0: ideal massless gas p=e/3, T=(e/cnst)^(1/4), mu=0, the SIMPLE branch
+ * of EoSs (src/eos.cpp:87-111). */
+enum { VH_EOS_IDEAL = 0 };
+typedef struct {
+  int kind;
+} vh_eos_t;
+
+/* TransportCoeff (src/trancoeff.h:8-11, ctor :14-20) as a POD, plus the
+ * T-peaked bulk parametrisation zetaSparam = 5 (this build):
+ * zeta/s = zetaS0 * exp(-(T-zetaSPeakT)^2 / (2 w^2)), w = zetaSWidthMinus
+ * below the peak and zetaSWidthPlus above it. */
+typedef struct {
+  double etaS0, zetaS0, ah, al, aRho, T0, etaSMin, etaSEpsilonMin,
+      etaSShiftMuB, etaSScaleMuB, zetaSPeakEpsilon, zetaSScaleBeta,
+      zetaSSigmaMinus, zetaSSigmaPlus;
+  int etaSparam, zetaSparam;
+  double zetaSPeakT, zetaSWidthMinus, zetaSWidthPlus;
+} vh_trcoeff_t;
+
+/* Host view of the whole per-cell state (SoA, see above).  On upload a NULL
+ * field is set to the Cell ctor default (src/cll.cpp:33-52: zeros,
+ * viscCorrCut = 1); on download a NULL field is skipped. */
+typedef struct {
+  double *Q, *Qh, *Qprev;   /* 7 x N each: Cell::Q, Qh, Qprev (src/cll.h:44-46) */
+  double *pi, *piH;         /* 10 x N each: Cell::pi, piH (src/cll.h:47) */
+  double *Pi, *PiH;         /* N each (src/cll.h:49) */
+  double *m;                /* 3 x N: Cell::m (src/cll.h:57) */
+  double *viscCorrCut;      /* N: Cell::viscCorrCut (src/cll.h:60) */
+} vh_state_t;
+
+typedef struct vh_ctx vh_ctx_t;
+
+/* Creates a context (device buffers for one Fluid + one Hydro).  Replaces
+ * Fluid::Fluid (src/fld.cpp:62-112; the neighbour links become index clamps)
+ * and Hydro::Hydro (src/hdo.cpp:50-57). */
+int vh_create(const vh_grid_t *grid, const vh_eos_t *eos,
+              const vh_trcoeff_t *trc, int precision, vh_ctx_t **out);
+/* Fluid::~Fluid (src/fld.cpp:114-117) */
+int vh_destroy(vh_ctx_t *ctx);
+/* number of cells nx*ny*nz */
+long vh_ncell(const vh_ctx_t *ctx);
+
+/* Writes the whole state (the cells' fields as the ICs leave them through
+ * Cell::setPrimVar / saveQprev / setAllM, src/cll.cpp:222-241). */
+int vh_upload(vh_ctx_t *ctx, const vh_state_t *host);
+/* Reads the state back (what host code sees through Fluid::getCell after a
+ * step: Cell::getQ/getQprev/getpi/getPi/getM/getViscCorrCutFlag). */
+int vh_download(vh_ctx_t *ctx, vh_state_t *host);
+
+/* IC helper on the device: Cell::setPrimVar (src/cll.cpp:222-241) on every
+ * cell from primitive arrays (N each), then saveQprev and setAllM(1.) where
+ * e > 0 -- the body of ICGubser::setIC (src/icGubser.cpp:47-57).  nb/nq/ns
+ * may be NULL (zero). */
+int vh_set_prim(vh_ctx_t *ctx, double tau, const double *e, const double *nb,
+                const double *nq, const double *ns, const double *vx,
+                const double *vy, const double *vz);
+
+/* Hydro::performStep (src/hdo.cpp:1016-1134): one predictor-corrector step
+ * (+ Israel-Stewart step when viscous) from *tau with timestep dt; *tau is
+ * advanced by dt.  *status receives VH_ST_* bits (0 = the reference would
+ * have continued). */
+int vh_step(vh_ctx_t *ctx, double *tau, double dt, int *status);
+/* nsteps of vh_step without host synchronisation in between (the step
+ * sequence is replayed as a HIP graph); stops early only on the host side
+ * after all steps if *status != 0. */
+int vh_run(vh_ctx_t *ctx, double *tau, double dt, int nsteps, int *status);
+/* Hydro::setDtau validity test (src/hdo.cpp:67-75): VH_OK or VH_ERR_DTAU. */
+int vh_check_dtau(const vh_ctx_t *ctx, double dt);
+
+/* Selects the con2prim algorithm for later steps (VH_C2P_*). */
+int vh_set_c2p_mode(vh_ctx_t *ctx, int mode);
+
+/* Per-kernel timing over the next vh_run: enable=1 records HIP events around
+ * every launch on the context's stream; vh_kernel_times returns, for up to
+ * max kernels, the name, launch count and summed milliseconds. */
+int vh_enable_timing(vh_ctx_t *ctx, int enable);
+int vh_kernel_times(const vh_ctx_t *ctx, int max, char names[][32],
+                    int *launches, double *ms);
+
+/* ---- per-element entry points (unit-level parity checks, tools) ---- */
+
+/* transformPV (src/rmn.cpp:28-139) on n inputs: Q n x 7 row-major host array
+ * (rescaled in place exactly as the reference rescales its argument when
+ * |Q_vec| > Q_t), out n x 8 {e,p,nb,nq,ns,vx,vy,vz}; iters (may be NULL):
+ * iterations used per input. */
+int vh_c2p(const vh_eos_t *eos, double *Q, long n, double *out, int *iters);
+
+/* transformCV (src/rmn.cpp:253-263): prim n x 8 -> Q n x 7 (host arrays). */
+int vh_p2c(const double *prim, long n, double *Q);
+
+/* device/driver information line (name, CUs, clocks) into buf */
+int vh_device_info(char *buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VHLLE_GPU_H */

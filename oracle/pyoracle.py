@@ -1,0 +1,176 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+The oracle is a serial C++ restatement of vHLLE's Hydro::performStep
+(src/hdo.cpp:1016-1134) in the reference's operation order.  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg may import this module,
+and only as the checker / timed CPU baseline, never as the product path.
+
+Parity status of each oracle function (how it is pinned without the reference,
+which cannot be built here: every hot-path file includes GSL headers that this
+image lacks):
+  con2prim         -- closed-form root of the conformal EoS, special cases,
+                      round trip with prim2con, brute-force bisection
+  prim2con         -- textbook T^{mu nu} of a perfect fluid
+  ideal step       -- Gubser analytic flow, uniform-state (Bjorken) recurrence,
+                      mirror symmetry, conservation in a closed box
+  viscous step     -- Bjorken viscous ODE (Israel-Stewart with the second-order
+                      coefficients of src/trancoeff.h:28-37), static fluid,
+                      symmetry of pi
+  ghost fill       -- copy identities
+  transport coeffs -- closed forms of src/trancoeff.cpp
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+class Trc(ctypes.Structure):
+    """Mirror of or_trc_t (vh_oracle.h)."""
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "etaS0", "zetaS0", "ah", "al", "aRho", "T0", "etaSMin", "etaSEpsilonMin",
+        "etaSShiftMuB", "etaSScaleMuB", "zetaSPeakEpsilon", "zetaSScaleBeta",
+        "zetaSSigmaMinus", "zetaSSigmaPlus")] + [
+        ("etaSparam", ctypes.c_int), ("zetaSparam", ctypes.c_int),
+        ("zetaSPeakT", ctypes.c_double), ("zetaSWidthMinus", ctypes.c_double),
+        ("zetaSWidthPlus", ctypes.c_double)]
+
+
+class State(ctypes.Structure):
+    _fields_ = [(n, ctypes.POINTER(ctypes.c_double)) for n in (
+        "Q", "Qh", "Qprev", "pi", "piH", "Pi", "PiH", "pi0", "piH0", "Pi0", "PiH0",
+        "m", "viscCorrCut")]
+
+
+# number of components of each state field
+FIELDS = {"Q": 7, "Qh": 7, "Qprev": 7, "pi": 10, "piH": 10, "Pi": 1, "PiH": 1,
+          "pi0": 10, "piH0": 10, "Pi0": 1, "PiH0": 1, "m": 3, "viscCorrCut": 1}
+
+
+def make_trc(etaS=0.0, zetaS=0.0, etaSparam=0, zetaSparam=0, **kw):
+    """TransportCoeff with the reference driver's defaults (src/main.cpp:70-81)."""
+    d = dict(etaS0=etaS, zetaS0=zetaS, ah=0.0, al=0.0, aRho=0.0, T0=0.15,
+             etaSMin=0.08, etaSEpsilonMin=5.0, etaSShiftMuB=0.0, etaSScaleMuB=0.0,
+             zetaSPeakEpsilon=5.0, zetaSScaleBeta=0.103, zetaSSigmaMinus=0.1,
+             zetaSSigmaPlus=0.1, etaSparam=etaSparam, zetaSparam=zetaSparam,
+             zetaSPeakT=0.170, zetaSWidthMinus=0.010, zetaSWidthPlus=0.030)
+    d.update(kw)
+    return Trc(**d)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run make -C oracle")
+        L = ctypes.CDLL(path)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.or_create.restype = ctypes.c_void_p
+        L.or_create.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 6 + [
+            ctypes.c_int, ctypes.POINTER(Trc)]
+        L.or_destroy.argtypes = [ctypes.c_void_p]
+        L.or_ncell.restype = ctypes.c_long
+        L.or_ncell.argtypes = [ctypes.c_void_p]
+        L.or_set_state.argtypes = [ctypes.c_void_p, ctypes.POINTER(State)]
+        L.or_get_state.argtypes = [ctypes.c_void_p, ctypes.POINTER(State)]
+        L.or_set_prim.argtypes = [ctypes.c_void_p, ctypes.c_double] + [dp] * 7
+        L.or_run.restype = ctypes.c_int
+        L.or_run.argtypes = [ctypes.c_void_p, dp, ctypes.c_double, ctypes.c_int]
+        L.or_check_dtau.restype = ctypes.c_int
+        L.or_check_dtau.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.or_c2p.argtypes = [ctypes.c_int, dp, ctypes.c_long, dp,
+                             ctypes.POINTER(ctypes.c_int)]
+        L.or_p2c.argtypes = [dp, ctypes.c_long, dp]
+        L.or_eos.argtypes = [ctypes.c_int, dp, ctypes.c_long, dp]
+        L.or_trcoeff.argtypes = [ctypes.POINTER(Trc), ctypes.c_int, dp, ctypes.c_long, dp]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def c2p(Q):
+    """transformPV over rows of Q (n x 7). Returns (prim n x 8, iters n, Q_after)."""
+    Q = np.ascontiguousarray(Q, dtype=np.float64).copy()
+    n = Q.shape[0]
+    out = np.zeros((n, 8))
+    it = np.zeros(n, dtype=np.int32)
+    lib().or_c2p(0, _p(Q), n, _p(out), it.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return out, it, Q
+
+
+def p2c(prim):
+    prim = np.ascontiguousarray(prim, dtype=np.float64)
+    Q = np.zeros((prim.shape[0], 7))
+    lib().or_p2c(_p(prim), prim.shape[0], _p(Q))
+    return Q
+
+
+def eos(e):
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    out = np.zeros((e.shape[0], 4))
+    lib().or_eos(0, _p(e), e.shape[0], _p(out))
+    return out
+
+
+def trcoeff(trc, inp):
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    out = np.zeros((inp.shape[0], 4))
+    lib().or_trcoeff(ctypes.byref(trc), 0, _p(inp), inp.shape[0], _p(out))
+    return out
+
+
+class Solver:
+    """Oracle hydro grid + Hydro::performStep driver (CPU, serial)."""
+
+    def __init__(self, nx, ny, nz, xlim, ylim, zlim, trc=None):
+        self.trc = trc if trc is not None else make_trc()
+        self.h = lib().or_create(nx, ny, nz, xlim[0], xlim[1], ylim[0], ylim[1],
+                                 zlim[0], zlim[1], 0, ctypes.byref(self.trc))
+        if not self.h:
+            raise ValueError("bad grid")
+        self.shape = (nx, ny, nz)
+        self.N = nx * ny * nz
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_destroy(self.h)
+            self.h = None
+
+    def set_prim(self, tau, e, vx, vy, vz, nb=None, nq=None, ns=None):
+        z = np.zeros(self.N)
+        arrs = [np.ascontiguousarray(a if a is not None else z, dtype=np.float64).ravel()
+                for a in (e, nb, nq, ns, vx, vy, vz)]
+        lib().or_set_prim(self.h, tau, *[_p(a) for a in arrs])
+
+    def get_state(self):
+        st = State()
+        out = {}
+        for k, nc in FIELDS.items():
+            a = np.zeros(nc * self.N)
+            out[k] = a
+            setattr(st, k, _p(a))
+        lib().or_get_state(self.h, ctypes.byref(st))
+        return {k: v.reshape(FIELDS[k], self.N) for k, v in out.items()}
+
+    def set_state(self, d):
+        st = State()
+        keep = []
+        for k, nc in FIELDS.items():
+            if k in d:
+                a = np.ascontiguousarray(d[k], dtype=np.float64).reshape(-1)
+                assert a.size == nc * self.N, k
+                keep.append(a)
+                setattr(st, k, _p(a))
+        lib().or_set_state(self.h, ctypes.byref(st))
+
+    def run(self, tau, dt, nsteps):
+        t = ctypes.c_double(tau)
+        rc = lib().or_run(self.h, ctypes.byref(t), dt, nsteps)
+        return rc, t.value

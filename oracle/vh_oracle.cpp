@@ -1,0 +1,1191 @@
+// CPU oracle for the vHLLE hydro step -- TEST INFRASTRUCTURE ONLY (see
+// vh_oracle.h).  Serial restatement of yukarpenko/vhlle's Hydro::performStep
+// in the reference's loop order and floating-point operation order.  Built
+// with -O3 -ffp-contract=off and no -march flags (x86-64 SSE2, like the
+// reference Makefile:3), so that each expression rounds exactly as the
+// reference's does.  Every function cites the reference lines it follows.
+#include "vh_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+// ----------------------------------------------------------------- constants
+// Q component slots (src/inc.h:4-10)
+enum { kT = 0, kX = 1, kY = 2, kZ = 3, kNB = 4, kNQ = 5, kNS = 6 };
+enum { kPredict = 0, kCorrect = 1 };  // src/inc.h:12-13
+
+// packed symmetric 4x4 index (src/cll.cpp:22-31), without the bounds exit
+inline int sym(int i, int j) {
+  return j < i ? (i * (i + 1)) / 2 + j : (j * (j + 1)) / 2 + i;
+}
+
+// slope limiter (src/cll.cpp:12-19)
+inline double minmod(double a, double b) {
+  if (a * b <= 0.) return 0.;
+  return std::fabs(a) > std::fabs(b) ? b : a;
+}
+
+// sign() of src/hdo.cpp:34-41
+inline double sgn(double x) { return x > 0 ? 1. : (x < 0. ? -1. : 0.); }
+
+// ------------------------------------------------------------------- EoS
+// Ideal massless gas, the SIMPLE branch of EoSs (src/eos.cpp:87-111).
+struct IdealEos {
+  // cnst of EoSs::t (src/eos.cpp:98-99); GCC folds pow(C_PI,2) and
+  // pow(0.197326968,3) at compile time with correct rounding.
+  double cnst;
+  IdealEos() {
+    const double pi_ = 3.14159265358979312;  // C_PI, src/inc.h:18
+    cnst = (16 + 0.5 * 21.0 * 2.5) * (pi_ * pi_) / 30.0 /
+           (0.197326968 * 0.197326968 * 0.197326968);
+  }
+  double p(double e) const { return e / 3.; }
+  double T(double e) const { return e > 0. ? 1.0 * std::pow(e / cnst, 0.25) : 0.; }
+  // EoS::eos for this EoS: p, T, mu=0
+  void eos(double e, double &T_, double &mub, double &muq, double &mus,
+           double &p_) const {
+    p_ = p(e);
+    T_ = T(e);
+    mub = muq = mus = 0.;
+  }
+  // EoS::s (src/eos.cpp:18-25)
+  double s(double e, double nb, double nq, double ns) const {
+    double T_, mub, muq, mus, p_;
+    eos(e, T_, mub, muq, mus, p_);
+    if (T_ > 0.0) return (e + p_ - mub * nb - muq * nq - mus * ns) / T_;
+    return 0.;
+  }
+  double cs2e(double) const { return 1. / 3.; }  // EoSs::dpe SIMPLE
+};
+
+// (C2P_COUNT) iteration counter hook for or_c2p
+thread_local int g_c2p_iters = 0;
+
+// ---------------------------------------------------------------- con2prim
+// transformPV (src/rmn.cpp:28-139).  Q is modified in place when |Q_vec| > Q_t
+// (src/rmn.cpp:65-70), like the reference.
+void con2prim(const IdealEos &eos, double Q[7], double &e, double &p,
+              double &nb, double &nq, double &ns, double &vx, double &vy,
+              double &vz) {
+  const int maxit = 100;
+  const double dpe = 1. / 3.;
+  const double corrf = 0.9999;
+  double vl = 0., vh = 1.;
+  double M = std::sqrt(Q[kX] * Q[kX] + Q[kY] * Q[kY] + Q[kZ] * Q[kZ]);
+  g_c2p_iters = 0;
+  if (Q[kT] <= 0.) {
+    e = p = 0.;
+    vx = vy = vz = 0.;
+    nb = nq = ns = 0.;
+    return;
+  }
+  if (M == 0.) {
+    e = Q[kT];
+    vx = vy = vz = 0.;
+    nb = Q[kNB];
+    nq = Q[kNQ];
+    ns = Q[kNS];
+    p = eos.p(e);
+    return;
+  }
+  if (M > Q[kT]) {
+    Q[kX] *= corrf * Q[kT] / M;
+    Q[kY] *= corrf * Q[kT] / M;
+    Q[kZ] *= corrf * Q[kT] / M;
+    M = Q[kT] * corrf;
+  }
+  double v = 0.5 * (vl + vh);
+  e = Q[kT] - M * v;
+  if (e < 0.) e = 0.;
+  nb = Q[kNB] * std::sqrt(1 - v * v);
+  nq = Q[kNQ] * std::sqrt(1 - v * v);
+  ns = Q[kNS] * std::sqrt(1 - v * v);
+  p = eos.p(e);
+  double f = (Q[kT] + p) * v - M;
+  double df = (Q[kT] + p) - M * v * dpe;
+  double dvold = vh - vl;
+  double dv = dvold;
+  for (int it = 0; it < maxit; it++) {
+    g_c2p_iters = it + 1;
+    const bool bisect = (f + df * (vh - v)) * (f + df * (vl - v)) >= 0. ||
+                        std::fabs(2. * f) > std::fabs(dvold * df);
+    dvold = dv;
+    if (bisect) {
+      dv = 0.5 * (vh - vl);
+      v = vl + dv;
+    } else {
+      dv = f / df;
+      v -= dv;
+    }
+    if (std::fabs(dv) < 0.00001) break;
+    e = Q[kT] - M * v;
+    if (e < 0.) e = 0.;
+    nb = Q[kNB] * std::sqrt(1 - v * v);
+    nq = Q[kNQ] * std::sqrt(1 - v * v);
+    ns = Q[kNS] * std::sqrt(1 - v * v);
+    p = eos.p(e);
+    f = (Q[kT] + p) * v - M;
+    df = (Q[kT] + p) - M * v * dpe;
+    if (f > 0.)
+      vh = v;
+    else
+      vl = v;
+  }
+  vx = v * Q[kX] / M;
+  vy = v * Q[kY] / M;
+  vz = v * Q[kZ] / M;
+  e = Q[kT] - M * v;
+  p = eos.p(e);  // uses the previous iterate's nb (src/rmn.cpp:125-126)
+  nb = Q[kNB] * std::sqrt(1 - vx * vx - vy * vy - vz * vz);
+  nq = Q[kNQ] * std::sqrt(1 - vx * vx - vy * vy - vz * vz);
+  ns = Q[kNS] * std::sqrt(1 - vx * vx - vy * vy - vz * vz);
+}
+
+// transformCV (src/rmn.cpp:253-263)
+void prim2con(double e, double p, double nb, double nq, double ns, double vx,
+              double vy, double vz, double Q[7]) {
+  const double g2 = 1. / (1 - vx * vx - vy * vy - vz * vz);
+  Q[kT] = (e + p * (vx * vx + vy * vy + vz * vz)) * g2;
+  Q[kX] = (e + p) * vx * g2;
+  Q[kY] = (e + p) * vy * g2;
+  Q[kZ] = (e + p) * vz * g2;
+  Q[kNB] = nb * std::sqrt(g2);
+  Q[kNQ] = nq * std::sqrt(g2);
+  Q[kNS] = ns * std::sqrt(g2);
+}
+
+// ------------------------------------------------------ transport coeffs
+// TransportCoeff (src/trancoeff.h:6-45, src/trancoeff.cpp:19-100)
+struct Trc {
+  or_trc_t c;
+  const IdealEos *eos;
+  double zetaS(double e, double T, double s, double P) const {
+    if (c.zetaSparam == 0)
+      return c.zetaS0 * (1. / 3. - eos->cs2e(e)) / (std::exp((0.16 - T) / 0.001) + 1.);
+    if (c.zetaSparam == 1) {
+      const double Tp = 0.180;
+      if (T < 0.180)
+        return 0.03 + (0.08 * std::exp(((T / Tp) - 1.) / (0.0025))) +
+               (0.22 * std::exp(((T / Tp) - 1) / (0.022)));
+      else if (T < 0.200)
+        return 27.55 * (T / Tp) - 13.45 - (13.77 * (T / Tp) * (T / Tp));
+      return 0.001 + (0.9 * std::exp(-((T / Tp) - 1.) / (0.0025))) +
+             (0.25 * std::exp(-((T / Tp) - 1.) / (0.13)));
+    }
+    if (c.zetaSparam == 2) {
+      const double Tpk = 0.165, Tw = 0.010, Bn = 0.24, Bw = 1.5;
+      if (T > Tpk)
+        return Bn * ((Bw * Bw) / ((((T / Tpk) - 1.) * ((T / Tpk) - 1.)) + (Bw * Bw)));
+      return Bn * (std::exp(-((T - Tpk) / Tw) * ((T - Tpk) / Tw)));
+    }
+    if (c.zetaSparam == 3) {
+      const double Tpk2 = 0.160, Bn2 = 0.13, B1 = 0.01, B2 = 0.12;
+      if (T < Tpk2) return Bn2 * std::exp(-((T - Tpk2) * (T - Tpk2) / (B1 * B1)));
+      return Bn2 * std::exp(-((T - Tpk2) * (T - Tpk2) / (B2 * B2)));
+    }
+    if (c.zetaSparam == 4) {
+      // operator precedence kept as in src/trancoeff.cpp:62,64
+      if (std::fabs(s) < 1e-10 || T < 1e-1) return 0;
+      const double sig = e < c.zetaSPeakEpsilon ? c.zetaSSigmaMinus : c.zetaSSigmaPlus;
+      const double d = c.zetaSScaleBeta *
+                       (std::pow(e, 0.25) - std::pow(c.zetaSPeakEpsilon, 0.25));
+      return ((e + P) / (s * T)) * c.zetaS0 * std::exp((d * d) / 2.0 * (sig * sig));
+    }
+    if (c.zetaSparam == 5) {
+      // T-peaked asymmetric Gaussian (new in this build):
+      // zetaS0 * exp(-(T-Tpk)^2 / (2 w^2)), w = width below/above the peak
+      if (T <= 0.) return 0.;
+      const double w = T < c.zetaSPeakT ? c.zetaSWidthMinus : c.zetaSWidthPlus;
+      const double d = (T - c.zetaSPeakT) / w;
+      return c.zetaS0 * std::exp(-0.5 * d * d);
+    }
+    return 0.;
+  }
+  double etaS(double e, double rho, double T, double muB, double s, double P) const {
+    if (c.etaSparam == 0) return c.etaS0;
+    if (c.etaSparam == 1) return c.etaSMin + ((T > c.T0) ? c.ah * (T - c.T0) : c.al * (T - c.T0));
+    if (c.etaSparam == 2)
+      return std::max(0.0, c.etaSMin + ((e > c.etaSEpsilonMin)
+                                            ? ((c.ah * (e - c.etaSEpsilonMin) + c.aRho * rho))
+                                            : c.al * (e - c.etaSEpsilonMin) + c.aRho * rho));
+    if (c.etaSparam == 3) {
+      if (std::fabs(s) < 1e-10 || T < 1e-1) return 0;
+      const double Tc = c.T0 + c.etaSShiftMuB * (muB / 0.938);
+      return ((e + P) / (s * T)) *
+             std::max(0.0, (c.etaSMin + ((T > Tc) ? c.ah * (T - Tc) : c.al * (T - Tc))) *
+                               (1 + c.etaSScaleMuB * (muB / 0.938)));
+    }
+    return 0.;
+  }
+  void getEta(double e, double rho, double T, double muB, double s, double P,
+              double &es, double &zs) const {
+    es = etaS(e, rho, T, muB, s, P);
+    zs = zetaS(e, T, s, P);
+  }
+  // getTau (src/trancoeff.cpp:92-100); abs() of a double resolves to |x| in
+  // the reference build (GSL's header pulls in <stdlib.h>).
+  void getTau(double e, double rho, double T, double muB, double s, double P,
+              double &taupi, double &tauPi) const {
+    if (T > 1e-1 && std::fabs(e + P) > 1.e-10) {
+      taupi = (5. / 5.068) * s * etaS(e, rho, T, muB, s, P) / (e + P);
+      tauPi = (6. / 5.068) * zetaS(e, T, s, P) * s / (e + P);
+    } else {
+      taupi = tauPi = 0.;
+    }
+  }
+  bool isViscous() const {
+    return c.etaS0 > 0. || c.zetaS0 > 0. || c.etaSparam > 0 || c.zetaSparam > 0;
+  }
+};
+
+// ------------------------------------------------------------------- cells
+// The per-cell record of src/cll.h:42-61 (AoS, as in the reference).
+struct OCell {
+  double Q[7], Qh[7], Qprev[7];
+  double pi[10], piH[10], Pi, PiH;
+  double pi0[10], piH0[10], Pi0, PiH0;
+  double flux[7];
+  double m[3], dm[3];
+  double vcc;
+  OCell() {
+    std::memset(this, 0, sizeof(*this));
+    vcc = 1.;
+  }
+  double maxM() const { return std::max(m[0], std::max(m[1], m[2])); }
+  void add_m(int d, double inc) {  // Cell::addM (src/cll.h:146-150)
+    m[d] += inc;
+    if (m[d] > 0.9) m[0] = m[1] = m[2] = 1.;
+  }
+  void add_flux(const double f[7]) {
+    for (int i = 0; i < 7; i++) flux[i] += f[i];
+  }
+  void clear_flux() {
+    for (int i = 0; i < 7; i++) flux[i] = 0.;
+  }
+  // Cell::setQ / setQh (src/cll.h:75-86)
+  static void set_clamped(double dst[7], const double src[7]) {
+    for (int i = 0; i < 7; i++) dst[i] = src[i];
+    if (dst[kT] < 0.)
+      for (int i = 0; i < 7; i++) dst[i] = 0.;
+  }
+};
+
+struct Grid {
+  int nx, ny, nz;
+  double minx, maxx, miny, maxy, minz, maxz, dx, dy, dz;
+  std::vector<OCell> c;
+  // Fluid::getCell with index clamping (src/fld.h:70-78)
+  OCell &at(int ix, int iy, int iz) {
+    ix = ix > 0 ? ix : 0;
+    ix = ix < nx ? ix : nx - 1;
+    iy = iy > 0 ? iy : 0;
+    iy = iy < ny ? iy : ny - 1;
+    iz = iz > 0 ? iz : 0;
+    iz = iz < nz ? iz : nz - 1;
+    return c[ix + nx * iy + nx * ny * iz];
+  }
+  // neighbour along axis d (1..3) at offset s, clamped (src/fld.cpp:89-99)
+  OCell &nb(int ix, int iy, int iz, int d, int s) {
+    return at(ix + (d == 1 ? s : 0), iy + (d == 2 ? s : 0), iz + (d == 3 ? s : 0));
+  }
+};
+
+// ------------------------------------------------------------------- hydro
+struct Solver {
+  Grid g;
+  IdealEos eos;
+  Trc trc;
+  double dt = 0, tau = 0;
+  int status = 0;  // 0 ok, 1 NaN exit(1), 2 e>1e10 exit(0)
+
+  // Cell::getPrimVar{Left,Right,HLeft,HRight} (src/cll.cpp:106-204):
+  // MUSCL half-slope reconstruction of Q (or Qh), face state (Q -+ dQ)/tau.
+  void face_state(int ix, int iy, int iz, int d, bool useQh, int side,
+                  double tau_, double prim[8]) {
+    OCell &c = g.at(ix, iy, iz);
+    OCell &nx_ = g.nb(ix, iy, iz, d, +1);
+    OCell &pv = g.nb(ix, iy, iz, d, -1);
+    const double *q = useQh ? c.Qh : c.Q;
+    const double *qr = useQh ? nx_.Qh : nx_.Q;
+    const double *ql = useQh ? pv.Qh : pv.Q;
+    double dQ[7], s[7];
+    for (int i = 0; i < 7; i++) dQ[i] = minmod((qr[i] - q[i]) / 2., (q[i] - ql[i]) / 2.);
+    if (side < 0)
+      for (int i = 0; i < 7; i++) s[i] = (q[i] - dQ[i]) / tau_;
+    else
+      for (int i = 0; i < 7; i++) s[i] = (q[i] + dQ[i]) / tau_;
+    con2prim(eos, s, prim[0], prim[1], prim[2], prim[3], prim[4], prim[5], prim[6], prim[7]);
+  }
+
+  // Hydro::hlle_flux (src/hdo.cpp:77-305); the left cell is (ix,iy,iz), the
+  // right cell its +1 neighbour along axis d.
+  void hlle(int ix, int iy, int iz, int d, int mode) {
+    OCell &L = g.at(ix, iy, iz);
+    OCell &R = g.nb(ix, iy, iz, d, +1);
+    const int rx = ix + (d == 1), ry = iy + (d == 2), rz = iz + (d == 3);
+    const double dta = mode == 0 ? dt / 2. : dt;
+    double l[8], r[8];
+    const bool h = mode != kPredict;
+    face_state(ix, iy, iz, d, h, +1, tau, l);
+    face_state(rx, ry, rz, d, h, -1, tau, r);
+    double &el = l[0], &pl = l[1], &vxl = l[5], &vyl = l[6], &vzl = l[7];
+    double &er = r[0], &pr = r[1], &vxr = r[5], &vyr = r[6], &vzr = r[7];
+    const double El = (el + pl) / (1 - vxl * vxl - vyl * vyl - vzl * vzl);
+    const double Er = (er + pr) / (1 - vxr * vxr - vyr * vyr - vzr * vzr);
+    const double tauFactor = mode == kPredict ? tau + 0.25 * dt : tau + 0.5 * dt;
+    if (el < 0.) {
+      el = 0.;
+      pl = 0.;
+    }
+    if (er < 0.) {
+      er = 0.;
+      pr = 0.;
+    }
+    if (el > 1e10) {
+      status = 2;
+      return;
+    }
+    if (el == 0. && er == 0.) return;
+    if (pr < 0.) {  // the reference re-evaluates both states from Q (not Qh)
+                    // in either mode (src/hdo.cpp:134-140)
+      face_state(ix, iy, iz, d, false, +1, tau, l);
+      face_state(rx, ry, rz, d, false, -1, tau, r);
+    }
+    if (L.m[d - 1] < 1. && R.m[d - 1] < 1.) return;
+
+    const double gl = 1. / std::sqrt(1 - vxl * vxl - vyl * vyl - vzl * vzl);
+    const double gr = 1. / std::sqrt(1 - vxr * vxr - vyr * vyr - vzr * vzr);
+    // conserved densities U (order: t, x, y, z, b, q, s)
+    double Ul[7], Ur[7], Fl[7], Fr[7];
+    Ul[kX] = gl * gl * (el + pl) * vxl;
+    Ul[kY] = gl * gl * (el + pl) * vyl;
+    Ul[kZ] = gl * gl * (el + pl) * vzl;
+    Ul[kT] = gl * gl * (el + pl) - pl;
+    Ul[kNB] = gl * l[2];
+    Ul[kNQ] = gl * l[3];
+    Ul[kNS] = gl * l[4];
+    Ur[kX] = gr * gr * (er + pr) * vxr;
+    Ur[kY] = gr * gr * (er + pr) * vyr;
+    Ur[kZ] = gr * gr * (er + pr) * vzr;
+    Ur[kT] = gr * gr * (er + pr) - pr;
+    Ur[kNB] = gr * r[2];
+    Ur[kNQ] = gr * r[3];
+    Ur[kNS] = gr * r[4];
+    const double cs2 = 1. / 3.;            // EoS::cs2() (src/eos.h:34)
+    const double cs = std::sqrt(1. / 3.);  // EoS::cs()
+    double bl, br, dx;
+    const double vnl = d == 1 ? vxl : (d == 2 ? vyl : vzl);
+    const double vnr = d == 1 ? vxr : (d == 2 ? vyr : vzr);
+    if (d != 3) {
+      for (int i = 0; i < 7; i++) {
+        Fl[i] = (i == kT) ? Ul[kT] * vnl + pl * vnl : Ul[i] * vnl;
+        Fr[i] = (i == kT) ? Ur[kT] * vnr + pr * vnr : Ur[i] * vnr;
+      }
+      Fl[d] = Ul[d] * vnl + pl;
+      Fr[d] = Ur[d] * vnr + pr;
+    } else {
+      const double t1 = tauFactor;
+      for (int i = 0; i < 7; i++) {
+        Fl[i] = (i == kT) ? Ul[kT] * vnl / t1 + pl * vnl / t1 : Ul[i] * vnl / t1;
+        Fr[i] = (i == kT) ? Ur[kT] * vnr / t1 + pr * vnr / t1 : Ur[i] * vnr / t1;
+      }
+      Fl[kZ] = Ul[kZ] * vnl / t1 + pl / t1;
+      Fr[kZ] = Ur[kZ] * vnr / t1 + pr / t1;
+    }
+    const double sEl = std::sqrt(El), sEr = std::sqrt(Er);
+    const double dv = vnl - vnr;
+    const double sumE = std::sqrt(El) + std::sqrt(Er);
+    const double csb = std::sqrt(cs2 + 0.5 * std::sqrt(El * Er) / (sumE * sumE) * (dv * dv));
+    const double vb = (sEl * vnl + sEr * vnr) / (std::sqrt(El) + std::sqrt(Er));
+    bl = std::min(0., std::min((vb - csb) / (1 - vb * csb), (vnl - cs) / (1 - vnl * cs)));
+    br = std::max(0., std::max((vb + csb) / (1 + vb * csb), (vnr + cs) / (1 + vnr * cs)));
+    if (d == 3) {
+      bl = 1. / tau * bl;
+      br = 1. / tau * br;
+      if (el == 0.) bl = -1. / tau;
+      if (er == 0.) br = 1. / tau;
+      dx = g.dz;
+    } else {
+      if (el == 0.) bl = -1.;
+      if (er == 0.) br = 1.;
+      dx = d == 1 ? g.dx : g.dy;
+    }
+    if (bl == 0. && br == 0.) return;
+    double F[7];
+    for (int i = 0; i < 7; i++)
+      F[i] = tauFactor * dta / dx * (-bl * br * (Ul[i] - Ur[i]) + br * Fl[i] - bl * Fr[i]) /
+             (-bl + br);
+    if (F[kNB] != F[kNB]) {
+      status = 1;
+      return;
+    }
+    double mF[7];
+    for (int i = 0; i < 7; i++) mF[i] = -F[i];
+    L.add_flux(mF);
+    R.add_flux(F);
+  }
+
+  // Hydro::source + source_step (src/hdo.cpp:307-353)
+  void source_step(int ix, int iy, int iz, int mode) {
+    OCell &c = g.at(ix, iy, iz);
+    const double sdt = mode == kPredict ? dt / 2. : dt;
+    const double tau1 = mode == kPredict ? tau : tau + 0.5 * dt;
+    const double *Q = mode == kPredict ? c.Q : c.Qh;
+    double q[7], S[7], pr[8];
+    for (int i = 0; i < 7; i++) q[i] = Q[i] / tau1;
+    con2prim(eos, q, pr[0], pr[1], pr[2], pr[3], pr[4], pr[5], pr[6], pr[7]);
+    const double p = pr[1], vz = pr[7];
+    S[kT] = -q[kT] * vz * vz - p * (1. + vz * vz);
+    S[kX] = 0.;
+    S[kY] = 0.;
+    S[kZ] = -q[kZ];
+    S[kNB] = S[kNQ] = S[kNS] = 0.;
+    for (int i = 0; i < 7; i++) S[i] *= sdt;
+    if (S[kNB] != S[kNB]) {
+      status = 1;
+      return;
+    }
+    c.add_flux(S);
+  }
+
+  // con2prim of a cell's Q/Qh/Qprev divided by tau (Cell::getPrimVar*,
+  // src/cll.cpp:90-104,206-220)
+  void prim_of(const double *Qsrc, double tau_, double pr[8]) {
+    double q[7];
+    for (int i = 0; i < 7; i++) q[i] = Qsrc[i] / tau_;
+    con2prim(eos, q, pr[0], pr[1], pr[2], pr[3], pr[4], pr[5], pr[6], pr[7]);
+  }
+
+  // Hydro::NSquant (src/hdo.cpp:381-652), vorticity off.
+  void nsquant(int ix, int iy, int iz, double piNS[4][4], double &PiNS,
+               double dmu[4][4], double &du) {
+    const double VMIN = 1e-2, UDIFF = 3.0;
+    const double gmn[4][4] = {{1, 0, 0, 0}, {0, -1, 0, 0}, {0, 0, -1, 0}, {0, 0, 0, -1}};
+    OCell &c = g.at(ix, iy, iz);
+    if (g.at(ix + 1, iy, iz).maxM() <= 0.9 || g.at(ix, iy + 1, iz).maxM() <= 0.9 ||
+        g.at(ix - 1, iy, iz).maxM() <= 0.9 || g.at(ix, iy - 1, iz).maxM() <= 0.9 ||
+        g.at(ix + 1, iy + 1, iz).maxM() <= 0.9 || g.at(ix + 1, iy - 1, iz).maxM() <= 0.9 ||
+        g.at(ix - 1, iy + 1, iz).maxM() <= 0.9 || g.at(ix - 1, iy - 1, iz).maxM() <= 0.9) {
+      for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) piNS[i][j] = dmu[i][j] = 0.;
+      PiNS = du = 0.;
+      return;
+    }
+    double p0[8], p1[8], ph[8];
+    prim_of(c.Qprev, tau - dt, p0);
+    prim_of(c.Q, tau, p1);
+    prim_of(c.Qh, tau - 0.5 * dt, ph);
+    double e0 = p0[0], e1 = ph[0];  // e1 is overwritten by the H-centre state
+    const double s = eos.s(ph[0], ph[2], ph[3], ph[4]);
+    double T, mub, muq, mus, p;
+    eos.eos(ph[0], T, mub, muq, mus, p);
+    double etaS, zetaS;
+    trc.getEta(ph[0], ph[2], T, mub, s, p, etaS, zetaS);
+    double ut0 = 1.0 / std::sqrt(1.0 - p0[5] * p0[5] - p0[6] * p0[6] - p0[7] * p0[7]);
+    double ux0 = ut0 * p0[5], uy0 = ut0 * p0[6], uz0 = ut0 * p0[7];
+    double ut1 = 1.0 / std::sqrt(1.0 - p1[5] * p1[5] - p1[6] * p1[6] - p1[7] * p1[7]);
+    double ux1 = ut1 * p1[5], uy1 = ut1 * p1[6], uz1 = ut1 * p1[7];
+    double u[4];
+    u[0] = 1.0 / std::sqrt(1.0 - ph[5] * ph[5] - ph[6] * ph[6] - ph[7] * ph[7]);
+    u[1] = u[0] * ph[5];
+    u[2] = u[0] * ph[6];
+    u[3] = u[0] * ph[7];
+    // time derivatives
+    dmu[0][0] = (ut1 * ut1 - ut0 * ut0) / 2. / u[0] / dt;
+    dmu[0][1] = (ux1 * ux1 - ux0 * ux0) / 2. / u[1] / dt;
+    dmu[0][2] = (uy1 * uy1 - uy0 * uy0) / 2. / u[2] / dt;
+    dmu[0][3] = (uz1 * uz1 - uz0 * uz0) / 2. / u[3] / dt;
+    if (std::fabs(0.5 * (ut1 + ut0) / ut1) > UDIFF) dmu[0][0] = (ut1 - ut0) / dt;
+    if (std::fabs(u[1]) < VMIN || std::fabs(0.5 * (ux1 + ux0) / ux1) > UDIFF)
+      dmu[0][1] = (ux1 - ux0) / dt;
+    if (std::fabs(u[2]) < VMIN || std::fabs(0.5 * (uy1 + uy0) / uy1) > UDIFF)
+      dmu[0][2] = (uy1 - uy0) / dt;
+    if (std::fabs(u[3]) < VMIN || std::fabs(0.5 * (uz1 + uz0) / uz1) > UDIFF)
+      dmu[0][3] = (uz1 - uz0) / dt;
+    if (e1 <= 0. || e0 <= 0.) dmu[0][0] = dmu[0][1] = dmu[0][2] = dmu[0][3] = 0.;
+    // space derivatives, neighbours' Qh / tau_{n+1}
+    for (int d = 1; d <= 3; d++) {
+      double a[8], b[8];
+      prim_of(g.nb(ix, iy, iz, d, +1).Qh, tau, a);
+      prim_of(g.nb(ix, iy, iz, d, -1).Qh, tau, b);
+      e1 = a[0];
+      e0 = b[0];
+      if (e1 > 0. && e0 > 0.) {
+        ut0 = 1.0 / std::sqrt(1.0 - b[5] * b[5] - b[6] * b[6] - b[7] * b[7]);
+        ux0 = ut0 * b[5];
+        uy0 = ut0 * b[6];
+        uz0 = ut0 * b[7];
+        ut1 = 1.0 / std::sqrt(1.0 - a[5] * a[5] - a[6] * a[6] - a[7] * a[7]);
+        ux1 = ut1 * a[5];
+        uy1 = ut1 * a[6];
+        uz1 = ut1 * a[7];
+        if (d < 3) {
+          const double h = d == 1 ? g.dx : g.dy;
+          dmu[d][0] = 0.25 * (ut1 * ut1 - ut0 * ut0) / u[0] / h;
+          dmu[d][1] = 0.25 * (ux1 * ux1 - ux0 * ux0) / u[1] / h;
+          dmu[d][2] = 0.25 * (uy1 * uy1 - uy0 * uy0) / u[2] / h;
+          dmu[d][3] = 0.25 * (uz1 * uz1 - uz0 * uz0) / u[3] / h;
+          if (std::fabs(0.5 * (ut1 + ut0) / u[0]) > UDIFF) dmu[d][0] = 0.5 * (ut1 - ut0) / h;
+          if (std::fabs(u[1]) < VMIN || std::fabs(0.5 * (ux1 + ux0) / u[1]) > UDIFF)
+            dmu[d][1] = 0.5 * (ux1 - ux0) / h;
+          if (std::fabs(u[2]) < VMIN || std::fabs(0.5 * (uy1 + uy0) / u[2]) > UDIFF)
+            dmu[d][2] = 0.5 * (uy1 - uy0) / h;
+          if (std::fabs(u[3]) < VMIN || std::fabs(0.5 * (uz1 + uz0) / u[3]) > UDIFF)
+            dmu[d][3] = 0.5 * (uz1 - uz0) / h;
+        } else {
+          const double dz = g.dz, tz = tau + 0.5 * dt;
+          dmu[3][0] = 0.25 * (ut1 * ut1 - ut0 * ut0) / u[0] / dz / tz;
+          dmu[3][1] = 0.25 * (ux1 * ux1 - ux0 * ux0) / u[1] / dz / tz;
+          dmu[3][2] = 0.25 * (uy1 * uy1 - uy0 * uy0) / u[2] / dz / tz;
+          dmu[3][3] = 0.25 * (uz1 * uz1 - uz0 * uz0) / u[3] / dz / tz;
+          if (std::fabs(0.5 * (ut1 + ut0) / u[0]) > UDIFF) dmu[3][0] = 0.5 * (ut1 - ut0) / dz / tz;
+          if (std::fabs(u[1]) < VMIN || std::fabs(0.5 * (ux1 + ux0) / u[1]) > UDIFF)
+            dmu[3][1] = 0.5 * (ux1 - ux0) / dz / tz;
+          if (std::fabs(u[2]) < VMIN || std::fabs(0.5 * (uy1 + uy0) / u[2]) > UDIFF)
+            dmu[3][2] = 0.5 * (uy1 - uy0) / dz / tz;
+          if (std::fabs(u[3]) < VMIN || std::fabs(0.5 * (uz1 + uz0) / u[3]) > UDIFF)
+            dmu[3][3] = 0.5 * (uz1 - uz0) / dz / tz;
+        }
+      } else {
+        dmu[d][0] = dmu[d][1] = dmu[d][2] = dmu[d][3] = 0.;
+      }
+    }
+    // Christoffel terms (src/hdo.cpp:606-607)
+    dmu[3][0] += u[3] / (tau - 0.5 * dt);
+    dmu[3][3] += u[0] / (tau - 0.5 * dt);
+    // projector Z (src/hdo.cpp:613-628) and sigma contraction (:630-637)
+    static thread_local double Z[4][4][4][4];
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++)
+        for (int k = 0; k < 4; k++)
+          for (int l = 0; l < 4; l++) Z[i][j][k][l] = 0.0;
+    for (int mu = 0; mu < 4; mu++)
+      for (int nu = 0; nu < 4; nu++)
+        for (int lam = 0; lam < 4; lam++)
+          for (int rho = 0; rho < 4; rho++) {
+            if (nu == rho) Z[mu][nu][lam][rho] += 0.5 * (gmn[mu][lam] - u[mu] * u[lam]);
+            if (mu == rho) Z[mu][nu][lam][rho] += 0.5 * (gmn[nu][lam] - u[nu] * u[lam]);
+            if (lam == rho) Z[mu][nu][lam][rho] -= (gmn[mu][nu] - u[mu] * u[nu]) / 3.0;
+          }
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++) {
+        piNS[i][j] = 0.0;
+        for (int k = 0; k < 4; k++)
+          for (int l = 0; l < 4; l++)
+            piNS[i][j] += Z[i][j][k][l] * dmu[k][l] * 2.0 * etaS * s / 5.068;
+      }
+    PiNS = -zetaS * s * (dmu[0][0] + dmu[1][1] + dmu[2][2] + dmu[3][3]) / 5.068;
+    du = dmu[0][0] + dmu[1][1] + dmu[2][2] + dmu[3][3];
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++)
+        if (std::isinf(piNS[i][j]) || std::isnan(piNS[i][j])) status = 1;
+  }
+
+  // Hydro::ISformal (src/hdo.cpp:682-963), vorticity off, no FORMAL_SOLUTION
+  void isformal() {
+    const double gm[4] = {1., -1., -1., -1.};
+    // loop #1: relaxation + sources
+    for (int ix = 0; ix < g.nx; ix++)
+      for (int iy = 0; iy < g.ny; iy++)
+        for (int iz = 0; iz < g.nz; iz++) {
+          OCell &c = g.at(ix, iy, iz);
+          double pr[8];
+          prim_of(c.Qh, tau - dt / 2., pr);
+          double e = pr[0], nb = pr[2], nq = pr[3], ns = pr[4];
+          const double vx = pr[5], vy = pr[6], vz = pr[7];
+          if (e <= 0.) {
+            for (int i = 0; i < 10; i++) c.piH0[i] = c.pi0[i] = 0.0;
+            c.PiH0 = c.Pi0 = 0.0;
+            continue;
+          }
+          const double gamma = 1.0 / std::sqrt(1.0 - vx * vx - vy * vy - vz * vz);
+          double u[4] = {gamma, gamma * vx, gamma * vy, gamma * vz};
+          double fl[7] = {0, 0, 0, 0, 0., 0., 0.};
+          for (int i = 0; i < 4; i++) fl[i] = (tau - dt) * (c.pi[sym(0, i)] + c.Pi * u[0] * u[i]);
+          fl[0] += -(tau - dt) * c.Pi;
+          c.add_flux(fl);
+          double piNS[4][4], PiNS, dmu[4][4], du;
+          nsquant(ix, iy, iz, piNS, PiNS, dmu, du);
+          if (status) return;
+          double T, mub, muq, mus, p;
+          eos.eos(e, T, mub, muq, mus, p);
+          double etaS, zetaS;
+          const double s = eos.s(e, nb, nq, ns);
+          trc.getEta(e, nb, T, mub, s, p, etaS, zetaS);
+          const double eta = etaS * s;
+          double sig[4][4];
+          for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) {
+              sig[i][j] = 0.5 * piNS[i][j] / eta * 5.068;
+              if (eta <= 0.0) sig[i][j] = 0.0;
+            }
+          double taupi, tauPi;
+          trc.getTau(e, nb, T, mub, s, p, taupi, tauPi);
+          // getOther / getOtherBulk (src/trancoeff.h:28-37)
+          double dpp = 4. / 3., tpp = 10. / 7., lpP = 6. / 5.;
+          double phi7 = 9. / 70. / eos.p(e);
+          if (std::isinf(phi7)) phi7 = 0.0;
+          phi7 = phi7 / taupi;
+          if (taupi < 0.5 * dt) dpp = tpp = lpP = phi7 = 0.0;
+          double dPP = 2. / 3., lPp = 8. / 5. * (1. / 3. - eos.cs2e(e));
+          if (tauPi < 0.5 * dt) dPP = lPp = 0.0;
+          double Delta[10];
+          const double *pi = c.pi;
+          double *pH = c.piH0;
+          // half step
+          for (int i = 0; i < 4; i++)
+            for (int j = 0; j <= i; j++) {
+              Delta[sym(i, j)] = -u[i] * u[j];
+              if (i == j) Delta[sym(i, j)] += gm[i];
+              if (taupi > 0.5 * dt)
+                pH[sym(i, j)] = pi[sym(i, j)] - (pi[sym(i, j)] - piNS[i][j]) * dt / 2.0 / gamma / taupi;
+              else
+                pH[sym(i, j)] = piNS[i][j];
+            }
+          if (tauPi > 0.5 * dt)
+            c.PiH0 = c.Pi - (c.Pi - PiNS) * dt / 2.0 / gamma / tauPi;
+          else
+            c.PiH0 = PiNS;
+          double tau1 = tau - dt * 0.75;
+          pH[sym(0, 0)] += -2. * vz * pi[sym(0, 3)] / tau1 * dt / 2.;
+          pH[sym(3, 3)] += -(2. * vz / tau1 * pi[sym(0, 3)]) * dt / 2.;
+          pH[sym(3, 0)] += -(vz / tau1 * pi[sym(0, 0)] + vz / tau1 * pi[sym(3, 3)]) * dt / 2.;
+          pH[sym(1, 0)] += -vz / tau1 * pi[sym(1, 3)] * dt / 2.;
+          pH[sym(2, 0)] += -vz / tau1 * pi[sym(2, 3)] * dt / 2.;
+          pH[sym(3, 1)] += -(vz / tau1 * pi[sym(0, 1)]) * dt / 2.;
+          pH[sym(3, 2)] += -(vz / tau1 * pi[sym(0, 2)]) * dt / 2.;
+          for (int i = 0; i < 4; i++)
+            for (int j = 0; j <= i; j++) {
+              double &t = pH[sym(i, j)];
+              t += (-dpp * pi[sym(i, j)] * du + lpP * c.Pi * sig[i][j]) / gamma * 0.5 * dt;
+              for (int k = 0; k < 4; k++) {
+                t += (phi7 * pi[sym(i, k)] * pi[sym(j, k)] * gm[k] -
+                      tpp * 0.5 * (pi[sym(i, k)] * sig[j][k] * gm[k] + pi[sym(j, k)] * sig[i][k] * gm[k])) /
+                     gamma * 0.5 * dt;
+                for (int l = 0; l < 4; l++)
+                  t += (-pi[sym(i, k)] * u[j] - pi[sym(j, k)] * u[i]) * u[l] * dmu[l][k] * gm[k] / gamma * 0.5 * dt -
+                       1. / 3. * Delta[sym(i, j)] * pi[sym(k, l)] * (phi7 * pi[sym(k, l)] - tpp * sig[k][l]) *
+                           gm[k] * gm[l] / gamma * 0.5 * dt;
+              }
+            }
+          for (int k = 0; k < 4; k++)
+            for (int l = 0; l < 4; l++)
+              c.PiH0 += lPp * pi[sym(k, l)] * sig[k][l] * gm[k] * gm[l] / gamma * 0.5 * dt;
+          c.PiH0 += -dPP * c.Pi * du / gamma * 0.5 * dt;
+          // full step
+          double *p0 = c.pi0;
+          for (int i = 0; i < 4; i++)
+            for (int j = 0; j <= i; j++) {
+              if (taupi > 0.5 * dt)
+                p0[sym(i, j)] = pi[sym(i, j)] - (pH[sym(i, j)] - piNS[i][j]) * dt / gamma / taupi;
+              else
+                p0[sym(i, j)] = piNS[i][j];
+            }
+          if (tauPi > 0.5 * dt)
+            c.Pi0 = c.Pi - (c.PiH0 - PiNS) * dt / gamma / tauPi;
+          else
+            c.Pi0 = PiNS;
+          tau1 = tau - dt * 0.5;
+          p0[sym(0, 0)] += -2. * vz / tau1 * pH[sym(0, 3)] * dt;
+          p0[sym(3, 3)] += -(2. * vz / tau1 * pH[sym(0, 3)]) * dt;
+          p0[sym(3, 0)] += -(vz / tau1 * pH[sym(0, 0)] + vz / tau1 * pH[sym(3, 3)]) * dt;
+          p0[sym(1, 0)] += -vz / tau1 * pH[sym(1, 3)] * dt;
+          p0[sym(2, 0)] += -vz / tau1 * pH[sym(2, 3)] * dt;
+          p0[sym(3, 1)] += -(vz / tau1 * pH[sym(0, 1)]) * dt;
+          p0[sym(3, 2)] += -(vz / tau1 * pH[sym(0, 2)]) * dt;
+          for (int i = 0; i < 4; i++)
+            for (int j = 0; j <= i; j++) {
+              double &t = p0[sym(i, j)];
+              t += (-dpp * pH[sym(i, j)] * du + lpP * c.PiH0 * sig[i][j]) / gamma * dt;
+              for (int k = 0; k < 4; k++) {
+                t += (phi7 * pi[sym(i, k)] * pH[sym(j, k)] * gm[k] -
+                      tpp * 0.5 * (pH[sym(i, k)] * sig[j][k] * gm[k] + pH[sym(j, k)] * sig[i][k] * gm[k])) /
+                     gamma * dt;
+                for (int l = 0; l < 4; l++)
+                  t += ((-pH[sym(i, k)] * u[j] - pH[sym(j, k)] * u[i]) * u[l] * dmu[l][k] * gm[k] -
+                        1. / 3. * Delta[sym(i, j)] * pH[sym(k, l)] * (phi7 * pH[sym(k, l)] - tpp * sig[k][l]) *
+                            gm[k] * gm[l]) /
+                       gamma * dt;
+              }
+            }
+          for (int k = 0; k < 4; k++)
+            for (int l = 0; l < 4; l++)
+              c.Pi0 += lPp * pH[sym(k, l)] * sig[k][l] * gm[k] * gm[l] / gamma * dt;
+          c.Pi0 += -dPP * c.PiH0 * du / gamma * dt;
+        }
+    // loop #2: advection + limiter
+    for (int ix = 0; ix < g.nx; ix++)
+      for (int iy = 0; iy < g.ny; iy++)
+        for (int iz = 0; iz < g.nz; iz++) {
+          OCell &c = g.at(ix, iy, iz);
+          double pr[8];
+          prim_of(c.Qh, tau - 0.5 * dt, pr);
+          const double e = pr[0], p = pr[1], vx = pr[5], vy = pr[6], vz = pr[7];
+          if (e <= 0.) continue;
+          const double xm = -vx * dt / g.dx;
+          const double ym = -vy * dt / g.dy;
+          const double zm = -vz * dt / g.dz / (tau - 0.5 * dt);
+          const double xmH = -vx * dt / g.dx / 2.0;
+          const double ymH = -vy * dt / g.dy / 2.0;
+          const double zmH = -vz * dt / g.dz / (tau - 0.5 * dt) / 2.0;
+          const double wx[2] = {(1. - std::fabs(xm)), std::fabs(xm)};
+          const double wy[2] = {(1. - std::fabs(ym)), std::fabs(ym)};
+          const double wz[2] = {(1. - std::fabs(zm)), std::fabs(zm)};
+          const double wxH[2] = {(1. - std::fabs(xmH)), std::fabs(xmH)};
+          const double wyH[2] = {(1. - std::fabs(ymH)), std::fabs(ymH)};
+          const double wzH[2] = {(1. - std::fabs(zmH)), std::fabs(zmH)};
+          double pi[4][4], piH[4][4], Pi = 0.0, PiH = 0.0;
+          for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) pi[i][j] = piH[i][j] = 0.0;
+          for (int jx = 0; jx < 2; jx++)
+            for (int jy = 0; jy < 2; jy++)
+              for (int jz = 0; jz < 2; jz++) {
+                OCell &c1 = g.at(ix + jx * sgn(xm), iy + jy * sgn(ym), iz + jz * sgn(zm));
+                for (int i = 0; i < 4; i++)
+                  for (int j = 0; j < 4; j++) {
+                    pi[i][j] += wx[jx] * wy[jy] * wz[jz] * c1.pi0[sym(i, j)];
+                    piH[i][j] += wxH[jx] * wyH[jy] * wzH[jz] * c1.piH0[sym(i, j)];
+                  }
+                Pi += wx[jx] * wy[jy] * wz[jz] * c1.Pi0;
+                PiH += wxH[jx] * wyH[jy] * wzH[jz] * c1.PiH0;
+              }
+          const double maxT0 = std::max((e + p) / (1. - vx * vx - vy * vy - vz * vz) - p,
+                                        (e + p) * (vx * vx + vy * vy + vz * vz) /
+                                                (1. - vx * vx - vy * vy - vz * vz) +
+                                            p);
+          double maxpi = 0.;
+          for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++)
+              if (std::fabs(pi[i][j]) > maxpi) maxpi = std::fabs(pi[i][j]);
+          bool rescaled = false;
+          if (maxT0 / maxpi < 1.0) {
+            for (int i = 0; i < 4; i++)
+              for (int j = 0; j < 4; j++) {
+                pi[i][j] = 0.1 * pi[i][j] * maxT0 / maxpi;
+                piH[i][j] = 0.1 * piH[i][j] * maxT0 / maxpi;
+              }
+            rescaled = true;
+          }
+          if (std::fabs(Pi) > p) {
+            if (Pi != 0.) Pi = 1 * Pi / std::fabs(Pi) * p;
+            if (PiH != 0.) PiH = 1 * PiH / std::fabs(PiH) * p;
+            rescaled = true;
+          }
+          c.vcc = rescaled ? maxT0 / maxpi : 1.;
+          for (int i = 0; i < 4; i++)
+            for (int j = 0; j <= i; j++) {
+              c.pi[sym(i, j)] = pi[i][j];
+              c.piH[sym(i, j)] = piH[i][j];
+            }
+          c.Pi = Pi;
+          c.PiH = PiH;
+          const double gamma = 1.0 / std::sqrt(1.0 - vx * vx - vy * vy - vz * vz);
+          const double u[4] = {gamma, gamma * vx, gamma * vy, gamma * vz};
+          double fl[7] = {0, 0, 0, 0, 0., 0., 0.};
+          for (int i = 0; i < 4; i++) fl[i] = -tau * (c.pi[sym(0, i)] + c.Pi * u[0] * u[i]);
+          fl[0] += tau * c.Pi;
+          c.add_flux(fl);
+        }
+  }
+
+  // Hydro::visc_flux (src/hdo.cpp:966-1014)
+  void visc_flux(int ix, int iy, int iz, int d) {
+    OCell &L = g.at(ix, iy, iz);
+    OCell &R = g.nb(ix, iy, iz, d, +1);
+    if (L.m[d - 1] < 1. && R.m[d - 1] < 1.) return;
+    double dxa = d == 1 ? g.dx : (d == 2 ? g.dy : g.dz * (tau + 0.5 * dt));
+    double a[8], b[8];
+    prim_of(L.Qh, tau - dt / 2., a);
+    prim_of(R.Qh, tau - dt / 2., b);
+    double vx = 0.5 * (a[5] + b[5]);
+    double vy = 0.5 * (a[6] + b[6]);
+    double vz = 0.5 * (a[7] + b[7]);
+    const double v = std::sqrt(vx * vx + vy * vy + vz * vz);
+    if (v > 1.) {
+      vx = 0.99 * vx / v;
+      vy = 0.99 * vy / v;
+      vz = 0.99 * vz / v;
+    }
+    const double gamma = 1. / std::sqrt(1. - v * v);
+    const double u[4] = {gamma, gamma * vx, gamma * vy, gamma * vz};
+    const double gm[4] = {1., -1., -1., -1.};
+    double F[7] = {0, 0, 0, 0, 0., 0., 0.};
+    for (int m = 0; m < 4; m++) {
+      F[m] = 0.5 * (L.piH[sym(m, d)] + R.piH[sym(m, d)]);
+      if (m == d) F[m] += -0.5 * (L.PiH + R.PiH) * gm[m];
+      F[m] += 0.5 * (L.PiH + R.PiH) * u[m] * u[d];
+    }
+    for (int m = 0; m < 4; m++) F[m] = F[m] * (tau - 0.5 * dt) * dt / dxa;
+    double mF[7] = {-F[0], -F[1], -F[2], -F[3], 0., 0., 0.};
+    double pF[7] = {F[0], F[1], F[2], F[3], 0., 0., 0.};
+    L.add_flux(mF);
+    R.add_flux(pF);
+  }
+
+  // Hydro::visc_source_step (src/hdo.cpp:355-375)
+  void visc_source_step(int ix, int iy, int iz) {
+    OCell &c = g.at(ix, iy, iz);
+    double pr[8];
+    prim_of(c.Qh, tau - dt / 2., pr);
+    if (pr[0] <= 0.) return;
+    const double vx = pr[5], vy = pr[6], vz = pr[7];
+    double u[4];
+    u[0] = 1. / std::sqrt(1. - vx * vx - vy * vy - vz * vz);
+    u[1] = u[0] * vx;
+    u[2] = u[0] * vy;
+    u[3] = u[0] * vz;
+    double k[7] = {0, 0, 0, 0, 0., 0., 0.};
+    k[kT] = -c.piH[sym(3, 3)] + c.PiH * (-1.0 - u[3] * u[3]);
+    k[kX] = 0.;
+    k[kY] = 0.;
+    k[kZ] = -(c.piH[sym(0, 3)] + c.PiH * u[0] * u[3]);
+    for (int i = 0; i < 4; i++) k[i] *= dt;
+    c.add_flux(k);
+  }
+
+  // Fluid::correctImagCells (src/fld.cpp:211-273): ghost layers of Q and Qh
+  // copied from the 3rd cell inwards, axes in order z, y, x.
+  void ghost_ideal() {
+    auto copy2 = [&](OCell &src, OCell &d1, OCell &d0) {
+      double q[7], qh[7];
+      std::memcpy(q, src.Q, sizeof q);
+      OCell::set_clamped(d1.Q, q);
+      OCell::set_clamped(d0.Q, q);
+      std::memcpy(qh, src.Qh, sizeof qh);
+      OCell::set_clamped(d1.Qh, qh);
+      OCell::set_clamped(d0.Qh, qh);
+    };
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    for (int ix = 0; ix < nx; ix++)
+      for (int iy = 0; iy < ny; iy++) {
+        copy2(g.at(ix, iy, 2), g.at(ix, iy, 1), g.at(ix, iy, 0));
+        copy2(g.at(ix, iy, nz - 3), g.at(ix, iy, nz - 2), g.at(ix, iy, nz - 1));
+      }
+    for (int ix = 0; ix < nx; ix++)
+      for (int iz = 0; iz < nz; iz++) {
+        copy2(g.at(ix, 2, iz), g.at(ix, 1, iz), g.at(ix, 0, iz));
+        copy2(g.at(ix, ny - 3, iz), g.at(ix, ny - 2, iz), g.at(ix, ny - 1, iz));
+      }
+    for (int iy = 0; iy < ny; iy++)
+      for (int iz = 0; iz < nz; iz++) {
+        copy2(g.at(2, iy, iz), g.at(1, iy, iz), g.at(0, iy, iz));
+        copy2(g.at(nx - 3, iy, iz), g.at(nx - 2, iy, iz), g.at(nx - 1, iy, iz));
+      }
+  }
+
+  // Fluid::correctImagCellsFull (src/fld.cpp:275-382): Q, pi, Pi
+  void ghost_full() {
+    auto copyf = [&](OCell &src, OCell &d1, OCell &d0) {
+      double q[7], p[10], P;
+      std::memcpy(q, src.Q, sizeof q);
+      OCell::set_clamped(d1.Q, q);
+      OCell::set_clamped(d0.Q, q);
+      std::memcpy(p, src.pi, sizeof p);
+      P = src.Pi;
+      // the reference writes ghost 0 first, then ghost 1 (src/fld.cpp:288-294)
+      std::memcpy(d0.pi, p, sizeof p);
+      std::memcpy(d1.pi, p, sizeof p);
+      d0.Pi = P;
+      d1.Pi = P;
+    };
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    for (int ix = 0; ix < nx; ix++)
+      for (int iy = 0; iy < ny; iy++) {
+        copyf(g.at(ix, iy, 2), g.at(ix, iy, 1), g.at(ix, iy, 0));
+        copyf(g.at(ix, iy, nz - 3), g.at(ix, iy, nz - 2), g.at(ix, iy, nz - 1));
+      }
+    for (int ix = 0; ix < nx; ix++)
+      for (int iz = 0; iz < nz; iz++) {
+        copyf(g.at(ix, 2, iz), g.at(ix, 1, iz), g.at(ix, 0, iz));
+        copyf(g.at(ix, ny - 3, iz), g.at(ix, ny - 2, iz), g.at(ix, ny - 1, iz));
+      }
+    for (int iy = 0; iy < ny; iy++)
+      for (int iz = 0; iz < nz; iz++) {
+        copyf(g.at(2, iy, iz), g.at(1, iy, iz), g.at(0, iy, iz));
+        copyf(g.at(nx - 3, iy, iz), g.at(nx - 2, iy, iz), g.at(nx - 1, iy, iz));
+      }
+  }
+
+  // Fluid::updateM (src/fld.cpp:384-423)
+  void update_m() {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    for (int ix = 0; ix < nx; ix++)
+      for (int iy = 0; iy < ny; iy++)
+        for (int iz = 0; iz < nz; iz++) {
+          OCell &c = g.at(ix, iy, iz);
+          c.dm[0] = c.dm[1] = c.dm[2] = 0.;
+          if (c.maxM() < 1.) {
+            if (g.at(ix + 1, iy, iz).m[0] >= 1. || g.at(ix - 1, iy, iz).m[0] >= 1.) c.dm[0] = dt / g.dx;
+            if (g.at(ix, iy + 1, iz).m[1] >= 1. || g.at(ix, iy - 1, iz).m[1] >= 1.) c.dm[1] = dt / g.dy;
+            if (g.at(ix, iy, iz + 1).m[2] >= 1. || g.at(ix, iy, iz - 1).m[2] >= 1.)
+              c.dm[2] = dt / g.dz / tau;
+            if (c.dm[0] == 0. && c.dm[1] == 0.) {
+              if (g.at(ix + 1, iy + 1, iz).maxM() >= 1. || g.at(ix + 1, iy - 1, iz).maxM() >= 1. ||
+                  g.at(ix - 1, iy + 1, iz).maxM() >= 1. || g.at(ix - 1, iy - 1, iz).maxM() >= 1.) {
+                c.dm[0] = 0.707 * dt / g.dx;
+                c.dm[1] = 0.707 * dt / g.dy;
+              }
+            }
+          }
+        }
+    for (int ix = 0; ix < nx; ix++)
+      for (int iy = 0; iy < ny; iy++)
+        for (int iz = 0; iz < nz; iz++) {
+          OCell &c = g.at(ix, iy, iz);
+          c.add_m(0, c.dm[0]);
+          c.add_m(1, c.dm[1]);
+          c.add_m(2, c.dm[2]);
+        }
+  }
+
+  // Hydro::performStep (src/hdo.cpp:1016-1134)
+  void step() {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    update_m();
+    for (int iy = 0; iy < ny; iy++)
+      for (int iz = 0; iz < nz; iz++)
+        for (int ix = 0; ix < nx; ix++) {
+          OCell &c = g.at(ix, iy, iz);
+          std::memcpy(c.Qprev, c.Q, sizeof c.Q);
+          c.clear_flux();
+        }
+    for (int mode = kPredict; mode <= kCorrect; mode++) {
+      for (int iy = 0; iy < ny; iy++)
+        for (int iz = 0; iz < nz; iz++)
+          for (int ix = 0; ix < nx - 1; ix++) {
+            hlle(ix, iy, iz, 1, mode);
+            if (status) return;
+          }
+      for (int iz = 0; iz < nz; iz++)
+        for (int ix = 0; ix < nx; ix++)
+          for (int iy = 0; iy < ny - 1; iy++) {
+            hlle(ix, iy, iz, 2, mode);
+            if (status) return;
+          }
+      for (int ix = 0; ix < nx; ix++)
+        for (int iy = 0; iy < ny; iy++)
+          for (int iz = 0; iz < nz - 1; iz++) {
+            hlle(ix, iy, iz, 3, mode);
+            if (status) return;
+          }
+      for (int iy = 0; iy < ny; iy++)
+        for (int iz = 0; iz < nz; iz++)
+          for (int ix = 0; ix < nx; ix++) {
+            OCell &c = g.at(ix, iy, iz);
+            source_step(ix, iy, iz, mode);
+            if (status) return;
+            if (mode == kPredict) {
+              for (int i = 0; i < 7; i++) c.Qh[i] = c.Q[i] + c.flux[i];  // updateQtoQhByFlux
+            } else if (!(c.Q[0] + c.flux[0] < 0.)) {                    // updateByFlux
+              for (int i = 0; i < 7; i++) c.Q[i] += c.flux[i];
+            }
+            c.clear_flux();
+          }
+    }
+    tau += dt;
+    ghost_ideal();
+    if (trc.isViscous()) {
+      isformal();
+      if (status) return;
+      for (int iy = 0; iy < ny; iy++)
+        for (int iz = 0; iz < nz; iz++)
+          for (int ix = 0; ix < nx - 1; ix++) visc_flux(ix, iy, iz, 1);
+      for (int iz = 0; iz < nz; iz++)
+        for (int ix = 0; ix < nx; ix++)
+          for (int iy = 0; iy < ny - 1; iy++) visc_flux(ix, iy, iz, 2);
+      for (int ix = 0; ix < nx; ix++)
+        for (int iy = 0; iy < ny; iy++)
+          for (int iz = 0; iz < nz - 1; iz++) visc_flux(ix, iy, iz, 3);
+      for (int iy = 0; iy < ny; iy++)
+        for (int iz = 0; iz < nz; iz++)
+          for (int ix = 0; ix < nx; ix++) {
+            visc_source_step(ix, iy, iz);
+            OCell &c = g.at(ix, iy, iz);
+            // Cell::updateByViscFlux (src/cll.cpp:76-84)
+            if (std::fabs(c.flux[0]) <= 0.5 * c.Q[0]) {
+              for (int i = 0; i < 7; i++) c.Q[i] += c.flux[i];
+            } else if (c.flux[0] != 0.) {
+              const double fac = std::fabs(0.5 * c.Q[0] / c.flux[0]);
+              for (int i = 0; i < 7; i++) c.Q[i] += fac * c.flux[i];
+            }
+            c.clear_flux();
+          }
+    }
+    ghost_full();
+  }
+};
+
+}  // namespace
+
+// ------------------------------------------------------------------- C API
+extern "C" {
+
+void *or_create(int nx, int ny, int nz, double minx, double maxx, double miny,
+                double maxy, double minz, double maxz, int eos_kind,
+                const or_trc_t *trc) {
+  if (eos_kind != 0 || nx < 1 || ny < 1 || nz < 1) return nullptr;
+  Solver *s = new Solver;
+  Grid &g = s->g;
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  g.minx = minx;
+  g.maxx = maxx;
+  g.miny = miny;
+  g.maxy = maxy;
+  g.minz = minz;
+  g.maxz = maxz;
+  // Fluid ctor (src/fld.cpp:77-79); nz=1 gives dz=inf like the reference
+  g.dx = (maxx - minx) / (nx - 1);
+  g.dy = (maxy - miny) / (ny - 1);
+  g.dz = (maxz - minz) / (nz - 1);
+  g.c.assign((size_t)nx * ny * nz, OCell());
+  s->trc.c = *trc;
+  s->trc.eos = &s->eos;
+  return s;
+}
+
+void or_destroy(void *h) { delete static_cast<Solver *>(h); }
+
+long or_ncell(void *h) { return (long)static_cast<Solver *>(h)->g.c.size(); }
+
+void or_set_state(void *h, const or_state_t *st) {
+  Solver *s = static_cast<Solver *>(h);
+  const long N = (long)s->g.c.size();
+  for (long n = 0; n < N; n++) {
+    OCell &c = s->g.c[n];
+    for (int k = 0; k < 7; k++) {
+      if (st->Q) c.Q[k] = st->Q[k * N + n];
+      if (st->Qh) c.Qh[k] = st->Qh[k * N + n];
+      if (st->Qprev) c.Qprev[k] = st->Qprev[k * N + n];
+    }
+    for (int k = 0; k < 10; k++) {
+      if (st->pi) c.pi[k] = st->pi[k * N + n];
+      if (st->piH) c.piH[k] = st->piH[k * N + n];
+      if (st->pi0) c.pi0[k] = st->pi0[k * N + n];
+      if (st->piH0) c.piH0[k] = st->piH0[k * N + n];
+    }
+    if (st->Pi) c.Pi = st->Pi[n];
+    if (st->PiH) c.PiH = st->PiH[n];
+    if (st->Pi0) c.Pi0 = st->Pi0[n];
+    if (st->PiH0) c.PiH0 = st->PiH0[n];
+    for (int k = 0; k < 3; k++)
+      if (st->m) c.m[k] = st->m[k * N + n];
+    if (st->viscCorrCut) c.vcc = st->viscCorrCut[n];
+  }
+}
+
+void or_get_state(void *h, or_state_t *st) {
+  Solver *s = static_cast<Solver *>(h);
+  const long N = (long)s->g.c.size();
+  for (long n = 0; n < N; n++) {
+    const OCell &c = s->g.c[n];
+    for (int k = 0; k < 7; k++) {
+      if (st->Q) st->Q[k * N + n] = c.Q[k];
+      if (st->Qh) st->Qh[k * N + n] = c.Qh[k];
+      if (st->Qprev) st->Qprev[k * N + n] = c.Qprev[k];
+    }
+    for (int k = 0; k < 10; k++) {
+      if (st->pi) st->pi[k * N + n] = c.pi[k];
+      if (st->piH) st->piH[k * N + n] = c.piH[k];
+      if (st->pi0) st->pi0[k * N + n] = c.pi0[k];
+      if (st->piH0) st->piH0[k * N + n] = c.piH0[k];
+    }
+    if (st->Pi) st->Pi[n] = c.Pi;
+    if (st->PiH) st->PiH[n] = c.PiH;
+    if (st->Pi0) st->Pi0[n] = c.Pi0;
+    if (st->PiH0) st->PiH0[n] = c.PiH0;
+    for (int k = 0; k < 3; k++)
+      if (st->m) st->m[k * N + n] = c.m[k];
+    if (st->viscCorrCut) st->viscCorrCut[n] = c.vcc;
+  }
+}
+
+void or_set_prim(void *h, double tau, const double *e, const double *nb,
+                 const double *nq, const double *ns, const double *vx,
+                 const double *vy, const double *vz) {
+  Solver *s = static_cast<Solver *>(h);
+  const long N = (long)s->g.c.size();
+  for (long n = 0; n < N; n++) {
+    OCell &c = s->g.c[n];
+    // Cell::setPrimVar (src/cll.cpp:222-241)
+    const double g2 = 1. / (1 - vx[n] * vx[n] - vy[n] * vy[n] - vz[n] * vz[n]);
+    const double p = s->eos.p(e[n]);
+    c.Q[kT] = tau * (e[n] + p * (vx[n] * vx[n] + vy[n] * vy[n] + vz[n] * vz[n])) * g2;
+    c.Q[kX] = tau * (e[n] + p) * vx[n] * g2;
+    c.Q[kY] = tau * (e[n] + p) * vy[n] * g2;
+    c.Q[kZ] = tau * (e[n] + p) * vz[n] * g2;
+    c.Q[kNB] = tau * nb[n] * std::sqrt(g2);
+    c.Q[kNQ] = tau * nq[n] * std::sqrt(g2);
+    c.Q[kNS] = tau * ns[n] * std::sqrt(g2);
+    std::memcpy(c.Qprev, c.Q, sizeof c.Q);  // saveQprev
+    if (e[n] > 0.) c.m[0] = c.m[1] = c.m[2] = 1.;
+  }
+}
+
+int or_run(void *h, double *tau, double dt, int nsteps) {
+  Solver *s = static_cast<Solver *>(h);
+  s->tau = *tau;
+  s->dt = dt;
+  s->status = 0;
+  for (int i = 0; i < nsteps && !s->status; i++) s->step();
+  *tau = s->tau;
+  return s->status;
+}
+
+int or_check_dtau(void *h, double dt) {
+  Solver *s = static_cast<Solver *>(h);
+  return (dt > s->g.dx / 2. || dt > s->g.dy / 2.) ? 1 : 0;
+}
+
+void or_c2p(int, double *Q, long n, double *out, int *iters) {
+  IdealEos eos;
+  for (long i = 0; i < n; i++) {
+    double *o = out + 8 * i;
+    con2prim(eos, Q + 7 * i, o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7]);
+    if (iters) iters[i] = g_c2p_iters;
+  }
+}
+
+void or_p2c(const double *prim, long n, double *Q) {
+  for (long i = 0; i < n; i++) {
+    const double *p = prim + 8 * i;
+    prim2con(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], Q + 7 * i);
+  }
+}
+
+void or_eos(int, const double *e, long n, double *out) {
+  IdealEos eos;
+  for (long i = 0; i < n; i++) {
+    double T, mub, muq, mus, p;
+    eos.eos(e[i], T, mub, muq, mus, p);
+    out[4 * i + 0] = p;
+    out[4 * i + 1] = T;
+    out[4 * i + 2] = eos.s(e[i], 0., 0., 0.);
+    out[4 * i + 3] = eos.cs2e(e[i]);
+  }
+}
+
+void or_trcoeff(const or_trc_t *c, int, const double *in, long n, double *out) {
+  IdealEos eos;
+  Trc t;
+  t.c = *c;
+  t.eos = &eos;
+  for (long i = 0; i < n; i++) {
+    const double *a = in + 6 * i;
+    double es, zs, tp, tP;
+    t.getEta(a[0], a[1], a[2], a[3], a[4], a[5], es, zs);
+    t.getTau(a[0], a[1], a[2], a[3], a[4], a[5], tp, tP);
+    out[4 * i + 0] = es;
+    out[4 * i + 1] = zs;
+    out[4 * i + 2] = tp;
+    out[4 * i + 3] = tP;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+/*
+ * CPU oracle for the vHLLE hydro step: TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain, serial C++ restatement of the reference algorithm (yukarpenko/vhlle,
+ * src/hdo.cpp, src/cll.cpp, src/rmn.cpp, src/fld.cpp, src/trancoeff.*,
+ * src/eos.*) in the reference's own loop order and operation order, so that it
+ * reproduces the reference's FP64 results to rounding.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and only
+ * as the checker (or the timed CPU baseline) -- never as the product path.
+ *
+ * State layout used at this interface: structure-of-arrays, component-major,
+ * arr[comp * ncell + cell] with cell = ix + nx*(iy + ny*iz), i.e. the same cell
+ * ordering as Fluid::getCell (src/fld.h:70-78).
+ */
+#pragma once
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Transport coefficients: the fields of TransportCoeff (src/trancoeff.h:8-11)
+ * plus the T-peaked bulk parametrisation (zetaSparam 5) added by this build. */
+typedef struct {
+  double etaS0, zetaS0, ah, al, aRho, T0, etaSMin, etaSEpsilonMin, etaSShiftMuB,
+      etaSScaleMuB, zetaSPeakEpsilon, zetaSScaleBeta, zetaSSigmaMinus,
+      zetaSSigmaPlus;
+  int etaSparam, zetaSparam;
+  double zetaSPeakT, zetaSWidthMinus, zetaSWidthPlus;
+} or_trc_t;
+
+/* Whole-state view, SoA host arrays (any pointer may be NULL on set: field
+ * then left unchanged; on get: not written). */
+typedef struct {
+  double *Q, *Qh, *Qprev;     /* 7 x N each */
+  double *pi, *piH;           /* 10 x N each, packed index44 order */
+  double *Pi, *PiH;           /* N each */
+  double *pi0, *piH0;         /* 10 x N: auxiliary IS arrays */
+  double *Pi0, *PiH0;         /* N */
+  double *m;                  /* 3 x N */
+  double *viscCorrCut;        /* N */
+} or_state_t;
+
+void *or_create(int nx, int ny, int nz, double minx, double maxx, double miny,
+                double maxy, double minz, double maxz, int eos_kind,
+                const or_trc_t *trc);
+void or_destroy(void *h);
+long or_ncell(void *h);
+void or_set_state(void *h, const or_state_t *s);
+void or_get_state(void *h, or_state_t *s);
+/* Cell::setPrimVar + saveQprev + setAllM(1 if e>0): the IC path of
+ * ICGubser::setIC (src/icGubser.cpp:17-62). prim arrays are N each. */
+void or_set_prim(void *h, double tau, const double *e, const double *nb,
+                 const double *nq, const double *ns, const double *vx,
+                 const double *vy, const double *vz);
+/* Hydro::performStep (src/hdo.cpp:1016-1134) nsteps times from *tau with
+ * timestep dt.  Returns 0 on success, otherwise the reference's exit reason:
+ * 1 = NaN in hlle_flux / source_step / NSquant, 2 = e>1e10 (exit(0) in the
+ * reference). */
+int or_run(void *h, double *tau, double dt, int nsteps);
+/* Hydro::setDtau validity test (src/hdo.cpp:67-75): 0 if accepted, 1 if the
+ * reference would exit(1). */
+int or_check_dtau(void *h, double dt);
+
+/* transformPV (src/rmn.cpp:28-139) over n inputs.  Q: n x 7 row-major (it is
+ * modified in place exactly like the reference modifies its argument).
+ * out: n x 8 row-major {e, p, nb, nq, ns, vx, vy, vz}.  iters (may be NULL):
+ * loop iterations executed per input. */
+void or_c2p(int eos_kind, double *Q, long n, double *out, int *iters);
+/* transformCV (src/rmn.cpp:253-263): prim n x 8 {e,p,nb,nq,ns,vx,vy,vz} -> Q n x 7 */
+void or_p2c(const double *prim, long n, double *Q);
+/* EoS relations of the ideal gas p=e/3 (src/eos.cpp:87-111 SIMPLE branch):
+ * out n x 4 {p, T, s, cs2}. */
+void or_eos(int eos_kind, const double *e, long n, double *out);
+/* TransportCoeff::getEta + getTau (src/trancoeff.cpp:19-100) on n states
+ * {e, nb, T, muB, s, p} -> out n x 4 {etaS, zetaS, taupi, tauPi}. */
+void or_trcoeff(const or_trc_t *trc, int eos_kind, const double *in, long n,
+                double *out);
+
+#ifdef __cplusplus
+}
+#endif

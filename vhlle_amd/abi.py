@@ -1,0 +1,217 @@
+"""ctypes binding of the C-ABI in include/vhlle_gpu.h (libvhlle_gpu.so).
+
+This is plumbing for tests and the benchmark: it loads the in-tree HIP library
+and fails loudly if it is missing -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvhlle_gpu.so")
+
+VH_OK, VH_ERR_ARG, VH_ERR_HIP, VH_ERR_NODEVICE, VH_ERR_DTAU = 0, -1, -2, -3, -4
+ST_NAN_FLUX, ST_BIG_E, ST_NAN_SOURCE, ST_NAN_PI = 1, 2, 4, 8
+PREC_FP64, PREC_MIXED = 0, 1
+C2P_REFERENCE, C2P_LEARNED, C2P_CLOSED = 0, 1, 2
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int)] + [
+        (n, ctypes.c_double) for n in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")]
+
+
+class Eos(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int)]
+
+
+class Trc(ctypes.Structure):
+    """vh_trcoeff_t: TransportCoeff fields (src/trancoeff.h:8-11) + zetaSparam 5."""
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "etaS0", "zetaS0", "ah", "al", "aRho", "T0", "etaSMin", "etaSEpsilonMin",
+        "etaSShiftMuB", "etaSScaleMuB", "zetaSPeakEpsilon", "zetaSScaleBeta",
+        "zetaSSigmaMinus", "zetaSSigmaPlus")] + [
+        ("etaSparam", ctypes.c_int), ("zetaSparam", ctypes.c_int),
+        ("zetaSPeakT", ctypes.c_double), ("zetaSWidthMinus", ctypes.c_double),
+        ("zetaSWidthPlus", ctypes.c_double)]
+
+
+class State(ctypes.Structure):
+    _fields_ = [(n, _dp) for n in ("Q", "Qh", "Qprev", "pi", "piH", "Pi", "PiH", "m",
+                                   "viscCorrCut")]
+
+
+STATE_FIELDS = {"Q": 7, "Qh": 7, "Qprev": 7, "pi": 10, "piH": 10, "Pi": 1, "PiH": 1, "m": 3,
+                "viscCorrCut": 1}
+
+
+def make_trc(etaS=0.0, zetaS=0.0, etaSparam=0, zetaSparam=0, **kw):
+    """TransportCoeff with the reference driver's defaults (src/main.cpp:70-81)."""
+    d = dict(etaS0=etaS, zetaS0=zetaS, ah=0.0, al=0.0, aRho=0.0, T0=0.15, etaSMin=0.08,
+             etaSEpsilonMin=5.0, etaSShiftMuB=0.0, etaSScaleMuB=0.0, zetaSPeakEpsilon=5.0,
+             zetaSScaleBeta=0.103, zetaSSigmaMinus=0.1, zetaSSigmaPlus=0.1,
+             etaSparam=etaSparam, zetaSparam=zetaSparam, zetaSPeakT=0.170,
+             zetaSWidthMinus=0.010, zetaSWidthPlus=0.030)
+    d.update(kw)
+    return Trc(**d)
+
+
+_LIB = None
+
+
+def lib():
+    """Loads libvhlle_gpu.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} missing: run `python __graft_entry__.py build` first")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    ip = ctypes.POINTER(ctypes.c_int)
+    sig = {
+        "vh_create": (ctypes.c_int, [ctypes.POINTER(Grid), ctypes.POINTER(Eos),
+                                     ctypes.POINTER(Trc), ctypes.c_int, ctypes.POINTER(vp)]),
+        "vh_destroy": (ctypes.c_int, [vp]),
+        "vh_ncell": (ctypes.c_long, [vp]),
+        "vh_upload": (ctypes.c_int, [vp, ctypes.POINTER(State)]),
+        "vh_download": (ctypes.c_int, [vp, ctypes.POINTER(State)]),
+        "vh_set_prim": (ctypes.c_int, [vp, ctypes.c_double] + [_dp] * 7),
+        "vh_step": (ctypes.c_int, [vp, _dp, ctypes.c_double, ip]),
+        "vh_run": (ctypes.c_int, [vp, _dp, ctypes.c_double, ctypes.c_int, ip]),
+        "vh_check_dtau": (ctypes.c_int, [vp, ctypes.c_double]),
+        "vh_set_c2p_mode": (ctypes.c_int, [vp, ctypes.c_int]),
+        "vh_enable_timing": (ctypes.c_int, [vp, ctypes.c_int]),
+        "vh_kernel_times": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_void_p, ip, _dp]),
+        "vh_c2p": (ctypes.c_int, [ctypes.POINTER(Eos), _dp, ctypes.c_long, _dp, ip]),
+        "vh_p2c": (ctypes.c_int, [_dp, ctypes.c_long, _dp]),
+        "vh_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def exported_symbols():
+    return ["vh_create", "vh_destroy", "vh_ncell", "vh_upload", "vh_download", "vh_set_prim",
+            "vh_step", "vh_run", "vh_check_dtau", "vh_set_c2p_mode", "vh_enable_timing",
+            "vh_kernel_times", "vh_c2p", "vh_p2c", "vh_device_info"]
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def check(rc, what="call"):
+    if rc != VH_OK:
+        raise RuntimeError(f"vhlle_gpu {what} failed with code {rc}")
+
+
+def device_info():
+    buf = ctypes.create_string_buffer(256)
+    check(lib().vh_device_info(buf, 256), "vh_device_info")
+    return buf.value.decode()
+
+
+def c2p(Q):
+    """transformPV on the GPU over rows of Q (n x 7): (prim n x 8, iters, Q_after)."""
+    Q = np.ascontiguousarray(Q, dtype=np.float64).copy()
+    n = Q.shape[0]
+    out = np.zeros((n, 8))
+    it = np.zeros(n, dtype=np.int32)
+    e = Eos(0)
+    check(lib().vh_c2p(ctypes.byref(e), _p(Q), n, _p(out),
+                       it.ctypes.data_as(ctypes.POINTER(ctypes.c_int))), "vh_c2p")
+    return out, it, Q
+
+
+def p2c(prim):
+    prim = np.ascontiguousarray(prim, dtype=np.float64)
+    Q = np.zeros((prim.shape[0], 7))
+    check(lib().vh_p2c(_p(prim), prim.shape[0], _p(Q)), "vh_p2c")
+    return Q
+
+
+class Context:
+    """One Fluid + Hydro on the GPU through the C-ABI."""
+
+    def __init__(self, nx, ny, nz, xlim, ylim, zlim, trc=None, precision=PREC_FP64):
+        self.trc = trc if trc is not None else make_trc()
+        g = Grid(nx, ny, nz, xlim[0], xlim[1], ylim[0], ylim[1], zlim[0], zlim[1])
+        h = ctypes.c_void_p()
+        check(lib().vh_create(ctypes.byref(g), ctypes.byref(Eos(0)), ctypes.byref(self.trc),
+                              precision, ctypes.byref(h)), "vh_create")
+        self.h = h
+        self.N = nx * ny * nz
+        self.shape = (nx, ny, nz)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_prim(self, tau, e, vx, vy, vz, nb=None, nq=None, ns=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel()
+                for a in (e, nb, nq, ns, vx, vy, vz)]
+        ptrs = [None if a is None else _p(a) for a in arrs]
+        check(lib().vh_set_prim(self.h, tau, *ptrs), "vh_set_prim")
+
+    def upload(self, d):
+        st = State()
+        keep = []
+        for k, nc in STATE_FIELDS.items():
+            if k in d:
+                a = np.ascontiguousarray(d[k], dtype=np.float64).reshape(-1)
+                assert a.size == nc * self.N, k
+                keep.append(a)
+                setattr(st, k, _p(a))
+        check(lib().vh_upload(self.h, ctypes.byref(st)), "vh_upload")
+
+    def download(self):
+        st = State()
+        out = {}
+        for k, nc in STATE_FIELDS.items():
+            a = np.zeros(nc * self.N)
+            out[k] = a
+            setattr(st, k, _p(a))
+        check(lib().vh_download(self.h, ctypes.byref(st)), "vh_download")
+        return {k: v.reshape(STATE_FIELDS[k], self.N) for k, v in out.items()}
+
+    def step(self, tau, dt):
+        t = ctypes.c_double(tau)
+        s = ctypes.c_int(0)
+        check(lib().vh_step(self.h, ctypes.byref(t), dt, ctypes.byref(s)), "vh_step")
+        return s.value, t.value
+
+    def run(self, tau, dt, nsteps):
+        t = ctypes.c_double(tau)
+        s = ctypes.c_int(0)
+        check(lib().vh_run(self.h, ctypes.byref(t), dt, nsteps, ctypes.byref(s)), "vh_run")
+        return s.value, t.value
+
+    def enable_timing(self, on=True):
+        check(lib().vh_enable_timing(self.h, 1 if on else 0), "vh_enable_timing")
+
+    def kernel_times(self):
+        mx = 64
+        names = (ctypes.c_char * 32 * mx)()
+        launches = np.zeros(mx, dtype=np.int32)
+        ms = np.zeros(mx)
+        n = lib().vh_kernel_times(self.h, mx, names,
+                                  launches.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _p(ms))
+        return {names[i].value.decode(): (int(launches[i]), float(ms[i])) for i in range(n)}
+
+    def check_dtau(self, dt):
+        return lib().vh_check_dtau(self.h, dt)

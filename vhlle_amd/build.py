@@ -1,0 +1,80 @@
+"""Builds the native libraries of vhlle_amd in-tree.
+
+libvhlle_gpu.so : HIP kernels + C-ABI (include/vhlle_gpu.h) + C++ facade,
+                  compiled for gfx950 by hipcc.
+The parity-critical FP64 kernels are compiled with -ffp-contract=off (no FMA
+contraction), matching the reference's generic x86-64 build.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+HOST = os.path.join(HERE, "host")
+LIB = os.path.join(HERE, "libvhlle_gpu.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# packed FP32 instructions are disabled on purpose (wrong results seen on the
+# target pool); the host pass warns that it does not know the feature.
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-command-line-argument"]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l
+                    and "not a recognized feature" not in l)
+    if r.returncode != 0:
+        sys.stderr.write(out + "\n")
+        raise RuntimeError("build failed: " + " ".join(cmd[:3]) + " ...")
+    return out
+
+
+def hip_sources():
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+
+
+def host_sources():
+    if not os.path.isdir(HOST):
+        return []
+    return sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".cpp"))
+
+
+def build_lib(verbose=False):
+    """Compile every .hip kernel/API source and the host facade into one .so."""
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    for src in hip_sources() + host_sources():
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        if src.endswith(".hip"):
+            cmd = [HIPCC] + HIP_FLAGS + ["-I", ROOT, "-c", src, "-o", obj]
+        else:
+            cmd = [HIPCC, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-I", ROOT,
+                   "-c", src, "-o", obj]
+        out = _run(cmd)
+        if verbose and out.strip():
+            print(out)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle():
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+
+
+def build_all(verbose=False):
+    build_oracle()
+    return build_lib(verbose)
+
+
+if __name__ == "__main__":
+    print(build_all(verbose=True))

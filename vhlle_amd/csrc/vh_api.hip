@@ -1,0 +1,532 @@
+// C-ABI of the vHLLE hydro step (include/vhlle_gpu.h): context, device
+// memory, the step sequence of Hydro::performStep (src/hdo.cpp:1016-1134) as
+// kernel launches on one HIP stream, and its replay as HIP graphs.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/vhlle_gpu.h"
+#include "vh_kernels.h"
+
+#define VH_HIP(call)                                                         \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      std::fprintf(stderr, "vhlle_gpu: %s failed: %s (%s:%d)\n", #call,      \
+                   hipGetErrorString(e_), __FILE__, __LINE__);               \
+      return VH_ERR_HIP;                                                     \
+    }                                                                        \
+  } while (0)
+
+namespace {
+
+// EoSs::t constant (src/eos.cpp:98-99), evaluated like the reference's
+// compile-time expression.
+double ideal_cnst() {
+  const double pi_ = 3.14159265358979312;
+  return (16 + 0.5 * 21.0 * 2.5) * (pi_ * pi_) / 30.0 /
+         (0.197326968 * 0.197326968 * 0.197326968);
+}
+
+struct KTime {
+  int launches = 0;
+  double ms = 0.;
+};
+
+}  // namespace
+
+struct vh_ctx {
+  vh::Geom g;
+  vh::DevEos eos;
+  vh::DevTrc trc;
+  vh_grid_t grid;
+  int precision = 0, c2p_mode = 0;
+  bool viscous = false;
+  int dev = 0;
+  hipStream_t st = nullptr;
+  // device memory (one allocation)
+  char *pool = nullptr;
+  double *Q[2] = {nullptr, nullptr}, *Qh = nullptr;
+  double *pi = nullptr, *piH = nullptr, *Pi = nullptr, *PiH = nullptr;
+  double *pi0 = nullptr, *piH0 = nullptr, *Pi0 = nullptr, *PiH0 = nullptr;
+  double *m = nullptr, *dm = nullptr, *vcc = nullptr;
+  double *F[3] = {nullptr, nullptr, nullptr};
+  unsigned char *fl[3] = {nullptr, nullptr, nullptr};
+  double *ph = nullptr, *pt = nullptr, *vf = nullptr;
+  double *tp = nullptr;
+  int *status = nullptr;
+  int cur = 0;
+  int m_full = 0;
+  // graph cache: one executable per parity of cur
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  double g_dt = 0.;
+  int g_mfull = -1;
+  // timing
+  bool timing = false;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> spare;
+  std::map<std::string, KTime> times;
+};
+
+namespace {
+
+template <class F>
+int timed(vh_ctx *c, const char *name, F &&launch) {
+  if (!c->timing) {
+    launch();
+    return VH_OK;
+  }
+  std::pair<hipEvent_t, hipEvent_t> ev;
+  if (!c->spare.empty()) {
+    ev = c->spare.back();
+    c->spare.pop_back();
+  } else {
+    VH_HIP(hipEventCreate(&ev.first));
+    VH_HIP(hipEventCreate(&ev.second));
+  }
+  VH_HIP(hipEventRecord(ev.first, c->st));
+  launch();
+  VH_HIP(hipEventRecord(ev.second, c->st));
+  c->pending.push_back({name, ev});
+  return VH_OK;
+}
+
+int collect_times(vh_ctx *c) {
+  for (auto &p : c->pending) {
+    float ms = 0.f;
+    VH_HIP(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+    KTime &t = c->times[p.first];
+    t.launches++;
+    t.ms += ms;
+    c->spare.push_back(p.second);
+  }
+  c->pending.clear();
+  return VH_OK;
+}
+
+const char *face_name(int D, int mode) {
+  static const char *n[2][3] = {{"faces_x_pred", "faces_y_pred", "faces_z_pred"},
+                                {"faces_x_corr", "faces_y_corr", "faces_z_corr"}};
+  return n[mode][D - 1];
+}
+
+// Enqueues one Hydro::performStep on c->st (src/hdo.cpp:1016-1134).  Reads
+// tau from c->tp, advances it on the device.  Q[cur] -> Q[cur^1].
+int enqueue_step(vh_ctx *c, double dt) {
+  const vh::Geom &g = c->g;
+  hipStream_t st = c->st;
+  double *Qo = c->Q[c->cur], *Qn = c->Q[c->cur ^ 1];
+  int rc = VH_OK;
+  // Fluid::updateM (src/fld.cpp:384-423): the identity once every m == 1
+  if (!c->m_full) {
+    rc |= timed(c, "update_m_dm", [&] { vh::launch_update_m(0, g, c->m, c->dm, c->tp, dt, st); });
+    rc |= timed(c, "update_m_apply", [&] { vh::launch_update_m(1, g, c->m, c->dm, c->tp, dt, st); });
+  }
+  const int ext[3] = {g.nx, g.ny, g.nz};
+  for (int mode = 0; mode < 2; mode++) {
+    const double *src = mode == 0 ? Qo : c->Qh;
+    for (int D = 1; D <= 3; D++) {
+      if (ext[D - 1] < 2) continue;  // no interfaces along a 1-cell axis
+      rc |= timed(c, face_name(D, mode), [&] {
+        vh::launch_faces(D, mode, g, c->eos, src, c->m, c->m_full, c->tp, dt, c->F[D - 1],
+                         c->fl[D - 1], c->status, st);
+      });
+    }
+    const double *Fs[3] = {c->F[0], c->F[1], c->F[2]};
+    const unsigned char *fs[3] = {c->fl[0], c->fl[1], c->fl[2]};
+    rc |= timed(c, mode == 0 ? "update_pred" : "update_corr", [&] {
+      vh::launch_update(mode, g, c->eos, Qo, c->Qh, Fs, fs, c->tp, dt, mode == 0 ? c->Qh : Qn,
+                        c->status, st);
+    });
+  }
+  rc |= timed(c, "advance_tau", [&] { vh::launch_advance_tau(c->tp, dt, st); });
+  // Fluid::correctImagCells (src/fld.cpp:211-273): z, y, x
+  for (int ax = 3; ax >= 1; ax--)
+    rc |= timed(c, "ghost_ideal", [&] { vh::launch_ghost(0, ax, g, Qn, c->Qh, c->pi, c->Pi, st); });
+  if (c->viscous) {
+    rc |= timed(c, "visc_prim", [&] { vh::launch_vprim(g, c->eos, c->Qh, c->tp, dt, c->ph, c->pt, st); });
+    rc |= timed(c, "visc_relax", [&] {
+      vh::launch_visc1(g, c->eos, c->trc, Qo, Qn, c->ph, c->pt, c->m, c->m_full, c->pi, c->Pi,
+                       c->tp, dt, c->pi0, c->piH0, c->Pi0, c->PiH0, c->status, st);
+    });
+    rc |= timed(c, "visc_advect", [&] {
+      vh::launch_visc2(g, c->ph, c->pi0, c->piH0, c->Pi0, c->PiH0, c->pi, c->piH, c->Pi, c->PiH,
+                       c->vcc, c->vf, c->tp, dt, st);
+    });
+    rc |= timed(c, "visc_flux_update", [&] {
+      vh::launch_visc3(g, c->ph, c->piH, c->PiH, c->vf, c->m, c->m_full, c->tp, dt, Qn, st);
+    });
+  }
+  // Fluid::correctImagCellsFull (src/fld.cpp:275-382): z, y, x
+  for (int ax = 3; ax >= 1; ax--)
+    rc |= timed(c, "ghost_full", [&] { vh::launch_ghost(1, ax, g, Qn, c->Qh, c->pi, c->Pi, st); });
+  VH_HIP(hipGetLastError());
+  c->cur ^= 1;
+  return rc ? VH_ERR_HIP : VH_OK;
+}
+
+void drop_graphs(vh_ctx *c) {
+  for (int i = 0; i < 2; i++)
+    if (c->gexec[i]) {
+      (void)hipGraphExecDestroy(c->gexec[i]);
+      c->gexec[i] = nullptr;
+    }
+  c->g_mfull = -1;
+}
+
+int build_graphs(vh_ctx *c, double dt) {
+  if (c->gexec[0] && c->gexec[1] && c->g_dt == dt && c->g_mfull == c->m_full) return VH_OK;
+  drop_graphs(c);
+  const int cur0 = c->cur;
+  for (int k = 0; k < 2; k++) {
+    const int par = cur0 ^ k;  // graph for steps that start from Q[par]
+    c->cur = par;
+    hipGraph_t graph;
+    VH_HIP(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    const bool t = c->timing;
+    c->timing = false;
+    int rc = enqueue_step(c, dt);
+    c->timing = t;
+    VH_HIP(hipStreamEndCapture(c->st, &graph));
+    if (rc != VH_OK) {
+      (void)hipGraphDestroy(graph);
+      c->cur = cur0;
+      return rc;
+    }
+    VH_HIP(hipGraphInstantiate(&c->gexec[par], graph, nullptr, nullptr, 0));
+    VH_HIP(hipGraphDestroy(graph));
+  }
+  c->cur = cur0;
+  c->g_dt = dt;
+  c->g_mfull = c->m_full;
+  return VH_OK;
+}
+
+int refresh_m_full(vh_ctx *c) {
+  std::vector<double> h((size_t)3 * c->g.N);
+  VH_HIP(hipMemcpyAsync(h.data(), c->m, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  VH_HIP(hipStreamSynchronize(c->st));
+  int full = 1;
+  for (double v : h)
+    if (!(v >= 1.)) {
+      full = 0;
+      break;
+    }
+  c->m_full = full;
+  return VH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vh_create(const vh_grid_t *grid, const vh_eos_t *eos, const vh_trcoeff_t *trc,
+              int precision, vh_ctx_t **out) {
+  if (!grid || !eos || !trc || !out) return VH_ERR_ARG;
+  if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1 || eos->kind != VH_EOS_IDEAL) return VH_ERR_ARG;
+  if (precision != VH_PREC_FP64) return VH_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return VH_ERR_NODEVICE;
+  vh_ctx *c = new vh_ctx;
+  c->grid = *grid;
+  c->precision = precision;
+  vh::Geom &g = c->g;
+  g.nx = grid->nx;
+  g.ny = grid->ny;
+  g.nz = grid->nz;
+  g.N = (long)g.nx * g.ny * g.nz;
+  // Fluid ctor (src/fld.cpp:77-79)
+  g.dx = (grid->xmax - grid->xmin) / (g.nx - 1);
+  g.dy = (grid->ymax - grid->ymin) / (g.ny - 1);
+  g.dz = (grid->zmax - grid->zmin) / (g.nz - 1);
+  c->eos.kind = eos->kind;
+  c->eos.cnst = ideal_cnst();
+  c->trc.c = *trc;
+  // TransportCoeff::isViscous (src/trancoeff.h:39-44)
+  c->viscous = trc->etaS0 > 0. || trc->zetaS0 > 0. || trc->etaSparam > 0 || trc->zetaSparam > 0;
+  if (hipGetDevice(&c->dev) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess) {
+    delete c;
+    return VH_ERR_HIP;
+  }
+  // one pool, every array 256-byte aligned
+  const long N = g.N;
+  const size_t al = 256;
+  auto rnd = [&](size_t b) { return (b + al - 1) / al * al; };
+  const size_t dN = rnd(N * sizeof(double)), bN = rnd(N);
+  const size_t ndoubles_arrays = 14 + 7 + 20 + 2 + 20 + 2 + 3 + 3 + 1 + 21 + 8 + 4 + 4 + 1;
+  const size_t total = ndoubles_arrays * dN + 3 * bN + 2 * al;
+  if (hipMalloc(&c->pool, total) != hipSuccess) {
+    (void)hipStreamDestroy(c->st);
+    delete c;
+    return VH_ERR_HIP;
+  }
+  char *p = c->pool;
+  // component k of a field lives at base + k*N: pack components contiguously
+  auto takef = [&](int ncomp) {
+    double *r = reinterpret_cast<double *>(p);
+    p += rnd((size_t)ncomp * N * sizeof(double));
+    return r;
+  };
+  c->Q[0] = takef(7);
+  c->Q[1] = takef(7);
+  c->Qh = takef(7);
+  c->pi = takef(10);
+  c->piH = takef(10);
+  c->Pi = takef(1);
+  c->PiH = takef(1);
+  c->pi0 = takef(10);
+  c->piH0 = takef(10);
+  c->Pi0 = takef(1);
+  c->PiH0 = takef(1);
+  c->m = takef(3);
+  c->dm = takef(3);
+  c->vcc = takef(1);
+  for (int d = 0; d < 3; d++) c->F[d] = takef(7);
+  c->ph = takef(8);
+  c->pt = takef(4);
+  c->vf = takef(4);
+  c->tp = takef(1);
+  c->status = reinterpret_cast<int *>(p);
+  p += al;
+  for (int d = 0; d < 3; d++) {
+    c->fl[d] = reinterpret_cast<unsigned char *>(p);
+    p += bN;
+  }
+  // Cell ctor defaults (src/cll.cpp:33-52): zeros, viscCorrCut = 1, m = 0
+  if (hipMemsetAsync(c->pool, 0, total, c->st) != hipSuccess) {
+    vh_destroy(c);
+    return VH_ERR_HIP;
+  }
+  std::vector<double> ones(N, 1.0);
+  if (hipMemcpyAsync(c->vcc, ones.data(), N * sizeof(double), hipMemcpyHostToDevice, c->st) !=
+          hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess) {
+    vh_destroy(c);
+    return VH_ERR_HIP;
+  }
+  *out = c;
+  return VH_OK;
+}
+
+int vh_destroy(vh_ctx_t *c) {
+  if (!c) return VH_ERR_ARG;
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  drop_graphs(c);
+  for (auto &p : c->pending) c->spare.push_back(p.second);
+  for (auto &e : c->spare) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  if (c->pool) (void)hipFree(c->pool);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+  return VH_OK;
+}
+
+long vh_ncell(const vh_ctx_t *c) { return c ? c->g.N : 0; }
+
+int vh_upload(vh_ctx_t *c, const vh_state_t *h) {
+  if (!c || !h) return VH_ERR_ARG;
+  const long N = c->g.N;
+  const size_t d = sizeof(double);
+  auto put = [&](double *dst, const double *src, int nc, double dflt) -> int {
+    if (src) {
+      VH_HIP(hipMemcpyAsync(dst, src, nc * N * d, hipMemcpyHostToDevice, c->st));
+    } else {
+      std::vector<double> z((size_t)nc * N, dflt);
+      VH_HIP(hipMemcpyAsync(dst, z.data(), nc * N * d, hipMemcpyHostToDevice, c->st));
+      VH_HIP(hipStreamSynchronize(c->st));
+    }
+    return VH_OK;
+  };
+  int rc = VH_OK;
+  rc |= put(c->Q[c->cur], h->Q, 7, 0.);
+  rc |= put(c->Q[c->cur ^ 1], h->Qprev, 7, 0.);
+  rc |= put(c->Qh, h->Qh, 7, 0.);
+  rc |= put(c->pi, h->pi, 10, 0.);
+  rc |= put(c->piH, h->piH, 10, 0.);
+  rc |= put(c->Pi, h->Pi, 1, 0.);
+  rc |= put(c->PiH, h->PiH, 1, 0.);
+  rc |= put(c->m, h->m, 3, 0.);
+  rc |= put(c->vcc, h->viscCorrCut, 1, 1.);
+  VH_HIP(hipStreamSynchronize(c->st));
+  if (rc != VH_OK) return VH_ERR_HIP;
+  return refresh_m_full(c);
+}
+
+int vh_download(vh_ctx_t *c, vh_state_t *h) {
+  if (!c || !h) return VH_ERR_ARG;
+  const long N = c->g.N;
+  const size_t d = sizeof(double);
+  auto get = [&](double *dst, const double *src, int nc) -> int {
+    if (dst) VH_HIP(hipMemcpyAsync(dst, src, nc * N * d, hipMemcpyDeviceToHost, c->st));
+    return VH_OK;
+  };
+  int rc = VH_OK;
+  rc |= get(h->Q, c->Q[c->cur], 7);
+  rc |= get(h->Qprev, c->Q[c->cur ^ 1], 7);
+  rc |= get(h->Qh, c->Qh, 7);
+  rc |= get(h->pi, c->pi, 10);
+  rc |= get(h->piH, c->piH, 10);
+  rc |= get(h->Pi, c->Pi, 1);
+  rc |= get(h->PiH, c->PiH, 1);
+  rc |= get(h->m, c->m, 3);
+  rc |= get(h->viscCorrCut, c->vcc, 1);
+  VH_HIP(hipStreamSynchronize(c->st));
+  return rc ? VH_ERR_HIP : VH_OK;
+}
+
+int vh_set_prim(vh_ctx_t *c, double tau, const double *e, const double *nb, const double *nq,
+                const double *ns, const double *vx, const double *vy, const double *vz) {
+  if (!c || !e || !vx || !vy || !vz) return VH_ERR_ARG;
+  const long N = c->g.N;
+  std::vector<double> h((size_t)7 * N, 0.);
+  const double *src[7] = {e, nb, nq, ns, vx, vy, vz};
+  for (int k = 0; k < 7; k++)
+    if (src[k]) std::memcpy(&h[(size_t)k * N], src[k], N * sizeof(double));
+  double *d = nullptr;
+  VH_HIP(hipMalloc(&d, h.size() * sizeof(double)));
+  int rc = VH_OK;
+  if (hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->st) !=
+      hipSuccess)
+    rc = VH_ERR_HIP;
+  if (rc == VH_OK) {
+    vh::launch_set_prim(c->g, c->eos, tau, d, c->Q[c->cur], c->Q[c->cur ^ 1], c->m, c->st);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->st) != hipSuccess)
+      rc = VH_ERR_HIP;
+  }
+  (void)hipFree(d);
+  if (rc != VH_OK) return rc;
+  return refresh_m_full(c);
+}
+
+int vh_check_dtau(const vh_ctx_t *c, double dt) {
+  if (!c) return VH_ERR_ARG;
+  return (dt > c->g.dx / 2. || dt > c->g.dy / 2.) ? VH_ERR_DTAU : VH_OK;
+}
+
+int vh_set_c2p_mode(vh_ctx_t *c, int mode) {
+  if (!c || mode != VH_C2P_REFERENCE) return VH_ERR_ARG;
+  c->c2p_mode = mode;
+  drop_graphs(c);
+  return VH_OK;
+}
+
+static int run_impl(vh_ctx_t *c, double *tau, double dt, int nsteps, int *status, bool graph) {
+  if (!c || !tau || nsteps < 0) return VH_ERR_ARG;
+  const bool use_graph = graph && !c->timing && nsteps > 1;
+  if (use_graph) {  // capture before any work of this call is enqueued
+    int rc = build_graphs(c, dt);
+    if (rc != VH_OK) return rc;
+  }
+  vh::launch_fill_int(c->status, 0, c->st);
+  VH_HIP(hipMemcpyAsync(c->tp, tau, sizeof(double), hipMemcpyHostToDevice, c->st));
+  if (use_graph) {
+    for (int s = 0; s < nsteps; s++) {
+      VH_HIP(hipGraphLaunch(c->gexec[c->cur], c->st));
+      c->cur ^= 1;
+    }
+  } else {
+    for (int s = 0; s < nsteps; s++) {
+      int rc = enqueue_step(c, dt);
+      if (rc != VH_OK) return rc;
+    }
+  }
+  int st = 0;
+  VH_HIP(hipMemcpyAsync(tau, c->tp, sizeof(double), hipMemcpyDeviceToHost, c->st));
+  VH_HIP(hipMemcpyAsync(&st, c->status, sizeof(int), hipMemcpyDeviceToHost, c->st));
+  VH_HIP(hipStreamSynchronize(c->st));
+  if (c->timing) {
+    int rc = collect_times(c);
+    if (rc != VH_OK) return rc;
+  }
+  if (status) *status = st;
+  if (!c->m_full) return refresh_m_full(c);
+  return VH_OK;
+}
+
+int vh_step(vh_ctx_t *c, double *tau, double dt, int *status) {
+  return run_impl(c, tau, dt, 1, status, false);
+}
+
+int vh_run(vh_ctx_t *c, double *tau, double dt, int nsteps, int *status) {
+  return run_impl(c, tau, dt, nsteps, status, true);
+}
+
+int vh_enable_timing(vh_ctx_t *c, int enable) {
+  if (!c) return VH_ERR_ARG;
+  c->timing = enable != 0;
+  c->times.clear();
+  return VH_OK;
+}
+
+int vh_kernel_times(const vh_ctx_t *c, int max, char names[][32], int *launches, double *ms) {
+  if (!c) return VH_ERR_ARG;
+  int i = 0;
+  for (auto &kv : c->times) {
+    if (i >= max) break;
+    std::snprintf(names[i], 32, "%s", kv.first.c_str());
+    launches[i] = kv.second.launches;
+    ms[i] = kv.second.ms;
+    i++;
+  }
+  return i;
+}
+
+int vh_c2p(const vh_eos_t *eos, double *Q, long n, double *out, int *iters) {
+  if (!eos || !Q || !out || n < 0) return VH_ERR_ARG;
+  if (n == 0) return VH_OK;
+  vh::DevEos de;
+  de.kind = eos->kind;
+  de.cnst = ideal_cnst();
+  double *dQ = nullptr, *dO = nullptr;
+  int *dI = nullptr;
+  VH_HIP(hipMalloc(&dQ, n * 7 * sizeof(double)));
+  VH_HIP(hipMalloc(&dO, n * 8 * sizeof(double)));
+  if (iters) VH_HIP(hipMalloc(&dI, n * sizeof(int)));
+  VH_HIP(hipMemcpy(dQ, Q, n * 7 * sizeof(double), hipMemcpyHostToDevice));
+  vh::launch_c2p_batch(de, dQ, n, dO, dI, nullptr);
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipMemcpy(out, dO, n * 8 * sizeof(double), hipMemcpyDeviceToHost));
+  VH_HIP(hipMemcpy(Q, dQ, n * 7 * sizeof(double), hipMemcpyDeviceToHost));
+  if (iters) VH_HIP(hipMemcpy(iters, dI, n * sizeof(int), hipMemcpyDeviceToHost));
+  (void)hipFree(dQ);
+  (void)hipFree(dO);
+  if (dI) (void)hipFree(dI);
+  return VH_OK;
+}
+
+int vh_p2c(const double *prim, long n, double *Q) {
+  if (!prim || !Q || n < 0) return VH_ERR_ARG;
+  if (n == 0) return VH_OK;
+  double *dP = nullptr, *dQ = nullptr;
+  VH_HIP(hipMalloc(&dP, n * 8 * sizeof(double)));
+  VH_HIP(hipMalloc(&dQ, n * 7 * sizeof(double)));
+  VH_HIP(hipMemcpy(dP, prim, n * 8 * sizeof(double), hipMemcpyHostToDevice));
+  vh::launch_p2c_batch(dP, n, dQ, nullptr);
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipMemcpy(Q, dQ, n * 7 * sizeof(double), hipMemcpyDeviceToHost));
+  (void)hipFree(dP);
+  (void)hipFree(dQ);
+  return VH_OK;
+}
+
+int vh_device_info(char *buf, int len) {
+  if (!buf || len <= 0) return VH_ERR_ARG;
+  int dev = 0, n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return VH_ERR_NODEVICE;
+  VH_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t p;
+  VH_HIP(hipGetDeviceProperties(&p, dev));
+  std::snprintf(buf, len, "%s (%s) CUs=%d clock=%d kHz mem=%.1f GiB memclk=%d kHz bus=%d bit",
+                p.name, p.gcnArchName, p.multiProcessorCount, p.clockRate,
+                p.totalGlobalMem / 1073741824.0, p.memoryClockRate, p.memoryBusWidth);
+  return VH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,251 @@
+// Device-side physics of the vHLLE hydro step for gfx950 (FP64 faithful
+// path).  Every function reproduces the reference's floating-point operation
+// order so that, compiled with -ffp-contract=off, results agree with the
+// reference's x86-64 build to rounding (sqrt and division are correctly
+// rounded on both; pow/exp come from the ROCm device library).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/vhlle_gpu.h"
+
+namespace vh {
+
+// Q component slots (src/inc.h:4-10)
+enum { T_ = 0, X_ = 1, Y_ = 2, Z_ = 3, NB_ = 4, NQ_ = 5, NS_ = 6 };
+
+// packed symmetric 4x4 index (src/cll.cpp:22-31), compile-time foldable
+__host__ __device__ constexpr inline int sym(int i, int j) {
+  return j < i ? (i * (i + 1)) / 2 + j : (j * (j + 1)) / 2 + i;
+}
+
+// slope limiter (src/cll.cpp:12-19)
+__device__ inline double minmod(double a, double b) {
+  if (a * b <= 0.) return 0.;
+  return fabs(a) > fabs(b) ? b : a;
+}
+
+// sign (src/hdo.cpp:34-41), as an int offset
+__device__ inline int isgn(double x) { return x > 0 ? 1 : (x < 0. ? -1 : 0); }
+
+// std::min / std::max semantics (argument order matters for NaN and -0)
+__device__ inline double smin(double a, double b) { return (b < a) ? b : a; }
+__device__ inline double smax(double a, double b) { return (a < b) ? b : a; }
+
+// --------------------------------------------------------------------- EoS
+// Device EoS: the ideal massless gas (SIMPLE branch of EoSs,
+// src/eos.cpp:87-111).  cnst is computed on the host exactly as the
+// reference's compile-time constant.
+struct DevEos {
+  int kind;
+  double cnst;
+  __device__ double p(double e) const { return e / 3.; }
+  __device__ double T(double e) const { return e > 0. ? 1.0 * pow(e / cnst, 0.25) : 0.; }
+  // EoS::s (src/eos.cpp:18-25) with mu = 0
+  __device__ double s(double e, double nb, double nq, double ns) const {
+    const double p_ = p(e), T_ = T(e);
+    const double mub = 0., muq = 0., mus = 0.;
+    if (T_ > 0.0) return (e + p_ - mub * nb - muq * nq - mus * ns) / T_;
+    return 0.;
+  }
+  __device__ double cs2e(double) const { return 1. / 3.; }
+};
+
+// ----------------------------------------------------- con2prim (FP64)
+struct Prim {
+  double e, p, nb, nq, ns, vx, vy, vz;
+};
+
+// transformPV (src/rmn.cpp:28-139), FP64, reference operation order.  The
+// input q is rescaled in place when |q_vec| > q_t (src/rmn.cpp:65-70).
+// Returns the number of loop iterations executed.
+__device__ inline int c2p_ref(const DevEos &eos, double q[7], Prim &r) {
+  const double dpe = 1. / 3.;
+  const double corrf = 0.9999;
+  double M = sqrt(q[X_] * q[X_] + q[Y_] * q[Y_] + q[Z_] * q[Z_]);
+  if (q[T_] <= 0.) {
+    r.e = r.p = 0.;
+    r.vx = r.vy = r.vz = 0.;
+    r.nb = r.nq = r.ns = 0.;
+    return 0;
+  }
+  if (M == 0.) {
+    r.e = q[T_];
+    r.vx = r.vy = r.vz = 0.;
+    r.nb = q[NB_];
+    r.nq = q[NQ_];
+    r.ns = q[NS_];
+    r.p = eos.p(r.e);
+    return 0;
+  }
+  if (M > q[T_]) {
+    q[X_] *= corrf * q[T_] / M;
+    q[Y_] *= corrf * q[T_] / M;
+    q[Z_] *= corrf * q[T_] / M;
+    M = q[T_] * corrf;
+  }
+  double vl = 0., vh = 1.;
+  double v = 0.5 * (vl + vh);
+  double e = q[T_] - M * v;
+  if (e < 0.) e = 0.;
+  double sq = sqrt(1 - v * v);
+  double nb = q[NB_] * sq, nq = q[NQ_] * sq, ns = q[NS_] * sq;
+  double p = eos.p(e);
+  double f = (q[T_] + p) * v - M;
+  double df = (q[T_] + p) - M * v * dpe;
+  double dvold = vh - vl;
+  double dv = dvold;
+  int it = 0;
+  for (; it < 100;) {
+    const bool bis = (f + df * (vh - v)) * (f + df * (vl - v)) >= 0. ||
+                     fabs(2. * f) > fabs(dvold * df);
+    dvold = dv;
+    if (bis) {
+      dv = 0.5 * (vh - vl);
+      v = vl + dv;
+    } else {
+      dv = f / df;
+      v -= dv;
+    }
+    it++;
+    if (fabs(dv) < 0.00001) break;
+    e = q[T_] - M * v;
+    if (e < 0.) e = 0.;
+    sq = sqrt(1 - v * v);
+    nb = q[NB_] * sq;
+    nq = q[NQ_] * sq;
+    ns = q[NS_] * sq;
+    p = eos.p(e);
+    f = (q[T_] + p) * v - M;
+    df = (q[T_] + p) - M * v * dpe;
+    if (f > 0.)
+      vh = v;
+    else
+      vl = v;
+  }
+  r.vx = v * q[X_] / M;
+  r.vy = v * q[Y_] / M;
+  r.vz = v * q[Z_] / M;
+  r.e = q[T_] - M * v;
+  r.p = eos.p(r.e);  // p(e, previous iterate's n): n does not enter p here
+  (void)nb;
+  (void)nq;
+  (void)ns;
+  const double s2 = sqrt(1 - r.vx * r.vx - r.vy * r.vy - r.vz * r.vz);
+  r.nb = q[NB_] * s2;
+  r.nq = q[NQ_] * s2;
+  r.ns = q[NS_] * s2;
+  return it;
+}
+
+// transformCV (src/rmn.cpp:253-263)
+__device__ inline void p2c(const Prim &a, double Q[7]) {
+  const double g2 = 1. / (1 - a.vx * a.vx - a.vy * a.vy - a.vz * a.vz);
+  Q[T_] = (a.e + a.p * (a.vx * a.vx + a.vy * a.vy + a.vz * a.vz)) * g2;
+  Q[X_] = (a.e + a.p) * a.vx * g2;
+  Q[Y_] = (a.e + a.p) * a.vy * g2;
+  Q[Z_] = (a.e + a.p) * a.vz * g2;
+  Q[NB_] = a.nb * sqrt(g2);
+  Q[NQ_] = a.nq * sqrt(g2);
+  Q[NS_] = a.ns * sqrt(g2);
+}
+
+// ------------------------------------------------- transport coefficients
+// TransportCoeff (src/trancoeff.h:6-45, src/trancoeff.cpp:19-100)
+struct DevTrc {
+  vh_trcoeff_t c;
+  __device__ double zetaS(const DevEos &eos, double e, double T, double s, double P) const {
+    switch (c.zetaSparam) {
+      case 0:
+        return c.zetaS0 * (1. / 3. - eos.cs2e(e)) / (exp((0.16 - T) / 0.001) + 1.);
+      case 1: {
+        const double Tp = 0.180;
+        if (T < 0.180)
+          return 0.03 + (0.08 * exp(((T / Tp) - 1.) / (0.0025))) +
+                 (0.22 * exp(((T / Tp) - 1) / (0.022)));
+        else if (T < 0.200)
+          return 27.55 * (T / Tp) - 13.45 - (13.77 * (T / Tp) * (T / Tp));
+        return 0.001 + (0.9 * exp(-((T / Tp) - 1.) / (0.0025))) +
+               (0.25 * exp(-((T / Tp) - 1.) / (0.13)));
+      }
+      case 2: {
+        const double Tpk = 0.165, Tw = 0.010, Bn = 0.24, Bw = 1.5;
+        if (T > Tpk)
+          return Bn * ((Bw * Bw) / ((((T / Tpk) - 1.) * ((T / Tpk) - 1.)) + (Bw * Bw)));
+        return Bn * (exp(-((T - Tpk) / Tw) * ((T - Tpk) / Tw)));
+      }
+      case 3: {
+        const double Tpk2 = 0.160, Bn2 = 0.13, B1 = 0.01, B2 = 0.12;
+        if (T < Tpk2) return Bn2 * exp(-((T - Tpk2) * (T - Tpk2) / (B1 * B1)));
+        return Bn2 * exp(-((T - Tpk2) * (T - Tpk2) / (B2 * B2)));
+      }
+      case 4: {
+        if (fabs(s) < 1e-10 || T < 1e-1) return 0;
+        const double sig = e < c.zetaSPeakEpsilon ? c.zetaSSigmaMinus : c.zetaSSigmaPlus;
+        const double d = c.zetaSScaleBeta * (pow(e, 0.25) - pow(c.zetaSPeakEpsilon, 0.25));
+        return ((e + P) / (s * T)) * c.zetaS0 * exp((d * d) / 2.0 * (sig * sig));
+      }
+      case 5: {
+        if (T <= 0.) return 0.;
+        const double w = T < c.zetaSPeakT ? c.zetaSWidthMinus : c.zetaSWidthPlus;
+        const double d = (T - c.zetaSPeakT) / w;
+        return c.zetaS0 * exp(-0.5 * d * d);
+      }
+      default:
+        return 0.;
+    }
+  }
+  __device__ double etaS(double e, double rho, double T, double muB, double s, double P) const {
+    switch (c.etaSparam) {
+      case 0:
+        return c.etaS0;
+      case 1:
+        return c.etaSMin + ((T > c.T0) ? c.ah * (T - c.T0) : c.al * (T - c.T0));
+      case 2:
+        return smax(0.0, c.etaSMin + ((e > c.etaSEpsilonMin)
+                                          ? ((c.ah * (e - c.etaSEpsilonMin) + c.aRho * rho))
+                                          : c.al * (e - c.etaSEpsilonMin) + c.aRho * rho));
+      case 3: {
+        if (fabs(s) < 1e-10 || T < 1e-1) return 0;
+        const double Tc = c.T0 + c.etaSShiftMuB * (muB / 0.938);
+        return ((e + P) / (s * T)) *
+               smax(0.0, (c.etaSMin + ((T > Tc) ? c.ah * (T - Tc) : c.al * (T - Tc))) *
+                             (1 + c.etaSScaleMuB * (muB / 0.938)));
+      }
+      default:
+        return 0.;
+    }
+  }
+  // getTau (src/trancoeff.cpp:92-100); abs() of the reference resolves to the
+  // double overload in its GSL-enabled build.
+  __device__ void getTau(const DevEos &eos, double e, double rho, double T, double muB,
+                         double s, double P, double &taupi, double &tauPi) const {
+    if (T > 1e-1 && fabs(e + P) > 1.e-10) {
+      taupi = (5. / 5.068) * s * etaS(e, rho, T, muB, s, P) / (e + P);
+      tauPi = (6. / 5.068) * zetaS(eos, e, T, s, P) * s / (e + P);
+    } else {
+      taupi = tauPi = 0.;
+    }
+  }
+};
+
+// --------------------------------------------------------------- geometry
+struct Geom {
+  int nx, ny, nz;
+  long N;
+  double dx, dy, dz;
+  __device__ inline long idx(int ix, int iy, int iz) const {
+    return ix + (long)nx * (iy + (long)ny * iz);
+  }
+  // Fluid::getCell clamp (src/fld.h:70-78)
+  __device__ inline long cidx(int ix, int iy, int iz) const {
+    ix = ix > 0 ? ix : 0;
+    ix = ix < nx ? ix : nx - 1;
+    iy = iy > 0 ? iy : 0;
+    iy = iy < ny ? iy : ny - 1;
+    iz = iz > 0 ? iz : 0;
+    iz = iz < nz ? iz : nz - 1;
+    return idx(ix, iy, iz);
+  }
+};
+
+}  // namespace vh

@@ -1,0 +1,42 @@
+// Launch interface of the hydro-step kernels (vh_kernels.hip).  All launches
+// are asynchronous on the given stream; tp points to the device-resident
+// proper time tau so that one captured step graph can be replayed.
+#pragma once
+#include "vh_device.h"
+
+namespace vh {
+
+void launch_faces(int D, int mode, const Geom &g, const DevEos &eos, const double *Qs,
+                  const double *m, int m_full, const double *tp, double dt, double *F,
+                  unsigned char *flag, int *status, hipStream_t st);
+void launch_update(int mode, const Geom &g, const DevEos &eos, const double *Q,
+                   const double *Qh, const double *const F[3],
+                   const unsigned char *const fl[3], const double *tp, double dt, double *out,
+                   int *status, hipStream_t st);
+void launch_ghost(int full, int axis, const Geom &g, double *Q, double *Qh, double *pi,
+                  double *Pi, hipStream_t st);
+void launch_update_m(int pass, const Geom &g, double *m, double *dm, const double *tp, double dt,
+                     hipStream_t st);
+void launch_vprim(const Geom &g, const DevEos &eos, const double *Qh, const double *tp,
+                  double dt, double *ph, double *pt, hipStream_t st);
+void launch_visc1(const Geom &g, const DevEos &eos, const DevTrc &trc, const double *Qprev,
+                  const double *Qn, const double *ph, const double *pt, const double *m,
+                  int m_full, const double *pi, const double *Pi, const double *tp, double dt,
+                  double *pi0, double *piH0, double *Pi0, double *PiH0, int *status,
+                  hipStream_t st);
+void launch_visc2(const Geom &g, const double *ph, const double *pi0, const double *piH0,
+                  const double *Pi0, const double *PiH0, double *pi, double *piH, double *Pi,
+                  double *PiH, double *vcc, double *vf, const double *tp, double dt,
+                  hipStream_t st);
+void launch_visc3(const Geom &g, const double *ph, const double *piH, const double *PiH,
+                  const double *vf, const double *m, int m_full, const double *tp, double dt,
+                  double *Q, hipStream_t st);
+void launch_advance_tau(double *tp, double dt, hipStream_t st);
+void launch_fill_int(int *p, int v, hipStream_t st);
+void launch_set_prim(const Geom &g, const DevEos &eos, double tau, const double *pr, double *Q,
+                     double *Qprev, double *m, hipStream_t st);
+void launch_c2p_batch(const DevEos &eos, double *Q, long n, double *out, int *iters,
+                      hipStream_t st);
+void launch_p2c_batch(const double *prim, long n, double *Q, hipStream_t st);
+
+}  // namespace vh

@@ -1,0 +1,1059 @@
+// FP64 faithful kernels of the vHLLE hydro step on gfx950.
+//
+// Layout in HBM: structure-of-arrays, component-major, a[k*N + cell] with
+// cell = ix + nx*(iy + ny*iz) (x fastest), so that every kernel's loads of one
+// component by consecutive lanes are consecutive doubles.  Q is double
+// buffered: the corrector reads Q[cur] and writes Q[cur^1], so the previous
+// Q (Cell::Qprev, src/cll.h:46) is just the other buffer and saveQprev
+// (src/hdo.cpp:1028) costs nothing.
+//
+// The reference scatters each interface flux into its two cells
+// (src/hdo.cpp:301-304).  Here every interface is evaluated once by one thread
+// and stored; each cell then gathers its interfaces in the reference's
+// accumulation order (+x-, -x+, +y-, -y+, +z-, -z+, then the source), which
+// reproduces the reference's sums bit for bit.
+#include "vh_kernels.h"
+
+namespace vh {
+
+// ------------------------------------------------------------- helpers
+__device__ inline void decompose(const Geom &g, long c, int &ix, int &iy, int &iz) {
+  ix = (int)(c % g.nx);
+  const long r = c / g.nx;
+  iy = (int)(r % g.ny);
+  iz = (int)(r / g.ny);
+}
+
+template <int D>
+__device__ inline int coord(int ix, int iy, int iz) {
+  return D == 1 ? ix : (D == 2 ? iy : iz);
+}
+template <int D>
+__device__ inline int extent(const Geom &g) {
+  return D == 1 ? g.nx : (D == 2 ? g.ny : g.nz);
+}
+template <int D>
+__device__ inline long stride(const Geom &g) {
+  return D == 1 ? 1L : (D == 2 ? (long)g.nx : (long)g.nx * g.ny);
+}
+
+// MUSCL face state of a cell (Cell::getPrimVar{Left,Right}[H],
+// src/cll.cpp:106-204): half slopes limited by minmod, state (q -+ dq)/tau,
+// then con2prim.  cm/c0/cp: the cell and its clamped neighbours.
+template <int SIDE>
+__device__ inline void face_state(const DevEos &eos, const double *__restrict__ Qs, long N,
+                                  long cm, long c0, long cp, double tau, Prim &r) {
+  double q[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    const double a = Qs[k * N + c0];
+    const double dq = minmod((Qs[k * N + cp] - a) / 2., (a - Qs[k * N + cm]) / 2.);
+    q[k] = SIDE < 0 ? (a - dq) / tau : (a + dq) / tau;
+  }
+  c2p_ref(eos, q, r);
+}
+
+// ------------------------------------------------------- interface flux
+// Hydro::hlle_flux (src/hdo.cpp:77-305) for the interface between cell c and
+// its +1 neighbour along axis D.  Writes F (7 x N, slot c) and flag[c]
+// (1 = flux present, 0 = the reference returned early without a flux).
+template <int D, int MODE>
+__global__ void __launch_bounds__(256)
+k_faces(Geom g, DevEos eos, const double *__restrict__ Qs, const double *__restrict__ m,
+        int m_full, const double *__restrict__ tp, double dt, double *__restrict__ F,
+        unsigned char *__restrict__ flag, int *__restrict__ status) {
+  const double tau = *tp;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  int ix, iy, iz;
+  decompose(g, c, ix, iy, iz);
+  const int i = coord<D>(ix, iy, iz), n = extent<D>(g);
+  if (i >= n - 1) return;
+  const long s = stride<D>(g);
+  const long cl = c, cr = c + s;
+  const long cll = i > 0 ? c - s : c;          // clamped prev of the left cell
+  const long crr = i + 2 < n ? cr + s : cr;    // clamped next of the right cell
+  const long N = g.N;
+  Prim L, R;
+  face_state<+1>(eos, Qs, N, cll, cl, cr, tau, L);
+  face_state<-1>(eos, Qs, N, cl, cr, crr, tau, R);
+  unsigned char ok = 0;
+  const double dta = MODE == 0 ? dt / 2. : dt;
+  const double El = (L.e + L.p) / (1 - L.vx * L.vx - L.vy * L.vy - L.vz * L.vz);
+  const double Er = (R.e + R.p) / (1 - R.vx * R.vx - R.vy * R.vy - R.vz * R.vz);
+  const double tauFactor = MODE == 0 ? tau + 0.25 * dt : tau + 0.5 * dt;
+  if (L.e < 0.) {
+    L.e = 0.;
+    L.p = 0.;
+  }
+  if (R.e < 0.) {
+    R.e = 0.;
+    R.p = 0.;
+  }
+  // (the reference's "pr < 0" re-evaluation, src/hdo.cpp:134-140, cannot
+  // trigger for p = e/3 once e >= 0 and is therefore absent here)
+  do {
+    if (L.e > 1e10) {
+      atomicOr(status, VH_ST_BIG_E);
+      break;
+    }
+    if (L.e == 0. && R.e == 0.) break;
+    if (!m_full && m[(D - 1) * N + cl] < 1. && m[(D - 1) * N + cr] < 1.) break;
+    const double gl = 1. / sqrt(1 - L.vx * L.vx - L.vy * L.vy - L.vz * L.vz);
+    const double gr = 1. / sqrt(1 - R.vx * R.vx - R.vy * R.vy - R.vz * R.vz);
+    double Ul[7], Ur[7], Fl[7], Fr[7];
+    Ul[X_] = gl * gl * (L.e + L.p) * L.vx;
+    Ul[Y_] = gl * gl * (L.e + L.p) * L.vy;
+    Ul[Z_] = gl * gl * (L.e + L.p) * L.vz;
+    Ul[T_] = gl * gl * (L.e + L.p) - L.p;
+    Ul[NB_] = gl * L.nb;
+    Ul[NQ_] = gl * L.nq;
+    Ul[NS_] = gl * L.ns;
+    Ur[X_] = gr * gr * (R.e + R.p) * R.vx;
+    Ur[Y_] = gr * gr * (R.e + R.p) * R.vy;
+    Ur[Z_] = gr * gr * (R.e + R.p) * R.vz;
+    Ur[T_] = gr * gr * (R.e + R.p) - R.p;
+    Ur[NB_] = gr * R.nb;
+    Ur[NQ_] = gr * R.nq;
+    Ur[NS_] = gr * R.ns;
+    const double vl = D == 1 ? L.vx : (D == 2 ? L.vy : L.vz);
+    const double vr = D == 1 ? R.vx : (D == 2 ? R.vy : R.vz);
+    if (D != 3) {
+#pragma unroll
+      for (int k = 0; k < 7; k++) {
+        Fl[k] = Ul[k] * vl;
+        Fr[k] = Ur[k] * vr;
+      }
+      Fl[T_] = Ul[T_] * vl + L.p * vl;
+      Fr[T_] = Ur[T_] * vr + R.p * vr;
+      Fl[D] = Ul[D] * vl + L.p;
+      Fr[D] = Ur[D] * vr + R.p;
+    } else {
+      const double t1 = tauFactor;
+#pragma unroll
+      for (int k = 0; k < 7; k++) {
+        Fl[k] = Ul[k] * vl / t1;
+        Fr[k] = Ur[k] * vr / t1;
+      }
+      Fl[T_] = Ul[T_] * vl / t1 + L.p * vl / t1;
+      Fr[T_] = Ur[T_] * vr / t1 + R.p * vr / t1;
+      Fl[Z_] = Ul[Z_] * vl / t1 + L.p / t1;
+      Fr[Z_] = Ur[Z_] * vr / t1 + R.p / t1;
+    }
+    // wave speeds with cs^2 = 1/3 always (src/eos.h:34-35, src/hdo.cpp:181-188)
+    const double cs2 = 1. / 3.;
+    const double cs = sqrt(1. / 3.);  // EoS::cs() (src/eos.h:35)
+    const double sEl = sqrt(El), sEr = sqrt(Er);
+    const double sum = sEl + sEr;
+    const double dv = vl - vr;
+    const double csb = sqrt(cs2 + 0.5 * sqrt(El * Er) / (sum * sum) * (dv * dv));
+    const double vb = (sEl * vl + sEr * vr) / sum;
+    double bl = smin(0., smin((vb - csb) / (1 - vb * csb), (vl - cs) / (1 - vl * cs)));
+    double br = smax(0., smax((vb + csb) / (1 + vb * csb), (vr + cs) / (1 + vr * cs)));
+    double dx;
+    if (D == 3) {
+      bl = 1. / tau * bl;
+      br = 1. / tau * br;
+      if (L.e == 0.) bl = -1. / tau;
+      if (R.e == 0.) br = 1. / tau;
+      dx = g.dz;
+    } else {
+      if (L.e == 0.) bl = -1.;
+      if (R.e == 0.) br = 1.;
+      dx = D == 1 ? g.dx : g.dy;
+    }
+    if (bl == 0. && br == 0.) break;
+    const double pre = tauFactor * dta / dx;
+    const double den = -bl + br;
+    double f[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++)
+      f[k] = pre * (-bl * br * (Ul[k] - Ur[k]) + br * Fl[k] - bl * Fr[k]) / den;
+    if (f[NB_] != f[NB_]) {
+      atomicOr(status, VH_ST_NAN_FLUX);
+      break;
+    }
+#pragma unroll
+    for (int k = 0; k < 7; k++) F[k * N + c] = f[k];
+    ok = 1;
+  } while (0);
+  flag[c] = ok;
+}
+
+// ------------------------------------------------------ stage update
+// Gathers the six interface fluxes of each cell in the reference's order,
+// adds the Milne source (Hydro::source/source_step, src/hdo.cpp:307-353) and
+// applies the stage update: predictor Qh = Q + flux (Cell::updateQtoQhByFlux,
+// src/cll.cpp:86-88); corrector Q += flux unless Q0 + flux0 < 0
+// (Cell::updateByFlux, src/cll.cpp:70-74).
+template <int MODE>
+__global__ void __launch_bounds__(256)
+k_update(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
+         const double *__restrict__ Fx, const unsigned char *__restrict__ fx,
+         const double *__restrict__ Fy, const unsigned char *__restrict__ fy,
+         const double *__restrict__ Fz, const unsigned char *__restrict__ fz,
+         const double *__restrict__ tp, double dt, double *__restrict__ out,
+         int *__restrict__ status) {
+  const double tau = *tp;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  int ix, iy, iz;
+  decompose(g, c, ix, iy, iz);
+  const long N = g.N;
+  double acc[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) acc[k] = 0.;
+  const long sy = g.nx, sz = (long)g.nx * g.ny;
+  if (ix > 0 && fx[c - 1])
+#pragma unroll
+    for (int k = 0; k < 7; k++) acc[k] += Fx[k * N + c - 1];
+  if (ix < g.nx - 1 && fx[c])
+#pragma unroll
+    for (int k = 0; k < 7; k++) acc[k] += -Fx[k * N + c];
+  if (iy > 0 && fy[c - sy])
+#pragma unroll
+    for (int k = 0; k < 7; k++) acc[k] += Fy[k * N + c - sy];
+  if (iy < g.ny - 1 && fy[c])
+#pragma unroll
+    for (int k = 0; k < 7; k++) acc[k] += -Fy[k * N + c];
+  if (iz > 0 && fz[c - sz])
+#pragma unroll
+    for (int k = 0; k < 7; k++) acc[k] += Fz[k * N + c - sz];
+  if (iz < g.nz - 1 && fz[c])
+#pragma unroll
+    for (int k = 0; k < 7; k++) acc[k] += -Fz[k * N + c];
+  // source on the unreconstructed state (src/hdo.cpp:335-341)
+  const double *Qsrc = MODE == 0 ? Q : Qh;
+  const double tau1 = MODE == 0 ? tau : tau + 0.5 * dt;
+  const double sdt = MODE == 0 ? dt / 2. : dt;
+  double q[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) q[k] = Qsrc[k * N + c] / tau1;
+  Prim r;
+  c2p_ref(eos, q, r);
+  double S[7];
+  S[T_] = -q[T_] * r.vz * r.vz - r.p * (1. + r.vz * r.vz);
+  S[X_] = 0.;
+  S[Y_] = 0.;
+  S[Z_] = -q[Z_];
+  S[NB_] = S[NQ_] = S[NS_] = 0.;
+#pragma unroll
+  for (int k = 0; k < 7; k++) S[k] *= sdt;
+  if (S[NB_] != S[NB_]) atomicOr(status, VH_ST_NAN_SOURCE);
+#pragma unroll
+  for (int k = 0; k < 7; k++) acc[k] += S[k];
+  if (MODE == 0) {
+#pragma unroll
+    for (int k = 0; k < 7; k++) out[k * N + c] = Q[k * N + c] + acc[k];
+  } else {
+    const bool skip = Q[c] + acc[0] < 0.;
+#pragma unroll
+    for (int k = 0; k < 7; k++) out[k * N + c] = skip ? Q[k * N + c] : Q[k * N + c] + acc[k];
+  }
+}
+
+// ------------------------------------------------------------ ghost fill
+// Fluid::correctImagCells / correctImagCellsFull (src/fld.cpp:211-382): per
+// boundary line one thread runs the reference's copy sequence with clamped
+// cell indices (so degenerate small extents behave as in the reference).
+// AXIS: 3 = z pass (thread per (ix,iy)), 2 = y pass, 1 = x pass.
+__device__ inline void set_clamped(double *a, long N, long dst, const double v[7]) {
+  const bool z = v[0] < 0.;
+#pragma unroll
+  for (int k = 0; k < 7; k++) a[k * N + dst] = z ? 0. : v[k];
+}
+
+template <int AXIS, int FULL>
+__global__ void __launch_bounds__(256)
+k_ghost(Geom g, double *__restrict__ Q, double *__restrict__ Qh, double *__restrict__ pi,
+        double *__restrict__ Pi) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int a, b;  // the two coordinates spanning the boundary face
+  long nline;
+  if (AXIS == 3) nline = (long)g.nx * g.ny;
+  else if (AXIS == 2) nline = (long)g.nx * g.nz;
+  else nline = (long)g.ny * g.nz;
+  if (t >= nline) return;
+  const long N = g.N;
+  const int n = AXIS == 3 ? g.nz : (AXIS == 2 ? g.ny : g.nx);
+  if (AXIS == 3) { a = (int)(t % g.nx); b = (int)(t / g.nx); }
+  else if (AXIS == 2) { a = (int)(t % g.nx); b = (int)(t / g.nx); }
+  else { a = (int)(t % g.ny); b = (int)(t / g.ny); }
+  auto cell = [&](int i) -> long {
+    if (AXIS == 3) return g.cidx(a, b, i);
+    if (AXIS == 2) return g.cidx(a, i, b);
+    return g.cidx(i, a, b);
+  };
+  const int src[2] = {2, n - 3}, d1[2] = {1, n - 2}, d0[2] = {0, n - 1};
+  for (int side = 0; side < 2; side++) {
+    const long cs = cell(src[side]), c1 = cell(d1[side]), c0 = cell(d0[side]);
+    double v[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++) v[k] = Q[k * N + cs];
+    set_clamped(Q, N, c1, v);
+    set_clamped(Q, N, c0, v);
+    if (!FULL) {
+#pragma unroll
+      for (int k = 0; k < 7; k++) v[k] = Qh[k * N + cs];
+      set_clamped(Qh, N, c1, v);
+      set_clamped(Qh, N, c0, v);
+    } else {
+      double p[10];
+#pragma unroll
+      for (int k = 0; k < 10; k++) p[k] = pi[k * N + cs];
+      const double P = Pi[cs];
+#pragma unroll
+      for (int k = 0; k < 10; k++) {
+        pi[k * N + c0] = p[k];
+        pi[k * N + c1] = p[k];
+      }
+      Pi[c0] = P;
+      Pi[c1] = P;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- updateM
+// Fluid::updateM (src/fld.cpp:384-423), pass 1: dm from the neighbours' m.
+__global__ void __launch_bounds__(256)
+k_update_m_dm(Geom g, const double *__restrict__ m, double *__restrict__ dm,
+              const double *__restrict__ tp, double dt) {
+  const double tau = *tp;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  int ix, iy, iz;
+  decompose(g, c, ix, iy, iz);
+  const long N = g.N;
+  auto M = [&](int d, long cc) { return m[d * N + cc]; };
+  auto maxM = [&](long cc) { return smax(M(0, cc), smax(M(1, cc), M(2, cc))); };
+  double d0 = 0., d1 = 0., d2 = 0.;
+  if (maxM(c) < 1.) {
+    if (M(0, g.cidx(ix + 1, iy, iz)) >= 1. || M(0, g.cidx(ix - 1, iy, iz)) >= 1.) d0 = dt / g.dx;
+    if (M(1, g.cidx(ix, iy + 1, iz)) >= 1. || M(1, g.cidx(ix, iy - 1, iz)) >= 1.) d1 = dt / g.dy;
+    if (M(2, g.cidx(ix, iy, iz + 1)) >= 1. || M(2, g.cidx(ix, iy, iz - 1)) >= 1.)
+      d2 = dt / g.dz / tau;
+    if (d0 == 0. && d1 == 0.) {
+      if (maxM(g.cidx(ix + 1, iy + 1, iz)) >= 1. || maxM(g.cidx(ix + 1, iy - 1, iz)) >= 1. ||
+          maxM(g.cidx(ix - 1, iy + 1, iz)) >= 1. || maxM(g.cidx(ix - 1, iy - 1, iz)) >= 1.) {
+        d0 = 0.707 * dt / g.dx;
+        d1 = 0.707 * dt / g.dy;
+      }
+    }
+  }
+  dm[c] = d0;
+  dm[N + c] = d1;
+  dm[2 * N + c] = d2;
+}
+
+// pass 2: Cell::addM for x, y, z in turn (src/cll.h:146-150)
+__global__ void __launch_bounds__(256)
+k_update_m_apply(Geom g, double *__restrict__ m, const double *__restrict__ dm) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  const long N = g.N;
+  double mm[3] = {m[c], m[N + c], m[2 * N + c]};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    mm[d] += dm[d * N + c];
+    if (mm[d] > 0.9) mm[0] = mm[1] = mm[2] = 1.;
+  }
+  m[c] = mm[0];
+  m[N + c] = mm[1];
+  m[2 * N + c] = mm[2];
+}
+
+// --------------------------------------------------------- viscous part
+// Primitive states shared by the Israel-Stewart kernels: con2prim of
+// Qh/(tau - dt/2) (the cell-centre state of ISformal, NSquant's own u,
+// visc_flux and visc_source_step) -> ph[8][N]; con2prim of Qh/tau (NSquant's
+// neighbour states, src/hdo.cpp:472-568) -> pt[4][N] = {e, vx, vy, vz}.
+// tau here is tau_{n+1}.
+__global__ void __launch_bounds__(256)
+k_vprim(Geom g, DevEos eos, const double *__restrict__ Qh, const double *__restrict__ tp,
+        double dt, double *__restrict__ ph, double *__restrict__ pt) {
+  const double tau = *tp;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  const long N = g.N;
+  double q[7];
+  Prim r;
+  const double th = tau - dt / 2.;
+#pragma unroll
+  for (int k = 0; k < 7; k++) q[k] = Qh[k * N + c] / th;
+  c2p_ref(eos, q, r);
+  ph[c] = r.e;
+  ph[N + c] = r.p;
+  ph[2 * N + c] = r.nb;
+  ph[3 * N + c] = r.nq;
+  ph[4 * N + c] = r.ns;
+  ph[5 * N + c] = r.vx;
+  ph[6 * N + c] = r.vy;
+  ph[7 * N + c] = r.vz;
+#pragma unroll
+  for (int k = 0; k < 7; k++) q[k] = Qh[k * N + c] / tau;
+  c2p_ref(eos, q, r);
+  pt[c] = r.e;
+  pt[N + c] = r.vx;
+  pt[2 * N + c] = r.vy;
+  pt[3 * N + c] = r.vz;
+}
+
+// ISformal loop #1 (src/hdo.cpp:691-864) with NSquant (src/hdo.cpp:381-652):
+// Navier-Stokes values, relaxation and second-order/cross terms; writes
+// pi0, piH0, Pi0, PiH0.  (The loop's flux contribution (tau-dt)*dQ_n is
+// formed in k_visc2, which reads the same old pi/Pi before replacing them.)
+__global__ void __launch_bounds__(256)
+k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
+        const double *__restrict__ Qn, const double *__restrict__ ph,
+        const double *__restrict__ pt, const double *__restrict__ m, int m_full,
+        const double *__restrict__ pi, const double *__restrict__ Pi,
+        const double *__restrict__ tp, double dt, double *__restrict__ pi0, double *__restrict__ piH0, double *__restrict__ Pi0,
+        double *__restrict__ PiH0, int *__restrict__ status) {
+  const double tau = *tp;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  int ix, iy, iz;
+  decompose(g, c, ix, iy, iz);
+  const long N = g.N;
+  const double e = ph[c], nb = ph[2 * N + c], nq = ph[3 * N + c], ns = ph[4 * N + c];
+  const double vx = ph[5 * N + c], vy = ph[6 * N + c], vz = ph[7 * N + c];
+  if (e <= 0.) {
+#pragma unroll
+    for (int k = 0; k < 10; k++) {
+      piH0[k * N + c] = 0.0;
+      pi0[k * N + c] = 0.0;
+    }
+    PiH0[c] = 0.0;
+    Pi0[c] = 0.0;
+    return;
+  }
+  const double gamma = 1.0 / sqrt(1.0 - vx * vx - vy * vy - vz * vz);
+  const double u[4] = {gamma, gamma * vx, gamma * vy, gamma * vz};
+  const double gm[4] = {1., -1., -1., -1.};
+
+  // ---- NSquant
+  double dmu[4][4], piNS[10], PiNS, du;
+  bool vac = false;
+  if (!m_full) {
+    auto maxM = [&](int a, int b) {
+      const long cc = g.cidx(ix + a, iy + b, iz);
+      return smax(m[cc], smax(m[N + cc], m[2 * N + cc]));
+    };
+    vac = maxM(1, 0) <= 0.9 || maxM(0, 1) <= 0.9 || maxM(-1, 0) <= 0.9 || maxM(0, -1) <= 0.9 ||
+          maxM(1, 1) <= 0.9 || maxM(1, -1) <= 0.9 || maxM(-1, 1) <= 0.9 || maxM(-1, -1) <= 0.9;
+  }
+  if (vac) {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) dmu[i][j] = 0.;
+#pragma unroll
+    for (int k = 0; k < 10; k++) piNS[k] = 0.;
+    PiNS = du = 0.;
+  } else {
+    const double VMIN = 1e-2, UDIFF = 3.0;
+    double q[7];
+    Prim p0, p1;
+#pragma unroll
+    for (int k = 0; k < 7; k++) q[k] = Qprev[k * N + c] / (tau - dt);
+    c2p_ref(eos, q, p0);
+#pragma unroll
+    for (int k = 0; k < 7; k++) q[k] = Qn[k * N + c] / tau;
+    c2p_ref(eos, q, p1);
+    const double e0 = p0.e, e1 = e;  // e1: the H-centre state (src/hdo.cpp:427)
+    double ut0 = 1.0 / sqrt(1.0 - p0.vx * p0.vx - p0.vy * p0.vy - p0.vz * p0.vz);
+    double ux0 = ut0 * p0.vx, uy0 = ut0 * p0.vy, uz0 = ut0 * p0.vz;
+    double ut1 = 1.0 / sqrt(1.0 - p1.vx * p1.vx - p1.vy * p1.vy - p1.vz * p1.vz);
+    double ux1 = ut1 * p1.vx, uy1 = ut1 * p1.vy, uz1 = ut1 * p1.vz;
+    dmu[0][0] = (ut1 * ut1 - ut0 * ut0) / 2. / u[0] / dt;
+    dmu[0][1] = (ux1 * ux1 - ux0 * ux0) / 2. / u[1] / dt;
+    dmu[0][2] = (uy1 * uy1 - uy0 * uy0) / 2. / u[2] / dt;
+    dmu[0][3] = (uz1 * uz1 - uz0 * uz0) / 2. / u[3] / dt;
+    if (fabs(0.5 * (ut1 + ut0) / ut1) > UDIFF) dmu[0][0] = (ut1 - ut0) / dt;
+    if (fabs(u[1]) < VMIN || fabs(0.5 * (ux1 + ux0) / ux1) > UDIFF) dmu[0][1] = (ux1 - ux0) / dt;
+    if (fabs(u[2]) < VMIN || fabs(0.5 * (uy1 + uy0) / uy1) > UDIFF) dmu[0][2] = (uy1 - uy0) / dt;
+    if (fabs(u[3]) < VMIN || fabs(0.5 * (uz1 + uz0) / uz1) > UDIFF) dmu[0][3] = (uz1 - uz0) / dt;
+    if (e1 <= 0. || e0 <= 0.) dmu[0][0] = dmu[0][1] = dmu[0][2] = dmu[0][3] = 0.;
+    // space derivatives from the neighbours' Qh/tau_{n+1} states
+#pragma unroll
+    for (int d = 1; d <= 3; d++) {
+      const long ca = g.cidx(ix + (d == 1), iy + (d == 2), iz + (d == 3));
+      const long cb = g.cidx(ix - (d == 1), iy - (d == 2), iz - (d == 3));
+      const double ea = pt[ca], eb = pt[cb];
+      if (ea > 0. && eb > 0.) {
+        const double avx = pt[N + ca], avy = pt[2 * N + ca], avz = pt[3 * N + ca];
+        const double bvx = pt[N + cb], bvy = pt[2 * N + cb], bvz = pt[3 * N + cb];
+        ut0 = 1.0 / sqrt(1.0 - bvx * bvx - bvy * bvy - bvz * bvz);
+        ux0 = ut0 * bvx;
+        uy0 = ut0 * bvy;
+        uz0 = ut0 * bvz;
+        ut1 = 1.0 / sqrt(1.0 - avx * avx - avy * avy - avz * avz);
+        ux1 = ut1 * avx;
+        uy1 = ut1 * avy;
+        uz1 = ut1 * avz;
+        if (d < 3) {
+          const double h = d == 1 ? g.dx : g.dy;
+          dmu[d][0] = 0.25 * (ut1 * ut1 - ut0 * ut0) / u[0] / h;
+          dmu[d][1] = 0.25 * (ux1 * ux1 - ux0 * ux0) / u[1] / h;
+          dmu[d][2] = 0.25 * (uy1 * uy1 - uy0 * uy0) / u[2] / h;
+          dmu[d][3] = 0.25 * (uz1 * uz1 - uz0 * uz0) / u[3] / h;
+          if (fabs(0.5 * (ut1 + ut0) / u[0]) > UDIFF) dmu[d][0] = 0.5 * (ut1 - ut0) / h;
+          if (fabs(u[1]) < VMIN || fabs(0.5 * (ux1 + ux0) / u[1]) > UDIFF)
+            dmu[d][1] = 0.5 * (ux1 - ux0) / h;
+          if (fabs(u[2]) < VMIN || fabs(0.5 * (uy1 + uy0) / u[2]) > UDIFF)
+            dmu[d][2] = 0.5 * (uy1 - uy0) / h;
+          if (fabs(u[3]) < VMIN || fabs(0.5 * (uz1 + uz0) / u[3]) > UDIFF)
+            dmu[d][3] = 0.5 * (uz1 - uz0) / h;
+        } else {
+          const double dz = g.dz, tz = tau + 0.5 * dt;
+          dmu[3][0] = 0.25 * (ut1 * ut1 - ut0 * ut0) / u[0] / dz / tz;
+          dmu[3][1] = 0.25 * (ux1 * ux1 - ux0 * ux0) / u[1] / dz / tz;
+          dmu[3][2] = 0.25 * (uy1 * uy1 - uy0 * uy0) / u[2] / dz / tz;
+          dmu[3][3] = 0.25 * (uz1 * uz1 - uz0 * uz0) / u[3] / dz / tz;
+          if (fabs(0.5 * (ut1 + ut0) / u[0]) > UDIFF) dmu[3][0] = 0.5 * (ut1 - ut0) / dz / tz;
+          if (fabs(u[1]) < VMIN || fabs(0.5 * (ux1 + ux0) / u[1]) > UDIFF)
+            dmu[3][1] = 0.5 * (ux1 - ux0) / dz / tz;
+          if (fabs(u[2]) < VMIN || fabs(0.5 * (uy1 + uy0) / u[2]) > UDIFF)
+            dmu[3][2] = 0.5 * (uy1 - uy0) / dz / tz;
+          if (fabs(u[3]) < VMIN || fabs(0.5 * (uz1 + uz0) / u[3]) > UDIFF)
+            dmu[3][3] = 0.5 * (uz1 - uz0) / dz / tz;
+        }
+      } else {
+        dmu[d][0] = dmu[d][1] = dmu[d][2] = dmu[d][3] = 0.;
+      }
+    }
+    dmu[3][0] += u[3] / (tau - 0.5 * dt);
+    dmu[3][3] += u[0] / (tau - 0.5 * dt);
+    // transport coefficients at the H-centre state
+    const double s = eos.s(e, nb, nq, ns);
+    const double T = eos.T(e), p = eos.p(e);
+    const double etaS = trc.etaS(e, nb, T, 0., s, p);
+    const double zetaS = trc.zetaS(eos, e, T, s, p);
+    // sigma^{mu nu} through the projector Z (src/hdo.cpp:613-637), evaluated
+    // term by term; the terms where Z vanishes identically add +-0 and are
+    // dropped by the compiler after unrolling (bit-identical sums).
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j <= i; j++) {
+        double a = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+          for (int l = 0; l < 4; l++) {
+            if (!(j == l || i == l || k == l)) continue;
+            double z = 0.0;
+            if (j == l) z += 0.5 * ((i == k ? gm[i] : 0.) - u[i] * u[k]);
+            if (i == l) z += 0.5 * ((j == k ? gm[j] : 0.) - u[j] * u[k]);
+            if (k == l) z -= ((i == j ? gm[i] : 0.) - u[i] * u[j]) / 3.0;
+            a += z * dmu[k][l] * 2.0 * etaS * s / 5.068;
+          }
+        piNS[sym(i, j)] = a;
+      }
+    PiNS = -zetaS * s * (dmu[0][0] + dmu[1][1] + dmu[2][2] + dmu[3][3]) / 5.068;
+    du = dmu[0][0] + dmu[1][1] + dmu[2][2] + dmu[3][3];
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < 10; k++) bad |= (isinf(piNS[k]) || isnan(piNS[k]));
+    if (bad) atomicOr(status, VH_ST_NAN_PI);
+  }
+
+  // ---- relaxation + sources
+  const double s = eos.s(e, nb, nq, ns);
+  const double T = eos.T(e), p = eos.p(e);
+  const double etaS = trc.etaS(e, nb, T, 0., s, p);
+  const double eta = etaS * s;
+  double sig[10];
+#pragma unroll
+  for (int k = 0; k < 10; k++) sig[k] = eta <= 0.0 ? 0.0 : 0.5 * piNS[k] / eta * 5.068;
+  double taupi, tauPi;
+  trc.getTau(eos, e, nb, T, 0., s, p, taupi, tauPi);
+  double dpp = 4. / 3., tpp = 10. / 7., lpP = 6. / 5.;
+  double phi7 = 9. / 70. / eos.p(e);
+  if (isinf(phi7)) phi7 = 0.0;
+  phi7 = phi7 / taupi;
+  if (taupi < 0.5 * dt) dpp = tpp = lpP = phi7 = 0.0;
+  double dPP = 2. / 3., lPp = 8. / 5. * (1. / 3. - eos.cs2e(e));
+  if (tauPi < 0.5 * dt) dPP = lPp = 0.0;
+
+  double P[10], D[10], H[10], F[10];
+  const double Pc = Pi[c];
+#pragma unroll
+  for (int k = 0; k < 10; k++) P[k] = pi[k * N + c];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j <= i; j++) {
+      const int ij = sym(i, j);
+      D[ij] = -u[i] * u[j];
+      if (i == j) D[ij] += gm[i];
+      H[ij] = taupi > 0.5 * dt ? P[ij] - (P[ij] - piNS[ij]) * dt / 2.0 / gamma / taupi : piNS[ij];
+    }
+  double PH = tauPi > 0.5 * dt ? Pc - (Pc - PiNS) * dt / 2.0 / gamma / tauPi : PiNS;
+  double tau1 = tau - dt * 0.75;
+  H[sym(0, 0)] += -2. * vz * P[sym(0, 3)] / tau1 * dt / 2.;
+  H[sym(3, 3)] += -(2. * vz / tau1 * P[sym(0, 3)]) * dt / 2.;
+  H[sym(3, 0)] += -(vz / tau1 * P[sym(0, 0)] + vz / tau1 * P[sym(3, 3)]) * dt / 2.;
+  H[sym(1, 0)] += -vz / tau1 * P[sym(1, 3)] * dt / 2.;
+  H[sym(2, 0)] += -vz / tau1 * P[sym(2, 3)] * dt / 2.;
+  H[sym(3, 1)] += -(vz / tau1 * P[sym(0, 1)]) * dt / 2.;
+  H[sym(3, 2)] += -(vz / tau1 * P[sym(0, 2)]) * dt / 2.;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j <= i; j++) {
+      const int ij = sym(i, j);
+      double t = H[ij];
+      t += (-dpp * P[ij] * du + lpP * Pc * sig[ij]) / gamma * 0.5 * dt;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        t += (phi7 * P[sym(i, k)] * P[sym(j, k)] * gm[k] -
+              tpp * 0.5 * (P[sym(i, k)] * sig[sym(j, k)] * gm[k] + P[sym(j, k)] * sig[sym(i, k)] * gm[k])) /
+             gamma * 0.5 * dt;
+#pragma unroll
+        for (int l = 0; l < 4; l++)
+          t += (-P[sym(i, k)] * u[j] - P[sym(j, k)] * u[i]) * u[l] * dmu[l][k] * gm[k] / gamma * 0.5 * dt -
+               1. / 3. * D[ij] * P[sym(k, l)] * (phi7 * P[sym(k, l)] - tpp * sig[sym(k, l)]) * gm[k] * gm[l] /
+                   gamma * 0.5 * dt;
+      }
+      H[ij] = t;
+    }
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+#pragma unroll
+    for (int l = 0; l < 4; l++)
+      PH += lPp * P[sym(k, l)] * sig[sym(k, l)] * gm[k] * gm[l] / gamma * 0.5 * dt;
+  PH += -dPP * Pc * du / gamma * 0.5 * dt;
+  // full step
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j <= i; j++) {
+      const int ij = sym(i, j);
+      F[ij] = taupi > 0.5 * dt ? P[ij] - (H[ij] - piNS[ij]) * dt / gamma / taupi : piNS[ij];
+    }
+  double PF = tauPi > 0.5 * dt ? Pc - (PH - PiNS) * dt / gamma / tauPi : PiNS;
+  tau1 = tau - dt * 0.5;
+  F[sym(0, 0)] += -2. * vz / tau1 * H[sym(0, 3)] * dt;
+  F[sym(3, 3)] += -(2. * vz / tau1 * H[sym(0, 3)]) * dt;
+  F[sym(3, 0)] += -(vz / tau1 * H[sym(0, 0)] + vz / tau1 * H[sym(3, 3)]) * dt;
+  F[sym(1, 0)] += -vz / tau1 * H[sym(1, 3)] * dt;
+  F[sym(2, 0)] += -vz / tau1 * H[sym(2, 3)] * dt;
+  F[sym(3, 1)] += -(vz / tau1 * H[sym(0, 1)]) * dt;
+  F[sym(3, 2)] += -(vz / tau1 * H[sym(0, 2)]) * dt;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j <= i; j++) {
+      const int ij = sym(i, j);
+      double t = F[ij];
+      t += (-dpp * H[ij] * du + lpP * PH * sig[ij]) / gamma * dt;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        t += (phi7 * P[sym(i, k)] * H[sym(j, k)] * gm[k] -
+              tpp * 0.5 * (H[sym(i, k)] * sig[sym(j, k)] * gm[k] + H[sym(j, k)] * sig[sym(i, k)] * gm[k])) /
+             gamma * dt;
+#pragma unroll
+        for (int l = 0; l < 4; l++)
+          t += ((-H[sym(i, k)] * u[j] - H[sym(j, k)] * u[i]) * u[l] * dmu[l][k] * gm[k] -
+                1. / 3. * D[ij] * H[sym(k, l)] * (phi7 * H[sym(k, l)] - tpp * sig[sym(k, l)]) * gm[k] * gm[l]) /
+               gamma * dt;
+      }
+      F[ij] = t;
+    }
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+#pragma unroll
+    for (int l = 0; l < 4; l++)
+      PF += lPp * H[sym(k, l)] * sig[sym(k, l)] * gm[k] * gm[l] / gamma * dt;
+  PF += -dPP * PH * du / gamma * dt;
+#pragma unroll
+  for (int k = 0; k < 10; k++) {
+    piH0[k * N + c] = H[k];
+    pi0[k * N + c] = F[k];
+  }
+  PiH0[c] = PH;
+  Pi0[c] = PF;
+}
+
+// ISformal loop #2 (src/hdo.cpp:866-962) plus loop #1's flux term
+// (src/hdo.cpp:717-721): upwind advection of pi0/piH0/Pi0/PiH0 from the 8
+// cells toward -v, limiter, new pi/piH/Pi/PiH/viscCorrCut, and the cell's
+// viscous flux accumulator vf = (tau-dt)*dQ_n - tau*dQ_{n+1} (4 x N).
+__global__ void __launch_bounds__(256)
+k_visc2(Geom g, const double *__restrict__ ph, const double *__restrict__ pi0,
+        const double *__restrict__ piH0, const double *__restrict__ Pi0,
+        const double *__restrict__ PiH0, double *__restrict__ pi, double *__restrict__ piH,
+        double *__restrict__ Pi, double *__restrict__ PiH, double *__restrict__ vcc,
+        double *__restrict__ vf, const double *__restrict__ tp, double dt) {
+  const double tau = *tp;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  int ix, iy, iz;
+  decompose(g, c, ix, iy, iz);
+  const long N = g.N;
+  const double e = ph[c], p = ph[N + c];
+  const double vx = ph[5 * N + c], vy = ph[6 * N + c], vz = ph[7 * N + c];
+  if (e <= 0.) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) vf[k * N + c] = 0.;
+    return;
+  }
+  const double gamma = 1.0 / sqrt(1.0 - vx * vx - vy * vy - vz * vz);
+  const double u[4] = {gamma, gamma * vx, gamma * vy, gamma * vz};
+  // loop #1 flux with the old pi, Pi
+  double fl[4];
+  {
+    const double Po = Pi[c];
+#pragma unroll
+    for (int i = 0; i < 4; i++) fl[i] = (tau - dt) * (pi[sym(0, i) * N + c] + Po * u[0] * u[i]);
+    fl[0] += -(tau - dt) * Po;
+  }
+  const double xm = -vx * dt / g.dx;
+  const double ym = -vy * dt / g.dy;
+  const double zm = -vz * dt / g.dz / (tau - 0.5 * dt);
+  const double xmH = -vx * dt / g.dx / 2.0;
+  const double ymH = -vy * dt / g.dy / 2.0;
+  const double zmH = -vz * dt / g.dz / (tau - 0.5 * dt) / 2.0;
+  const double wx[2] = {(1. - fabs(xm)), fabs(xm)};
+  const double wy[2] = {(1. - fabs(ym)), fabs(ym)};
+  const double wz[2] = {(1. - fabs(zm)), fabs(zm)};
+  const double wxH[2] = {(1. - fabs(xmH)), fabs(xmH)};
+  const double wyH[2] = {(1. - fabs(ymH)), fabs(ymH)};
+  const double wzH[2] = {(1. - fabs(zmH)), fabs(zmH)};
+  const int sx = isgn(xm), sy = isgn(ym), sz = isgn(zm);
+  double a[10], aH[10], A = 0.0, AH = 0.0;
+#pragma unroll
+  for (int k = 0; k < 10; k++) a[k] = aH[k] = 0.0;
+#pragma unroll
+  for (int jx = 0; jx < 2; jx++)
+#pragma unroll
+    for (int jy = 0; jy < 2; jy++)
+#pragma unroll
+      for (int jz = 0; jz < 2; jz++) {
+        const long c1 = g.cidx(ix + jx * sx, iy + jy * sy, iz + jz * sz);
+        const double w = wx[jx] * wy[jy] * wz[jz];
+        const double wH = wxH[jx] * wyH[jy] * wzH[jz];
+#pragma unroll
+        for (int k = 0; k < 10; k++) {
+          a[k] += w * pi0[k * N + c1];
+          aH[k] += wH * piH0[k * N + c1];
+        }
+        A += w * Pi0[c1];
+        AH += wH * PiH0[c1];
+      }
+  const double den = 1. - vx * vx - vy * vy - vz * vz;
+  const double maxT0 = smax((e + p) / den - p, (e + p) * (vx * vx + vy * vy + vz * vz) / den + p);
+  // max |pi^{mu nu}| over all 16 entries = over the 10 packed ones
+  double maxpi = 0.;
+#pragma unroll
+  for (int k = 0; k < 10; k++)
+    if (fabs(a[k]) > maxpi) maxpi = fabs(a[k]);
+  bool rescaled = false;
+  if (maxT0 / maxpi < 1.0) {
+#pragma unroll
+    for (int k = 0; k < 10; k++) {
+      a[k] = 0.1 * a[k] * maxT0 / maxpi;
+      aH[k] = 0.1 * aH[k] * maxT0 / maxpi;
+    }
+    rescaled = true;
+  }
+  if (fabs(A) > p) {
+    if (A != 0.) A = A / fabs(A) * p;
+    if (AH != 0.) AH = AH / fabs(AH) * p;
+    rescaled = true;
+  }
+  vcc[c] = rescaled ? maxT0 / maxpi : 1.;
+#pragma unroll
+  for (int k = 0; k < 10; k++) {
+    pi[k * N + c] = a[k];
+    piH[k * N + c] = aH[k];
+  }
+  Pi[c] = A;
+  PiH[c] = AH;
+  // loop #2 flux with the new pi, Pi (src/hdo.cpp:950-961)
+  double f2[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) f2[i] = -tau * (a[sym(0, i)] + A * u[0] * u[i]);
+  f2[0] += tau * A;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    double t = 0.;
+    t += fl[i];
+    t += f2[i];
+    vf[i * N + c] = t;
+  }
+}
+
+// viscous interface flux (Hydro::visc_flux, src/hdo.cpp:966-1014) between
+// cells l (left) and r (right) along axis D; returns false if skipped.
+template <int D>
+__device__ inline bool visc_face(const Geom &g, long l, long r, const double *__restrict__ ph,
+                                 const double *__restrict__ piH, const double *__restrict__ PiH,
+                                 const double *__restrict__ m, int m_full, double tau, double dt,
+                                 double F[4]) {
+  const long N = g.N;
+  if (!m_full && m[(D - 1) * N + l] < 1. && m[(D - 1) * N + r] < 1.) return false;
+  const double dxa = D == 1 ? g.dx : (D == 2 ? g.dy : g.dz * (tau + 0.5 * dt));
+  double vx = 0.5 * (ph[5 * N + l] + ph[5 * N + r]);
+  double vy = 0.5 * (ph[6 * N + l] + ph[6 * N + r]);
+  double vz = 0.5 * (ph[7 * N + l] + ph[7 * N + r]);
+  const double v = sqrt(vx * vx + vy * vy + vz * vz);
+  if (v > 1.) {
+    vx = 0.99 * vx / v;
+    vy = 0.99 * vy / v;
+    vz = 0.99 * vz / v;
+  }
+  const double gamma = 1. / sqrt(1. - v * v);
+  const double u[4] = {gamma, gamma * vx, gamma * vy, gamma * vz};
+  const double gm[4] = {1., -1., -1., -1.};
+  const double sP = PiH[l] + PiH[r];
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++) {
+    double f = 0.5 * (piH[sym(mu, D) * N + l] + piH[sym(mu, D) * N + r]);
+    if (mu == D) f += -0.5 * sP * gm[mu];
+    f += 0.5 * sP * u[mu] * u[D];
+    F[mu] = f * (tau - 0.5 * dt) * dt / dxa;
+  }
+  return true;
+}
+
+// viscous interface fluxes gathered per cell in the reference order, the
+// viscous source (Hydro::visc_source_step, src/hdo.cpp:355-375) and the
+// limited update (Cell::updateByViscFlux, src/cll.cpp:76-84).  Q is updated
+// in place (each thread touches only its own cell).
+__global__ void __launch_bounds__(256)
+k_visc3(Geom g, const double *__restrict__ ph, const double *__restrict__ piH,
+        const double *__restrict__ PiH, const double *__restrict__ vf,
+        const double *__restrict__ m, int m_full, const double *__restrict__ tp, double dt,
+        double *__restrict__ Q) {
+  const double tau = *tp;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  int ix, iy, iz;
+  decompose(g, c, ix, iy, iz);
+  const long N = g.N;
+  double acc[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) acc[k] = vf[k * N + c];
+  double F[4];
+  const long sy = g.nx, sz = (long)g.nx * g.ny;
+  if (ix > 0 && visc_face<1>(g, c - 1, c, ph, piH, PiH, m, m_full, tau, dt, F))
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] += F[k];
+  if (ix < g.nx - 1 && visc_face<1>(g, c, c + 1, ph, piH, PiH, m, m_full, tau, dt, F))
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] += -F[k];
+  if (iy > 0 && visc_face<2>(g, c - sy, c, ph, piH, PiH, m, m_full, tau, dt, F))
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] += F[k];
+  if (iy < g.ny - 1 && visc_face<2>(g, c, c + sy, ph, piH, PiH, m, m_full, tau, dt, F))
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] += -F[k];
+  if (iz > 0 && visc_face<3>(g, c - sz, c, ph, piH, PiH, m, m_full, tau, dt, F))
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] += F[k];
+  if (iz < g.nz - 1 && visc_face<3>(g, c, c + sz, ph, piH, PiH, m, m_full, tau, dt, F))
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] += -F[k];
+  const double e = ph[c];
+  if (e > 0.) {
+    const double vx = ph[5 * N + c], vy = ph[6 * N + c], vz = ph[7 * N + c];
+    double u[4];
+    u[0] = 1. / sqrt(1. - vx * vx - vy * vy - vz * vz);
+    u[3] = u[0] * vz;
+    double k0 = -piH[sym(3, 3) * N + c] + PiH[c] * (-1.0 - u[3] * u[3]);
+    double k3 = -(piH[sym(0, 3) * N + c] + PiH[c] * u[0] * u[3]);
+    acc[0] += k0 * dt;
+    acc[1] += 0. * dt;
+    acc[2] += 0. * dt;
+    acc[3] += k3 * dt;
+  }
+  const double q0 = Q[c];
+  if (fabs(acc[0]) <= 0.5 * q0) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) Q[k * N + c] += acc[k];
+#pragma unroll
+    for (int k = 4; k < 7; k++) Q[k * N + c] += 0.;
+  } else if (acc[0] != 0.) {
+    const double fac = fabs(0.5 * q0 / acc[0]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) Q[k * N + c] += fac * acc[k];
+#pragma unroll
+    for (int k = 4; k < 7; k++) Q[k * N + c] += fac * 0.;
+  }
+}
+
+// ------------------------------------------------------- IC and batches
+// Cell::setPrimVar + saveQprev + setAllM (src/cll.cpp:222-241,
+// src/icGubser.cpp:47-57) on all cells.  prim: 7 arrays e,nb,nq,ns,vx,vy,vz.
+__global__ void __launch_bounds__(256)
+k_set_prim(Geom g, DevEos eos, double tau, const double *__restrict__ pr,
+           double *__restrict__ Q, double *__restrict__ Qprev, double *__restrict__ m) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  const long N = g.N;
+  const double e = pr[c], nb = pr[N + c], nq = pr[2 * N + c], ns = pr[3 * N + c];
+  const double vx = pr[4 * N + c], vy = pr[5 * N + c], vz = pr[6 * N + c];
+  const double g2 = 1. / (1 - vx * vx - vy * vy - vz * vz);
+  const double p = eos.p(e);
+  double q[7];
+  q[T_] = tau * (e + p * (vx * vx + vy * vy + vz * vz)) * g2;
+  q[X_] = tau * (e + p) * vx * g2;
+  q[Y_] = tau * (e + p) * vy * g2;
+  q[Z_] = tau * (e + p) * vz * g2;
+  q[NB_] = tau * nb * sqrt(g2);
+  q[NQ_] = tau * nq * sqrt(g2);
+  q[NS_] = tau * ns * sqrt(g2);
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    Q[k * N + c] = q[k];
+    Qprev[k * N + c] = q[k];
+  }
+  if (e > 0.) m[c] = m[N + c] = m[2 * N + c] = 1.;
+}
+
+__global__ void __launch_bounds__(256)
+k_c2p_batch(DevEos eos, double *__restrict__ Q, long n, double *__restrict__ out,
+            int *__restrict__ iters) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double q[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) q[k] = Q[7 * i + k];
+  Prim r;
+  const int it = c2p_ref(eos, q, r);
+#pragma unroll
+  for (int k = 0; k < 7; k++) Q[7 * i + k] = q[k];
+  double *o = out + 8 * i;
+  o[0] = r.e;
+  o[1] = r.p;
+  o[2] = r.nb;
+  o[3] = r.nq;
+  o[4] = r.ns;
+  o[5] = r.vx;
+  o[6] = r.vy;
+  o[7] = r.vz;
+  if (iters) iters[i] = it;
+}
+
+__global__ void __launch_bounds__(256)
+k_p2c_batch(const double *__restrict__ prim, long n, double *__restrict__ Q) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *a = prim + 8 * i;
+  Prim r{a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7]};
+  double q[7];
+  p2c(r, q);
+#pragma unroll
+  for (int k = 0; k < 7; k++) Q[7 * i + k] = q[k];
+}
+
+// tau_{n+1} = tau_n + dt on the device (Hydro::performStep, src/hdo.cpp:1095)
+__global__ void k_advance_tau(double *__restrict__ tp, double dt) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) tp[0] = tp[0] + dt;
+}
+
+__global__ void k_fill_int(int *__restrict__ p, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = v;
+}
+
+// ----------------------------------------------------------- launchers
+static inline unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
+
+void launch_faces(int D, int mode, const Geom &g, const DevEos &eos, const double *Qs,
+                  const double *m, int m_full, const double *tp, double dt, double *F,
+                  unsigned char *flag, int *status, hipStream_t st) {
+  const dim3 b(256), gr(blocks(g.N));
+#define VH_F(DD, MM)                                                                      \
+  if (D == DD && mode == MM)                                                              \
+    hipLaunchKernelGGL((k_faces<DD, MM>), gr, b, 0, st, g, eos, Qs, m, m_full, tp, dt, F, \
+                       flag, status);
+  VH_F(1, 0) VH_F(1, 1) VH_F(2, 0) VH_F(2, 1) VH_F(3, 0) VH_F(3, 1)
+#undef VH_F
+}
+
+void launch_update(int mode, const Geom &g, const DevEos &eos, const double *Q,
+                   const double *Qh, const double *const F[3],
+                   const unsigned char *const fl[3], const double *tp, double dt, double *out,
+                   int *status, hipStream_t st) {
+  if (mode == 0)
+    hipLaunchKernelGGL(k_update<0>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, Q, Qh, F[0],
+                       fl[0], F[1], fl[1], F[2], fl[2], tp, dt, out, status);
+  else
+    hipLaunchKernelGGL(k_update<1>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, Q, Qh, F[0],
+                       fl[0], F[1], fl[1], F[2], fl[2], tp, dt, out, status);
+}
+
+void launch_ghost(int full, int axis, const Geom &g, double *Q, double *Qh, double *pi,
+                  double *Pi, hipStream_t st) {
+  const long n = axis == 3 ? (long)g.nx * g.ny : (axis == 2 ? (long)g.nx * g.nz : (long)g.ny * g.nz);
+  const dim3 gr(blocks(n)), b(256);
+#define VH_G(AA, FF)                                                                          \
+  if (axis == AA && full == FF)                                                               \
+    hipLaunchKernelGGL((k_ghost<AA, FF>), gr, b, 0, st, g, Q, Qh, pi, Pi);
+  VH_G(3, 0) VH_G(2, 0) VH_G(1, 0) VH_G(3, 1) VH_G(2, 1) VH_G(1, 1)
+#undef VH_G
+}
+
+void launch_update_m(int pass, const Geom &g, double *m, double *dm, const double *tp, double dt,
+                     hipStream_t st) {
+  if (pass == 0)
+    hipLaunchKernelGGL(k_update_m_dm, dim3(blocks(g.N)), dim3(256), 0, st, g, m, dm, tp, dt);
+  else
+    hipLaunchKernelGGL(k_update_m_apply, dim3(blocks(g.N)), dim3(256), 0, st, g, m, dm);
+}
+
+void launch_vprim(const Geom &g, const DevEos &eos, const double *Qh, const double *tp,
+                  double dt, double *ph, double *pt, hipStream_t st) {
+  hipLaunchKernelGGL(k_vprim, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, Qh, tp, dt, ph, pt);
+}
+
+void launch_visc1(const Geom &g, const DevEos &eos, const DevTrc &trc, const double *Qprev,
+                  const double *Qn, const double *ph, const double *pt, const double *m,
+                  int m_full, const double *pi, const double *Pi, const double *tp, double dt,
+                  double *pi0, double *piH0, double *Pi0, double *PiH0, int *status,
+                  hipStream_t st) {
+  hipLaunchKernelGGL(k_visc1, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, trc, Qprev, Qn, ph,
+                     pt, m, m_full, pi, Pi, tp, dt, pi0, piH0, Pi0, PiH0, status);
+}
+
+void launch_visc2(const Geom &g, const double *ph, const double *pi0, const double *piH0,
+                  const double *Pi0, const double *PiH0, double *pi, double *piH, double *Pi,
+                  double *PiH, double *vcc, double *vf, const double *tp, double dt,
+                  hipStream_t st) {
+  hipLaunchKernelGGL(k_visc2, dim3(blocks(g.N)), dim3(256), 0, st, g, ph, pi0, piH0, Pi0, PiH0,
+                     pi, piH, Pi, PiH, vcc, vf, tp, dt);
+}
+
+void launch_visc3(const Geom &g, const double *ph, const double *piH, const double *PiH,
+                  const double *vf, const double *m, int m_full, const double *tp, double dt,
+                  double *Q, hipStream_t st) {
+  hipLaunchKernelGGL(k_visc3, dim3(blocks(g.N)), dim3(256), 0, st, g, ph, piH, PiH, vf, m,
+                     m_full, tp, dt, Q);
+}
+
+void launch_advance_tau(double *tp, double dt, hipStream_t st) {
+  hipLaunchKernelGGL(k_advance_tau, dim3(1), dim3(64), 0, st, tp, dt);
+}
+
+void launch_fill_int(int *p, int v, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, p, v);
+}
+
+void launch_set_prim(const Geom &g, const DevEos &eos, double tau, const double *pr, double *Q,
+                     double *Qprev, double *m, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_prim, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, tau, pr, Q, Qprev,
+                     m);
+}
+
+void launch_c2p_batch(const DevEos &eos, double *Q, long n, double *out, int *iters,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_c2p_batch, dim3(blocks(n)), dim3(256), 0, st, eos, Q, n, out, iters);
+}
+
+void launch_p2c_batch(const double *prim, long n, double *Q, hipStream_t st) {
+  hipLaunchKernelGGL(k_p2c_batch, dim3(blocks(n)), dim3(256), 0, st, prim, n, Q);
+}
+
+}  // namespace vh
