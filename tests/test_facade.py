@@ -1,0 +1,96 @@
+"""The C++ facade (Fluid/Hydro/Cell/EoS/TransportCoeff/ICGubser) and the
+params-file driver built on it (vhlle_amd/bin/vhlle)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from vhlle_amd import setups
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "vhlle_amd", "bin", "vhlle")
+
+
+def _params(tmp_path, **kw):
+    lines = [f"{k} {v}" for k, v in kw.items()]
+    lines.insert(0, "! test params")
+    p = tmp_path / "params.txt"
+    p.write_text("\n".join(lines) + "\nunknownKey 3\n")
+    return str(p)
+
+
+def test_params_reader_echo_cpu(tmp_path):
+    """Unknown keys echo 'UUU', comments 'CCC' (src/main.cpp:139-152); a
+    tabulated eosType is refused before any device work."""
+    p = _params(tmp_path, eosType=1, nx=11)
+    r = subprocess.run([EXE, "-params", p, "-outputDir", str(tmp_path)], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 0
+    assert "UUU unknownKey 3" in r.stdout
+    assert "CCC ! test params" in r.stdout
+    assert "eosType 1" in r.stdout
+
+
+def test_no_cli_params_exits_1_cpu():
+    r = subprocess.run([EXE], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "no CL params" in r.stdout
+
+
+@pytest.mark.gpu
+def test_driver_gubser_matches_oracle(tmp_path, oracle):
+    """Gubser run through params file -> ICGubser -> Hydro::performStep on the
+    GPU, final Q compared with the oracle on the same IC."""
+    n = 41
+    p = _params(tmp_path, eosType=5, icModel=4, gubserScale=10.0, nx=n, ny=n, nz=1,
+                xmin=-5.0, xmax=5.0, ymin=-5.0, ymax=5.0, etamin=-1.0, etamax=1.0,
+                tau0=1.0, tauMax=1.5, dtau=0.05, etaS=0.0, zetaS=0.0, tauGridResize=100.0,
+                dumpState=1)
+    r = subprocess.run([EXE, "-params", p, "-outputDir", str(tmp_path)], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-800:])
+    assert r.returncode == 0, r.stderr
+    steps = int(r.stdout.split("steps = ")[1].split()[0])
+    Qg = np.fromfile(tmp_path / "state.bin").reshape(7, n * n)
+    X, Y, _ = setups.axes(n, n, 1, (-5, 5), (-5, 5), (-1, 1))
+    e, vx, vy = setups.gubser(1.0, X, Y, amp=10.0)
+    o = oracle.Solver(n, n, 1, (-5, 5), (-5, 5), (-1, 1))
+    o.set_prim(1.0, e, vx, vy, np.zeros_like(e))
+    rc, tau = o.run(1.0, 0.05, steps)
+    assert rc == 0
+    Qo = o.get_state()["Q"]
+    assert np.max(np.abs(Qg - Qo)) <= 1e-10 * np.max(np.abs(Qo))
+
+
+@pytest.mark.gpu
+def test_driver_viscous_gaussian_substeps(tmp_path, oracle):
+    """3-D viscous run with the driver's eta sub-stepping
+    (dtau > tau*deta -> dtau/2, src/main.cpp:405-419) against the oracle."""
+    nx, ny, nz = 12, 10, 8
+    xl, yl, zl = (-2.75, 2.75), (-2.25, 2.25), (-0.6, 0.6)
+    p = _params(tmp_path, eosType=5, icModel=10, icE0=20.0, icSigmaX=1.2, icSigmaY=1.0,
+                icEtaPlateau=0.3, icEtaSigma=0.4, nx=nx, ny=ny, nz=nz, xmin=xl[0], xmax=xl[1],
+                ymin=yl[0], ymax=yl[1], etamin=zl[0], etamax=zl[1], tau0=0.5, tauMax=0.7,
+                dtau=0.1, etaS=0.1, tauGridResize=100.0, dumpState=1)
+    r = subprocess.run([EXE, "-params", p, "-outputDir", str(tmp_path)], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-800:])
+    assert r.returncode == 0, r.stderr
+    assert "timestep reduced by" in r.stdout
+    Qg = np.fromfile(tmp_path / "state.bin").reshape(7, nx * ny * nz)
+    e, vx, vy, vz = setups.gaussian3d_ic(nx, ny, nz, xl, yl, zl, e0=20.0, sx=1.2, sy=1.0,
+                                         eta_plateau=0.3, eta_sigma=0.4)
+    o = oracle.Solver(nx, ny, nz, xl, yl, zl, trc=oracle.make_trc(etaS=0.1))
+    o.set_prim(0.5, e, vx, vy, vz)
+    # replay the driver's schedule on the oracle
+    tau, dtau = 0.5, 0.1
+    while True:
+        nsub = 1
+        while dtau / nsub > 1.0 * tau * (zl[1] - zl[0]) / (nz - 1):
+            nsub *= 2
+        rc, tau = o.run(tau, dtau / nsub, nsub)
+        assert rc == 0
+        if not tau < 0.7 + 0.0001:
+            break
+    Qo = o.get_state()["Q"]
+    assert np.max(np.abs(Qg - Qo)) <= 1e-10 * np.max(np.abs(Qo))

@@ -1,7 +1,8 @@
 """Builds the native libraries of vhlle_amd in-tree.
 
-libvhlle_gpu.so : HIP kernels + C-ABI (include/vhlle_gpu.h) + C++ facade,
-                  compiled for gfx950 by hipcc.
+libvhlle_gpu.so : HIP kernels + C-ABI (include/vhlle_gpu.h) + C++ facade
+                  (host/vhlle.h), kernels compiled for gfx950 by hipcc.
+bin/vhlle       : the params-file driver (driver/main.cpp).
 The parity-critical FP64 kernels are compiled with -ffp-contract=off (no FMA
 contraction), matching the reference's generic x86-64 build.
 """
@@ -54,9 +55,9 @@ def build_lib(verbose=False):
         obj = os.path.join(objdir, os.path.basename(src) + ".o")
         if src.endswith(".hip"):
             cmd = [HIPCC] + HIP_FLAGS + ["-I", ROOT, "-c", src, "-o", obj]
-        else:
-            cmd = [HIPCC, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-I", ROOT,
-                   "-c", src, "-o", obj]
+        else:  # host-only C++ (the facade): no HIP headers needed
+            cmd = ["g++", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
+                   "-I", ROOT, "-c", src, "-o", obj]
         out = _run(cmd)
         if verbose and out.strip():
             print(out)
@@ -67,13 +68,27 @@ def build_lib(verbose=False):
     return LIB
 
 
+def build_driver():
+    """The params-file driver executable, linked against libvhlle_gpu.so."""
+    bindir = os.path.join(HERE, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    exe = os.path.join(bindir, "vhlle")
+    _run(["g++", "-O3", "-std=c++17", "-Wall", "-I", ROOT,
+          os.path.join(HERE, "driver", "main.cpp"), "-L", HERE, "-lvhlle_gpu",
+          "-Wl,-rpath,$ORIGIN/..", "-o", exe + ".tmp"])
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
 def build_oracle():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
 
 
 def build_all(verbose=False):
     build_oracle()
-    return build_lib(verbose)
+    lib = build_lib(verbose)
+    build_driver()
+    return lib
 
 
 if __name__ == "__main__":

@@ -57,88 +57,89 @@ struct Prim {
 
 // transformPV (src/rmn.cpp:28-139), FP64, reference operation order.  The
 // input q is rescaled in place when |q_vec| > q_t (src/rmn.cpp:65-70).
-// Returns the number of loop iterations executed.
-__device__ inline int c2p_ref(const DevEos &eos, double q[7], Prim &r) {
+// Returns the number of loop iterations executed.  Written on scalar locals
+// with one exit so that nothing is spilled to scratch memory.
+__device__ __forceinline__ int c2p_ref(const DevEos &eos, double q[7], Prim &r) {
   const double dpe = 1. / 3.;
   const double corrf = 0.9999;
-  double M = sqrt(q[X_] * q[X_] + q[Y_] * q[Y_] + q[Z_] * q[Z_]);
-  if (q[T_] <= 0.) {
-    r.e = r.p = 0.;
-    r.vx = r.vy = r.vz = 0.;
-    r.nb = r.nq = r.ns = 0.;
-    return 0;
-  }
-  if (M == 0.) {
-    r.e = q[T_];
-    r.vx = r.vy = r.vz = 0.;
-    r.nb = q[NB_];
-    r.nq = q[NQ_];
-    r.ns = q[NS_];
-    r.p = eos.p(r.e);
-    return 0;
-  }
-  if (M > q[T_]) {
-    q[X_] *= corrf * q[T_] / M;
-    q[Y_] *= corrf * q[T_] / M;
-    q[Z_] *= corrf * q[T_] / M;
-    M = q[T_] * corrf;
-  }
-  double vl = 0., vh = 1.;
-  double v = 0.5 * (vl + vh);
-  double e = q[T_] - M * v;
-  if (e < 0.) e = 0.;
-  double sq = sqrt(1 - v * v);
-  double nb = q[NB_] * sq, nq = q[NQ_] * sq, ns = q[NS_] * sq;
-  double p = eos.p(e);
-  double f = (q[T_] + p) * v - M;
-  double df = (q[T_] + p) - M * v * dpe;
-  double dvold = vh - vl;
-  double dv = dvold;
+  const double Qt = q[T_], Qb = q[NB_], Qq = q[NQ_], Qs = q[NS_];
+  double Qx = q[X_], Qy = q[Y_], Qz = q[Z_];
+  double M = sqrt(Qx * Qx + Qy * Qy + Qz * Qz);
+  double e = 0., p = 0., vx = 0., vy = 0., vz = 0., nb = 0., nq = 0., ns = 0.;
   int it = 0;
-  for (; it < 100;) {
-    const bool bis = (f + df * (vh - v)) * (f + df * (vl - v)) >= 0. ||
-                     fabs(2. * f) > fabs(dvold * df);
-    dvold = dv;
-    if (bis) {
-      dv = 0.5 * (vh - vl);
-      v = vl + dv;
-    } else {
-      dv = f / df;
-      v -= dv;
-    }
-    it++;
-    if (fabs(dv) < 0.00001) break;
-    e = q[T_] - M * v;
-    if (e < 0.) e = 0.;
-    sq = sqrt(1 - v * v);
-    nb = q[NB_] * sq;
-    nq = q[NQ_] * sq;
-    ns = q[NS_] * sq;
+  if (Qt <= 0.) {
+    // vacuum: everything zero
+  } else if (M == 0.) {
+    e = Qt;
+    nb = Qb;
+    nq = Qq;
+    ns = Qs;
     p = eos.p(e);
-    f = (q[T_] + p) * v - M;
-    df = (q[T_] + p) - M * v * dpe;
-    if (f > 0.)
-      vh = v;
-    else
-      vl = v;
+  } else {
+    if (M > Qt) {
+      Qx *= corrf * Qt / M;
+      Qy *= corrf * Qt / M;
+      Qz *= corrf * Qt / M;
+      M = Qt * corrf;
+    }
+    double vl = 0., vh = 1.;
+    double v = 0.5 * (vl + vh);
+    e = Qt - M * v;
+    if (e < 0.) e = 0.;
+    p = eos.p(e);
+    double f = (Qt + p) * v - M;
+    double df = (Qt + p) - M * v * dpe;
+    double dvold = vh - vl;
+    double dv = dvold;
+    for (; it < 100;) {
+      const bool bis = (f + df * (vh - v)) * (f + df * (vl - v)) >= 0. ||
+                       fabs(2. * f) > fabs(dvold * df);
+      dvold = dv;
+      if (bis) {
+        dv = 0.5 * (vh - vl);
+        v = vl + dv;
+      } else {
+        dv = f / df;
+        v -= dv;
+      }
+      it++;
+      if (fabs(dv) < 0.00001) break;
+      e = Qt - M * v;
+      if (e < 0.) e = 0.;
+      p = eos.p(e);  // p(e, n) of this EoS does not depend on n
+      f = (Qt + p) * v - M;
+      df = (Qt + p) - M * v * dpe;
+      if (f > 0.)
+        vh = v;
+      else
+        vl = v;
+    }
+    vx = v * Qx / M;
+    vy = v * Qy / M;
+    vz = v * Qz / M;
+    e = Qt - M * v;
+    p = eos.p(e);
+    const double s2 = sqrt(1 - vx * vx - vy * vy - vz * vz);
+    nb = Qb * s2;
+    nq = Qq * s2;
+    ns = Qs * s2;
   }
-  r.vx = v * q[X_] / M;
-  r.vy = v * q[Y_] / M;
-  r.vz = v * q[Z_] / M;
-  r.e = q[T_] - M * v;
-  r.p = eos.p(r.e);  // p(e, previous iterate's n): n does not enter p here
-  (void)nb;
-  (void)nq;
-  (void)ns;
-  const double s2 = sqrt(1 - r.vx * r.vx - r.vy * r.vy - r.vz * r.vz);
-  r.nb = q[NB_] * s2;
-  r.nq = q[NQ_] * s2;
-  r.ns = q[NS_] * s2;
+  q[X_] = Qx;
+  q[Y_] = Qy;
+  q[Z_] = Qz;
+  r.e = e;
+  r.p = p;
+  r.nb = nb;
+  r.nq = nq;
+  r.ns = ns;
+  r.vx = vx;
+  r.vy = vy;
+  r.vz = vz;
   return it;
 }
 
 // transformCV (src/rmn.cpp:253-263)
-__device__ inline void p2c(const Prim &a, double Q[7]) {
+__device__ __forceinline__ void p2c(const Prim &a, double Q[7]) {
   const double g2 = 1. / (1 - a.vx * a.vx - a.vy * a.vy - a.vz * a.vz);
   Q[T_] = (a.e + a.p * (a.vx * a.vx + a.vy * a.vy + a.vz * a.vz)) * g2;
   Q[X_] = (a.e + a.p) * a.vx * g2;

@@ -41,7 +41,7 @@ __device__ inline long stride(const Geom &g) {
 // src/cll.cpp:106-204): half slopes limited by minmod, state (q -+ dq)/tau,
 // then con2prim.  cm/c0/cp: the cell and its clamped neighbours.
 template <int SIDE>
-__device__ inline void face_state(const DevEos &eos, const double *__restrict__ Qs, long N,
+__device__ __forceinline__ void face_state(const DevEos &eos, const double *__restrict__ Qs, long N,
                                   long cm, long c0, long cp, double tau, Prim &r) {
   double q[7];
 #pragma unroll
@@ -788,7 +788,7 @@ k_visc2(Geom g, const double *__restrict__ ph, const double *__restrict__ pi0,
 // viscous interface flux (Hydro::visc_flux, src/hdo.cpp:966-1014) between
 // cells l (left) and r (right) along axis D; returns false if skipped.
 template <int D>
-__device__ inline bool visc_face(const Geom &g, long l, long r, const double *__restrict__ ph,
+__device__ __forceinline__ bool visc_face(const Geom &g, long l, long r, const double *__restrict__ ph,
                                  const double *__restrict__ piH, const double *__restrict__ PiH,
                                  const double *__restrict__ m, int m_full, double tau, double dt,
                                  double F[4]) {

@@ -1,0 +1,236 @@
+// vhlle: command-line driver over the GPU hydro step, with the reference's
+// params-file interface (src/main.cpp:70-156, 226-243, 399-428).
+//
+// Same keys, defaults and CLI (-params, -system, -ISinput, -outputDir) as the
+// reference; keys the reference does not know are echoed "UUU", comment lines
+// ('!') "CCC".  Added keys: eosType 5 = ideal gas p=e/3 (the reference's
+// tabulated EoS files are not part of this build); icModel 10 = smooth
+// Gaussian state (icE0, icSigmaX, icSigmaY, icEtaPlateau, icEtaSigma);
+// gubserScale (energy scale of icModel 4); zetaSPeakT, zetaSWidthMinus,
+// zetaSWidthPlus (zetaSparam 5); dumpState 1 writes the final conserved
+// state to outputDir/state.bin (7 x N doubles, C-ABI layout).
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <chrono>
+#include <fstream>
+#include <functional>
+#include <iostream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <sys/stat.h>
+
+#include "../host/vhlle.h"
+
+using namespace std;
+
+namespace {
+
+int nx{100}, ny{100}, nz{100}, eosType{1}, etaSparam{0}, zetaSparam{0}, eosTypeHadron{0};
+bool freezeoutOnly{false}, freezeoutExtend{false}, vorticityOn{false};
+double xmin{-5.0}, xmax{5.0}, ymin{-5.0}, ymax{5.0}, etamin{-5.0}, etamax{5.0}, tau0{1.0},
+    tauMax{20.0}, tauResize{4.0}, dtau{0.05}, etaS{0.08}, zetaS{0.0}, eCrit{0.5},
+    etaSEpsilonMin{5.}, al{0.}, ah{0.}, aRho{0.}, T0{0.15}, etaSMin{0.08}, etaSShiftMuB{0.},
+    etaSScaleMuB{0.}, zetaSPeakEpsilon{5.}, zetaSScaleBeta{0.103}, zetaSSigmaMinus{0.1},
+    zetaSSigmaPlus{0.1}, epsilon0, Rgt{1.0}, Rgz{1.0}, impactPar, s0ScaleFactor;
+string collSystem, outputDir{"data"}, isInputFile, vtk_values{""};
+int icModel{1}, glauberVariable{1}, smoothingType{0};
+// added keys
+double gubserScale{1.0}, icE0{50.0}, icSigmaX{2.5}, icSigmaY{3.0}, icEtaPlateau{1.5},
+    icEtaSigma{1.0}, zetaSPeakT{0.170}, zetaSWidthMinus{0.010}, zetaSWidthPlus{0.030};
+int dumpState{0};
+
+void readParameters(const char *parFile) {
+  ifstream fin(parFile);
+  if (!fin.is_open()) {
+    cout << "cannot open parameters file " << parFile << endl;
+    exit(1);
+  }
+  cout << "vhlle: reading parameters from " << parFile << endl;
+  auto I = [](int &v) { return [&v](const string &s) { v = atoi(s.c_str()); }; };
+  auto D = [](double &v) { return [&v](const string &s) { v = atof(s.c_str()); }; };
+  auto B = [](bool &v) { return [&v](const string &s) { v = atoi(s.c_str()); }; };
+  map<string, function<void(const string &)>> h = {
+      {"eosType", I(eosType)}, {"eosTypeHadron", I(eosTypeHadron)}, {"nx", I(nx)},
+      {"ny", I(ny)}, {"nz", I(nz)}, {"icModel", I(icModel)}, {"glauberVar", I(glauberVariable)},
+      {"xmin", D(xmin)}, {"xmax", D(xmax)}, {"ymin", D(ymin)}, {"ymax", D(ymax)},
+      {"etamin", D(etamin)}, {"etamax", D(etamax)}, {"tau0", D(tau0)}, {"tauMax", D(tauMax)},
+      {"tauGridResize", D(tauResize)}, {"dtau", D(dtau)}, {"e_crit", D(eCrit)},
+      {"etaS", D(etaS)}, {"zetaS", D(zetaS)}, {"zetaSparam", I(zetaSparam)},
+      {"zetaSScaleBeta", D(zetaSScaleBeta)}, {"zetaSPeakEpsilon", D(zetaSPeakEpsilon)},
+      {"zetaSSigmaMinus", D(zetaSSigmaMinus)}, {"zetaSSigmaPlus", D(zetaSSigmaPlus)},
+      {"epsilon0", D(epsilon0)}, {"Rg", D(Rgt)}, {"Rgz", D(Rgz)}, {"impactPar", D(impactPar)},
+      {"s0ScaleFactor", D(s0ScaleFactor)},
+      {"VTK_output_values", [](const string &s) { vtk_values = s; }},
+      {"etaSparam", I(etaSparam)}, {"aRho", D(aRho)}, {"ah", D(ah)}, {"al", D(al)},
+      {"T0", D(T0)}, {"etaSEpsilonMin", D(etaSEpsilonMin)}, {"etaSMin", D(etaSMin)},
+      {"etaSShiftMuB", D(etaSShiftMuB)}, {"etaSScaleMuB", D(etaSScaleMuB)},
+      {"freezeoutOnly", B(freezeoutOnly)}, {"freezeoutExtend", B(freezeoutExtend)},
+      {"vorticity", B(vorticityOn)}, {"smoothingType", I(smoothingType)},
+      // added by this build
+      {"gubserScale", D(gubserScale)}, {"icE0", D(icE0)}, {"icSigmaX", D(icSigmaX)},
+      {"icSigmaY", D(icSigmaY)}, {"icEtaPlateau", D(icEtaPlateau)},
+      {"icEtaSigma", D(icEtaSigma)}, {"zetaSPeakT", D(zetaSPeakT)},
+      {"zetaSWidthMinus", D(zetaSWidthMinus)}, {"zetaSWidthPlus", D(zetaSWidthPlus)},
+      {"dumpState", I(dumpState)},
+  };
+  while (fin.good()) {
+    string line;
+    getline(fin, line);
+    istringstream sl(line);
+    string name, value;
+    sl >> name >> value;
+    if (name.empty()) continue;
+    auto it = h.find(name);
+    if (it != h.end())
+      it->second(value);
+    else if (name[0] == '!')
+      cout << "CCC " << sl.str() << endl;
+    else
+      cout << "UUU " << sl.str() << endl;
+  }
+  auto borders = [](double mn, double mx, const char *x) {
+    if (mn >= mx) {
+      cerr << "FATAL: The hydrodynamic grid is not set up correctly! \n"
+           << x << "min >= " << x << "max \n";
+      exit(1);
+    }
+  };
+  borders(xmin, xmax, "x");
+  borders(ymin, ymax, "y");
+  borders(etamin, etamax, "eta");
+}
+
+void readCommandLine(int argc, char **argv) {
+  if (argc == 1) {
+    cout << "no CL params - exiting.\n";
+    exit(1);
+  }
+  for (int i = 1; i < argc - 1; i++) {
+    if (strcmp(argv[i], "-system") == 0) collSystem = argv[i + 1];
+    if (strcmp(argv[i], "-params") == 0) readParameters(argv[i + 1]);
+    if (strcmp(argv[i], "-ISinput") == 0) isInputFile = argv[i + 1];
+    if (strcmp(argv[i], "-outputDir") == 0) outputDir = argv[i + 1];
+  }
+  cout << "vhlle: command line parameters are:\n"
+       << "collision system:  " << collSystem << "\nini.state input:  " << isInputFile
+       << "\noutput directory:  " << outputDir << endl;
+}
+
+// smooth state e = icE0 exp(-x^2/2sx^2 - y^2/2sy^2) H(eta), v = 0
+void setGaussianIC(Fluid *f, EoS *eos, double tau) {
+  for (int iz = 0; iz < f->getNZ(); iz++)
+    for (int iy = 0; iy < f->getNY(); iy++)
+      for (int ix = 0; ix < f->getNX(); ix++) {
+        const double x = f->getX(ix), y = f->getY(iy);
+        const double eta = f->getNZ() > 1 ? f->getZ(iz) : 0.0;
+        const double a = fabs(eta);
+        const double H = a < icEtaPlateau
+                             ? 1.0
+                             : exp(-(a - icEtaPlateau) * (a - icEtaPlateau) /
+                                   (2 * icEtaSigma * icEtaSigma));
+        const double e = icE0 *
+                         exp(-x * x / (2 * icSigmaX * icSigmaX) - y * y / (2 * icSigmaY * icSigmaY)) *
+                         H;
+        Cell *c = f->getCell(ix, iy, iz);
+        c->setPrimVar(eos, tau, e, 0., 0., 0., 0., 0., 0.);
+        c->saveQprev();
+        if (e > 0.) c->setAllM(1.);
+      }
+}
+
+double totalEnergy(Fluid *f, double tau) {
+  double E = 0.;
+  double Q[7];
+  for (int iz = 0; iz < f->getNZ(); iz++)
+    for (int iy = 0; iy < f->getNY(); iy++)
+      for (int ix = 0; ix < f->getNX(); ix++) {
+        f->getCell(ix, iy, iz)->getQ(Q);
+        E += Q[T_];
+      }
+  return E;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  readCommandLine(argc, argv);
+  if (eosType != 5) {
+    cout << "eosType " << eosType
+         << " needs the reference's EoS tables, which this build does not ship; "
+            "use eosType 5 (ideal gas p=e/3)\n";
+    return 0;
+  }
+  EoS *eos = new EoSIdeal();
+  TransportCoeff *trcoeff =
+      new TransportCoeff(etaS, zetaS, ah, al, aRho, T0, etaSMin, etaSEpsilonMin, etaSShiftMuB,
+                         etaSScaleMuB, zetaSPeakEpsilon, zetaSScaleBeta, zetaSSigmaMinus,
+                         zetaSSigmaPlus, eos, etaSparam, zetaSparam);
+  trcoeff->zetaSPeakT = zetaSPeakT;
+  trcoeff->zetaSWidthMinus = zetaSWidthMinus;
+  trcoeff->zetaSWidthPlus = zetaSWidthPlus;
+  Fluid *f = new Fluid(eos, eos, trcoeff, nx, ny, nz, xmin, xmax, ymin, ymax, etamin, etamax,
+                       dtau, eCrit);
+  cout << "fluid allocation done\n";
+  if (icModel == 4) {
+    ICGubser ic(gubserScale);
+    ic.setIC(f, eos, tau0);
+  } else if (icModel == 10) {
+    setGaussianIC(f, eos, tau0);
+  } else {
+    cout << "icModel = " << icModel << " not implemented in this build (4, 10)\n";
+    return 0;
+  }
+  cout << "IC done\n";
+  if (tauResize < tauMax)
+    cout << "note: tauGridResize (expandGrid2x) is not part of this build; ignored\n";
+  Hydro *h = new Hydro(f, eos, trcoeff, tau0, dtau);
+  mkdir(outputDir.c_str(), 0755);
+  long steps = 0;
+  const auto t0 = chrono::steady_clock::now();
+  do {
+    // sub-steps while dtau exceeds tau * d(eta) (src/main.cpp:405-419)
+    int nSub = 1;
+    while (dtau / nSub > 1.0 * h->getTau() * (etamax - etamin) / (nz - 1)) nSub *= 2;
+    if (nSub > 1) {
+      h->setDtau(h->getDtau() / nSub);
+      h->performSteps(nSub);
+      h->setDtau(h->getDtau() * nSub);
+      cout << "timestep reduced by " << nSub << endl;
+    } else {
+      h->performStep();
+    }
+    steps += nSub;
+  } while (h->getTau() < tauMax + 0.0001);
+  const double secs = chrono::duration<double>(chrono::steady_clock::now() - t0).count();
+  cout << "tau = " << h->getTau() << "  steps = " << steps
+       << "  sum Q^tau = " << totalEnergy(f, h->getTau()) << endl;
+  cout << "Execution time = " << secs << " [sec], "
+       << (double)f->ncell() * steps / secs << " cell-updates/s" << endl;
+  if (dumpState) {
+    f->syncFromDevice();
+    const string fn = outputDir + "/state.bin";
+    FILE *o = fopen(fn.c_str(), "wb");
+    if (o) {
+      double Q[7];
+      std::vector<double> buf(7 * f->ncell());
+      for (int iz = 0; iz < nz; iz++)
+        for (int iy = 0; iy < ny; iy++)
+          for (int ix = 0; ix < nx; ix++) {
+            f->getCell(ix, iy, iz)->getQ(Q);
+            const long c = ix + (long)nx * (iy + (long)ny * iz);
+            for (int k = 0; k < 7; k++) buf[k * f->ncell() + c] = Q[k];
+          }
+      fwrite(buf.data(), sizeof(double), buf.size(), o);
+      fclose(o);
+      cout << "wrote " << fn << endl;
+    }
+  }
+  delete h;
+  delete f;
+  delete trcoeff;
+  delete eos;
+  return 0;
+}

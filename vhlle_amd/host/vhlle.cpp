@@ -1,0 +1,657 @@
+// Host side of the vHLLE class API on the GPU (see vhlle.h).
+#include "vhlle.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iomanip>
+#include <iostream>
+#include <stdexcept>
+
+// ------------------------------------------------------------- helpers
+int index44(const int &i, const int &j) {
+  if (i > 3 || j > 3 || i < 0 || j < 0) {
+    std::cout << "index44: i j " << i << " " << j << std::endl;
+    std::exit(1);
+  }
+  return j < i ? (i * (i + 1)) / 2 + j : (j * (j + 1)) / 2 + i;
+}
+
+// ------------------------------------------------------------------ EoS
+double EoS::s(double e, double nb, double nq, double ns) {
+  double T, mub, muq, mus, p_;
+  eos(e, nb, nq, ns, T, mub, muq, mus, p_);
+  if (T > 0.0) return (e + p_ - mub * nb - muq * nq - mus * ns) / T;
+  return 0.;
+}
+
+double EoSIdeal::t(double e) {
+  const double cnst = (16 + 0.5 * 21.0 * 2.5) * (C_PI * C_PI) / 30.0 /
+                      (0.197326968 * 0.197326968 * 0.197326968);
+  return e > 0. ? 1.0 * std::pow(e / cnst, 0.25) : 0.;
+}
+
+void EoSIdeal::eos(double e, double, double, double, double &T, double &mub, double &muq,
+                   double &mus, double &p_) {
+  p_ = e / 3.;
+  T = t(e);
+  mub = muq = mus = 0.;
+}
+
+vh_eos_t EoSIdeal::device() const {
+  vh_eos_t d;
+  d.kind = VH_EOS_IDEAL;
+  return d;
+}
+
+// ----------------------------------------------------- con2prim (host)
+// Bisection/Newton root of v = M / (Q_t + p(Q_t - M v, n)), from v = 0.5,
+// stopping at |dv| < 1e-5 (src/rmn.cpp:28-139).
+void transformPV(EoS *eos, double Q[7], double &e, double &p, double &nb, double &nq,
+                 double &ns, double &vx, double &vy, double &vz) {
+  const double dpe = 1. / 3., corrf = 0.9999;
+  double M = std::sqrt(Q[X_] * Q[X_] + Q[Y_] * Q[Y_] + Q[Z_] * Q[Z_]);
+  if (Q[T_] <= 0.) {
+    e = p = 0.;
+    vx = vy = vz = 0.;
+    nb = nq = ns = 0.;
+    return;
+  }
+  if (M == 0.) {
+    e = Q[T_];
+    vx = vy = vz = 0.;
+    nb = Q[NB_];
+    nq = Q[NQ_];
+    ns = Q[NS_];
+    p = eos->p(e, nb, nq, ns);
+    return;
+  }
+  if (M > Q[T_]) {
+    Q[X_] *= corrf * Q[T_] / M;
+    Q[Y_] *= corrf * Q[T_] / M;
+    Q[Z_] *= corrf * Q[T_] / M;
+    M = Q[T_] * corrf;
+  }
+  double lo = 0., hi = 1., v = 0.5 * (lo + hi);
+  auto refresh = [&](double &f, double &df) {
+    e = Q[T_] - M * v;
+    if (e < 0.) e = 0.;
+    const double g = std::sqrt(1 - v * v);
+    nb = Q[NB_] * g;
+    nq = Q[NQ_] * g;
+    ns = Q[NS_] * g;
+    p = eos->p(e, nb, nq, ns);
+    f = (Q[T_] + p) * v - M;
+    df = (Q[T_] + p) - M * v * dpe;
+  };
+  double f, df;
+  refresh(f, df);
+  double step_prev = hi - lo, step = step_prev;
+  for (int it = 0; it < 100; it++) {
+    const bool bisect = (f + df * (hi - v)) * (f + df * (lo - v)) >= 0. ||
+                        std::fabs(2. * f) > std::fabs(step_prev * df);
+    step_prev = step;
+    if (bisect) {
+      step = 0.5 * (hi - lo);
+      v = lo + step;
+    } else {
+      step = f / df;
+      v -= step;
+    }
+    if (std::fabs(step) < 0.00001) break;
+    refresh(f, df);
+    if (f > 0.)
+      hi = v;
+    else
+      lo = v;
+  }
+  vx = v * Q[X_] / M;
+  vy = v * Q[Y_] / M;
+  vz = v * Q[Z_] / M;
+  e = Q[T_] - M * v;
+  p = eos->p(e, nb, nq, ns);
+  const double g = std::sqrt(1 - vx * vx - vy * vy - vz * vz);
+  nb = Q[NB_] * g;
+  nq = Q[NQ_] * g;
+  ns = Q[NS_] * g;
+  if (e < 0. || std::sqrt(vx * vx + vy * vy + vz * vz) > 1.) {
+    std::cout << Q[T_] << "  " << Q[X_] << "  " << Q[Y_] << "  " << Q[Z_] << "  " << Q[NB_]
+              << std::endl;
+    std::cout << "transformRF::Error\n";
+  }
+}
+
+void transformCV(double e, double p, double nb, double nq, double ns, double vx, double vy,
+                 double vz, double Q[]) {
+  const double g2 = 1. / (1 - vx * vx - vy * vy - vz * vz);
+  Q[T_] = (e + p * (vx * vx + vy * vy + vz * vz)) * g2;
+  Q[X_] = (e + p) * vx * g2;
+  Q[Y_] = (e + p) * vy * g2;
+  Q[Z_] = (e + p) * vz * g2;
+  Q[NB_] = nb * std::sqrt(g2);
+  Q[NQ_] = nq * std::sqrt(g2);
+  Q[NS_] = ns * std::sqrt(g2);
+}
+
+// ------------------------------------------------------- TransportCoeff
+double TransportCoeff::zetaS(double e, double T, double s, double P) {
+  switch (zetaSparam) {
+    case 0:
+      return zetaS0 * (1. / 3. - eos->cs2(e)) / (std::exp((0.16 - T) / 0.001) + 1.);
+    case 1: {
+      const double Tp = 0.180;
+      if (T < 0.180)
+        return 0.03 + (0.08 * std::exp(((T / Tp) - 1.) / (0.0025))) +
+               (0.22 * std::exp(((T / Tp) - 1) / (0.022)));
+      if (T < 0.200) return 27.55 * (T / Tp) - 13.45 - (13.77 * (T / Tp) * (T / Tp));
+      return 0.001 + (0.9 * std::exp(-((T / Tp) - 1.) / (0.0025))) +
+             (0.25 * std::exp(-((T / Tp) - 1.) / (0.13)));
+    }
+    case 2: {
+      const double Tpk = 0.165, Tw = 0.010, Bn = 0.24, Bw = 1.5;
+      if (T > Tpk) return Bn * ((Bw * Bw) / ((((T / Tpk) - 1.) * ((T / Tpk) - 1.)) + (Bw * Bw)));
+      return Bn * (std::exp(-((T - Tpk) / Tw) * ((T - Tpk) / Tw)));
+    }
+    case 3: {
+      const double Tpk2 = 0.160, Bn2 = 0.13, B1 = 0.01, B2 = 0.12;
+      if (T < Tpk2) return Bn2 * std::exp(-((T - Tpk2) * (T - Tpk2) / (B1 * B1)));
+      return Bn2 * std::exp(-((T - Tpk2) * (T - Tpk2) / (B2 * B2)));
+    }
+    case 4: {
+      if (std::fabs(s) < 1e-10 || T < 1e-1) return 0;
+      const double sig = e < zetaSPeakEpsilon ? zetaSSigmaMinus : zetaSSigmaPlus;
+      const double d = zetaSScaleBeta * (std::pow(e, 0.25) - std::pow(zetaSPeakEpsilon, 0.25));
+      return ((e + P) / (s * T)) * zetaS0 * std::exp((d * d) / 2.0 * (sig * sig));
+    }
+    case 5: {
+      if (T <= 0.) return 0.;
+      const double w = T < zetaSPeakT ? zetaSWidthMinus : zetaSWidthPlus;
+      const double d = (T - zetaSPeakT) / w;
+      return zetaS0 * std::exp(-0.5 * d * d);
+    }
+    default:
+      return 0.;
+  }
+}
+
+double TransportCoeff::etaS(double e, double rho, double T, double muB, double s, double P) {
+  switch (etaSparam) {
+    case 0:
+      return etaS0;
+    case 1:
+      return etaSMin + ((T > T0) ? ah * (T - T0) : al * (T - T0));
+    case 2:
+      return std::max(0.0, etaSMin + ((e > etaSEpsilonMin)
+                                          ? ((ah * (e - etaSEpsilonMin) + aRho * rho))
+                                          : al * (e - etaSEpsilonMin) + aRho * rho));
+    case 3: {
+      if (std::fabs(s) < 1e-10 || T < 1e-1) return 0;
+      const double Tc = T0 + etaSShiftMuB * (muB / 0.938);
+      return ((e + P) / (s * T)) *
+             std::max(0.0, (etaSMin + ((T > Tc) ? ah * (T - Tc) : al * (T - Tc))) *
+                               (1 + etaSScaleMuB * (muB / 0.938)));
+    }
+    default:
+      return 0.;
+  }
+}
+
+void TransportCoeff::getEta(double e, double rho, double T, double muB, double s, double P,
+                            double &_etaS, double &_zetaS) {
+  _etaS = etaS(e, rho, T, muB, s, P);
+  _zetaS = zetaS(e, T, s, P);
+}
+
+void TransportCoeff::getTau(double e, double rho, double T, double muB, double s, double P,
+                            double &_taupi, double &_tauPi) {
+  if (T > 1e-1 && std::fabs(e + P) > 1.e-10) {
+    _taupi = (5. / 5.068) * s * etaS(e, rho, T, muB, s, P) / (e + P);
+    _tauPi = (6. / 5.068) * zetaS(e, T, s, P) * s / (e + P);
+  } else {
+    _taupi = _tauPi = 0.;
+  }
+}
+
+void TransportCoeff::getOther(double e, double nb, double nq, double ns, double &deltapipi,
+                              double &taupipi, double &lambdapiPi, double &phi7) {
+  deltapipi = 4. / 3.;
+  taupipi = 10. / 7.;
+  lambdapiPi = 6. / 5.;
+  phi7 = 9. / 70. / eos->p(e, nb, nq, ns);
+  if (std::isinf(phi7)) phi7 = 0.0;
+}
+
+void TransportCoeff::getOtherBulk(double e, double, double, double, double &delPiPi,
+                                  double &lamPipi) {
+  delPiPi = 2. / 3.;
+  lamPipi = 8. / 5. * (1. / 3. - eos->cs2(e));
+}
+
+vh_trcoeff_t TransportCoeff::device() const {
+  vh_trcoeff_t d;
+  d.etaS0 = etaS0;
+  d.zetaS0 = zetaS0;
+  d.ah = ah;
+  d.al = al;
+  d.aRho = aRho;
+  d.T0 = T0;
+  d.etaSMin = etaSMin;
+  d.etaSEpsilonMin = etaSEpsilonMin;
+  d.etaSShiftMuB = etaSShiftMuB;
+  d.etaSScaleMuB = etaSScaleMuB;
+  d.zetaSPeakEpsilon = zetaSPeakEpsilon;
+  d.zetaSScaleBeta = zetaSScaleBeta;
+  d.zetaSSigmaMinus = zetaSSigmaMinus;
+  d.zetaSSigmaPlus = zetaSSigmaPlus;
+  d.etaSparam = etaSparam;
+  d.zetaSparam = zetaSparam;
+  d.zetaSPeakT = zetaSPeakT;
+  d.zetaSWidthMinus = zetaSWidthMinus;
+  d.zetaSWidthPlus = zetaSWidthPlus;
+  return d;
+}
+
+// ----------------------------------------------------------------- Cell
+#define CELL_RD(vec, k) (f->hostRead(), f->vec[(size_t)(k) * f->N + n])
+
+void Cell::setQ(double *_Q) {
+  f->hostWrite();
+  const bool z = _Q[T_] < 0.;
+  for (int i = 0; i < 7; i++) f->hQ[(size_t)i * f->N + n] = z ? 0. : _Q[i];
+}
+void Cell::setQh(double *_Qh) {
+  f->hostWrite();
+  const bool z = _Qh[T_] < 0.;
+  for (int i = 0; i < 7; i++) f->hQh[(size_t)i * f->N + n] = z ? 0. : _Qh[i];
+}
+void Cell::getQ(double *_Q) {
+  for (int i = 0; i < 7; i++) _Q[i] = CELL_RD(hQ, i);
+}
+void Cell::getQh(double *_Qh) {
+  for (int i = 0; i < 7; i++) _Qh[i] = CELL_RD(hQh, i);
+}
+void Cell::getQprev(double *_Qp) {
+  for (int i = 0; i < 7; i++) _Qp[i] = CELL_RD(hQprev, i);
+}
+void Cell::saveQprev(void) {
+  f->hostWrite();
+  for (int i = 0; i < 7; i++) f->hQprev[(size_t)i * f->N + n] = f->hQ[(size_t)i * f->N + n];
+}
+double Cell::getpi(const int &i, const int &j) { return CELL_RD(hpi, index44(i, j)); }
+double Cell::getpiH(const int &i, const int &j) { return CELL_RD(hpiH, index44(i, j)); }
+double Cell::getPi(void) { return CELL_RD(hPi, 0); }
+double Cell::getPiH(void) { return CELL_RD(hPiH, 0); }
+void Cell::setpi(const int &i, const int &j, const double &val) {
+  f->hostWrite();
+  f->hpi[(size_t)index44(i, j) * f->N + n] = val;
+}
+void Cell::setpiH(const int &i, const int &j, const double &val) {
+  f->hostWrite();
+  f->hpiH[(size_t)index44(i, j) * f->N + n] = val;
+}
+void Cell::setPi(const double &val) {
+  f->hostWrite();
+  f->hPi[n] = val;
+}
+void Cell::setPiH(const double &val) {
+  f->hostWrite();
+  f->hPiH[n] = val;
+}
+void Cell::setAllM(double value) {
+  f->hostWrite();
+  for (int d = 0; d < 3; d++) f->hm[(size_t)d * f->N + n] = value;
+}
+double Cell::getM(int dir) { return CELL_RD(hm, dir - 1); }
+double Cell::getMaxM(void) {
+  return std::max(getM(1), std::max(getM(2), getM(3)));
+}
+void Cell::importVars(Cell *c) {  // src/cll.cpp:54-68
+  double q[7];
+  c->getQ(q);
+  for (int i = 0; i < 7; i++) {
+    f->hostWrite();
+    f->hQ[(size_t)i * f->N + n] = q[i];
+  }
+  c->getQh(q);
+  for (int i = 0; i < 7; i++) f->hQh[(size_t)i * f->N + n] = q[i];
+  c->getQprev(q);
+  for (int i = 0; i < 7; i++) f->hQprev[(size_t)i * f->N + n] = q[i];
+  setPi(c->getPi());
+  setPiH(c->getPiH());
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j <= i; j++) {
+      setpi(i, j, c->getpi(i, j));
+      setpiH(i, j, c->getpiH(i, j));
+    }
+  for (int d = 1; d <= 3; d++) f->hm[(size_t)(d - 1) * f->N + n] = c->getM(d);
+  setViscCorrCutFlag(c->getViscCorrCutFlag());
+}
+void Cell::getPrimVar(EoS *eos, double tau, double &_e, double &_p, double &_nb, double &_nq,
+                      double &_ns, double &_vx, double &_vy, double &_vz) {
+  double q[7];
+  getQ(q);
+  for (int i = 0; i < 7; i++) q[i] = q[i] / tau;
+  transformPV(eos, q, _e, _p, _nb, _nq, _ns, _vx, _vy, _vz);
+}
+void Cell::getPrimVarHCenter(EoS *eos, double tau, double &_e, double &_p, double &_nb,
+                             double &_nq, double &_ns, double &_vx, double &_vy, double &_vz) {
+  double q[7];
+  getQh(q);
+  for (int i = 0; i < 7; i++) q[i] = q[i] / tau;
+  transformPV(eos, q, _e, _p, _nb, _nq, _ns, _vx, _vy, _vz);
+}
+void Cell::getPrimVarPrev(EoS *eos, double tau, double &_e, double &_p, double &_nb,
+                          double &_nq, double &_ns, double &_vx, double &_vy, double &_vz) {
+  double q[7];
+  getQprev(q);
+  for (int i = 0; i < 7; i++) q[i] = q[i] / tau;
+  transformPV(eos, q, _e, _p, _nb, _nq, _ns, _vx, _vy, _vz);
+}
+void Cell::setPrimVar(EoS *eos, double tau, double _e, double _nb, double _nq, double _ns,
+                      double _vx, double _vy, double _vz) {  // src/cll.cpp:222-241
+  const double g2 = 1. / (1 - _vx * _vx - _vy * _vy - _vz * _vz);
+  const double p = eos->p(_e, _nb, _nq, _ns);
+  double q[7];
+  q[T_] = tau * (_e + p * (_vx * _vx + _vy * _vy + _vz * _vz)) * g2;
+  q[X_] = tau * (_e + p) * _vx * g2;
+  q[Y_] = tau * (_e + p) * _vy * g2;
+  q[Z_] = tau * (_e + p) * _vz * g2;
+  q[NB_] = tau * _nb * std::sqrt(g2);
+  q[NQ_] = tau * _nq * std::sqrt(g2);
+  q[NS_] = tau * _ns * std::sqrt(g2);
+  if (std::isinf(q[NB_]) || std::isnan(q[NB_])) {
+    std::cout << "init error!\n";
+    std::cout << "e = " << _e << " p = " << p << " vx = " << _vx << " vy = " << _vy
+              << " vz = " << _vz << " gamma2 = " << g2 << std::endl;
+    return;
+  }
+  f->hostWrite();
+  for (int i = 0; i < 7; i++) f->hQ[(size_t)i * f->N + n] = q[i];
+}
+void Cell::setViscCorrCutFlag(double value) {
+  f->hostWrite();
+  f->hvcc[n] = value;
+}
+double Cell::getViscCorrCutFlag(void) { return CELL_RD(hvcc, 0); }
+void Cell::Dump(double tau) {
+  double q[7], qh[7];
+  getQ(q);
+  getQh(qh);
+  std::cout << "---------cell values dump-------\n";
+  std::cout << std::setw(5) << ix << std::setw(5) << iy << std::setw(5) << iz << std::endl;
+  for (int i = 0; i < 7; i++) std::cout << std::setw(14) << q[i] / tau;
+  std::cout << std::endl;
+  for (int i = 0; i < 7; i++) std::cout << std::setw(14) << qh[i] / tau;
+  std::cout << "\n--------------------------------\n";
+}
+#undef CELL_RD
+
+// ---------------------------------------------------------------- Fluid
+static void vh_or_die(int rc, const char *what) {
+  if (rc != VH_OK) {
+    std::cerr << "vhlle_gpu: " << what << " failed (code " << rc << ")" << std::endl;
+    std::exit(1);
+  }
+}
+
+Fluid::Fluid(EoS *_eos, EoS *_eosH, TransportCoeff *_trcoeff, int _nx, int _ny, int _nz,
+             double _minx, double _maxx, double _miny, double _maxy, double _minz,
+             double _maxz, double _dt, double eCrit)
+    : eos(_eos), eosH(_eosH), trcoeff(_trcoeff), nx(_nx), ny(_ny), nz(_nz),
+      minx(_minx), maxx(_maxx), miny(_miny), maxy(_maxy), minz(_minz), maxz(_maxz),
+      dt(_dt), ecrit(eCrit) {
+  dx = (maxx - minx) / (nx - 1);
+  dy = (maxy - miny) / (ny - 1);
+  dz = (maxz - minz) / (nz - 1);
+  N = (long)nx * ny * nz;
+  vh_grid_t g{nx, ny, nz, minx, maxx, miny, maxy, minz, maxz};
+  vh_eos_t e = eos->device();
+  vh_trcoeff_t t = trcoeff->device();
+  vh_or_die(vh_create(&g, &e, &t, VH_PREC_FP64, &ctx), "vh_create");
+  // Cell ctor defaults (src/cll.cpp:33-52)
+  hQ.assign(7 * N, 0.);
+  hQh.assign(7 * N, 0.);
+  hQprev.assign(7 * N, 0.);
+  hpi.assign(10 * N, 0.);
+  hpiH.assign(10 * N, 0.);
+  hPi.assign(N, 0.);
+  hPiH.assign(N, 0.);
+  hm.assign(3 * N, 0.);
+  hvcc.assign(N, 1.);
+  host_dirty = true;
+}
+
+Fluid::~Fluid() {
+  if (ctx) vh_destroy(ctx);
+}
+
+Cell *Fluid::getCell(int ix, int iy, int iz) {
+  ix = ix > 0 ? ix : 0;
+  ix = ix < nx ? ix : nx - 1;
+  iy = iy > 0 ? iy : 0;
+  iy = iy < ny ? iy : ny - 1;
+  iz = iz > 0 ? iz : 0;
+  iz = iz < nz ? iz : nz - 1;
+  if (cells.empty()) {
+    cells.resize(N);
+    for (int z = 0; z < nz; z++)
+      for (int y = 0; y < ny; y++)
+        for (int x = 0; x < nx; x++) {
+          Cell &c = cells[x + (long)nx * (y + (long)ny * z)];
+          c.f = this;
+          c.n = x + (long)nx * (y + (long)ny * z);
+          c.ix = x;
+          c.iy = y;
+          c.iz = z;
+        }
+  }
+  return &cells[ix + (long)nx * iy + (long)nx * ny * iz];
+}
+
+void Fluid::hostRead() {
+  if (dev_newer) syncFromDevice();
+}
+
+void Fluid::hostWrite() {
+  if (dev_newer) syncFromDevice();
+  host_dirty = true;
+}
+
+void Fluid::syncToDevice() {
+  if (!host_dirty) return;
+  vh_state_t s{hQ.data(), hQh.data(), hQprev.data(), hpi.data(), hpiH.data(),
+               hPi.data(), hPiH.data(), hm.data(), hvcc.data()};
+  vh_or_die(vh_upload(ctx, &s), "vh_upload");
+  host_dirty = false;
+}
+
+void Fluid::syncFromDevice() {
+  if (!dev_newer) return;
+  vh_state_t s{hQ.data(), hQh.data(), hQprev.data(), hpi.data(), hpiH.data(),
+               hPi.data(), hPiH.data(), hm.data(), hvcc.data()};
+  vh_or_die(vh_download(ctx, &s), "vh_download");
+  dev_newer = false;
+}
+
+// The ghost fills and updateM are stages of performStep on the device; as
+// stand-alone calls they act on the host mirror with the reference's
+// sequential semantics (src/fld.cpp:211-423).
+void Fluid::correctImagCells(void) {
+  double Q[7], Qh[7];
+  auto edge = [&](Cell *s, Cell *a, Cell *b) {
+    s->getQ(Q);
+    a->setQ(Q);
+    b->setQ(Q);
+    s->getQh(Qh);
+    a->setQh(Qh);
+    b->setQh(Qh);
+  };
+  for (int ix = 0; ix < nx; ix++)
+    for (int iy = 0; iy < ny; iy++) {
+      edge(getCell(ix, iy, 2), getCell(ix, iy, 1), getCell(ix, iy, 0));
+      edge(getCell(ix, iy, nz - 3), getCell(ix, iy, nz - 2), getCell(ix, iy, nz - 1));
+    }
+  for (int ix = 0; ix < nx; ix++)
+    for (int iz = 0; iz < nz; iz++) {
+      edge(getCell(ix, 2, iz), getCell(ix, 1, iz), getCell(ix, 0, iz));
+      edge(getCell(ix, ny - 3, iz), getCell(ix, ny - 2, iz), getCell(ix, ny - 1, iz));
+    }
+  for (int iy = 0; iy < ny; iy++)
+    for (int iz = 0; iz < nz; iz++) {
+      edge(getCell(2, iy, iz), getCell(1, iy, iz), getCell(0, iy, iz));
+      edge(getCell(nx - 3, iy, iz), getCell(nx - 2, iy, iz), getCell(nx - 1, iy, iz));
+    }
+}
+
+void Fluid::correctImagCellsFull(void) {
+  double Q[7], pi[4][4], Pi;
+  auto edge = [&](Cell *s, Cell *a, Cell *b) {  // a = ghost 1, b = ghost 0
+    s->getQ(Q);
+    a->setQ(Q);
+    b->setQ(Q);
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++) pi[i][j] = s->getpi(i, j);
+    Pi = s->getPi();
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j <= i; j++) {
+        b->setpi(i, j, pi[i][j]);
+        a->setpi(i, j, pi[i][j]);
+      }
+    b->setPi(Pi);
+    a->setPi(Pi);
+  };
+  for (int ix = 0; ix < nx; ix++)
+    for (int iy = 0; iy < ny; iy++) {
+      edge(getCell(ix, iy, 2), getCell(ix, iy, 1), getCell(ix, iy, 0));
+      edge(getCell(ix, iy, nz - 3), getCell(ix, iy, nz - 2), getCell(ix, iy, nz - 1));
+    }
+  for (int ix = 0; ix < nx; ix++)
+    for (int iz = 0; iz < nz; iz++) {
+      edge(getCell(ix, 2, iz), getCell(ix, 1, iz), getCell(ix, 0, iz));
+      edge(getCell(ix, ny - 3, iz), getCell(ix, ny - 2, iz), getCell(ix, ny - 1, iz));
+    }
+  for (int iy = 0; iy < ny; iy++)
+    for (int iz = 0; iz < nz; iz++) {
+      edge(getCell(2, iy, iz), getCell(1, iy, iz), getCell(0, iy, iz));
+      edge(getCell(nx - 3, iy, iz), getCell(nx - 2, iy, iz), getCell(nx - 1, iy, iz));
+    }
+}
+
+void Fluid::updateM(double tau, double _dt) {
+  std::vector<double> dm(3 * N, 0.);
+  for (int ix = 0; ix < nx; ix++)
+    for (int iy = 0; iy < ny; iy++)
+      for (int iz = 0; iz < nz; iz++) {
+        Cell *c = getCell(ix, iy, iz);
+        double *d = &dm[3 * c->n];
+        if (c->getMaxM() < 1.) {
+          if (getCell(ix + 1, iy, iz)->getM(X_) >= 1. || getCell(ix - 1, iy, iz)->getM(X_) >= 1.)
+            d[0] = _dt / dx;
+          if (getCell(ix, iy + 1, iz)->getM(Y_) >= 1. || getCell(ix, iy - 1, iz)->getM(Y_) >= 1.)
+            d[1] = _dt / dy;
+          if (getCell(ix, iy, iz + 1)->getM(Z_) >= 1. || getCell(ix, iy, iz - 1)->getM(Z_) >= 1.)
+            d[2] = _dt / dz / tau;
+          if (d[0] == 0. && d[1] == 0.) {
+            if (getCell(ix + 1, iy + 1, iz)->getMaxM() >= 1. ||
+                getCell(ix + 1, iy - 1, iz)->getMaxM() >= 1. ||
+                getCell(ix - 1, iy + 1, iz)->getMaxM() >= 1. ||
+                getCell(ix - 1, iy - 1, iz)->getMaxM() >= 1.) {
+              d[0] = 0.707 * _dt / dx;
+              d[1] = 0.707 * _dt / dy;
+            }
+          }
+        }
+      }
+  hostWrite();
+  for (long c = 0; c < N; c++)
+    for (int k = 0; k < 3; k++) {
+      double &mk = hm[(size_t)k * N + c];
+      mk += dm[3 * c + k];
+      if (mk > 0.9)
+        for (int j = 0; j < 3; j++) hm[(size_t)j * N + c] = 1.;
+    }
+}
+
+// ---------------------------------------------------------------- Hydro
+Hydro::Hydro(Fluid *_f, EoS *_eos, TransportCoeff *_trcoeff, double _t0, double _dt)
+    : f(_f), eos(_eos), trcoeff(_trcoeff), dt(_dt), tau(_t0) {}
+
+Hydro::~Hydro() {}
+
+void Hydro::setDtau(double deltaTau) {  // src/hdo.cpp:67-75
+  dt = deltaTau;
+  if (dt > f->getDx() / 2. || dt > f->getDy() / 2.) {
+    std::cout << "too big delta_tau " << dt << "  " << f->getDx() << "  " << f->getDy()
+              << "  " << tau * f->getDz() << std::endl;
+    std::exit(1);
+  }
+}
+
+// maps device status bits to the reference's diagnostics and exits
+void Hydro::check(int rc, int status) {
+  vh_or_die(rc, "vh_step");
+  if (status & VH_ST_BIG_E) {
+    std::cout << "e>1e10; debug info below:\n";
+    std::exit(0);
+  }
+  if (status & VH_ST_NAN_FLUX) {
+    std::cout << "---- error in hlle_flux: f_nb undefined!\n";
+    std::exit(1);
+  }
+  if (status & VH_ST_NAN_SOURCE) {
+    std::cout << "---- error in source_step: k_nb undefined!\n";
+    std::exit(1);
+  }
+  if (status & VH_ST_NAN_PI) {
+    std::cout << "hydro:NSquant: inf/nan\n";
+    std::exit(1);
+  }
+}
+
+void Hydro::performStep(void) {
+  f->syncToDevice();
+  int status = 0;
+  const int rc = vh_step(f->context(), &tau, dt, &status);
+  f->markDeviceNewer();
+  check(rc, status);
+}
+
+void Hydro::performSteps(int nsteps) {
+  f->syncToDevice();
+  int status = 0;
+  const int rc = vh_run(f->context(), &tau, dt, nsteps, &status);
+  f->markDeviceNewer();
+  check(rc, status);
+}
+
+// ------------------------------------------------------------- ICGubser
+// Gubser flow at proper time tau with q = 1 and t = 1 as in the reference
+// (src/icGubser.cpp:17-62), energy density times scale.
+void ICGubser::setIC(Fluid *f, EoS *eos, double tau) {
+  double avv_num = 0., avv_den = 0., Etotal = 0.0;
+  for (int ix = 0; ix < f->getNX(); ix++)
+    for (int iy = 0; iy < f->getNY(); iy++)
+      for (int iz = 0; iz < f->getNZ(); iz++) {
+        Cell *c = f->getCell(ix, iy, iz);
+        const double x = f->getX(ix), y = f->getY(iy), eta = f->getZ(iz);
+        const double _t = 1.0, q = 1.0, r = std::sqrt(x * x + y * y);
+        const double k = 2. * q * q * _t * r / (1. + q * q * _t * _t + q * q * r * r);
+        const double qq = q * q * (_t * _t - r * r);
+        double e = 4. * q * q / (1. + 2. * q * q * (_t * _t + r * r) + qq * qq) / _t;
+        if (e < 0.) e = 0.;
+        e = scale_ * std::pow(e, 4. / 3.);
+        const double vx = x / (r + 1e-50) * k, vy = y / (r + 1e-50) * k, vz = 0.0;
+        avv_num += std::sqrt(vx * vx + vy * vy) * e;
+        avv_den += e;
+        c->setPrimVar(eos, tau, e, 0., 0., 0., vx, vy, vz);
+        const double p = eos->p(e, 0., 0., 0.);
+        const double g2 = 1.0 / (1.0 - vx * vx - vy * vy - vz * vz);
+        Etotal += ((e + p) * g2 * (std::cosh(eta) + vz * std::sinh(eta)) - p * std::cosh(eta));
+        c->saveQprev();
+        if (e > 0.) c->setAllM(1.);
+      }
+  std::cout << "average initial flow = " << avv_num / avv_den << std::endl;
+  std::cout << "total energy = " << Etotal * f->getDx() * f->getDy() * f->getDz() * tau
+            << std::endl;
+}

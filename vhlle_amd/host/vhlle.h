@@ -1,0 +1,250 @@
+// vhlle.h -- the reference's class API for the hydro step, backed by the GPU.
+//
+// Same class names, signatures, argument meaning and error behaviour as
+// yukarpenko/vhlle's EoS (src/eos.h:21-38), TransportCoeff
+// (src/trancoeff.h:6-45), Cell (src/cll.h:38-267), Fluid (src/fld.h:10-88),
+// Hydro (src/hdo.h:10-58), ICGubser (src/icGubser.h:6-12) and the free
+// functions transformPV/transformCV (src/rmn.h:8-20), so code written against
+// those headers compiles against this one.  Underneath, Fluid owns a device
+// context of the C-ABI (include/vhlle_gpu.h); the cells live in HBM and the
+// host sees them through a lazily synchronised mirror:
+//   * Fluid::getCell returns a Cell proxy; reading a cell after a step
+//     downloads the whole state once, writing one marks the mirror dirty;
+//   * Hydro::performStep uploads a dirty mirror, runs the step on the GPU and
+//     maps device status bits to the reference's messages and exit codes.
+#pragma once
+
+#include <cmath>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/vhlle_gpu.h"
+
+// Q[] component indexes and stage modes (src/inc.h:4-13)
+#ifndef T_
+#define T_ 0
+#define X_ 1
+#define Y_ 2
+#define Z_ 3
+#define NB_ 4
+#define NQ_ 5
+#define NS_ 6
+#define PREDICT 0
+#define CORRECT 1
+#endif
+
+const double C_PI = 3.14159265358979312;  // src/inc.h:18
+const double gevtofm = 5.067728853;       // src/inc.h:22
+
+// index of the pi^{mu nu} component (i,j) in the packed 10-array
+// (src/cll.cpp:22-31); exits on an out-of-range index like the reference.
+int index44(const int &i, const int &j);
+
+// ------------------------------------------------------------------ EoS
+// Abstract equation of state (src/eos.h:21-38).  device() describes the EoS
+// to the GPU kernels; an EoS the kernels do not implement throws.
+class EoS {
+ public:
+  virtual ~EoS() {}
+  virtual void eos(double e, double nb, double nq, double ns, double &T, double &mub,
+                   double &muq, double &mus, double &p) = 0;
+  virtual double p(double e, double nb, double ns, double nq) = 0;
+  double s(double e, double nb, double nq, double ns);
+  inline double cs2(void) { return 1. / 3.; }
+  inline double cs(void) { return std::sqrt(1. / 3.); }
+  virtual inline double cs2(double) { return 1. / 3.; }
+  virtual vh_eos_t device() const = 0;
+};
+
+// Ideal massless gas: the SIMPLE branch of EoSs (src/eos.cpp:87-111),
+// p = e/3, T = (e/cnst)^(1/4), mu = 0.
+class EoSIdeal : public EoS {
+ public:
+  void eos(double e, double nb, double nq, double ns, double &T, double &mub, double &muq,
+           double &mus, double &p) override;
+  double p(double e, double nb, double ns, double nq) override { return e / 3.; }
+  double t(double e);
+  double cs2(double) override { return 1. / 3.; }
+  vh_eos_t device() const override;
+};
+
+// con2prim / prim2con on the host (src/rmn.cpp:28-139, 253-263), used by ICs
+// and by host code reading cells; the hot path runs them on the GPU.
+void transformPV(EoS *eos, double Q[7], double &e, double &p, double &nb, double &nq,
+                 double &ns, double &vx, double &vy, double &vz);
+void transformCV(double e, double p, double nb, double nq, double ns, double vx, double vy,
+                 double vz, double Q[]);
+
+// ------------------------------------------------------- TransportCoeff
+// (src/trancoeff.h:6-45, src/trancoeff.cpp:19-100).  zetaSparam 5 is the
+// T-peaked bulk viscosity of this build: zetaS0*exp(-(T-Tpk)^2/(2w^2)).
+class TransportCoeff {
+ public:
+  const double etaS0, zetaS0, ah, al, aRho, T0, etaSMin, etaSEpsilonMin, etaSShiftMuB,
+      etaSScaleMuB, zetaSPeakEpsilon, zetaSScaleBeta, zetaSSigmaMinus, zetaSSigmaPlus;
+  EoS *eos;
+  const int etaSparam, zetaSparam;
+  double zetaSPeakT = 0.170, zetaSWidthMinus = 0.010, zetaSWidthPlus = 0.030;
+  double zetaS(double e, double T, double s, double P);
+  double etaS(double e, double rho, double T, double muB, double s, double P);
+  TransportCoeff(double _etaS0, double _zetaS0, double _ah, double _al, double _aRho,
+                 double _T0, double _etaSMin, double _etaSEpsilonMin, double _etaSShiftMuB,
+                 double _etaSScaleMuB, double _zetaSPeakEpsilon, double _zetaSScaleBeta,
+                 double _zetaSSigmaMinus, double _zetaSSigmaPlus, EoS *_eos, int _etaSparam,
+                 int _zetaSparam)
+      : etaS0(_etaS0), zetaS0(_zetaS0), ah(_ah), al(_al), aRho(_aRho), T0(_T0),
+        etaSMin(_etaSMin), etaSEpsilonMin(_etaSEpsilonMin), etaSShiftMuB(_etaSShiftMuB),
+        etaSScaleMuB(_etaSScaleMuB), zetaSPeakEpsilon(_zetaSPeakEpsilon),
+        zetaSScaleBeta(_zetaSScaleBeta), zetaSSigmaMinus(_zetaSSigmaMinus),
+        zetaSSigmaPlus(_zetaSSigmaPlus), eos(_eos), etaSparam(_etaSparam),
+        zetaSparam(_zetaSparam) {}
+  ~TransportCoeff() {}
+  void getEta(double e, double rho, double T, double muB, double s, double P, double &_etaS,
+              double &_zetaS);
+  void getTau(double e, double rho, double T, double muB, double s, double P, double &_taupi,
+              double &_tauPi);
+  void getOther(double e, double nb, double nq, double ns, double &deltapipi,
+                double &taupipi, double &lambdapiPi, double &phi7);
+  void getOtherBulk(double e, double nb, double nq, double ns, double &delPiPi,
+                    double &lamPipi);
+  bool isViscous() const {
+    return etaS0 > 0. || zetaS0 > 0. || etaSparam > 0 || zetaSparam > 0;
+  }
+  vh_trcoeff_t device() const;
+};
+
+class Fluid;
+
+// ----------------------------------------------------------------- Cell
+// Proxy for one cell of a Fluid (src/cll.h:38-267).  Storage is the Fluid's
+// host mirror of the device state; every accessor keeps the reference's
+// semantics (setQ zeroes a state with Q[T_] < 0, getpi is symmetric, ...).
+class Cell {
+ public:
+  Cell() {}
+  inline int getX(void) { return ix; }
+  inline int getY(void) { return iy; }
+  inline int getZ(void) { return iz; }
+  void setQ(double *_Q);
+  void setQh(double *_Qh);
+  void getQ(double *_Q);
+  void getQh(double *_Qh);
+  void getQprev(double *_Qp);
+  void saveQprev(void);
+  double getpi(const int &i, const int &j);
+  double getpiH(const int &i, const int &j);
+  double getPi(void);
+  double getPiH(void);
+  void setpi(const int &i, const int &j, const double &val);
+  void setpiH(const int &i, const int &j, const double &val);
+  void setPi(const double &val);
+  void setPiH(const double &val);
+  void setAllM(double value);
+  double getM(int dir);
+  double getMaxM(void);
+  void importVars(Cell *c);
+  void getPrimVar(EoS *eos, double tau, double &_e, double &_p, double &_nb, double &_nq,
+                  double &_ns, double &_vx, double &_vy, double &_vz);
+  void getPrimVarHCenter(EoS *eos, double tau, double &_e, double &_p, double &_nb,
+                         double &_nq, double &_ns, double &_vx, double &_vy, double &_vz);
+  void getPrimVarPrev(EoS *eos, double tau, double &_e, double &_p, double &_nb,
+                      double &_nq, double &_ns, double &_vx, double &_vy, double &_vz);
+  void setPrimVar(EoS *eos, double tau, double _e, double _nb, double _nq, double _ns,
+                  double _vx, double _vy, double _vz);
+  void setViscCorrCutFlag(double value);
+  double getViscCorrCutFlag(void);
+  void Dump(double tau);
+
+ private:
+  friend class Fluid;
+  Fluid *f = nullptr;
+  long n = 0;
+  int ix = 0, iy = 0, iz = 0;
+};
+
+// ---------------------------------------------------------------- Fluid
+// The hydro grid (src/fld.h:10-88, src/fld.cpp:62-117).  Owns the device
+// context; dx = (max-min)/(n-1) as in the reference (nz = 1 gives dz = inf).
+class Fluid {
+ public:
+  Fluid(EoS *_eos, EoS *_eosH, TransportCoeff *_trcoeff, int _nx, int _ny, int _nz,
+        double _minx, double _maxx, double _miny, double _maxy, double _minz, double _maxz,
+        double dt, double eCrit);
+  ~Fluid();
+  inline int getNX() { return nx; }
+  inline int getNY() { return ny; }
+  inline int getNZ() { return nz; }
+  inline double getDx() { return dx; }
+  inline double getDy() { return dy; }
+  inline double getDz() { return dz; }
+  inline double getX(int ix) { return minx + ix * dx; }
+  inline double getY(int iy) { return miny + iy * dy; }
+  inline double getZ(int iz) { return minz + iz * dz; }
+  inline double geteCrit() { return ecrit; }
+  // clamped index access (src/fld.h:70-78)
+  Cell *getCell(int ix, int iy, int iz);
+  // ghost-cell fills and vacuum-front update of the reference (run on the
+  // device; performStep calls them itself)
+  void correctImagCells(void);
+  void correctImagCellsFull(void);
+  void updateM(double tau, double dt);
+
+  // ---- GPU residency (new): the state lives in HBM between steps
+  vh_ctx_t *context() { return ctx; }
+  void syncToDevice();    // uploads the host mirror if host code changed it
+  void syncFromDevice();  // downloads the state if a step changed it
+  void markDeviceNewer() { dev_newer = true; }
+  long ncell() const { return N; }
+
+ private:
+  friend class Cell;
+  EoS *eos, *eosH;
+  TransportCoeff *trcoeff;
+  int nx, ny, nz;
+  long N;
+  double minx, maxx, miny, maxy, minz, maxz;
+  double dx, dy, dz, dt, ecrit;
+  vh_ctx_t *ctx = nullptr;
+  // host mirror, SoA component-major (the C-ABI layout)
+  std::vector<double> hQ, hQh, hQprev, hpi, hpiH, hPi, hPiH, hm, hvcc;
+  std::vector<Cell> cells;
+  bool host_dirty = false, dev_newer = false;
+  void hostRead();
+  void hostWrite();
+};
+
+// ---------------------------------------------------------------- Hydro
+// (src/hdo.h:10-58).  performStep runs one full predictor-corrector +
+// Israel-Stewart step on the GPU.
+class Hydro {
+ public:
+  Hydro(Fluid *_f, EoS *_eos, TransportCoeff *_trcoeff, double _t0, double _dt);
+  ~Hydro();
+  void setDtau(double deltaTau);
+  double getDtau() { return dt; }
+  void setFluid(Fluid *_f) { f = _f; }
+  Fluid *getFluid() const { return f; }
+  void performStep(void);
+  // nsteps calls of performStep replayed as HIP graphs (no host round trip)
+  void performSteps(int nsteps);
+  inline double getTau() const { return tau; }
+
+ private:
+  Fluid *f;
+  EoS *eos;
+  TransportCoeff *trcoeff;
+  double dt, tau;
+  void check(int rc, int status);
+};
+
+// Gubser analytic IC (src/icGubser.h:6-12, src/icGubser.cpp:17-62), with an
+// optional energy scale (1 = the reference's normalisation).
+class ICGubser {
+ public:
+  explicit ICGubser(double scale = 1.0) : scale_(scale) {}
+  void setIC(Fluid *f, EoS *eos, double tau);
+
+ private:
+  double scale_;
+};
