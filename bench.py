@@ -4,16 +4,26 @@
 Workload (one GPU): the 3-D viscous run, grid 200x200x100, dx=dy=0.1 fm,
 d(eta)=0.05, tau0=0.5 fm, dtau=0.025 fm, 300 predictor-corrector steps, smooth
 Gaussian initial energy density e0=50 GeV/fm^3 with an eta plateau, zero
-initial flow, p=e/3, eta/s=0.08, zeta/s=0 -- plain FP64 (faithful) step.
+initial flow, p=e/3, eta/s=0.08, zeta/s=0.
+
+Variants (the metric compares plain FP64 with mixed precision + faster
+con2prim):
+  fp64          faithful FP64 step (reference operation order, parity path)
+  mixed_ref     FP32 fluxes/viscous sources/pi, FP64 Q + reference con2prim
+  mixed_closed  as mixed_ref with the closed-form conformal con2prim + FP64 Newton
+Each mixed variant's final state is compared with the fp64 one of the same
+run: relative error of T^{tau mu} (max over cells, normalised by max T^{tau tau})
+and of pi^{mu nu}.
 
 "step" = one Hydro::performStep (src/hdo.cpp:1016-1134) over the whole grid,
-ghost cells included.  value = cells * steps / time of the timed region, with
-the state resident in HBM; the timed region runs the steps as replayed HIP
-graphs from the initial state at tau0.  Per-kernel durations for the roofline
-come from HIP events recorded on the context's stream around every launch
-during a second, identical pass.
+ghost cells included.  value = cells * steps / time of the timed region of the
+headline variant (the fastest one whose T^{tau mu} error is <= 1e-6), with the
+state resident in HBM; the timed region replays the step as HIP graphs from the
+initial state at tau0.  Per-kernel durations for the roofline come from HIP
+events recorded on the context's stream around every launch during a second,
+identical pass.
 
-python bench.py [--gpus 1] [--steps 300] [--warmup 5] [--no-cpu]
+python bench.py [--gpus 1] [--steps 300] [--warmup 5] [--variants a,b] [--no-cpu]
 """
 import argparse
 import json
@@ -29,18 +39,37 @@ sys.path.insert(0, ROOT)
 
 METRIC = "viscous cell-updates/s (plain FP64 vs mixed+learned c2p) + % of HBM peak"
 HBM_PEAK_GBS = 8000.0
+ERR_BAR = 1e-6
 
-# Algorithmic (compulsory) HBM bytes per cell for each kernel of the FP64
-# step: every input array read once and every output array written once, at
-# 8 bytes per value ("Kernels and their rooflines").
+# Algorithmic (compulsory) HBM bytes per cell of each kernel: every input
+# array read once and every output written once at its stored width (FP64 8 B,
+# FP32 4 B).  Derivations in DESIGN.md, "Kernels and their rooflines".
 ALG_BYTES = {
-    "faces_x_pred": 7 * 8 + 7 * 8, "faces_y_pred": 7 * 8 + 7 * 8, "faces_z_pred": 7 * 8 + 7 * 8,
-    "faces_x_corr": 7 * 8 + 7 * 8, "faces_y_corr": 7 * 8 + 7 * 8, "faces_z_corr": 7 * 8 + 7 * 8,
-    "update_pred": (7 + 21 + 7) * 8, "update_corr": (7 + 7 + 21 + 7) * 8,
-    "visc_prim": 7 * 8 + 12 * 8,
-    "visc_relax": (8 + 4 + 7 + 7 + 10 + 1) * 8 + 22 * 8,
-    "visc_advect": (5 + 22 + 11) * 8 + (22 + 1 + 4) * 8,
-    "visc_flux_update": (3 + 11 + 4 + 7) * 8 + 7 * 8,
+    "fp64": {
+        "faces_x_pred": 112, "faces_y_pred": 112, "faces_z_pred": 112,
+        "faces_x_corr": 112, "faces_y_corr": 112, "faces_z_corr": 112,
+        "update_pred": (7 + 21 + 7) * 8, "update_corr": (7 + 7 + 21 + 7) * 8,
+        "visc_prim": 7 * 8 + 12 * 8,
+        "visc_relax": (8 + 4 + 7 + 7 + 10 + 1) * 8 + 22 * 8,
+        "visc_advect": (5 + 22 + 11) * 8 + (22 + 1 + 4) * 8,
+        "visc_flux_update": (3 + 11 + 4 + 7) * 8 + 7 * 8,
+    },
+    "mixed": {
+        "stage_pred": 56 + 56, "stage_corr": 56 + 56 + 56,
+        "visc_prim": 56 + 96,
+        "visc_relax": 64 + 32 + 56 + 56 + 44 + 88,
+        "visc_advect": 40 + 88 + 20 + 80 + 8 + 8 + 16,
+        "visc_flux_update": 32 + 44 + 16 + 56 + 56,
+    },
+}
+# whole-step compulsory state traffic per cell-update:
+# 2*(7*s_Q + 11*s_pi) -- FP64 288 B, mixed (pi in FP32) 200 B
+STEP_BYTES = {"fp64": 288.0, "mixed": 200.0}
+
+VARIANTS = {
+    "fp64": ("fp64", "reference"),
+    "mixed_ref": ("mixed", "reference"),
+    "mixed_closed": ("mixed", "closed"),
 }
 
 
@@ -55,59 +84,50 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def run_gpu(args, cfg):
-    from vhlle_amd import abi, setups
+def run_variant(args, cfg, prec, c2p, ic):
+    from vhlle_amd import abi
     nx, ny, nz = cfg["nx"], cfg["ny"], cfg["nz"]
-    N = nx * ny * nz
     trc = abi.make_trc(etaS=cfg["etaS"])
-    ctx = abi.Context(nx, ny, nz, cfg["xlim"], cfg["ylim"], cfg["zlim"], trc=trc)
-    e, vx, vy, vz = setups.gaussian3d_ic(nx, ny, nz, cfg["xlim"], cfg["ylim"], cfg["zlim"],
-                                         e0=cfg["e0"], sx=cfg["sx"], sy=cfg["sy"],
-                                         eta_plateau=cfg["eta_plateau"],
-                                         eta_sigma=cfg["eta_sigma"])
+    p = abi.PREC_FP64 if prec == "fp64" else abi.PREC_MIXED
+    ctx = abi.Context(nx, ny, nz, cfg["xlim"], cfg["ylim"], cfg["zlim"], trc=trc, precision=p)
+    cm = {"reference": abi.C2P_REFERENCE, "closed": abi.C2P_CLOSED,
+          "learned": abi.C2P_LEARNED}[c2p]
+    if cm != abi.C2P_REFERENCE:
+        abi.check(abi.lib().vh_set_c2p_mode(ctx.h, cm), "vh_set_c2p_mode")
     dt, tau0 = cfg["dt"], cfg["tau0"]
-    print(f"[bench] device: {abi.device_info()}", file=sys.stderr, flush=True)
-    # warmup (also builds the step graphs)
-    ctx.set_prim(tau0, e, vx, vy, vz)
+    ctx.set_prim(tau0, *ic)
     if args.warmup:
-        st, _ = ctx.run(tau0, dt, args.warmup)
-    # timed region: K steps from the initial state, resident in HBM
-    ctx.set_prim(tau0, e, vx, vy, vz)
+        ctx.run(tau0, dt, args.warmup)  # also captures the step graphs
+    ctx.set_prim(tau0, *ic)
     t0 = time.perf_counter()
     st, tau = ctx.run(tau0, dt, args.steps)  # synchronises before returning
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    print(f"[bench] timed {args.steps} steps: {elapsed:.3f} s, status {st}, tau {tau:.4f}",
-          file=sys.stderr, flush=True)
-    # physical sanity of the final state
-    s = ctx.download()
-    tot = float(np.sum(s["Q"][0]))
-    # per-kernel pass with HIP events on the context stream
-    ctx.set_prim(tau0, e, vx, vy, vz)
+    elapsed = time.perf_counter() - t0
+    state = ctx.download()
+    ctx.set_prim(tau0, *ic)
     ctx.enable_timing(True)
     st2, _ = ctx.run(tau0, dt, args.steps)
     kt = ctx.kernel_times()
     ctx.enable_timing(False)
     ctx.close()
-    return dict(N=N, elapsed=elapsed, status=st | st2, kernel_times=kt, Qt_sum=tot)
+    print(f"[bench] {prec}/{c2p}: {args.steps} steps in {elapsed:.3f} s, status {st | st2}",
+          file=sys.stderr, flush=True)
+    return dict(elapsed=elapsed, status=st | st2, kernel_times=kt, state=state, tau=tau)
 
 
-def roofline(kt, N):
-    best = max(kt.items(), key=lambda kv: kv[1][1])
-    name, (launches, ms) = best
+def roofline(kt, N, prec):
+    name, (launches, ms) = max(kt.items(), key=lambda kv: kv[1][1])
     avg_s = ms / 1e3 / launches
-    alg = ALG_BYTES.get(name, 0) * N
+    alg = ALG_BYTES[prec].get(name, 0) * N
     achieved = alg / avg_s / 1e9
-    return name, dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                      frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None,
-                      kernel=name, avg_ms=round(avg_s * 1e3, 4),
-                      alg_bytes_per_launch=alg)
+    return dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None, kernel=name,
+                avg_ms=round(avg_s * 1e3, 4), alg_bytes_per_launch=alg)
 
 
 def cpu_baseline(cfg):
     """The oracle (single-threaded CPU restatement) on a bounded sample of the
     same workload: a 200x200x10 slab of the grid (same spacings, centre of the
-    eta plateau) for 5 steps (~20 s on one core)."""
+    eta plateau) for 5 steps (~10-20 s on one core)."""
     from oracle import pyoracle
     from vhlle_amd import setups
     nz = 10
@@ -123,7 +143,7 @@ def cpu_baseline(cfg):
     s.set_prim(cfg["tau0"], e, vx, vy, vz)
     steps = 5
     t0 = time.perf_counter()
-    rc, _ = s.run(cfg["tau0"], cfg["dt"], steps)
+    s.run(cfg["tau0"], cfg["dt"], steps)
     dt = time.perf_counter() - t0
     return dict(value=round(nx * ny * nz * steps / dt, 1), unit="cell-updates/s", cores=1,
                 kind="port", sample=f"oracle, {nx}x{ny}x{nz} slab of the workload grid, "
@@ -135,45 +155,69 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--variants", default="fp64,mixed_ref,mixed_closed")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding of the hydro "
               "grid is designed (DESIGN.md) but not built.", file=sys.stderr)
         sys.exit(2)
-    from vhlle_amd import setups
+    from vhlle_amd import abi, setups
     cfg = setups.VISCOUS_3D
-    r = run_gpu(args, cfg)
-    N = r["N"]
-    value = N * args.steps / r["elapsed"]
-    kname, roof = roofline(r["kernel_times"], N)
-    # whole-step % of HBM peak with the compulsory 288 B per viscous cell-update
-    step_frac = 288.0 * value / (HBM_PEAK_GBS * 1e9)
+    N = cfg["nx"] * cfg["ny"] * cfg["nz"]
+    ic = setups.gaussian3d_ic(cfg["nx"], cfg["ny"], cfg["nz"], cfg["xlim"], cfg["ylim"],
+                              cfg["zlim"], e0=cfg["e0"], sx=cfg["sx"], sy=cfg["sy"],
+                              eta_plateau=cfg["eta_plateau"], eta_sigma=cfg["eta_sigma"])
+    print(f"[bench] device: {abi.device_info()}", file=sys.stderr, flush=True)
+    names = [v for v in args.variants.split(",") if v]
+    if "fp64" not in names:
+        names.insert(0, "fp64")
+    results = {}
+    for name in names:
+        prec, c2p = VARIANTS[name]
+        results[name] = run_variant(args, cfg, prec, c2p, ic)
+    ref = results["fp64"]["state"]
+    qs = np.max(np.abs(ref["Q"][0]))
+    ps = np.max(np.abs(ref["pi"]))
+    variants = {}
+    for name, r in results.items():
+        prec, c2p = VARIANTS[name]
+        v = N * args.steps / r["elapsed"]
+        eT = float(np.max(np.abs(r["state"]["Q"][:4] - ref["Q"][:4])) / qs)
+        epi = float(np.max(np.abs(r["state"]["pi"] - ref["pi"])) / ps)
+        variants[name] = dict(
+            value=round(v, 1), ms_per_step=round(r["elapsed"] / args.steps * 1e3, 4),
+            precision=prec, c2p=c2p, status=r["status"],
+            err_T_vs_fp64=eT, err_pi_vs_fp64=epi,
+            step_hbm_frac=round(STEP_BYTES[prec] * v / (HBM_PEAK_GBS * 1e9), 5),
+            speedup_vs_fp64=round(v / (N * args.steps / results["fp64"]["elapsed"]), 3),
+            roofline=roofline(r["kernel_times"], N, prec),
+            kernels_ms={k: round(t[1], 3) for k, t in sorted(r["kernel_times"].items())})
+    ok = [n for n in variants if variants[n]["err_T_vs_fp64"] <= ERR_BAR and
+          variants[n]["status"] == 0]
+    head = max(ok, key=lambda n: variants[n]["value"]) if ok else "fp64"
+    H = variants[head]
     out = {
         "metric": METRIC,
-        "value": round(value, 1),
+        "value": H["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(r["elapsed"] / args.steps * 1e3, 4),
+        "ms_per_step": H["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if H["precision"] == "fp64" else "f64+f32",
         "data": "synthetic",
         "config": {"workload": f"{cfg['name']}: 3-D viscous, dx=dy=0.1, deta=0.05, tau0=0.5, "
-                               f"dtau=0.025, eta/s=0.08, p=e/3, plain FP64 step",
-                   "cells": N, "precision": "fp64", "c2p": "reference"},
-        "step_hbm_frac_288B": round(step_frac, 5),
-        "status": r["status"],
-        "roofline": roof,
-        "kernels_ms": {k: round(v[1], 3) for k, v in sorted(r["kernel_times"].items())},
+                               f"dtau=0.025, eta/s=0.08, p=e/3",
+                   "cells": N, "variant": head, "precision": H["precision"], "c2p": H["c2p"]},
+        "hbm_frac_step": H["step_hbm_frac"],
+        "roofline": H["roofline"],
+        "variants": variants,
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cfg)
-    else:
-        out["cpu_baseline"] = None
+    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(cfg)
     print(json.dumps(out), flush=True)
 
 

@@ -1,0 +1,89 @@
+"""Mixed precision (FP32 fluxes / viscous sources / pi storage, FP64 Q and
+con2prim) against the FP64 faithful path, through the C-ABI.
+
+Error measures (BASELINE.json target: <= 1e-6 relative error in T^{mu nu}):
+  T^{tau mu}:  max_cells |Q_mixed - Q_fp64| / max |Q^tau_fp64|
+  pi, Pi:      max |a - b| / max |b|
+"""
+import numpy as np
+import pytest
+
+from vhlle_amd import abi, setups
+
+pytestmark = pytest.mark.gpu
+
+
+def run(prec, c2p, nx, ny, nz, xl, yl, zl, ic, tau0, dt, steps, etaS):
+    ctx = abi.Context(nx, ny, nz, xl, yl, zl, trc=abi.make_trc(etaS=etaS), precision=prec)
+    if c2p != abi.C2P_REFERENCE:
+        abi.check(abi.lib().vh_set_c2p_mode(ctx.h, c2p), "vh_set_c2p_mode")
+    e, vx, vy, vz = ic
+    ctx.set_prim(tau0, e, vx, vy, vz)
+    st, tau = ctx.run(tau0, dt, steps)
+    assert st == 0
+    s = ctx.download()
+    ctx.close()
+    return s
+
+
+def errors(a, b):
+    qs = np.max(np.abs(b["Q"][0]))
+    eT = np.max(np.abs(a["Q"][:4] - b["Q"][:4])) / qs
+    ps = max(np.max(np.abs(b["pi"])), 1e-300)
+    epi = np.max(np.abs(a["pi"] - b["pi"])) / ps
+    return eT, epi
+
+
+@pytest.mark.parametrize("c2p", [abi.C2P_REFERENCE, abi.C2P_CLOSED])
+def test_mixed_viscous_error_budget(gpu, c2p):
+    """3-D viscous workload of the benchmark at 1/4 resolution per axis
+    (50x50x25, same physical box), 100 steps."""
+    cfg = setups.VISCOUS_3D
+    nx, ny, nz = 50, 50, 25
+    ic = setups.gaussian3d_ic(nx, ny, nz, cfg["xlim"], cfg["ylim"], cfg["zlim"], e0=cfg["e0"],
+                              sx=cfg["sx"], sy=cfg["sy"], eta_plateau=cfg["eta_plateau"],
+                              eta_sigma=cfg["eta_sigma"])
+    args = (nx, ny, nz, cfg["xlim"], cfg["ylim"], cfg["zlim"], ic, cfg["tau0"], 0.05, 100,
+            cfg["etaS"])
+    ref = run(abi.PREC_FP64, abi.C2P_REFERENCE, *args)
+    mix = run(abi.PREC_MIXED, c2p, *args)
+    eT, epi = errors(mix, ref)
+    print(f"c2p={c2p}: T^tau-mu rel err {eT:.3e}, pi rel err {epi:.3e}", flush=True)
+    assert eT < 1e-5 and epi < 1e-3
+
+
+def test_mixed_gubser_vs_fp64(gpu):
+    cfg = setups.GUBSER_2D
+    n = 101
+    e, vx, vy, vz = setups.gubser_ic(cfg, nx=n)
+    args = (n, n, 1, cfg["xlim"], cfg["ylim"], cfg["zlim"], (e, vx, vy, vz), 1.0, 0.025, 200, 0.0)
+    ref = run(abi.PREC_FP64, abi.C2P_REFERENCE, *args)
+    mix = run(abi.PREC_MIXED, abi.C2P_CLOSED, *args)
+    eT, _ = errors(mix, ref)
+    print(f"gubser mixed+closed vs fp64: {eT:.3e}", flush=True)
+    assert eT < 1e-5
+
+
+def test_mixed_graph_equals_steps(gpu):
+    nx, ny, nz = 12, 10, 6
+    xl, yl, zl = (-2.75, 2.75), (-2.25, 2.25), (-1.0, 1.0)
+    ic = setups.gaussian3d_ic(nx, ny, nz, xl, yl, zl, e0=20.0, sx=1.0, sy=1.2,
+                              eta_plateau=0.5, eta_sigma=0.5)
+    a = abi.Context(nx, ny, nz, xl, yl, zl, trc=abi.make_trc(etaS=0.1), precision=abi.PREC_MIXED)
+    b = abi.Context(nx, ny, nz, xl, yl, zl, trc=abi.make_trc(etaS=0.1), precision=abi.PREC_MIXED)
+    for c in (a, b):
+        abi.check(abi.lib().vh_set_c2p_mode(c.h, abi.C2P_CLOSED))
+        c.set_prim(0.6, *ic)
+    ta = tb = 0.6
+    for rep in range(3):
+        _, ta = a.run(ta, 0.02, 4)
+        for _ in range(4):
+            _, tb = b.step(tb, 0.02)
+        da, db = a.download(), b.download()
+        for f in ("Q", "pi", "Pi"):
+            assert np.array_equal(da[f], db[f]), (rep, f)
+
+
+def test_fp64_path_refuses_non_reference_c2p(gpu):
+    ctx = abi.Context(8, 8, 1, (-1, 1), (-1, 1), (-1, 1))
+    assert abi.lib().vh_set_c2p_mode(ctx.h, abi.C2P_CLOSED) == abi.VH_ERR_ARG

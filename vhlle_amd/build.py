@@ -26,6 +26,10 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contr
              "-Wno-unused-command-line-argument"]
 
 
+# kernels that are not bound to the reference's rounding: FMA contraction allowed
+FAST_SOURCES = {"vh_kernels_mixed.hip", "vh_mlp.hip", "vh_batch.hip", "vh_freezeout.hip"}
+
+
 def _run(cmd, cwd=None):
     r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l
@@ -54,7 +58,10 @@ def build_lib(verbose=False):
     for src in hip_sources() + host_sources():
         obj = os.path.join(objdir, os.path.basename(src) + ".o")
         if src.endswith(".hip"):
-            cmd = [HIPCC] + HIP_FLAGS + ["-I", ROOT, "-c", src, "-o", obj]
+            flags = list(HIP_FLAGS)
+            if os.path.basename(src) in FAST_SOURCES:  # not bound to the reference's rounding
+                flags.remove("-ffp-contract=off")
+            cmd = [HIPCC] + flags + ["-I", ROOT, "-c", src, "-o", obj]
         else:  # host-only C++ (the facade): no HIP headers needed
             cmd = ["g++", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
                    "-I", ROOT, "-c", src, "-o", obj]

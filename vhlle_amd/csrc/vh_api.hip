@@ -12,6 +12,7 @@
 
 #include "../../include/vhlle_gpu.h"
 #include "vh_kernels.h"
+#include "vh_kernels_mixed.h"
 
 #define VH_HIP(call)                                                         \
   do {                                                                       \
@@ -117,7 +118,55 @@ const char *face_name(int D, int mode) {
 
 // Enqueues one Hydro::performStep on c->st (src/hdo.cpp:1016-1134).  Reads
 // tau from c->tp, advances it on the device.  Q[cur] -> Q[cur^1].
-int enqueue_step(vh_ctx *c, double dt) {
+// FP32 views of the pi-like arrays in mixed precision (same allocation)
+inline float *fp(double *a) { return reinterpret_cast<float *>(a); }
+
+// mixed precision: FP64 Q/Qh/con2prim, FP32 fluxes, viscous sources and pi
+int enqueue_step_mixed(vh_ctx *c, double dt) {
+  const vh::Geom &g = c->g;
+  hipStream_t st = c->st;
+  double *Qo = c->Q[c->cur], *Qn = c->Q[c->cur ^ 1];
+  const int cm = c->c2p_mode;
+  int rc = VH_OK;
+  if (!c->m_full) {
+    rc |= timed(c, "update_m_dm", [&] { vh::launch_update_m(0, g, c->m, c->dm, c->tp, dt, st); });
+    rc |= timed(c, "update_m_apply", [&] { vh::launch_update_m(1, g, c->m, c->dm, c->tp, dt, st); });
+  }
+  rc |= timed(c, "stage_pred", [&] {
+    vh::mx::launch_stage(0, cm, g, c->eos, Qo, c->Qh, c->m, c->m_full, c->tp, dt, c->Qh,
+                         c->status, st);
+  });
+  rc |= timed(c, "stage_corr", [&] {
+    vh::mx::launch_stage(1, cm, g, c->eos, Qo, c->Qh, c->m, c->m_full, c->tp, dt, Qn, c->status,
+                         st);
+  });
+  rc |= timed(c, "advance_tau", [&] { vh::launch_advance_tau(c->tp, dt, st); });
+  for (int ax = 3; ax >= 1; ax--)
+    rc |= timed(c, "ghost_ideal", [&] { vh::launch_ghost(0, ax, g, Qn, c->Qh, c->pi, c->Pi, st); });
+  if (c->viscous) {
+    rc |= timed(c, "visc_prim", [&] { vh::mx::launch_vprim(cm, g, c->eos, c->Qh, c->tp, dt, c->ph, c->pt, st); });
+    rc |= timed(c, "visc_relax", [&] {
+      vh::mx::launch_visc1(cm, g, c->eos, c->trc, Qo, Qn, c->ph, c->pt, c->m, c->m_full,
+                           fp(c->pi), fp(c->Pi), c->tp, dt, fp(c->pi0), fp(c->piH0), fp(c->Pi0),
+                           fp(c->PiH0), c->status, st);
+    });
+    rc |= timed(c, "visc_advect", [&] {
+      vh::mx::launch_visc2(g, c->ph, fp(c->pi0), fp(c->piH0), fp(c->Pi0), fp(c->PiH0), fp(c->pi),
+                           fp(c->piH), fp(c->Pi), fp(c->PiH), c->vcc, fp(c->vf), c->tp, dt, st);
+    });
+    rc |= timed(c, "visc_flux_update", [&] {
+      vh::mx::launch_visc3(g, c->ph, fp(c->piH), fp(c->PiH), fp(c->vf), c->m, c->m_full, c->tp,
+                           dt, Qn, st);
+    });
+  }
+  for (int ax = 3; ax >= 1; ax--)
+    rc |= timed(c, "ghost_full", [&] { vh::mx::launch_ghost_full(ax, g, Qn, fp(c->pi), fp(c->Pi), st); });
+  VH_HIP(hipGetLastError());
+  c->cur ^= 1;
+  return rc ? VH_ERR_HIP : VH_OK;
+}
+
+int enqueue_step_fp64(vh_ctx *c, double dt) {
   const vh::Geom &g = c->g;
   hipStream_t st = c->st;
   double *Qo = c->Q[c->cur], *Qn = c->Q[c->cur ^ 1];
@@ -168,6 +217,10 @@ int enqueue_step(vh_ctx *c, double dt) {
   VH_HIP(hipGetLastError());
   c->cur ^= 1;
   return rc ? VH_ERR_HIP : VH_OK;
+}
+
+int enqueue_step(vh_ctx *c, double dt) {
+  return c->precision == VH_PREC_MIXED ? enqueue_step_mixed(c, dt) : enqueue_step_fp64(c, dt);
 }
 
 void drop_graphs(vh_ctx *c) {
@@ -229,7 +282,7 @@ int vh_create(const vh_grid_t *grid, const vh_eos_t *eos, const vh_trcoeff_t *tr
               int precision, vh_ctx_t **out) {
   if (!grid || !eos || !trc || !out) return VH_ERR_ARG;
   if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1 || eos->kind != VH_EOS_IDEAL) return VH_ERR_ARG;
-  if (precision != VH_PREC_FP64) return VH_ERR_ARG;
+  if (precision != VH_PREC_FP64 && precision != VH_PREC_MIXED) return VH_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return VH_ERR_NODEVICE;
   vh_ctx *c = new vh_ctx;
@@ -348,10 +401,25 @@ int vh_upload(vh_ctx_t *c, const vh_state_t *h) {
   rc |= put(c->Q[c->cur], h->Q, 7, 0.);
   rc |= put(c->Q[c->cur ^ 1], h->Qprev, 7, 0.);
   rc |= put(c->Qh, h->Qh, 7, 0.);
-  rc |= put(c->pi, h->pi, 10, 0.);
-  rc |= put(c->piH, h->piH, 10, 0.);
-  rc |= put(c->Pi, h->Pi, 1, 0.);
-  rc |= put(c->PiH, h->PiH, 1, 0.);
+  if (c->precision == VH_PREC_MIXED) {  // pi, Pi stored in FP32
+    auto putf = [&](double *dst, const double *src, int nc) -> int {
+      std::vector<float> f((size_t)nc * N, 0.f);
+      if (src)
+        for (size_t i = 0; i < f.size(); i++) f[i] = (float)src[i];
+      VH_HIP(hipMemcpyAsync(dst, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, c->st));
+      VH_HIP(hipStreamSynchronize(c->st));
+      return VH_OK;
+    };
+    rc |= putf(c->pi, h->pi, 10);
+    rc |= putf(c->piH, h->piH, 10);
+    rc |= putf(c->Pi, h->Pi, 1);
+    rc |= putf(c->PiH, h->PiH, 1);
+  } else {
+    rc |= put(c->pi, h->pi, 10, 0.);
+    rc |= put(c->piH, h->piH, 10, 0.);
+    rc |= put(c->Pi, h->Pi, 1, 0.);
+    rc |= put(c->PiH, h->PiH, 1, 0.);
+  }
   rc |= put(c->m, h->m, 3, 0.);
   rc |= put(c->vcc, h->viscCorrCut, 1, 1.);
   VH_HIP(hipStreamSynchronize(c->st));
@@ -371,10 +439,25 @@ int vh_download(vh_ctx_t *c, vh_state_t *h) {
   rc |= get(h->Q, c->Q[c->cur], 7);
   rc |= get(h->Qprev, c->Q[c->cur ^ 1], 7);
   rc |= get(h->Qh, c->Qh, 7);
-  rc |= get(h->pi, c->pi, 10);
-  rc |= get(h->piH, c->piH, 10);
-  rc |= get(h->Pi, c->Pi, 1);
-  rc |= get(h->PiH, c->PiH, 1);
+  if (c->precision == VH_PREC_MIXED) {
+    auto getf = [&](double *dst, const double *src, int nc) -> int {
+      if (!dst) return VH_OK;
+      std::vector<float> f((size_t)nc * N);
+      VH_HIP(hipMemcpyAsync(f.data(), src, f.size() * sizeof(float), hipMemcpyDeviceToHost, c->st));
+      VH_HIP(hipStreamSynchronize(c->st));
+      for (size_t i = 0; i < f.size(); i++) dst[i] = (double)f[i];
+      return VH_OK;
+    };
+    rc |= getf(h->pi, c->pi, 10);
+    rc |= getf(h->piH, c->piH, 10);
+    rc |= getf(h->Pi, c->Pi, 1);
+    rc |= getf(h->PiH, c->PiH, 1);
+  } else {
+    rc |= get(h->pi, c->pi, 10);
+    rc |= get(h->piH, c->piH, 10);
+    rc |= get(h->Pi, c->Pi, 1);
+    rc |= get(h->PiH, c->PiH, 1);
+  }
   rc |= get(h->m, c->m, 3);
   rc |= get(h->viscCorrCut, c->vcc, 1);
   VH_HIP(hipStreamSynchronize(c->st));
@@ -411,7 +494,10 @@ int vh_check_dtau(const vh_ctx_t *c, double dt) {
 }
 
 int vh_set_c2p_mode(vh_ctx_t *c, int mode) {
-  if (!c || mode != VH_C2P_REFERENCE) return VH_ERR_ARG;
+  if (!c) return VH_ERR_ARG;
+  // the FP64 path is the faithful one: reference iteration only
+  if (c->precision == VH_PREC_FP64 && mode != VH_C2P_REFERENCE) return VH_ERR_ARG;
+  if (mode != VH_C2P_REFERENCE && mode != VH_C2P_CLOSED) return VH_ERR_ARG;
   c->c2p_mode = mode;
   drop_graphs(c);
   return VH_OK;

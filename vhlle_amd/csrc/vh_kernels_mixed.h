@@ -1,0 +1,30 @@
+// Launch interface of the mixed-precision kernels (vh_kernels_mixed.hip):
+// FP64 conserved variables and con2prim, FP32 interface fluxes, viscous
+// sources and pi storage.
+#pragma once
+#include "vh_device.h"
+
+namespace vh {
+namespace mx {
+
+void launch_stage(int mode, int cm, const Geom &g, const DevEos &eos, const double *Q,
+                  const double *Qh, const double *m, int m_full, const double *tp, double dt,
+                  double *out, int *status, hipStream_t st);
+void launch_vprim(int cm, const Geom &g, const DevEos &eos, const double *Qh, const double *tp,
+                  double dt, double *ph, double *pt, hipStream_t st);
+void launch_visc1(int cm, const Geom &g, const DevEos &eos, const DevTrc &trc,
+                  const double *Qprev, const double *Qn, const double *ph, const double *pt,
+                  const double *m, int m_full, const float *pi, const float *Pi,
+                  const double *tp, double dt, float *pi0, float *piH0, float *Pi0,
+                  float *PiH0, int *status, hipStream_t st);
+void launch_visc2(const Geom &g, const double *ph, const float *pi0, const float *piH0,
+                  const float *Pi0, const float *PiH0, float *pi, float *piH, float *Pi,
+                  float *PiH, double *vcc, float *vf, const double *tp, double dt,
+                  hipStream_t st);
+void launch_visc3(const Geom &g, const double *ph, const float *piH, const float *PiH,
+                  const float *vf, const double *m, int m_full, const double *tp, double dt,
+                  double *Q, hipStream_t st);
+void launch_ghost_full(int axis, const Geom &g, double *Q, float *pi, float *Pi, hipStream_t st);
+
+}  // namespace mx
+}  // namespace vh

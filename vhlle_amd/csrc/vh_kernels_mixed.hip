@@ -1,0 +1,850 @@
+// Mixed-precision kernels of the hydro step on gfx950.
+//
+// Precision split: the conserved variables Q and Qh, their accumulation, the
+// MUSCL reconstruction, con2prim and the velocity gradients of NSquant stay
+// FP64.  The HLLE interface fluxes, the viscous interface fluxes and sources,
+// and the Israel-Stewart relaxation with its cross terms run in FP32.  pi^{mu nu}
+// and Pi are stored in FP32.
+//
+// Structure: one fused kernel per predictor/corrector stage.  Each thread owns
+// one cell and evaluates its six interfaces itself; the two con2prim of an
+// interface are recomputed by both neighbours.  Nothing is written to HBM but
+// the stage result, so a stage moves about its compulsory bytes (read Q, and
+// Qh in the corrector, write Qh or Q).  The con2prim algorithm is a template
+// parameter: the reference iteration (C2P_REF), or the closed-form conformal
+// root followed by one FP64 Newton step (C2P_CLOSED).  The learned initial
+// guess (C2P_LEARNED) is taken from a per-state MLP (vh_mlp.h).
+#include "vh_kernels_mixed.h"
+
+namespace vh {
+namespace mx {
+
+// ------------------------------------------------------------- helpers
+__device__ __forceinline__ void decompose(const Geom &g, long c, int &ix, int &iy, int &iz) {
+  ix = (int)(c % g.nx);
+  const long r = c / g.nx;
+  iy = (int)(r % g.ny);
+  iz = (int)(r / g.ny);
+}
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fmin_(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float fmax_(float a, float b) { return (a < b) ? b : a; }
+
+struct PrimF {
+  float e, p, nb, nq, ns, vx, vy, vz;
+};
+
+__device__ __forceinline__ PrimF to_f(const Prim &a) {
+  return PrimF{(float)a.e, (float)a.p, (float)a.nb, (float)a.nq,
+               (float)a.ns, (float)a.vx, (float)a.vy, (float)a.vz};
+}
+
+// Closed-form con2prim of the conformal EoS p = e/3: the root of
+// M v^2 - 4 Q_t v + 3 M = 0 in its cancellation-free form, then one FP64
+// Newton step on f(v) = (Q_t + p) v - M with f' = Q_t + p - M v/3.  Same
+// branches as transformPV (src/rmn.cpp:47-70): vacuum, at rest, rescale of
+// |Q_vec| > Q_t to 0.9999 Q_t.
+__device__ __forceinline__ void c2p_closed(double q[7], Prim &r) {
+  const double Qt = q[T_];
+  double Qx = q[X_], Qy = q[Y_], Qz = q[Z_];
+  double M = sqrt(Qx * Qx + Qy * Qy + Qz * Qz);
+  double e = 0., p = 0., vx = 0., vy = 0., vz = 0., nb = 0., nq = 0., ns = 0.;
+  if (Qt > 0.) {
+    if (M == 0.) {
+      e = Qt;
+      nb = q[NB_];
+      nq = q[NQ_];
+      ns = q[NS_];
+      p = e * (1. / 3.);
+    } else {
+      if (M > Qt) {
+        const double s = 0.9999 * Qt / M;
+        Qx *= s;
+        Qy *= s;
+        Qz *= s;
+        M = Qt * 0.9999;
+      }
+      double v = 3. * M / (2. * Qt + sqrt(4. * Qt * Qt - 3. * M * M));
+      const double pp = (Qt - M * v) * (1. / 3.);
+      v -= ((Qt + pp) * v - M) / ((Qt + pp) - M * v * (1. / 3.));
+      e = Qt - M * v;
+      p = e * (1. / 3.);
+      q[X_] = Qx;  // the rescaled input, as transformPV leaves it
+      q[Y_] = Qy;
+      q[Z_] = Qz;
+      const double iM = v / M;
+      vx = Qx * iM;
+      vy = Qy * iM;
+      vz = Qz * iM;
+      const double g = sqrt(1. - v * v);
+      nb = q[NB_] * g;
+      nq = q[NQ_] * g;
+      ns = q[NS_] * g;
+    }
+  }
+  r = Prim{e, p, nb, nq, ns, vx, vy, vz};
+}
+
+template <int CM>
+__device__ __forceinline__ void c2p_mode(const DevEos &eos, double q[7], Prim &r) {
+  if (CM == VH_C2P_CLOSED)
+    c2p_closed(q, r);
+  else
+    c2p_ref(eos, q, r);
+}
+
+// MUSCL face state (src/cll.cpp:106-204) in FP64, con2prim by mode.
+template <int SIDE, int CM>
+__device__ __forceinline__ void face_state(const DevEos &eos, const double *__restrict__ Qs,
+                                           long N, long cm, long c0, long cp, double itau,
+                                           Prim &r) {
+  double q[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    const double a = Qs[k * N + c0];
+    const double dq = minmod((Qs[k * N + cp] - a) * 0.5, (a - Qs[k * N + cm]) * 0.5);
+    q[k] = (SIDE < 0 ? a - dq : a + dq) * itau;
+  }
+  c2p_mode<CM>(eos, q, r);
+}
+
+// HLLE interface flux in FP32 (Hydro::hlle_flux, src/hdo.cpp:77-305: same
+// wave speeds, vacuum rules and skips).  Returns false if the reference would
+// return without a flux.  pre = tauFactor * dta / dx.
+template <int D>
+__device__ __forceinline__ bool hlle_f32(PrimF L, PrimF R, float itau, float itf, float pre,
+                                         bool mskip, float F[7], int *status) {
+  if (L.e < 0.f) L.e = L.p = 0.f;
+  if (R.e < 0.f) R.e = R.p = 0.f;
+  if (L.e > 1e10f) {
+    atomicOr(status, VH_ST_BIG_E);
+    return false;
+  }
+  if (L.e == 0.f && R.e == 0.f) return false;
+  if (mskip) return false;
+  const float v2l = L.vx * L.vx + L.vy * L.vy + L.vz * L.vz;
+  const float v2r = R.vx * R.vx + R.vy * R.vy + R.vz * R.vz;
+  const float g2l = frcp(1.f - v2l), g2r = frcp(1.f - v2r);
+  const float wl = (L.e + L.p) * g2l, wr = (R.e + R.p) * g2r;  // El, Er of the reference
+  const float gl = fsqrt(g2l), gr = fsqrt(g2r);
+  float Ul[7], Ur[7];
+  Ul[X_] = wl * L.vx;
+  Ul[Y_] = wl * L.vy;
+  Ul[Z_] = wl * L.vz;
+  Ul[T_] = wl - L.p;
+  Ul[NB_] = gl * L.nb;
+  Ul[NQ_] = gl * L.nq;
+  Ul[NS_] = gl * L.ns;
+  Ur[X_] = wr * R.vx;
+  Ur[Y_] = wr * R.vy;
+  Ur[Z_] = wr * R.vz;
+  Ur[T_] = wr - R.p;
+  Ur[NB_] = gr * R.nb;
+  Ur[NQ_] = gr * R.nq;
+  Ur[NS_] = gr * R.ns;
+  const float vl = D == 1 ? L.vx : (D == 2 ? L.vy : L.vz);
+  const float vr = D == 1 ? R.vx : (D == 2 ? R.vy : R.vz);
+  float Fl[7], Fr[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    Fl[k] = Ul[k] * vl;
+    Fr[k] = Ur[k] * vr;
+  }
+  Fl[T_] += L.p * vl;
+  Fr[T_] += R.p * vr;
+  Fl[D] += L.p;
+  Fr[D] += R.p;
+  if (D == 3) {
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+      Fl[k] *= itf;
+      Fr[k] *= itf;
+    }
+  }
+  const float cs = 0.57735026918962576f;
+  const float sEl = fsqrt(wl), sEr = fsqrt(wr);
+  const float isum = frcp(sEl + sEr);
+  const float dv = vl - vr;
+  const float csb = fsqrt(1.f / 3.f + 0.5f * sEl * sEr * isum * isum * dv * dv);
+  const float vb = (sEl * vl + sEr * vr) * isum;
+  float bl = fmin_(0.f, fmin_((vb - csb) * frcp(1.f - vb * csb), (vl - cs) * frcp(1.f - vl * cs)));
+  float br = fmax_(0.f, fmax_((vb + csb) * frcp(1.f + vb * csb), (vr + cs) * frcp(1.f + vr * cs)));
+  if (D == 3) {
+    bl *= itau;
+    br *= itau;
+    if (L.e == 0.f) bl = -itau;
+    if (R.e == 0.f) br = itau;
+  } else {
+    if (L.e == 0.f) bl = -1.f;
+    if (R.e == 0.f) br = 1.f;
+  }
+  if (bl == 0.f && br == 0.f) return false;
+  const float s = pre * frcp(br - bl);
+  const float bb = bl * br;
+#pragma unroll
+  for (int k = 0; k < 7; k++) F[k] = s * (br * Fl[k] - bl * Fr[k] - bb * (Ul[k] - Ur[k]));
+  if (F[NB_] != F[NB_]) {
+    atomicOr(status, VH_ST_NAN_FLUX);
+    return false;
+  }
+  return true;
+}
+
+// the two interfaces of cell c along axis D, added into acc in the order
+// +F(i-1/2), -F(i+1/2)
+template <int D, int MODE, int CM>
+__device__ __forceinline__ void axis_fluxes(const Geom &g, const DevEos &eos,
+                                            const double *__restrict__ Qs,
+                                            const double *__restrict__ m, int m_full, int i,
+                                            int n, long c, long s, double tau, double dt,
+                                            double acc[7], int *status) {
+  if (n < 2) return;
+  const long N = g.N;
+  const double itau = 1. / tau;
+  const double tauFactor = MODE == 0 ? tau + 0.25 * dt : tau + 0.5 * dt;
+  const double dta = MODE == 0 ? 0.5 * dt : dt;
+  const double dx = D == 1 ? g.dx : (D == 2 ? g.dy : g.dz);
+  const float pre = (float)(tauFactor * dta / dx);
+  const float itf = (float)(1. / tauFactor), itauf = (float)itau;
+  const long cm = i > 0 ? c - s : c, cp = i < n - 1 ? c + s : c;
+  Prim L0, R0;
+  face_state<-1, CM>(eos, Qs, N, cm, c, cp, itau, L0);
+  face_state<+1, CM>(eos, Qs, N, cm, c, cp, itau, R0);
+  float F[7];
+  if (i > 0) {
+    const long cmm = i > 1 ? cm - s : cm;
+    Prim Rm;
+    face_state<+1, CM>(eos, Qs, N, cmm, cm, c, itau, Rm);
+    const bool mk = !m_full && m[(D - 1) * N + cm] < 1. && m[(D - 1) * N + c] < 1.;
+    if (hlle_f32<D>(to_f(Rm), to_f(L0), itauf, itf, pre, mk, F, status))
+#pragma unroll
+      for (int k = 0; k < 7; k++) acc[k] += (double)F[k];
+  }
+  if (i < n - 1) {
+    const long cpp = i + 2 < n ? cp + s : cp;
+    Prim Lp;
+    face_state<-1, CM>(eos, Qs, N, c, cp, cpp, itau, Lp);
+    const bool mk = !m_full && m[(D - 1) * N + c] < 1. && m[(D - 1) * N + cp] < 1.;
+    if (hlle_f32<D>(to_f(R0), to_f(Lp), itauf, itf, pre, mk, F, status))
+#pragma unroll
+      for (int k = 0; k < 7; k++) acc[k] -= (double)F[k];
+  }
+}
+
+// Fused stage: interface fluxes along x, y, eta + Milne source + update.
+// MODE 0 (predictor): reads Q, writes Qh = Q + flux.
+// MODE 1 (corrector): reads Q and Qh, writes Qnew = Q + flux unless
+// Q0 + flux0 < 0 (src/cll.cpp:70-74).
+template <int MODE, int CM>
+__global__ void __launch_bounds__(256)
+k_stage(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
+        const double *__restrict__ m, int m_full, const double *__restrict__ tp, double dt,
+        double *__restrict__ out, int *__restrict__ status) {
+  const double tau = *tp;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  int ix, iy, iz;
+  decompose(g, c, ix, iy, iz);
+  const long N = g.N;
+  const double *Qs = MODE == 0 ? Q : Qh;
+  double acc[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) acc[k] = 0.;
+  axis_fluxes<1, MODE, CM>(g, eos, Qs, m, m_full, ix, g.nx, c, 1, tau, dt, acc, status);
+  axis_fluxes<2, MODE, CM>(g, eos, Qs, m, m_full, iy, g.ny, c, g.nx, tau, dt, acc, status);
+  axis_fluxes<3, MODE, CM>(g, eos, Qs, m, m_full, iz, g.nz, c, (long)g.nx * g.ny, tau, dt, acc,
+                           status);
+  // Milne source on the unreconstructed state (src/hdo.cpp:307-353)
+  const double tau1 = MODE == 0 ? tau : tau + 0.5 * dt;
+  const double sdt = MODE == 0 ? 0.5 * dt : dt;
+  const double it1 = 1. / tau1;
+  double q[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) q[k] = Qs[k * N + c] * it1;
+  Prim r;
+  c2p_mode<CM>(eos, q, r);
+  acc[T_] += (-q[T_] * r.vz * r.vz - r.p * (1. + r.vz * r.vz)) * sdt;
+  acc[Z_] += -q[Z_] * sdt;
+  if (MODE == 0) {
+#pragma unroll
+    for (int k = 0; k < 7; k++) out[k * N + c] = Q[k * N + c] + acc[k];
+  } else {
+    const bool skip = Q[c] + acc[0] < 0.;
+#pragma unroll
+    for (int k = 0; k < 7; k++) out[k * N + c] = skip ? Q[k * N + c] : Q[k * N + c] + acc[k];
+  }
+}
+
+// ---------------------------------------------------------------- viscous
+template <int CM>
+__global__ void __launch_bounds__(256)
+k_vprim(Geom g, DevEos eos, const double *__restrict__ Qh, const double *__restrict__ tp,
+        double dt, double *__restrict__ ph, double *__restrict__ pt) {
+  const double tau = *tp;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  const long N = g.N;
+  double q[7];
+  Prim r;
+  const double ith = 1. / (tau - 0.5 * dt), it = 1. / tau;
+#pragma unroll
+  for (int k = 0; k < 7; k++) q[k] = Qh[k * N + c] * ith;
+  c2p_mode<CM>(eos, q, r);
+  ph[c] = r.e;
+  ph[N + c] = r.p;
+  ph[2 * N + c] = r.nb;
+  ph[3 * N + c] = r.nq;
+  ph[4 * N + c] = r.ns;
+  ph[5 * N + c] = r.vx;
+  ph[6 * N + c] = r.vy;
+  ph[7 * N + c] = r.vz;
+#pragma unroll
+  for (int k = 0; k < 7; k++) q[k] = Qh[k * N + c] * it;
+  c2p_mode<CM>(eos, q, r);
+  pt[c] = r.e;
+  pt[N + c] = r.vx;
+  pt[2 * N + c] = r.vy;
+  pt[3 * N + c] = r.vz;
+}
+
+// one Israel-Stewart update of the packed tensor (FP32), in the algebraically
+// reduced form of src/hdo.cpp:790-809 (half step) and 843-862 (full step):
+//   sum_l u_l dmu[l][k]               -> a_k  (4 terms instead of 16 per pair)
+//   sum_kl P_kl(phi7 P_kl - tpp s_kl) -> one scalar S for all pairs
+// X: the tensor the cross terms act on (pi for the half step, piH0 for the
+// full step); Pm: the tensor of the phi7 term's first factor (pi in both).
+__device__ __forceinline__ void is_cross(const float *X, const float *Pm, const float *sig,
+                                         const float u[4], const float a[4], float du,
+                                         float Xbulk, float dpp, float tpp, float lpP,
+                                         float phi7, float fac, float out[10]) {
+  const float gm[4] = {1.f, -1.f, -1.f, -1.f};
+  float b[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    float t = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; k++) t += X[sym(i, k)] * gm[k] * a[k];
+    b[i] = t;
+  }
+  float S = 0.f;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+#pragma unroll
+    for (int l = 0; l < 4; l++)
+      S += X[sym(k, l)] * (phi7 * X[sym(k, l)] - tpp * sig[sym(k, l)]) * gm[k] * gm[l];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j <= i; j++) {
+      const int ij = sym(i, j);
+      float cij = 0.f, dij = 0.f, dji = 0.f;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        cij += Pm[sym(i, k)] * X[sym(j, k)] * gm[k];
+        dij += X[sym(i, k)] * sig[sym(j, k)] * gm[k];
+        dji += X[sym(j, k)] * sig[sym(i, k)] * gm[k];
+      }
+      const float Dij = (i == j ? gm[i] : 0.f) - u[i] * u[j];
+      out[ij] = (-dpp * X[ij] * du + lpP * Xbulk * sig[ij] + phi7 * cij - 0.5f * tpp * (dij + dji) -
+                 u[j] * b[i] - u[i] * b[j] - (1.f / 3.f) * Dij * S) *
+                fac;
+    }
+}
+
+template <int CM>
+__global__ void __launch_bounds__(256)
+k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
+        const double *__restrict__ Qn, const double *__restrict__ ph,
+        const double *__restrict__ pt, const double *__restrict__ m, int m_full,
+        const float *__restrict__ pi, const float *__restrict__ Pi,
+        const double *__restrict__ tp, double dt, float *__restrict__ pi0,
+        float *__restrict__ piH0, float *__restrict__ Pi0, float *__restrict__ PiH0,
+        int *__restrict__ status) {
+  const double tau = *tp;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  int ix, iy, iz;
+  decompose(g, c, ix, iy, iz);
+  const long N = g.N;
+  const double e = ph[c], nb = ph[2 * N + c], nq = ph[3 * N + c], ns = ph[4 * N + c];
+  const double vx = ph[5 * N + c], vy = ph[6 * N + c], vz = ph[7 * N + c];
+  if (e <= 0.) {
+#pragma unroll
+    for (int k = 0; k < 10; k++) {
+      piH0[k * N + c] = 0.f;
+      pi0[k * N + c] = 0.f;
+    }
+    PiH0[c] = 0.f;
+    Pi0[c] = 0.f;
+    return;
+  }
+  const double gamma = 1.0 / sqrt(1.0 - vx * vx - vy * vy - vz * vz);
+  const double u[4] = {gamma, gamma * vx, gamma * vy, gamma * vz};
+  // ---- NSquant in FP64 (src/hdo.cpp:381-652)
+  double dmu[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) dmu[i][j] = 0.;
+  bool vac = false;
+  if (!m_full) {
+    auto maxM = [&](int a, int b) {
+      const long cc = g.cidx(ix + a, iy + b, iz);
+      return smax(m[cc], smax(m[N + cc], m[2 * N + cc]));
+    };
+    vac = maxM(1, 0) <= 0.9 || maxM(0, 1) <= 0.9 || maxM(-1, 0) <= 0.9 || maxM(0, -1) <= 0.9 ||
+          maxM(1, 1) <= 0.9 || maxM(1, -1) <= 0.9 || maxM(-1, 1) <= 0.9 || maxM(-1, -1) <= 0.9;
+  }
+  const double VMIN = 1e-2, UDIFF = 3.0;
+  if (!vac) {
+    double q[7];
+    Prim p0, p1;
+    const double itn = 1. / (tau - dt), it1 = 1. / tau;
+#pragma unroll
+    for (int k = 0; k < 7; k++) q[k] = Qprev[k * N + c] * itn;
+    c2p_mode<CM>(eos, q, p0);
+#pragma unroll
+    for (int k = 0; k < 7; k++) q[k] = Qn[k * N + c] * it1;
+    c2p_mode<CM>(eos, q, p1);
+    const double idt = 1. / dt;
+    const double ut0 = 1.0 / sqrt(1.0 - p0.vx * p0.vx - p0.vy * p0.vy - p0.vz * p0.vz);
+    const double ut1 = 1.0 / sqrt(1.0 - p1.vx * p1.vx - p1.vy * p1.vy - p1.vz * p1.vz);
+    const double a0[4] = {ut0, ut0 * p0.vx, ut0 * p0.vy, ut0 * p0.vz};
+    const double a1[4] = {ut1, ut1 * p1.vx, ut1 * p1.vy, ut1 * p1.vz};
+    if (e > 0. && p0.e > 0.) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        // d(u^2)/2u form; plain difference where u^k is small or changes fast
+        const bool plain = k == 0 ? fabs(0.5 * (a1[0] + a0[0]) / a1[0]) > UDIFF
+                                  : (fabs(u[k]) < VMIN || fabs(0.5 * (a1[k] + a0[k]) / a1[k]) > UDIFF);
+        dmu[0][k] = plain ? (a1[k] - a0[k]) * idt : (a1[k] * a1[k] - a0[k] * a0[k]) * 0.5 / u[k] * idt;
+      }
+    }
+#pragma unroll
+    for (int d = 1; d <= 3; d++) {
+      const long ca = g.cidx(ix + (d == 1), iy + (d == 2), iz + (d == 3));
+      const long cb = g.cidx(ix - (d == 1), iy - (d == 2), iz - (d == 3));
+      if (pt[ca] > 0. && pt[cb] > 0.) {
+        const double avx = pt[N + ca], avy = pt[2 * N + ca], avz = pt[3 * N + ca];
+        const double bvx = pt[N + cb], bvy = pt[2 * N + cb], bvz = pt[3 * N + cb];
+        const double gb = 1.0 / sqrt(1.0 - bvx * bvx - bvy * bvy - bvz * bvz);
+        const double ga = 1.0 / sqrt(1.0 - avx * avx - avy * avy - avz * avz);
+        const double B[4] = {gb, gb * bvx, gb * bvy, gb * bvz};
+        const double A[4] = {ga, ga * avx, ga * avy, ga * avz};
+        const double ih = d == 1 ? 1. / g.dx : (d == 2 ? 1. / g.dy : 1. / (g.dz * (tau + 0.5 * dt)));
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const bool plain = k == 0 ? fabs(0.5 * (A[0] + B[0]) / u[0]) > UDIFF
+                                    : (fabs(u[k]) < VMIN || fabs(0.5 * (A[k] + B[k]) / u[k]) > UDIFF);
+          dmu[d][k] = plain ? 0.5 * (A[k] - B[k]) * ih : 0.25 * (A[k] * A[k] - B[k] * B[k]) / u[k] * ih;
+        }
+      }
+    }
+    dmu[3][0] += u[3] / (tau - 0.5 * dt);
+    dmu[3][3] += u[0] / (tau - 0.5 * dt);
+  }
+  const double du = dmu[0][0] + dmu[1][1] + dmu[2][2] + dmu[3][3];
+  // sigma^{ij} = 1/2 (D^{ik} dmu_kj + D^{jk} dmu_ki) - 1/3 D^{ij} theta, D = g - uu
+  const double gmd[4] = {1., -1., -1., -1.};
+  double Dm[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) Dm[i][j] = (i == j ? gmd[i] : 0.) - u[i] * u[j];
+  const double s = eos.s(e, nb, nq, ns);
+  const double T = eos.T(e), p = e * (1. / 3.);
+  const double etaS = trc.etaS(e, nb, T, 0., s, p);
+  const double zetaS = trc.zetaS(eos, e, T, s, p);
+  const double eta = etaS * s;
+  const double kpi = 2.0 * eta / 5.068;
+  float sig[10], piNS[10];
+  bool bad = false;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j <= i; j++) {
+      double t = 0.;
+#pragma unroll
+      for (int k = 0; k < 4; k++) t += 0.5 * (Dm[i][k] * dmu[k][j] + Dm[j][k] * dmu[k][i]);
+      t -= Dm[i][j] * du * (1. / 3.);
+      const double pns = kpi * t;
+      bad |= !(fabs(pns) <= 1.7976931348623157e308);
+      piNS[sym(i, j)] = (float)pns;
+      sig[sym(i, j)] = eta > 0. ? (float)t : 0.f;
+    }
+  if (bad) atomicOr(status, VH_ST_NAN_PI);
+  const float PiNS = (float)(-zetaS * s * du / 5.068);
+  // ---- relaxation times and second-order coefficients (src/trancoeff.h:28-37)
+  double taupi_d, tauPi_d;
+  trc.getTau(eos, e, nb, T, 0., s, p, taupi_d, tauPi_d);
+  const float taupi = (float)taupi_d, tauPi = (float)tauPi_d;
+  const float hdt = (float)(0.5 * dt), fdt = (float)dt;
+  float dpp = 4.f / 3.f, tpp = 10.f / 7.f, lpP = 6.f / 5.f;
+  float phi7 = p > 0. ? (float)(9. / 70. / p / taupi_d) : 0.f;
+  if (taupi < hdt) dpp = tpp = lpP = phi7 = 0.f;
+  float dPP = 2.f / 3.f, lPp = (float)(8. / 5. * (1. / 3. - eos.cs2e(e)));
+  if (tauPi < hdt) dPP = lPp = 0.f;
+  const float ig = (float)(1. / gamma);
+  const float uf[4] = {(float)u[0], (float)u[1], (float)u[2], (float)u[3]};
+  float af[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    af[k] = (float)(u[0] * dmu[0][k] + u[1] * dmu[1][k] + u[2] * dmu[2][k] + u[3] * dmu[3][k]);
+  const float duf = (float)du, vzf = (float)vz;
+  float P[10], H[10], F[10], cr[10];
+#pragma unroll
+  for (int k = 0; k < 10; k++) P[k] = pi[k * N + c];
+  const float Pc = Pi[c];
+  const float gmf[4] = {1.f, -1.f, -1.f, -1.f};
+  // half step
+  const bool rel = taupi > hdt, relB = tauPi > hdt;
+  const float rH = rel ? hdt * ig / taupi : 0.f, rHB = relB ? hdt * ig / tauPi : 0.f;
+#pragma unroll
+  for (int k = 0; k < 10; k++) H[k] = rel ? P[k] - (P[k] - piNS[k]) * rH : piNS[k];
+  float PH = relB ? Pc - (Pc - PiNS) * rHB : PiNS;
+  {
+    const float w = vzf / (float)(tau - dt * 0.75) * hdt;
+    H[sym(0, 0)] += -2.f * w * P[sym(0, 3)];
+    H[sym(3, 3)] += -2.f * w * P[sym(0, 3)];
+    H[sym(3, 0)] += -w * (P[sym(0, 0)] + P[sym(3, 3)]);
+    H[sym(1, 0)] += -w * P[sym(1, 3)];
+    H[sym(2, 0)] += -w * P[sym(2, 3)];
+    H[sym(3, 1)] += -w * P[sym(0, 1)];
+    H[sym(3, 2)] += -w * P[sym(0, 2)];
+  }
+  is_cross(P, P, sig, uf, af, duf, Pc, dpp, tpp, lpP, phi7, ig * hdt, cr);
+#pragma unroll
+  for (int k = 0; k < 10; k++) H[k] += cr[k];
+  {
+    float t = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+      for (int l = 0; l < 4; l++) t += P[sym(k, l)] * sig[sym(k, l)] * gmf[k] * gmf[l];
+    PH += (lPp * t - dPP * Pc * duf) * ig * hdt;
+  }
+  // full step
+  const float rF = rel ? fdt * ig / taupi : 0.f, rFB = relB ? fdt * ig / tauPi : 0.f;
+#pragma unroll
+  for (int k = 0; k < 10; k++) F[k] = rel ? P[k] - (H[k] - piNS[k]) * rF : piNS[k];
+  float PF = relB ? Pc - (PH - PiNS) * rFB : PiNS;
+  {
+    const float w = vzf / (float)(tau - dt * 0.5) * fdt;
+    F[sym(0, 0)] += -2.f * w * H[sym(0, 3)];
+    F[sym(3, 3)] += -2.f * w * H[sym(0, 3)];
+    F[sym(3, 0)] += -w * (H[sym(0, 0)] + H[sym(3, 3)]);
+    F[sym(1, 0)] += -w * H[sym(1, 3)];
+    F[sym(2, 0)] += -w * H[sym(2, 3)];
+    F[sym(3, 1)] += -w * H[sym(0, 1)];
+    F[sym(3, 2)] += -w * H[sym(0, 2)];
+  }
+  is_cross(H, P, sig, uf, af, duf, PH, dpp, tpp, lpP, phi7, ig * fdt, cr);
+#pragma unroll
+  for (int k = 0; k < 10; k++) F[k] += cr[k];
+  {
+    float t = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+      for (int l = 0; l < 4; l++) t += H[sym(k, l)] * sig[sym(k, l)] * gmf[k] * gmf[l];
+    PF += (lPp * t - dPP * PH * duf) * ig * fdt;
+  }
+#pragma unroll
+  for (int k = 0; k < 10; k++) {
+    piH0[k * N + c] = H[k];
+    pi0[k * N + c] = F[k];
+  }
+  PiH0[c] = PH;
+  Pi0[c] = PF;
+}
+
+// ISformal loop #2 (src/hdo.cpp:866-962) in FP32 with FP32 pi storage.
+__global__ void __launch_bounds__(256)
+k_visc2(Geom g, const double *__restrict__ ph, const float *__restrict__ pi0,
+        const float *__restrict__ piH0, const float *__restrict__ Pi0,
+        const float *__restrict__ PiH0, float *__restrict__ pi, float *__restrict__ piH,
+        float *__restrict__ Pi, float *__restrict__ PiH, double *__restrict__ vcc,
+        float *__restrict__ vf, const double *__restrict__ tp, double dt) {
+  const double tau = *tp;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  int ix, iy, iz;
+  decompose(g, c, ix, iy, iz);
+  const long N = g.N;
+  const float e = (float)ph[c], p = (float)ph[N + c];
+  const float vx = (float)ph[5 * N + c], vy = (float)ph[6 * N + c], vz = (float)ph[7 * N + c];
+  if (!(e > 0.f)) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) vf[k * N + c] = 0.f;
+    return;
+  }
+  const float v2 = vx * vx + vy * vy + vz * vz;
+  const float g2 = frcp(1.f - v2), gamma = fsqrt(g2);
+  const float u[4] = {gamma, gamma * vx, gamma * vy, gamma * vz};
+  const float tf = (float)tau, dtf = (float)dt;
+  float fl[4];
+  {
+    const float Po = Pi[c];
+#pragma unroll
+    for (int i = 0; i < 4; i++) fl[i] = (tf - dtf) * (pi[sym(0, i) * N + c] + Po * u[0] * u[i]);
+    fl[0] -= (tf - dtf) * Po;
+  }
+  const float xm = (float)(-ph[5 * N + c] * dt / g.dx);
+  const float ym = (float)(-ph[6 * N + c] * dt / g.dy);
+  const float zm = (float)(-ph[7 * N + c] * dt / g.dz / (tau - 0.5 * dt));
+  const float wx[2] = {1.f - fabsf(xm), fabsf(xm)}, wy[2] = {1.f - fabsf(ym), fabsf(ym)},
+              wz[2] = {1.f - fabsf(zm), fabsf(zm)};
+  const float wxH[2] = {1.f - 0.5f * fabsf(xm), 0.5f * fabsf(xm)},
+              wyH[2] = {1.f - 0.5f * fabsf(ym), 0.5f * fabsf(ym)},
+              wzH[2] = {1.f - 0.5f * fabsf(zm), 0.5f * fabsf(zm)};
+  const int sx = xm > 0.f ? 1 : (xm < 0.f ? -1 : 0);
+  const int sy = ym > 0.f ? 1 : (ym < 0.f ? -1 : 0);
+  const int sz = zm > 0.f ? 1 : (zm < 0.f ? -1 : 0);
+  float a[10], aH[10], A = 0.f, AH = 0.f;
+#pragma unroll
+  for (int k = 0; k < 10; k++) a[k] = aH[k] = 0.f;
+#pragma unroll
+  for (int jx = 0; jx < 2; jx++)
+#pragma unroll
+    for (int jy = 0; jy < 2; jy++)
+#pragma unroll
+      for (int jz = 0; jz < 2; jz++) {
+        const long c1 = g.cidx(ix + jx * sx, iy + jy * sy, iz + jz * sz);
+        const float w = wx[jx] * wy[jy] * wz[jz], wH = wxH[jx] * wyH[jy] * wzH[jz];
+#pragma unroll
+        for (int k = 0; k < 10; k++) {
+          a[k] += w * pi0[k * N + c1];
+          aH[k] += wH * piH0[k * N + c1];
+        }
+        A += w * Pi0[c1];
+        AH += wH * PiH0[c1];
+      }
+  const float maxT0 = fmax_((e + p) * g2 - p, (e + p) * v2 * g2 + p);
+  float maxpi = 0.f;
+#pragma unroll
+  for (int k = 0; k < 10; k++) maxpi = fmaxf(maxpi, fabsf(a[k]));
+  bool rescaled = false;
+  if (maxT0 < maxpi) {
+    const float f = 0.1f * maxT0 / maxpi;
+#pragma unroll
+    for (int k = 0; k < 10; k++) {
+      a[k] *= f;
+      aH[k] *= f;
+    }
+    rescaled = true;
+  }
+  if (fabsf(A) > p) {
+    if (A != 0.f) A = copysignf(p, A);
+    if (AH != 0.f) AH = copysignf(p, AH);
+    rescaled = true;
+  }
+  vcc[c] = rescaled ? (double)(maxT0 / maxpi) : 1.;
+#pragma unroll
+  for (int k = 0; k < 10; k++) {
+    pi[k * N + c] = a[k];
+    piH[k * N + c] = aH[k];
+  }
+  Pi[c] = A;
+  PiH[c] = AH;
+#pragma unroll
+  for (int i = 0; i < 4; i++) fl[i] += -tf * (a[sym(0, i)] + A * u[0] * u[i]);
+  fl[0] += tf * A;
+#pragma unroll
+  for (int i = 0; i < 4; i++) vf[i * N + c] = fl[i];
+}
+
+// viscous interface flux in FP32 (Hydro::visc_flux, src/hdo.cpp:966-1014)
+template <int D>
+__device__ __forceinline__ bool visc_face(const Geom &g, long l, long r,
+                                          const double *__restrict__ ph,
+                                          const float *__restrict__ piH,
+                                          const float *__restrict__ PiH,
+                                          const double *__restrict__ m, int m_full, float pre,
+                                          float F[4]) {
+  const long N = g.N;
+  if (!m_full && m[(D - 1) * N + l] < 1. && m[(D - 1) * N + r] < 1.) return false;
+  float vx = (float)(0.5 * (ph[5 * N + l] + ph[5 * N + r]));
+  float vy = (float)(0.5 * (ph[6 * N + l] + ph[6 * N + r]));
+  float vz = (float)(0.5 * (ph[7 * N + l] + ph[7 * N + r]));
+  const float v2 = vx * vx + vy * vy + vz * vz;
+  const float gamma = fsqrt(frcp(1.f - v2));
+  const float u[4] = {gamma, gamma * vx, gamma * vy, gamma * vz};
+  const float sP = 0.5f * (PiH[l] + PiH[r]);
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++) {
+    float f = 0.5f * (piH[sym(mu, D) * N + l] + piH[sym(mu, D) * N + r]);
+    if (mu == D) f += sP;  // -sP * g^{DD} with g^{DD} = -1
+    f += sP * u[mu] * u[D];
+    F[mu] = f * pre;
+  }
+  return true;
+}
+
+__global__ void __launch_bounds__(256)
+k_visc3(Geom g, const double *__restrict__ ph, const float *__restrict__ piH,
+        const float *__restrict__ PiH, const float *__restrict__ vf,
+        const double *__restrict__ m, int m_full, const double *__restrict__ tp, double dt,
+        double *__restrict__ Q) {
+  const double tau = *tp;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  int ix, iy, iz;
+  decompose(g, c, ix, iy, iz);
+  const long N = g.N;
+  float acc[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) acc[k] = vf[k * N + c];
+  const double pre0 = (tau - 0.5 * dt) * dt;
+  const float px = (float)(pre0 / g.dx), py = (float)(pre0 / g.dy),
+              pz = (float)(pre0 / (g.dz * (tau + 0.5 * dt)));
+  float F[4];
+  const long sy = g.nx, sz = (long)g.nx * g.ny;
+  if (ix > 0 && visc_face<1>(g, c - 1, c, ph, piH, PiH, m, m_full, px, F))
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] += F[k];
+  if (ix < g.nx - 1 && visc_face<1>(g, c, c + 1, ph, piH, PiH, m, m_full, px, F))
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] -= F[k];
+  if (iy > 0 && visc_face<2>(g, c - sy, c, ph, piH, PiH, m, m_full, py, F))
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] += F[k];
+  if (iy < g.ny - 1 && visc_face<2>(g, c, c + sy, ph, piH, PiH, m, m_full, py, F))
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] -= F[k];
+  if (iz > 0 && visc_face<3>(g, c - sz, c, ph, piH, PiH, m, m_full, pz, F))
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] += F[k];
+  if (iz < g.nz - 1 && visc_face<3>(g, c, c + sz, ph, piH, PiH, m, m_full, pz, F))
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] -= F[k];
+  if (ph[c] > 0.) {
+    const float vx = (float)ph[5 * N + c], vy = (float)ph[6 * N + c], vz = (float)ph[7 * N + c];
+    const float u0 = fsqrt(frcp(1.f - vx * vx - vy * vy - vz * vz)), u3 = u0 * vz;
+    const float dtf = (float)dt;
+    acc[0] += (-piH[sym(3, 3) * N + c] - PiH[c] * (1.f + u3 * u3)) * dtf;
+    acc[3] += -(piH[sym(0, 3) * N + c] + PiH[c] * u0 * u3) * dtf;
+  }
+  const double q0 = Q[c];
+  const double a0 = (double)acc[0];
+  double fac = 1.;
+  if (fabs(a0) > 0.5 * q0) fac = a0 != 0. ? fabs(0.5 * q0 / a0) : 0.;
+#pragma unroll
+  for (int k = 0; k < 4; k++) Q[k * N + c] += fac * (double)acc[k];
+}
+
+// ghost fill of Q and the FP32 pi, Pi (src/fld.cpp:275-382)
+template <int AXIS>
+__global__ void __launch_bounds__(256)
+k_ghost_full(Geom g, double *__restrict__ Q, float *__restrict__ pi, float *__restrict__ Pi) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long nline;
+  if (AXIS == 3) nline = (long)g.nx * g.ny;
+  else if (AXIS == 2) nline = (long)g.nx * g.nz;
+  else nline = (long)g.ny * g.nz;
+  if (t >= nline) return;
+  const long N = g.N;
+  const int n = AXIS == 3 ? g.nz : (AXIS == 2 ? g.ny : g.nx);
+  int a, b;
+  if (AXIS == 1) { a = (int)(t % g.ny); b = (int)(t / g.ny); }
+  else { a = (int)(t % g.nx); b = (int)(t / g.nx); }
+  auto cell = [&](int i) -> long {
+    if (AXIS == 3) return g.cidx(a, b, i);
+    if (AXIS == 2) return g.cidx(a, i, b);
+    return g.cidx(i, a, b);
+  };
+  const int src[2] = {2, n - 3}, d1[2] = {1, n - 2}, d0[2] = {0, n - 1};
+  for (int side = 0; side < 2; side++) {
+    const long cs = cell(src[side]), c1 = cell(d1[side]), c0 = cell(d0[side]);
+    double v[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++) v[k] = Q[k * N + cs];
+    const bool z = v[0] < 0.;
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+      Q[k * N + c1] = z ? 0. : v[k];
+      Q[k * N + c0] = z ? 0. : v[k];
+    }
+    float pv[10];
+#pragma unroll
+    for (int k = 0; k < 10; k++) pv[k] = pi[k * N + cs];
+    const float P = Pi[cs];
+#pragma unroll
+    for (int k = 0; k < 10; k++) {
+      pi[k * N + c0] = pv[k];
+      pi[k * N + c1] = pv[k];
+    }
+    Pi[c0] = P;
+    Pi[c1] = P;
+  }
+}
+
+// ---------------------------------------------------------- launchers
+static inline unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
+
+void launch_stage(int mode, int cm, const Geom &g, const DevEos &eos, const double *Q,
+                  const double *Qh, const double *m, int m_full, const double *tp, double dt,
+                  double *out, int *status, hipStream_t st) {
+  const dim3 gr(blocks(g.N)), b(256);
+#define VH_S(MM, CC)                                                                            \
+  if (mode == MM && cm == CC)                                                                   \
+    hipLaunchKernelGGL((k_stage<MM, CC>), gr, b, 0, st, g, eos, Q, Qh, m, m_full, tp, dt, out, \
+                       status);
+  VH_S(0, VH_C2P_REFERENCE) VH_S(1, VH_C2P_REFERENCE) VH_S(0, VH_C2P_CLOSED)
+  VH_S(1, VH_C2P_CLOSED)
+#undef VH_S
+}
+
+void launch_vprim(int cm, const Geom &g, const DevEos &eos, const double *Qh, const double *tp,
+                  double dt, double *ph, double *pt, hipStream_t st) {
+  if (cm == VH_C2P_CLOSED)
+    hipLaunchKernelGGL(k_vprim<VH_C2P_CLOSED>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, Qh,
+                       tp, dt, ph, pt);
+  else
+    hipLaunchKernelGGL(k_vprim<VH_C2P_REFERENCE>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos,
+                       Qh, tp, dt, ph, pt);
+}
+
+void launch_visc1(int cm, const Geom &g, const DevEos &eos, const DevTrc &trc,
+                  const double *Qprev, const double *Qn, const double *ph, const double *pt,
+                  const double *m, int m_full, const float *pi, const float *Pi,
+                  const double *tp, double dt, float *pi0, float *piH0, float *Pi0,
+                  float *PiH0, int *status, hipStream_t st) {
+  if (cm == VH_C2P_CLOSED)
+    hipLaunchKernelGGL(k_visc1<VH_C2P_CLOSED>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, trc,
+                       Qprev, Qn, ph, pt, m, m_full, pi, Pi, tp, dt, pi0, piH0, Pi0, PiH0,
+                       status);
+  else
+    hipLaunchKernelGGL(k_visc1<VH_C2P_REFERENCE>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos,
+                       trc, Qprev, Qn, ph, pt, m, m_full, pi, Pi, tp, dt, pi0, piH0, Pi0, PiH0,
+                       status);
+}
+
+void launch_visc2(const Geom &g, const double *ph, const float *pi0, const float *piH0,
+                  const float *Pi0, const float *PiH0, float *pi, float *piH, float *Pi,
+                  float *PiH, double *vcc, float *vf, const double *tp, double dt,
+                  hipStream_t st) {
+  hipLaunchKernelGGL(k_visc2, dim3(blocks(g.N)), dim3(256), 0, st, g, ph, pi0, piH0, Pi0, PiH0,
+                     pi, piH, Pi, PiH, vcc, vf, tp, dt);
+}
+
+void launch_visc3(const Geom &g, const double *ph, const float *piH, const float *PiH,
+                  const float *vf, const double *m, int m_full, const double *tp, double dt,
+                  double *Q, hipStream_t st) {
+  hipLaunchKernelGGL(k_visc3, dim3(blocks(g.N)), dim3(256), 0, st, g, ph, piH, PiH, vf, m,
+                     m_full, tp, dt, Q);
+}
+
+void launch_ghost_full(int axis, const Geom &g, double *Q, float *pi, float *Pi,
+                       hipStream_t st) {
+  const long n = axis == 3 ? (long)g.nx * g.ny : (axis == 2 ? (long)g.nx * g.nz : (long)g.ny * g.nz);
+  if (axis == 3)
+    hipLaunchKernelGGL(k_ghost_full<3>, dim3(blocks(n)), dim3(256), 0, st, g, Q, pi, Pi);
+  else if (axis == 2)
+    hipLaunchKernelGGL(k_ghost_full<2>, dim3(blocks(n)), dim3(256), 0, st, g, Q, pi, Pi);
+  else
+    hipLaunchKernelGGL(k_ghost_full<1>, dim3(blocks(n)), dim3(256), 0, st, g, Q, pi, Pi);
+}
+
+}  // namespace mx
+}  // namespace vh
