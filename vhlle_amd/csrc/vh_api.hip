@@ -120,6 +120,9 @@ const char *face_name(int D, int mode) {
 // tau from c->tp, advances it on the device.  Q[cur] -> Q[cur^1].
 // FP32 views of the pi-like arrays in mixed precision (same allocation)
 inline float *fp(double *a) { return reinterpret_cast<float *>(a); }
+// mixed precision packs pi0/Pi0/piH0/PiH0 into 96-byte per-cell records; the
+// 24 x N floats occupy the (adjacent, 256-aligned) pi0 and piH0 arrays
+inline float4 *rec(vh_ctx *c) { return reinterpret_cast<float4 *>(c->pi0); }
 
 // mixed precision: FP64 Q/Qh/con2prim, FP32 fluxes, viscous sources and pi
 int enqueue_step_mixed(vh_ctx *c, double dt) {
@@ -147,12 +150,11 @@ int enqueue_step_mixed(vh_ctx *c, double dt) {
     rc |= timed(c, "visc_prim", [&] { vh::mx::launch_vprim(cm, g, c->eos, c->Qh, c->tp, dt, c->ph, c->pt, st); });
     rc |= timed(c, "visc_relax", [&] {
       vh::mx::launch_visc1(cm, g, c->eos, c->trc, Qo, Qn, c->ph, c->pt, c->m, c->m_full,
-                           fp(c->pi), fp(c->Pi), c->tp, dt, fp(c->pi0), fp(c->piH0), fp(c->Pi0),
-                           fp(c->PiH0), c->status, st);
+                           fp(c->pi), fp(c->Pi), c->tp, dt, rec(c), c->status, st);
     });
     rc |= timed(c, "visc_advect", [&] {
-      vh::mx::launch_visc2(g, c->ph, fp(c->pi0), fp(c->piH0), fp(c->Pi0), fp(c->PiH0), fp(c->pi),
-                           fp(c->piH), fp(c->Pi), fp(c->PiH), c->vcc, fp(c->vf), c->tp, dt, st);
+      vh::mx::launch_visc2(g, c->ph, rec(c), fp(c->pi), fp(c->piH), fp(c->Pi), fp(c->PiH), c->vcc,
+                           fp(c->vf), c->tp, dt, st);
     });
     rc |= timed(c, "visc_flux_update", [&] {
       vh::mx::launch_visc3(g, c->ph, fp(c->piH), fp(c->PiH), fp(c->vf), c->m, c->m_full, c->tp,

@@ -15,11 +15,11 @@ void launch_vprim(int cm, const Geom &g, const DevEos &eos, const double *Qh, co
 void launch_visc1(int cm, const Geom &g, const DevEos &eos, const DevTrc &trc,
                   const double *Qprev, const double *Qn, const double *ph, const double *pt,
                   const double *m, int m_full, const float *pi, const float *Pi,
-                  const double *tp, double dt, float *pi0, float *piH0, float *Pi0,
-                  float *PiH0, int *status, hipStream_t st);
-void launch_visc2(const Geom &g, const double *ph, const float *pi0, const float *piH0,
-                  const float *Pi0, const float *PiH0, float *pi, float *piH, float *Pi,
-                  float *PiH, double *vcc, float *vf, const double *tp, double dt,
+                  const double *tp, double dt, float4 *rec, int *status, hipStream_t st);
+// rec: per-cell records of the IS auxiliaries, 6 float4 per cell
+// (pi0[10], Pi0, pad, piH0[10], PiH0, pad)
+void launch_visc2(const Geom &g, const double *ph, const float4 *rec, float *pi, float *piH,
+                  float *Pi, float *PiH, double *vcc, float *vf, const double *tp, double dt,
                   hipStream_t st);
 void launch_visc3(const Geom &g, const double *ph, const float *piH, const float *PiH,
                   const float *vf, const double *m, int m_full, const double *tp, double dt,

@@ -20,7 +20,7 @@ namespace vh {
 namespace mx {
 
 // ------------------------------------------------------------- helpers
-__device__ __forceinline__ void decompose(const Geom &g, long c, int &ix, int &iy, int &iz) {
+__device__ __forceinline__ void coords_of(const Geom &g, long c, int &ix, int &iy, int &iz) {
   ix = (int)(c % g.nx);
   const long r = c / g.nx;
   iy = (int)(r % g.ny);
@@ -245,7 +245,7 @@ k_stage(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restri
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.N) return;
   int ix, iy, iz;
-  decompose(g, c, ix, iy, iz);
+  coords_of(g, c, ix, iy, iz);
   const long N = g.N;
   const double *Qs = MODE == 0 ? Q : Qh;
   double acc[7];
@@ -358,25 +358,20 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
         const double *__restrict__ Qn, const double *__restrict__ ph,
         const double *__restrict__ pt, const double *__restrict__ m, int m_full,
         const float *__restrict__ pi, const float *__restrict__ Pi,
-        const double *__restrict__ tp, double dt, float *__restrict__ pi0,
-        float *__restrict__ piH0, float *__restrict__ Pi0, float *__restrict__ PiH0,
+        const double *__restrict__ tp, double dt, float4 *__restrict__ rec,
         int *__restrict__ status) {
   const double tau = *tp;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.N) return;
   int ix, iy, iz;
-  decompose(g, c, ix, iy, iz);
+  coords_of(g, c, ix, iy, iz);
   const long N = g.N;
   const double e = ph[c], nb = ph[2 * N + c], nq = ph[3 * N + c], ns = ph[4 * N + c];
   const double vx = ph[5 * N + c], vy = ph[6 * N + c], vz = ph[7 * N + c];
   if (e <= 0.) {
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-    for (int k = 0; k < 10; k++) {
-      piH0[k * N + c] = 0.f;
-      pi0[k * N + c] = 0.f;
-    }
-    PiH0[c] = 0.f;
-    Pi0[c] = 0.f;
+    for (int k = 0; k < 6; k++) rec[6 * c + k] = z;
     return;
   }
   const double gamma = 1.0 / sqrt(1.0 - vx * vx - vy * vy - vz * vz);
@@ -550,27 +545,26 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
       for (int l = 0; l < 4; l++) t += H[sym(k, l)] * sig[sym(k, l)] * gmf[k] * gmf[l];
     PF += (lPp * t - dPP * PH * duf) * ig * fdt;
   }
-#pragma unroll
-  for (int k = 0; k < 10; k++) {
-    piH0[k * N + c] = H[k];
-    pi0[k * N + c] = F[k];
-  }
-  PiH0[c] = PH;
-  Pi0[c] = PF;
+  // per-cell record: pi0[10], Pi0, pad, piH0[10], PiH0, pad (6 x float4)
+  rec[6 * c + 0] = make_float4(F[0], F[1], F[2], F[3]);
+  rec[6 * c + 1] = make_float4(F[4], F[5], F[6], F[7]);
+  rec[6 * c + 2] = make_float4(F[8], F[9], PF, 0.f);
+  rec[6 * c + 3] = make_float4(H[0], H[1], H[2], H[3]);
+  rec[6 * c + 4] = make_float4(H[4], H[5], H[6], H[7]);
+  rec[6 * c + 5] = make_float4(H[8], H[9], PH, 0.f);
 }
 
 // ISformal loop #2 (src/hdo.cpp:866-962) in FP32 with FP32 pi storage.
 __global__ void __launch_bounds__(256)
-k_visc2(Geom g, const double *__restrict__ ph, const float *__restrict__ pi0,
-        const float *__restrict__ piH0, const float *__restrict__ Pi0,
-        const float *__restrict__ PiH0, float *__restrict__ pi, float *__restrict__ piH,
+k_visc2(Geom g, const double *__restrict__ ph, const float4 *__restrict__ rec,
+        float *__restrict__ pi, float *__restrict__ piH,
         float *__restrict__ Pi, float *__restrict__ PiH, double *__restrict__ vcc,
         float *__restrict__ vf, const double *__restrict__ tp, double dt) {
   const double tau = *tp;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.N) return;
   int ix, iy, iz;
-  decompose(g, c, ix, iy, iz);
+  coords_of(g, c, ix, iy, iz);
   const long N = g.N;
   const float e = (float)ph[c], p = (float)ph[N + c];
   const float vx = (float)ph[5 * N + c], vy = (float)ph[6 * N + c], vz = (float)ph[7 * N + c];
@@ -612,13 +606,17 @@ k_visc2(Geom g, const double *__restrict__ ph, const float *__restrict__ pi0,
       for (int jz = 0; jz < 2; jz++) {
         const long c1 = g.cidx(ix + jx * sx, iy + jy * sy, iz + jz * sz);
         const float w = wx[jx] * wy[jy] * wz[jz], wH = wxH[jx] * wyH[jy] * wzH[jz];
+        const float4 r0 = rec[6 * c1], r1 = rec[6 * c1 + 1], r2 = rec[6 * c1 + 2];
+        const float4 r3 = rec[6 * c1 + 3], r4 = rec[6 * c1 + 4], r5 = rec[6 * c1 + 5];
+        const float p0[12] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, 0.f};
+        const float p1[12] = {r3.x, r3.y, r3.z, r3.w, r4.x, r4.y, r4.z, r4.w, r5.x, r5.y, r5.z, 0.f};
 #pragma unroll
         for (int k = 0; k < 10; k++) {
-          a[k] += w * pi0[k * N + c1];
-          aH[k] += wH * piH0[k * N + c1];
+          a[k] += w * p0[k];
+          aH[k] += wH * p1[k];
         }
-        A += w * Pi0[c1];
-        AH += wH * PiH0[c1];
+        A += w * p0[10];
+        AH += wH * p1[10];
       }
   const float maxT0 = fmax_((e + p) * g2 - p, (e + p) * v2 * g2 + p);
   float maxpi = 0.f;
@@ -690,7 +688,7 @@ k_visc3(Geom g, const double *__restrict__ ph, const float *__restrict__ piH,
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.N) return;
   int ix, iy, iz;
-  decompose(g, c, ix, iy, iz);
+  coords_of(g, c, ix, iy, iz);
   const long N = g.N;
   float acc[4];
 #pragma unroll
@@ -808,24 +806,20 @@ void launch_vprim(int cm, const Geom &g, const DevEos &eos, const double *Qh, co
 void launch_visc1(int cm, const Geom &g, const DevEos &eos, const DevTrc &trc,
                   const double *Qprev, const double *Qn, const double *ph, const double *pt,
                   const double *m, int m_full, const float *pi, const float *Pi,
-                  const double *tp, double dt, float *pi0, float *piH0, float *Pi0,
-                  float *PiH0, int *status, hipStream_t st) {
+                  const double *tp, double dt, float4 *rec, int *status, hipStream_t st) {
   if (cm == VH_C2P_CLOSED)
     hipLaunchKernelGGL(k_visc1<VH_C2P_CLOSED>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, trc,
-                       Qprev, Qn, ph, pt, m, m_full, pi, Pi, tp, dt, pi0, piH0, Pi0, PiH0,
-                       status);
+                       Qprev, Qn, ph, pt, m, m_full, pi, Pi, tp, dt, rec, status);
   else
     hipLaunchKernelGGL(k_visc1<VH_C2P_REFERENCE>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos,
-                       trc, Qprev, Qn, ph, pt, m, m_full, pi, Pi, tp, dt, pi0, piH0, Pi0, PiH0,
-                       status);
+                       trc, Qprev, Qn, ph, pt, m, m_full, pi, Pi, tp, dt, rec, status);
 }
 
-void launch_visc2(const Geom &g, const double *ph, const float *pi0, const float *piH0,
-                  const float *Pi0, const float *PiH0, float *pi, float *piH, float *Pi,
-                  float *PiH, double *vcc, float *vf, const double *tp, double dt,
+void launch_visc2(const Geom &g, const double *ph, const float4 *rec, float *pi, float *piH,
+                  float *Pi, float *PiH, double *vcc, float *vf, const double *tp, double dt,
                   hipStream_t st) {
-  hipLaunchKernelGGL(k_visc2, dim3(blocks(g.N)), dim3(256), 0, st, g, ph, pi0, piH0, Pi0, PiH0,
-                     pi, piH, Pi, PiH, vcc, vf, tp, dt);
+  hipLaunchKernelGGL(k_visc2, dim3(blocks(g.N)), dim3(256), 0, st, g, ph, rec, pi, piH, Pi, PiH,
+                     vcc, vf, tp, dt);
 }
 
 void launch_visc3(const Geom &g, const double *ph, const float *piH, const float *PiH,
