@@ -120,8 +120,8 @@ const char *face_name(int D, int mode) {
 // tau from c->tp, advances it on the device.  Q[cur] -> Q[cur^1].
 // FP32 views of the pi-like arrays in mixed precision (same allocation)
 inline float *fp(double *a) { return reinterpret_cast<float *>(a); }
-// mixed precision packs pi0/Pi0/piH0/PiH0 into 96-byte per-cell records; the
-// 24 x N floats occupy the (adjacent, 256-aligned) pi0 and piH0 arrays
+// mixed precision keeps pi0/Pi0/piH0/PiH0 as six planes of N float4 (96 N
+// bytes); they occupy the adjacent, 256-aligned pi0 and piH0 arrays (160 N)
 inline float4 *rec(vh_ctx *c) { return reinterpret_cast<float4 *>(c->pi0); }
 
 // mixed precision: FP64 Q/Qh/con2prim, FP32 fluxes, viscous sources and pi

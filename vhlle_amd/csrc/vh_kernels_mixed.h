@@ -16,8 +16,8 @@ void launch_visc1(int cm, const Geom &g, const DevEos &eos, const DevTrc &trc,
                   const double *Qprev, const double *Qn, const double *ph, const double *pt,
                   const double *m, int m_full, const float *pi, const float *Pi,
                   const double *tp, double dt, float4 *rec, int *status, hipStream_t st);
-// rec: per-cell records of the IS auxiliaries, 6 float4 per cell
-// (pi0[10], Pi0, pad, piH0[10], PiH0, pad)
+// rec: the Israel-Stewart auxiliaries pi0, Pi0, piH0, PiH0 as six planes of
+// N float4: (pi0 0-3) (pi0 4-7) (pi0 8-9, Pi0, 0) and the same for piH0
 void launch_visc2(const Geom &g, const double *ph, const float4 *rec, float *pi, float *piH,
                   float *Pi, float *PiH, double *vcc, float *vf, const double *tp, double dt,
                   hipStream_t st);

@@ -31,13 +31,16 @@ __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(
 __device__ __forceinline__ float fmin_(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float fmax_(float a, float b) { return (a < b) ? b : a; }
 
+// FP32 copy of a primitive state; gamma^2 is formed in FP64 first so that
+// |v| close to 1 does not round to 1 in FP32
 struct PrimF {
-  float e, p, nb, nq, ns, vx, vy, vz;
+  float e, p, nb, nq, ns, vx, vy, vz, g2;
 };
 
 __device__ __forceinline__ PrimF to_f(const Prim &a) {
-  return PrimF{(float)a.e, (float)a.p, (float)a.nb, (float)a.nq,
-               (float)a.ns, (float)a.vx, (float)a.vy, (float)a.vz};
+  const double g2 = 1. / (1. - (a.vx * a.vx + a.vy * a.vy + a.vz * a.vz));
+  return PrimF{(float)a.e, (float)a.p, (float)a.nb, (float)a.nq, (float)a.ns,
+               (float)a.vx, (float)a.vy, (float)a.vz, (float)g2};
 }
 
 // Closed-form con2prim of the conformal EoS p = e/3: the root of
@@ -123,9 +126,7 @@ __device__ __forceinline__ bool hlle_f32(PrimF L, PrimF R, float itau, float itf
   }
   if (L.e == 0.f && R.e == 0.f) return false;
   if (mskip) return false;
-  const float v2l = L.vx * L.vx + L.vy * L.vy + L.vz * L.vz;
-  const float v2r = R.vx * R.vx + R.vy * R.vy + R.vz * R.vz;
-  const float g2l = frcp(1.f - v2l), g2r = frcp(1.f - v2r);
+  const float g2l = L.g2, g2r = R.g2;
   const float wl = (L.e + L.p) * g2l, wr = (R.e + R.p) * g2r;  // El, Er of the reference
   const float gl = fsqrt(g2l), gr = fsqrt(g2r);
   float Ul[7], Ur[7];
@@ -371,7 +372,7 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
   if (e <= 0.) {
     const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-    for (int k = 0; k < 6; k++) rec[6 * c + k] = z;
+    for (int q = 0; q < 6; q++) rec[q * N + c] = z;
     return;
   }
   const double gamma = 1.0 / sqrt(1.0 - vx * vx - vy * vy - vz * vz);
@@ -545,13 +546,13 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
       for (int l = 0; l < 4; l++) t += H[sym(k, l)] * sig[sym(k, l)] * gmf[k] * gmf[l];
     PF += (lPp * t - dPP * PH * duf) * ig * fdt;
   }
-  // per-cell record: pi0[10], Pi0, pad, piH0[10], PiH0, pad (6 x float4)
-  rec[6 * c + 0] = make_float4(F[0], F[1], F[2], F[3]);
-  rec[6 * c + 1] = make_float4(F[4], F[5], F[6], F[7]);
-  rec[6 * c + 2] = make_float4(F[8], F[9], PF, 0.f);
-  rec[6 * c + 3] = make_float4(H[0], H[1], H[2], H[3]);
-  rec[6 * c + 4] = make_float4(H[4], H[5], H[6], H[7]);
-  rec[6 * c + 5] = make_float4(H[8], H[9], PH, 0.f);
+  // six planes of float4: (pi0 0-3) (pi0 4-7) (pi0 8-9, Pi0, 0) and the same for piH0
+  rec[c] = make_float4(F[0], F[1], F[2], F[3]);
+  rec[N + c] = make_float4(F[4], F[5], F[6], F[7]);
+  rec[2 * N + c] = make_float4(F[8], F[9], PF, 0.f);
+  rec[3 * N + c] = make_float4(H[0], H[1], H[2], H[3]);
+  rec[4 * N + c] = make_float4(H[4], H[5], H[6], H[7]);
+  rec[5 * N + c] = make_float4(H[8], H[9], PH, 0.f);
 }
 
 // ISformal loop #2 (src/hdo.cpp:866-962) in FP32 with FP32 pi storage.
@@ -573,8 +574,10 @@ k_visc2(Geom g, const double *__restrict__ ph, const float4 *__restrict__ rec,
     for (int k = 0; k < 4; k++) vf[k * N + c] = 0.f;
     return;
   }
-  const float v2 = vx * vx + vy * vy + vz * vz;
-  const float g2 = frcp(1.f - v2), gamma = fsqrt(g2);
+  const double v2d = ph[5 * N + c] * ph[5 * N + c] + ph[6 * N + c] * ph[6 * N + c] +
+                     ph[7 * N + c] * ph[7 * N + c];
+  const float v2 = (float)v2d;
+  const float g2 = (float)(1. / (1. - v2d)), gamma = fsqrt(g2);
   const float u[4] = {gamma, gamma * vx, gamma * vy, gamma * vz};
   const float tf = (float)tau, dtf = (float)dt;
   float fl[4];
@@ -606,17 +609,14 @@ k_visc2(Geom g, const double *__restrict__ ph, const float4 *__restrict__ rec,
       for (int jz = 0; jz < 2; jz++) {
         const long c1 = g.cidx(ix + jx * sx, iy + jy * sy, iz + jz * sz);
         const float w = wx[jx] * wy[jy] * wz[jz], wH = wxH[jx] * wyH[jy] * wzH[jz];
-        const float4 r0 = rec[6 * c1], r1 = rec[6 * c1 + 1], r2 = rec[6 * c1 + 2];
-        const float4 r3 = rec[6 * c1 + 3], r4 = rec[6 * c1 + 4], r5 = rec[6 * c1 + 5];
-        const float p0[12] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, 0.f};
-        const float p1[12] = {r3.x, r3.y, r3.z, r3.w, r4.x, r4.y, r4.z, r4.w, r5.x, r5.y, r5.z, 0.f};
-#pragma unroll
-        for (int k = 0; k < 10; k++) {
-          a[k] += w * p0[k];
-          aH[k] += wH * p1[k];
-        }
-        A += w * p0[10];
-        AH += wH * p1[10];
+        const float4 r0 = rec[c1], r1 = rec[N + c1], r2 = rec[2 * N + c1];
+        const float4 r3 = rec[3 * N + c1], r4 = rec[4 * N + c1], r5 = rec[5 * N + c1];
+        a[0] += w * r0.x; a[1] += w * r0.y; a[2] += w * r0.z; a[3] += w * r0.w;
+        a[4] += w * r1.x; a[5] += w * r1.y; a[6] += w * r1.z; a[7] += w * r1.w;
+        a[8] += w * r2.x; a[9] += w * r2.y; A += w * r2.z;
+        aH[0] += wH * r3.x; aH[1] += wH * r3.y; aH[2] += wH * r3.z; aH[3] += wH * r3.w;
+        aH[4] += wH * r4.x; aH[5] += wH * r4.y; aH[6] += wH * r4.z; aH[7] += wH * r4.w;
+        aH[8] += wH * r5.x; aH[9] += wH * r5.y; AH += wH * r5.z;
       }
   const float maxT0 = fmax_((e + p) * g2 - p, (e + p) * v2 * g2 + p);
   float maxpi = 0.f;
@@ -662,11 +662,11 @@ __device__ __forceinline__ bool visc_face(const Geom &g, long l, long r,
                                           float F[4]) {
   const long N = g.N;
   if (!m_full && m[(D - 1) * N + l] < 1. && m[(D - 1) * N + r] < 1.) return false;
-  float vx = (float)(0.5 * (ph[5 * N + l] + ph[5 * N + r]));
-  float vy = (float)(0.5 * (ph[6 * N + l] + ph[6 * N + r]));
-  float vz = (float)(0.5 * (ph[7 * N + l] + ph[7 * N + r]));
-  const float v2 = vx * vx + vy * vy + vz * vz;
-  const float gamma = fsqrt(frcp(1.f - v2));
+  const double dvx = 0.5 * (ph[5 * N + l] + ph[5 * N + r]);
+  const double dvy = 0.5 * (ph[6 * N + l] + ph[6 * N + r]);
+  const double dvz = 0.5 * (ph[7 * N + l] + ph[7 * N + r]);
+  const float vx = (float)dvx, vy = (float)dvy, vz = (float)dvz;
+  const float gamma = (float)(1. / sqrt(1. - (dvx * dvx + dvy * dvy + dvz * dvz)));
   const float u[4] = {gamma, gamma * vx, gamma * vy, gamma * vz};
   const float sP = 0.5f * (PiH[l] + PiH[r]);
 #pragma unroll
@@ -717,8 +717,9 @@ k_visc3(Geom g, const double *__restrict__ ph, const float *__restrict__ piH,
 #pragma unroll
     for (int k = 0; k < 4; k++) acc[k] -= F[k];
   if (ph[c] > 0.) {
-    const float vx = (float)ph[5 * N + c], vy = (float)ph[6 * N + c], vz = (float)ph[7 * N + c];
-    const float u0 = fsqrt(frcp(1.f - vx * vx - vy * vy - vz * vz)), u3 = u0 * vz;
+    const double dvx = ph[5 * N + c], dvy = ph[6 * N + c], dvz = ph[7 * N + c];
+    const float u0 = (float)(1. / sqrt(1. - dvx * dvx - dvy * dvy - dvz * dvz)),
+                u3 = u0 * (float)dvz;
     const float dtf = (float)dt;
     acc[0] += (-piH[sym(3, 3) * N + c] - PiH[c] * (1.f + u3 * u3)) * dtf;
     acc[3] += -(piH[sym(0, 3) * N + c] + PiH[c] * u0 * u3) * dtf;
