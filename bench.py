@@ -20,7 +20,8 @@ a perturbed FP64 run is reported as the floor ("chaos_floor").
 
 "step" = one Hydro::performStep (src/hdo.cpp:1016-1134) over the whole grid,
 ghost cells included.  value = cells * steps / time of the timed region of the
-headline variant (the fastest one whose T^{tau mu} error is <= 1e-6), with the
+headline variant (the fastest one whose T^{tau mu} error is <= max(1e-6, twice the
+chaos floor) and whose run raised no error status), with the
 state resident in HBM; the timed region replays the step as HIP graphs from the
 initial state at tau0.  Per-kernel durations for the roofline come from HIP
 events recorded on the context's stream around every launch during a second,
@@ -231,7 +232,10 @@ def main():
             speedup_vs_fp64=round(v / (N * args.steps / results["fp64"]["elapsed"]), 3),
             roofline=roofline(r["kernel_times"], N, prec),
             kernels_ms={k: round(t[1], 3) for k, t in sorted(r["kernel_times"].items())})
-    ok = [n for n in variants if variants[n]["err_T_vs_fp64"] <= ERR_BAR and
+    # accepted: T error within 1e-6 or within twice the FP64 scheme's own
+    # sensitivity to a 1e-12 perturbation of its initial state
+    bar = max(ERR_BAR, 2.0 * floor[0])
+    ok = [n for n in variants if variants[n]["err_T_vs_fp64"] <= bar and
           variants[n]["status"] == 0]
     head = max(ok, key=lambda n: variants[n]["value"]) if ok else "fp64"
     H = variants[head]
@@ -253,7 +257,7 @@ def main():
                    "cells": N, "variant": head, "precision": H["precision"], "c2p": H["c2p"]},
         "hbm_frac_step": H["step_hbm_frac"],
         "error_measure": {"steps": ERR_STEPS, "tau": round(tau_e, 4), "region": "e_fp64 > 0.5 GeV/fm^3",
-                          "cells": int(mask.sum()), "chaos_floor_T": floor[0],
+                          "cells": int(mask.sum()), "accept_bar_T": bar, "chaos_floor_T": floor[0],
                           "chaos_floor_pi": floor[1]},
         "roofline": H["roofline"],
         "variants": variants,

@@ -118,6 +118,26 @@ const char *face_name(int D, int mode) {
 
 // Enqueues one Hydro::performStep on c->st (src/hdo.cpp:1016-1134).  Reads
 // tau from c->tp, advances it on the device.  Q[cur] -> Q[cur^1].
+// Ghost-cell fill of the reference's z, y, x passes (src/fld.cpp:211-382):
+// one fused kernel when every extent is 1 or >= 5, else the three passes.
+void ghost_fill(vh_ctx *c, int full, double *Qn, hipStream_t st, int &rc) {
+  const vh::Geom &g = c->g;
+  const int pf = c->precision == VH_PREC_MIXED;
+  const char *name = full ? "ghost_full" : "ghost_ideal";
+  if (vh::ghost_fusable(g)) {
+    rc |= timed(c, name, [&] { vh::launch_ghost1(full, pf, g, Qn, c->Qh, c->pi, c->Pi, st); });
+    return;
+  }
+  for (int ax = 3; ax >= 1; ax--)
+    rc |= timed(c, name, [&] {
+      if (full && pf)
+        vh::mx::launch_ghost_full(ax, g, Qn, reinterpret_cast<float *>(c->pi),
+                                  reinterpret_cast<float *>(c->Pi), st);
+      else
+        vh::launch_ghost(full, ax, g, Qn, c->Qh, c->pi, c->Pi, st);
+    });
+}
+
 // FP32 views of the pi-like arrays in mixed precision (same allocation)
 inline float *fp(double *a) { return reinterpret_cast<float *>(a); }
 // mixed precision keeps pi0/Pi0/piH0/PiH0 as six planes of N float4 (96 N
@@ -144,8 +164,7 @@ int enqueue_step_mixed(vh_ctx *c, double dt) {
                          st);
   });
   rc |= timed(c, "advance_tau", [&] { vh::launch_advance_tau(c->tp, dt, st); });
-  for (int ax = 3; ax >= 1; ax--)
-    rc |= timed(c, "ghost_ideal", [&] { vh::launch_ghost(0, ax, g, Qn, c->Qh, c->pi, c->Pi, st); });
+  ghost_fill(c, 0, Qn, st, rc);
   if (c->viscous) {
     rc |= timed(c, "visc_prim", [&] { vh::mx::launch_vprim(cm, g, c->eos, c->Qh, c->tp, dt, c->ph, c->pt, st); });
     rc |= timed(c, "visc_relax", [&] {
@@ -161,8 +180,7 @@ int enqueue_step_mixed(vh_ctx *c, double dt) {
                            dt, Qn, st);
     });
   }
-  for (int ax = 3; ax >= 1; ax--)
-    rc |= timed(c, "ghost_full", [&] { vh::mx::launch_ghost_full(ax, g, Qn, fp(c->pi), fp(c->Pi), st); });
+  ghost_fill(c, 1, Qn, st, rc);
   VH_HIP(hipGetLastError());
   c->cur ^= 1;
   return rc ? VH_ERR_HIP : VH_OK;
@@ -197,8 +215,7 @@ int enqueue_step_fp64(vh_ctx *c, double dt) {
   }
   rc |= timed(c, "advance_tau", [&] { vh::launch_advance_tau(c->tp, dt, st); });
   // Fluid::correctImagCells (src/fld.cpp:211-273): z, y, x
-  for (int ax = 3; ax >= 1; ax--)
-    rc |= timed(c, "ghost_ideal", [&] { vh::launch_ghost(0, ax, g, Qn, c->Qh, c->pi, c->Pi, st); });
+  ghost_fill(c, 0, Qn, st, rc);
   if (c->viscous) {
     rc |= timed(c, "visc_prim", [&] { vh::launch_vprim(g, c->eos, c->Qh, c->tp, dt, c->ph, c->pt, st); });
     rc |= timed(c, "visc_relax", [&] {
@@ -214,8 +231,7 @@ int enqueue_step_fp64(vh_ctx *c, double dt) {
     });
   }
   // Fluid::correctImagCellsFull (src/fld.cpp:275-382): z, y, x
-  for (int ax = 3; ax >= 1; ax--)
-    rc |= timed(c, "ghost_full", [&] { vh::launch_ghost(1, ax, g, Qn, c->Qh, c->pi, c->Pi, st); });
+  ghost_fill(c, 1, Qn, st, rc);
   VH_HIP(hipGetLastError());
   c->cur ^= 1;
   return rc ? VH_ERR_HIP : VH_OK;

@@ -15,6 +15,11 @@ void launch_update(int mode, const Geom &g, const DevEos &eos, const double *Q,
                    int *status, hipStream_t st);
 void launch_ghost(int full, int axis, const Geom &g, double *Q, double *Qh, double *pi,
                   double *Pi, hipStream_t st);
+// single-pass ghost fill (valid when ghost_fusable): Q and Qh (full = 0) or
+// Q, pi, Pi (full = 1); pi_float selects FP32 pi storage
+bool ghost_fusable(const Geom &g);
+void launch_ghost1(int full, int pi_float, const Geom &g, double *Q, double *Qh, void *pi,
+                   void *Pi, hipStream_t st);
 void launch_update_m(int pass, const Geom &g, double *m, double *dm, const double *tp, double dt,
                      hipStream_t st);
 void launch_vprim(const Geom &g, const DevEos &eos, const double *Qh, const double *tp,

@@ -313,6 +313,77 @@ k_ghost(Geom g, double *__restrict__ Q, double *__restrict__ Qh, double *__restr
   }
 }
 
+// Single-pass form of the three ghost passes.  When every axis has n = 1 or
+// n >= 5, the z, y, x copies of the reference compose to "copy from the cell
+// whose ghost coordinates are mapped to the nearest interior index"
+// (0,1 -> 2 and n-2,n-1 -> n-3), because those sources are never ghosts of an
+// axis with n >= 5 and the Q[T_] < 0 zeroing of setQ is idempotent.  An axis
+// with n = 1 makes every cell a ghost of itself: the reference then re-sets
+// every cell, which only applies the zeroing; k_ghost_zero does that first, so
+// that k_ghost1 never writes a cell another thread reads.  One thread per
+// cell; PT: storage type of pi/Pi.
+__device__ __forceinline__ int ghost_map(int i, int n) {
+  return n == 1 ? 0 : (i < 2 ? 2 : (i > n - 3 ? n - 3 : i));
+}
+
+template <int FULL>
+__global__ void __launch_bounds__(256)
+k_ghost_zero(Geom g, double *__restrict__ Q, double *__restrict__ Qh) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  const long N = g.N;
+  if (Q[c] < 0.)
+    for (int k = 0; k < 7; k++) Q[k * N + c] = 0.;
+  if (!FULL && Qh[c] < 0.)
+    for (int k = 0; k < 7; k++) Qh[k * N + c] = 0.;
+}
+
+template <int FULL, typename PT>
+__global__ void __launch_bounds__(256)
+k_ghost1(Geom g, double *__restrict__ Q, double *__restrict__ Qh, PT *__restrict__ pi,
+         PT *__restrict__ Pi) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  int ix, iy, iz;
+  decompose(g, c, ix, iy, iz);
+  const int sx = ghost_map(ix, g.nx), sy = ghost_map(iy, g.ny), sz = ghost_map(iz, g.nz);
+  if (sx == ix && sy == iy && sz == iz) return;
+  const long N = g.N, src = g.idx(sx, sy, sz);
+  double v[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) v[k] = Q[k * N + src];
+  set_clamped(Q, N, c, v);
+  if (!FULL) {
+#pragma unroll
+    for (int k = 0; k < 7; k++) v[k] = Qh[k * N + src];
+    set_clamped(Qh, N, c, v);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 10; k++) pi[k * N + c] = pi[k * N + src];
+    Pi[c] = Pi[src];
+  }
+}
+
+bool ghost_fusable(const Geom &g) {
+  auto ok = [](int n) { return n == 1 || n >= 5; };
+  return ok(g.nx) && ok(g.ny) && ok(g.nz);
+}
+
+void launch_ghost1(int full, int pi_float, const Geom &g, double *Q, double *Qh, void *pi,
+                   void *Pi, hipStream_t st) {
+  const dim3 gr((unsigned)((g.N + 255) / 256)), b(256);
+  if (g.nx == 1 || g.ny == 1 || g.nz == 1) {
+    if (full) hipLaunchKernelGGL((k_ghost_zero<1>), gr, b, 0, st, g, Q, Qh);
+    else hipLaunchKernelGGL((k_ghost_zero<0>), gr, b, 0, st, g, Q, Qh);
+  }
+  if (!full)
+    hipLaunchKernelGGL((k_ghost1<0, double>), gr, b, 0, st, g, Q, Qh, (double *)pi, (double *)Pi);
+  else if (pi_float)
+    hipLaunchKernelGGL((k_ghost1<1, float>), gr, b, 0, st, g, Q, Qh, (float *)pi, (float *)Pi);
+  else
+    hipLaunchKernelGGL((k_ghost1<1, double>), gr, b, 0, st, g, Q, Qh, (double *)pi, (double *)Pi);
+}
+
 // ---------------------------------------------------------------- updateM
 // Fluid::updateM (src/fld.cpp:384-423), pass 1: dm from the neighbours' m.
 __global__ void __launch_bounds__(256)

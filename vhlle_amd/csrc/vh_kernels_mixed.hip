@@ -89,6 +89,57 @@ __device__ __forceinline__ void c2p_closed(double q[7], Prim &r) {
   r = Prim{e, p, nb, nq, ns, vx, vy, vz};
 }
 
+// FP32 closed-form con2prim of p = e/3 for an FP64 state, for the interface
+// states of the FP32 flux.  Only |Q|^2 and Q_t^2 - |Q|^2 are formed in FP64
+// (the latter cancels as |v| -> 1), scaled by an FP32 reciprocal of Q_t so that
+// nothing overflows FP32.  With x = |Q|/Q_t and y = 1 - x:
+//   s = sqrt(4 - 3x^2) = sqrt(1 + y (6 - 3y)),   v = 3x / (2 + s),
+//   1 - v = 3y (3 - y + s) / ((1 + s)(2 + s)),
+//   e = Q_t ((1 - v) + y v),   1/gamma^2 = (1 - v)(1 + v),
+// all sums of positive terms, so v, e and gamma^2 keep FP32 relative accuracy
+// (a few ulp) for every |v| < 1.  Branches as transformPV (src/rmn.cpp:47-70):
+// vacuum (Q_t <= 0), at rest, and |Q| > Q_t treated as |Q| = 0.9999 Q_t.
+__device__ __forceinline__ PrimF c2p_fast(const double q[7]) {
+  PrimF r{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 1.f};
+  const double Qt = q[T_];
+  if (!(Qt > 0.)) return r;
+  const double M2 = q[X_] * q[X_] + q[Y_] * q[Y_] + q[Z_] * q[Z_];
+  if (M2 == 0.) {
+    r.e = (float)Qt;
+    r.p = r.e * (1.f / 3.f);
+    r.nb = (float)q[NB_];
+    r.nq = (float)q[NQ_];
+    r.ns = (float)q[NS_];
+    return r;
+  }
+  const double rq = (double)frcp((float)Qt);
+  const double rq2 = rq * rq;
+  const float a = (float)((Qt * Qt - M2) * rq2);  // (1 - x^2) t^2
+  const float t = (float)(Qt * rq);                 // 1 up to the rcp error
+  const float mm = fsqrt((float)(M2 * rq2));        // x t
+  float y = a * frcp(t * (t + mm)), x = mm * frcp(t);
+  if (!(a > 0.f)) {
+    y = 1e-4f;
+    x = 0.9999f;
+  }
+  const float s = fsqrt(1.f + y * (6.f - 3.f * y));
+  const float omv = 3.f * y * (3.f - y + s) * frcp((1.f + s) * (2.f + s));
+  const float v = 3.f * x * frcp(2.f + s);
+  const float ig2 = omv * (2.f - omv);
+  r.e = (float)Qt * (omv + y * v);
+  r.p = r.e * (1.f / 3.f);
+  r.g2 = frcp(ig2);
+  const float vm = v * frcp(mm);
+  r.vx = (float)(q[X_] * rq) * vm;
+  r.vy = (float)(q[Y_] * rq) * vm;
+  r.vz = (float)(q[Z_] * rq) * vm;
+  const float ig = fsqrt(ig2);
+  r.nb = (float)q[NB_] * ig;
+  r.nq = (float)q[NQ_] * ig;
+  r.ns = (float)q[NS_] * ig;
+  return r;
+}
+
 template <int CM>
 __device__ __forceinline__ void c2p_mode(const DevEos &eos, double q[7], Prim &r) {
   if (CM == VH_C2P_CLOSED)
@@ -97,11 +148,11 @@ __device__ __forceinline__ void c2p_mode(const DevEos &eos, double q[7], Prim &r
     c2p_ref(eos, q, r);
 }
 
-// MUSCL face state (src/cll.cpp:106-204) in FP64, con2prim by mode.
+// MUSCL face state (src/cll.cpp:106-204) in FP64, con2prim by mode, as the
+// FP32 state the flux takes.
 template <int SIDE, int CM>
-__device__ __forceinline__ void face_state(const DevEos &eos, const double *__restrict__ Qs,
-                                           long N, long cm, long c0, long cp, double itau,
-                                           Prim &r) {
+__device__ __forceinline__ PrimF face_state(const DevEos &eos, const double *__restrict__ Qs,
+                                            long N, long cm, long c0, long cp, double itau) {
   double q[7];
 #pragma unroll
   for (int k = 0; k < 7; k++) {
@@ -109,7 +160,10 @@ __device__ __forceinline__ void face_state(const DevEos &eos, const double *__re
     const double dq = minmod((Qs[k * N + cp] - a) * 0.5, (a - Qs[k * N + cm]) * 0.5);
     q[k] = (SIDE < 0 ? a - dq : a + dq) * itau;
   }
+  if (CM == VH_C2P_CLOSED) return c2p_fast(q);
+  Prim r;
   c2p_mode<CM>(eos, q, r);
+  return to_f(r);
 }
 
 // HLLE interface flux in FP32 (Hydro::hlle_flux, src/hdo.cpp:77-305: same
@@ -209,25 +263,22 @@ __device__ __forceinline__ void axis_fluxes(const Geom &g, const DevEos &eos,
   const float pre = (float)(tauFactor * dta / dx);
   const float itf = (float)(1. / tauFactor), itauf = (float)itau;
   const long cm = i > 0 ? c - s : c, cp = i < n - 1 ? c + s : c;
-  Prim L0, R0;
-  face_state<-1, CM>(eos, Qs, N, cm, c, cp, itau, L0);
-  face_state<+1, CM>(eos, Qs, N, cm, c, cp, itau, R0);
+  const PrimF L0 = face_state<-1, CM>(eos, Qs, N, cm, c, cp, itau);
+  const PrimF R0 = face_state<+1, CM>(eos, Qs, N, cm, c, cp, itau);
   float F[7];
   if (i > 0) {
     const long cmm = i > 1 ? cm - s : cm;
-    Prim Rm;
-    face_state<+1, CM>(eos, Qs, N, cmm, cm, c, itau, Rm);
+    const PrimF Rm = face_state<+1, CM>(eos, Qs, N, cmm, cm, c, itau);
     const bool mk = !m_full && m[(D - 1) * N + cm] < 1. && m[(D - 1) * N + c] < 1.;
-    if (hlle_f32<D>(to_f(Rm), to_f(L0), itauf, itf, pre, mk, F, status))
+    if (hlle_f32<D>(Rm, L0, itauf, itf, pre, mk, F, status))
 #pragma unroll
       for (int k = 0; k < 7; k++) acc[k] += (double)F[k];
   }
   if (i < n - 1) {
     const long cpp = i + 2 < n ? cp + s : cp;
-    Prim Lp;
-    face_state<-1, CM>(eos, Qs, N, c, cp, cpp, itau, Lp);
+    const PrimF Lp = face_state<-1, CM>(eos, Qs, N, c, cp, cpp, itau);
     const bool mk = !m_full && m[(D - 1) * N + c] < 1. && m[(D - 1) * N + cp] < 1.;
-    if (hlle_f32<D>(to_f(R0), to_f(Lp), itauf, itf, pre, mk, F, status))
+    if (hlle_f32<D>(R0, Lp, itauf, itf, pre, mk, F, status))
 #pragma unroll
       for (int k = 0; k < 7; k++) acc[k] -= (double)F[k];
   }
@@ -263,9 +314,18 @@ k_stage(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restri
   double q[7];
 #pragma unroll
   for (int k = 0; k < 7; k++) q[k] = Qs[k * N + c] * it1;
-  Prim r;
-  c2p_mode<CM>(eos, q, r);
-  acc[T_] += (-q[T_] * r.vz * r.vz - r.p * (1. + r.vz * r.vz)) * sdt;
+  double vz, pr;
+  if (CM == VH_C2P_CLOSED) {
+    const PrimF r = c2p_fast(q);
+    vz = r.vz;
+    pr = r.p;
+  } else {
+    Prim r;
+    c2p_mode<CM>(eos, q, r);
+    vz = r.vz;
+    pr = r.p;
+  }
+  acc[T_] += (-q[T_] * vz * vz - pr * (1. + vz * vz)) * sdt;
   acc[Z_] += -q[Z_] * sdt;
   if (MODE == 0) {
 #pragma unroll
