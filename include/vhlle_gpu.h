@@ -51,8 +51,12 @@ enum {
 /* con2prim modes (vh_set_c2p_mode) */
 enum {
   VH_C2P_REFERENCE = 0,   /* transformPV's bisection/Newton from v=0.5 */
-  VH_C2P_LEARNED = 1,     /* MLP initial guess + FP64 Newton, fallback to 0 */
-  VH_C2P_CLOSED = 2       /* closed-form conformal root + FP64 Newton polish */
+  VH_C2P_LEARNED = 1,     /* MLP initial guess + FP64 Newton polish to a residual
+                             <= 1e-12 Q_t, fallback to the reference iteration
+                             (needs vh_set_c2p_model) */
+  VH_C2P_CLOSED = 2,      /* closed-form root of the conformal EoS */
+  VH_C2P_NEWTON = 3       /* the same Newton polish started at v = 0.5, no MLP
+                             (the ablation of VH_C2P_LEARNED) */
 };
 
 /* Fluid geometry: the Fluid ctor arguments (src/fld.cpp:62-80).
@@ -153,6 +157,58 @@ int vh_c2p(const vh_eos_t *eos, double *Q, long n, double *out, int *iters);
 /* transformCV (src/rmn.cpp:253-263): prim n x 8 -> Q n x 7 (host arrays). */
 int vh_p2c(const double *prim, long n, double *Q);
 
+/* ---- learned con2prim: MLP 5 -> 64 -> 64 -> 2 (ReLU) ----
+ * No counterpart in the reference: an initial guess for transformPV's root
+ * (src/rmn.cpp:28-139).  params: VH_MLP_NPARAM floats,
+ *   W1[64][5] | b1[64] | W2[64][64] | b2[64] | W3[2][64] | b3[2];
+ * inputs of a state q = Q/tau: ln(q_t)/10, |q|/q_t, q_eta/q_t, n_b/q_t,
+ * sqrt(1 - (|q|/q_t)^2); outputs: |v| and e/q_t.  Inference runs in BF16 MFMA
+ * with FP32 accumulation; W1, b1, W2, the inputs and the first hidden layer
+ * are rounded to BF16, the rest is FP32. */
+#define VH_MLP_NPARAM 4674
+enum { VH_START_MLP = 0, VH_START_HALF = 1 };
+typedef struct {
+  int batch;                  /* samples per Adam step */
+  int epochs;
+  double lr, lr_final;        /* cosine schedule from lr to lr_final */
+  unsigned long long seed;    /* shuffling */
+} vh_mlp_train_cfg_t;
+/* He-uniform initial weights (zero biases) from a SplitMix64 stream */
+int vh_mlp_init(unsigned long long seed, float *params);
+/* MFMA inference: out n x 2 (|v|, e/q_t) for states Q (n x 7, already / tau) */
+int vh_mlp_forward(const float *params, const double *Q, long n, float *out);
+/* gradient of the loss (1/n) sum_i |MLP(q_i) - (|v_i|, e_i/q_t,i)|^2 over n
+ * states Q (n x 7) with reference primitives prim (n x 8, vh_c2p layout);
+ * grad: VH_MLP_NPARAM floats; *loss (may be NULL) the loss.  n <= 2^20. */
+int vh_mlp_grad(const float *params, const double *Q, const double *prim, long n,
+                float *grad, double *loss);
+/* Adam training on the GPU from params (updated in place); loss_hist (may be
+ * NULL): mean sampled batch loss per epoch (cfg->epochs entries). */
+int vh_mlp_train(const vh_mlp_train_cfg_t *cfg, const double *Q, const double *prim,
+                 long n, float *params, double *loss_hist);
+/* con2prim of n states (as vh_c2p) started from the MLP guess
+ * (VH_START_MLP) or from v = 0.5 (VH_START_HALF): Newton polish to
+ * |f| <= 1e-12 q_t, else the reference iteration.  iters (may be NULL): polish
+ * updates, or -1 - (reference iterations) where it fell back. */
+int vh_c2p_learned(const vh_eos_t *eos, const float *params, double *Q, long n,
+                   int start, double *out, int *iters);
+/* Installs an MLP for VH_C2P_LEARNED in a context (copied to the device). */
+int vh_set_c2p_model(vh_ctx_t *ctx, const float *params);
+/* Training pairs from ideal Gubser runs (the 101 x 101 x 1 grid, dx = 0.1,
+ * tau0 = 1, dtau = 0.025) on the FP64 path: n states q = Q/tau drawn (seeded)
+ * from every cell of every step of runs at the IC amplitudes amp[0..namp),
+ * the fraction augment of them replaced by transformCV images of
+ * random (e, v, n_b) with |v| <= vmax; prim = vh_c2p of each state.
+ * Q: n x 7, prim: n x 8. */
+typedef struct {
+  unsigned long long seed;
+  int steps;                  /* steps per run */
+  int namp;
+  double amp[4];
+  double augment;             /* fraction in [0, 1] */
+  double vmax;
+} vh_harvest_cfg_t;
+int vh_mlp_harvest(const vh_harvest_cfg_t *cfg, long n, double *Q, double *prim);
 /* device/driver information line (name, CUs, clocks) into buf */
 int vh_device_info(char *buf, int len);
 

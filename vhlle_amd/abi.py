@@ -14,9 +14,12 @@ LIB_PATH = os.path.join(HERE, "libvhlle_gpu.so")
 VH_OK, VH_ERR_ARG, VH_ERR_HIP, VH_ERR_NODEVICE, VH_ERR_DTAU = 0, -1, -2, -3, -4
 ST_NAN_FLUX, ST_BIG_E, ST_NAN_SOURCE, ST_NAN_PI = 1, 2, 4, 8
 PREC_FP64, PREC_MIXED = 0, 1
-C2P_REFERENCE, C2P_LEARNED, C2P_CLOSED = 0, 1, 2
+C2P_REFERENCE, C2P_LEARNED, C2P_CLOSED, C2P_NEWTON = 0, 1, 2, 3
+START_MLP, START_HALF = 0, 1
+MLP_NPARAM = 4674
 
 _dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
 
 
 class Grid(ctypes.Structure):
@@ -37,6 +40,17 @@ class Trc(ctypes.Structure):
         ("etaSparam", ctypes.c_int), ("zetaSparam", ctypes.c_int),
         ("zetaSPeakT", ctypes.c_double), ("zetaSWidthMinus", ctypes.c_double),
         ("zetaSWidthPlus", ctypes.c_double)]
+
+
+class MlpTrainCfg(ctypes.Structure):
+    _fields_ = [("batch", ctypes.c_int), ("epochs", ctypes.c_int), ("lr", ctypes.c_double),
+                ("lr_final", ctypes.c_double), ("seed", ctypes.c_ulonglong)]
+
+
+class HarvestCfg(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_ulonglong), ("steps", ctypes.c_int), ("namp", ctypes.c_int),
+                ("amp", ctypes.c_double * 4), ("augment", ctypes.c_double),
+                ("vmax", ctypes.c_double)]
 
 
 class State(ctypes.Structure):
@@ -89,6 +103,15 @@ def lib():
         "vh_c2p": (ctypes.c_int, [ctypes.POINTER(Eos), _dp, ctypes.c_long, _dp, ip]),
         "vh_p2c": (ctypes.c_int, [_dp, ctypes.c_long, _dp]),
         "vh_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
+        "vh_mlp_init": (ctypes.c_int, [ctypes.c_ulonglong, _fp]),
+        "vh_mlp_forward": (ctypes.c_int, [_fp, _dp, ctypes.c_long, _fp]),
+        "vh_mlp_grad": (ctypes.c_int, [_fp, _dp, _dp, ctypes.c_long, _fp, _dp]),
+        "vh_mlp_train": (ctypes.c_int, [ctypes.POINTER(MlpTrainCfg), _dp, _dp, ctypes.c_long,
+                                        _fp, _dp]),
+        "vh_c2p_learned": (ctypes.c_int, [ctypes.POINTER(Eos), _fp, _dp, ctypes.c_long,
+                                          ctypes.c_int, _dp, ip]),
+        "vh_set_c2p_model": (ctypes.c_int, [vp, _fp]),
+        "vh_mlp_harvest": (ctypes.c_int, [ctypes.POINTER(HarvestCfg), ctypes.c_long, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -101,7 +124,9 @@ def lib():
 def exported_symbols():
     return ["vh_create", "vh_destroy", "vh_ncell", "vh_upload", "vh_download", "vh_set_prim",
             "vh_step", "vh_run", "vh_check_dtau", "vh_set_c2p_mode", "vh_enable_timing",
-            "vh_kernel_times", "vh_c2p", "vh_p2c", "vh_device_info"]
+            "vh_kernel_times", "vh_c2p", "vh_p2c", "vh_device_info", "vh_mlp_init",
+            "vh_mlp_forward", "vh_mlp_grad", "vh_mlp_train", "vh_c2p_learned",
+            "vh_set_c2p_model", "vh_mlp_harvest"]
 
 
 def _p(a):
@@ -129,6 +154,73 @@ def c2p(Q):
     check(lib().vh_c2p(ctypes.byref(e), _p(Q), n, _p(out),
                        it.ctypes.data_as(ctypes.POINTER(ctypes.c_int))), "vh_c2p")
     return out, it, Q
+
+
+def _f(a):
+    return a.ctypes.data_as(_fp)
+
+
+def mlp_init(seed):
+    """He-uniform initial MLP parameters (float32 vector)."""
+    p = np.zeros(MLP_NPARAM, dtype=np.float32)
+    check(lib().vh_mlp_init(seed, _f(p)), "vh_mlp_init")
+    return p
+
+
+def mlp_forward(params, Q):
+    """MFMA inference over states Q (n x 7, already / tau): (n x 2) float32."""
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    out = np.zeros((Q.shape[0], 2), dtype=np.float32)
+    check(lib().vh_mlp_forward(_f(params), _p(Q), Q.shape[0], _f(out)), "vh_mlp_forward")
+    return out
+
+
+def mlp_grad(params, Q, prim):
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    prim = np.ascontiguousarray(prim, dtype=np.float64)
+    g = np.zeros(MLP_NPARAM, dtype=np.float32)
+    L = ctypes.c_double(0.0)
+    check(lib().vh_mlp_grad(_f(params), _p(Q), _p(prim), Q.shape[0], _f(g), ctypes.byref(L)),
+          "vh_mlp_grad")
+    return g, L.value
+
+
+def mlp_train(params, Q, prim, batch=4096, epochs=4, lr=2e-3, lr_final=1e-5, seed=1):
+    """Adam training on the GPU: (trained params, per-epoch loss)."""
+    p = np.ascontiguousarray(params, dtype=np.float32).copy()
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    prim = np.ascontiguousarray(prim, dtype=np.float64)
+    hist = np.zeros(epochs)
+    cfg = MlpTrainCfg(batch, epochs, lr, lr_final, seed)
+    check(lib().vh_mlp_train(ctypes.byref(cfg), _p(Q), _p(prim), Q.shape[0], _f(p), _p(hist)),
+          "vh_mlp_train")
+    return p, hist
+
+
+def c2p_learned(params, Q, start=START_MLP):
+    """con2prim from the MLP guess (or v = 0.5): (prim n x 8, iters, Q_after)."""
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    Q = np.ascontiguousarray(Q, dtype=np.float64).copy()
+    n = Q.shape[0]
+    out = np.zeros((n, 8))
+    it = np.zeros(n, dtype=np.int32)
+    e = Eos(0)
+    check(lib().vh_c2p_learned(ctypes.byref(e), _f(params), _p(Q), n, start, _p(out),
+                               it.ctypes.data_as(ctypes.POINTER(ctypes.c_int))),
+          "vh_c2p_learned")
+    return out, it, Q
+
+
+def harvest(n, seed=1, steps=200, amps=(10.0, 1.0), augment=0.5, vmax=0.99):
+    """Training pairs (Q n x 7, prim n x 8) from Gubser runs + augmentation."""
+    a = (ctypes.c_double * 4)(*(list(amps) + [0.0] * (4 - len(amps))))
+    cfg = HarvestCfg(seed, steps, len(amps), a, augment, vmax)
+    Q = np.zeros((n, 7))
+    prim = np.zeros((n, 8))
+    check(lib().vh_mlp_harvest(ctypes.byref(cfg), n, _p(Q), _p(prim)), "vh_mlp_harvest")
+    return Q, prim
 
 
 def p2c(prim):
@@ -200,6 +292,13 @@ class Context:
         s = ctypes.c_int(0)
         check(lib().vh_run(self.h, ctypes.byref(t), dt, nsteps, ctypes.byref(s)), "vh_run")
         return s.value, t.value
+
+    def set_c2p(self, mode, params=None):
+        """con2prim mode (C2P_*); params: the MLP for C2P_LEARNED."""
+        if params is not None:
+            p = np.ascontiguousarray(params, dtype=np.float32)
+            check(lib().vh_set_c2p_model(self.h, _f(p)), "vh_set_c2p_model")
+        check(lib().vh_set_c2p_mode(self.h, mode), "vh_set_c2p_mode")
 
     def enable_timing(self, on=True):
         check(lib().vh_enable_timing(self.h, 1 if on else 0), "vh_enable_timing")

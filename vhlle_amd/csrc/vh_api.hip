@@ -13,6 +13,7 @@
 #include "../../include/vhlle_gpu.h"
 #include "vh_kernels.h"
 #include "vh_kernels_mixed.h"
+#include "vh_mlp_host.h"
 
 #define VH_HIP(call)                                                         \
   do {                                                                       \
@@ -61,6 +62,10 @@ struct vh_ctx {
   double *ph = nullptr, *pt = nullptr, *vf = nullptr;
   double *tp = nullptr;
   int *status = nullptr;
+  // learned con2prim: MFMA fragments of the installed MLP and the per-cell
+  // guesses (mixed precision only; gv reuses the FP64 path's F[0] storage)
+  vh::ml::Packed *mlp = nullptr;
+  float *gv = nullptr;
   int cur = 0;
   int m_full = 0;
   // graph cache: one executable per parity of cur
@@ -155,20 +160,27 @@ int enqueue_step_mixed(vh_ctx *c, double dt) {
     rc |= timed(c, "update_m_dm", [&] { vh::launch_update_m(0, g, c->m, c->dm, c->tp, dt, st); });
     rc |= timed(c, "update_m_apply", [&] { vh::launch_update_m(1, g, c->m, c->dm, c->tp, dt, st); });
   }
+  const bool learned = cm == VH_C2P_LEARNED;
+  // the MLP guess of every cell of the stage's input state (Q/tau_n, then
+  // Qh/tau_n: both stages reconstruct on tau_n-scaled states)
+  if (learned)
+    rc |= timed(c, "c2p_guess", [&] { vh::ml::launch_guess(Qo, g.N, c->tp, 0., c->mlp, c->gv, st); });
   rc |= timed(c, "stage_pred", [&] {
-    vh::mx::launch_stage(0, cm, g, c->eos, Qo, c->Qh, c->m, c->m_full, c->tp, dt, c->Qh,
+    vh::mx::launch_stage(0, cm, g, c->eos, Qo, c->Qh, c->gv, c->m, c->m_full, c->tp, dt, c->Qh,
                          c->status, st);
   });
+  if (learned)
+    rc |= timed(c, "c2p_guess", [&] { vh::ml::launch_guess(c->Qh, g.N, c->tp, 0., c->mlp, c->gv, st); });
   rc |= timed(c, "stage_corr", [&] {
-    vh::mx::launch_stage(1, cm, g, c->eos, Qo, c->Qh, c->m, c->m_full, c->tp, dt, Qn, c->status,
-                         st);
+    vh::mx::launch_stage(1, cm, g, c->eos, Qo, c->Qh, c->gv, c->m, c->m_full, c->tp, dt, Qn,
+                         c->status, st);
   });
   rc |= timed(c, "advance_tau", [&] { vh::launch_advance_tau(c->tp, dt, st); });
   ghost_fill(c, 0, Qn, st, rc);
   if (c->viscous) {
-    rc |= timed(c, "visc_prim", [&] { vh::mx::launch_vprim(cm, g, c->eos, c->Qh, c->tp, dt, c->ph, c->pt, st); });
+    rc |= timed(c, "visc_prim", [&] { vh::mx::launch_vprim(cm, g, c->eos, c->Qh, c->gv, c->tp, dt, c->ph, c->pt, st); });
     rc |= timed(c, "visc_relax", [&] {
-      vh::mx::launch_visc1(cm, g, c->eos, c->trc, Qo, Qn, c->ph, c->pt, c->m, c->m_full,
+      vh::mx::launch_visc1(cm, g, c->eos, c->trc, Qo, Qn, c->gv, c->ph, c->pt, c->m, c->m_full,
                            fp(c->pi), fp(c->Pi), c->tp, dt, rec(c), c->status, st);
     });
     rc |= timed(c, "visc_advect", [&] {
@@ -362,6 +374,7 @@ int vh_create(const vh_grid_t *grid, const vh_eos_t *eos, const vh_trcoeff_t *tr
   c->pt = takef(4);
   c->vf = takef(4);
   c->tp = takef(1);
+  c->gv = reinterpret_cast<float *>(c->F[0]);  // the fused mixed stages use no F
   c->status = reinterpret_cast<int *>(p);
   p += al;
   for (int d = 0; d < 3; d++) {
@@ -394,6 +407,7 @@ int vh_destroy(vh_ctx_t *c) {
     (void)hipEventDestroy(e.second);
   }
   if (c->pool) (void)hipFree(c->pool);
+  if (c->mlp) (void)hipFree(c->mlp);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
   return VH_OK;
@@ -515,9 +529,23 @@ int vh_set_c2p_mode(vh_ctx_t *c, int mode) {
   if (!c) return VH_ERR_ARG;
   // the FP64 path is the faithful one: reference iteration only
   if (c->precision == VH_PREC_FP64 && mode != VH_C2P_REFERENCE) return VH_ERR_ARG;
-  if (mode != VH_C2P_REFERENCE && mode != VH_C2P_CLOSED) return VH_ERR_ARG;
+  if (mode != VH_C2P_REFERENCE && mode != VH_C2P_CLOSED && mode != VH_C2P_LEARNED &&
+      mode != VH_C2P_NEWTON)
+    return VH_ERR_ARG;
+  if (mode == VH_C2P_LEARNED && !c->mlp) return VH_ERR_ARG;  // no model installed
   c->c2p_mode = mode;
   drop_graphs(c);
+  return VH_OK;
+}
+
+int vh_set_c2p_model(vh_ctx_t *c, const float *params) {
+  if (!c || !params) return VH_ERR_ARG;
+  if (c->precision != VH_PREC_MIXED) return VH_ERR_ARG;
+  vh::ml::Packed hp;
+  vh::ml::pack(params, hp);
+  VH_HIP(hipStreamSynchronize(c->st));
+  if (!c->mlp) VH_HIP(hipMalloc(&c->mlp, sizeof(vh::ml::Packed)));
+  VH_HIP(hipMemcpy(c->mlp, &hp, sizeof hp, hipMemcpyHostToDevice));
   return VH_OK;
 }
 

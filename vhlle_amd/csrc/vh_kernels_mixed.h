@@ -7,15 +7,17 @@
 namespace vh {
 namespace mx {
 
+// gv: per-cell con2prim guesses (used by VH_C2P_LEARNED only, may be null otherwise)
 void launch_stage(int mode, int cm, const Geom &g, const DevEos &eos, const double *Q,
-                  const double *Qh, const double *m, int m_full, const double *tp, double dt,
-                  double *out, int *status, hipStream_t st);
-void launch_vprim(int cm, const Geom &g, const DevEos &eos, const double *Qh, const double *tp,
-                  double dt, double *ph, double *pt, hipStream_t st);
+                  const double *Qh, const float *gv, const double *m, int m_full,
+                  const double *tp, double dt, double *out, int *status, hipStream_t st);
+void launch_vprim(int cm, const Geom &g, const DevEos &eos, const double *Qh, const float *gv,
+                  const double *tp, double dt, double *ph, double *pt, hipStream_t st);
 void launch_visc1(int cm, const Geom &g, const DevEos &eos, const DevTrc &trc,
-                  const double *Qprev, const double *Qn, const double *ph, const double *pt,
-                  const double *m, int m_full, const float *pi, const float *Pi,
-                  const double *tp, double dt, float4 *rec, int *status, hipStream_t st);
+                  const double *Qprev, const double *Qn, const float *gv, const double *ph,
+                  const double *pt, const double *m, int m_full, const float *pi,
+                  const float *Pi, const double *tp, double dt, float4 *rec, int *status,
+                  hipStream_t st);
 // rec: the Israel-Stewart auxiliaries pi0, Pi0, piH0, PiH0 as six planes of
 // N float4: (pi0 0-3) (pi0 4-7) (pi0 8-9, Pi0, 0) and the same for piH0
 void launch_visc2(const Geom &g, const double *ph, const float4 *rec, float *pi, float *piH,

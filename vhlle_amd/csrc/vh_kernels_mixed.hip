@@ -11,10 +11,12 @@
 // interface are recomputed by both neighbours.  Nothing is written to HBM but
 // the stage result, so a stage moves about its compulsory bytes (read Q, and
 // Qh in the corrector, write Qh or Q).  The con2prim algorithm is a template
-// parameter: the reference iteration (C2P_REF), or the closed-form conformal
-// root followed by one FP64 Newton step (C2P_CLOSED).  The learned initial
-// guess (C2P_LEARNED) is taken from a per-state MLP (vh_mlp.h).
+// parameter: the reference iteration (C2P_REFERENCE), the closed-form
+// conformal root (C2P_CLOSED), or an FP64 Newton polish (vh_mlp.h) started
+// from a per-cell guess array gv (C2P_LEARNED: the MLP guess of the cell that
+// owns the state; C2P_NEWTON: v = 0.5 everywhere, the ablation).
 #include "vh_kernels_mixed.h"
+#include "vh_mlp.h"
 
 namespace vh {
 namespace mx {
@@ -140,19 +142,29 @@ __device__ __forceinline__ PrimF c2p_fast(const double q[7]) {
   return r;
 }
 
+// con2prim by mode; v0: the learned guess (C2P_LEARNED only)
 template <int CM>
-__device__ __forceinline__ void c2p_mode(const DevEos &eos, double q[7], Prim &r) {
+__device__ __forceinline__ void c2p_mode(const DevEos &eos, double q[7], Prim &r, float v0) {
   if (CM == VH_C2P_CLOSED)
     c2p_closed(q, r);
+  else if (CM == VH_C2P_LEARNED)
+    ml::c2p_guess(eos, q, (double)v0, r);
+  else if (CM == VH_C2P_NEWTON)
+    ml::c2p_guess(eos, q, 0.5, r);
   else
     c2p_ref(eos, q, r);
+}
+template <int CM>
+__device__ __forceinline__ float guess_of(const float *__restrict__ gv, long c) {
+  return CM == VH_C2P_LEARNED ? gv[c] : 0.5f;
 }
 
 // MUSCL face state (src/cll.cpp:106-204) in FP64, con2prim by mode, as the
 // FP32 state the flux takes.
 template <int SIDE, int CM>
 __device__ __forceinline__ PrimF face_state(const DevEos &eos, const double *__restrict__ Qs,
-                                            long N, long cm, long c0, long cp, double itau) {
+                                            const float *__restrict__ gv, long N, long cm,
+                                            long c0, long cp, double itau) {
   double q[7];
 #pragma unroll
   for (int k = 0; k < 7; k++) {
@@ -162,7 +174,7 @@ __device__ __forceinline__ PrimF face_state(const DevEos &eos, const double *__r
   }
   if (CM == VH_C2P_CLOSED) return c2p_fast(q);
   Prim r;
-  c2p_mode<CM>(eos, q, r);
+  c2p_mode<CM>(eos, q, r, guess_of<CM>(gv, c0));
   return to_f(r);
 }
 
@@ -251,6 +263,7 @@ __device__ __forceinline__ bool hlle_f32(PrimF L, PrimF R, float itau, float itf
 template <int D, int MODE, int CM>
 __device__ __forceinline__ void axis_fluxes(const Geom &g, const DevEos &eos,
                                             const double *__restrict__ Qs,
+                                            const float *__restrict__ gv,
                                             const double *__restrict__ m, int m_full, int i,
                                             int n, long c, long s, double tau, double dt,
                                             double acc[7], int *status) {
@@ -263,12 +276,12 @@ __device__ __forceinline__ void axis_fluxes(const Geom &g, const DevEos &eos,
   const float pre = (float)(tauFactor * dta / dx);
   const float itf = (float)(1. / tauFactor), itauf = (float)itau;
   const long cm = i > 0 ? c - s : c, cp = i < n - 1 ? c + s : c;
-  const PrimF L0 = face_state<-1, CM>(eos, Qs, N, cm, c, cp, itau);
-  const PrimF R0 = face_state<+1, CM>(eos, Qs, N, cm, c, cp, itau);
+  const PrimF L0 = face_state<-1, CM>(eos, Qs, gv, N, cm, c, cp, itau);
+  const PrimF R0 = face_state<+1, CM>(eos, Qs, gv, N, cm, c, cp, itau);
   float F[7];
   if (i > 0) {
     const long cmm = i > 1 ? cm - s : cm;
-    const PrimF Rm = face_state<+1, CM>(eos, Qs, N, cmm, cm, c, itau);
+    const PrimF Rm = face_state<+1, CM>(eos, Qs, gv, N, cmm, cm, c, itau);
     const bool mk = !m_full && m[(D - 1) * N + cm] < 1. && m[(D - 1) * N + c] < 1.;
     if (hlle_f32<D>(Rm, L0, itauf, itf, pre, mk, F, status))
 #pragma unroll
@@ -276,7 +289,7 @@ __device__ __forceinline__ void axis_fluxes(const Geom &g, const DevEos &eos,
   }
   if (i < n - 1) {
     const long cpp = i + 2 < n ? cp + s : cp;
-    const PrimF Lp = face_state<-1, CM>(eos, Qs, N, c, cp, cpp, itau);
+    const PrimF Lp = face_state<-1, CM>(eos, Qs, gv, N, c, cp, cpp, itau);
     const bool mk = !m_full && m[(D - 1) * N + c] < 1. && m[(D - 1) * N + cp] < 1.;
     if (hlle_f32<D>(R0, Lp, itauf, itf, pre, mk, F, status))
 #pragma unroll
@@ -291,8 +304,9 @@ __device__ __forceinline__ void axis_fluxes(const Geom &g, const DevEos &eos,
 template <int MODE, int CM>
 __global__ void __launch_bounds__(256)
 k_stage(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
-        const double *__restrict__ m, int m_full, const double *__restrict__ tp, double dt,
-        double *__restrict__ out, int *__restrict__ status) {
+        const float *__restrict__ gv, const double *__restrict__ m, int m_full,
+        const double *__restrict__ tp, double dt, double *__restrict__ out,
+        int *__restrict__ status) {
   const double tau = *tp;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.N) return;
@@ -303,9 +317,9 @@ k_stage(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restri
   double acc[7];
 #pragma unroll
   for (int k = 0; k < 7; k++) acc[k] = 0.;
-  axis_fluxes<1, MODE, CM>(g, eos, Qs, m, m_full, ix, g.nx, c, 1, tau, dt, acc, status);
-  axis_fluxes<2, MODE, CM>(g, eos, Qs, m, m_full, iy, g.ny, c, g.nx, tau, dt, acc, status);
-  axis_fluxes<3, MODE, CM>(g, eos, Qs, m, m_full, iz, g.nz, c, (long)g.nx * g.ny, tau, dt, acc,
+  axis_fluxes<1, MODE, CM>(g, eos, Qs, gv, m, m_full, ix, g.nx, c, 1, tau, dt, acc, status);
+  axis_fluxes<2, MODE, CM>(g, eos, Qs, gv, m, m_full, iy, g.ny, c, g.nx, tau, dt, acc, status);
+  axis_fluxes<3, MODE, CM>(g, eos, Qs, gv, m, m_full, iz, g.nz, c, (long)g.nx * g.ny, tau, dt, acc,
                            status);
   // Milne source on the unreconstructed state (src/hdo.cpp:307-353)
   const double tau1 = MODE == 0 ? tau : tau + 0.5 * dt;
@@ -321,7 +335,7 @@ k_stage(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restri
     pr = r.p;
   } else {
     Prim r;
-    c2p_mode<CM>(eos, q, r);
+    c2p_mode<CM>(eos, q, r, guess_of<CM>(gv, c));
     vz = r.vz;
     pr = r.p;
   }
@@ -340,8 +354,9 @@ k_stage(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restri
 // ---------------------------------------------------------------- viscous
 template <int CM>
 __global__ void __launch_bounds__(256)
-k_vprim(Geom g, DevEos eos, const double *__restrict__ Qh, const double *__restrict__ tp,
-        double dt, double *__restrict__ ph, double *__restrict__ pt) {
+k_vprim(Geom g, DevEos eos, const double *__restrict__ Qh, const float *__restrict__ gv,
+        const double *__restrict__ tp, double dt, double *__restrict__ ph,
+        double *__restrict__ pt) {
   const double tau = *tp;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.N) return;
@@ -351,7 +366,7 @@ k_vprim(Geom g, DevEos eos, const double *__restrict__ Qh, const double *__restr
   const double ith = 1. / (tau - 0.5 * dt), it = 1. / tau;
 #pragma unroll
   for (int k = 0; k < 7; k++) q[k] = Qh[k * N + c] * ith;
-  c2p_mode<CM>(eos, q, r);
+  c2p_mode<CM>(eos, q, r, guess_of<CM>(gv, c));
   ph[c] = r.e;
   ph[N + c] = r.p;
   ph[2 * N + c] = r.nb;
@@ -362,7 +377,7 @@ k_vprim(Geom g, DevEos eos, const double *__restrict__ Qh, const double *__restr
   ph[7 * N + c] = r.vz;
 #pragma unroll
   for (int k = 0; k < 7; k++) q[k] = Qh[k * N + c] * it;
-  c2p_mode<CM>(eos, q, r);
+  c2p_mode<CM>(eos, q, r, guess_of<CM>(gv, c));
   pt[c] = r.e;
   pt[N + c] = r.vx;
   pt[2 * N + c] = r.vy;
@@ -416,7 +431,8 @@ __device__ __forceinline__ void is_cross(const float *X, const float *Pm, const 
 template <int CM>
 __global__ void __launch_bounds__(256)
 k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
-        const double *__restrict__ Qn, const double *__restrict__ ph,
+        const double *__restrict__ Qn, const float *__restrict__ gv,
+        const double *__restrict__ ph,
         const double *__restrict__ pt, const double *__restrict__ m, int m_full,
         const float *__restrict__ pi, const float *__restrict__ Pi,
         const double *__restrict__ tp, double dt, float4 *__restrict__ rec,
@@ -459,10 +475,10 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
     const double itn = 1. / (tau - dt), it1 = 1. / tau;
 #pragma unroll
     for (int k = 0; k < 7; k++) q[k] = Qprev[k * N + c] * itn;
-    c2p_mode<CM>(eos, q, p0);
+    c2p_mode<CM>(eos, q, p0, guess_of<CM>(gv, c));
 #pragma unroll
     for (int k = 0; k < 7; k++) q[k] = Qn[k * N + c] * it1;
-    c2p_mode<CM>(eos, q, p1);
+    c2p_mode<CM>(eos, q, p1, guess_of<CM>(gv, c));
     const double idt = 1. / dt;
     const double ut0 = 1.0 / sqrt(1.0 - p0.vx * p0.vx - p0.vy * p0.vy - p0.vz * p0.vz);
     const double ut1 = 1.0 / sqrt(1.0 - p1.vx * p1.vx - p1.vy * p1.vy - p1.vz * p1.vz);
@@ -842,38 +858,40 @@ k_ghost_full(Geom g, double *__restrict__ Q, float *__restrict__ pi, float *__re
 static inline unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
 
 void launch_stage(int mode, int cm, const Geom &g, const DevEos &eos, const double *Q,
-                  const double *Qh, const double *m, int m_full, const double *tp, double dt,
-                  double *out, int *status, hipStream_t st) {
+                  const double *Qh, const float *gv, const double *m, int m_full,
+                  const double *tp, double dt, double *out, int *status, hipStream_t st) {
   const dim3 gr(blocks(g.N)), b(256);
 #define VH_S(MM, CC)                                                                            \
   if (mode == MM && cm == CC)                                                                   \
-    hipLaunchKernelGGL((k_stage<MM, CC>), gr, b, 0, st, g, eos, Q, Qh, m, m_full, tp, dt, out, \
-                       status);
+    hipLaunchKernelGGL((k_stage<MM, CC>), gr, b, 0, st, g, eos, Q, Qh, gv, m, m_full, tp, dt,  \
+                       out, status);
   VH_S(0, VH_C2P_REFERENCE) VH_S(1, VH_C2P_REFERENCE) VH_S(0, VH_C2P_CLOSED)
-  VH_S(1, VH_C2P_CLOSED)
+  VH_S(1, VH_C2P_CLOSED) VH_S(0, VH_C2P_LEARNED) VH_S(1, VH_C2P_LEARNED)
+  VH_S(0, VH_C2P_NEWTON) VH_S(1, VH_C2P_NEWTON)
 #undef VH_S
 }
 
-void launch_vprim(int cm, const Geom &g, const DevEos &eos, const double *Qh, const double *tp,
-                  double dt, double *ph, double *pt, hipStream_t st) {
-  if (cm == VH_C2P_CLOSED)
-    hipLaunchKernelGGL(k_vprim<VH_C2P_CLOSED>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, Qh,
-                       tp, dt, ph, pt);
-  else
-    hipLaunchKernelGGL(k_vprim<VH_C2P_REFERENCE>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos,
-                       Qh, tp, dt, ph, pt);
+void launch_vprim(int cm, const Geom &g, const DevEos &eos, const double *Qh, const float *gv,
+                  const double *tp, double dt, double *ph, double *pt, hipStream_t st) {
+  const dim3 gr(blocks(g.N)), b(256);
+#define VH_V(CC) \
+  if (cm == CC) hipLaunchKernelGGL(k_vprim<CC>, gr, b, 0, st, g, eos, Qh, gv, tp, dt, ph, pt);
+  VH_V(VH_C2P_REFERENCE) VH_V(VH_C2P_CLOSED) VH_V(VH_C2P_LEARNED) VH_V(VH_C2P_NEWTON)
+#undef VH_V
 }
 
 void launch_visc1(int cm, const Geom &g, const DevEos &eos, const DevTrc &trc,
-                  const double *Qprev, const double *Qn, const double *ph, const double *pt,
-                  const double *m, int m_full, const float *pi, const float *Pi,
-                  const double *tp, double dt, float4 *rec, int *status, hipStream_t st) {
-  if (cm == VH_C2P_CLOSED)
-    hipLaunchKernelGGL(k_visc1<VH_C2P_CLOSED>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, trc,
-                       Qprev, Qn, ph, pt, m, m_full, pi, Pi, tp, dt, rec, status);
-  else
-    hipLaunchKernelGGL(k_visc1<VH_C2P_REFERENCE>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos,
-                       trc, Qprev, Qn, ph, pt, m, m_full, pi, Pi, tp, dt, rec, status);
+                  const double *Qprev, const double *Qn, const float *gv, const double *ph,
+                  const double *pt, const double *m, int m_full, const float *pi,
+                  const float *Pi, const double *tp, double dt, float4 *rec, int *status,
+                  hipStream_t st) {
+  const dim3 gr(blocks(g.N)), b(256);
+#define VH_V(CC)                                                                               \
+  if (cm == CC)                                                                                \
+    hipLaunchKernelGGL(k_visc1<CC>, gr, b, 0, st, g, eos, trc, Qprev, Qn, gv, ph, pt, m, m_full, \
+                       pi, Pi, tp, dt, rec, status);
+  VH_V(VH_C2P_REFERENCE) VH_V(VH_C2P_CLOSED) VH_V(VH_C2P_LEARNED) VH_V(VH_C2P_NEWTON)
+#undef VH_V
 }
 
 void launch_visc2(const Geom &g, const double *ph, const float4 *rec, float *pi, float *piH,
