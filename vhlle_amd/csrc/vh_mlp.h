@@ -81,7 +81,7 @@ __device__ inline float2 mlp_wave(const Packed &w, const float f[5]) {
 #pragma unroll
   for (int k = 0; k < 6; k++) xp[k] = __shfl_xor(x[k], 32, 64);
   float2 res = make_float2(0.f, 0.f);
-#pragma unroll
+#pragma unroll 1
   for (int bt = 0; bt < 2; bt++) {
     // batch bt = the samples of lanes 32 bt .. 32 bt + 31; lanes of the low
     // half carry input rows 0..7 of sample l % 32, the high half rows 8..15 (zero)
