@@ -162,10 +162,15 @@ def run_variant(args, cfg, prec, c2p, ic):
     st2, _ = ctx.run(tau0, dt, args.steps)
     kt = ctx.kernel_times()
     ctx.enable_timing(False)
+    stage = None
+    if p == abi.PREC_MIXED:
+        k, tx, ty, cz, ms = ctx.get_stage()
+        stage = dict(kernel={1: "cells", 2: "rows", 3: "tiles"}.get(k, "auto"), TX=tx, TY=ty,
+                     CZ=cz, tuned_pred_ms=round(ms, 4))
     ctx.close()
     print(f"[bench] {prec}/{c2p}: {args.steps} steps in {elapsed:.3f} s, status {st | st2}",
           file=sys.stderr, flush=True)
-    return dict(elapsed=elapsed, status=st | st2, kernel_times=kt, tau=tau)
+    return dict(elapsed=elapsed, status=st | st2, kernel_times=kt, tau=tau, stage=stage)
 
 
 def error_run(cfg, prec, c2p, ic, steps):
@@ -278,6 +283,8 @@ def main():
             speedup_vs_fp64=round(v / (N * args.steps / results["fp64"]["elapsed"]), 3),
             roofline=roofline(r["kernel_times"], N, prec),
             kernels_ms={k: round(t[1], 3) for k, t in sorted(r["kernel_times"].items())})
+        if r["stage"]:
+            variants[name]["stage_kernel"] = r["stage"]
     # accepted: T error within 1e-6 or within twice the FP64 scheme's own
     # sensitivity to a 1e-12 perturbation of its initial state
     bar = max(ERR_BAR, 2.0 * floor[0])

@@ -139,6 +139,20 @@ int vh_check_dtau(const vh_ctx_t *ctx, double dt);
 /* Selects the con2prim algorithm for later steps (VH_C2P_*). */
 int vh_set_c2p_mode(vh_ctx_t *ctx, int mode);
 
+/* Shape of the mixed-precision stage kernel (no counterpart in the
+ * reference).  VH_STAGE_CELLS: one thread per cell evaluating its six
+ * interfaces; VH_STAGE_ROWS: waves of 64 consecutive cells of the (x, y)
+ * plane marching CZ cells in eta, every x- and eta-interface evaluated once;
+ * VH_STAGE_TILES: blocks of TX x TY (x, y) columns (TX*TY <= 512) marching CZ
+ * cells in eta, every interface evaluated once, shared through LDS.  Every
+ * shape gives identical results.  VH_STAGE_AUTO (default): an autotuner times
+ * the candidates on the first vh_step/vh_run (and after vh_set_c2p_mode) and
+ * keeps the fastest.  vh_get_stage reports the shape in use (kind
+ * VH_STAGE_AUTO: not tuned yet) and the tuned stage time in ms.  Mixed
+ * precision only (VH_ERR_ARG otherwise). */
+enum { VH_STAGE_AUTO = 0, VH_STAGE_CELLS = 1, VH_STAGE_ROWS = 2, VH_STAGE_TILES = 3 };
+int vh_set_stage(vh_ctx_t *ctx, int kind, int TX, int TY, int CZ);
+int vh_get_stage(const vh_ctx_t *ctx, int *kind, int *TX, int *TY, int *CZ, double *ms);
 /* Per-kernel timing over the next vh_run: enable=1 records HIP events around
  * every launch on the context's stream; vh_kernel_times returns, for up to
  * max kernels, the name, launch count and summed milliseconds. */

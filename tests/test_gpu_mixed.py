@@ -141,3 +141,38 @@ def test_mixed_graph_equals_steps(gpu):
 def test_fp64_path_refuses_non_reference_c2p(gpu):
     ctx = abi.Context(8, 8, 1, (-1, 1), (-1, 1), (-1, 1))
     assert abi.lib().vh_set_c2p_mode(ctx.h, abi.C2P_CLOSED) == abi.VH_ERR_ARG
+
+
+@pytest.mark.parametrize("dims,c2p", [((37, 29, 13), abi.C2P_CLOSED), ((41, 41, 1), abi.C2P_CLOSED),
+                                      ((24, 20, 9), abi.C2P_LEARNED)])
+def test_tile_shapes_identical(gpu, dims, c2p):
+    """Every stage kernel shape (rows and block tiles, each interface evaluated
+    once; autotuned or forced) gives bit-identical states to the per-cell
+    kernel."""
+    nx, ny, nz = dims
+    xl, yl, zl = (-4.0, 4.0), (-3.5, 3.5), (-1.2, 1.2)
+    ic = setups.gaussian3d_ic(nx, ny, nz, xl, yl, zl, e0=30.0, sx=1.5, sy=1.2, eta_plateau=0.4,
+                              eta_sigma=0.4)
+    shapes = [(abi.STAGE_CELLS, 0, 0, 0), (abi.STAGE_AUTO, 0, 0, 0), (abi.STAGE_ROWS, 0, 0, 1),
+              (abi.STAGE_ROWS, 0, 0, 5), (abi.STAGE_ROWS, 0, 0, 32), (abi.STAGE_TILES, 8, 4, 1),
+              (abi.STAGE_TILES, 13, 7, 5), (abi.STAGE_TILES, 38, 10, 13),
+              (abi.STAGE_TILES, 64, 8, 4)]
+    model = abi.mlp_init(9) if c2p == abi.C2P_LEARNED else None
+    ref = None
+    for sh in shapes:
+        ctx = abi.Context(nx, ny, nz, xl, yl, zl, trc=abi.make_trc(etaS=0.1),
+                          precision=abi.PREC_MIXED)
+        ctx.set_c2p(c2p, model)
+        ctx.set_stage(*sh)
+        ctx.set_prim(0.6, *ic)
+        st, _ = ctx.run(0.6, 0.02, 6)
+        assert st == 0
+        tile = ctx.get_stage()
+        s = ctx.download()
+        ctx.close()
+        if ref is None:
+            ref = s
+            continue
+        for f in ("Q", "pi", "Pi"):
+            d = np.max(np.abs(s[f] - ref[f])) / max(np.max(np.abs(ref[f])), 1e-300)
+            assert np.array_equal(s[f], ref[f]), (sh, tile, f, d)

@@ -16,6 +16,7 @@ ST_NAN_FLUX, ST_BIG_E, ST_NAN_SOURCE, ST_NAN_PI = 1, 2, 4, 8
 PREC_FP64, PREC_MIXED = 0, 1
 C2P_REFERENCE, C2P_LEARNED, C2P_CLOSED, C2P_NEWTON = 0, 1, 2, 3
 START_MLP, START_HALF = 0, 1
+STAGE_AUTO, STAGE_CELLS, STAGE_ROWS, STAGE_TILES = 0, 1, 2, 3
 MLP_NPARAM = 4674
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -112,6 +113,8 @@ def lib():
                                           ctypes.c_int, _dp, ip]),
         "vh_set_c2p_model": (ctypes.c_int, [vp, _fp]),
         "vh_mlp_harvest": (ctypes.c_int, [ctypes.POINTER(HarvestCfg), ctypes.c_long, _dp, _dp]),
+        "vh_set_stage": (ctypes.c_int, [vp] + [ctypes.c_int] * 4),
+        "vh_get_stage": (ctypes.c_int, [vp, ip, ip, ip, ip, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -126,7 +129,7 @@ def exported_symbols():
             "vh_step", "vh_run", "vh_check_dtau", "vh_set_c2p_mode", "vh_enable_timing",
             "vh_kernel_times", "vh_c2p", "vh_p2c", "vh_device_info", "vh_mlp_init",
             "vh_mlp_forward", "vh_mlp_grad", "vh_mlp_train", "vh_c2p_learned",
-            "vh_set_c2p_model", "vh_mlp_harvest"]
+            "vh_set_c2p_model", "vh_mlp_harvest", "vh_set_stage", "vh_get_stage"]
 
 
 def _p(a):
@@ -299,6 +302,16 @@ class Context:
             p = np.ascontiguousarray(params, dtype=np.float32)
             check(lib().vh_set_c2p_model(self.h, _f(p)), "vh_set_c2p_model")
         check(lib().vh_set_c2p_mode(self.h, mode), "vh_set_c2p_mode")
+
+    def set_stage(self, kind=0, TX=0, TY=0, CZ=0):
+        check(lib().vh_set_stage(self.h, kind, TX, TY, CZ), "vh_set_stage")
+
+    def get_stage(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        ms = ctypes.c_double()
+        check(lib().vh_get_stage(self.h, *[ctypes.byref(x) for x in v], ctypes.byref(ms)),
+              "vh_get_stage")
+        return tuple(x.value for x in v) + (ms.value,)
 
     def enable_timing(self, on=True):
         check(lib().vh_enable_timing(self.h, 1 if on else 0), "vh_enable_timing")

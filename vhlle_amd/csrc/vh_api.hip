@@ -66,6 +66,10 @@ struct vh_ctx {
   // guesses (mixed precision only; gv reuses the FP64 path's F[0] storage)
   vh::ml::Packed *mlp = nullptr;
   float *gv = nullptr;
+  // stage kernel shape (mixed precision): kind 0 until the autotuner ran
+  vh::mx::Tile tile = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  bool tile_forced = false;
+  double tile_ms = 0.;
   int cur = 0;
   int m_full = 0;
   // graph cache: one executable per parity of cur
@@ -165,16 +169,18 @@ int enqueue_step_mixed(vh_ctx *c, double dt) {
   // Qh/tau_n: both stages reconstruct on tau_n-scaled states)
   if (learned)
     rc |= timed(c, "c2p_guess", [&] { vh::ml::launch_guess(Qo, g.N, c->tp, 0., c->mlp, c->gv, st); });
-  rc |= timed(c, "stage_pred", [&] {
-    vh::mx::launch_stage(0, cm, g, c->eos, Qo, c->Qh, c->gv, c->m, c->m_full, c->tp, dt, c->Qh,
-                         c->status, st);
-  });
+  auto stage = [&](int mode, double *out) {
+    if (c->tile.kind)
+      vh::mx::launch_stage_tile(mode, cm, g, c->eos, Qo, c->Qh, c->gv, c->m, c->m_full, c->tp,
+                                dt, out, c->status, c->tile, st);
+    else
+      vh::mx::launch_stage(mode, cm, g, c->eos, Qo, c->Qh, c->gv, c->m, c->m_full, c->tp, dt,
+                           out, c->status, st);
+  };
+  rc |= timed(c, "stage_pred", [&] { stage(0, c->Qh); });
   if (learned)
     rc |= timed(c, "c2p_guess", [&] { vh::ml::launch_guess(c->Qh, g.N, c->tp, 0., c->mlp, c->gv, st); });
-  rc |= timed(c, "stage_corr", [&] {
-    vh::mx::launch_stage(1, cm, g, c->eos, Qo, c->Qh, c->gv, c->m, c->m_full, c->tp, dt, Qn,
-                         c->status, st);
-  });
+  rc |= timed(c, "stage_corr", [&] { stage(1, Qn); });
   rc |= timed(c, "advance_tau", [&] { vh::launch_advance_tau(c->tp, dt, st); });
   ghost_fill(c, 0, Qn, st, rc);
   if (c->viscous) {
@@ -534,7 +540,45 @@ int vh_set_c2p_mode(vh_ctx_t *c, int mode) {
     return VH_ERR_ARG;
   if (mode == VH_C2P_LEARNED && !c->mlp) return VH_ERR_ARG;  // no model installed
   c->c2p_mode = mode;
+  if (!c->tile_forced) c->tile.kind = 0;  // retune for this con2prim
   drop_graphs(c);
+  return VH_OK;
+}
+
+int vh_set_stage(vh_ctx_t *c, int kind, int TX, int TY, int CZ) {
+  if (!c || c->precision != VH_PREC_MIXED) return VH_ERR_ARG;
+  vh::mx::Tile t;
+  if (kind == VH_STAGE_AUTO) {
+    c->tile.kind = 0;
+    c->tile_forced = false;
+    drop_graphs(c);
+    return VH_OK;
+  } else if (kind == VH_STAGE_CELLS) {
+    t = vh::mx::make_cells();
+  } else if (kind == VH_STAGE_ROWS) {
+    if (CZ < 1) return VH_ERR_ARG;
+    t = vh::mx::make_rows(c->g, CZ);
+  } else if (kind == VH_STAGE_TILES) {
+    if (TX < 1 || TY < 1 || CZ < 1) return VH_ERR_ARG;
+    t = vh::mx::make_tile(c->g, TX, TY, CZ);
+  } else {
+    return VH_ERR_ARG;
+  }
+  if (!vh::mx::tile_ok(t)) return VH_ERR_ARG;
+  c->tile = t;
+  c->tile_forced = true;
+  c->tile_ms = 0.;
+  drop_graphs(c);
+  return VH_OK;
+}
+
+int vh_get_stage(const vh_ctx_t *c, int *kind, int *TX, int *TY, int *CZ, double *ms) {
+  if (!c || !kind || !TX || !TY || !CZ) return VH_ERR_ARG;
+  *kind = c->tile.kind;
+  *TX = c->tile.TX;
+  *TY = c->tile.TY;
+  *CZ = c->tile.CZ;
+  if (ms) *ms = c->tile_ms;
   return VH_OK;
 }
 
@@ -549,8 +593,59 @@ int vh_set_c2p_model(vh_ctx_t *c, const float *params) {
   return VH_OK;
 }
 
+// Autotuner of the stage kernel: times the predictor stage of every candidate
+// shape (vh::mx::tile_candidates) on the current state, writing into scratch
+// memory, and keeps the fastest.  Every shape gives the same result.
+static int ensure_tile(vh_ctx_t *c, double tau, double dt) {
+  if (c->precision != VH_PREC_MIXED || c->tile.kind || c->tile_forced) return VH_OK;
+  const std::vector<vh::mx::Tile> cand = vh::mx::tile_candidates(c->g);
+  const vh::Geom &g = c->g;
+  hipStream_t st = c->st;
+  double *so = c->F[1];                               // unused by the mixed path
+  int *ss = reinterpret_cast<int *>(c->fl[0]);        // scratch status word
+  VH_HIP(hipMemcpyAsync(c->tp, &tau, sizeof(double), hipMemcpyHostToDevice, st));
+  if (c->c2p_mode == VH_C2P_LEARNED)
+    vh::ml::launch_guess(c->Q[c->cur], g.N, c->tp, 0., c->mlp, c->gv, st);
+  hipEvent_t e0, e1;
+  VH_HIP(hipEventCreate(&e0));
+  VH_HIP(hipEventCreate(&e1));
+  float best = 3.0e38f;
+  vh::mx::Tile bt = cand[0];
+  int rc = VH_OK;
+  for (const vh::mx::Tile &t : cand) {
+    auto go = [&] {
+      vh::mx::launch_stage_tile(0, c->c2p_mode, g, c->eos, c->Q[c->cur], c->Qh, c->gv, c->m,
+                                c->m_full, c->tp, dt, so, ss, t, st);
+    };
+    go();
+    if (hipEventRecord(e0, st) != hipSuccess) rc = VH_ERR_HIP;
+    for (int r = 0; r < 3; r++) go();
+    if (hipEventRecord(e1, st) != hipSuccess) rc = VH_ERR_HIP;
+    float ms = 0.f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      rc = VH_ERR_HIP;
+    if (rc != VH_OK) break;
+    if (ms < best) {
+      best = ms;
+      bt = t;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != VH_OK) return rc;
+  VH_HIP(hipGetLastError());
+  c->tile = bt;
+  c->tile_ms = best / 3.;
+  drop_graphs(c);
+  return VH_OK;
+}
+
 static int run_impl(vh_ctx_t *c, double *tau, double dt, int nsteps, int *status, bool graph) {
   if (!c || !tau || nsteps < 0) return VH_ERR_ARG;
+  {
+    int rc = ensure_tile(c, *tau, dt);
+    if (rc != VH_OK) return rc;
+  }
   const bool use_graph = graph && !c->timing && nsteps > 1;
   if (use_graph) {  // capture before any work of this call is enqueued
     int rc = build_graphs(c, dt);

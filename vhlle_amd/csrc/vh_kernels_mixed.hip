@@ -18,6 +18,9 @@
 #include "vh_kernels_mixed.h"
 #include "vh_mlp.h"
 
+#include <algorithm>
+#include <vector>
+
 namespace vh {
 namespace mx {
 
@@ -297,31 +300,14 @@ __device__ __forceinline__ void axis_fluxes(const Geom &g, const DevEos &eos,
   }
 }
 
-// Fused stage: interface fluxes along x, y, eta + Milne source + update.
-// MODE 0 (predictor): reads Q, writes Qh = Q + flux.
-// MODE 1 (corrector): reads Q and Qh, writes Qnew = Q + flux unless
-// Q0 + flux0 < 0 (src/cll.cpp:70-74).
+// Milne source on the unreconstructed state (src/hdo.cpp:307-353) and the
+// stage update of cell c with the accumulated interface fluxes acc.
 template <int MODE, int CM>
-__global__ void __launch_bounds__(256)
-k_stage(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
-        const float *__restrict__ gv, const double *__restrict__ m, int m_full,
-        const double *__restrict__ tp, double dt, double *__restrict__ out,
-        int *__restrict__ status) {
-  const double tau = *tp;
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.N) return;
-  int ix, iy, iz;
-  coords_of(g, c, ix, iy, iz);
-  const long N = g.N;
-  const double *Qs = MODE == 0 ? Q : Qh;
-  double acc[7];
-#pragma unroll
-  for (int k = 0; k < 7; k++) acc[k] = 0.;
-  axis_fluxes<1, MODE, CM>(g, eos, Qs, gv, m, m_full, ix, g.nx, c, 1, tau, dt, acc, status);
-  axis_fluxes<2, MODE, CM>(g, eos, Qs, gv, m, m_full, iy, g.ny, c, g.nx, tau, dt, acc, status);
-  axis_fluxes<3, MODE, CM>(g, eos, Qs, gv, m, m_full, iz, g.nz, c, (long)g.nx * g.ny, tau, dt, acc,
-                           status);
-  // Milne source on the unreconstructed state (src/hdo.cpp:307-353)
+__device__ __forceinline__ void finish_cell(const DevEos &eos, const double *__restrict__ Q,
+                                            const double *__restrict__ Qs,
+                                            const float *__restrict__ gv, long N, long c,
+                                            double tau, double dt, double acc[7],
+                                            double *__restrict__ out) {
   const double tau1 = MODE == 0 ? tau : tau + 0.5 * dt;
   const double sdt = MODE == 0 ? 0.5 * dt : dt;
   const double it1 = 1. / tau1;
@@ -348,6 +334,239 @@ k_stage(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restri
     const bool skip = Q[c] + acc[0] < 0.;
 #pragma unroll
     for (int k = 0; k < 7; k++) out[k * N + c] = skip ? Q[k * N + c] : Q[k * N + c] + acc[k];
+  }
+}
+
+// Fused stage: interface fluxes along x, y, eta + Milne source + update.
+// MODE 0 (predictor): reads Q, writes Qh = Q + flux.
+// MODE 1 (corrector): reads Q and Qh, writes Qnew = Q + flux unless
+// Q0 + flux0 < 0 (src/cll.cpp:70-74).
+template <int MODE, int CM>
+__global__ void __launch_bounds__(256)
+k_stage(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
+        const float *__restrict__ gv, const double *__restrict__ m, int m_full,
+        const double *__restrict__ tp, double dt, double *__restrict__ out,
+        int *__restrict__ status) {
+  const double tau = *tp;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  int ix, iy, iz;
+  coords_of(g, c, ix, iy, iz);
+  const long N = g.N;
+  const double *Qs = MODE == 0 ? Q : Qh;
+  double acc[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) acc[k] = 0.;
+  axis_fluxes<1, MODE, CM>(g, eos, Qs, gv, m, m_full, ix, g.nx, c, 1, tau, dt, acc, status);
+  axis_fluxes<2, MODE, CM>(g, eos, Qs, gv, m, m_full, iy, g.ny, c, g.nx, tau, dt, acc, status);
+  axis_fluxes<3, MODE, CM>(g, eos, Qs, gv, m, m_full, iz, g.nz, c, (long)g.nx * g.ny, tau, dt, acc,
+                           status);
+  finish_cell<MODE, CM>(eos, Q, Qs, gv, N, c, tau, dt, acc, out);
+}
+
+// Interface flux between cells a (left) and b (right) along axis D: MUSCL
+// states of a (neighbours amm, b) and b (neighbours a, bp), HLLE in FP32.
+// Same states, mask and flux as the interfaces of axis_fluxes.
+template <int D, int CM>
+__device__ __forceinline__ bool iface(const DevEos &eos, const double *__restrict__ Qs,
+                                      const float *__restrict__ gv,
+                                      const double *__restrict__ m, int m_full, long N, long amm,
+                                      long a, long b, long bp, double itau, float itauf,
+                                      float itf, float pre, float F[7], int *status) {
+  const PrimF L = face_state<+1, CM>(eos, Qs, gv, N, amm, a, b, itau);
+  const PrimF R = face_state<-1, CM>(eos, Qs, gv, N, a, b, bp, itau);
+  const bool mk = !m_full && m[(D - 1) * N + a] < 1. && m[(D - 1) * N + b] < 1.;
+  return hlle_f32<D>(L, R, itauf, itf, pre, mk, F, status);
+}
+
+// Tiled stage: every interface is evaluated once.  A block is a TX x TY tile
+// of (x, y) columns marching over CZ cells in eta.  Thread (tx, ty) evaluates
+// the lower x- and y-interfaces of its cell and shares them through LDS; its
+// upper eta-interface stays in registers and is the lower one of the next
+// cell of the column.  The last tile column and row (the halo, hx = hy = 1
+// when the axis has interfaces) only supply interfaces, so tiles advance by
+// TX - hx and TY - hy cells.  Interface states, fluxes and the per-cell
+// accumulation order (+x-, -x+, +y-, -y+, +eta-, -eta+, source) are those of
+// k_stage: both kernels give identical results for every tile shape.
+template <int MODE, int CM>
+__global__ void __launch_bounds__(512)
+k_stage_tile(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
+             const float *__restrict__ gv, const double *__restrict__ m, int m_full,
+             const double *__restrict__ tp, double dt, double *__restrict__ out,
+             int *__restrict__ status, Tile tl) {
+  extern __shared__ float sh[];
+  const int T = tl.TX * tl.TY;
+  float *fxs = sh, *fys = sh + 7 * T;
+  int *fls = reinterpret_cast<int *>(sh + 14 * T);
+  const int t = threadIdx.x;
+  const int tx = t % tl.TX, ty = t / tl.TX;
+  const int bx = blockIdx.x % tl.ntx, rr = blockIdx.x / tl.ntx;
+  const int by = rr % tl.nty, bz = rr / tl.nty;
+  const int ix = bx * (tl.TX - tl.hx) + tx, iy = by * (tl.TY - tl.hy) + ty;
+  const bool cx = t < T && ix < g.nx && iy < g.ny;
+  const bool upd = cx && tx < tl.TX - tl.hx && ty < tl.TY - tl.hy;
+  const int z0 = bz * tl.CZ, z1 = min(g.nz, z0 + tl.CZ);
+  const long N = g.N, sy = g.nx, sz = (long)g.nx * g.ny;
+  const double tau = *tp;
+  const double *Qs = MODE == 0 ? Q : Qh;
+  const double itau = 1. / tau;
+  const double tauFactor = MODE == 0 ? tau + 0.25 * dt : tau + 0.5 * dt;
+  const double dta = MODE == 0 ? 0.5 * dt : dt;
+  const float prx = (float)(tauFactor * dta / g.dx), pry = (float)(tauFactor * dta / g.dy),
+              prz = (float)(tauFactor * dta / g.dz);
+  const float itf = (float)(1. / tauFactor), itauf = (float)itau;
+  float fzl[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float fzu[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  bool hzl = false;
+  for (int iz = z0; iz < z1; iz++) {
+    const long c = cx ? g.idx(ix, iy, iz) : 0;
+    float fx[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float fy[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    bool hx = false, hy = false, hzu = false;
+    if (cx && ix >= 1)
+      hx = iface<1, CM>(eos, Qs, gv, m, m_full, N, ix >= 2 ? c - 2 : c - 1, c - 1, c,
+                        ix + 1 < g.nx ? c + 1 : c, itau, itauf, itf, prx, fx, status);
+    if (cx && iy >= 1)
+      hy = iface<2, CM>(eos, Qs, gv, m, m_full, N, iy >= 2 ? c - 2 * sy : c - sy, c - sy, c,
+                        iy + 1 < g.ny ? c + sy : c, itau, itauf, itf, pry, fy, status);
+    if (iz == z0 && cx && iz >= 1)
+      hzl = iface<3, CM>(eos, Qs, gv, m, m_full, N, iz >= 2 ? c - 2 * sz : c - sz, c - sz, c,
+                         iz + 1 < g.nz ? c + sz : c, itau, itauf, itf, prz, fzl, status);
+    if (cx && iz + 1 < g.nz)
+      hzu = iface<3, CM>(eos, Qs, gv, m, m_full, N, iz >= 1 ? c - sz : c, c, c + sz,
+                         iz + 2 < g.nz ? c + 2 * sz : c + sz, itau, itauf, itf, prz, fzu,
+                         status);
+    __syncthreads();  // the previous slice's reads of the shared interfaces are done
+    if (t < T) {
+#pragma unroll
+      for (int k = 0; k < 7; k++) {
+        fxs[k * T + t] = fx[k];
+        fys[k * T + t] = fy[k];
+      }
+      fls[t] = (hx ? 1 : 0) | (hy ? 2 : 0);
+    }
+    __syncthreads();
+    if (upd) {
+      double acc[7];
+#pragma unroll
+      for (int k = 0; k < 7; k++) acc[k] = 0.;
+      if (hx)
+#pragma unroll
+        for (int k = 0; k < 7; k++) acc[k] += (double)fx[k];
+      if (ix + 1 < g.nx && (fls[t + 1] & 1))
+#pragma unroll
+        for (int k = 0; k < 7; k++) acc[k] -= (double)fxs[k * T + t + 1];
+      if (hy)
+#pragma unroll
+        for (int k = 0; k < 7; k++) acc[k] += (double)fy[k];
+      if (iy + 1 < g.ny && (fls[t + tl.TX] & 2))
+#pragma unroll
+        for (int k = 0; k < 7; k++) acc[k] -= (double)fys[k * T + t + tl.TX];
+      if (hzl)
+#pragma unroll
+        for (int k = 0; k < 7; k++) acc[k] += (double)fzl[k];
+      if (hzu)
+#pragma unroll
+        for (int k = 0; k < 7; k++) acc[k] -= (double)fzu[k];
+      finish_cell<MODE, CM>(eos, Q, Qs, gv, N, c, tau, dt, acc, out);
+    }
+#pragma unroll
+    for (int k = 0; k < 7; k++) fzl[k] = fzu[k];
+    hzl = hzu;
+  }
+}
+
+// Row stage: every x- and eta-interface is evaluated once, without barriers.
+// A wave owns 64 consecutive cells of the flattened (x, y) plane (waves
+// overlap by one cell, so 63 of 64 lanes update a cell whatever nx is) and
+// marches over CZ cells in eta.  Lane l evaluates the lower x-interface of its
+// cell and receives the upper one from lane l + 1 by a cross-lane shuffle; the
+// upper eta-interface stays in registers as the next cell's lower one; the
+// two y-interfaces are evaluated per lane.  Interface states, fluxes and the
+// accumulation order are those of k_stage: identical results.
+template <int MODE, int CM>
+__global__ void __launch_bounds__(256)
+k_stage_rows(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
+             const float *__restrict__ gv, const double *__restrict__ m, int m_full,
+             const double *__restrict__ tp, double dt, double *__restrict__ out,
+             int *__restrict__ status, Tile tl) {
+  const int lane = threadIdx.x & 63;
+  const long w = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long nxy = (long)g.nx * g.ny;
+  const long seg = w % tl.ntx, zc = w / tl.ntx;
+  const int z0 = (int)zc * tl.CZ;
+  if (z0 >= g.nz) return;  // whole wave
+  const int z1 = min(g.nz, z0 + tl.CZ);
+  const long f = seg * 63 + lane;
+  const bool cx = f < nxy;
+  const bool upd = cx && lane < 63;
+  const int ix = cx ? (int)(f % g.nx) : 0, iy = cx ? (int)(f / g.nx) : 0;
+  const long N = g.N, sy = g.nx, sz = nxy;
+  const double tau = *tp;
+  const double *Qs = MODE == 0 ? Q : Qh;
+  const double itau = 1. / tau;
+  const double tauFactor = MODE == 0 ? tau + 0.25 * dt : tau + 0.5 * dt;
+  const double dta = MODE == 0 ? 0.5 * dt : dt;
+  const float prx = (float)(tauFactor * dta / g.dx), pry = (float)(tauFactor * dta / g.dy),
+              prz = (float)(tauFactor * dta / g.dz);
+  const float itf = (float)(1. / tauFactor), itauf = (float)itau;
+  float fzl[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float fzu[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  bool hzl = false;
+  for (int iz = z0; iz < z1; iz++) {
+    const long c = cx ? f + iz * sz : 0;
+    float fx[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float fyl[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float fyu[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    bool hx = false, hyl = false, hyu = false, hzu = false;
+    if (cx && ix >= 1)
+      hx = iface<1, CM>(eos, Qs, gv, m, m_full, N, ix >= 2 ? c - 2 : c - 1, c - 1, c,
+                        ix + 1 < g.nx ? c + 1 : c, itau, itauf, itf, prx, fx, status);
+    float fxu[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++) fxu[k] = __shfl_down(fx[k], 1, 64);
+    const bool hxu = __shfl_down(hx ? 1 : 0, 1, 64) != 0 && ix + 1 < g.nx;
+    if (cx && iy >= 1)
+      hyl = iface<2, CM>(eos, Qs, gv, m, m_full, N, iy >= 2 ? c - 2 * sy : c - sy, c - sy, c,
+                         iy + 1 < g.ny ? c + sy : c, itau, itauf, itf, pry, fyl, status);
+    if (cx && iy + 1 < g.ny)
+      hyu = iface<2, CM>(eos, Qs, gv, m, m_full, N, iy >= 1 ? c - sy : c, c, c + sy,
+                         iy + 2 < g.ny ? c + 2 * sy : c + sy, itau, itauf, itf, pry, fyu,
+                         status);
+    if (iz == z0 && cx && iz >= 1)
+      hzl = iface<3, CM>(eos, Qs, gv, m, m_full, N, iz >= 2 ? c - 2 * sz : c - sz, c - sz, c,
+                         iz + 1 < g.nz ? c + sz : c, itau, itauf, itf, prz, fzl, status);
+    if (cx && iz + 1 < g.nz)
+      hzu = iface<3, CM>(eos, Qs, gv, m, m_full, N, iz >= 1 ? c - sz : c, c, c + sz,
+                         iz + 2 < g.nz ? c + 2 * sz : c + sz, itau, itauf, itf, prz, fzu,
+                         status);
+    if (upd) {
+      double acc[7];
+#pragma unroll
+      for (int k = 0; k < 7; k++) acc[k] = 0.;
+      if (hx)
+#pragma unroll
+        for (int k = 0; k < 7; k++) acc[k] += (double)fx[k];
+      if (hxu)
+#pragma unroll
+        for (int k = 0; k < 7; k++) acc[k] -= (double)fxu[k];
+      if (hyl)
+#pragma unroll
+        for (int k = 0; k < 7; k++) acc[k] += (double)fyl[k];
+      if (hyu)
+#pragma unroll
+        for (int k = 0; k < 7; k++) acc[k] -= (double)fyu[k];
+      if (hzl)
+#pragma unroll
+        for (int k = 0; k < 7; k++) acc[k] += (double)fzl[k];
+      if (hzu)
+#pragma unroll
+        for (int k = 0; k < 7; k++) acc[k] -= (double)fzu[k];
+      finish_cell<MODE, CM>(eos, Q, Qs, gv, N, c, tau, dt, acc, out);
+    }
+#pragma unroll
+    for (int k = 0; k < 7; k++) fzl[k] = fzu[k];
+    hzl = hzu;
   }
 }
 
@@ -865,6 +1084,125 @@ void launch_stage(int mode, int cm, const Geom &g, const DevEos &eos, const doub
   if (mode == MM && cm == CC)                                                                   \
     hipLaunchKernelGGL((k_stage<MM, CC>), gr, b, 0, st, g, eos, Q, Qh, gv, m, m_full, tp, dt,  \
                        out, status);
+  VH_S(0, VH_C2P_REFERENCE) VH_S(1, VH_C2P_REFERENCE) VH_S(0, VH_C2P_CLOSED)
+  VH_S(1, VH_C2P_CLOSED) VH_S(0, VH_C2P_LEARNED) VH_S(1, VH_C2P_LEARNED)
+  VH_S(0, VH_C2P_NEWTON) VH_S(1, VH_C2P_NEWTON)
+#undef VH_S
+}
+
+Tile make_tile(const Geom &g, int TX, int TY, int CZ) {
+  Tile t;
+  t.kind = STAGE_TILES;
+  t.hx = g.nx >= 2 ? 1 : 0;
+  t.hy = g.ny >= 2 ? 1 : 0;
+  t.TX = g.nx >= 2 ? TX : 1;
+  t.TY = g.ny >= 2 ? TY : 1;
+  t.CZ = CZ < 1 ? 1 : (CZ > g.nz ? g.nz : CZ);
+  const int sx = t.TX - t.hx, sy = t.TY - t.hy;
+  t.ntx = sx > 0 ? (g.nx + sx - 1) / sx : 0;
+  t.nty = sy > 0 ? (g.ny + sy - 1) / sy : 0;
+  t.ntz = (g.nz + t.CZ - 1) / t.CZ;
+  return t;
+}
+
+Tile make_rows(const Geom &g, int CZ) {
+  Tile t;
+  t.kind = STAGE_ROWS;
+  t.TX = 64;
+  t.TY = 1;
+  t.hx = t.hy = 0;
+  t.CZ = CZ < 1 ? 1 : (CZ > g.nz ? g.nz : CZ);
+  const long nxy = (long)g.nx * g.ny;
+  t.ntx = (int)((nxy + 62) / 63);  // segment s updates cells [63 s, 63 s + 63)
+  t.nty = 1;
+  t.ntz = (g.nz + t.CZ - 1) / t.CZ;
+  return t;
+}
+
+Tile make_cells() {
+  Tile t = {STAGE_CELLS, 0, 0, 0, 0, 0, 0, 0, 0};
+  return t;
+}
+
+bool tile_ok(const Tile &t) {
+  if (t.kind == STAGE_CELLS) return true;
+  if (t.kind == STAGE_ROWS) return t.CZ >= 1 && t.ntx > 0 && (long)t.ntx * t.ntz < (1L << 33);
+  return t.kind == STAGE_TILES && t.TX >= 1 && t.TY >= 1 && t.TX * t.TY <= 512 &&
+         t.TX - t.hx >= 1 && t.TY - t.hy >= 1 && t.ntx > 0 && t.nty > 0 && t.ntz > 0 &&
+         (long)t.ntx * t.nty * t.ntz < (1L << 31);
+}
+
+std::vector<Tile> tile_candidates(const Geom &g) {
+  std::vector<Tile> out;
+  out.push_back(make_cells());
+  // rows: eta chunks of 1..32 cells (distinct after clamping to nz)
+  const int czs[] = {1, 2, 4, 8, 16, 32};
+  for (int cz : czs) {
+    const Tile t = make_rows(g, cz);
+    bool dup = false;
+    for (auto &o : out) dup |= o.kind == STAGE_ROWS && o.CZ == t.CZ;
+    if (!dup) out.push_back(t);
+  }
+  // block tiles that cover the x extent with little waste, ranked by the
+  // fraction of launched threads that update a cell; the best three
+  std::vector<std::pair<double, Tile>> c;
+  const int tys[] = {4, 6, 8, 10, 12, 16};
+  const int tzs[] = {4, 8, 16};
+  for (int ty : tys)
+    for (int kx = 1; kx <= 16; kx++) {
+      const int tx = g.nx >= 2 ? (g.nx + kx - 1) / kx + 1 : 1;
+      if (tx * ty > 512 || tx < 8) continue;
+      for (int cz : tzs) {
+        const Tile t = make_tile(g, tx, ty, cz);
+        if (!tile_ok(t)) continue;
+        const double launched = (double)t.ntx * t.nty * t.ntz * ((t.TX * t.TY + 63) / 64 * 64);
+        const double zextra = 1.0 + (g.nz > 1 ? 1.0 / t.CZ : 0.0);
+        c.push_back({(double)g.N / launched / zextra, t});
+      }
+    }
+  std::sort(c.begin(), c.end(), [](const std::pair<double, Tile> &a,
+                                   const std::pair<double, Tile> &b) { return a.first > b.first; });
+  int added = 0;
+  for (auto &p : c) {
+    bool dup = false;
+    for (auto &o : out)
+      dup |= o.kind == STAGE_TILES && o.TX == p.second.TX && o.TY == p.second.TY &&
+             o.CZ == p.second.CZ;
+    if (dup) continue;
+    out.push_back(p.second);
+    if (++added >= 3) break;
+  }
+  return out;
+}
+
+void launch_stage_tile(int mode, int cm, const Geom &g, const DevEos &eos, const double *Q,
+                       const double *Qh, const float *gv, const double *m, int m_full,
+                       const double *tp, double dt, double *out, int *status, const Tile &tl,
+                       hipStream_t st) {
+  if (tl.kind == STAGE_CELLS) {
+    launch_stage(mode, cm, g, eos, Q, Qh, gv, m, m_full, tp, dt, out, status, st);
+    return;
+  }
+  if (tl.kind == STAGE_ROWS) {
+    const long waves = (long)tl.ntx * tl.ntz;
+    const dim3 gr((unsigned)((waves + 3) / 4)), b(256);
+#define VH_S(MM, CC)                                                                          \
+  if (mode == MM && cm == CC)                                                                 \
+    hipLaunchKernelGGL((k_stage_rows<MM, CC>), gr, b, 0, st, g, eos, Q, Qh, gv, m, m_full,   \
+                       tp, dt, out, status, tl);
+    VH_S(0, VH_C2P_REFERENCE) VH_S(1, VH_C2P_REFERENCE) VH_S(0, VH_C2P_CLOSED)
+    VH_S(1, VH_C2P_CLOSED) VH_S(0, VH_C2P_LEARNED) VH_S(1, VH_C2P_LEARNED)
+    VH_S(0, VH_C2P_NEWTON) VH_S(1, VH_C2P_NEWTON)
+#undef VH_S
+    return;
+  }
+  const int T = tl.TX * tl.TY;
+  const dim3 gr((unsigned)((long)tl.ntx * tl.nty * tl.ntz)), b((unsigned)((T + 63) / 64 * 64));
+  const size_t lds = (size_t)15 * T * sizeof(float);
+#define VH_S(MM, CC)                                                                          \
+  if (mode == MM && cm == CC)                                                                 \
+    hipLaunchKernelGGL((k_stage_tile<MM, CC>), gr, b, lds, st, g, eos, Q, Qh, gv, m, m_full, \
+                       tp, dt, out, status, tl);
   VH_S(0, VH_C2P_REFERENCE) VH_S(1, VH_C2P_REFERENCE) VH_S(0, VH_C2P_CLOSED)
   VH_S(1, VH_C2P_CLOSED) VH_S(0, VH_C2P_LEARNED) VH_S(1, VH_C2P_LEARNED)
   VH_S(0, VH_C2P_NEWTON) VH_S(1, VH_C2P_NEWTON)
