@@ -104,6 +104,17 @@ typedef struct vh_ctx vh_ctx_t;
  * and Hydro::Hydro (src/hdo.cpp:50-57). */
 int vh_create(const vh_grid_t *grid, const vh_eos_t *eos,
               const vh_trcoeff_t *trc, int precision, vh_ctx_t **out);
+/* Several independent grids in one context, for event-by-event batches (no
+ * counterpart in the reference, which runs one Fluid per process): the state
+ * arrays hold the cells of grid 0, then grid 1, ... (each in its own cell
+ * order) with the component pitch N = the total cell count, which vh_ncell
+ * returns; vh_upload/download/set_prim take arrays of that layout.  One
+ * vh_step/vh_run advances every grid by the same dt, all grids in the same
+ * kernel launches (one graph per step).  Mixed precision only; each extent 1
+ * or >= 5; e > 0 in every cell (the batch has no vacuum fronts: steps return
+ * VH_ERR_ARG otherwise). */
+int vh_create_batch(int ngrid, const vh_grid_t *grids, const vh_eos_t *eos,
+                    const vh_trcoeff_t *trc, int precision, vh_ctx_t **out);
 /* Fluid::~Fluid (src/fld.cpp:114-117) */
 int vh_destroy(vh_ctx_t *ctx);
 /* number of cells nx*ny*nz */
@@ -141,18 +152,16 @@ int vh_set_c2p_mode(vh_ctx_t *ctx, int mode);
 
 /* Shape of the mixed-precision stage kernel (no counterpart in the
  * reference).  VH_STAGE_CELLS: one thread per cell evaluating its six
- * interfaces; VH_STAGE_ROWS: waves of 64 consecutive cells of the (x, y)
- * plane marching CZ cells in eta, every x- and eta-interface evaluated once;
- * VH_STAGE_TILES: blocks of TX x TY (x, y) columns (TX*TY <= 512) marching CZ
- * cells in eta, every interface evaluated once, shared through LDS.  Every
- * shape gives identical results.  VH_STAGE_AUTO (default): an autotuner times
- * the candidates on the first vh_step/vh_run (and after vh_set_c2p_mode) and
- * keeps the fastest.  vh_get_stage reports the shape in use (kind
- * VH_STAGE_AUTO: not tuned yet) and the tuned stage time in ms.  Mixed
- * precision only (VH_ERR_ARG otherwise). */
-enum { VH_STAGE_AUTO = 0, VH_STAGE_CELLS = 1, VH_STAGE_ROWS = 2, VH_STAGE_TILES = 3 };
-int vh_set_stage(vh_ctx_t *ctx, int kind, int TX, int TY, int CZ);
-int vh_get_stage(const vh_ctx_t *ctx, int *kind, int *TX, int *TY, int *CZ, double *ms);
+ * interfaces; VH_STAGE_ROWS: waves of 64 consecutive cells of an (x, y)
+ * plane marching CZ cells in eta, every x- and eta-interface evaluated once.
+ * Both give identical results.  VH_STAGE_AUTO (default): an autotuner times
+ * the candidates (cells; rows with CZ = 1..32) on the first vh_step/vh_run
+ * (and after vh_set_c2p_mode) and keeps the fastest.  vh_get_stage reports
+ * the kernel in use (VH_STAGE_AUTO: not tuned yet), CZ and the tuned stage
+ * time in ms.  Mixed precision only (VH_ERR_ARG otherwise). */
+enum { VH_STAGE_AUTO = 0, VH_STAGE_CELLS = 1, VH_STAGE_ROWS = 2 };
+int vh_set_stage(vh_ctx_t *ctx, int kind, int CZ);
+int vh_get_stage(const vh_ctx_t *ctx, int *kind, int *CZ, double *ms);
 /* Per-kernel timing over the next vh_run: enable=1 records HIP events around
  * every launch on the context's stream; vh_kernel_times returns, for up to
  * max kernels, the name, launch count and summed milliseconds. */
@@ -223,6 +232,28 @@ typedef struct {
   double vmax;
 } vh_harvest_cfg_t;
 int vh_mlp_harvest(const vh_harvest_cfg_t *cfg, long n, double *Q, double *prim);
+/* ---- hot-spot initial states (event-by-event batches) ----
+ * No direct counterpart in the reference (its Glissando IC,
+ * src/icGlissando.cpp:209-246, reads the spots from a file): per grid g,
+ * N_part ~ Poisson(npart_mean) spots at 2-D Woods-Saxon positions
+ * (density r dr dphi / (1 + exp((r - R)/a))) with Gamma(k = 1, theta = 1)
+ * weights w, sampled from a SplitMix64 stream seeded by (seed, g); then
+ *   e(x, y, eta) = amp * sum_i w_i exp(-|x - x_i|^2 / (2 sigma^2)) * H(eta),
+ * H = 1 for |eta| < eta_plateau (and for nz = 1), else Gaussian tails of width
+ * eta_sigma, deposited on the GPU.  e: the cells of all grids in the layout of
+ * vh_create_batch; npart (may be NULL): the spot count of each grid; spots
+ * (may be NULL): the first max_spots spots of all grids in order, as
+ * (x, y, w) triples. */
+typedef struct {
+  unsigned long long seed;
+  double npart_mean;          /* <= 500 */
+  double R, a;                /* Woods-Saxon radius and diffuseness, fm */
+  double sigma;               /* spot width, fm */
+  double amp;                 /* GeV/fm^3 per unit weight */
+  double eta_plateau, eta_sigma;
+} vh_hotspot_cfg_t;
+int vh_hotspot_ic(const vh_hotspot_cfg_t *cfg, int ngrid, const vh_grid_t *grids,
+                  double *e, int *npart, double *spots, long max_spots);
 /* device/driver information line (name, CUs, clocks) into buf */
 int vh_device_info(char *buf, int len);
 

@@ -146,17 +146,15 @@ def test_fp64_path_refuses_non_reference_c2p(gpu):
 @pytest.mark.parametrize("dims,c2p", [((37, 29, 13), abi.C2P_CLOSED), ((41, 41, 1), abi.C2P_CLOSED),
                                       ((24, 20, 9), abi.C2P_LEARNED)])
 def test_tile_shapes_identical(gpu, dims, c2p):
-    """Every stage kernel shape (rows and block tiles, each interface evaluated
-    once; autotuned or forced) gives bit-identical states to the per-cell
+    """The row stage kernel (each x- and eta-interface evaluated once; any eta
+    chunk, autotuned or forced) gives bit-identical states to the per-cell
     kernel."""
     nx, ny, nz = dims
     xl, yl, zl = (-4.0, 4.0), (-3.5, 3.5), (-1.2, 1.2)
     ic = setups.gaussian3d_ic(nx, ny, nz, xl, yl, zl, e0=30.0, sx=1.5, sy=1.2, eta_plateau=0.4,
                               eta_sigma=0.4)
-    shapes = [(abi.STAGE_CELLS, 0, 0, 0), (abi.STAGE_AUTO, 0, 0, 0), (abi.STAGE_ROWS, 0, 0, 1),
-              (abi.STAGE_ROWS, 0, 0, 5), (abi.STAGE_ROWS, 0, 0, 32), (abi.STAGE_TILES, 8, 4, 1),
-              (abi.STAGE_TILES, 13, 7, 5), (abi.STAGE_TILES, 38, 10, 13),
-              (abi.STAGE_TILES, 64, 8, 4)]
+    shapes = [(abi.STAGE_CELLS, 0), (abi.STAGE_AUTO, 0), (abi.STAGE_ROWS, 1),
+              (abi.STAGE_ROWS, 5), (abi.STAGE_ROWS, 32)]
     model = abi.mlp_init(9) if c2p == abi.C2P_LEARNED else None
     ref = None
     for sh in shapes:

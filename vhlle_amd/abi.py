@@ -16,7 +16,7 @@ ST_NAN_FLUX, ST_BIG_E, ST_NAN_SOURCE, ST_NAN_PI = 1, 2, 4, 8
 PREC_FP64, PREC_MIXED = 0, 1
 C2P_REFERENCE, C2P_LEARNED, C2P_CLOSED, C2P_NEWTON = 0, 1, 2, 3
 START_MLP, START_HALF = 0, 1
-STAGE_AUTO, STAGE_CELLS, STAGE_ROWS, STAGE_TILES = 0, 1, 2, 3
+STAGE_AUTO, STAGE_CELLS, STAGE_ROWS = 0, 1, 2
 MLP_NPARAM = 4674
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -52,6 +52,11 @@ class HarvestCfg(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_ulonglong), ("steps", ctypes.c_int), ("namp", ctypes.c_int),
                 ("amp", ctypes.c_double * 4), ("augment", ctypes.c_double),
                 ("vmax", ctypes.c_double)]
+
+
+class HotspotCfg(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_ulonglong)] + [(n, ctypes.c_double) for n in (
+        "npart_mean", "R", "a", "sigma", "amp", "eta_plateau", "eta_sigma")]
 
 
 class State(ctypes.Structure):
@@ -113,8 +118,12 @@ def lib():
                                           ctypes.c_int, _dp, ip]),
         "vh_set_c2p_model": (ctypes.c_int, [vp, _fp]),
         "vh_mlp_harvest": (ctypes.c_int, [ctypes.POINTER(HarvestCfg), ctypes.c_long, _dp, _dp]),
-        "vh_set_stage": (ctypes.c_int, [vp] + [ctypes.c_int] * 4),
-        "vh_get_stage": (ctypes.c_int, [vp, ip, ip, ip, ip, _dp]),
+        "vh_set_stage": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
+        "vh_get_stage": (ctypes.c_int, [vp, ip, ip, _dp]),
+        "vh_hotspot_ic": (ctypes.c_int, [ctypes.POINTER(HotspotCfg), ctypes.c_int,
+                                         ctypes.POINTER(Grid), _dp, ip, _dp, ctypes.c_long]),
+        "vh_create_batch": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Grid), ctypes.POINTER(Eos),
+                                           ctypes.POINTER(Trc), ctypes.c_int, ctypes.POINTER(vp)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -129,7 +138,7 @@ def exported_symbols():
             "vh_step", "vh_run", "vh_check_dtau", "vh_set_c2p_mode", "vh_enable_timing",
             "vh_kernel_times", "vh_c2p", "vh_p2c", "vh_device_info", "vh_mlp_init",
             "vh_mlp_forward", "vh_mlp_grad", "vh_mlp_train", "vh_c2p_learned",
-            "vh_set_c2p_model", "vh_mlp_harvest", "vh_set_stage", "vh_get_stage"]
+            "vh_set_c2p_model", "vh_mlp_harvest", "vh_set_stage", "vh_get_stage", "vh_create_batch", "vh_hotspot_ic"]
 
 
 def _p(a):
@@ -226,6 +235,27 @@ def harvest(n, seed=1, steps=200, amps=(10.0, 1.0), augment=0.5, vmax=0.99):
     return Q, prim
 
 
+def hotspot_ic(grids, seed, npart_mean=150.0, R=6.6, a=0.55, sigma=0.5, amp=1.0,
+               eta_plateau=2.0, eta_sigma=1.0, spots=False):
+    """Hot-spot e of every cell of `grids` ((nx, ny, nz, xlim, ylim, zlim) list),
+    concatenated in the batch layout: (e, npart per grid[, spots (n, 3)])."""
+    arr = (Grid * len(grids))(*[Grid(a_, b, c, x[0], x[1], y[0], y[1], z[0], z[1])
+                                for (a_, b, c, x, y, z) in grids])
+    n = sum(a_ * b * c for (a_, b, c, _, _, _) in grids)
+    e = np.zeros(n)
+    npart = np.zeros(len(grids), dtype=np.int32)
+    cfg = HotspotCfg(seed, npart_mean, R, a, sigma, amp, eta_plateau, eta_sigma)
+    maxs = int(len(grids) * (npart_mean + 20 * np.sqrt(npart_mean) + 50)) if spots else 0
+    sp = np.zeros((max(maxs, 1), 3))
+    check(lib().vh_hotspot_ic(ctypes.byref(cfg), len(grids), arr, _p(e),
+                              npart.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                              _p(sp) if spots else None, maxs), "vh_hotspot_ic")
+    if spots:
+        assert npart.sum() <= maxs
+        return e, npart, sp[:npart.sum()]
+    return e, npart
+
+
 def p2c(prim):
     prim = np.ascontiguousarray(prim, dtype=np.float64)
     Q = np.zeros((prim.shape[0], 7))
@@ -234,17 +264,28 @@ def p2c(prim):
 
 
 class Context:
-    """One Fluid + Hydro on the GPU through the C-ABI."""
+    """One Fluid + Hydro on the GPU through the C-ABI (or, with `grids`, a batch
+    of independent grids: a list of (nx, ny, nz, xlim, ylim, zlim))."""
 
-    def __init__(self, nx, ny, nz, xlim, ylim, zlim, trc=None, precision=PREC_FP64):
+    def __init__(self, nx=None, ny=None, nz=None, xlim=None, ylim=None, zlim=None, trc=None,
+                 precision=PREC_FP64, grids=None):
         self.trc = trc if trc is not None else make_trc()
-        g = Grid(nx, ny, nz, xlim[0], xlim[1], ylim[0], ylim[1], zlim[0], zlim[1])
         h = ctypes.c_void_p()
-        check(lib().vh_create(ctypes.byref(g), ctypes.byref(Eos(0)), ctypes.byref(self.trc),
-                              precision, ctypes.byref(h)), "vh_create")
+        if grids is None:
+            g = Grid(nx, ny, nz, xlim[0], xlim[1], ylim[0], ylim[1], zlim[0], zlim[1])
+            check(lib().vh_create(ctypes.byref(g), ctypes.byref(Eos(0)), ctypes.byref(self.trc),
+                                  precision, ctypes.byref(h)), "vh_create")
+            self.N = nx * ny * nz
+            self.shape = (nx, ny, nz)
+        else:
+            arr = (Grid * len(grids))(*[Grid(a, b, c, x[0], x[1], y[0], y[1], z[0], z[1])
+                                        for (a, b, c, x, y, z) in grids])
+            check(lib().vh_create_batch(len(grids), arr, ctypes.byref(Eos(0)),
+                                        ctypes.byref(self.trc), precision, ctypes.byref(h)),
+                  "vh_create_batch")
+            self.N = sum(a * b * c for (a, b, c, _, _, _) in grids)
+            self.shape = None
         self.h = h
-        self.N = nx * ny * nz
-        self.shape = (nx, ny, nz)
 
     def close(self):
         if getattr(self, "h", None):
@@ -303,15 +344,15 @@ class Context:
             check(lib().vh_set_c2p_model(self.h, _f(p)), "vh_set_c2p_model")
         check(lib().vh_set_c2p_mode(self.h, mode), "vh_set_c2p_mode")
 
-    def set_stage(self, kind=0, TX=0, TY=0, CZ=0):
-        check(lib().vh_set_stage(self.h, kind, TX, TY, CZ), "vh_set_stage")
+    def set_stage(self, kind=0, CZ=0):
+        check(lib().vh_set_stage(self.h, kind, CZ), "vh_set_stage")
 
     def get_stage(self):
-        v = [ctypes.c_int() for _ in range(4)]
+        k, cz = ctypes.c_int(), ctypes.c_int()
         ms = ctypes.c_double()
-        check(lib().vh_get_stage(self.h, *[ctypes.byref(x) for x in v], ctypes.byref(ms)),
+        check(lib().vh_get_stage(self.h, ctypes.byref(k), ctypes.byref(cz), ctypes.byref(ms)),
               "vh_get_stage")
-        return tuple(x.value for x in v) + (ms.value,)
+        return k.value, cz.value, ms.value
 
     def enable_timing(self, on=True):
         check(lib().vh_enable_timing(self.h, 1 if on else 0), "vh_enable_timing")

@@ -66,8 +66,14 @@ struct vh_ctx {
   // guesses (mixed precision only; gv reuses the FP64 path's F[0] storage)
   vh::ml::Packed *mlp = nullptr;
   float *gv = nullptr;
+  // grids of the context (one, or a batch of independent grids sharing the
+  // state arrays) and their device lookup tables (mixed precision kernels)
+  std::vector<vh::Geom> evh;
+  vh::mx::Grids G = {};
+  char *gtab = nullptr;
+  bool fusable = true, degenerate = false;
   // stage kernel shape (mixed precision): kind 0 until the autotuner ran
-  vh::mx::Tile tile = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  vh::mx::Tile tile = {0, 0};
   bool tile_forced = false;
   double tile_ms = 0.;
   int cur = 0;
@@ -133,6 +139,14 @@ void ghost_fill(vh_ctx *c, int full, double *Qn, hipStream_t st, int &rc) {
   const vh::Geom &g = c->g;
   const int pf = c->precision == VH_PREC_MIXED;
   const char *name = full ? "ghost_full" : "ghost_ideal";
+  if (pf && c->fusable) {
+    rc |= timed(c, name, [&] {
+      vh::mx::launch_ghost_cells(full, c->G, c->degenerate, Qn, c->Qh,
+                                 reinterpret_cast<float *>(c->pi),
+                                 reinterpret_cast<float *>(c->Pi), st);
+    });
+    return;
+  }
   if (vh::ghost_fusable(g)) {
     rc |= timed(c, name, [&] { vh::launch_ghost1(full, pf, g, Qn, c->Qh, c->pi, c->Pi, st); });
     return;
@@ -159,8 +173,10 @@ int enqueue_step_mixed(vh_ctx *c, double dt) {
   hipStream_t st = c->st;
   double *Qo = c->Q[c->cur], *Qn = c->Q[c->cur ^ 1];
   const int cm = c->c2p_mode;
+  const vh::mx::Grids &G = c->G;
   int rc = VH_OK;
   if (!c->m_full) {
+    if (c->evh.size() > 1) return VH_ERR_ARG;  // batches have no vacuum fronts
     rc |= timed(c, "update_m_dm", [&] { vh::launch_update_m(0, g, c->m, c->dm, c->tp, dt, st); });
     rc |= timed(c, "update_m_apply", [&] { vh::launch_update_m(1, g, c->m, c->dm, c->tp, dt, st); });
   }
@@ -170,12 +186,8 @@ int enqueue_step_mixed(vh_ctx *c, double dt) {
   if (learned)
     rc |= timed(c, "c2p_guess", [&] { vh::ml::launch_guess(Qo, g.N, c->tp, 0., c->mlp, c->gv, st); });
   auto stage = [&](int mode, double *out) {
-    if (c->tile.kind)
-      vh::mx::launch_stage_tile(mode, cm, g, c->eos, Qo, c->Qh, c->gv, c->m, c->m_full, c->tp,
-                                dt, out, c->status, c->tile, st);
-    else
-      vh::mx::launch_stage(mode, cm, g, c->eos, Qo, c->Qh, c->gv, c->m, c->m_full, c->tp, dt,
-                           out, c->status, st);
+    vh::mx::launch_stage_tile(mode, cm, G, c->eos, Qo, c->Qh, c->gv, c->m, c->m_full, c->tp,
+                              dt, out, c->status, c->tile, st);
   };
   rc |= timed(c, "stage_pred", [&] { stage(0, c->Qh); });
   if (learned)
@@ -184,17 +196,17 @@ int enqueue_step_mixed(vh_ctx *c, double dt) {
   rc |= timed(c, "advance_tau", [&] { vh::launch_advance_tau(c->tp, dt, st); });
   ghost_fill(c, 0, Qn, st, rc);
   if (c->viscous) {
-    rc |= timed(c, "visc_prim", [&] { vh::mx::launch_vprim(cm, g, c->eos, c->Qh, c->gv, c->tp, dt, c->ph, c->pt, st); });
+    rc |= timed(c, "visc_prim", [&] { vh::mx::launch_vprim(cm, g.N, c->eos, c->Qh, c->gv, c->tp, dt, c->ph, c->pt, st); });
     rc |= timed(c, "visc_relax", [&] {
-      vh::mx::launch_visc1(cm, g, c->eos, c->trc, Qo, Qn, c->gv, c->ph, c->pt, c->m, c->m_full,
+      vh::mx::launch_visc1(cm, G, c->eos, c->trc, Qo, Qn, c->gv, c->ph, c->pt, c->m, c->m_full,
                            fp(c->pi), fp(c->Pi), c->tp, dt, rec(c), c->status, st);
     });
     rc |= timed(c, "visc_advect", [&] {
-      vh::mx::launch_visc2(g, c->ph, rec(c), fp(c->pi), fp(c->piH), fp(c->Pi), fp(c->PiH), c->vcc,
+      vh::mx::launch_visc2(G, c->ph, rec(c), fp(c->pi), fp(c->piH), fp(c->Pi), fp(c->PiH), c->vcc,
                            fp(c->vf), c->tp, dt, st);
     });
     rc |= timed(c, "visc_flux_update", [&] {
-      vh::mx::launch_visc3(g, c->ph, fp(c->piH), fp(c->PiH), fp(c->vf), c->m, c->m_full, c->tp,
+      vh::mx::launch_visc3(G, c->ph, fp(c->piH), fp(c->PiH), fp(c->vf), c->m, c->m_full, c->tp,
                            dt, Qn, st);
     });
   }
@@ -314,25 +326,113 @@ int refresh_m_full(vh_ctx *c) {
 
 extern "C" {
 
-int vh_create(const vh_grid_t *grid, const vh_eos_t *eos, const vh_trcoeff_t *trc,
-              int precision, vh_ctx_t **out) {
-  if (!grid || !eos || !trc || !out) return VH_ERR_ARG;
-  if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1 || eos->kind != VH_EOS_IDEAL) return VH_ERR_ARG;
+}  // extern "C"
+
+namespace {
+
+// Fluid ctor geometry (src/fld.cpp:77-79)
+vh::Geom geom_of(const vh_grid_t &gr) {
+  vh::Geom g;
+  g.nx = gr.nx;
+  g.ny = gr.ny;
+  g.nz = gr.nz;
+  g.N = (long)g.nx * g.ny * g.nz;
+  g.dx = (gr.xmax - gr.xmin) / (g.nx - 1);
+  g.dy = (gr.ymax - gr.ymin) / (g.ny - 1);
+  g.dz = (gr.zmax - gr.zmin) / (g.nz - 1);
+  return g;
+}
+
+bool fusable_extent(int n) { return n == 1 || n >= 5; }
+
+// device lookup tables of the grids (vh::mx::Grids)
+int build_grid_tables(vh_ctx *c) {
+  const int nev = (int)c->evh.size();
+  std::vector<int> bev, bfirst(nev), sev, sfirst(nev);
+  int maxnz = 1;
+  for (int e = 0; e < nev; e++) {
+    const vh::Geom &g = c->evh[e];
+    bfirst[e] = (int)bev.size();
+    for (long b = 0; b < (g.N + 255) / 256; b++) bev.push_back(e);
+    sfirst[e] = (int)sev.size();
+    const long nxy = (long)g.nx * g.ny;
+    for (long q = 0; q < (nxy + 62) / 63; q++) sev.push_back(e);
+    maxnz = std::max(maxnz, g.nz);
+  }
+  const size_t al = 256;
+  auto rnd = [&](size_t b) { return (b + al - 1) / al * al; };
+  const size_t sz = rnd(nev * sizeof(vh::Geom)) + rnd(bev.size() * 4) + rnd(nev * 4) +
+                    rnd(sev.size() * 4) + rnd(nev * 4);
+  VH_HIP(hipMalloc(&c->gtab, sz));
+  char *p = c->gtab;
+  auto put = [&](const void *src, size_t bytes) -> char * {
+    char *d = p;
+    if (hipMemcpy(d, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    p += rnd(bytes);
+    return d;
+  };
+  char *ev = put(c->evh.data(), nev * sizeof(vh::Geom));
+  char *pb = put(bev.data(), bev.size() * 4), *pbf = put(bfirst.data(), nev * 4);
+  char *ps = put(sev.data(), sev.size() * 4), *psf = put(sfirst.data(), nev * 4);
+  if (!ev || !pb || !pbf || !ps || !psf) return VH_ERR_HIP;
+  vh::mx::Grids &G = c->G;
+  G.ev = reinterpret_cast<const vh::Geom *>(ev);
+  G.bev = reinterpret_cast<const int *>(pb);
+  G.bfirst = reinterpret_cast<const int *>(pbf);
+  G.sev = reinterpret_cast<const int *>(ps);
+  G.sfirst = reinterpret_cast<const int *>(psf);
+  G.P = c->g.N;
+  G.nev = nev;
+  G.nblk = (int)bev.size();
+  G.nseg = (int)sev.size();
+  G.maxnz = maxnz;
+  return VH_OK;
+}
+
+int create_impl(const vh_grid_t *grids, int ngrid, const vh_eos_t *eos, const vh_trcoeff_t *trc,
+                int precision, vh_ctx_t **out) {
+  if (!grids || ngrid < 1 || !eos || !trc || !out) return VH_ERR_ARG;
+  if (eos->kind != VH_EOS_IDEAL) return VH_ERR_ARG;
   if (precision != VH_PREC_FP64 && precision != VH_PREC_MIXED) return VH_ERR_ARG;
+  if (ngrid > 1 && precision != VH_PREC_MIXED) return VH_ERR_ARG;
+  long ncell = 0;
+  for (int e = 0; e < ngrid; e++) {
+    const vh_grid_t &gr = grids[e];
+    if (gr.nx < 1 || gr.ny < 1 || gr.nz < 1) return VH_ERR_ARG;
+    if (ngrid > 1 && !(fusable_extent(gr.nx) && fusable_extent(gr.ny) && fusable_extent(gr.nz)))
+      return VH_ERR_ARG;
+    ncell += (long)gr.nx * gr.ny * gr.nz;
+  }
+  if (ncell > (1L << 31) / 4) return VH_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return VH_ERR_NODEVICE;
   vh_ctx *c = new vh_ctx;
-  c->grid = *grid;
+  c->grid = grids[0];
   c->precision = precision;
-  vh::Geom &g = c->g;
-  g.nx = grid->nx;
-  g.ny = grid->ny;
-  g.nz = grid->nz;
-  g.N = (long)g.nx * g.ny * g.nz;
-  // Fluid ctor (src/fld.cpp:77-79)
-  g.dx = (grid->xmax - grid->xmin) / (g.nx - 1);
-  g.dy = (grid->ymax - grid->ymin) / (g.ny - 1);
-  g.dz = (grid->zmax - grid->zmin) / (g.nz - 1);
+  for (int e = 0; e < ngrid; e++) {
+    vh::Geom g = geom_of(grids[e]);
+    c->fusable &= fusable_extent(g.nx) && fusable_extent(g.ny) && fusable_extent(g.nz);
+    c->degenerate |= g.nx == 1 || g.ny == 1 || g.nz == 1;
+    c->evh.push_back(g);
+  }
+  long base = 0;
+  for (auto &g : c->evh) {
+    g.base = base;
+    g.P = ncell;
+    base += g.N;
+  }
+  // the context's own geometry: the grid itself, or (batches) only the total
+  if (ngrid == 1) {
+    c->g = c->evh[0];
+  } else {
+    c->g = vh::Geom();
+    c->g.nx = c->g.ny = c->g.nz = 0;
+    c->g.dx = c->g.dy = c->g.dz = 0.;
+  }
+  c->g.N = ncell;
+  c->g.base = 0;
+  c->g.P = ncell;
+  const vh::Geom &g = c->g;
   c->eos.kind = eos->kind;
   c->eos.cnst = ideal_cnst();
   c->trc.c = *trc;
@@ -399,8 +499,27 @@ int vh_create(const vh_grid_t *grid, const vh_eos_t *eos, const vh_trcoeff_t *tr
     vh_destroy(c);
     return VH_ERR_HIP;
   }
+  if (build_grid_tables(c) != VH_OK) {
+    vh_destroy(c);
+    return VH_ERR_HIP;
+  }
   *out = c;
   return VH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vh_create(const vh_grid_t *grid, const vh_eos_t *eos, const vh_trcoeff_t *trc,
+              int precision, vh_ctx_t **out) {
+  if (!grid) return VH_ERR_ARG;
+  return create_impl(grid, 1, eos, trc, precision, out);
+}
+
+int vh_create_batch(int ngrid, const vh_grid_t *grids, const vh_eos_t *eos,
+                    const vh_trcoeff_t *trc, int precision, vh_ctx_t **out) {
+  return create_impl(grids, ngrid, eos, trc, precision, out);
 }
 
 int vh_destroy(vh_ctx_t *c) {
@@ -413,6 +532,7 @@ int vh_destroy(vh_ctx_t *c) {
     (void)hipEventDestroy(e.second);
   }
   if (c->pool) (void)hipFree(c->pool);
+  if (c->gtab) (void)hipFree(c->gtab);
   if (c->mlp) (void)hipFree(c->mlp);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
@@ -528,7 +648,9 @@ int vh_set_prim(vh_ctx_t *c, double tau, const double *e, const double *nb, cons
 
 int vh_check_dtau(const vh_ctx_t *c, double dt) {
   if (!c) return VH_ERR_ARG;
-  return (dt > c->g.dx / 2. || dt > c->g.dy / 2.) ? VH_ERR_DTAU : VH_OK;
+  for (const vh::Geom &g : c->evh)
+    if (dt > g.dx / 2. || dt > g.dy / 2.) return VH_ERR_DTAU;
+  return VH_OK;
 }
 
 int vh_set_c2p_mode(vh_ctx_t *c, int mode) {
@@ -545,7 +667,7 @@ int vh_set_c2p_mode(vh_ctx_t *c, int mode) {
   return VH_OK;
 }
 
-int vh_set_stage(vh_ctx_t *c, int kind, int TX, int TY, int CZ) {
+int vh_set_stage(vh_ctx_t *c, int kind, int CZ) {
   if (!c || c->precision != VH_PREC_MIXED) return VH_ERR_ARG;
   vh::mx::Tile t;
   if (kind == VH_STAGE_AUTO) {
@@ -557,14 +679,10 @@ int vh_set_stage(vh_ctx_t *c, int kind, int TX, int TY, int CZ) {
     t = vh::mx::make_cells();
   } else if (kind == VH_STAGE_ROWS) {
     if (CZ < 1) return VH_ERR_ARG;
-    t = vh::mx::make_rows(c->g, CZ);
-  } else if (kind == VH_STAGE_TILES) {
-    if (TX < 1 || TY < 1 || CZ < 1) return VH_ERR_ARG;
-    t = vh::mx::make_tile(c->g, TX, TY, CZ);
+    t = vh::mx::make_rows(c->G, CZ);
   } else {
     return VH_ERR_ARG;
   }
-  if (!vh::mx::tile_ok(t)) return VH_ERR_ARG;
   c->tile = t;
   c->tile_forced = true;
   c->tile_ms = 0.;
@@ -572,11 +690,9 @@ int vh_set_stage(vh_ctx_t *c, int kind, int TX, int TY, int CZ) {
   return VH_OK;
 }
 
-int vh_get_stage(const vh_ctx_t *c, int *kind, int *TX, int *TY, int *CZ, double *ms) {
-  if (!c || !kind || !TX || !TY || !CZ) return VH_ERR_ARG;
+int vh_get_stage(const vh_ctx_t *c, int *kind, int *CZ, double *ms) {
+  if (!c || !kind || !CZ) return VH_ERR_ARG;
   *kind = c->tile.kind;
-  *TX = c->tile.TX;
-  *TY = c->tile.TY;
   *CZ = c->tile.CZ;
   if (ms) *ms = c->tile_ms;
   return VH_OK;
@@ -598,7 +714,7 @@ int vh_set_c2p_model(vh_ctx_t *c, const float *params) {
 // memory, and keeps the fastest.  Every shape gives the same result.
 static int ensure_tile(vh_ctx_t *c, double tau, double dt) {
   if (c->precision != VH_PREC_MIXED || c->tile.kind || c->tile_forced) return VH_OK;
-  const std::vector<vh::mx::Tile> cand = vh::mx::tile_candidates(c->g);
+  const std::vector<vh::mx::Tile> cand = vh::mx::tile_candidates(c->G);
   const vh::Geom &g = c->g;
   hipStream_t st = c->st;
   double *so = c->F[1];                               // unused by the mixed path
@@ -614,7 +730,7 @@ static int ensure_tile(vh_ctx_t *c, double tau, double dt) {
   int rc = VH_OK;
   for (const vh::mx::Tile &t : cand) {
     auto go = [&] {
-      vh::mx::launch_stage_tile(0, c->c2p_mode, g, c->eos, c->Q[c->cur], c->Qh, c->gv, c->m,
+      vh::mx::launch_stage_tile(0, c->c2p_mode, c->G, c->eos, c->Q[c->cur], c->Qh, c->gv, c->m,
                                 c->m_full, c->tp, dt, so, ss, t, st);
     };
     go();

@@ -230,12 +230,16 @@ struct DevTrc {
 };
 
 // --------------------------------------------------------------- geometry
+// Grid geometry.  N: cells of this grid; base: index of its cell 0 in the
+// state arrays; P: the arrays' component pitch (cells of all grids stored
+// together; base = 0 and P = N for a single grid).
 struct Geom {
   int nx, ny, nz;
   long N;
   double dx, dy, dz;
+  long base = 0, P = 0;
   __device__ inline long idx(int ix, int iy, int iz) const {
-    return ix + (long)nx * (iy + (long)ny * iz);
+    return base + ix + (long)nx * (iy + (long)ny * iz);
   }
   // Fluid::getCell clamp (src/fld.h:70-78)
   __device__ inline long cidx(int ix, int iy, int iz) const {

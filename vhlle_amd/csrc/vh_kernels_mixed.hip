@@ -271,7 +271,7 @@ __device__ __forceinline__ void axis_fluxes(const Geom &g, const DevEos &eos,
                                             int n, long c, long s, double tau, double dt,
                                             double acc[7], int *status) {
   if (n < 2) return;
-  const long N = g.N;
+  const long N = g.P;
   const double itau = 1. / tau;
   const double tauFactor = MODE == 0 ? tau + 0.25 * dt : tau + 0.5 * dt;
   const double dta = MODE == 0 ? 0.5 * dt : dt;
@@ -343,16 +343,19 @@ __device__ __forceinline__ void finish_cell(const DevEos &eos, const double *__r
 // Q0 + flux0 < 0 (src/cll.cpp:70-74).
 template <int MODE, int CM>
 __global__ void __launch_bounds__(256)
-k_stage(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
+k_stage(Grids G, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
         const float *__restrict__ gv, const double *__restrict__ m, int m_full,
         const double *__restrict__ tp, double dt, double *__restrict__ out,
         int *__restrict__ status) {
   const double tau = *tp;
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.N) return;
+  const int gi = G.bev[blockIdx.x];
+  const Geom g = G.ev[gi];
+  const long cl = (long)(blockIdx.x - G.bfirst[gi]) * blockDim.x + threadIdx.x;
+  if (cl >= g.N) return;
+  const long c = g.base + cl;
   int ix, iy, iz;
-  coords_of(g, c, ix, iy, iz);
-  const long N = g.N;
+  coords_of(g, cl, ix, iy, iz);
+  const long N = g.P;
   const double *Qs = MODE == 0 ? Q : Qh;
   double acc[7];
 #pragma unroll
@@ -379,103 +382,6 @@ __device__ __forceinline__ bool iface(const DevEos &eos, const double *__restric
   return hlle_f32<D>(L, R, itauf, itf, pre, mk, F, status);
 }
 
-// Tiled stage: every interface is evaluated once.  A block is a TX x TY tile
-// of (x, y) columns marching over CZ cells in eta.  Thread (tx, ty) evaluates
-// the lower x- and y-interfaces of its cell and shares them through LDS; its
-// upper eta-interface stays in registers and is the lower one of the next
-// cell of the column.  The last tile column and row (the halo, hx = hy = 1
-// when the axis has interfaces) only supply interfaces, so tiles advance by
-// TX - hx and TY - hy cells.  Interface states, fluxes and the per-cell
-// accumulation order (+x-, -x+, +y-, -y+, +eta-, -eta+, source) are those of
-// k_stage: both kernels give identical results for every tile shape.
-template <int MODE, int CM>
-__global__ void __launch_bounds__(512)
-k_stage_tile(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
-             const float *__restrict__ gv, const double *__restrict__ m, int m_full,
-             const double *__restrict__ tp, double dt, double *__restrict__ out,
-             int *__restrict__ status, Tile tl) {
-  extern __shared__ float sh[];
-  const int T = tl.TX * tl.TY;
-  float *fxs = sh, *fys = sh + 7 * T;
-  int *fls = reinterpret_cast<int *>(sh + 14 * T);
-  const int t = threadIdx.x;
-  const int tx = t % tl.TX, ty = t / tl.TX;
-  const int bx = blockIdx.x % tl.ntx, rr = blockIdx.x / tl.ntx;
-  const int by = rr % tl.nty, bz = rr / tl.nty;
-  const int ix = bx * (tl.TX - tl.hx) + tx, iy = by * (tl.TY - tl.hy) + ty;
-  const bool cx = t < T && ix < g.nx && iy < g.ny;
-  const bool upd = cx && tx < tl.TX - tl.hx && ty < tl.TY - tl.hy;
-  const int z0 = bz * tl.CZ, z1 = min(g.nz, z0 + tl.CZ);
-  const long N = g.N, sy = g.nx, sz = (long)g.nx * g.ny;
-  const double tau = *tp;
-  const double *Qs = MODE == 0 ? Q : Qh;
-  const double itau = 1. / tau;
-  const double tauFactor = MODE == 0 ? tau + 0.25 * dt : tau + 0.5 * dt;
-  const double dta = MODE == 0 ? 0.5 * dt : dt;
-  const float prx = (float)(tauFactor * dta / g.dx), pry = (float)(tauFactor * dta / g.dy),
-              prz = (float)(tauFactor * dta / g.dz);
-  const float itf = (float)(1. / tauFactor), itauf = (float)itau;
-  float fzl[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  float fzu[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  bool hzl = false;
-  for (int iz = z0; iz < z1; iz++) {
-    const long c = cx ? g.idx(ix, iy, iz) : 0;
-    float fx[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    float fy[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    bool hx = false, hy = false, hzu = false;
-    if (cx && ix >= 1)
-      hx = iface<1, CM>(eos, Qs, gv, m, m_full, N, ix >= 2 ? c - 2 : c - 1, c - 1, c,
-                        ix + 1 < g.nx ? c + 1 : c, itau, itauf, itf, prx, fx, status);
-    if (cx && iy >= 1)
-      hy = iface<2, CM>(eos, Qs, gv, m, m_full, N, iy >= 2 ? c - 2 * sy : c - sy, c - sy, c,
-                        iy + 1 < g.ny ? c + sy : c, itau, itauf, itf, pry, fy, status);
-    if (iz == z0 && cx && iz >= 1)
-      hzl = iface<3, CM>(eos, Qs, gv, m, m_full, N, iz >= 2 ? c - 2 * sz : c - sz, c - sz, c,
-                         iz + 1 < g.nz ? c + sz : c, itau, itauf, itf, prz, fzl, status);
-    if (cx && iz + 1 < g.nz)
-      hzu = iface<3, CM>(eos, Qs, gv, m, m_full, N, iz >= 1 ? c - sz : c, c, c + sz,
-                         iz + 2 < g.nz ? c + 2 * sz : c + sz, itau, itauf, itf, prz, fzu,
-                         status);
-    __syncthreads();  // the previous slice's reads of the shared interfaces are done
-    if (t < T) {
-#pragma unroll
-      for (int k = 0; k < 7; k++) {
-        fxs[k * T + t] = fx[k];
-        fys[k * T + t] = fy[k];
-      }
-      fls[t] = (hx ? 1 : 0) | (hy ? 2 : 0);
-    }
-    __syncthreads();
-    if (upd) {
-      double acc[7];
-#pragma unroll
-      for (int k = 0; k < 7; k++) acc[k] = 0.;
-      if (hx)
-#pragma unroll
-        for (int k = 0; k < 7; k++) acc[k] += (double)fx[k];
-      if (ix + 1 < g.nx && (fls[t + 1] & 1))
-#pragma unroll
-        for (int k = 0; k < 7; k++) acc[k] -= (double)fxs[k * T + t + 1];
-      if (hy)
-#pragma unroll
-        for (int k = 0; k < 7; k++) acc[k] += (double)fy[k];
-      if (iy + 1 < g.ny && (fls[t + tl.TX] & 2))
-#pragma unroll
-        for (int k = 0; k < 7; k++) acc[k] -= (double)fys[k * T + t + tl.TX];
-      if (hzl)
-#pragma unroll
-        for (int k = 0; k < 7; k++) acc[k] += (double)fzl[k];
-      if (hzu)
-#pragma unroll
-        for (int k = 0; k < 7; k++) acc[k] -= (double)fzu[k];
-      finish_cell<MODE, CM>(eos, Q, Qs, gv, N, c, tau, dt, acc, out);
-    }
-#pragma unroll
-    for (int k = 0; k < 7; k++) fzl[k] = fzu[k];
-    hzl = hzu;
-  }
-}
-
 // Row stage: every x- and eta-interface is evaluated once, without barriers.
 // A wave owns 64 consecutive cells of the flattened (x, y) plane (waves
 // overlap by one cell, so 63 of 64 lanes update a cell whatever nx is) and
@@ -486,22 +392,25 @@ k_stage_tile(Geom g, DevEos eos, const double *__restrict__ Q, const double *__r
 // accumulation order are those of k_stage: identical results.
 template <int MODE, int CM>
 __global__ void __launch_bounds__(256)
-k_stage_rows(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
+k_stage_rows(Grids G, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
              const float *__restrict__ gv, const double *__restrict__ m, int m_full,
              const double *__restrict__ tp, double dt, double *__restrict__ out,
-             int *__restrict__ status, Tile tl) {
+             int *__restrict__ status, int CZ) {
   const int lane = threadIdx.x & 63;
   const long w = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long sgl = w % G.nseg, zc = w / G.nseg;  // row segment over all grids, eta chunk
+  const int gi = G.sev[sgl];
+  const Geom g = G.ev[gi];
   const long nxy = (long)g.nx * g.ny;
-  const long seg = w % tl.ntx, zc = w / tl.ntx;
-  const int z0 = (int)zc * tl.CZ;
+  const long seg = sgl - G.sfirst[gi];
+  const int z0 = (int)zc * CZ;
   if (z0 >= g.nz) return;  // whole wave
-  const int z1 = min(g.nz, z0 + tl.CZ);
+  const int z1 = min(g.nz, z0 + CZ);
   const long f = seg * 63 + lane;
   const bool cx = f < nxy;
   const bool upd = cx && lane < 63;
   const int ix = cx ? (int)(f % g.nx) : 0, iy = cx ? (int)(f / g.nx) : 0;
-  const long N = g.N, sy = g.nx, sz = nxy;
+  const long N = g.P, sy = g.nx, sz = nxy;
   const double tau = *tp;
   const double *Qs = MODE == 0 ? Q : Qh;
   const double itau = 1. / tau;
@@ -514,7 +423,7 @@ k_stage_rows(Geom g, DevEos eos, const double *__restrict__ Q, const double *__r
   float fzu[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   bool hzl = false;
   for (int iz = z0; iz < z1; iz++) {
-    const long c = cx ? f + iz * sz : 0;
+    const long c = cx ? g.base + f + iz * sz : 0;
     float fx[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float fyl[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float fyu[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -573,13 +482,12 @@ k_stage_rows(Geom g, DevEos eos, const double *__restrict__ Q, const double *__r
 // ---------------------------------------------------------------- viscous
 template <int CM>
 __global__ void __launch_bounds__(256)
-k_vprim(Geom g, DevEos eos, const double *__restrict__ Qh, const float *__restrict__ gv,
+k_vprim(long N, DevEos eos, const double *__restrict__ Qh, const float *__restrict__ gv,
         const double *__restrict__ tp, double dt, double *__restrict__ ph,
         double *__restrict__ pt) {
   const double tau = *tp;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.N) return;
-  const long N = g.N;
+  if (c >= N) return;
   double q[7];
   Prim r;
   const double ith = 1. / (tau - 0.5 * dt), it = 1. / tau;
@@ -649,7 +557,7 @@ __device__ __forceinline__ void is_cross(const float *X, const float *Pm, const 
 
 template <int CM>
 __global__ void __launch_bounds__(256)
-k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
+k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
         const double *__restrict__ Qn, const float *__restrict__ gv,
         const double *__restrict__ ph,
         const double *__restrict__ pt, const double *__restrict__ m, int m_full,
@@ -657,11 +565,14 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
         const double *__restrict__ tp, double dt, float4 *__restrict__ rec,
         int *__restrict__ status) {
   const double tau = *tp;
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.N) return;
+  const int gi = G.bev[blockIdx.x];
+  const Geom g = G.ev[gi];
+  const long cl = (long)(blockIdx.x - G.bfirst[gi]) * blockDim.x + threadIdx.x;
+  if (cl >= g.N) return;
+  const long c = g.base + cl;
   int ix, iy, iz;
-  coords_of(g, c, ix, iy, iz);
-  const long N = g.N;
+  coords_of(g, cl, ix, iy, iz);
+  const long N = g.P;
   const double e = ph[c], nb = ph[2 * N + c], nq = ph[3 * N + c], ns = ph[4 * N + c];
   const double vx = ph[5 * N + c], vy = ph[6 * N + c], vz = ph[7 * N + c];
   if (e <= 0.) {
@@ -852,16 +763,19 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
 
 // ISformal loop #2 (src/hdo.cpp:866-962) in FP32 with FP32 pi storage.
 __global__ void __launch_bounds__(256)
-k_visc2(Geom g, const double *__restrict__ ph, const float4 *__restrict__ rec,
+k_visc2(Grids G, const double *__restrict__ ph, const float4 *__restrict__ rec,
         float *__restrict__ pi, float *__restrict__ piH,
         float *__restrict__ Pi, float *__restrict__ PiH, double *__restrict__ vcc,
         float *__restrict__ vf, const double *__restrict__ tp, double dt) {
   const double tau = *tp;
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.N) return;
+  const int gi = G.bev[blockIdx.x];
+  const Geom g = G.ev[gi];
+  const long cl = (long)(blockIdx.x - G.bfirst[gi]) * blockDim.x + threadIdx.x;
+  if (cl >= g.N) return;
+  const long c = g.base + cl;
   int ix, iy, iz;
-  coords_of(g, c, ix, iy, iz);
-  const long N = g.N;
+  coords_of(g, cl, ix, iy, iz);
+  const long N = g.P;
   const float e = (float)ph[c], p = (float)ph[N + c];
   const float vx = (float)ph[5 * N + c], vy = (float)ph[6 * N + c], vz = (float)ph[7 * N + c];
   if (!(e > 0.f)) {
@@ -955,7 +869,7 @@ __device__ __forceinline__ bool visc_face(const Geom &g, long l, long r,
                                           const float *__restrict__ PiH,
                                           const double *__restrict__ m, int m_full, float pre,
                                           float F[4]) {
-  const long N = g.N;
+  const long N = g.P;
   if (!m_full && m[(D - 1) * N + l] < 1. && m[(D - 1) * N + r] < 1.) return false;
   const double dvx = 0.5 * (ph[5 * N + l] + ph[5 * N + r]);
   const double dvy = 0.5 * (ph[6 * N + l] + ph[6 * N + r]);
@@ -975,16 +889,19 @@ __device__ __forceinline__ bool visc_face(const Geom &g, long l, long r,
 }
 
 __global__ void __launch_bounds__(256)
-k_visc3(Geom g, const double *__restrict__ ph, const float *__restrict__ piH,
+k_visc3(Grids G, const double *__restrict__ ph, const float *__restrict__ piH,
         const float *__restrict__ PiH, const float *__restrict__ vf,
         const double *__restrict__ m, int m_full, const double *__restrict__ tp, double dt,
         double *__restrict__ Q) {
   const double tau = *tp;
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.N) return;
+  const int gi = G.bev[blockIdx.x];
+  const Geom g = G.ev[gi];
+  const long cl = (long)(blockIdx.x - G.bfirst[gi]) * blockDim.x + threadIdx.x;
+  if (cl >= g.N) return;
+  const long c = g.base + cl;
   int ix, iy, iz;
-  coords_of(g, c, ix, iy, iz);
-  const long N = g.N;
+  coords_of(g, cl, ix, iy, iz);
+  const long N = g.P;
   float acc[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) acc[k] = vf[k * N + c];
@@ -1037,7 +954,7 @@ k_ghost_full(Geom g, double *__restrict__ Q, float *__restrict__ pi, float *__re
   else if (AXIS == 2) nline = (long)g.nx * g.nz;
   else nline = (long)g.ny * g.nz;
   if (t >= nline) return;
-  const long N = g.N;
+  const long N = g.P;
   const int n = AXIS == 3 ? g.nz : (AXIS == 2 ? g.ny : g.nx);
   int a, b;
   if (AXIS == 1) { a = (int)(t % g.ny); b = (int)(t / g.ny); }
@@ -1073,16 +990,74 @@ k_ghost_full(Geom g, double *__restrict__ Q, float *__restrict__ pi, float *__re
   }
 }
 
+// Single-pass ghost fill over a set of grids (each with every extent 1 or
+// >= 5): a ghost cell takes the value of the cell whose ghost coordinates
+// are mapped to the nearest interior index (0,1 -> 2 and n-2,n-1 -> n-3),
+// the composition of the reference's z, y, x passes (src/fld.cpp:211-382;
+// setQ's Q[T_] < 0 zeroing is idempotent).  An axis with n = 1 makes every
+// cell a ghost of itself, which only applies the zeroing: k_ghost_zero_b
+// does it first for such grids, so k_ghost_b never writes a cell another
+// thread reads.  FULL = 0: Q and Qh; FULL = 1: Q, pi, Pi (FP32).
+__device__ __forceinline__ int ghost_src(int i, int n) {
+  return n == 1 ? 0 : (i < 2 ? 2 : (i > n - 3 ? n - 3 : i));
+}
+
+template <int FULL>
+__global__ void __launch_bounds__(256)
+k_ghost_zero_b(Grids G, double *__restrict__ Q, double *__restrict__ Qh) {
+  const int gi = G.bev[blockIdx.x];
+  const Geom g = G.ev[gi];
+  const long cl = (long)(blockIdx.x - G.bfirst[gi]) * blockDim.x + threadIdx.x;
+  if (cl >= g.N || (g.nx > 1 && g.ny > 1 && g.nz > 1)) return;
+  const long c = g.base + cl, N = g.P;
+  if (Q[c] < 0.)
+    for (int k = 0; k < 7; k++) Q[k * N + c] = 0.;
+  if (!FULL && Qh[c] < 0.)
+    for (int k = 0; k < 7; k++) Qh[k * N + c] = 0.;
+}
+
+template <int FULL>
+__global__ void __launch_bounds__(256)
+k_ghost_b(Grids G, double *__restrict__ Q, double *__restrict__ Qh, float *__restrict__ pi,
+          float *__restrict__ Pi) {
+  const int gi = G.bev[blockIdx.x];
+  const Geom g = G.ev[gi];
+  const long cl = (long)(blockIdx.x - G.bfirst[gi]) * blockDim.x + threadIdx.x;
+  if (cl >= g.N) return;
+  int ix, iy, iz;
+  coords_of(g, cl, ix, iy, iz);
+  const int sx = ghost_src(ix, g.nx), sy = ghost_src(iy, g.ny), sz = ghost_src(iz, g.nz);
+  if (sx == ix && sy == iy && sz == iz) return;
+  const long N = g.P, c = g.base + cl, src = g.idx(sx, sy, sz);
+  double v[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) v[k] = Q[k * N + src];
+  const bool z = v[0] < 0.;
+#pragma unroll
+  for (int k = 0; k < 7; k++) Q[k * N + c] = z ? 0. : v[k];
+  if (!FULL) {
+#pragma unroll
+    for (int k = 0; k < 7; k++) v[k] = Qh[k * N + src];
+    const bool zh = v[0] < 0.;
+#pragma unroll
+    for (int k = 0; k < 7; k++) Qh[k * N + c] = zh ? 0. : v[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 10; k++) pi[k * N + c] = pi[k * N + src];
+    Pi[c] = Pi[src];
+  }
+}
+
 // ---------------------------------------------------------- launchers
 static inline unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
 
-void launch_stage(int mode, int cm, const Geom &g, const DevEos &eos, const double *Q,
+void launch_stage(int mode, int cm, const Grids &G, const DevEos &eos, const double *Q,
                   const double *Qh, const float *gv, const double *m, int m_full,
                   const double *tp, double dt, double *out, int *status, hipStream_t st) {
-  const dim3 gr(blocks(g.N)), b(256);
+  const dim3 gr((unsigned)G.nblk), b(256);
 #define VH_S(MM, CC)                                                                            \
   if (mode == MM && cm == CC)                                                                   \
-    hipLaunchKernelGGL((k_stage<MM, CC>), gr, b, 0, st, g, eos, Q, Qh, gv, m, m_full, tp, dt,  \
+    hipLaunchKernelGGL((k_stage<MM, CC>), gr, b, 0, st, G, eos, Q, Qh, gv, m, m_full, tp, dt,  \
                        out, status);
   VH_S(0, VH_C2P_REFERENCE) VH_S(1, VH_C2P_REFERENCE) VH_S(0, VH_C2P_CLOSED)
   VH_S(1, VH_C2P_CLOSED) VH_S(0, VH_C2P_LEARNED) VH_S(1, VH_C2P_LEARNED)
@@ -1090,160 +1065,101 @@ void launch_stage(int mode, int cm, const Geom &g, const DevEos &eos, const doub
 #undef VH_S
 }
 
-Tile make_tile(const Geom &g, int TX, int TY, int CZ) {
-  Tile t;
-  t.kind = STAGE_TILES;
-  t.hx = g.nx >= 2 ? 1 : 0;
-  t.hy = g.ny >= 2 ? 1 : 0;
-  t.TX = g.nx >= 2 ? TX : 1;
-  t.TY = g.ny >= 2 ? TY : 1;
-  t.CZ = CZ < 1 ? 1 : (CZ > g.nz ? g.nz : CZ);
-  const int sx = t.TX - t.hx, sy = t.TY - t.hy;
-  t.ntx = sx > 0 ? (g.nx + sx - 1) / sx : 0;
-  t.nty = sy > 0 ? (g.ny + sy - 1) / sy : 0;
-  t.ntz = (g.nz + t.CZ - 1) / t.CZ;
-  return t;
-}
-
-Tile make_rows(const Geom &g, int CZ) {
+Tile make_rows(const Grids &G, int CZ) {
   Tile t;
   t.kind = STAGE_ROWS;
-  t.TX = 64;
-  t.TY = 1;
-  t.hx = t.hy = 0;
-  t.CZ = CZ < 1 ? 1 : (CZ > g.nz ? g.nz : CZ);
-  const long nxy = (long)g.nx * g.ny;
-  t.ntx = (int)((nxy + 62) / 63);  // segment s updates cells [63 s, 63 s + 63)
-  t.nty = 1;
-  t.ntz = (g.nz + t.CZ - 1) / t.CZ;
+  t.CZ = CZ < 1 ? 1 : (CZ > G.maxnz ? G.maxnz : CZ);
   return t;
 }
 
 Tile make_cells() {
-  Tile t = {STAGE_CELLS, 0, 0, 0, 0, 0, 0, 0, 0};
+  Tile t;
+  t.kind = STAGE_CELLS;
+  t.CZ = 0;
   return t;
 }
 
-bool tile_ok(const Tile &t) {
-  if (t.kind == STAGE_CELLS) return true;
-  if (t.kind == STAGE_ROWS) return t.CZ >= 1 && t.ntx > 0 && (long)t.ntx * t.ntz < (1L << 33);
-  return t.kind == STAGE_TILES && t.TX >= 1 && t.TY >= 1 && t.TX * t.TY <= 512 &&
-         t.TX - t.hx >= 1 && t.TY - t.hy >= 1 && t.ntx > 0 && t.nty > 0 && t.ntz > 0 &&
-         (long)t.ntx * t.nty * t.ntz < (1L << 31);
-}
-
-std::vector<Tile> tile_candidates(const Geom &g) {
+std::vector<Tile> tile_candidates(const Grids &G) {
   std::vector<Tile> out;
   out.push_back(make_cells());
-  // rows: eta chunks of 1..32 cells (distinct after clamping to nz)
+  // rows: eta chunks of 1..32 cells (distinct after clamping to the largest nz)
   const int czs[] = {1, 2, 4, 8, 16, 32};
   for (int cz : czs) {
-    const Tile t = make_rows(g, cz);
+    const Tile t = make_rows(G, cz);
     bool dup = false;
     for (auto &o : out) dup |= o.kind == STAGE_ROWS && o.CZ == t.CZ;
     if (!dup) out.push_back(t);
   }
-  // block tiles that cover the x extent with little waste, ranked by the
-  // fraction of launched threads that update a cell; the best three
-  std::vector<std::pair<double, Tile>> c;
-  const int tys[] = {4, 6, 8, 10, 12, 16};
-  const int tzs[] = {4, 8, 16};
-  for (int ty : tys)
-    for (int kx = 1; kx <= 16; kx++) {
-      const int tx = g.nx >= 2 ? (g.nx + kx - 1) / kx + 1 : 1;
-      if (tx * ty > 512 || tx < 8) continue;
-      for (int cz : tzs) {
-        const Tile t = make_tile(g, tx, ty, cz);
-        if (!tile_ok(t)) continue;
-        const double launched = (double)t.ntx * t.nty * t.ntz * ((t.TX * t.TY + 63) / 64 * 64);
-        const double zextra = 1.0 + (g.nz > 1 ? 1.0 / t.CZ : 0.0);
-        c.push_back({(double)g.N / launched / zextra, t});
-      }
-    }
-  std::sort(c.begin(), c.end(), [](const std::pair<double, Tile> &a,
-                                   const std::pair<double, Tile> &b) { return a.first > b.first; });
-  int added = 0;
-  for (auto &p : c) {
-    bool dup = false;
-    for (auto &o : out)
-      dup |= o.kind == STAGE_TILES && o.TX == p.second.TX && o.TY == p.second.TY &&
-             o.CZ == p.second.CZ;
-    if (dup) continue;
-    out.push_back(p.second);
-    if (++added >= 3) break;
-  }
   return out;
 }
 
-void launch_stage_tile(int mode, int cm, const Geom &g, const DevEos &eos, const double *Q,
+void launch_stage_tile(int mode, int cm, const Grids &G, const DevEos &eos, const double *Q,
                        const double *Qh, const float *gv, const double *m, int m_full,
                        const double *tp, double dt, double *out, int *status, const Tile &tl,
                        hipStream_t st) {
-  if (tl.kind == STAGE_CELLS) {
-    launch_stage(mode, cm, g, eos, Q, Qh, gv, m, m_full, tp, dt, out, status, st);
+  if (tl.kind != STAGE_ROWS) {
+    launch_stage(mode, cm, G, eos, Q, Qh, gv, m, m_full, tp, dt, out, status, st);
     return;
   }
-  if (tl.kind == STAGE_ROWS) {
-    const long waves = (long)tl.ntx * tl.ntz;
-    const dim3 gr((unsigned)((waves + 3) / 4)), b(256);
+  const long ntz = (G.maxnz + tl.CZ - 1) / tl.CZ;
+  const long waves = (long)G.nseg * ntz;
+  const dim3 gr((unsigned)((waves + 3) / 4)), b(256);
 #define VH_S(MM, CC)                                                                          \
   if (mode == MM && cm == CC)                                                                 \
-    hipLaunchKernelGGL((k_stage_rows<MM, CC>), gr, b, 0, st, g, eos, Q, Qh, gv, m, m_full,   \
-                       tp, dt, out, status, tl);
-    VH_S(0, VH_C2P_REFERENCE) VH_S(1, VH_C2P_REFERENCE) VH_S(0, VH_C2P_CLOSED)
-    VH_S(1, VH_C2P_CLOSED) VH_S(0, VH_C2P_LEARNED) VH_S(1, VH_C2P_LEARNED)
-    VH_S(0, VH_C2P_NEWTON) VH_S(1, VH_C2P_NEWTON)
-#undef VH_S
-    return;
-  }
-  const int T = tl.TX * tl.TY;
-  const dim3 gr((unsigned)((long)tl.ntx * tl.nty * tl.ntz)), b((unsigned)((T + 63) / 64 * 64));
-  const size_t lds = (size_t)15 * T * sizeof(float);
-#define VH_S(MM, CC)                                                                          \
-  if (mode == MM && cm == CC)                                                                 \
-    hipLaunchKernelGGL((k_stage_tile<MM, CC>), gr, b, lds, st, g, eos, Q, Qh, gv, m, m_full, \
-                       tp, dt, out, status, tl);
+    hipLaunchKernelGGL((k_stage_rows<MM, CC>), gr, b, 0, st, G, eos, Q, Qh, gv, m, m_full,   \
+                       tp, dt, out, status, tl.CZ);
   VH_S(0, VH_C2P_REFERENCE) VH_S(1, VH_C2P_REFERENCE) VH_S(0, VH_C2P_CLOSED)
   VH_S(1, VH_C2P_CLOSED) VH_S(0, VH_C2P_LEARNED) VH_S(1, VH_C2P_LEARNED)
   VH_S(0, VH_C2P_NEWTON) VH_S(1, VH_C2P_NEWTON)
 #undef VH_S
 }
 
-void launch_vprim(int cm, const Geom &g, const DevEos &eos, const double *Qh, const float *gv,
+void launch_vprim(int cm, long N, const DevEos &eos, const double *Qh, const float *gv,
                   const double *tp, double dt, double *ph, double *pt, hipStream_t st) {
-  const dim3 gr(blocks(g.N)), b(256);
+  const dim3 gr(blocks(N)), b(256);
 #define VH_V(CC) \
-  if (cm == CC) hipLaunchKernelGGL(k_vprim<CC>, gr, b, 0, st, g, eos, Qh, gv, tp, dt, ph, pt);
+  if (cm == CC) hipLaunchKernelGGL(k_vprim<CC>, gr, b, 0, st, N, eos, Qh, gv, tp, dt, ph, pt);
   VH_V(VH_C2P_REFERENCE) VH_V(VH_C2P_CLOSED) VH_V(VH_C2P_LEARNED) VH_V(VH_C2P_NEWTON)
 #undef VH_V
 }
 
-void launch_visc1(int cm, const Geom &g, const DevEos &eos, const DevTrc &trc,
+void launch_visc1(int cm, const Grids &G, const DevEos &eos, const DevTrc &trc,
                   const double *Qprev, const double *Qn, const float *gv, const double *ph,
                   const double *pt, const double *m, int m_full, const float *pi,
                   const float *Pi, const double *tp, double dt, float4 *rec, int *status,
                   hipStream_t st) {
-  const dim3 gr(blocks(g.N)), b(256);
+  const dim3 gr((unsigned)G.nblk), b(256);
 #define VH_V(CC)                                                                               \
   if (cm == CC)                                                                                \
-    hipLaunchKernelGGL(k_visc1<CC>, gr, b, 0, st, g, eos, trc, Qprev, Qn, gv, ph, pt, m, m_full, \
+    hipLaunchKernelGGL(k_visc1<CC>, gr, b, 0, st, G, eos, trc, Qprev, Qn, gv, ph, pt, m, m_full, \
                        pi, Pi, tp, dt, rec, status);
   VH_V(VH_C2P_REFERENCE) VH_V(VH_C2P_CLOSED) VH_V(VH_C2P_LEARNED) VH_V(VH_C2P_NEWTON)
 #undef VH_V
 }
 
-void launch_visc2(const Geom &g, const double *ph, const float4 *rec, float *pi, float *piH,
+void launch_visc2(const Grids &G, const double *ph, const float4 *rec, float *pi, float *piH,
                   float *Pi, float *PiH, double *vcc, float *vf, const double *tp, double dt,
                   hipStream_t st) {
-  hipLaunchKernelGGL(k_visc2, dim3(blocks(g.N)), dim3(256), 0, st, g, ph, rec, pi, piH, Pi, PiH,
-                     vcc, vf, tp, dt);
+  hipLaunchKernelGGL(k_visc2, dim3((unsigned)G.nblk), dim3(256), 0, st, G, ph, rec, pi, piH, Pi,
+                     PiH, vcc, vf, tp, dt);
 }
 
-void launch_visc3(const Geom &g, const double *ph, const float *piH, const float *PiH,
+void launch_visc3(const Grids &G, const double *ph, const float *piH, const float *PiH,
                   const float *vf, const double *m, int m_full, const double *tp, double dt,
                   double *Q, hipStream_t st) {
-  hipLaunchKernelGGL(k_visc3, dim3(blocks(g.N)), dim3(256), 0, st, g, ph, piH, PiH, vf, m,
+  hipLaunchKernelGGL(k_visc3, dim3((unsigned)G.nblk), dim3(256), 0, st, G, ph, piH, PiH, vf, m,
                      m_full, tp, dt, Q);
+}
+
+void launch_ghost_cells(int full, const Grids &G, bool degenerate, double *Q, double *Qh,
+                        float *pi, float *Pi, hipStream_t st) {
+  const dim3 gr((unsigned)G.nblk), b(256);
+  if (degenerate) {
+    if (full) hipLaunchKernelGGL(k_ghost_zero_b<1>, gr, b, 0, st, G, Q, Qh);
+    else hipLaunchKernelGGL(k_ghost_zero_b<0>, gr, b, 0, st, G, Q, Qh);
+  }
+  if (full) hipLaunchKernelGGL(k_ghost_b<1>, gr, b, 0, st, G, Q, Qh, pi, Pi);
+  else hipLaunchKernelGGL(k_ghost_b<0>, gr, b, 0, st, G, Q, Qh, pi, Pi);
 }
 
 void launch_ghost_full(int axis, const Geom &g, double *Q, float *pi, float *Pi,

@@ -61,3 +61,32 @@ def gaussian3d_ic(nx, ny, nz, xlim, ylim, zlim, e0=50.0, sx=2.5, sy=3.0, eta_pla
     e = e0 * np.exp(-X * X / (2 * sx * sx) - Y * Y / (2 * sy * sy)) * H
     z = np.zeros_like(e)
     return e, z, z.copy(), z.copy()
+
+
+# Event-by-event batch: 256 boost-invariant (nz = 1) grids with sides drawn
+# uniformly from 100..160 cells (dx = dy = 0.1 fm), hot-spot initial states
+# (N_part ~ Poisson(150), 2-D Woods-Saxon R = 6.6 fm, a = 0.55 fm, Gaussian
+# spots sigma = 0.5 fm, Gamma(1) weights; amp = 5.2 GeV/fm^3 per unit weight
+# gives <e_max> ~ 40 GeV/fm^3), eta/s = 0.12, zeta/s peaked at 0.05 at
+# T = 170 MeV (zetaSparam 5), tau0 = 0.5 fm, dtau = 0.025 fm, 200 steps.
+BATCH_EBE = dict(name="batch_ebe_256x(100..160)^2", events=256, side_lo=100, side_hi=160, dx=0.1,
+                 tau0=0.5, dt=0.025, steps=200, etaS=0.12, zetaS=0.05, npart_mean=150.0,
+                 R=6.6, a=0.55, sigma=0.5, amp=5.2, seed=20240501)
+
+
+def batch_grids(cfg=BATCH_EBE, events=None, seed=None):
+    """(nx, ny, 1, xlim, ylim, zlim) of each event: sides uniform in
+    side_lo..side_hi from numpy's default_rng(seed), centred boxes."""
+    rng = np.random.default_rng(cfg["seed"] if seed is None else seed)
+    n = cfg["events"] if events is None else events
+    sides = rng.integers(cfg["side_lo"], cfg["side_hi"] + 1, size=n)
+    out = []
+    for s in sides:
+        h = (int(s) - 1) * cfg["dx"] / 2
+        out.append((int(s), int(s), 1, (-h, h), (-h, h), (-0.5, 0.5)))
+    return out
+
+
+def batch_trc(cfg=BATCH_EBE):
+    from vhlle_amd import abi
+    return abi.make_trc(etaS=cfg["etaS"], zetaS=cfg["zetaS"], zetaSparam=5)
