@@ -34,6 +34,11 @@ events recorded on the context's stream around every launch during a second,
 identical pass.
 
 python bench.py [--gpus 1] [--steps 300] [--warmup 5] [--variants a,b] [--no-cpu]
+                [--no-batch]
+
+Also measured (field "batch_ebe"): 256 event-by-event boost-invariant grids of
+sides 100..160 with hot-spot initial states in one batch context, against the
+same events run one context at a time.
 """
 import argparse
 import json
@@ -164,9 +169,9 @@ def run_variant(args, cfg, prec, c2p, ic):
     ctx.enable_timing(False)
     stage = None
     if p == abi.PREC_MIXED:
-        k, tx, ty, cz, ms = ctx.get_stage()
-        stage = dict(kernel={1: "cells", 2: "rows", 3: "tiles"}.get(k, "auto"), TX=tx, TY=ty,
-                     CZ=cz, tuned_pred_ms=round(ms, 4))
+        k, cz, ms = ctx.get_stage()
+        stage = dict(kernel={1: "cells", 2: "rows"}.get(k, "auto"), CZ=cz,
+                     tuned_pred_ms=round(ms, 4))
     ctx.close()
     print(f"[bench] {prec}/{c2p}: {args.steps} steps in {elapsed:.3f} s, status {st | st2}",
           file=sys.stderr, flush=True)
@@ -231,6 +236,56 @@ def cpu_baseline(cfg):
                 f"{steps} steps, {dt:.1f} s on 1 core of {cpu_model()}")
 
 
+def batch_bench(args, c2p):
+    """Event-by-event batch: 256 boost-invariant grids of sides 100..160 with
+    hot-spot initial states, all advanced by the same kernel launches (one HIP
+    graph per step), against the same events run one context at a time."""
+    from vhlle_amd import abi, setups
+    cfg = setups.BATCH_EBE
+    grids = setups.batch_grids()
+    e, npart = abi.hotspot_ic(grids, seed=cfg["seed"], npart_mean=cfg["npart_mean"], R=cfg["R"],
+                              a=cfg["a"], sigma=cfg["sigma"], amp=cfg["amp"])
+    z = np.zeros_like(e)
+    N = len(e)
+    steps = cfg["steps"]
+    ctx = abi.Context(grids=grids, trc=setups.batch_trc(), precision=abi.PREC_MIXED)
+    set_c2p(ctx, c2p)
+    ctx.set_prim(cfg["tau0"], e, z, z, z)
+    ctx.run(cfg["tau0"], cfg["dt"], 3)
+    ctx.set_prim(cfg["tau0"], e, z, z, z)
+    t0 = time.perf_counter()
+    st, _ = ctx.run(cfg["tau0"], cfg["dt"], steps)
+    el = time.perf_counter() - t0
+    k, cz, _ = ctx.get_stage()
+    ctx.close()
+    # the first 16 events one context each, same steps
+    nseq = 16
+    off, seq_cells, seq_t = 0, 0, 0.0
+    for g in grids[:nseq]:
+        n = g[0] * g[1] * g[2]
+        c = abi.Context(*g, trc=setups.batch_trc(), precision=abi.PREC_MIXED)
+        set_c2p(c, c2p)
+        c.set_prim(cfg["tau0"], e[off:off + n], z[:n], z[:n], z[:n])
+        c.run(cfg["tau0"], cfg["dt"], 3)
+        c.set_prim(cfg["tau0"], e[off:off + n], z[:n], z[:n], z[:n])
+        t0 = time.perf_counter()
+        c.run(cfg["tau0"], cfg["dt"], steps)
+        seq_t += time.perf_counter() - t0
+        c.close()
+        seq_cells += n
+        off += n
+    v = N * steps / el
+    print(f"[bench] batch {c2p}: {len(grids)} events, {N} cells, {steps} steps in {el:.3f} s, "
+          f"status {st}", file=sys.stderr, flush=True)
+    return dict(workload=cfg["name"], events=len(grids), cells=N, steps=steps, c2p=c2p,
+                status=st, value=round(v, 1), unit="cell-updates/s",
+                ms_per_step=round(el / steps * 1e3, 4),
+                stage_kernel={1: "cells", 2: "rows"}.get(k, "auto"), CZ=cz,
+                npart_mean=float(npart.mean()), e_max=float(e.max()),
+                one_context_per_event=dict(events=nseq, value=round(seq_cells * steps / seq_t, 1),
+                                           batch_speedup=round(v / (seq_cells * steps / seq_t), 2)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,6 +293,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--variants", default="fp64,mixed_ref,mixed_closed,mixed_newton,mixed_learned")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding of the hydro "
@@ -320,6 +376,8 @@ def main():
     }
     if "info" in _MODEL:
         out["learned_c2p"] = _MODEL["info"]
+    if not args.no_batch:
+        out["batch_ebe"] = [batch_bench(args, c) for c in ("closed", "learned")]
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(cfg)
     print(json.dumps(out), flush=True)
 

@@ -1082,10 +1082,13 @@ Tile make_cells() {
 std::vector<Tile> tile_candidates(const Grids &G) {
   std::vector<Tile> out;
   out.push_back(make_cells());
-  // rows: eta chunks of 1..32 cells (distinct after clamping to the largest nz)
+  // rows: eta chunks of 1..32 cells (distinct after clamping to the largest
+  // nz), keeping at least 4096 waves (4 per SIMD of the chip) in flight
   const int czs[] = {1, 2, 4, 8, 16, 32};
   for (int cz : czs) {
     const Tile t = make_rows(G, cz);
+    const long waves = (long)G.nseg * ((G.maxnz + t.CZ - 1) / t.CZ);
+    if (t.CZ > 1 && waves < 4096) continue;
     bool dup = false;
     for (auto &o : out) dup |= o.kind == STAGE_ROWS && o.CZ == t.CZ;
     if (!dup) out.push_back(t);
