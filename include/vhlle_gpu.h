@@ -254,6 +254,10 @@ typedef struct {
 } vh_hotspot_cfg_t;
 int vh_hotspot_ic(const vh_hotspot_cfg_t *cfg, int ngrid, const vh_grid_t *grids,
                   double *e, int *npart, double *spots, long max_spots);
+/* Achieved HBM bandwidth (GB/s, read + write) of a plain 16-byte-vector copy
+ * of `bytes` bytes, averaged over reps launches (no counterpart in the
+ * reference: the measured bandwidth reference of the rooflines). */
+int vh_copy_bandwidth(long bytes, int reps, double *gbs);
 /* device/driver information line (name, CUs, clocks) into buf */
 int vh_device_info(char *buf, int len);
 

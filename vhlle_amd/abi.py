@@ -122,6 +122,7 @@ def lib():
         "vh_get_stage": (ctypes.c_int, [vp, ip, ip, _dp]),
         "vh_hotspot_ic": (ctypes.c_int, [ctypes.POINTER(HotspotCfg), ctypes.c_int,
                                          ctypes.POINTER(Grid), _dp, ip, _dp, ctypes.c_long]),
+        "vh_copy_bandwidth": (ctypes.c_int, [ctypes.c_long, ctypes.c_int, _dp]),
         "vh_create_batch": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Grid), ctypes.POINTER(Eos),
                                            ctypes.POINTER(Trc), ctypes.c_int, ctypes.POINTER(vp)]),
     }
@@ -138,7 +139,8 @@ def exported_symbols():
             "vh_step", "vh_run", "vh_check_dtau", "vh_set_c2p_mode", "vh_enable_timing",
             "vh_kernel_times", "vh_c2p", "vh_p2c", "vh_device_info", "vh_mlp_init",
             "vh_mlp_forward", "vh_mlp_grad", "vh_mlp_train", "vh_c2p_learned",
-            "vh_set_c2p_model", "vh_mlp_harvest", "vh_set_stage", "vh_get_stage", "vh_create_batch", "vh_hotspot_ic"]
+            "vh_set_c2p_model", "vh_mlp_harvest", "vh_set_stage", "vh_get_stage", "vh_create_batch", "vh_hotspot_ic",
+            "vh_copy_bandwidth"]
 
 
 def _p(a):
@@ -148,6 +150,13 @@ def _p(a):
 def check(rc, what="call"):
     if rc != VH_OK:
         raise RuntimeError(f"vhlle_gpu {what} failed with code {rc}")
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """GB/s of a plain device copy (read + write bytes / time)."""
+    g = ctypes.c_double(0.0)
+    check(lib().vh_copy_bandwidth(nbytes, reps, ctypes.byref(g)), "vh_copy_bandwidth")
+    return g.value
 
 
 def device_info():

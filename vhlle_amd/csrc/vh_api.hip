@@ -864,6 +864,61 @@ int vh_p2c(const double *prim, long n, double *Q) {
   return VH_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// plain streaming copy (16-byte vectors, four per thread, loads issued
+// before stores): the HBM bandwidth reference for the rooflines and the
+// scale of the traffic counters
+__global__ void __launch_bounds__(256)
+k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, long n) {
+  const long i0 = (long)blockIdx.x * 1024 + threadIdx.x;
+  uint4 v[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (i0 + 256 * k < n) v[k] = src[i0 + 256 * k];
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (i0 + 256 * k < n) dst[i0 + 256 * k] = v[k];
+}
+
+}  // namespace
+
+extern "C" {
+
+int vh_copy_bandwidth(long bytes, int reps, double *gbs) {
+  if (bytes < 16 * 256 || reps < 1 || !gbs) return VH_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return VH_ERR_NODEVICE;
+  const long n = bytes / 16;
+  uint4 *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = VH_OK;
+  if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess ||
+      hipMemset(a, 1, n * 16) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess)
+    rc = VH_ERR_HIP;
+  if (rc == VH_OK) {
+    const unsigned grid = (unsigned)((n + 1023) / 1024);
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, a, b, n);
+    (void)hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, a, b, n);
+    (void)hipEventRecord(e1, 0);
+    float ms = 0.f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess ||
+        hipGetLastError() != hipSuccess)
+      rc = VH_ERR_HIP;
+    else
+      *gbs = 2.0 * n * 16.0 * reps / (ms * 1e-3) / 1e9;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
+}
+
 int vh_device_info(char *buf, int len) {
   if (!buf || len <= 0) return VH_ERR_ARG;
   int dev = 0, n = 0;

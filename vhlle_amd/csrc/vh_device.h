@@ -230,6 +230,17 @@ struct DevTrc {
 };
 
 // --------------------------------------------------------------- geometry
+// Workgroups are dispatched to the 8 XCDs of the chip round-robin (workgroup
+// b runs on XCD b % 8), each XCD with its own L2.  This renumbering gives XCD
+// x one contiguous range of workgroup indices, so that the stencil neighbours
+// a workgroup reads were mostly fetched into the same L2 by its neighbours.
+// A bijection of [0, gridDim.x).
+__device__ __forceinline__ long xcd_block() {
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned q = nb >> 3, r = nb & 7u, x = b & 7u, i = b >> 3;
+  return (long)x * q + (x < r ? x : r) + i;
+}
+
 // Grid geometry.  N: cells of this grid; base: index of its cell 0 in the
 // state arrays; P: the arrays' component pitch (cells of all grids stored
 // together; base = 0 and P = N for a single grid).

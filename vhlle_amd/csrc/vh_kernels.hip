@@ -63,7 +63,7 @@ k_faces(Geom g, DevEos eos, const double *__restrict__ Qs, const double *__restr
         int m_full, const double *__restrict__ tp, double dt, double *__restrict__ F,
         unsigned char *__restrict__ flag, int *__restrict__ status) {
   const double tau = *tp;
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long c = xcd_block() * blockDim.x + threadIdx.x;
   if (c >= g.N) return;
   int ix, iy, iz;
   decompose(g, c, ix, iy, iz);
@@ -195,7 +195,7 @@ k_update(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restr
          const double *__restrict__ tp, double dt, double *__restrict__ out,
          int *__restrict__ status) {
   const double tau = *tp;
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long c = xcd_block() * blockDim.x + threadIdx.x;
   if (c >= g.N) return;
   int ix, iy, iz;
   decompose(g, c, ix, iy, iz);
@@ -481,7 +481,7 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
         const double *__restrict__ tp, double dt, double *__restrict__ pi0, double *__restrict__ piH0, double *__restrict__ Pi0,
         double *__restrict__ PiH0, int *__restrict__ status) {
   const double tau = *tp;
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long c = xcd_block() * blockDim.x + threadIdx.x;
   if (c >= g.N) return;
   int ix, iy, iz;
   decompose(g, c, ix, iy, iz);
@@ -758,7 +758,7 @@ k_visc2(Geom g, const double *__restrict__ ph, const double *__restrict__ pi0,
         double *__restrict__ Pi, double *__restrict__ PiH, double *__restrict__ vcc,
         double *__restrict__ vf, const double *__restrict__ tp, double dt) {
   const double tau = *tp;
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long c = xcd_block() * blockDim.x + threadIdx.x;
   if (c >= g.N) return;
   int ix, iy, iz;
   decompose(g, c, ix, iy, iz);
@@ -899,7 +899,7 @@ k_visc3(Geom g, const double *__restrict__ ph, const double *__restrict__ piH,
         const double *__restrict__ m, int m_full, const double *__restrict__ tp, double dt,
         double *__restrict__ Q) {
   const double tau = *tp;
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long c = xcd_block() * blockDim.x + threadIdx.x;
   if (c >= g.N) return;
   int ix, iy, iz;
   decompose(g, c, ix, iy, iz);

@@ -348,9 +348,10 @@ k_stage(Grids G, DevEos eos, const double *__restrict__ Q, const double *__restr
         const double *__restrict__ tp, double dt, double *__restrict__ out,
         int *__restrict__ status) {
   const double tau = *tp;
-  const int gi = G.bev[blockIdx.x];
+  const long bk = blockIdx.x;
+  const int gi = G.bev[bk];
   const Geom g = G.ev[gi];
-  const long cl = (long)(blockIdx.x - G.bfirst[gi]) * blockDim.x + threadIdx.x;
+  const long cl = (bk - G.bfirst[gi]) * blockDim.x + threadIdx.x;
   if (cl >= g.N) return;
   const long c = g.base + cl;
   int ix, iy, iz;
@@ -565,9 +566,10 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
         const double *__restrict__ tp, double dt, float4 *__restrict__ rec,
         int *__restrict__ status) {
   const double tau = *tp;
-  const int gi = G.bev[blockIdx.x];
+  const long bk = blockIdx.x;
+  const int gi = G.bev[bk];
   const Geom g = G.ev[gi];
-  const long cl = (long)(blockIdx.x - G.bfirst[gi]) * blockDim.x + threadIdx.x;
+  const long cl = (bk - G.bfirst[gi]) * blockDim.x + threadIdx.x;
   if (cl >= g.N) return;
   const long c = g.base + cl;
   int ix, iy, iz;
@@ -768,9 +770,10 @@ k_visc2(Grids G, const double *__restrict__ ph, const float4 *__restrict__ rec,
         float *__restrict__ Pi, float *__restrict__ PiH, double *__restrict__ vcc,
         float *__restrict__ vf, const double *__restrict__ tp, double dt) {
   const double tau = *tp;
-  const int gi = G.bev[blockIdx.x];
+  const long bk = blockIdx.x;
+  const int gi = G.bev[bk];
   const Geom g = G.ev[gi];
-  const long cl = (long)(blockIdx.x - G.bfirst[gi]) * blockDim.x + threadIdx.x;
+  const long cl = (bk - G.bfirst[gi]) * blockDim.x + threadIdx.x;
   if (cl >= g.N) return;
   const long c = g.base + cl;
   int ix, iy, iz;
@@ -894,9 +897,10 @@ k_visc3(Grids G, const double *__restrict__ ph, const float *__restrict__ piH,
         const double *__restrict__ m, int m_full, const double *__restrict__ tp, double dt,
         double *__restrict__ Q) {
   const double tau = *tp;
-  const int gi = G.bev[blockIdx.x];
+  const long bk = blockIdx.x;
+  const int gi = G.bev[bk];
   const Geom g = G.ev[gi];
-  const long cl = (long)(blockIdx.x - G.bfirst[gi]) * blockDim.x + threadIdx.x;
+  const long cl = (bk - G.bfirst[gi]) * blockDim.x + threadIdx.x;
   if (cl >= g.N) return;
   const long c = g.base + cl;
   int ix, iy, iz;
