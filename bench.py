@@ -79,6 +79,31 @@ ALG_BYTES = {
         "visc_flux_update": 32 + 44 + 16 + 56 + 56,
     },
 }
+# HBM bytes per launch measured by rocprofv3 (profiles/traffic_viscous_200x200x100.md):
+# 2 x FETCH_SIZE + WRITE_SIZE in KiB, the factors those counters need to read
+# 1 GiB for a plain copy of 1 GiB.  Keyed by (precision, kernel, stage kernel
+# shape, con2prim); None where not profiled.
+KIB = 1024
+TRAFFIC = {
+    ("fp64", "visc_relax", None, "reference"): (2 * 610117 + 687500) * KIB,
+    ("fp64", "visc_advect", None, "reference"): (2 * 736369 + 843750) * KIB,
+    ("mixed", "stage_pred", "cells", "closed"): (2 * 802101 + 218750) * KIB,
+    ("mixed", "stage_corr", "cells", "closed"): (2 * 932879 + 218750) * KIB,
+    ("mixed", "stage_pred", "rows", "learned"): (2 * 756584 + 224744) * KIB,
+    ("mixed", "stage_corr", "rows", "learned"): (2 * 964323 + 225744) * KIB,
+    ("mixed", "visc_advect", None, None): (2 * 1946175 + 437500) * KIB,
+    ("mixed", "visc_relax", None, None): (2 * 596154 + 375000) * KIB,
+    ("mixed", "visc_flux_update", None, None): (2 * 570616 + 125000) * KIB,
+}
+
+
+def traffic_of(prec, kernel, stage, c2p):
+    for key in ((prec, kernel, stage, c2p), (prec, kernel, None, c2p), (prec, kernel, None, None)):
+        if key in TRAFFIC:
+            return TRAFFIC[key]
+    return None
+
+
 # whole-step compulsory state traffic per cell-update:
 # 2*(7*s_Q + 11*s_pi) -- FP64 288 B, mixed (pi in FP32) 200 B
 STEP_BYTES = {"fp64": 288.0, "mixed": 200.0}
@@ -201,14 +226,16 @@ def state_errors(a, b, mask):
     return eT, epi
 
 
-def roofline(kt, N, prec):
+def roofline(kt, N, prec, stage=None, c2p=None):
     name, (launches, ms) = max(kt.items(), key=lambda kv: kv[1][1])
     avg_s = ms / 1e3 / launches
     alg = ALG_BYTES[prec].get(name, 0) * N
     achieved = alg / avg_s / 1e9
+    tr = traffic_of(prec, name, stage, c2p)
     return dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None, kernel=name,
-                avg_ms=round(avg_s * 1e3, 4), alg_bytes_per_launch=alg)
+                frac=round(achieved / HBM_PEAK_GBS, 5), traffic=tr, kernel=name,
+                avg_ms=round(avg_s * 1e3, 4), alg_bytes_per_launch=alg,
+                traffic_GBs=round(tr / avg_s / 1e9, 1) if tr else None)
 
 
 def cpu_baseline(cfg):
@@ -350,7 +377,8 @@ def main():
             err_T_vs_fp64=eT, err_pi_vs_fp64=epi,
             step_hbm_frac=round(STEP_BYTES[prec] * v / (HBM_PEAK_GBS * 1e9), 5),
             speedup_vs_fp64=round(v / (N * args.steps / results["fp64"]["elapsed"]), 3),
-            roofline=roofline(r["kernel_times"], N, prec),
+            roofline=roofline(r["kernel_times"], N, prec,
+                              r["stage"]["kernel"] if r["stage"] else None, c2p),
             kernels_ms={k: round(t[1], 3) for k, t in sorted(r["kernel_times"].items())})
         if r["stage"]:
             variants[name]["stage_kernel"] = r["stage"]
