@@ -254,6 +254,36 @@ typedef struct {
 } vh_hotspot_cfg_t;
 int vh_hotspot_ic(const vh_hotspot_cfg_t *cfg, int ngrid, const vh_grid_t *grids,
                   double *e, int *npart, double *spots, long max_spots);
+/* ---- freeze-out hypersurface (Fluid::outputSurface, src/fld.cpp:517-874;
+ * Cornelius, src/cornelius.cpp:780-2205) ----
+ * Surface elements of e = e_crit between a lower time level and the current
+ * state at tau, one or more per interior hypercube (ix, iy, iz in [2, n-3)),
+ * in the reference's loop order.  lower = VH_FO_QPREV: the previous step's Q
+ * at tau - dt (outputSurface after every step, dt = the Fluid's dt);
+ * VH_FO_SNAPSHOT: the state saved by vh_fo_snapshot (dt ignored).  rows (may
+ * be NULL): up to max_rows records of 27 doubles (29 with extend) in the
+ * column order of the reference's freezeout.dat (OUTPI): tau, x, y, eta;
+ * dsigma_mu (4); u^mu (4); T, mu_B, mu_Q, mu_S; pi^{mu nu} (10, Cartesian,
+ * index44 order); Pi [; e, n_b].  *nrows: the element count.  stats (may be
+ * NULL): sums over the elements (the reference's diagnostics).  Single-grid
+ * contexts only. */
+enum { VH_FO_QPREV = 0, VH_FO_SNAPSHOT = 1 };
+typedef struct {
+  long nelements, nsusp;       /* elements; those with e_C outside [e_crit/2, 2 e_crit] */
+  double dsigma[4];            /* sum of dsigma_mu */
+  double v_eff;                /* sum of dsigma_mu u^mu (vEff increment) */
+  double e_surf, nb_surf;      /* EtotSurf and nbSurf increments */
+  int err;                     /* hypercubes the construction could not resolve (bits) */
+} vh_fo_stats_t;
+int vh_fo_snapshot(vh_ctx_t *ctx, double tau);
+int vh_freezeout(vh_ctx_t *ctx, double tau, double dt, double e_crit, int extend, int lower,
+                 double *rows, long max_rows, long *nrows, vh_fo_stats_t *stats);
+/* The Cornelius construction alone on the GPU (Cornelius::find_surface_4d):
+ * n hypercubes h (n x 16 corner values, index [t][x][y][z]), edge lengths
+ * dx[4] (t, x, y, z), level E0; cen, nrm: n x 10 x 4 (unused elements zero);
+ * nel: elements per hypercube (negative: the construction failed). */
+int vh_cornelius(const double *h, long n, const double *dx, double E0, double *cen,
+                 double *nrm, int *nel);
 /* Achieved HBM bandwidth (GB/s, read + write) of a plain 16-byte-vector copy
  * of `bytes` bytes, averaged over reps launches (no counterpart in the
  * reference: the measured bandwidth reference of the rooflines). */

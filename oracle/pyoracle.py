@@ -50,6 +50,11 @@ FIELDS = {"Q": 7, "Qh": 7, "Qprev": 7, "pi": 10, "piH": 10, "Pi": 1, "PiH": 1,
           "pi0": 10, "piH0": 10, "Pi0": 1, "PiH0": 1, "m": 3, "viscCorrCut": 1}
 
 
+class FoGrid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int)] + [
+        (n, ctypes.c_double) for n in ("dx", "dy", "dz", "xmin", "ymin", "zmin")]
+
+
 def make_trc(etaS=0.0, zetaS=0.0, etaSparam=0, zetaSparam=0, **kw):
     """TransportCoeff with the reference driver's defaults (src/main.cpp:70-81)."""
     d = dict(etaS0=etaS, zetaS0=zetaS, ah=0.0, al=0.0, aRho=0.0, T0=0.15,
@@ -87,6 +92,12 @@ def lib():
         L.or_p2c.argtypes = [dp, ctypes.c_long, dp]
         L.or_eos.argtypes = [ctypes.c_int, dp, ctypes.c_long, dp]
         L.or_trcoeff.argtypes = [ctypes.POINTER(Trc), ctypes.c_int, dp, ctypes.c_long, dp]
+        L.or_cornelius.argtypes = [dp, ctypes.c_long, dp, ctypes.c_double, dp, dp,
+                                   ctypes.POINTER(ctypes.c_int)]
+        L.or_freezeout.argtypes = [ctypes.POINTER(FoGrid), dp, dp, dp, dp, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_int, dp,
+                                   ctypes.c_long]
+        L.or_freezeout.restype = ctypes.c_long
         _LIB = L
     return _LIB
 
@@ -174,3 +185,32 @@ class Solver:
         t = ctypes.c_double(tau)
         rc = lib().or_run(self.h, ctypes.byref(t), dt, nsteps)
         return rc, t.value
+
+
+def cornelius(h, dx, E0):
+    """Cornelius::find_surface_4d over hypercubes h (n x 2 x 2 x 2 x 2):
+    (n_elements per hypercube, centroids n x 10 x 4, normals n x 10 x 4)."""
+    h = np.ascontiguousarray(h, dtype=np.float64).reshape(-1, 16)
+    n = h.shape[0]
+    cen = np.zeros((n, 10, 4))
+    nrm = np.zeros((n, 10, 4))
+    nel = np.zeros(n, dtype=np.int32)
+    d = np.ascontiguousarray(dx, dtype=np.float64)
+    lib().or_cornelius(_p(h), n, _p(d), E0, _p(cen), _p(nrm),
+                       nel.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return nel, cen, nrm
+
+
+def freezeout(shape, spacing, mins, Q, Qlow, pi, Pi, tau, dt, ecrit, extend=False,
+              max_rows=1000000):
+    """Fluid::outputSurface's element records from SoA state arrays."""
+    g = FoGrid(shape[0], shape[1], shape[2], spacing[0], spacing[1], spacing[2], mins[0],
+               mins[1], mins[2])
+    ncol = 29 if extend else 27
+    rows = np.zeros((max_rows, ncol))
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (Q, Qlow, pi, Pi)]
+    n = lib().or_freezeout(ctypes.byref(g), *[_p(a) for a in arrs], tau, dt, ecrit,
+                           int(extend), _p(rows), max_rows)
+    if n < 0:
+        raise RuntimeError("or_freezeout: construction failed")
+    return rows[:min(n, max_rows)]

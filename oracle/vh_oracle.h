@@ -75,6 +75,22 @@ void or_eos(int eos_kind, const double *e, long n, double *out);
 void or_trcoeff(const or_trc_t *trc, int eos_kind, const double *in, long n,
                 double *out);
 
+/* ---- freeze-out (oracle/fo_oracle.cpp) ---- */
+/* Cornelius::find_surface_4d on n hypercubes h (n x 16, [t][x][y][z]);
+ * cen, nrm: n x 10 x 4; nel: element counts (negative: failed) */
+int or_cornelius(const double *h, long n, const double dx[4], double E0, double *cen,
+                 double *nrm, int *nel);
+typedef struct {
+  int nx, ny, nz;
+  double dx, dy, dz, xmin, ymin, zmin;
+} or_fo_grid_t;
+/* Fluid::outputSurface's element records (27 or 29 columns, see
+ * fo_oracle.cpp) from SoA Q, Qlow (7 x N), pi (10 x N), Pi (N); returns the
+ * element count (rows beyond max_rows are counted, not written), -1 on a
+ * construction failure. */
+long or_freezeout(const or_fo_grid_t *g, const double *Q, const double *Qlow, const double *pi,
+                  const double *Pi, double tau, double dt, double ecrit, int extend,
+                  double *rows, long max_rows);
 #ifdef __cplusplus
 }
 #endif

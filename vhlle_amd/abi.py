@@ -16,6 +16,7 @@ ST_NAN_FLUX, ST_BIG_E, ST_NAN_SOURCE, ST_NAN_PI = 1, 2, 4, 8
 PREC_FP64, PREC_MIXED = 0, 1
 C2P_REFERENCE, C2P_LEARNED, C2P_CLOSED, C2P_NEWTON = 0, 1, 2, 3
 START_MLP, START_HALF = 0, 1
+FO_QPREV, FO_SNAPSHOT = 0, 1
 STAGE_AUTO, STAGE_CELLS, STAGE_ROWS = 0, 1, 2
 MLP_NPARAM = 4674
 
@@ -57,6 +58,13 @@ class HarvestCfg(ctypes.Structure):
 class HotspotCfg(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_ulonglong)] + [(n, ctypes.c_double) for n in (
         "npart_mean", "R", "a", "sigma", "amp", "eta_plateau", "eta_sigma")]
+
+
+class FoStats(ctypes.Structure):
+    _fields_ = [("nelements", ctypes.c_long), ("nsusp", ctypes.c_long),
+                ("dsigma", ctypes.c_double * 4), ("v_eff", ctypes.c_double),
+                ("e_surf", ctypes.c_double), ("nb_surf", ctypes.c_double),
+                ("err", ctypes.c_int)]
 
 
 class State(ctypes.Structure):
@@ -123,6 +131,11 @@ def lib():
         "vh_hotspot_ic": (ctypes.c_int, [ctypes.POINTER(HotspotCfg), ctypes.c_int,
                                          ctypes.POINTER(Grid), _dp, ip, _dp, ctypes.c_long]),
         "vh_copy_bandwidth": (ctypes.c_int, [ctypes.c_long, ctypes.c_int, _dp]),
+        "vh_fo_snapshot": (ctypes.c_int, [vp, ctypes.c_double]),
+        "vh_freezeout": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int, ctypes.c_int, _dp, ctypes.c_long,
+                                        ctypes.POINTER(ctypes.c_long), ctypes.POINTER(FoStats)]),
+        "vh_cornelius": (ctypes.c_int, [_dp, ctypes.c_long, _dp, ctypes.c_double, _dp, _dp, ip]),
         "vh_create_batch": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Grid), ctypes.POINTER(Eos),
                                            ctypes.POINTER(Trc), ctypes.c_int, ctypes.POINTER(vp)]),
     }
@@ -140,7 +153,7 @@ def exported_symbols():
             "vh_kernel_times", "vh_c2p", "vh_p2c", "vh_device_info", "vh_mlp_init",
             "vh_mlp_forward", "vh_mlp_grad", "vh_mlp_train", "vh_c2p_learned",
             "vh_set_c2p_model", "vh_mlp_harvest", "vh_set_stage", "vh_get_stage", "vh_create_batch", "vh_hotspot_ic",
-            "vh_copy_bandwidth"]
+            "vh_copy_bandwidth", "vh_fo_snapshot", "vh_freezeout", "vh_cornelius"]
 
 
 def _p(a):
@@ -265,6 +278,20 @@ def hotspot_ic(grids, seed, npart_mean=150.0, R=6.6, a=0.55, sigma=0.5, amp=1.0,
     return e, npart
 
 
+def cornelius(h, dx, E0):
+    """The Cornelius construction on the GPU over hypercubes h (n x 2 x 2 x 2 x 2):
+    (elements per hypercube, centroids n x 10 x 4, normals n x 10 x 4)."""
+    h = np.ascontiguousarray(h, dtype=np.float64).reshape(-1, 16)
+    n = h.shape[0]
+    cen = np.zeros((n, 10, 4))
+    nrm = np.zeros((n, 10, 4))
+    nel = np.zeros(n, dtype=np.int32)
+    d = np.ascontiguousarray(dx, dtype=np.float64)
+    check(lib().vh_cornelius(_p(h), n, _p(d), E0, _p(cen), _p(nrm),
+                             nel.ctypes.data_as(ctypes.POINTER(ctypes.c_int))), "vh_cornelius")
+    return nel, cen, nrm
+
+
 def p2c(prim):
     prim = np.ascontiguousarray(prim, dtype=np.float64)
     Q = np.zeros((prim.shape[0], 7))
@@ -362,6 +389,25 @@ class Context:
         check(lib().vh_get_stage(self.h, ctypes.byref(k), ctypes.byref(cz), ctypes.byref(ms)),
               "vh_get_stage")
         return k.value, cz.value, ms.value
+
+    def fo_snapshot(self, tau):
+        check(lib().vh_fo_snapshot(self.h, tau), "vh_fo_snapshot")
+
+    def freezeout(self, tau, dt, ecrit, extend=False, lower=FO_QPREV, max_rows=2000000):
+        """Surface elements (rows of 27 or 29 columns) and the stats."""
+        ncol = 29 if extend else 27
+        n = ctypes.c_long(0)
+        st = FoStats()
+        check(lib().vh_freezeout(self.h, tau, dt, ecrit, int(extend), lower, None, 0,
+                                 ctypes.byref(n), ctypes.byref(st)), "vh_freezeout")
+        rows = np.zeros((max(min(n.value, max_rows), 1), ncol))
+        if n.value:
+            check(lib().vh_freezeout(self.h, tau, dt, ecrit, int(extend), lower, _p(rows),
+                                     min(n.value, max_rows), ctypes.byref(n), ctypes.byref(st)),
+                  "vh_freezeout")
+        return rows[:min(n.value, max_rows)], dict(
+            nelements=st.nelements, nsusp=st.nsusp, dsigma=list(st.dsigma), v_eff=st.v_eff,
+            e_surf=st.e_surf, nb_surf=st.nb_surf, err=st.err)
 
     def enable_timing(self, on=True):
         check(lib().vh_enable_timing(self.h, 1 if on else 0), "vh_enable_timing")

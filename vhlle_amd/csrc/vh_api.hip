@@ -14,6 +14,7 @@
 #include "vh_kernels.h"
 #include "vh_kernels_mixed.h"
 #include "vh_mlp_host.h"
+#include "vh_freezeout.h"
 
 #define VH_HIP(call)                                                         \
   do {                                                                       \
@@ -66,6 +67,9 @@ struct vh_ctx {
   // guesses (mixed precision only; gv reuses the FP64 path's F[0] storage)
   vh::ml::Packed *mlp = nullptr;
   float *gv = nullptr;
+  // freeze-out snapshot (the lower time level of vh_freezeout)
+  double *fo_snap = nullptr;
+  double fo_tau = 0.;
   // grids of the context (one, or a batch of independent grids sharing the
   // state arrays) and their device lookup tables (mixed precision kernels)
   std::vector<vh::Geom> evh;
@@ -533,6 +537,7 @@ int vh_destroy(vh_ctx_t *c) {
   }
   if (c->pool) (void)hipFree(c->pool);
   if (c->gtab) (void)hipFree(c->gtab);
+  if (c->fo_snap) (void)hipFree(c->fo_snap);
   if (c->mlp) (void)hipFree(c->mlp);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
@@ -886,6 +891,83 @@ k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, long n) {
 }  // namespace
 
 extern "C" {
+
+int vh_fo_snapshot(vh_ctx_t *c, double tau) {
+  if (!c || c->evh.size() != 1) return VH_ERR_ARG;
+  const long N = c->g.N;
+  if (!c->fo_snap) VH_HIP(hipMalloc(&c->fo_snap, 7 * N * sizeof(double)));
+  VH_HIP(hipMemcpyAsync(c->fo_snap, c->Q[c->cur], 7 * N * sizeof(double),
+                        hipMemcpyDeviceToDevice, c->st));
+  VH_HIP(hipStreamSynchronize(c->st));
+  c->fo_tau = tau;
+  return VH_OK;
+}
+
+int vh_freezeout(vh_ctx_t *c, double tau, double dt, double e_crit, int extend, int lower,
+                 double *rows, long max_rows, long *nrows, vh_fo_stats_t *stats) {
+  if (!c || c->evh.size() != 1 || (lower != VH_FO_QPREV && lower != VH_FO_SNAPSHOT) ||
+      (rows && max_rows < 0))
+    return VH_ERR_ARG;
+  if (lower == VH_FO_SNAPSHOT && !c->fo_snap) return VH_ERR_ARG;
+  const double edge = lower == VH_FO_SNAPSHOT ? tau - c->fo_tau : dt;
+  if (!(edge > 0.)) return VH_ERR_ARG;
+  VH_HIP(hipStreamSynchronize(c->st));
+  vh::fo::FoArgs a;
+  a.nx = c->g.nx;
+  a.ny = c->g.ny;
+  a.nz = c->g.nz;
+  a.eos = c->eos;
+  a.P.tau = tau;
+  a.P.dt = edge;
+  a.P.dx = c->g.dx;
+  a.P.dy = c->g.dy;
+  a.P.dz = c->g.dz;
+  a.P.xmin = c->grid.xmin;
+  a.P.ymin = c->grid.ymin;
+  a.P.zmin = c->grid.zmin;
+  a.P.ecrit = e_crit;
+  const int rec = extend ? vh::fo::ROW_EXT : vh::fo::ROW_BASE;
+  a.P.ncol = rec + vh::fo::AUX;
+  a.Q = c->Q[c->cur];
+  a.Qlow = lower == VH_FO_SNAPSHOT ? c->fo_snap : c->Q[c->cur ^ 1];
+  a.pi = c->pi;
+  a.Pi = c->Pi;
+  a.pi_float = c->precision == VH_PREC_MIXED;
+  a.stream = c->st;
+  std::vector<double> r;
+  long n = 0;
+  int err = 0;
+  const int rc = vh::fo::run(a, r, n, err);
+  if (rc != VH_OK) return rc;
+  if (nrows) *nrows = n;
+  if (rows)
+    for (long i = 0; i < n && i < max_rows; i++)
+      std::memcpy(rows + i * rec, &r[i * a.P.ncol], rec * sizeof(double));
+  if (stats) {
+    std::memset(stats, 0, sizeof *stats);
+    stats->nelements = n;
+    stats->err = err;
+    for (long i = 0; i < n; i++) {
+      const double *row = &r[i * a.P.ncol];
+      const double *aux = row + rec;
+      for (int k = 0; k < 4; k++) stats->dsigma[k] += row[4 + k];
+      stats->v_eff += aux[0];
+      stats->e_surf += aux[1];
+      stats->nb_surf += aux[2];
+      if (aux[3] > e_crit * 2.0 || aux[3] < e_crit * 0.5) stats->nsusp++;
+    }
+  }
+  return VH_OK;
+}
+
+int vh_cornelius(const double *h, long n, const double *dx, double E0, double *cen,
+                 double *nrm, int *nel) {
+  if (!h || n < 0 || !dx || !cen || !nrm || !nel) return VH_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return VH_ERR_NODEVICE;
+  if (n == 0) return VH_OK;
+  return vh::fo::cornelius_batch(h, n, dx, E0, cen, nrm, nel);
+}
 
 int vh_copy_bandwidth(long bytes, int reps, double *gbs) {
   if (bytes < 16 * 256 || reps < 1 || !gbs) return VH_ERR_ARG;

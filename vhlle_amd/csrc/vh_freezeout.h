@@ -1,0 +1,45 @@
+// Host interface of the freeze-out kernels (vh_freezeout.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "vh_device.h"
+
+namespace vh {
+namespace fo {
+
+// columns of one surface element: the record of the reference's
+// freezeout.dat (src/fld.cpp:656-797 with OUTPI: centroid tau, x, y, eta;
+// dsigma_mu (4); u^mu (4); T, mu_B, mu_Q, mu_S; pi^{mu nu} Cartesian (10);
+// Pi) = 27 columns, 29 with the extended output (e, n_b), then AUX columns:
+// dV_eff = dsigma_mu u^mu, the element's EtotSurf and nbSurf increments, e.
+constexpr int ROW_BASE = 27, ROW_EXT = 29, AUX = 4;
+
+struct FoParams {
+  double tau, dt, dx, dy, dz, xmin, ymin, zmin, ecrit;
+  int ncol;  // record columns + AUX
+};
+
+struct Dx4 {
+  double v[4];
+};
+
+struct FoArgs {
+  int nx, ny, nz;
+  DevEos eos;
+  FoParams P;
+  const double *Q, *Qlow;     // upper (tau) and lower (tau - dt) levels, 7 x N
+  const void *pi, *Pi;        // pi (10 x N), Pi (N) of the upper level
+  int pi_float;               // pi, Pi stored in FP32
+  hipStream_t stream;
+};
+
+// elements in the reference's loop order, rows of P.ncol doubles
+int run(const FoArgs &a, std::vector<double> &rows, long &nrows, int &errbits);
+// the construction alone on n hypercubes h (n x 16, [t][x][y][z]) on the GPU
+int cornelius_batch(const double *h, long n, const double *dx, double E0, double *cen,
+                    double *nrm, int *nel);
+
+}  // namespace fo
+}  // namespace vh

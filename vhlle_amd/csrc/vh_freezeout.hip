@@ -1,0 +1,356 @@
+// Freeze-out hypersurface of the hydro state (Fluid::outputSurface,
+// src/fld.cpp:517-874, with the Cornelius construction of vh_cornelius.h).
+//
+// Between a lower time level (the previous step's Q, or a snapshot) and the
+// current state, every interior hypercube (ix, iy, iz in [2, n-3), the
+// reference's loop range) whose 16 corner energy densities straddle e_crit
+// yields surface elements.  Kernels: e at both levels per cell (the
+// reference con2prim); a cut flag per hypercube; ordered compaction of the
+// cut hypercubes (hipcub, so that elements come out in the reference's
+// ix, iy, iz loop order); one thread per cut hypercube for the geometry and
+// the element record: quadrilinear Q at the centroid, con2prim, trilinear
+// pi, Pi of the upper level, u^mu in the lab frame and dsigma_mu transformed
+// from Milne to Cartesian components with element 0's normal (as the
+// reference does for every element of a hypercube).  FP64 throughout.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/vhlle_gpu.h"
+#include "vh_cornelius.h"
+#include "vh_freezeout.h"
+
+namespace vh {
+namespace fo {
+
+namespace {
+
+// e of every cell at both levels: c2p of Q/tau and of Qlow/tau_low
+__global__ void __launch_bounds__(256)
+k_fo_energy(long N, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qlow,
+            double tau, double tlow, double *__restrict__ e1, double *__restrict__ e0) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  double q[7];
+  Prim r;
+  for (int k = 0; k < 7; k++) q[k] = Q[k * N + c] / tau;
+  c2p_ref(eos, q, r);
+  e1[c] = r.e;
+  for (int k = 0; k < 7; k++) q[k] = Qlow[k * N + c] / tlow;
+  c2p_ref(eos, q, r);
+  e0[c] = r.e;
+}
+
+struct CubeGeom {
+  int nx, ny, nz, mx, my, mz;  // grid, and interior hypercubes per axis
+  long N;
+};
+
+__device__ __forceinline__ void cube_origin(const CubeGeom &g, long k, int &ix, int &iy, int &iz) {
+  iz = (int)(k % g.mz) + 2;
+  const long r = k / g.mz;
+  iy = (int)(r % g.my) + 2;
+  ix = (int)(r / g.my) + 2;
+}
+
+__global__ void __launch_bounds__(256)
+k_fo_flag(CubeGeom g, const double *__restrict__ e1, const double *__restrict__ e0, double E0,
+          unsigned char *__restrict__ flag) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (long)g.mx * g.my * g.mz) return;
+  int ix, iy, iz;
+  cube_origin(g, k, ix, iy, iz);
+  int above = 0;
+  for (int jx = 0; jx < 2; jx++)
+    for (int jy = 0; jy < 2; jy++)
+      for (int jz = 0; jz < 2; jz++) {
+        const long c = (ix + jx) + (long)g.nx * ((iy + jy) + (long)g.ny * (iz + jz));
+        above += (e1[c] >= E0) + (e0[c] >= E0);
+      }
+  flag[k] = above > 0 && above < 16;
+}
+
+__device__ __forceinline__ void transform_to_lab(double eta, double &vx, double &vy, double &vz) {
+  // transformToLab (src/fld.cpp:508-513)
+  const double Y = eta + 1. / 2. * log((1. + vz) / (1. - vz));
+  vx = vx * cosh(Y - eta) / cosh(Y);
+  vy = vy * cosh(Y - eta) / cosh(Y);
+  vz = tanh(Y);
+}
+
+template <typename PT>
+__global__ void __launch_bounds__(64)
+k_fo_cubes(CubeGeom g, FoParams P, DevEos eos, const long *__restrict__ cut,
+           const int *__restrict__ ncut_p, const double *__restrict__ e1,
+           const double *__restrict__ e0, const double *__restrict__ Q,
+           const double *__restrict__ Qlow, const PT *__restrict__ pi,
+           const PT *__restrict__ Pi, double *__restrict__ rows, int *__restrict__ count,
+           int *__restrict__ err) {
+  const int ncut = *ncut_p;
+  const long N = g.N;
+  const double dxs[4] = {P.dt, P.dx, P.dy, P.dz};
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ncut;
+       t += (long)gridDim.x * blockDim.x) {
+    int ix, iy, iz;
+    cube_origin(g, cut[t], ix, iy, iz);
+    double h[2][2][2][2];
+    long cc[2][2][2];
+    for (int jx = 0; jx < 2; jx++)
+      for (int jy = 0; jy < 2; jy++)
+        for (int jz = 0; jz < 2; jz++) {
+          const long c = (ix + jx) + (long)g.nx * ((iy + jy) + (long)g.ny * (iz + jz));
+          cc[jx][jy][jz] = c;
+          h[1][jx][jy][jz] = e1[c];
+          h[0][jx][jy][jz] = e0[c];
+        }
+    Corn C;
+    double cen[MAXE][4], nrm[MAXE][4];
+    const int ne = hypercube(C, h, dxs, P.ecrit, cen, nrm);
+    if (C.err) atomicOr(err, C.err);
+    count[t] = ne;
+    for (int el = 0; el < ne; el++) {
+      double *row = rows + ((long)t * MAXE + el) * P.ncol;
+      const double *c4 = cen[el];
+      row[0] = P.tau + c4[0];
+      row[1] = P.xmin + ix * P.dx + c4[1];
+      row[2] = P.ymin + iy * P.dy + c4[2];
+      row[3] = P.zmin + iz * P.dz + c4[3];
+      // quadrilinear Q at the centroid (src/fld.cpp:675-695)
+      const double wT[2] = {1. - c4[0] / P.dt, c4[0] / P.dt};
+      const double wX[2] = {1. - c4[1] / P.dx, c4[1] / P.dx};
+      const double wY[2] = {1. - c4[2] / P.dy, c4[2] / P.dy};
+      const double wZ[2] = {1. - c4[3] / P.dz, c4[3] / P.dz};
+      double QC[7] = {0., 0., 0., 0., 0., 0., 0.};
+      for (int jt = 0; jt < 2; jt++)
+        for (int jx = 0; jx < 2; jx++)
+          for (int jy = 0; jy < 2; jy++)
+            for (int jz = 0; jz < 2; jz++) {
+              const double *src = jt ? Q : Qlow;
+              const long c = cc[jx][jy][jz];
+              for (int i = 0; i < 7; i++)
+                QC[i] += src[i * N + c] * wT[jt] * wX[jx] * wY[jy] * wZ[jz];
+            }
+      for (int i = 0; i < 7; i++) QC[i] = QC[i] / (P.tau + c4[0]);
+      Prim pr;
+      c2p_ref(eos, QC, pr);
+      const double eC = pr.e, nbC = pr.nb;
+      double vxC = pr.vx, vyC = pr.vy, vzC = pr.vz;
+      const double TC = eos.T(eC), mubC = 0., muqC = 0., musC = 0.;
+      const double pC = eos.p(eC);
+      // trilinear pi, Pi of the upper level (src/fld.cpp:709-728)
+      double piC[10] = {0., 0., 0., 0., 0., 0., 0., 0., 0., 0.}, PiC = 0.;
+      for (int jx = 0; jx < 2; jx++)
+        for (int jy = 0; jy < 2; jy++)
+          for (int jz = 0; jz < 2; jz++) {
+            const long c = cc[jx][jy][jz];
+            for (int ii = 0; ii < 10; ii++)
+              piC[ii] += (double)pi[ii * N + c] * wX[jx] * wY[jy] * wZ[jz];
+            PiC += (double)Pi[c] * wX[jx] * wY[jy] * wZ[jz];
+          }
+      double v2C = vxC * vxC + vyC * vyC + vzC * vzC;
+      if (v2C > 1.) {
+        vxC *= sqrt(0.99 / v2C);
+        vyC *= sqrt(0.99 / v2C);
+        vzC *= sqrt(0.99 / v2C);
+        v2C = 0.99;
+      }
+      const double etaC = P.zmin + iz * P.dz + c4[3];
+      transform_to_lab(etaC, vxC, vyC, vzC);
+      const double gammaC = 1. / sqrt(1. - vxC * vxC - vyC * vyC - vzC * vzC);
+      const double uC[4] = {gammaC, gammaC * vxC, gammaC * vyC, gammaC * vzC};
+      const double tauC = P.tau + c4[0];
+      const double ch = cosh(etaC), sh = sinh(etaC);
+      const double *n0 = nrm[0];  // element 0's normal for every element (src/fld.cpp:746-751)
+      double ds[4];
+      ds[0] = tauC * (ch * n0[0] - sh / tauC * n0[3]);
+      ds[3] = tauC * (-sh * n0[0] + ch / tauC * n0[3]);
+      ds[1] = tauC * n0[1];
+      ds[2] = tauC * n0[2];
+      for (int i = 0; i < 4; i++) row[4 + i] = ds[i];
+      for (int i = 0; i < 4; i++) row[8 + i] = uC[i];
+      row[12] = TC;
+      row[13] = mubC;
+      row[14] = muqC;
+      row[15] = musC;
+      double pc[10];
+      pc[sym(0, 0)] = ch * ch * piC[sym(0, 0)] + 2. * ch * sh * piC[sym(0, 3)] +
+                      sh * sh * piC[sym(3, 3)];
+      pc[sym(0, 1)] = ch * piC[sym(0, 1)] + sh * piC[sym(3, 1)];
+      pc[sym(0, 2)] = ch * piC[sym(0, 2)] + sh * piC[sym(3, 2)];
+      pc[sym(0, 3)] = ch * sh * (piC[sym(0, 0)] + piC[sym(3, 3)]) +
+                      (ch * ch + sh * sh) * piC[sym(0, 3)];
+      pc[sym(1, 1)] = piC[sym(1, 1)];
+      pc[sym(1, 2)] = piC[sym(1, 2)];
+      pc[sym(1, 3)] = sh * piC[sym(0, 1)] + ch * piC[sym(3, 1)];
+      pc[sym(2, 2)] = piC[sym(2, 2)];
+      pc[sym(2, 3)] = sh * piC[sym(0, 2)] + ch * piC[sym(3, 2)];
+      pc[sym(3, 3)] = sh * sh * piC[sym(0, 0)] + ch * ch * piC[sym(3, 3)] +
+                      2. * sh * ch * piC[sym(0, 3)];
+      for (int i = 0; i < 10; i++) row[16 + i] = pc[i];
+      row[26] = PiC;
+      if (P.ncol > ROW_BASE) {
+        row[27] = eC;
+        row[28] = nbC;
+      }
+      // diagnostics: dVEff, EtotSurf increment, nb flux, e_C
+      double dV = 0.0;
+      for (int ii = 0; ii < 4; ii++) dV += ds[ii] * uC[ii];
+      double dEv = 0.;
+      for (int i = 0; i < 4; i++) dEv += pc[sym(0, i)] * ds[i];
+      double *aux = row + P.ncol - AUX;
+      aux[0] = dV;
+      aux[1] = (eC + pC) * uC[0] * dV - pC * ds[0] + dEv;
+      aux[2] = nbC * dV;
+      aux[3] = eC;
+    }
+  }
+}
+
+}  // namespace
+
+int run(const FoArgs &a, std::vector<double> &rows_out, long &nrows, int &errbits) {
+  const CubeGeom g{a.nx, a.ny, a.nz, a.nx - 4, a.ny - 4, a.nz - 4, (long)a.nx * a.ny * a.nz};
+  nrows = 0;
+  errbits = 0;
+  rows_out.clear();
+  if (g.mx < 1 || g.my < 1 || g.mz < 1) return VH_OK;  // no interior hypercubes
+  const long ncube = (long)g.mx * g.my * g.mz, N = g.N;
+  double *e1 = nullptr, *e0 = nullptr, *rows = nullptr;
+  unsigned char *flag = nullptr;
+  long *ids = nullptr, *cut = nullptr;
+  int *ncut = nullptr, *cnt = nullptr, *err = nullptr;
+  void *tmp = nullptr;
+  size_t tmpb = 0;
+  int rc = VH_OK;
+  auto ok = [&](hipError_t e) {
+    if (e != hipSuccess && rc == VH_OK) {
+      std::fprintf(stderr, "vhlle_gpu freeze-out: %s\n", hipGetErrorString(e));
+      rc = VH_ERR_HIP;
+    }
+    return rc == VH_OK;
+  };
+  hipStream_t st = a.stream;
+  if (ok(hipMalloc(&e1, N * sizeof(double))) && ok(hipMalloc(&e0, N * sizeof(double))) &&
+      ok(hipMalloc(&flag, ncube)) && ok(hipMalloc(&cut, ncube * sizeof(long))) &&
+      ok(hipMalloc(&ncut, 2 * sizeof(int)))) {
+    ok(hipMalloc(&err, sizeof(int)));
+  }
+  if (rc == VH_OK) {
+    hipLaunchKernelGGL(k_fo_energy, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, N,
+                       a.eos, a.Q, a.Qlow, a.P.tau, a.P.tau - a.P.dt, e1, e0);
+    hipLaunchKernelGGL(k_fo_flag, dim3((unsigned)((ncube + 255) / 256)), dim3(256), 0, st, g,
+                       e1, e0, a.P.ecrit, flag);
+    ok(hipMemsetAsync(err, 0, sizeof(int), st));
+    hipcub::CountingInputIterator<long> it(0);
+    ok(hipcub::DeviceSelect::Flagged(nullptr, tmpb, it, flag, cut, ncut, ncube, st));
+  }
+  if (rc == VH_OK && ok(hipMalloc(&tmp, tmpb ? tmpb : 1))) {
+    hipcub::CountingInputIterator<long> it(0);
+    ok(hipcub::DeviceSelect::Flagged(tmp, tmpb, it, flag, cut, ncut, ncube, st));
+  }
+  int nc = 0;
+  if (rc == VH_OK) {
+    ok(hipMemcpyAsync(&nc, ncut, sizeof(int), hipMemcpyDeviceToHost, st));
+    ok(hipStreamSynchronize(st));
+  }
+  std::vector<int> hcnt;
+  if (rc == VH_OK && nc > 0) {
+    const long slots = (long)nc * MAXE;
+    if (ok(hipMalloc(&rows, slots * a.P.ncol * sizeof(double))) &&
+        ok(hipMalloc(&cnt, nc * sizeof(int)))) {
+      const unsigned grid = (unsigned)std::min<long>((nc + 63) / 64, 2048);
+      if (a.pi_float)
+        hipLaunchKernelGGL(k_fo_cubes<float>, dim3(grid), dim3(64), 0, st, g, a.P, a.eos, cut,
+                           ncut, e1, e0, a.Q, a.Qlow, (const float *)a.pi, (const float *)a.Pi,
+                           rows, cnt, err);
+      else
+        hipLaunchKernelGGL(k_fo_cubes<double>, dim3(grid), dim3(64), 0, st, g, a.P, a.eos, cut,
+                           ncut, e1, e0, a.Q, a.Qlow, (const double *)a.pi,
+                           (const double *)a.Pi, rows, cnt, err);
+      ok(hipGetLastError());
+      hcnt.resize(nc);
+      ok(hipMemcpyAsync(hcnt.data(), cnt, nc * sizeof(int), hipMemcpyDeviceToHost, st));
+      ok(hipMemcpyAsync(&errbits, err, sizeof(int), hipMemcpyDeviceToHost, st));
+      ok(hipStreamSynchronize(st));
+    }
+    if (rc == VH_OK) {
+      long total = 0;
+      for (int k = 0; k < nc; k++) total += hcnt[k];
+      rows_out.resize(total * a.P.ncol);
+      std::vector<double> all((size_t)slots * a.P.ncol);
+      ok(hipMemcpy(all.data(), rows, all.size() * sizeof(double), hipMemcpyDeviceToHost));
+      long w = 0;
+      for (int k = 0; k < nc; k++)
+        for (int el = 0; el < hcnt[k]; el++, w++)
+          std::copy(&all[((size_t)k * MAXE + el) * a.P.ncol],
+                    &all[((size_t)k * MAXE + el + 1) * a.P.ncol], &rows_out[w * a.P.ncol]);
+      nrows = total;
+    }
+  }
+  if (e1) (void)hipFree(e1);
+  if (e0) (void)hipFree(e0);
+  if (flag) (void)hipFree(flag);
+  if (cut) (void)hipFree(cut);
+  if (ncut) (void)hipFree(ncut);
+  if (err) (void)hipFree(err);
+  if (tmp) (void)hipFree(tmp);
+  if (rows) (void)hipFree(rows);
+  if (cnt) (void)hipFree(cnt);
+  return rc;
+}
+
+namespace {
+__global__ void __launch_bounds__(64)
+k_cornelius_batch(const double *__restrict__ h, long n, Dx4 dx, double E0,
+                  double *__restrict__ cen, double *__restrict__ nrm, int *__restrict__ nel) {
+  const double d[4] = {dx.v[0], dx.v[1], dx.v[2], dx.v[3]};
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    double hc[2][2][2][2], c[MAXE][4], m[MAXE][4];
+    for (int k = 0; k < 16; k++) (&hc[0][0][0][0])[k] = h[i * 16 + k];
+    Corn C;
+    const int ne = hypercube(C, hc, d, E0, c, m);
+    nel[i] = C.err ? -C.err : ne;
+    for (int e = 0; e < MAXE; e++)
+      for (int k = 0; k < 4; k++) {
+        cen[(i * MAXE + e) * 4 + k] = e < ne ? c[e][k] : 0.;
+        nrm[(i * MAXE + e) * 4 + k] = e < ne ? m[e][k] : 0.;
+      }
+  }
+}
+}  // namespace
+
+int cornelius_batch(const double *h, long n, const double *dx, double E0, double *cen,
+                    double *nrm, int *nel) {
+  double *dh = nullptr, *dc = nullptr, *dn = nullptr;
+  int *de = nullptr;
+  int rc = VH_OK;
+  if (hipMalloc(&dh, n * 16 * sizeof(double)) != hipSuccess ||
+      hipMalloc(&dc, n * MAXE * 4 * sizeof(double)) != hipSuccess ||
+      hipMalloc(&dn, n * MAXE * 4 * sizeof(double)) != hipSuccess ||
+      hipMalloc(&de, n * sizeof(int)) != hipSuccess ||
+      hipMemcpy(dh, h, n * 16 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    rc = VH_ERR_HIP;
+  if (rc == VH_OK) {
+    const Dx4 d{{dx[0], dx[1], dx[2], dx[3]}};
+    hipLaunchKernelGGL(k_cornelius_batch, dim3((unsigned)std::min<long>((n + 63) / 64, 4096)),
+                       dim3(64), 0, 0, dh, n, d, E0, dc, dn, de);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpy(cen, dc, n * MAXE * 4 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(nrm, dn, n * MAXE * 4 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(nel, de, n * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = VH_ERR_HIP;
+  }
+  if (dh) (void)hipFree(dh);
+  if (dc) (void)hipFree(dc);
+  if (dn) (void)hipFree(dn);
+  if (de) (void)hipFree(de);
+  return rc;
+}
+
+}  // namespace fo
+}  // namespace vh
