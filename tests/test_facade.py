@@ -45,7 +45,7 @@ def test_driver_gubser_matches_oracle(tmp_path, oracle):
     p = _params(tmp_path, eosType=5, icModel=4, gubserScale=10.0, nx=n, ny=n, nz=1,
                 xmin=-5.0, xmax=5.0, ymin=-5.0, ymax=5.0, etamin=-1.0, etamax=1.0,
                 tau0=1.0, tauMax=1.5, dtau=0.05, etaS=0.0, zetaS=0.0, tauGridResize=100.0,
-                dumpState=1)
+                dumpState=1, stopOnEmptySurface=0)
     r = subprocess.run([EXE, "-params", p, "-outputDir", str(tmp_path)], capture_output=True,
                        text=True, timeout=120)
     print(r.stdout[-800:])
@@ -71,7 +71,7 @@ def test_driver_viscous_gaussian_substeps(tmp_path, oracle):
     p = _params(tmp_path, eosType=5, icModel=10, icE0=20.0, icSigmaX=1.2, icSigmaY=1.0,
                 icEtaPlateau=0.3, icEtaSigma=0.4, nx=nx, ny=ny, nz=nz, xmin=xl[0], xmax=xl[1],
                 ymin=yl[0], ymax=yl[1], etamin=zl[0], etamax=zl[1], tau0=0.5, tauMax=0.7,
-                dtau=0.1, etaS=0.1, tauGridResize=100.0, dumpState=1)
+                dtau=0.1, etaS=0.1, tauGridResize=100.0, dumpState=1, stopOnEmptySurface=0)
     r = subprocess.run([EXE, "-params", p, "-outputDir", str(tmp_path)], capture_output=True,
                        text=True, timeout=120)
     print(r.stdout[-800:])
@@ -94,3 +94,37 @@ def test_driver_viscous_gaussian_substeps(tmp_path, oracle):
             break
     Qo = o.get_state()["Q"]
     assert np.max(np.abs(Qg - Qo)) <= 1e-10 * np.max(np.abs(Qo))
+
+
+@pytest.mark.gpu
+def test_driver_stops_like_the_reference_in_2d(tmp_path):
+    """With nz < 5 there are no interior hypercubes, so the reference's loop
+    (src/main.cpp:399-428) ends after the first step."""
+    p = _params(tmp_path, eosType=5, icModel=4, gubserScale=10.0, nx=21, ny=21, nz=1,
+                xmin=-5.0, xmax=5.0, ymin=-5.0, ymax=5.0, etamin=-1.0, etamax=1.0,
+                tau0=1.0, tauMax=3.0, dtau=0.05, etaS=0.0, tauGridResize=100.0)
+    r = subprocess.run([EXE, "-params", p, "-outputDir", str(tmp_path)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "steps = 1 " in r.stdout
+
+
+@pytest.mark.gpu
+def test_driver_freezeout_file(tmp_path):
+    """3-D viscous run with FP32 fluxes: freezeout.dat holds the reference's
+    27 columns, one row per element reported per step, until the fluid has
+    cooled below e_crit everywhere (the reference's loop exit)."""
+    nx, ny, nz = 20, 18, 9
+    p = _params(tmp_path, eosType=5, icModel=10, icE0=3.0, icSigmaX=0.8, icSigmaY=0.7,
+                icEtaPlateau=0.2, icEtaSigma=0.3, nx=nx, ny=ny, nz=nz, xmin=-2.85, xmax=2.85,
+                ymin=-2.55, ymax=2.55, etamin=-0.4, etamax=0.4, tau0=0.6, tauMax=30.0,
+                dtau=0.05, etaS=0.1, tauGridResize=100.0, e_crit=0.5, precision=1, c2pMode=2)
+    r = subprocess.run([EXE, "-params", p, "-outputDir", str(tmp_path)], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    rows = np.loadtxt(tmp_path / "freezeout.dat", ndmin=2)
+    counts = [int(l.split()[3]) for l in r.stdout.splitlines()
+              if len(l.split()) == 5 and l.split()[3].isdigit()]
+    print(f"{len(rows)} rows; last counts {counts[-5:]}", flush=True)
+    assert rows.shape[1] == 27 and len(rows) == sum(counts) and counts[-1] == 0
+    assert np.all(np.isfinite(rows)) and np.all(rows[:, 8] >= 1.0)

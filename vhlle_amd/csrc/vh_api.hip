@@ -934,28 +934,27 @@ int vh_freezeout(vh_ctx_t *c, double tau, double dt, double e_crit, int extend, 
   a.Pi = c->Pi;
   a.pi_float = c->precision == VH_PREC_MIXED;
   a.stream = c->st;
+  a.want_rows = rows != nullptr && max_rows > 0;
+  double sums[vh::fo::NSTAT];
+  a.stats = sums;
   std::vector<double> r;
   long n = 0;
   int err = 0;
   const int rc = vh::fo::run(a, r, n, err);
   if (rc != VH_OK) return rc;
   if (nrows) *nrows = n;
-  if (rows)
+  if (a.want_rows)
     for (long i = 0; i < n && i < max_rows; i++)
       std::memcpy(rows + i * rec, &r[i * a.P.ncol], rec * sizeof(double));
   if (stats) {
     std::memset(stats, 0, sizeof *stats);
-    stats->nelements = n;
+    stats->nelements = (long)sums[0];
+    stats->nsusp = (long)sums[1];
+    for (int k = 0; k < 4; k++) stats->dsigma[k] = sums[2 + k];
+    stats->v_eff = sums[6];
+    stats->e_surf = sums[7];
+    stats->nb_surf = sums[8];
     stats->err = err;
-    for (long i = 0; i < n; i++) {
-      const double *row = &r[i * a.P.ncol];
-      const double *aux = row + rec;
-      for (int k = 0; k < 4; k++) stats->dsigma[k] += row[4 + k];
-      stats->v_eff += aux[0];
-      stats->e_surf += aux[1];
-      stats->nb_surf += aux[2];
-      if (aux[3] > e_crit * 2.0 || aux[3] < e_crit * 0.5) stats->nsusp++;
-    }
   }
   return VH_OK;
 }

@@ -15,6 +15,9 @@ namespace fo {
 // Pi) = 27 columns, 29 with the extended output (e, n_b), then AUX columns:
 // dV_eff = dsigma_mu u^mu, the element's EtotSurf and nbSurf increments, e.
 constexpr int ROW_BASE = 27, ROW_EXT = 29, AUX = 4;
+// sums over the elements: count, suspicious, dsigma_mu (4), dV_eff,
+// EtotSurf and nbSurf increments
+constexpr int NSTAT = 9;
 
 struct FoParams {
   double tau, dt, dx, dy, dz, xmin, ymin, zmin, ecrit;
@@ -33,6 +36,8 @@ struct FoArgs {
   const void *pi, *Pi;        // pi (10 x N), Pi (N) of the upper level
   int pi_float;               // pi, Pi stored in FP32
   hipStream_t stream;
+  bool want_rows;             // copy the records to the host
+  double *stats;              // host, NSTAT sums
 };
 
 // elements in the reference's loop order, rows of P.ncol doubles

@@ -210,6 +210,37 @@ k_fo_cubes(CubeGeom g, FoParams P, DevEos eos, const long *__restrict__ cut,
   }
 }
 
+// sums over all elements in a fixed order (one block): element count, the
+// suspicious ones, dsigma_mu, dV_eff, the EtotSurf and nbSurf increments
+__global__ void __launch_bounds__(1024)
+k_fo_stats(const int *__restrict__ ncut_p, const int *__restrict__ count,
+           const double *__restrict__ rows, int ncol, double ecrit, double *__restrict__ out) {
+  __shared__ double red[1024];
+  const int ncut = *ncut_p;
+  double acc[NSTAT] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
+  for (long t = threadIdx.x; t < ncut; t += blockDim.x)
+    for (int el = 0; el < count[t]; el++) {
+      const double *row = rows + ((long)t * MAXE + el) * ncol;
+      const double *aux = row + ncol - AUX;
+      acc[0] += 1.;
+      acc[1] += (aux[3] > ecrit * 2.0 || aux[3] < ecrit * 0.5) ? 1. : 0.;
+      for (int k = 0; k < 4; k++) acc[2 + k] += row[4 + k];
+      acc[6] += aux[0];
+      acc[7] += aux[1];
+      acc[8] += aux[2];
+    }
+  for (int q = 0; q < NSTAT; q++) {
+    red[threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = red[0];
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 int run(const FoArgs &a, std::vector<double> &rows_out, long &nrows, int &errbits) {
@@ -217,6 +248,7 @@ int run(const FoArgs &a, std::vector<double> &rows_out, long &nrows, int &errbit
   nrows = 0;
   errbits = 0;
   rows_out.clear();
+  for (int q = 0; q < NSTAT; q++) a.stats[q] = 0.;
   if (g.mx < 1 || g.my < 1 || g.mz < 1) return VH_OK;  // no interior hypercubes
   const long ncube = (long)g.mx * g.my * g.mz, N = g.N;
   double *e1 = nullptr, *e0 = nullptr, *rows = nullptr;
@@ -272,12 +304,23 @@ int run(const FoArgs &a, std::vector<double> &rows_out, long &nrows, int &errbit
                            ncut, e1, e0, a.Q, a.Qlow, (const double *)a.pi,
                            (const double *)a.Pi, rows, cnt, err);
       ok(hipGetLastError());
+      double *dstat = nullptr;
+      if (ok(hipMalloc(&dstat, NSTAT * sizeof(double)))) {
+        hipLaunchKernelGGL(k_fo_stats, dim3(1), dim3(1024), 0, st, ncut, cnt, rows, a.P.ncol,
+                           a.P.ecrit, dstat);
+        ok(hipMemcpyAsync(a.stats, dstat, NSTAT * sizeof(double), hipMemcpyDeviceToHost, st));
+      }
       hcnt.resize(nc);
       ok(hipMemcpyAsync(hcnt.data(), cnt, nc * sizeof(int), hipMemcpyDeviceToHost, st));
       ok(hipMemcpyAsync(&errbits, err, sizeof(int), hipMemcpyDeviceToHost, st));
       ok(hipStreamSynchronize(st));
+      if (dstat) (void)hipFree(dstat);
     }
-    if (rc == VH_OK) {
+    if (rc == VH_OK && !a.want_rows) {
+      long total = 0;
+      for (int k = 0; k < nc; k++) total += hcnt[k];
+      nrows = total;
+    } else if (rc == VH_OK) {
       long total = 0;
       for (int k = 0; k < nc; k++) total += hcnt[k];
       rows_out.resize(total * a.P.ncol);

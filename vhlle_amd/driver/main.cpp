@@ -8,7 +8,13 @@
 // Gaussian state (icE0, icSigmaX, icSigmaY, icEtaPlateau, icEtaSigma);
 // gubserScale (energy scale of icModel 4); zetaSPeakT, zetaSWidthMinus,
 // zetaSWidthPlus (zetaSparam 5); dumpState 1 writes the final conserved
-// state to outputDir/state.bin (7 x N doubles, C-ABI layout).
+// state to outputDir/state.bin (7 x N doubles, C-ABI layout); precision
+// (0 FP64, 1 FP32 fluxes), c2pMode (0 reference, 1 learned, 2 closed form,
+// 3 Newton from v = 0.5), c2pModel (a file written by -trainC2P);
+// stopOnEmptySurface (1, the reference's loop condition: stop at the first
+// step without freeze-out elements, which for nz < 5 is the first step).
+// `vhlle -trainC2P file [-seed S]` harvests 4 M conserved -> primitive pairs
+// from Gubser runs, trains the con2prim MLP on the GPU and writes it.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -21,6 +27,7 @@
 #include <sstream>
 #include <string>
 #include <sys/stat.h>
+#include <vector>
 
 #include "../host/vhlle.h"
 
@@ -40,7 +47,8 @@ int icModel{1}, glauberVariable{1}, smoothingType{0};
 // added keys
 double gubserScale{1.0}, icE0{50.0}, icSigmaX{2.5}, icSigmaY{3.0}, icEtaPlateau{1.5},
     icEtaSigma{1.0}, zetaSPeakT{0.170}, zetaSWidthMinus{0.010}, zetaSWidthPlus{0.030};
-int dumpState{0};
+int dumpState{0}, precision{0}, c2pMode{0}, stopOnEmptySurface{1};
+string c2pModel;
 
 void readParameters(const char *parFile) {
   ifstream fin(parFile);
@@ -74,7 +82,9 @@ void readParameters(const char *parFile) {
       {"icSigmaY", D(icSigmaY)}, {"icEtaPlateau", D(icEtaPlateau)},
       {"icEtaSigma", D(icEtaSigma)}, {"zetaSPeakT", D(zetaSPeakT)},
       {"zetaSWidthMinus", D(zetaSWidthMinus)}, {"zetaSWidthPlus", D(zetaSWidthPlus)},
-      {"dumpState", I(dumpState)},
+      {"dumpState", I(dumpState)}, {"precision", I(precision)}, {"c2pMode", I(c2pMode)},
+      {"c2pModel", [](const string &s) { c2pModel = s; }},
+      {"stopOnEmptySurface", I(stopOnEmptySurface)},
   };
   while (fin.good()) {
     string line;
@@ -155,7 +165,40 @@ double totalEnergy(Fluid *f, double tau) {
 
 }  // namespace
 
+// harvest + GPU training of the con2prim MLP (vh_mlp_harvest, vh_mlp_train)
+int trainC2P(const char *out, unsigned long long seed) {
+  const long n = 4L << 20;
+  std::vector<double> Q(7 * n), prim(8 * n);
+  vh_harvest_cfg_t hc{seed, 200, 2, {10.0, 1.0, 0., 0.}, 0.5, 0.99};
+  if (vh_mlp_harvest(&hc, n, Q.data(), prim.data()) != VH_OK) {
+    cout << "harvest failed\n";
+    return 1;
+  }
+  std::vector<float> p(VH_MLP_NPARAM);
+  vh_mlp_init(seed + 1, p.data());
+  vh_mlp_train_cfg_t tc{512, 8, 2e-3, 1e-6, seed + 2};
+  std::vector<double> hist(tc.epochs);
+  if (vh_mlp_train(&tc, Q.data(), prim.data(), n, p.data(), hist.data()) != VH_OK) {
+    cout << "training failed\n";
+    return 1;
+  }
+  for (int e = 0; e < tc.epochs; e++) cout << "epoch " << e << " loss " << hist[e] << endl;
+  if (!vhlle_learn::saveC2PModel(out, p.data())) {
+    cout << "cannot write " << out << endl;
+    return 1;
+  }
+  cout << "wrote " << out << endl;
+  return 0;
+}
+
 int main(int argc, char **argv) {
+  for (int i = 1; i < argc - 1; i++)
+    if (strcmp(argv[i], "-trainC2P") == 0) {
+      unsigned long long seed = 1;
+      for (int j = 1; j < argc - 1; j++)
+        if (strcmp(argv[j], "-seed") == 0) seed = strtoull(argv[j + 1], nullptr, 10);
+      return trainC2P(argv[i + 1], seed);
+    }
   readCommandLine(argc, argv);
   if (eosType != 5) {
     cout << "eosType " << eosType
@@ -173,6 +216,18 @@ int main(int argc, char **argv) {
   trcoeff->zetaSWidthPlus = zetaSWidthPlus;
   Fluid *f = new Fluid(eos, eos, trcoeff, nx, ny, nz, xmin, xmax, ymin, ymax, etamin, etamax,
                        dtau, eCrit);
+  if (precision != 0 || c2pMode != 0) {
+    std::vector<float> model;
+    if (c2pMode == VH_C2P_LEARNED) {
+      model.resize(VH_MLP_NPARAM);
+      if (c2pModel.empty() || !vhlle_learn::loadC2PModel(c2pModel.c_str(), model.data())) {
+        cout << "c2pMode 1 needs c2pModel (a file written by vhlle -trainC2P)\n";
+        return 1;
+      }
+    }
+    f->configure(precision == 1 ? VH_PREC_MIXED : VH_PREC_FP64, c2pMode,
+                 model.empty() ? nullptr : model.data());
+  }
   cout << "fluid allocation done\n";
   if (icModel == 4) {
     ICGubser ic(gubserScale);
@@ -188,7 +243,9 @@ int main(int argc, char **argv) {
     cout << "note: tauGridResize (expandGrid2x) is not part of this build; ignored\n";
   Hydro *h = new Hydro(f, eos, trcoeff, tau0, dtau);
   mkdir(outputDir.c_str(), 0755);
+  f->initOutput(outputDir.c_str());
   long steps = 0;
+  int nelements = 0;
   const auto t0 = chrono::steady_clock::now();
   do {
     // sub-steps while dtau exceeds tau * d(eta) (src/main.cpp:405-419)
@@ -203,7 +260,9 @@ int main(int argc, char **argv) {
       h->performStep();
     }
     steps += nSub;
-  } while (h->getTau() < tauMax + 0.0001);
+    nelements = f->outputSurface(h->getTau(), freezeoutExtend);
+  } while (h->getTau() < tauMax + 0.0001 && (nelements > 0 || !stopOnEmptySurface));
+  f->closeOutput();
   const double secs = chrono::duration<double>(chrono::steady_clock::now() - t0).count();
   cout << "tau = " << h->getTau() << "  steps = " << steps
        << "  sum Q^tau = " << totalEnergy(f, h->getTau()) << endl;

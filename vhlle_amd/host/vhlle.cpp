@@ -423,6 +423,7 @@ Fluid::Fluid(EoS *_eos, EoS *_eosH, TransportCoeff *_trcoeff, int _nx, int _ny, 
 }
 
 Fluid::~Fluid() {
+  closeOutput();
   if (ctx) vh_destroy(ctx);
 }
 
@@ -456,6 +457,57 @@ void Fluid::hostRead() {
 void Fluid::hostWrite() {
   if (dev_newer) syncFromDevice();
   host_dirty = true;
+}
+
+void Fluid::initOutput(const char *dir) {
+  closeOutput();
+  const std::string fn = std::string(dir) + "/freezeout.dat";
+  ffreeze = std::fopen(fn.c_str(), "w");
+  if (!ffreeze) std::cout << "cannot open " << fn << std::endl;
+}
+
+void Fluid::closeOutput() {
+  if (ffreeze) std::fclose(ffreeze);
+  ffreeze = nullptr;
+}
+
+int Fluid::outputSurface(double tau, bool extendFO) {
+  syncToDevice();
+  long n = 0;
+  vh_fo_stats_t st;
+  const int ncol = extendFO ? 29 : 27;
+  vh_or_die(vh_freezeout(ctx, tau, dt, ecrit, extendFO, VH_FO_QPREV, nullptr, 0, &n, &st),
+            "vh_freezeout");
+  if (ffreeze && n > 0) {
+    std::vector<double> rows((size_t)n * ncol);
+    vh_or_die(vh_freezeout(ctx, tau, dt, ecrit, extendFO, VH_FO_QPREV, rows.data(), n, &n, &st),
+              "vh_freezeout");
+    for (long i = 0; i < n; i++) {
+      for (int k = 0; k < ncol; k++) std::fprintf(ffreeze, "%24.15g", rows[i * ncol + k]);
+      std::fprintf(ffreeze, "\n");
+    }
+    std::fflush(ffreeze);
+  }
+  if (st.err) std::cout << "#### Error (surface): hypercube construction failed ####\n";
+  EtotSurf += st.e_surf;
+  nbSurf += st.nb_surf;
+  vEff += st.v_eff;
+  std::cout << std::setw(10) << tau << std::setw(13) << nbSurf << std::setw(13) << EtotSurf
+            << std::setw(10) << n << std::setw(10) << st.nsusp << std::endl;
+  return (int)n;
+}
+
+void Fluid::configure(int precision, int c2pMode, const float *model) {
+  vh_grid_t g{nx, ny, nz, minx, maxx, miny, maxy, minz, maxz};
+  vh_eos_t e = eos->device();
+  vh_trcoeff_t t = trcoeff->device();
+  if (ctx) vh_destroy(ctx);
+  ctx = nullptr;
+  vh_or_die(vh_create(&g, &e, &t, precision, &ctx), "vh_create");
+  if (model) vh_or_die(vh_set_c2p_model(ctx, model), "vh_set_c2p_model");
+  if (c2pMode != VH_C2P_REFERENCE) vh_or_die(vh_set_c2p_mode(ctx, c2pMode), "vh_set_c2p_mode");
+  host_dirty = true;  // the host mirror is the state to upload
+  dev_newer = false;
 }
 
 void Fluid::syncToDevice() {

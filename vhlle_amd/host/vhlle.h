@@ -15,6 +15,7 @@
 #pragma once
 
 #include <cmath>
+#include <cstdio>
 #include <memory>
 #include <string>
 #include <vector>
@@ -190,6 +191,20 @@ class Fluid {
   void correctImagCellsFull(void);
   void updateM(double tau, double dt);
 
+  // Fluid::outputSurface (src/fld.cpp:517-874): the freeze-out elements at
+  // e = eCrit between the previous step (the Fluid's dt) and tau, appended to
+  // the file opened by initOutput in the reference's freezeout.dat format;
+  // returns the element count and prints the reference's summary line
+  // (tau, EtotSurf, nbSurf, elements, suspicious elements).
+  int outputSurface(double tau, bool extendFO = false);
+  // opens dir/freezeout.dat for outputSurface
+  void initOutput(const char *dir);
+  void closeOutput();
+  // ---- precision / con2prim of the device step (new): recreates the device
+  // context before the IC is set (VH_PREC_*, VH_C2P_*; model: the con2prim
+  // MLP for VH_C2P_LEARNED, VH_MLP_NPARAM floats)
+  void configure(int precision, int c2pMode, const float *model = nullptr);
+
   // ---- GPU residency (new): the state lives in HBM between steps
   vh_ctx_t *context() { return ctx; }
   void syncToDevice();    // uploads the host mirror if host code changed it
@@ -210,6 +225,8 @@ class Fluid {
   std::vector<double> hQ, hQh, hQprev, hpi, hpiH, hPi, hPiH, hm, hvcc;
   std::vector<Cell> cells;
   bool host_dirty = false, dev_newer = false;
+  std::FILE *ffreeze = nullptr;
+  double EtotSurf = 0., nbSurf = 0., vEff = 0.;
   void hostRead();
   void hostWrite();
 };
@@ -248,3 +265,9 @@ class ICGubser {
  private:
   double scale_;
 };
+
+// learned con2prim model files (magic + VH_MLP_NPARAM floats)
+namespace vhlle_learn {
+bool saveC2PModel(const char *path, const float *params);
+bool loadC2PModel(const char *path, float *params);
+}  // namespace vhlle_learn
