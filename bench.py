@@ -34,11 +34,13 @@ events recorded on the context's stream around every launch during a second,
 identical pass.
 
 python bench.py [--gpus 1] [--steps 300] [--warmup 5] [--variants a,b] [--no-cpu]
-                [--no-batch]
+                [--no-batch] [--no-full3d] [--quick]
 
 Also measured (field "batch_ebe"): 256 event-by-event boost-invariant grids of
 sides 100..160 with hot-spot initial states in one batch context, against the
-same events run one context at a time.
+same events run one context at a time; (field "full_3d_freezeout") the
+300x300x150 hot-spot run, 600 steps with the freeze-out surface every 5
+steps, FP64 against FP32 fluxes (element count and sum of dsigma_mu).
 """
 import argparse
 import json
@@ -314,6 +316,53 @@ def batch_bench(args, c2p):
                                            batch_speedup=round(v / (seq_cells * steps / seq_t), 2)))
 
 
+def full3d_bench(c2p_mixed="closed"):
+    """300 x 300 x 150 hot-spot run, 600 steps, freeze-out every 5 steps
+    (snapshot -> 5 steps as one graph replay -> surface), FP64 and FP32
+    fluxes: throughput and the surface (elements, sum of dsigma_mu)."""
+    from vhlle_amd import abi, setups
+    cfg = setups.FULL_3D_FO
+    ic, npart = setups.full3d_ic(cfg)
+    N = cfg["nx"] * cfg["ny"] * cfg["nz"]
+    res = {}
+    for name, prec, c2p in (("fp64", abi.PREC_FP64, "reference"),
+                            ("mixed_" + c2p_mixed, abi.PREC_MIXED, c2p_mixed)):
+        ctx = abi.Context(cfg["nx"], cfg["ny"], cfg["nz"], cfg["xlim"], cfg["ylim"], cfg["zlim"],
+                          trc=abi.make_trc(etaS=cfg["etaS"], zetaS=cfg["zetaS"], zetaSparam=5),
+                          precision=prec)
+        set_c2p(ctx, c2p)
+        ctx.set_prim(cfg["tau0"], *ic)
+        tau, t_step, t_fo, nel, ds, st_all, nsusp = cfg["tau0"], 0.0, 0.0, 0, np.zeros(4), 0, 0
+        for _ in range(cfg["steps"] // cfg["fo_every"]):
+            ctx.fo_snapshot(tau)
+            t0 = time.perf_counter()
+            st, tau = ctx.run(tau, cfg["dt"], cfg["fo_every"])
+            t1 = time.perf_counter()
+            _, fs = ctx.freezeout(tau, 0.0, cfg["e_sw"], lower=abi.FO_SNAPSHOT, max_rows=0)
+            t2 = time.perf_counter()
+            t_step += t1 - t0
+            t_fo += t2 - t1
+            st_all |= st
+            nel += fs["nelements"]
+            nsusp += fs["nsusp"]
+            ds += np.array(fs["dsigma"])
+        ctx.close()
+        v = N * cfg["steps"] / (t_step + t_fo)
+        res[name] = dict(value=round(v, 1), unit="cell-updates/s", status=st_all,
+                         ms_per_step=round(t_step / cfg["steps"] * 1e3, 4),
+                         fo_ms_per_call=round(t_fo / (cfg["steps"] // cfg["fo_every"]) * 1e3, 3),
+                         elements=int(nel), suspicious=int(nsusp),
+                         dsigma_sum=[float(x) for x in ds])
+        print(f"[bench] full 3-D {name}: {cfg['steps']} steps {t_step:.2f} s, freeze-out "
+              f"{t_fo:.2f} s, {nel} elements, status {st_all}", file=sys.stderr, flush=True)
+    a, b = res["fp64"], res["mixed_" + c2p_mixed]
+    da, db = np.array(a["dsigma_sum"]), np.array(b["dsigma_sum"])
+    return dict(workload=cfg["name"], cells=N, steps=cfg["steps"], fo_every=cfg["fo_every"],
+                e_sw=cfg["e_sw"], npart=npart, variants=res,
+                elements_rel_diff=abs(b["elements"] - a["elements"]) / max(a["elements"], 1),
+                dsigma_rel_diff=[float(x) for x in np.abs(db - da) / np.abs(da).max()])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -322,6 +371,7 @@ def main():
     ap.add_argument("--variants", default="fp64,mixed_ref,mixed_closed,mixed_newton,mixed_learned")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-full3d", action="store_true")
     ap.add_argument("--quick", action="store_true",
                     help="skip the error runs (profiling); errors reported as null")
     args = ap.parse_args()
@@ -421,6 +471,8 @@ def main():
         out["learned_c2p"] = _MODEL["info"]
     if not args.no_batch:
         out["batch_ebe"] = [batch_bench(args, c) for c in ("closed", "learned")]
+    if not args.no_full3d:
+        out["full_3d_freezeout"] = full3d_bench("closed")
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(cfg)
     print(json.dumps(out), flush=True)
 

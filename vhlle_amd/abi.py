@@ -401,7 +401,7 @@ class Context:
         check(lib().vh_freezeout(self.h, tau, dt, ecrit, int(extend), lower, None, 0,
                                  ctypes.byref(n), ctypes.byref(st)), "vh_freezeout")
         rows = np.zeros((max(min(n.value, max_rows), 1), ncol))
-        if n.value:
+        if n.value and max_rows > 0:
             check(lib().vh_freezeout(self.h, tau, dt, ecrit, int(extend), lower, _p(rows),
                                      min(n.value, max_rows), ctypes.byref(n), ctypes.byref(st)),
                   "vh_freezeout")

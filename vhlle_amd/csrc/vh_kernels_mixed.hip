@@ -73,7 +73,9 @@ __device__ __forceinline__ void c2p_closed(double q[7], Prim &r) {
         Qz *= s;
         M = Qt * 0.9999;
       }
-      double v = 3. * M / (2. * Qt + sqrt(4. * Qt * Qt - 3. * M * M));
+      // in x = |Q|/Q_t: Q_t^2 underflows FP64 for Q_t below 1e-154
+      const double x = M / Qt;
+      double v = 3. * x / (2. + sqrt(4. - 3. * x * x));
       const double pp = (Qt - M * v) * (1. / 3.);
       v -= ((Qt + pp) * v - M) / ((Qt + pp) - M * v * (1. / 3.));
       e = Qt - M * v;
@@ -107,7 +109,9 @@ __device__ __forceinline__ void c2p_closed(double q[7], Prim &r) {
 __device__ __forceinline__ PrimF c2p_fast(const double q[7]) {
   PrimF r{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 1.f};
   const double Qt = q[T_];
-  if (!(Qt > 0.)) return r;
+  // Q^tau below 1e-30 is vacuum at FP32 precision (and 1/(float)Q^tau would
+  // overflow to inf for Q^tau below the FP32 range, giving 0 * inf velocities)
+  if (!(Qt > 1e-30)) return r;
   const double M2 = q[X_] * q[X_] + q[Y_] * q[Y_] + q[Z_] * q[Z_];
   if (M2 == 0.) {
     r.e = (float)Qt;
@@ -234,6 +238,9 @@ __device__ __forceinline__ bool hlle_f32(PrimF L, PrimF R, float itau, float itf
   }
   const float cs = 0.57735026918962576f;
   const float sEl = fsqrt(wl), sEr = fsqrt(wr);
+  // both sides below the FP32 range (e ~ 1e-40 in the FP64 state, flushed
+  // here): vacuum to FP32 precision, no flux (instead of 0 * inf)
+  if (!(sEl + sEr > 0.f)) return false;
   const float isum = frcp(sEl + sEr);
   const float dv = vl - vr;
   const float csb = fsqrt(1.f / 3.f + 0.5f * sEl * sEr * isum * isum * dv * dv);

@@ -90,3 +90,25 @@ def batch_grids(cfg=BATCH_EBE, events=None, seed=None):
 def batch_trc(cfg=BATCH_EBE):
     from vhlle_amd import abi
     return abi.make_trc(etaS=cfg["etaS"], zetaS=cfg["zetaS"], zetaSparam=5)
+
+
+# Full 3-D viscous run with freeze-out: 300 x 300 x 150 cells, dx = dy = 0.08 fm,
+# d(eta) = 0.04 (x, y in [-11.96, 11.96], eta in [-2.98, 2.98]), hot-spot
+# initial state as the batch (same generator and amplitude) extended in eta
+# with a plateau |eta| < 2 and Gaussian tails of width 1.0, eta/s = 0.08 and
+# the T-peaked zeta/s, tau0 = 0.5 fm, dtau = 0.02 fm (= tau0 d(eta): no
+# sub-steps), 600 steps; Cornelius surface at e_sw = 0.5 every 5 steps.
+FULL_3D_FO = dict(name="full_3d_fo_300x300x150", nx=300, ny=300, nz=150, xlim=(-11.96, 11.96),
+                  ylim=(-11.96, 11.96), zlim=(-2.98, 2.98), tau0=0.5, dt=0.02, steps=600,
+                  fo_every=5, e_sw=0.5, etaS=0.08, zetaS=0.05, eta_plateau=2.0, eta_sigma=1.0,
+                  npart_mean=150.0, R=6.6, a=0.55, sigma=0.5, amp=5.2, seed=20240502)
+
+
+def full3d_ic(cfg=FULL_3D_FO):
+    from vhlle_amd import abi
+    g = [(cfg["nx"], cfg["ny"], cfg["nz"], cfg["xlim"], cfg["ylim"], cfg["zlim"])]
+    e, npart = abi.hotspot_ic(g, seed=cfg["seed"], npart_mean=cfg["npart_mean"], R=cfg["R"],
+                              a=cfg["a"], sigma=cfg["sigma"], amp=cfg["amp"],
+                              eta_plateau=cfg["eta_plateau"], eta_sigma=cfg["eta_sigma"])
+    z = np.zeros_like(e)
+    return (e, z, z.copy(), z.copy()), int(npart[0])
