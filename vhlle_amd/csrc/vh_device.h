@@ -48,6 +48,9 @@ struct DevEos {
     return 0.;
   }
   __device__ double cs2e(double) const { return 1. / 3.; }
+  // FP32 forms for the interface states of the FP32 flux
+  __device__ float pf(float e) const { return e * (1.f / 3.f); }
+  __device__ float cs2f(float) const { return 1.f / 3.f; }
 };
 
 // ----------------------------------------------------- con2prim (FP64)

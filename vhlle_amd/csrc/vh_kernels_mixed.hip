@@ -149,6 +149,75 @@ __device__ __forceinline__ PrimF c2p_fast(const double q[7]) {
   return r;
 }
 
+// FP32 Newton con2prim from a start v0 (the learned guess, or 0.5), for the
+// FP32 consumers (interface states of the FP32 flux, the Milne source of the
+// FP32-accurate stage).  Unknown w = 1 - v, so that |v| -> 1 keeps full
+// relative precision; with x = |Q|/Q_t, y = 1 - x (from FP64 Q_t^2 - |Q|^2 as
+// in c2p_fast) and e/Q_t = 1 - x v = y + x w:
+//   f(w) = (1 + p/Q_t)(1 - w) - x = y - w + (p/Q_t)(1 - w),
+//   f'(w) = -(1 + p/Q_t) + c_s^2 x (1 - w) < 0.
+// f(y) = x p/Q_t >= 0 and f(1) = -x <= 0 bracket the root in [y, 1].  For
+// p = e/3, f is concave, so Newton from any start in the bracket reaches the
+// right of the root in one update and then decreases monotonically to it.
+// With 2/3 <= |f'| <= 4/3 and |f''| <= 2/3 on the bracket, the error after an
+// update dw is at most 2 dw^2, so an update below 1.5e-4 w leaves w accurate
+// to 4.5e-8 w: the loop stops there.  Same vacuum, rest and
+// |Q| >= Q_t branches as c2p_fast.
+__device__ __forceinline__ PrimF c2p_newton_f32(const DevEos &eos, const double q[7], float v0) {
+  PrimF r{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 1.f};
+  const double Qt = q[T_];
+  if (!(Qt > 1e-30)) return r;
+  const double M2 = q[X_] * q[X_] + q[Y_] * q[Y_] + q[Z_] * q[Z_];
+  if (M2 == 0.) {
+    r.e = (float)Qt;
+    r.p = eos.pf(r.e);
+    r.nb = (float)q[NB_];
+    r.nq = (float)q[NQ_];
+    r.ns = (float)q[NS_];
+    return r;
+  }
+  const double rq = (double)frcp((float)Qt);
+  const double rq2 = rq * rq;
+  const float a = (float)((Qt * Qt - M2) * rq2);
+  const float t = (float)(Qt * rq);
+  const float mm = fsqrt((float)(M2 * rq2));
+  float y = a * frcp(t * (t + mm)), x = mm * frcp(t);
+  if (!(a > 0.f)) {
+    y = 1e-4f;
+    x = 0.9999f;
+  }
+  const float Qtf = (float)Qt, iQt = frcp(Qtf);
+  float w = 1.f - v0;
+  if (!(w >= y)) w = y;  // also a NaN guess
+  if (!(w <= 1.f)) w = 1.f;
+#pragma unroll 1
+  for (int it = 0; it < 16; it++) {
+    const float eh = y + x * w;
+    const float ph = eos.pf(eh * Qtf) * iQt;
+    const float f = y - w + ph * (1.f - w);
+    const float df = -(1.f + ph) + eos.cs2f(eh * Qtf) * x * (1.f - w);
+    float wn = w - f * frcp(df);
+    wn = fminf(fmaxf(wn, y), 1.f);
+    const float dw = fabsf(wn - w);
+    w = wn;
+    if (dw <= 1.5e-4f * w) break;
+  }
+  const float v = 1.f - w;
+  const float ig2 = w * (2.f - w);  // 1 - v^2
+  r.e = Qtf * (y + x * w);
+  r.p = eos.pf(r.e);
+  r.g2 = frcp(ig2);
+  const float vm = v * frcp(mm);
+  r.vx = (float)(q[X_] * rq) * vm;
+  r.vy = (float)(q[Y_] * rq) * vm;
+  r.vz = (float)(q[Z_] * rq) * vm;
+  const float ig = fsqrt(ig2);
+  r.nb = (float)q[NB_] * ig;
+  r.nq = (float)q[NQ_] * ig;
+  r.ns = (float)q[NS_] * ig;
+  return r;
+}
+
 // con2prim by mode; v0: the learned guess (C2P_LEARNED only)
 template <int CM>
 __device__ __forceinline__ void c2p_mode(const DevEos &eos, double q[7], Prim &r, float v0) {
@@ -180,6 +249,7 @@ __device__ __forceinline__ PrimF face_state(const DevEos &eos, const double *__r
     q[k] = (SIDE < 0 ? a - dq : a + dq) * itau;
   }
   if (CM == VH_C2P_CLOSED) return c2p_fast(q);
+  if (CM == VH_C2P_LEARNED || CM == VH_C2P_NEWTON) return c2p_newton_f32(eos, q, guess_of<CM>(gv, c0));
   Prim r;
   c2p_mode<CM>(eos, q, r, guess_of<CM>(gv, c0));
   return to_f(r);
@@ -324,6 +394,10 @@ __device__ __forceinline__ void finish_cell(const DevEos &eos, const double *__r
   double vz, pr;
   if (CM == VH_C2P_CLOSED) {
     const PrimF r = c2p_fast(q);
+    vz = r.vz;
+    pr = r.p;
+  } else if (CM == VH_C2P_LEARNED || CM == VH_C2P_NEWTON) {
+    const PrimF r = c2p_newton_f32(eos, q, guess_of<CM>(gv, c));
     vz = r.vz;
     pr = r.p;
   } else {
