@@ -51,9 +51,13 @@ def host_sources():
     return sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".cpp"))
 
 
-def build_lib(verbose=False):
-    """Compile every .hip kernel/API source and the host facade into one .so."""
-    objdir = os.path.join(HERE, "_build")
+def build_lib(verbose=False, defines=(), out=None, objdir=None):
+    """Compile every .hip kernel/API source and the host facade into one .so.
+    defines: extra -D settings of the kernel sources (e.g. "VH_W_FACES=2");
+    out, objdir: another library path and object folder (default: the
+    package's own)."""
+    lib = out or LIB
+    objdir = objdir or os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     objs = []
     for src in hip_sources() + host_sources():
@@ -64,18 +68,18 @@ def build_lib(verbose=False):
                 # FMA within an expression only, so that the same inlined code
                 # rounds the same way in every kernel that uses it
                 flags[flags.index("-ffp-contract=off")] = "-ffp-contract=on"
-            cmd = [HIPCC] + flags + ["-I", ROOT, "-c", src, "-o", obj]
+            cmd = [HIPCC] + flags + ["-D" + d for d in defines] + ["-I", ROOT, "-c", src, "-o", obj]
         else:  # host-only C++ (the facade): no HIP headers needed
             cmd = ["g++", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
                    "-I", ROOT, "-c", src, "-o", obj]
-        out = _run(cmd)
-        if verbose and out.strip():
-            print(out)
+        log = _run(cmd)
+        if verbose and log.strip():
+            print(log)
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 def build_driver():

@@ -390,6 +390,7 @@ int build_grid_tables(vh_ctx *c) {
   G.nblk = (int)bev.size();
   G.nseg = (int)sev.size();
   G.maxnz = maxnz;
+  G.g0 = c->evh[0];
   return VH_OK;
 }
 

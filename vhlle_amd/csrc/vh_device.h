@@ -48,9 +48,14 @@ struct DevEos {
     return 0.;
   }
   __device__ double cs2e(double) const { return 1. / 3.; }
-  // FP32 forms for the interface states of the FP32 flux
+  // FP32 forms (mixed precision: interface states of the FP32 flux, the
+  // thermodynamics of the FP32 relaxation step)
   __device__ float pf(float e) const { return e * (1.f / 3.f); }
   __device__ float cs2f(float) const { return 1.f / 3.f; }
+  __device__ float Tf(float e) const {
+    return e > 0.f ? __builtin_amdgcn_sqrtf(__builtin_amdgcn_sqrtf(e * (float)(1. / cnst))) : 0.f;
+  }
+  __device__ float sf(float e, float T) const { return T > 0.f ? (e + pf(e)) / T : 0.f; }
 };
 
 // ----------------------------------------------------- con2prim (FP64)
@@ -219,6 +224,87 @@ struct DevTrc {
         return 0.;
     }
   }
+  // FP32 forms of zetaS, etaS and getTau for the mixed-precision relaxation
+  // step (the same parametrisations, mu = 0)
+  __device__ float zetaSf(const DevEos &eos, float e, float T, float s, float P) const {
+    switch (c.zetaSparam) {
+      case 0:
+        return (float)c.zetaS0 * (1.f / 3.f - eos.cs2f(e)) / (expf((0.16f - T) * 1000.f) + 1.f);
+      case 1: {
+        const float Tp = 0.180f, r = T * (1.f / Tp);
+        if (T < 0.180f)
+          return 0.03f + 0.08f * expf((r - 1.f) * 400.f) + 0.22f * expf((r - 1.f) * (1.f / 0.022f));
+        else if (T < 0.200f)
+          return 27.55f * r - 13.45f - 13.77f * r * r;
+        return 0.001f + 0.9f * expf(-(r - 1.f) * 400.f) + 0.25f * expf(-(r - 1.f) * (1.f / 0.13f));
+      }
+      case 2: {
+        const float Tpk = 0.165f, Tw = 0.010f, Bn = 0.24f, Bw = 1.5f;
+        if (T > Tpk) {
+          const float d = T * (1.f / Tpk) - 1.f;
+          return Bn * (Bw * Bw) / (d * d + Bw * Bw);
+        }
+        const float d = (T - Tpk) * (1.f / Tw);
+        return Bn * expf(-d * d);
+      }
+      case 3: {
+        const float Tpk2 = 0.160f, Bn2 = 0.13f, B1 = 0.01f, B2 = 0.12f;
+        const float d = T - Tpk2;
+        if (T < Tpk2) return Bn2 * expf(-d * d * (1.f / (B1 * B1)));
+        return Bn2 * expf(-d * d * (1.f / (B2 * B2)));
+      }
+      case 4: {
+        if (fabsf(s) < 1e-10f || T < 1e-1f) return 0.f;
+        const float sig = e < (float)c.zetaSPeakEpsilon ? (float)c.zetaSSigmaMinus
+                                                        : (float)c.zetaSSigmaPlus;
+        const float q = __builtin_amdgcn_sqrtf(__builtin_amdgcn_sqrtf(e));
+        const float q0 = (float)pow(c.zetaSPeakEpsilon, 0.25);
+        const float d = (float)c.zetaSScaleBeta * (q - q0);
+        return ((e + P) / (s * T)) * (float)c.zetaS0 * expf(d * d * 0.5f * (sig * sig));
+      }
+      case 5: {
+        if (T <= 0.f) return 0.f;
+        const float w = T < (float)c.zetaSPeakT ? (float)c.zetaSWidthMinus : (float)c.zetaSWidthPlus;
+        const float d = (T - (float)c.zetaSPeakT) / w;
+        return (float)c.zetaS0 * expf(-0.5f * d * d);
+      }
+      default:
+        return 0.f;
+    }
+  }
+  __device__ float etaSf(float e, float rho, float T, float s, float P) const {
+    switch (c.etaSparam) {
+      case 0:
+        return (float)c.etaS0;
+      case 1:
+        return (float)c.etaSMin + ((T > (float)c.T0) ? (float)c.ah * (T - (float)c.T0)
+                                                     : (float)c.al * (T - (float)c.T0));
+      case 2: {
+        const float em = (float)c.etaSEpsilonMin;
+        const float v = (float)c.etaSMin + ((e > em) ? ((float)c.ah * (e - em) + (float)c.aRho * rho)
+                                                     : (float)c.al * (e - em) + (float)c.aRho * rho);
+        return v > 0.f ? v : 0.f;
+      }
+      case 3: {
+        if (fabsf(s) < 1e-10f || T < 1e-1f) return 0.f;
+        const float Tc = (float)c.T0;  // mu_B = 0
+        const float v = (float)c.etaSMin + ((T > Tc) ? (float)c.ah * (T - Tc) : (float)c.al * (T - Tc));
+        return ((e + P) / (s * T)) * (v > 0.f ? v : 0.f);
+      }
+      default:
+        return 0.f;
+    }
+  }
+  __device__ void getTauf(const DevEos &eos, float e, float rho, float T, float s, float P,
+                          float etaS, float zetaS, float &taupi, float &tauPi) const {
+    if (T > 1e-1f && fabsf(e + P) > 1.e-10f) {
+      const float r = s / (e + P);
+      taupi = (float)(5. / 5.068) * etaS * r;
+      tauPi = (float)(6. / 5.068) * zetaS * r;
+    } else {
+      taupi = tauPi = 0.f;
+    }
+  }
   // getTau (src/trancoeff.cpp:92-100); abs() of the reference resolves to the
   // double overload in its GSL-enabled build.
   __device__ void getTau(const DevEos &eos, double e, double rho, double T, double muB,
@@ -243,6 +329,60 @@ __device__ __forceinline__ long xcd_block() {
   const unsigned q = nb >> 3, r = nb & 7u, x = b & 7u, i = b >> 3;
   return (long)x * q + (x < r ? x : r) + i;
 }
+
+// Occupancy target of a kernel: VH_OCC(n) caps it at n waves per SIMD (0: the
+// compiler's default).  The stencil gathers are latency bound; a lower target
+// lets the scheduler keep more loads in flight per wave, which pays more than
+// the extra waves.  Values per kernel measured on the 200x200x100 step; each
+// can be overridden at build time (-DVH_W_<KERNEL>=n).
+#define VH_OCC_0
+#define VH_OCC_1 __attribute__((amdgpu_waves_per_eu(1, 1)))
+#define VH_OCC_2 __attribute__((amdgpu_waves_per_eu(1, 2)))
+#define VH_OCC_3 __attribute__((amdgpu_waves_per_eu(1, 3)))
+#define VH_OCC_4 __attribute__((amdgpu_waves_per_eu(1, 4)))
+#define VH_OCC_5 __attribute__((amdgpu_waves_per_eu(1, 5)))
+#define VH_OCC_6 __attribute__((amdgpu_waves_per_eu(1, 6)))
+#define VH_OCC_EXP(n) VH_OCC_##n
+#define VH_OCC(n) VH_OCC_EXP(n)
+#ifndef VH_W_FACES
+#define VH_W_FACES 0
+#endif
+#ifndef VH_W_UPDATE
+#define VH_W_UPDATE 0
+#endif
+#ifndef VH_W_VPRIM
+#define VH_W_VPRIM 0
+#endif
+#ifndef VH_W_VISC1
+#define VH_W_VISC1 0
+#endif
+#ifndef VH_W_VISC2
+#define VH_W_VISC2 0
+#endif
+#ifndef VH_W_VISC3
+#define VH_W_VISC3 0
+#endif
+#ifndef VH_W_MX_STAGE
+#define VH_W_MX_STAGE 0
+#endif
+#ifndef VH_W_MX_ROWS
+#define VH_W_MX_ROWS 0
+#endif
+#ifndef VH_W_MX_VPRIM
+#define VH_W_MX_VPRIM 0
+#endif
+#ifndef VH_W_MX_VISC1
+#define VH_W_MX_VISC1 0
+#endif
+#ifndef VH_W_MX_VISC2
+#define VH_W_MX_VISC2 2
+#endif
+#ifndef VH_W_MX_VISC3
+#define VH_W_MX_VISC3 0
+#endif
+#ifndef VH_W_GUESS
+#define VH_W_GUESS 0
+#endif
 
 // Grid geometry.  N: cells of this grid; base: index of its cell 0 in the
 // state arrays; P: the arrays' component pitch (cells of all grids stored

@@ -58,7 +58,7 @@ __device__ __forceinline__ void face_state(const DevEos &eos, const double *__re
 // its +1 neighbour along axis D.  Writes F (7 x N, slot c) and flag[c]
 // (1 = flux present, 0 = the reference returned early without a flux).
 template <int D, int MODE>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) VH_OCC(VH_W_FACES)
 k_faces(Geom g, DevEos eos, const double *__restrict__ Qs, const double *__restrict__ m,
         int m_full, const double *__restrict__ tp, double dt, double *__restrict__ F,
         unsigned char *__restrict__ flag, int *__restrict__ status) {
@@ -187,7 +187,7 @@ k_faces(Geom g, DevEos eos, const double *__restrict__ Qs, const double *__restr
 // src/cll.cpp:86-88); corrector Q += flux unless Q0 + flux0 < 0
 // (Cell::updateByFlux, src/cll.cpp:70-74).
 template <int MODE>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) VH_OCC(VH_W_UPDATE)
 k_update(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
          const double *__restrict__ Fx, const unsigned char *__restrict__ fx,
          const double *__restrict__ Fy, const unsigned char *__restrict__ fy,
@@ -439,7 +439,7 @@ k_update_m_apply(Geom g, double *__restrict__ m, const double *__restrict__ dm) 
 // visc_flux and visc_source_step) -> ph[8][N]; con2prim of Qh/tau (NSquant's
 // neighbour states, src/hdo.cpp:472-568) -> pt[4][N] = {e, vx, vy, vz}.
 // tau here is tau_{n+1}.
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) VH_OCC(VH_W_VPRIM)
 k_vprim(Geom g, DevEos eos, const double *__restrict__ Qh, const double *__restrict__ tp,
         double dt, double *__restrict__ ph, double *__restrict__ pt) {
   const double tau = *tp;
@@ -473,7 +473,7 @@ k_vprim(Geom g, DevEos eos, const double *__restrict__ Qh, const double *__restr
 // Navier-Stokes values, relaxation and second-order/cross terms; writes
 // pi0, piH0, Pi0, PiH0.  (The loop's flux contribution (tau-dt)*dQ_n is
 // formed in k_visc2, which reads the same old pi/Pi before replacing them.)
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) VH_OCC(VH_W_VISC1)
 k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
         const double *__restrict__ Qn, const double *__restrict__ ph,
         const double *__restrict__ pt, const double *__restrict__ m, int m_full,
@@ -751,7 +751,7 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
 // (src/hdo.cpp:717-721): upwind advection of pi0/piH0/Pi0/PiH0 from the 8
 // cells toward -v, limiter, new pi/piH/Pi/PiH/viscCorrCut, and the cell's
 // viscous flux accumulator vf = (tau-dt)*dQ_n - tau*dQ_{n+1} (4 x N).
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) VH_OCC(VH_W_VISC2)
 k_visc2(Geom g, const double *__restrict__ ph, const double *__restrict__ pi0,
         const double *__restrict__ piH0, const double *__restrict__ Pi0,
         const double *__restrict__ PiH0, double *__restrict__ pi, double *__restrict__ piH,
@@ -893,7 +893,7 @@ __device__ __forceinline__ bool visc_face(const Geom &g, long l, long r, const d
 // viscous source (Hydro::visc_source_step, src/hdo.cpp:355-375) and the
 // limited update (Cell::updateByViscFlux, src/cll.cpp:76-84).  Q is updated
 // in place (each thread touches only its own cell).
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) VH_OCC(VH_W_VISC3)
 k_visc3(Geom g, const double *__restrict__ ph, const double *__restrict__ piH,
         const double *__restrict__ PiH, const double *__restrict__ vf,
         const double *__restrict__ m, int m_full, const double *__restrict__ tp, double dt,

@@ -20,6 +20,7 @@ struct Grids {
   const int *bev, *bfirst, *sev, *sfirst;
   long P;
   int nev, nblk, nseg, maxnz;
+  Geom g0;  // grid 0 by value (the kernels use it when nev == 1)
 };
 
 // gv: per-cell con2prim guesses (used by VH_C2P_LEARNED only, may be null otherwise)

@@ -25,11 +25,38 @@ namespace vh {
 namespace mx {
 
 // ------------------------------------------------------------- helpers
+// cell coordinates of local index c (a context holds fewer than 2^29 cells,
+// checked at creation, so 32-bit divisions suffice)
 __device__ __forceinline__ void coords_of(const Geom &g, long c, int &ix, int &iy, int &iz) {
-  ix = (int)(c % g.nx);
-  const long r = c / g.nx;
-  iy = (int)(r % g.ny);
-  iz = (int)(r / g.ny);
+  const unsigned u = (unsigned)c, nx = (unsigned)g.nx, ny = (unsigned)g.ny;
+  const unsigned r = u / nx;
+  ix = (int)(u - r * nx);
+  const unsigned q = r / ny;
+  iy = (int)(r - q * ny);
+  iz = (int)q;
+}
+
+// Grid of workgroup bk and the thread's local cell cl.  A single grid comes
+// by value in the kernel arguments, so no dependent table load precedes the
+// cell's own loads; a batch looks its grid up by workgroup.
+__device__ __forceinline__ Geom block_grid(const Grids &G, long bk, long &cl) {
+  if (G.nev == 1) {
+    cl = bk * blockDim.x + threadIdx.x;
+    return G.g0;
+  }
+  const int gi = G.bev[bk];
+  cl = (bk - G.bfirst[gi]) * blockDim.x + threadIdx.x;
+  return G.ev[gi];
+}
+// grid of row segment sgl (over all grids) and the segment's index in it
+__device__ __forceinline__ Geom seg_grid(const Grids &G, long sgl, long &seg) {
+  if (G.nev == 1) {
+    seg = sgl;
+    return G.g0;
+  }
+  const int gi = G.sev[sgl];
+  seg = sgl - G.sfirst[gi];
+  return G.ev[gi];
 }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
@@ -423,16 +450,14 @@ __device__ __forceinline__ void finish_cell(const DevEos &eos, const double *__r
 // MODE 1 (corrector): reads Q and Qh, writes Qnew = Q + flux unless
 // Q0 + flux0 < 0 (src/cll.cpp:70-74).
 template <int MODE, int CM>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) VH_OCC(VH_W_MX_STAGE)
 k_stage(Grids G, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
         const float *__restrict__ gv, const double *__restrict__ m, int m_full,
         const double *__restrict__ tp, double dt, double *__restrict__ out,
         int *__restrict__ status) {
   const double tau = *tp;
-  const long bk = blockIdx.x;
-  const int gi = G.bev[bk];
-  const Geom g = G.ev[gi];
-  const long cl = (bk - G.bfirst[gi]) * blockDim.x + threadIdx.x;
+  long cl;
+  const Geom g = block_grid(G, blockIdx.x, cl);
   if (cl >= g.N) return;
   const long c = g.base + cl;
   int ix, iy, iz;
@@ -473,7 +498,7 @@ __device__ __forceinline__ bool iface(const DevEos &eos, const double *__restric
 // two y-interfaces are evaluated per lane.  Interface states, fluxes and the
 // accumulation order are those of k_stage: identical results.
 template <int MODE, int CM>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) VH_OCC(VH_W_MX_ROWS)
 k_stage_rows(Grids G, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
              const float *__restrict__ gv, const double *__restrict__ m, int m_full,
              const double *__restrict__ tp, double dt, double *__restrict__ out,
@@ -481,17 +506,17 @@ k_stage_rows(Grids G, DevEos eos, const double *__restrict__ Q, const double *__
   const int lane = threadIdx.x & 63;
   const long w = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const long sgl = w % G.nseg, zc = w / G.nseg;  // row segment over all grids, eta chunk
-  const int gi = G.sev[sgl];
-  const Geom g = G.ev[gi];
+  long seg;
+  const Geom g = seg_grid(G, sgl, seg);
   const long nxy = (long)g.nx * g.ny;
-  const long seg = sgl - G.sfirst[gi];
   const int z0 = (int)zc * CZ;
   if (z0 >= g.nz) return;  // whole wave
   const int z1 = min(g.nz, z0 + CZ);
   const long f = seg * 63 + lane;
   const bool cx = f < nxy;
   const bool upd = cx && lane < 63;
-  const int ix = cx ? (int)(f % g.nx) : 0, iy = cx ? (int)(f / g.nx) : 0;
+  const unsigned fu = (unsigned)f, nxu = (unsigned)g.nx;
+  const int iy = cx ? (int)(fu / nxu) : 0, ix = cx ? (int)(fu - (unsigned)iy * nxu) : 0;
   const long N = g.P, sy = g.nx, sz = nxy;
   const double tau = *tp;
   const double *Qs = MODE == 0 ? Q : Qh;
@@ -563,7 +588,7 @@ k_stage_rows(Grids G, DevEos eos, const double *__restrict__ Q, const double *__
 
 // ---------------------------------------------------------------- viscous
 template <int CM>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) VH_OCC(VH_W_MX_VPRIM)
 k_vprim(long N, DevEos eos, const double *__restrict__ Qh, const float *__restrict__ gv,
         const double *__restrict__ tp, double dt, double *__restrict__ ph,
         double *__restrict__ pt) {
@@ -638,7 +663,7 @@ __device__ __forceinline__ void is_cross(const float *X, const float *Pm, const 
 }
 
 template <int CM>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) VH_OCC(VH_W_MX_VISC1)
 k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
         const double *__restrict__ Qn, const float *__restrict__ gv,
         const double *__restrict__ ph,
@@ -647,16 +672,14 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
         const double *__restrict__ tp, double dt, float4 *__restrict__ rec,
         int *__restrict__ status) {
   const double tau = *tp;
-  const long bk = blockIdx.x;
-  const int gi = G.bev[bk];
-  const Geom g = G.ev[gi];
-  const long cl = (bk - G.bfirst[gi]) * blockDim.x + threadIdx.x;
+  long cl;
+  const Geom g = block_grid(G, blockIdx.x, cl);
   if (cl >= g.N) return;
   const long c = g.base + cl;
   int ix, iy, iz;
   coords_of(g, cl, ix, iy, iz);
   const long N = g.P;
-  const double e = ph[c], nb = ph[2 * N + c], nq = ph[3 * N + c], ns = ph[4 * N + c];
+  const double e = ph[c], nb = ph[2 * N + c];
   const double vx = ph[5 * N + c], vy = ph[6 * N + c], vz = ph[7 * N + c];
   if (e <= 0.) {
     const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -682,6 +705,11 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
           maxM(1, 1) <= 0.9 || maxM(1, -1) <= 0.9 || maxM(-1, 1) <= 0.9 || maxM(-1, -1) <= 0.9;
   }
   const double VMIN = 1e-2, UDIFF = 3.0;
+  // the reference's tests |0.5 (a + b) / u| > UDIFF as |0.5 (a + b)| > UDIFF |u|,
+  // and its divisions by u^k as products with 1/u^k (used only where
+  // |u^k| >= VMIN, or k = 0 with u^0 >= 1)
+  const double ru[4] = {1. / u[0], 1. / u[1], 1. / u[2], 1. / u[3]};
+  const double ihx = 1. / g.dx, ihy = 1. / g.dy, ihz = 1. / (g.dz * (tau + 0.5 * dt));
   if (!vac) {
     double q[7];
     Prim p0, p1;
@@ -701,9 +729,10 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         // d(u^2)/2u form; plain difference where u^k is small or changes fast
-        const bool plain = k == 0 ? fabs(0.5 * (a1[0] + a0[0]) / a1[0]) > UDIFF
-                                  : (fabs(u[k]) < VMIN || fabs(0.5 * (a1[k] + a0[k]) / a1[k]) > UDIFF);
-        dmu[0][k] = plain ? (a1[k] - a0[k]) * idt : (a1[k] * a1[k] - a0[k] * a0[k]) * 0.5 / u[k] * idt;
+        const bool fast = fabs(0.5 * (a1[k] + a0[k])) > UDIFF * fabs(a1[k]);
+        const bool plain = k == 0 ? fast : (fabs(u[k]) < VMIN || fast);
+        dmu[0][k] = plain ? (a1[k] - a0[k]) * idt
+                          : (a1[k] * a1[k] - a0[k] * a0[k]) * (0.5 * idt) * ru[k];
       }
     }
 #pragma unroll
@@ -717,17 +746,19 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
         const double ga = 1.0 / sqrt(1.0 - avx * avx - avy * avy - avz * avz);
         const double B[4] = {gb, gb * bvx, gb * bvy, gb * bvz};
         const double A[4] = {ga, ga * avx, ga * avy, ga * avz};
-        const double ih = d == 1 ? 1. / g.dx : (d == 2 ? 1. / g.dy : 1. / (g.dz * (tau + 0.5 * dt)));
+        const double ih = d == 1 ? ihx : (d == 2 ? ihy : ihz);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-          const bool plain = k == 0 ? fabs(0.5 * (A[0] + B[0]) / u[0]) > UDIFF
-                                    : (fabs(u[k]) < VMIN || fabs(0.5 * (A[k] + B[k]) / u[k]) > UDIFF);
-          dmu[d][k] = plain ? 0.5 * (A[k] - B[k]) * ih : 0.25 * (A[k] * A[k] - B[k] * B[k]) / u[k] * ih;
+          const bool fast = fabs(0.5 * (A[k] + B[k])) > UDIFF * fabs(u[k]);
+          const bool plain = k == 0 ? fast : (fabs(u[k]) < VMIN || fast);
+          dmu[d][k] = plain ? 0.5 * (A[k] - B[k]) * ih
+                            : (A[k] * A[k] - B[k] * B[k]) * (0.25 * ih) * ru[k];
         }
       }
     }
-    dmu[3][0] += u[3] / (tau - 0.5 * dt);
-    dmu[3][3] += u[0] / (tau - 0.5 * dt);
+    const double itm = 1. / (tau - 0.5 * dt);
+    dmu[3][0] += u[3] * itm;
+    dmu[3][3] += u[0] * itm;
   }
   const double du = dmu[0][0] + dmu[1][1] + dmu[2][2] + dmu[3][3];
   // sigma^{ij} = 1/2 (D^{ik} dmu_kj + D^{jk} dmu_ki) - 1/3 D^{ij} theta, D = g - uu
@@ -737,12 +768,14 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int j = 0; j < 4; j++) Dm[i][j] = (i == j ? gmd[i] : 0.) - u[i] * u[j];
-  const double s = eos.s(e, nb, nq, ns);
-  const double T = eos.T(e), p = e * (1. / 3.);
-  const double etaS = trc.etaS(e, nb, T, 0., s, p);
-  const double zetaS = trc.zetaS(eos, e, T, s, p);
-  const double eta = etaS * s;
-  const double kpi = 2.0 * eta / 5.068;
+  // thermodynamics and transport coefficients in FP32 (they scale pi, which
+  // is stored in FP32)
+  const float ef = (float)e, nbf = (float)nb;
+  const float p = eos.pf(ef), T = eos.Tf(ef), s = eos.sf(ef, T);
+  const float etaS = trc.etaSf(ef, nbf, T, s, p);
+  const float zetaS = trc.zetaSf(eos, ef, T, s, p);
+  const float eta = etaS * s;
+  const double kpi = (double)(eta * (float)(2.0 / 5.068));
   float sig[10], piNS[10];
   bool bad = false;
 #pragma unroll
@@ -759,16 +792,15 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
       sig[sym(i, j)] = eta > 0. ? (float)t : 0.f;
     }
   if (bad) atomicOr(status, VH_ST_NAN_PI);
-  const float PiNS = (float)(-zetaS * s * du / 5.068);
+  const float PiNS = -zetaS * s * (float)du * (float)(1. / 5.068);
   // ---- relaxation times and second-order coefficients (src/trancoeff.h:28-37)
-  double taupi_d, tauPi_d;
-  trc.getTau(eos, e, nb, T, 0., s, p, taupi_d, tauPi_d);
-  const float taupi = (float)taupi_d, tauPi = (float)tauPi_d;
+  float taupi, tauPi;
+  trc.getTauf(eos, ef, nbf, T, s, p, etaS, zetaS, taupi, tauPi);
   const float hdt = (float)(0.5 * dt), fdt = (float)dt;
   float dpp = 4.f / 3.f, tpp = 10.f / 7.f, lpP = 6.f / 5.f;
-  float phi7 = p > 0. ? (float)(9. / 70. / p / taupi_d) : 0.f;
+  float phi7 = p > 0.f ? (9.f / 70.f) / (p * taupi) : 0.f;
   if (taupi < hdt) dpp = tpp = lpP = phi7 = 0.f;
-  float dPP = 2.f / 3.f, lPp = (float)(8. / 5. * (1. / 3. - eos.cs2e(e)));
+  float dPP = 2.f / 3.f, lPp = 1.6f * (1.f / 3.f - eos.cs2f(ef));
   if (tauPi < hdt) dPP = lPp = 0.f;
   const float ig = (float)(1. / gamma);
   const float uf[4] = {(float)u[0], (float)u[1], (float)u[2], (float)u[3]};
@@ -844,21 +876,15 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
   rec[5 * N + c] = make_float4(H[8], H[9], PH, 0.f);
 }
 
-// ISformal loop #2 (src/hdo.cpp:866-962) in FP32 with FP32 pi storage.
-__global__ void __launch_bounds__(256)
-k_visc2(Grids G, const double *__restrict__ ph, const float4 *__restrict__ rec,
-        float *__restrict__ pi, float *__restrict__ piH,
-        float *__restrict__ Pi, float *__restrict__ PiH, double *__restrict__ vcc,
-        float *__restrict__ vf, const double *__restrict__ tp, double dt) {
-  const double tau = *tp;
-  const long bk = blockIdx.x;
-  const int gi = G.bev[bk];
-  const Geom g = G.ev[gi];
-  const long cl = (bk - G.bfirst[gi]) * blockDim.x + threadIdx.x;
-  if (cl >= g.N) return;
-  const long c = g.base + cl;
-  int ix, iy, iz;
-  coords_of(g, cl, ix, iy, iz);
+// ISformal loop #2 (src/hdo.cpp:866-962) in FP32 with FP32 pi storage, for
+// cell c = (ix, iy, iz) of grid g.
+__device__ __forceinline__ void visc2_cell(const Geom &g, long c, int ix, int iy, int iz,
+                                           const double *__restrict__ ph,
+                                           const float4 *__restrict__ rec,
+                                           float *__restrict__ pi, float *__restrict__ piH,
+                                           float *__restrict__ Pi, float *__restrict__ PiH,
+                                           double *__restrict__ vcc, float *__restrict__ vf,
+                                           double tau, double dt) {
   const long N = g.P;
   const float e = (float)ph[c], p = (float)ph[N + c];
   const float vx = (float)ph[5 * N + c], vy = (float)ph[6 * N + c], vz = (float)ph[7 * N + c];
@@ -945,6 +971,20 @@ k_visc2(Grids G, const double *__restrict__ ph, const float4 *__restrict__ rec,
   for (int i = 0; i < 4; i++) vf[i * N + c] = fl[i];
 }
 
+// one thread per cell
+__global__ void __launch_bounds__(256) VH_OCC(VH_W_MX_VISC2)
+k_visc2(Grids G, const double *__restrict__ ph, const float4 *__restrict__ rec,
+        float *__restrict__ pi, float *__restrict__ piH,
+        float *__restrict__ Pi, float *__restrict__ PiH, double *__restrict__ vcc,
+        float *__restrict__ vf, const double *__restrict__ tp, double dt) {
+  long cl;
+  const Geom g = block_grid(G, blockIdx.x, cl);
+  if (cl >= g.N) return;
+  int ix, iy, iz;
+  coords_of(g, cl, ix, iy, iz);
+  visc2_cell(g, g.base + cl, ix, iy, iz, ph, rec, pi, piH, Pi, PiH, vcc, vf, *tp, dt);
+}
+
 // viscous interface flux in FP32 (Hydro::visc_flux, src/hdo.cpp:966-1014)
 template <int D>
 __device__ __forceinline__ bool visc_face(const Geom &g, long l, long r,
@@ -972,16 +1012,14 @@ __device__ __forceinline__ bool visc_face(const Geom &g, long l, long r,
   return true;
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) VH_OCC(VH_W_MX_VISC3)
 k_visc3(Grids G, const double *__restrict__ ph, const float *__restrict__ piH,
         const float *__restrict__ PiH, const float *__restrict__ vf,
         const double *__restrict__ m, int m_full, const double *__restrict__ tp, double dt,
         double *__restrict__ Q) {
   const double tau = *tp;
-  const long bk = blockIdx.x;
-  const int gi = G.bev[bk];
-  const Geom g = G.ev[gi];
-  const long cl = (bk - G.bfirst[gi]) * blockDim.x + threadIdx.x;
+  long cl;
+  const Geom g = block_grid(G, blockIdx.x, cl);
   if (cl >= g.N) return;
   const long c = g.base + cl;
   int ix, iy, iz;
@@ -1090,9 +1128,8 @@ __device__ __forceinline__ int ghost_src(int i, int n) {
 template <int FULL>
 __global__ void __launch_bounds__(256)
 k_ghost_zero_b(Grids G, double *__restrict__ Q, double *__restrict__ Qh) {
-  const int gi = G.bev[blockIdx.x];
-  const Geom g = G.ev[gi];
-  const long cl = (long)(blockIdx.x - G.bfirst[gi]) * blockDim.x + threadIdx.x;
+  long cl;
+  const Geom g = block_grid(G, blockIdx.x, cl);
   if (cl >= g.N || (g.nx > 1 && g.ny > 1 && g.nz > 1)) return;
   const long c = g.base + cl, N = g.P;
   if (Q[c] < 0.)
@@ -1105,9 +1142,8 @@ template <int FULL>
 __global__ void __launch_bounds__(256)
 k_ghost_b(Grids G, double *__restrict__ Q, double *__restrict__ Qh, float *__restrict__ pi,
           float *__restrict__ Pi) {
-  const int gi = G.bev[blockIdx.x];
-  const Geom g = G.ev[gi];
-  const long cl = (long)(blockIdx.x - G.bfirst[gi]) * blockDim.x + threadIdx.x;
+  long cl;
+  const Geom g = block_grid(G, blockIdx.x, cl);
   if (cl >= g.N) return;
   int ix, iy, iz;
   coords_of(g, cl, ix, iy, iz);

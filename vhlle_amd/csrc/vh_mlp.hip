@@ -113,19 +113,33 @@ k_c2p_learned(DevEos eos, const Packed *__restrict__ P, double *__restrict__ Q, 
 }
 
 // per-cell guess of |v| for the hydro step: Qs (SoA 7 x N) / (tau + toff)
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) VH_OCC(VH_W_GUESS)
 k_guess(const double *__restrict__ Qs, long N, const double *__restrict__ tp, double toff,
         const Packed *__restrict__ P, float *__restrict__ gv) {
   __shared__ Packed w;
   load_packed(w, P);
   __syncthreads();
   const double it = 1. / (*tp + toff);
-  VH_WAVE_LOOP(base, N) {
+  // the next iteration's state is loaded before this one's MLP runs
+  const long stride = (long)gridDim.x * blockDim.x;
+  long base = (long)blockIdx.x * blockDim.x;
+  double qn[7];
+  {
+    const long j = base + threadIdx.x < N ? base + threadIdx.x : N - 1;
+#pragma unroll
+    for (int k = 0; k < 7; k++) qn[k] = Qs[k * N + j];
+  }
+  for (; base < N; base += stride) {
     const long c = base + threadIdx.x;
-    const long j = c < N ? c : N - 1;
     double q[7];
 #pragma unroll
-    for (int k = 0; k < 7; k++) q[k] = Qs[k * N + j] * it;
+    for (int k = 0; k < 7; k++) q[k] = qn[k] * it;
+    const long cn = c + stride;
+    if (base + stride < N) {
+      const long j = cn < N ? cn : N - 1;
+#pragma unroll
+      for (int k = 0; k < 7; k++) qn[k] = Qs[k * N + j];
+    }
     float f[5];
     features(q, f);
     const float2 o = mlp_wave(w, f);
