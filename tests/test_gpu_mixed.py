@@ -202,3 +202,31 @@ def test_vacuum_tails_below_fp32_and_fp64_squares(gpu, c2p):
             eT, _ = errors(mix, ref, ref["tau"])
             print(f"c2p={c2p} ideal: T^tau-mu rel err {eT:.3e} (e > 0.5)", flush=True)
             assert eT < 1e-6
+
+
+@pytest.mark.parametrize("c2p", [abi.C2P_CLOSED, abi.C2P_NEWTON])
+def test_charge_free_kernels_match_charged(gpu, c2p):
+    """A state without charges runs the 4-component kernels (n_b, n_q, n_s
+    are zero and stay zero); the same state with n_b = 1e-300 (rounded to 0 in
+    every FP32 operation) runs the 7-component kernels.  T^{tau mu} and pi
+    must be bit-identical, and the charge components of the first run zero."""
+    cfg = setups.VISCOUS_3D
+    nx, ny, nz = 30, 28, 12
+    e, vx, vy, vz = setups.gaussian3d_ic(nx, ny, nz, cfg["xlim"], cfg["ylim"], cfg["zlim"],
+                                         e0=cfg["e0"], sx=cfg["sx"], sy=cfg["sy"],
+                                         eta_plateau=cfg["eta_plateau"],
+                                         eta_sigma=cfg["eta_sigma"])
+    vx = 0.1 * np.tanh(setups.axes(nx, ny, nz, cfg["xlim"], cfg["ylim"], cfg["zlim"])[0])
+    out = []
+    for nb in (None, np.full_like(e, 1e-300)):
+        ctx = abi.Context(nx, ny, nz, cfg["xlim"], cfg["ylim"], cfg["zlim"],
+                          trc=abi.make_trc(etaS=0.1), precision=abi.PREC_MIXED)
+        ctx.set_c2p(c2p)
+        ctx.set_prim(0.6, e, vx, vy, vz, nb=nb)
+        st, _ = ctx.run(0.6, 0.03, 20)
+        assert st == 0
+        out.append(ctx.download())
+        ctx.close()
+    a, b = out
+    assert np.array_equal(a["Q"][:4], b["Q"][:4]) and np.array_equal(a["pi"], b["pi"])
+    assert not a["Q"][4:].any() and not a["Qh"][4:].any() and b["Q"][4].any()

@@ -76,6 +76,9 @@ struct vh_ctx {
   vh::mx::Grids G = {};
   char *gtab = nullptr;
   bool fusable = true, degenerate = false;
+  // some n_b, n_q or n_s nonzero: the mixed kernels carry 7 conserved
+  // components, else 4 (charges that are zero everywhere stay zero)
+  bool charged = true;
   // stage kernel shape (mixed precision): kind 0 until the autotuner ran
   vh::mx::Tile tile = {0, 0};
   bool tile_forced = false;
@@ -176,7 +179,7 @@ int enqueue_step_mixed(vh_ctx *c, double dt) {
   const vh::Geom &g = c->g;
   hipStream_t st = c->st;
   double *Qo = c->Q[c->cur], *Qn = c->Q[c->cur ^ 1];
-  const int cm = c->c2p_mode;
+  const int cm = c->c2p_mode, nc = c->charged ? 7 : 4;
   const vh::mx::Grids &G = c->G;
   int rc = VH_OK;
   if (!c->m_full) {
@@ -188,21 +191,21 @@ int enqueue_step_mixed(vh_ctx *c, double dt) {
   // the MLP guess of every cell of the stage's input state (Q/tau_n, then
   // Qh/tau_n: both stages reconstruct on tau_n-scaled states)
   if (learned)
-    rc |= timed(c, "c2p_guess", [&] { vh::ml::launch_guess(Qo, g.N, c->tp, 0., c->mlp, c->gv, st); });
+    rc |= timed(c, "c2p_guess", [&] { vh::ml::launch_guess(Qo, g.N, nc, c->tp, 0., c->mlp, c->gv, st); });
   auto stage = [&](int mode, double *out) {
-    vh::mx::launch_stage_tile(mode, cm, G, c->eos, Qo, c->Qh, c->gv, c->m, c->m_full, c->tp,
+    vh::mx::launch_stage_tile(mode, cm, nc, G, c->eos, Qo, c->Qh, c->gv, c->m, c->m_full, c->tp,
                               dt, out, c->status, c->tile, st);
   };
   rc |= timed(c, "stage_pred", [&] { stage(0, c->Qh); });
   if (learned)
-    rc |= timed(c, "c2p_guess", [&] { vh::ml::launch_guess(c->Qh, g.N, c->tp, 0., c->mlp, c->gv, st); });
+    rc |= timed(c, "c2p_guess", [&] { vh::ml::launch_guess(c->Qh, g.N, nc, c->tp, 0., c->mlp, c->gv, st); });
   rc |= timed(c, "stage_corr", [&] { stage(1, Qn); });
   rc |= timed(c, "advance_tau", [&] { vh::launch_advance_tau(c->tp, dt, st); });
   ghost_fill(c, 0, Qn, st, rc);
   if (c->viscous) {
-    rc |= timed(c, "visc_prim", [&] { vh::mx::launch_vprim(cm, g.N, c->eos, c->Qh, c->gv, c->tp, dt, c->ph, c->pt, st); });
+    rc |= timed(c, "visc_prim", [&] { vh::mx::launch_vprim(cm, nc, g.N, c->eos, c->Qh, c->gv, c->tp, dt, c->ph, c->pt, st); });
     rc |= timed(c, "visc_relax", [&] {
-      vh::mx::launch_visc1(cm, G, c->eos, c->trc, Qo, Qn, c->gv, c->ph, c->pt, c->m, c->m_full,
+      vh::mx::launch_visc1(cm, nc, G, c->eos, c->trc, Qo, Qn, c->gv, c->ph, c->pt, c->m, c->m_full,
                            fp(c->pi), fp(c->Pi), c->tp, dt, rec(c), c->status, st);
     });
     rc |= timed(c, "visc_advect", [&] {
@@ -282,6 +285,20 @@ void drop_graphs(vh_ctx *c) {
       c->gexec[i] = nullptr;
     }
   c->g_mfull = -1;
+}
+
+bool any_nonzero(const double *a, long n) {
+  for (long i = 0; i < n; i++)
+    if (a[i] != 0.) return true;
+  return false;
+}
+
+// switches the mixed kernels between 7 and 4 conserved components
+void set_charged(vh_ctx *c, bool ch) {
+  if (c->charged == ch) return;
+  c->charged = ch;
+  if (!c->tile_forced) c->tile.kind = 0;  // retune for the other kernels
+  drop_graphs(c);
 }
 
 int build_graphs(vh_ctx *c, double dt) {
@@ -549,6 +566,11 @@ long vh_ncell(const vh_ctx_t *c) { return c ? c->g.N : 0; }
 
 int vh_upload(vh_ctx_t *c, const vh_state_t *h) {
   if (!c || !h) return VH_ERR_ARG;
+  {
+    bool ch = false;
+    for (const double *a : {h->Q, h->Qprev, h->Qh}) ch |= a && any_nonzero(a + 4 * c->g.N, 3 * c->g.N);
+    set_charged(c, ch);
+  }
   const long N = c->g.N;
   const size_t d = sizeof(double);
   auto put = [&](double *dst, const double *src, int nc, double dflt) -> int {
@@ -643,6 +665,15 @@ int vh_set_prim(vh_ctx_t *c, double tau, const double *e, const double *nb, cons
       hipSuccess)
     rc = VH_ERR_HIP;
   if (rc == VH_OK) {
+    bool ch = false;
+    for (const double *a : {nb, nq, ns}) ch |= a && any_nonzero(a, N);
+    set_charged(c, ch);
+    // Qh is not written here: clear its charge components for the
+    // 4-component kernels, which leave them untouched
+    if (!ch && hipMemsetAsync(c->Qh + 4 * N, 0, 3 * N * sizeof(double), c->st) != hipSuccess)
+      rc = VH_ERR_HIP;
+  }
+  if (rc == VH_OK) {
     vh::launch_set_prim(c->g, c->eos, tau, d, c->Q[c->cur], c->Q[c->cur ^ 1], c->m, c->st);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->st) != hipSuccess)
       rc = VH_ERR_HIP;
@@ -726,8 +757,9 @@ static int ensure_tile(vh_ctx_t *c, double tau, double dt) {
   double *so = c->F[1];                               // unused by the mixed path
   int *ss = reinterpret_cast<int *>(c->fl[0]);        // scratch status word
   VH_HIP(hipMemcpyAsync(c->tp, &tau, sizeof(double), hipMemcpyHostToDevice, st));
+  const int nc = c->charged ? 7 : 4;
   if (c->c2p_mode == VH_C2P_LEARNED)
-    vh::ml::launch_guess(c->Q[c->cur], g.N, c->tp, 0., c->mlp, c->gv, st);
+    vh::ml::launch_guess(c->Q[c->cur], g.N, nc, c->tp, 0., c->mlp, c->gv, st);
   hipEvent_t e0, e1;
   VH_HIP(hipEventCreate(&e0));
   VH_HIP(hipEventCreate(&e1));
@@ -736,8 +768,8 @@ static int ensure_tile(vh_ctx_t *c, double tau, double dt) {
   int rc = VH_OK;
   for (const vh::mx::Tile &t : cand) {
     auto go = [&] {
-      vh::mx::launch_stage_tile(0, c->c2p_mode, c->G, c->eos, c->Q[c->cur], c->Qh, c->gv, c->m,
-                                c->m_full, c->tp, dt, so, ss, t, st);
+      vh::mx::launch_stage_tile(0, c->c2p_mode, nc, c->G, c->eos, c->Q[c->cur], c->Qh, c->gv,
+                                c->m, c->m_full, c->tp, dt, so, ss, t, st);
     };
     go();
     go();

@@ -24,7 +24,7 @@ struct Grids {
 };
 
 // gv: per-cell con2prim guesses (used by VH_C2P_LEARNED only, may be null otherwise)
-void launch_stage(int mode, int cm, const Grids &G, const DevEos &eos, const double *Q,
+void launch_stage(int mode, int cm, int nc, const Grids &G, const DevEos &eos, const double *Q,
                   const double *Qh, const float *gv, const double *m, int m_full,
                   const double *tp, double dt, double *out, int *status, hipStream_t st);
 // Stage kernel shapes (same results for both): STAGE_CELLS one thread per
@@ -38,15 +38,15 @@ Tile make_rows(const Grids &G, int CZ);
 Tile make_cells();
 // candidates of the autotuner: the cell kernel and rows with several eta chunks
 std::vector<Tile> tile_candidates(const Grids &G);
-void launch_stage_tile(int mode, int cm, const Grids &G, const DevEos &eos, const double *Q,
+void launch_stage_tile(int mode, int cm, int nc, const Grids &G, const DevEos &eos, const double *Q,
                        const double *Qh, const float *gv, const double *m, int m_full,
                        const double *tp, double dt, double *out, int *status, const Tile &tl,
                        hipStream_t st);
 // con2prim of every cell (N cells, pitch N): ph = prims of Qh at tau - dt/2,
 // pt = (e, v) of Qh at tau
-void launch_vprim(int cm, long N, const DevEos &eos, const double *Qh, const float *gv,
+void launch_vprim(int cm, int nc, long N, const DevEos &eos, const double *Qh, const float *gv,
                   const double *tp, double dt, double *ph, double *pt, hipStream_t st);
-void launch_visc1(int cm, const Grids &G, const DevEos &eos, const DevTrc &trc,
+void launch_visc1(int cm, int nc, const Grids &G, const DevEos &eos, const DevTrc &trc,
                   const double *Qprev, const double *Qn, const float *gv, const double *ph,
                   const double *pt, const double *m, int m_full, const float *pi,
                   const float *Pi, const double *tp, double dt, float4 *rec, int *status,

@@ -263,14 +263,19 @@ __device__ __forceinline__ float guess_of(const float *__restrict__ gv, long c) 
 }
 
 // MUSCL face state (src/cll.cpp:106-204) in FP64, con2prim by mode, as the
-// FP32 state the flux takes.
-template <int SIDE, int CM>
+// FP32 state the flux takes.  NC: conserved components carried (7, or 4 for
+// a state without charges, whose N^tau components are zero and stay zero).
+template <int SIDE, int CM, int NC>
 __device__ __forceinline__ PrimF face_state(const DevEos &eos, const double *__restrict__ Qs,
                                             const float *__restrict__ gv, long N, long cm,
                                             long c0, long cp, double itau) {
   double q[7];
 #pragma unroll
   for (int k = 0; k < 7; k++) {
+    if (k >= NC) {
+      q[k] = 0.;
+      continue;
+    }
     const double a = Qs[k * N + c0];
     const double dq = minmod((Qs[k * N + cp] - a) * 0.5, (a - Qs[k * N + cm]) * 0.5);
     q[k] = (SIDE < 0 ? a - dq : a + dq) * itau;
@@ -285,7 +290,7 @@ __device__ __forceinline__ PrimF face_state(const DevEos &eos, const double *__r
 // HLLE interface flux in FP32 (Hydro::hlle_flux, src/hdo.cpp:77-305: same
 // wave speeds, vacuum rules and skips).  Returns false if the reference would
 // return without a flux.  pre = tauFactor * dta / dx.
-template <int D>
+template <int D, int NC>
 __device__ __forceinline__ bool hlle_f32(PrimF L, PrimF R, float itau, float itf, float pre,
                                          bool mskip, float F[7], int *status) {
   if (L.e < 0.f) L.e = L.p = 0.f;
@@ -304,21 +309,25 @@ __device__ __forceinline__ bool hlle_f32(PrimF L, PrimF R, float itau, float itf
   Ul[Y_] = wl * L.vy;
   Ul[Z_] = wl * L.vz;
   Ul[T_] = wl - L.p;
-  Ul[NB_] = gl * L.nb;
-  Ul[NQ_] = gl * L.nq;
-  Ul[NS_] = gl * L.ns;
+  if (NC == 7) {
+    Ul[NB_] = gl * L.nb;
+    Ul[NQ_] = gl * L.nq;
+    Ul[NS_] = gl * L.ns;
+  }
   Ur[X_] = wr * R.vx;
   Ur[Y_] = wr * R.vy;
   Ur[Z_] = wr * R.vz;
   Ur[T_] = wr - R.p;
-  Ur[NB_] = gr * R.nb;
-  Ur[NQ_] = gr * R.nq;
-  Ur[NS_] = gr * R.ns;
+  if (NC == 7) {
+    Ur[NB_] = gr * R.nb;
+    Ur[NQ_] = gr * R.nq;
+    Ur[NS_] = gr * R.ns;
+  }
   const float vl = D == 1 ? L.vx : (D == 2 ? L.vy : L.vz);
   const float vr = D == 1 ? R.vx : (D == 2 ? R.vy : R.vz);
   float Fl[7], Fr[7];
 #pragma unroll
-  for (int k = 0; k < 7; k++) {
+  for (int k = 0; k < NC; k++) {
     Fl[k] = Ul[k] * vl;
     Fr[k] = Ur[k] * vr;
   }
@@ -328,7 +337,7 @@ __device__ __forceinline__ bool hlle_f32(PrimF L, PrimF R, float itau, float itf
   Fr[D] += R.p;
   if (D == 3) {
 #pragma unroll
-    for (int k = 0; k < 7; k++) {
+    for (int k = 0; k < NC; k++) {
       Fl[k] *= itf;
       Fr[k] *= itf;
     }
@@ -357,8 +366,9 @@ __device__ __forceinline__ bool hlle_f32(PrimF L, PrimF R, float itau, float itf
   const float s = pre * frcp(br - bl);
   const float bb = bl * br;
 #pragma unroll
-  for (int k = 0; k < 7; k++) F[k] = s * (br * Fl[k] - bl * Fr[k] - bb * (Ul[k] - Ur[k]));
-  if (F[NB_] != F[NB_]) {
+  for (int k = 0; k < NC; k++) F[k] = s * (br * Fl[k] - bl * Fr[k] - bb * (Ul[k] - Ur[k]));
+  // the reference tests the N_b flux; without charges a NaN state shows in F^tau
+  if (F[NC == 7 ? NB_ : T_] != F[NC == 7 ? NB_ : T_]) {
     atomicOr(status, VH_ST_NAN_FLUX);
     return false;
   }
@@ -367,7 +377,7 @@ __device__ __forceinline__ bool hlle_f32(PrimF L, PrimF R, float itau, float itf
 
 // the two interfaces of cell c along axis D, added into acc in the order
 // +F(i-1/2), -F(i+1/2)
-template <int D, int MODE, int CM>
+template <int D, int MODE, int CM, int NC>
 __device__ __forceinline__ void axis_fluxes(const Geom &g, const DevEos &eos,
                                             const double *__restrict__ Qs,
                                             const float *__restrict__ gv,
@@ -383,30 +393,30 @@ __device__ __forceinline__ void axis_fluxes(const Geom &g, const DevEos &eos,
   const float pre = (float)(tauFactor * dta / dx);
   const float itf = (float)(1. / tauFactor), itauf = (float)itau;
   const long cm = i > 0 ? c - s : c, cp = i < n - 1 ? c + s : c;
-  const PrimF L0 = face_state<-1, CM>(eos, Qs, gv, N, cm, c, cp, itau);
-  const PrimF R0 = face_state<+1, CM>(eos, Qs, gv, N, cm, c, cp, itau);
+  const PrimF L0 = face_state<-1, CM, NC>(eos, Qs, gv, N, cm, c, cp, itau);
+  const PrimF R0 = face_state<+1, CM, NC>(eos, Qs, gv, N, cm, c, cp, itau);
   float F[7];
   if (i > 0) {
     const long cmm = i > 1 ? cm - s : cm;
-    const PrimF Rm = face_state<+1, CM>(eos, Qs, gv, N, cmm, cm, c, itau);
+    const PrimF Rm = face_state<+1, CM, NC>(eos, Qs, gv, N, cmm, cm, c, itau);
     const bool mk = !m_full && m[(D - 1) * N + cm] < 1. && m[(D - 1) * N + c] < 1.;
-    if (hlle_f32<D>(Rm, L0, itauf, itf, pre, mk, F, status))
+    if (hlle_f32<D, NC>(Rm, L0, itauf, itf, pre, mk, F, status))
 #pragma unroll
-      for (int k = 0; k < 7; k++) acc[k] += (double)F[k];
+      for (int k = 0; k < NC; k++) acc[k] += (double)F[k];
   }
   if (i < n - 1) {
     const long cpp = i + 2 < n ? cp + s : cp;
-    const PrimF Lp = face_state<-1, CM>(eos, Qs, gv, N, c, cp, cpp, itau);
+    const PrimF Lp = face_state<-1, CM, NC>(eos, Qs, gv, N, c, cp, cpp, itau);
     const bool mk = !m_full && m[(D - 1) * N + c] < 1. && m[(D - 1) * N + cp] < 1.;
-    if (hlle_f32<D>(R0, Lp, itauf, itf, pre, mk, F, status))
+    if (hlle_f32<D, NC>(R0, Lp, itauf, itf, pre, mk, F, status))
 #pragma unroll
-      for (int k = 0; k < 7; k++) acc[k] -= (double)F[k];
+      for (int k = 0; k < NC; k++) acc[k] -= (double)F[k];
   }
 }
 
 // Milne source on the unreconstructed state (src/hdo.cpp:307-353) and the
 // stage update of cell c with the accumulated interface fluxes acc.
-template <int MODE, int CM>
+template <int MODE, int CM, int NC>
 __device__ __forceinline__ void finish_cell(const DevEos &eos, const double *__restrict__ Q,
                                             const double *__restrict__ Qs,
                                             const float *__restrict__ gv, long N, long c,
@@ -417,7 +427,7 @@ __device__ __forceinline__ void finish_cell(const DevEos &eos, const double *__r
   const double it1 = 1. / tau1;
   double q[7];
 #pragma unroll
-  for (int k = 0; k < 7; k++) q[k] = Qs[k * N + c] * it1;
+  for (int k = 0; k < 7; k++) q[k] = k < NC ? Qs[k * N + c] * it1 : 0.;
   double vz, pr;
   if (CM == VH_C2P_CLOSED) {
     const PrimF r = c2p_fast(q);
@@ -437,11 +447,11 @@ __device__ __forceinline__ void finish_cell(const DevEos &eos, const double *__r
   acc[Z_] += -q[Z_] * sdt;
   if (MODE == 0) {
 #pragma unroll
-    for (int k = 0; k < 7; k++) out[k * N + c] = Q[k * N + c] + acc[k];
+    for (int k = 0; k < NC; k++) out[k * N + c] = Q[k * N + c] + acc[k];
   } else {
     const bool skip = Q[c] + acc[0] < 0.;
 #pragma unroll
-    for (int k = 0; k < 7; k++) out[k * N + c] = skip ? Q[k * N + c] : Q[k * N + c] + acc[k];
+    for (int k = 0; k < NC; k++) out[k * N + c] = skip ? Q[k * N + c] : Q[k * N + c] + acc[k];
   }
 }
 
@@ -449,7 +459,7 @@ __device__ __forceinline__ void finish_cell(const DevEos &eos, const double *__r
 // MODE 0 (predictor): reads Q, writes Qh = Q + flux.
 // MODE 1 (corrector): reads Q and Qh, writes Qnew = Q + flux unless
 // Q0 + flux0 < 0 (src/cll.cpp:70-74).
-template <int MODE, int CM>
+template <int MODE, int CM, int NC>
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_MX_STAGE)
 k_stage(Grids G, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
         const float *__restrict__ gv, const double *__restrict__ m, int m_full,
@@ -467,26 +477,26 @@ k_stage(Grids G, DevEos eos, const double *__restrict__ Q, const double *__restr
   double acc[7];
 #pragma unroll
   for (int k = 0; k < 7; k++) acc[k] = 0.;
-  axis_fluxes<1, MODE, CM>(g, eos, Qs, gv, m, m_full, ix, g.nx, c, 1, tau, dt, acc, status);
-  axis_fluxes<2, MODE, CM>(g, eos, Qs, gv, m, m_full, iy, g.ny, c, g.nx, tau, dt, acc, status);
-  axis_fluxes<3, MODE, CM>(g, eos, Qs, gv, m, m_full, iz, g.nz, c, (long)g.nx * g.ny, tau, dt, acc,
-                           status);
-  finish_cell<MODE, CM>(eos, Q, Qs, gv, N, c, tau, dt, acc, out);
+  axis_fluxes<1, MODE, CM, NC>(g, eos, Qs, gv, m, m_full, ix, g.nx, c, 1, tau, dt, acc, status);
+  axis_fluxes<2, MODE, CM, NC>(g, eos, Qs, gv, m, m_full, iy, g.ny, c, g.nx, tau, dt, acc, status);
+  axis_fluxes<3, MODE, CM, NC>(g, eos, Qs, gv, m, m_full, iz, g.nz, c, (long)g.nx * g.ny, tau, dt,
+                               acc, status);
+  finish_cell<MODE, CM, NC>(eos, Q, Qs, gv, N, c, tau, dt, acc, out);
 }
 
 // Interface flux between cells a (left) and b (right) along axis D: MUSCL
 // states of a (neighbours amm, b) and b (neighbours a, bp), HLLE in FP32.
 // Same states, mask and flux as the interfaces of axis_fluxes.
-template <int D, int CM>
+template <int D, int CM, int NC>
 __device__ __forceinline__ bool iface(const DevEos &eos, const double *__restrict__ Qs,
                                       const float *__restrict__ gv,
                                       const double *__restrict__ m, int m_full, long N, long amm,
                                       long a, long b, long bp, double itau, float itauf,
                                       float itf, float pre, float F[7], int *status) {
-  const PrimF L = face_state<+1, CM>(eos, Qs, gv, N, amm, a, b, itau);
-  const PrimF R = face_state<-1, CM>(eos, Qs, gv, N, a, b, bp, itau);
+  const PrimF L = face_state<+1, CM, NC>(eos, Qs, gv, N, amm, a, b, itau);
+  const PrimF R = face_state<-1, CM, NC>(eos, Qs, gv, N, a, b, bp, itau);
   const bool mk = !m_full && m[(D - 1) * N + a] < 1. && m[(D - 1) * N + b] < 1.;
-  return hlle_f32<D>(L, R, itauf, itf, pre, mk, F, status);
+  return hlle_f32<D, NC>(L, R, itauf, itf, pre, mk, F, status);
 }
 
 // Row stage: every x- and eta-interface is evaluated once, without barriers.
@@ -497,7 +507,7 @@ __device__ __forceinline__ bool iface(const DevEos &eos, const double *__restric
 // upper eta-interface stays in registers as the next cell's lower one; the
 // two y-interfaces are evaluated per lane.  Interface states, fluxes and the
 // accumulation order are those of k_stage: identical results.
-template <int MODE, int CM>
+template <int MODE, int CM, int NC>
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_MX_ROWS)
 k_stage_rows(Grids G, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
              const float *__restrict__ gv, const double *__restrict__ m, int m_full,
@@ -536,24 +546,24 @@ k_stage_rows(Grids G, DevEos eos, const double *__restrict__ Q, const double *__
     float fyu[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     bool hx = false, hyl = false, hyu = false, hzu = false;
     if (cx && ix >= 1)
-      hx = iface<1, CM>(eos, Qs, gv, m, m_full, N, ix >= 2 ? c - 2 : c - 1, c - 1, c,
+      hx = iface<1, CM, NC>(eos, Qs, gv, m, m_full, N, ix >= 2 ? c - 2 : c - 1, c - 1, c,
                         ix + 1 < g.nx ? c + 1 : c, itau, itauf, itf, prx, fx, status);
     float fxu[7];
 #pragma unroll
-    for (int k = 0; k < 7; k++) fxu[k] = __shfl_down(fx[k], 1, 64);
+    for (int k = 0; k < NC; k++) fxu[k] = __shfl_down(fx[k], 1, 64);
     const bool hxu = __shfl_down(hx ? 1 : 0, 1, 64) != 0 && ix + 1 < g.nx;
     if (cx && iy >= 1)
-      hyl = iface<2, CM>(eos, Qs, gv, m, m_full, N, iy >= 2 ? c - 2 * sy : c - sy, c - sy, c,
+      hyl = iface<2, CM, NC>(eos, Qs, gv, m, m_full, N, iy >= 2 ? c - 2 * sy : c - sy, c - sy, c,
                          iy + 1 < g.ny ? c + sy : c, itau, itauf, itf, pry, fyl, status);
     if (cx && iy + 1 < g.ny)
-      hyu = iface<2, CM>(eos, Qs, gv, m, m_full, N, iy >= 1 ? c - sy : c, c, c + sy,
+      hyu = iface<2, CM, NC>(eos, Qs, gv, m, m_full, N, iy >= 1 ? c - sy : c, c, c + sy,
                          iy + 2 < g.ny ? c + 2 * sy : c + sy, itau, itauf, itf, pry, fyu,
                          status);
     if (iz == z0 && cx && iz >= 1)
-      hzl = iface<3, CM>(eos, Qs, gv, m, m_full, N, iz >= 2 ? c - 2 * sz : c - sz, c - sz, c,
+      hzl = iface<3, CM, NC>(eos, Qs, gv, m, m_full, N, iz >= 2 ? c - 2 * sz : c - sz, c - sz, c,
                          iz + 1 < g.nz ? c + sz : c, itau, itauf, itf, prz, fzl, status);
     if (cx && iz + 1 < g.nz)
-      hzu = iface<3, CM>(eos, Qs, gv, m, m_full, N, iz >= 1 ? c - sz : c, c, c + sz,
+      hzu = iface<3, CM, NC>(eos, Qs, gv, m, m_full, N, iz >= 1 ? c - sz : c, c, c + sz,
                          iz + 2 < g.nz ? c + 2 * sz : c + sz, itau, itauf, itf, prz, fzu,
                          status);
     if (upd) {
@@ -562,32 +572,32 @@ k_stage_rows(Grids G, DevEos eos, const double *__restrict__ Q, const double *__
       for (int k = 0; k < 7; k++) acc[k] = 0.;
       if (hx)
 #pragma unroll
-        for (int k = 0; k < 7; k++) acc[k] += (double)fx[k];
+        for (int k = 0; k < NC; k++) acc[k] += (double)fx[k];
       if (hxu)
 #pragma unroll
-        for (int k = 0; k < 7; k++) acc[k] -= (double)fxu[k];
+        for (int k = 0; k < NC; k++) acc[k] -= (double)fxu[k];
       if (hyl)
 #pragma unroll
-        for (int k = 0; k < 7; k++) acc[k] += (double)fyl[k];
+        for (int k = 0; k < NC; k++) acc[k] += (double)fyl[k];
       if (hyu)
 #pragma unroll
-        for (int k = 0; k < 7; k++) acc[k] -= (double)fyu[k];
+        for (int k = 0; k < NC; k++) acc[k] -= (double)fyu[k];
       if (hzl)
 #pragma unroll
-        for (int k = 0; k < 7; k++) acc[k] += (double)fzl[k];
+        for (int k = 0; k < NC; k++) acc[k] += (double)fzl[k];
       if (hzu)
 #pragma unroll
-        for (int k = 0; k < 7; k++) acc[k] -= (double)fzu[k];
-      finish_cell<MODE, CM>(eos, Q, Qs, gv, N, c, tau, dt, acc, out);
+        for (int k = 0; k < NC; k++) acc[k] -= (double)fzu[k];
+      finish_cell<MODE, CM, NC>(eos, Q, Qs, gv, N, c, tau, dt, acc, out);
     }
 #pragma unroll
-    for (int k = 0; k < 7; k++) fzl[k] = fzu[k];
+    for (int k = 0; k < NC; k++) fzl[k] = fzu[k];
     hzl = hzu;
   }
 }
 
 // ---------------------------------------------------------------- viscous
-template <int CM>
+template <int CM, int NC>
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_MX_VPRIM)
 k_vprim(long N, DevEos eos, const double *__restrict__ Qh, const float *__restrict__ gv,
         const double *__restrict__ tp, double dt, double *__restrict__ ph,
@@ -598,19 +608,24 @@ k_vprim(long N, DevEos eos, const double *__restrict__ Qh, const float *__restri
   double q[7];
   Prim r;
   const double ith = 1. / (tau - 0.5 * dt), it = 1. / tau;
+  double qs[NC];  // Qh is read once for both time levels
 #pragma unroll
-  for (int k = 0; k < 7; k++) q[k] = Qh[k * N + c] * ith;
+  for (int k = 0; k < NC; k++) qs[k] = Qh[k * N + c];
+#pragma unroll
+  for (int k = 0; k < 7; k++) q[k] = k < NC ? qs[k] * ith : 0.;
   c2p_mode<CM>(eos, q, r, guess_of<CM>(gv, c));
   ph[c] = r.e;
   ph[N + c] = r.p;
-  ph[2 * N + c] = r.nb;
-  ph[3 * N + c] = r.nq;
-  ph[4 * N + c] = r.ns;
+  if (NC == 7) {  // without charges n_b, n_q, n_s are zero and not stored
+    ph[2 * N + c] = r.nb;
+    ph[3 * N + c] = r.nq;
+    ph[4 * N + c] = r.ns;
+  }
   ph[5 * N + c] = r.vx;
   ph[6 * N + c] = r.vy;
   ph[7 * N + c] = r.vz;
 #pragma unroll
-  for (int k = 0; k < 7; k++) q[k] = Qh[k * N + c] * it;
+  for (int k = 0; k < 7; k++) q[k] = k < NC ? qs[k] * it : 0.;
   c2p_mode<CM>(eos, q, r, guess_of<CM>(gv, c));
   pt[c] = r.e;
   pt[N + c] = r.vx;
@@ -662,7 +677,7 @@ __device__ __forceinline__ void is_cross(const float *X, const float *Pm, const 
     }
 }
 
-template <int CM>
+template <int CM, int NC>
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_MX_VISC1)
 k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
         const double *__restrict__ Qn, const float *__restrict__ gv,
@@ -679,7 +694,7 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
   int ix, iy, iz;
   coords_of(g, cl, ix, iy, iz);
   const long N = g.P;
-  const double e = ph[c], nb = ph[2 * N + c];
+  const double e = ph[c], nb = NC == 7 ? ph[2 * N + c] : 0.;
   const double vx = ph[5 * N + c], vy = ph[6 * N + c], vz = ph[7 * N + c];
   if (e <= 0.) {
     const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -715,10 +730,10 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
     Prim p0, p1;
     const double itn = 1. / (tau - dt), it1 = 1. / tau;
 #pragma unroll
-    for (int k = 0; k < 7; k++) q[k] = Qprev[k * N + c] * itn;
+    for (int k = 0; k < 7; k++) q[k] = k < NC ? Qprev[k * N + c] * itn : 0.;
     c2p_mode<CM>(eos, q, p0, guess_of<CM>(gv, c));
 #pragma unroll
-    for (int k = 0; k < 7; k++) q[k] = Qn[k * N + c] * it1;
+    for (int k = 0; k < 7; k++) q[k] = k < NC ? Qn[k * N + c] * it1 : 0.;
     c2p_mode<CM>(eos, q, p1, guess_of<CM>(gv, c));
     const double idt = 1. / dt;
     const double ut0 = 1.0 / sqrt(1.0 - p0.vx * p0.vx - p0.vy * p0.vy - p0.vz * p0.vz);
@@ -1172,13 +1187,16 @@ k_ghost_b(Grids G, double *__restrict__ Q, double *__restrict__ Qh, float *__res
 // ---------------------------------------------------------- launchers
 static inline unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
 
-void launch_stage(int mode, int cm, const Grids &G, const DevEos &eos, const double *Q,
+void launch_stage(int mode, int cm, int nc, const Grids &G, const DevEos &eos, const double *Q,
                   const double *Qh, const float *gv, const double *m, int m_full,
                   const double *tp, double dt, double *out, int *status, hipStream_t st) {
   const dim3 gr((unsigned)G.nblk), b(256);
 #define VH_S(MM, CC)                                                                            \
-  if (mode == MM && cm == CC)                                                                   \
-    hipLaunchKernelGGL((k_stage<MM, CC>), gr, b, 0, st, G, eos, Q, Qh, gv, m, m_full, tp, dt,  \
+  if (mode == MM && cm == CC && nc == 7)                                                        \
+    hipLaunchKernelGGL((k_stage<MM, CC, 7>), gr, b, 0, st, G, eos, Q, Qh, gv, m, m_full, tp, dt, \
+                       out, status);                                                            \
+  if (mode == MM && cm == CC && nc == 4)                                                        \
+    hipLaunchKernelGGL((k_stage<MM, CC, 4>), gr, b, 0, st, G, eos, Q, Qh, gv, m, m_full, tp, dt, \
                        out, status);
   VH_S(0, VH_C2P_REFERENCE) VH_S(1, VH_C2P_REFERENCE) VH_S(0, VH_C2P_CLOSED)
   VH_S(1, VH_C2P_CLOSED) VH_S(0, VH_C2P_LEARNED) VH_S(1, VH_C2P_LEARNED)
@@ -1217,20 +1235,23 @@ std::vector<Tile> tile_candidates(const Grids &G) {
   return out;
 }
 
-void launch_stage_tile(int mode, int cm, const Grids &G, const DevEos &eos, const double *Q,
+void launch_stage_tile(int mode, int cm, int nc, const Grids &G, const DevEos &eos, const double *Q,
                        const double *Qh, const float *gv, const double *m, int m_full,
                        const double *tp, double dt, double *out, int *status, const Tile &tl,
                        hipStream_t st) {
   if (tl.kind != STAGE_ROWS) {
-    launch_stage(mode, cm, G, eos, Q, Qh, gv, m, m_full, tp, dt, out, status, st);
+    launch_stage(mode, cm, nc, G, eos, Q, Qh, gv, m, m_full, tp, dt, out, status, st);
     return;
   }
   const long ntz = (G.maxnz + tl.CZ - 1) / tl.CZ;
   const long waves = (long)G.nseg * ntz;
   const dim3 gr((unsigned)((waves + 3) / 4)), b(256);
 #define VH_S(MM, CC)                                                                          \
-  if (mode == MM && cm == CC)                                                                 \
-    hipLaunchKernelGGL((k_stage_rows<MM, CC>), gr, b, 0, st, G, eos, Q, Qh, gv, m, m_full,   \
+  if (mode == MM && cm == CC && nc == 7)                                                      \
+    hipLaunchKernelGGL((k_stage_rows<MM, CC, 7>), gr, b, 0, st, G, eos, Q, Qh, gv, m, m_full, \
+                       tp, dt, out, status, tl.CZ);                                           \
+  if (mode == MM && cm == CC && nc == 4)                                                      \
+    hipLaunchKernelGGL((k_stage_rows<MM, CC, 4>), gr, b, 0, st, G, eos, Q, Qh, gv, m, m_full, \
                        tp, dt, out, status, tl.CZ);
   VH_S(0, VH_C2P_REFERENCE) VH_S(1, VH_C2P_REFERENCE) VH_S(0, VH_C2P_CLOSED)
   VH_S(1, VH_C2P_CLOSED) VH_S(0, VH_C2P_LEARNED) VH_S(1, VH_C2P_LEARNED)
@@ -1238,25 +1259,31 @@ void launch_stage_tile(int mode, int cm, const Grids &G, const DevEos &eos, cons
 #undef VH_S
 }
 
-void launch_vprim(int cm, long N, const DevEos &eos, const double *Qh, const float *gv,
+void launch_vprim(int cm, int nc, long N, const DevEos &eos, const double *Qh, const float *gv,
                   const double *tp, double dt, double *ph, double *pt, hipStream_t st) {
   const dim3 gr(blocks(N)), b(256);
-#define VH_V(CC) \
-  if (cm == CC) hipLaunchKernelGGL(k_vprim<CC>, gr, b, 0, st, N, eos, Qh, gv, tp, dt, ph, pt);
+#define VH_V(CC)                                                                              \
+  if (cm == CC && nc == 7)                                                                    \
+    hipLaunchKernelGGL((k_vprim<CC, 7>), gr, b, 0, st, N, eos, Qh, gv, tp, dt, ph, pt);        \
+  if (cm == CC && nc == 4)                                                                    \
+    hipLaunchKernelGGL((k_vprim<CC, 4>), gr, b, 0, st, N, eos, Qh, gv, tp, dt, ph, pt);
   VH_V(VH_C2P_REFERENCE) VH_V(VH_C2P_CLOSED) VH_V(VH_C2P_LEARNED) VH_V(VH_C2P_NEWTON)
 #undef VH_V
 }
 
-void launch_visc1(int cm, const Grids &G, const DevEos &eos, const DevTrc &trc,
+void launch_visc1(int cm, int nc, const Grids &G, const DevEos &eos, const DevTrc &trc,
                   const double *Qprev, const double *Qn, const float *gv, const double *ph,
                   const double *pt, const double *m, int m_full, const float *pi,
                   const float *Pi, const double *tp, double dt, float4 *rec, int *status,
                   hipStream_t st) {
   const dim3 gr((unsigned)G.nblk), b(256);
 #define VH_V(CC)                                                                               \
-  if (cm == CC)                                                                                \
-    hipLaunchKernelGGL(k_visc1<CC>, gr, b, 0, st, G, eos, trc, Qprev, Qn, gv, ph, pt, m, m_full, \
-                       pi, Pi, tp, dt, rec, status);
+  if (cm == CC && nc == 7)                                                                     \
+    hipLaunchKernelGGL((k_visc1<CC, 7>), gr, b, 0, st, G, eos, trc, Qprev, Qn, gv, ph, pt, m,    \
+                       m_full, pi, Pi, tp, dt, rec, status);                                   \
+  if (cm == CC && nc == 4)                                                                     \
+    hipLaunchKernelGGL((k_visc1<CC, 4>), gr, b, 0, st, G, eos, trc, Qprev, Qn, gv, ph, pt, m,    \
+                       m_full, pi, Pi, tp, dt, rec, status);
   VH_V(VH_C2P_REFERENCE) VH_V(VH_C2P_CLOSED) VH_V(VH_C2P_LEARNED) VH_V(VH_C2P_NEWTON)
 #undef VH_V
 }

@@ -114,7 +114,7 @@ k_c2p_learned(DevEos eos, const Packed *__restrict__ P, double *__restrict__ Q, 
 
 // per-cell guess of |v| for the hydro step: Qs (SoA 7 x N) / (tau + toff)
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_GUESS)
-k_guess(const double *__restrict__ Qs, long N, const double *__restrict__ tp, double toff,
+k_guess(const double *__restrict__ Qs, long N, int nc, const double *__restrict__ tp, double toff,
         const Packed *__restrict__ P, float *__restrict__ gv) {
   __shared__ Packed w;
   load_packed(w, P);
@@ -123,11 +123,13 @@ k_guess(const double *__restrict__ Qs, long N, const double *__restrict__ tp, do
   // the next iteration's state is loaded before this one's MLP runs
   const long stride = (long)gridDim.x * blockDim.x;
   long base = (long)blockIdx.x * blockDim.x;
-  double qn[7];
+  // nc = 4: no charges (the N^tau components are zero and not read)
+  double qn[7] = {0., 0., 0., 0., 0., 0., 0.};
   {
     const long j = base + threadIdx.x < N ? base + threadIdx.x : N - 1;
 #pragma unroll
-    for (int k = 0; k < 7; k++) qn[k] = Qs[k * N + j];
+    for (int k = 0; k < 7; k++)
+      if (k < nc) qn[k] = Qs[k * N + j];
   }
   for (; base < N; base += stride) {
     const long c = base + threadIdx.x;
@@ -138,7 +140,8 @@ k_guess(const double *__restrict__ Qs, long N, const double *__restrict__ tp, do
     if (base + stride < N) {
       const long j = cn < N ? cn : N - 1;
 #pragma unroll
-      for (int k = 0; k < 7; k++) qn[k] = Qs[k * N + j];
+      for (int k = 0; k < 7; k++)
+        if (k < nc) qn[k] = Qs[k * N + j];
     }
     float f[5];
     features(q, f);
@@ -415,9 +418,9 @@ void pack(const float *P, Packed &out) {
   out.b3[1] = P[OFF_B3 + 1];
 }
 
-void launch_guess(const double *Qs, long N, const double *tp, double toff, const Packed *P,
+void launch_guess(const double *Qs, long N, int nc, const double *tp, double toff, const Packed *P,
                   float *gv, hipStream_t st) {
-  hipLaunchKernelGGL(k_guess, dim3(wave_grid(N)), dim3(256), 0, st, Qs, N, tp, toff, P, gv);
+  hipLaunchKernelGGL(k_guess, dim3(wave_grid(N)), dim3(256), 0, st, Qs, N, nc, tp, toff, P, gv);
 }
 
 }  // namespace ml

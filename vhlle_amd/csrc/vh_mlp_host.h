@@ -18,7 +18,7 @@ inline double ideal_eos_cnst() {
 void pack(const float *params, Packed &out);
 
 // per-cell |v| guesses gv[N] of Qs (SoA 7 x N) / (*tp + toff)
-void launch_guess(const double *Qs, long N, const double *tp, double toff, const Packed *P,
+void launch_guess(const double *Qs, long N, int nc, const double *tp, double toff, const Packed *P,
                   float *gv, hipStream_t st);
 
 }  // namespace ml
