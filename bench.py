@@ -62,7 +62,7 @@ E_CRIT = 0.5
 
 # Algorithmic (compulsory) HBM bytes per cell of each kernel: every input
 # array read once and every output written once at its stored width (FP64 8 B,
-# FP32 4 B).  Derivations in DESIGN.md, "Kernels and their rooflines".
+# FP32 4 B).
 ALG_BYTES = {
     "fp64": {
         "faces_x_pred": 112, "faces_y_pred": 112, "faces_z_pred": 112,
@@ -73,12 +73,14 @@ ALG_BYTES = {
         "visc_advect": (5 + 22 + 11) * 8 + (22 + 1 + 4) * 8,
         "visc_flux_update": (3 + 11 + 4 + 7) * 8 + 7 * 8,
     },
+    # mixed kernels on a state without charges: 4 conserved components (the
+    # workload's n_b = n_q = n_s = 0), pi and Pi in FP32
     "mixed": {
-        "stage_pred": 56 + 56, "stage_corr": 56 + 56 + 56, "c2p_guess": 56 + 4,
-        "visc_prim": 56 + 96,
-        "visc_relax": 64 + 32 + 56 + 56 + 44 + 88,
+        "stage_pred": 32 + 32, "stage_corr": 32 + 32 + 32, "c2p_guess": 32 + 4,
+        "visc_prim": 32 + 40 + 32,
+        "visc_relax": 32 + 32 + 32 + 32 + 44 + 88,
         "visc_advect": 40 + 88 + 20 + 80 + 8 + 8 + 16,
-        "visc_flux_update": 32 + 44 + 16 + 56 + 56,
+        "visc_flux_update": 32 + 44 + 16 + 32 + 32,
     },
 }
 # HBM bytes per launch measured by rocprofv3 (profiles/traffic_viscous_200x200x100.md):
@@ -106,9 +108,10 @@ def traffic_of(prec, kernel, stage, c2p):
     return None
 
 
-# whole-step compulsory state traffic per cell-update:
-# 2*(7*s_Q + 11*s_pi) -- FP64 288 B, mixed (pi in FP32) 200 B
-STEP_BYTES = {"fp64": 288.0, "mixed": 200.0}
+# whole-step compulsory state traffic per cell-update: 2*(n_Q*s_Q + 11*s_pi)
+# at the stored widths -- FP64 288 B (7 components); mixed 152 B (the 4
+# components a state without charges carries, pi and Pi in FP32)
+STEP_BYTES = {"fp64": 288.0, "mixed": 152.0}
 
 VARIANTS = {
     "fp64": ("fp64", "reference"),
@@ -380,7 +383,7 @@ def main():
         C2P_PAIRS, C2P_EPOCHS = 1 << 18, 2
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding of the hydro "
-              "grid is designed (DESIGN.md) but not built.", file=sys.stderr)
+              "grid (x-slabs with halo exchanges) is not built.", file=sys.stderr)
         sys.exit(2)
     from vhlle_amd import abi, setups
     cfg = setups.VISCOUS_3D

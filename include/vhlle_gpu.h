@@ -8,7 +8,7 @@
  * that class API: plain C, POD structs, host pointers and sizes, integer
  * status codes.  The C++ facade (vhlle_amd/host/vhlle.h: Fluid, Hydro, Cell, EoS,
  * TransportCoeff) is written on top of it, and any other host language can
- * bind it directly (see INTEGRATION.md).
+ * bind it directly.
  *
  * State layout crossing the boundary: structure-of-arrays, component-major,
  * arr[comp*N + cell], cell = ix + nx*(iy + ny*iz) -- the cell order of

@@ -1,7 +1,7 @@
 """Parity of the FP64 HIP path with the CPU oracle, through the C-ABI.
 
 Bar: for every state field f, max|gpu - oracle| <= 1e-10 * max|oracle| (the
-north-star tolerance for the plain FP64 step); con2prim/prim2con are expected
+tolerance of the plain FP64 step); con2prim/prim2con are expected
 bit-identical because sqrt and division are correctly rounded on both sides.
 """
 import numpy as np

@@ -188,8 +188,9 @@ int enqueue_step_mixed(vh_ctx *c, double dt) {
     rc |= timed(c, "update_m_apply", [&] { vh::launch_update_m(1, g, c->m, c->dm, c->tp, dt, st); });
   }
   const bool learned = cm == VH_C2P_LEARNED;
-  // the MLP guess of every cell of the stage's input state (Q/tau_n, then
-  // Qh/tau_n: both stages reconstruct on tau_n-scaled states)
+  // the MLP guess of every cell of the step's input state Q/tau_n: the Newton
+  // start of the predictor, corrector and viscous con2prim of the step (the
+  // states they solve differ from it by at most one step's change)
   if (learned)
     rc |= timed(c, "c2p_guess", [&] { vh::ml::launch_guess(Qo, g.N, nc, c->tp, 0., c->mlp, c->gv, st); });
   auto stage = [&](int mode, double *out) {
@@ -197,8 +198,6 @@ int enqueue_step_mixed(vh_ctx *c, double dt) {
                               dt, out, c->status, c->tile, st);
   };
   rc |= timed(c, "stage_pred", [&] { stage(0, c->Qh); });
-  if (learned)
-    rc |= timed(c, "c2p_guess", [&] { vh::ml::launch_guess(c->Qh, g.N, nc, c->tp, 0., c->mlp, c->gv, st); });
   rc |= timed(c, "stage_corr", [&] { stage(1, Qn); });
   rc |= timed(c, "advance_tau", [&] { vh::launch_advance_tau(c->tp, dt, st); });
   ghost_fill(c, 0, Qn, st, rc);
@@ -408,6 +407,7 @@ int build_grid_tables(vh_ctx *c) {
   G.nseg = (int)sev.size();
   G.maxnz = maxnz;
   G.g0 = c->evh[0];
+  G.sy = VH_STRIP_ROWS;
   return VH_OK;
 }
 

@@ -383,6 +383,10 @@ __device__ __forceinline__ long xcd_block() {
 #ifndef VH_W_GUESS
 #define VH_W_GUESS 0
 #endif
+// rows per y-strip of the mixed per-cell kernels' cell order (0: x-fastest)
+#ifndef VH_STRIP_ROWS
+#define VH_STRIP_ROWS 8
+#endif
 
 // Grid geometry.  N: cells of this grid; base: index of its cell 0 in the
 // state arrays; P: the arrays' component pitch (cells of all grids stored

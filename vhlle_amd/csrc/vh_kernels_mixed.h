@@ -21,6 +21,9 @@ struct Grids {
   long P;
   int nev, nblk, nseg, maxnz;
   Geom g0;  // grid 0 by value (the kernels use it when nev == 1)
+  // cell order of the per-cell kernels: y-strips of sy rows swept over all
+  // eta, XCD-contiguous workgroups (0: plain x-fastest order)
+  int sy;
 };
 
 // gv: per-cell con2prim guesses (used by VH_C2P_LEARNED only, may be null otherwise)

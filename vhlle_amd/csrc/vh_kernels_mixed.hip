@@ -48,6 +48,40 @@ __device__ __forceinline__ Geom block_grid(const Grids &G, long bk, long &cl) {
   cl = (bk - G.bfirst[gi]) * blockDim.x + threadIdx.x;
   return G.ev[gi];
 }
+// The work item of a per-cell thread: grid g, its cell c (array index) and
+// coordinates; false past the grid's last cell.  With G.sy > 0 workgroups
+// are XCD-contiguous and a grid's cells are visited in y-strips of sy rows,
+// each swept over all eta in contiguous chunks of sy*nx cells: the eta- and
+// y-neighbours a workgroup gathers were then fetched into the same L2 by the
+// workgroups just before it (near-compulsory HBM traffic).  A bijection of
+// the grid's cells, so results do not depend on it.
+__device__ __forceinline__ bool work_cell(const Grids &G, Geom &g, long &c, int &ix, int &iy,
+                                          int &iz) {
+  long cl;
+  g = block_grid(G, G.sy > 0 ? xcd_block() : (long)blockIdx.x, cl);
+  if (cl >= g.N) return false;
+  if (G.sy > 0 && g.ny > G.sy) {
+    const unsigned nx = g.nx, ny = g.ny, nz = g.nz, sy = G.sy;
+    const unsigned nfull = ny / sy, plane = sy * nx, strip = plane * nz;
+    const unsigned u = (unsigned)cl;
+    unsigned s = u / strip, r = u - s * strip, pl = plane;
+    if (s >= nfull) {  // the last, shorter strip
+      s = nfull;
+      r = u - nfull * strip;
+      pl = (ny - nfull * sy) * nx;
+    }
+    const unsigned z = r / pl, f = r - z * pl, yy = f / nx;
+    iz = (int)z;
+    iy = (int)(s * sy + yy);
+    ix = (int)(f - yy * nx);
+    c = g.idx(ix, iy, iz);
+  } else {
+    coords_of(g, cl, ix, iy, iz);
+    c = g.base + cl;
+  }
+  return true;
+}
+
 // grid of row segment sgl (over all grids) and the segment's index in it
 __device__ __forceinline__ Geom seg_grid(const Grids &G, long sgl, long &seg) {
   if (G.nev == 1) {
@@ -466,12 +500,10 @@ k_stage(Grids G, DevEos eos, const double *__restrict__ Q, const double *__restr
         const double *__restrict__ tp, double dt, double *__restrict__ out,
         int *__restrict__ status) {
   const double tau = *tp;
-  long cl;
-  const Geom g = block_grid(G, blockIdx.x, cl);
-  if (cl >= g.N) return;
-  const long c = g.base + cl;
+  Geom g;
+  long c;
   int ix, iy, iz;
-  coords_of(g, cl, ix, iy, iz);
+  if (!work_cell(G, g, c, ix, iy, iz)) return;
   const long N = g.P;
   const double *Qs = MODE == 0 ? Q : Qh;
   double acc[7];
@@ -687,12 +719,10 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
         const double *__restrict__ tp, double dt, float4 *__restrict__ rec,
         int *__restrict__ status) {
   const double tau = *tp;
-  long cl;
-  const Geom g = block_grid(G, blockIdx.x, cl);
-  if (cl >= g.N) return;
-  const long c = g.base + cl;
+  Geom g;
+  long c;
   int ix, iy, iz;
-  coords_of(g, cl, ix, iy, iz);
+  if (!work_cell(G, g, c, ix, iy, iz)) return;
   const long N = g.P;
   const double e = ph[c], nb = NC == 7 ? ph[2 * N + c] : 0.;
   const double vx = ph[5 * N + c], vy = ph[6 * N + c], vz = ph[7 * N + c];
@@ -992,12 +1022,11 @@ k_visc2(Grids G, const double *__restrict__ ph, const float4 *__restrict__ rec,
         float *__restrict__ pi, float *__restrict__ piH,
         float *__restrict__ Pi, float *__restrict__ PiH, double *__restrict__ vcc,
         float *__restrict__ vf, const double *__restrict__ tp, double dt) {
-  long cl;
-  const Geom g = block_grid(G, blockIdx.x, cl);
-  if (cl >= g.N) return;
+  Geom g;
+  long c;
   int ix, iy, iz;
-  coords_of(g, cl, ix, iy, iz);
-  visc2_cell(g, g.base + cl, ix, iy, iz, ph, rec, pi, piH, Pi, PiH, vcc, vf, *tp, dt);
+  if (!work_cell(G, g, c, ix, iy, iz)) return;
+  visc2_cell(g, c, ix, iy, iz, ph, rec, pi, piH, Pi, PiH, vcc, vf, *tp, dt);
 }
 
 // viscous interface flux in FP32 (Hydro::visc_flux, src/hdo.cpp:966-1014)
@@ -1033,12 +1062,10 @@ k_visc3(Grids G, const double *__restrict__ ph, const float *__restrict__ piH,
         const double *__restrict__ m, int m_full, const double *__restrict__ tp, double dt,
         double *__restrict__ Q) {
   const double tau = *tp;
-  long cl;
-  const Geom g = block_grid(G, blockIdx.x, cl);
-  if (cl >= g.N) return;
-  const long c = g.base + cl;
+  Geom g;
+  long c;
   int ix, iy, iz;
-  coords_of(g, cl, ix, iy, iz);
+  if (!work_cell(G, g, c, ix, iy, iz)) return;
   const long N = g.P;
   float acc[4];
 #pragma unroll
