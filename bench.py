@@ -89,20 +89,27 @@ ALG_BYTES = {
 # shape, con2prim); None where not profiled.
 KIB = 1024
 TRAFFIC = {
-    ("fp64", "visc_relax", None, "reference"): (2 * 610117 + 687500) * KIB,
-    ("fp64", "visc_advect", None, "reference"): (2 * 736369 + 843750) * KIB,
-    ("mixed", "stage_pred", "cells", "closed"): (2 * 802101 + 218750) * KIB,
-    ("mixed", "stage_corr", "cells", "closed"): (2 * 932879 + 218750) * KIB,
-    ("mixed", "stage_pred", "rows", "learned"): (2 * 756584 + 224744) * KIB,
-    ("mixed", "stage_corr", "rows", "learned"): (2 * 964323 + 225744) * KIB,
-    ("mixed", "visc_advect", None, None): (2 * 1946175 + 437500) * KIB,
-    ("mixed", "visc_relax", None, None): (2 * 596154 + 375000) * KIB,
-    ("mixed", "visc_flux_update", None, None): (2 * 570616 + 125000) * KIB,
+    ("fp64", "visc_relax", None, "reference"): (2 * 608380 + 687500) * KIB,
+    ("fp64", "visc_advect", None, "reference"): (2 * 736509 + 843750) * KIB,
+    ("fp64", "visc_flux_update", None, "reference"): (2 * 764352 + 218771) * KIB,
+    ("mixed", "stage_pred", "cells", "closed"): (2 * 93408 + 125000) * KIB,
+    ("mixed", "stage_corr", "cells", "closed"): (2 * 155876 + 125000) * KIB,
+    ("mixed", "stage_pred", "cells", "learned"): (2 * 101221 + 125000) * KIB,
+    ("mixed", "stage_corr", "cells", "learned"): (2 * 165878 + 125000) * KIB,
+    ("mixed", "stage_pred", "rows", None): (2 * 390687 + 126686) * KIB,
+    ("mixed", "visc_relax", None, "closed"): (2 * 352668 + 375000) * KIB,
+    ("mixed", "visc_relax", None, "learned"): (2 * 360365 + 375000) * KIB,
+    ("mixed", "visc_advect", None, None): (2 * 330523 + 437500) * KIB,
+    ("mixed", "visc_flux_update", None, None): (2 * 263244 + 125000) * KIB,
+    ("mixed", "visc_prim", None, "closed"): (2 * 62530 + 281250) * KIB,
+    ("mixed", "visc_prim", None, "learned"): (2 * 70366 + 281250) * KIB,
+    ("mixed", "c2p_guess", None, "learned"): (2 * 62621 + 15625) * KIB,
 }
 
 
 def traffic_of(prec, kernel, stage, c2p):
-    for key in ((prec, kernel, stage, c2p), (prec, kernel, None, c2p), (prec, kernel, None, None)):
+    for key in ((prec, kernel, stage, c2p), (prec, kernel, stage, None), (prec, kernel, None, c2p),
+                (prec, kernel, None, None)):
         if key in TRAFFIC:
             return TRAFFIC[key]
     return None
