@@ -39,7 +39,11 @@ enum {
   VH_ST_NAN_FLUX = 1,     /* hlle_flux f_nb NaN -> exit(1)  (src/hdo.cpp:285-298) */
   VH_ST_BIG_E = 2,        /* e > 1e10 -> exit(0)            (src/hdo.cpp:118-130) */
   VH_ST_NAN_SOURCE = 4,   /* source_step k_nb NaN -> exit(1)(src/hdo.cpp:345-351) */
-  VH_ST_NAN_PI = 8        /* NSquant pi inf/nan -> exit(1)  (src/hdo.cpp:643-651) */
+  VH_ST_NAN_PI = 8,       /* NSquant pi inf/nan -> exit(1)  (src/hdo.cpp:643-651) */
+  VH_ST_EOS_RANGE = 16    /* tabulated EoS evaluated outside its table: GSL's
+                             domain error aborts the reference (gsl_spline_eval,
+                             src/eos.cpp:89-111); here the end interval's cubic
+                             is used and this bit raised */
 };
 
 /* precision modes of vh_create */
@@ -68,11 +72,24 @@ typedef struct {
 
 /* EoS descriptor replacing the virtual EoS (src/eos.h:21-38).
  * kind 0: ideal massless gas p=e/3, T=(e/cnst)^(1/4), mu=0, the SIMPLE branch
- * of EoSs (src/eos.cpp:87-111). */
-enum { VH_EOS_IDEAL = 0 };
+ * of EoSs (src/eos.cpp:87-111).
+ * kind 1: the TABLE branch of EoSs (src/eos.cpp:27-120): n rows of e (strictly
+ * increasing, GeV/fm^3), p (negative values clamped to 0 as the reference
+ * does), T (GeV) and mu (GeV, may be NULL = 0); natural cubic splines of p(e),
+ * T(e), mu(e) (gsl_interp_cspline); cs2(e) = dpe(e) = (p(1.1e) - p(e))/(0.1e);
+ * mu_b = mu_q = mu_s = 0 in eos().  The arrays are copied (host memory). */
+enum { VH_EOS_IDEAL = 0, VH_EOS_TABLE = 1 };
 typedef struct {
   int kind;
+  int n;                          /* table rows (kind 1), >= 3 */
+  const double *e, *p, *T, *mu;   /* table columns (kind 1) */
 } vh_eos_t;
+/* The natural cubic splines of a table EoS (gsl_spline_init of EoSs,
+ * src/eos.cpp:59-69), computed on the host (no GPU needed) into buf[13 n]:
+ * the n nodes e_i, then for p, T and mu in turn the arrays y_i, b_i, c_i, d_i
+ * (n each; y = y_i + dx (b_i + dx (c_i + dx d_i)) on [e_i, e_i+1], dx = e - e_i).
+ * Host code (the C++ facade's EoSs) evaluates the same splines as the GPU. */
+int vh_eos_table(const vh_eos_t *eos, double *buf);
 
 /* TransportCoeff (src/trancoeff.h:8-11, ctor :14-20) as a POD, plus the
  * T-peaked bulk parametrisation zetaSparam = 5 (this build):
@@ -230,6 +247,8 @@ typedef struct {
   double amp[4];
   double augment;             /* fraction in [0, 1] */
   double vmax;
+  const vh_eos_t *eos;        /* EoS of the runs and labels (NULL: ideal gas);
+                                 augmented e are drawn inside a table's range */
 } vh_harvest_cfg_t;
 int vh_mlp_harvest(const vh_harvest_cfg_t *cfg, long n, double *Q, double *prim);
 /* ---- hot-spot initial states (event-by-event batches) ----

@@ -486,18 +486,25 @@ int or_cornelius(const double *h, long n, const double dx[4], double E0, double 
 // Fluid::outputSurface's surface elements (src/fld.cpp:544-830), SoA inputs
 long or_freezeout(const or_fo_grid_t *g, const double *Q, const double *Qlow, const double *pi,
                   const double *Pi, double tau, double dt, double ecrit, int extend,
-                  double *rows, long max_rows) {
+                  double *rows, long max_rows, const or_eos_t *eos) {
   const int nx = g->nx, ny = g->ny, nz = g->nz;
   const long N = (long)nx * ny * nz;
   const int ncol = extend ? 29 : 27;
   const double dxs[4] = {dt, g->dx, g->dy, g->dz};
   auto cell = [&](int ix, int iy, int iz) { return ix + (long)nx * (iy + (long)ny * iz); };
-  auto energy = [&](const double *src, long c, double t) {
-    double q[7], out[8];
-    for (int k = 0; k < 7; k++) q[k] = src[k * N + c] / t;
-    or_c2p(0, q, 1, out, nullptr);
-    return out[0];
-  };
+  // e of every cell at both levels (one con2prim pass per level)
+  std::vector<double> elev[2];
+  for (int lv = 0; lv < 2; lv++) {
+    const double *src = lv ? Q : Qlow;
+    const double t = lv ? tau : tau - dt;
+    std::vector<double> q(7 * N), out(8 * N);
+    for (long c = 0; c < N; c++)
+      for (int k = 0; k < 7; k++) q[7 * c + k] = src[k * N + c] / t;
+    or_c2p(eos, q.data(), N, out.data(), nullptr);
+    elev[lv].resize(N);
+    for (long c = 0; c < N; c++) elev[lv][c] = out[8 * c];
+  }
+  auto energy = [&](int level, long c) { return elev[level][c]; };
   long nrow = 0;
   for (int ix = 2; ix < nx - 2; ix++)
     for (int iy = 2; iy < ny - 2; iy++)
@@ -507,8 +514,8 @@ long or_freezeout(const or_fo_grid_t *g, const double *Q, const double *Qlow, co
           for (int jy = 0; jy < 2; jy++)
             for (int jz = 0; jz < 2; jz++) {
               const long c = cell(ix + jx, iy + jy, iz + jz);
-              h[1][jx][jy][jz] = energy(Q, c, tau);
-              h[0][jx][jy][jz] = energy(Qlow, c, tau - dt);
+              h[1][jx][jy][jz] = energy(1, c);
+              h[0][jx][jy][jz] = energy(0, c);
             }
         double cen[10][4], nrm[10][4];
         int err = 0;
@@ -536,10 +543,10 @@ long or_freezeout(const or_fo_grid_t *g, const double *Q, const double *Qlow, co
                              wX[jx] * wY[jy] * wZ[jz];
           for (int i = 0; i < 7; i++) QC[i] = QC[i] / (tau + c4[0]);
           double pr[8], eo[4];
-          or_c2p(0, QC, 1, pr, nullptr);
+          or_c2p(eos, QC, 1, pr, nullptr);
           const double eC = pr[0], nbC = pr[2];
           double vxC = pr[5], vyC = pr[6], vzC = pr[7];
-          or_eos(0, &eC, 1, eo);
+          or_eos(eos, &eC, 1, eo);
           const double TC = eo[1];
           double piC[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, PiC = 0.;
           for (int jx = 0; jx < 2; jx++)

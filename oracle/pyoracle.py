@@ -66,6 +66,23 @@ def make_trc(etaS=0.0, zetaS=0.0, etaSparam=0, zetaSparam=0, **kw):
     return Trc(**d)
 
 
+class OrEos(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("n", ctypes.c_int)] + [
+        (k, ctypes.POINTER(ctypes.c_double)) for k in ("e", "p", "T", "mu")]
+
+
+def _eos(table):
+    """(pointer to an or_eos_t or None, arrays kept alive) of a table (any
+    object with arrays e, p, T and optionally mu) or None (the ideal gas)."""
+    if table is None:
+        return None, []
+    arrs = [np.ascontiguousarray(getattr(table, k), dtype=np.float64) for k in ("e", "p", "T")]
+    mu = getattr(table, "mu", None)
+    arrs.append(None if mu is None else np.ascontiguousarray(mu, dtype=np.float64))
+    d = OrEos(1, len(arrs[0]), *[None if a is None else _p(a) for a in arrs])
+    return ctypes.byref(d), [d] + arrs
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -76,7 +93,7 @@ def lib():
         dp = ctypes.POINTER(ctypes.c_double)
         L.or_create.restype = ctypes.c_void_p
         L.or_create.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 6 + [
-            ctypes.c_int, ctypes.POINTER(Trc)]
+            ctypes.POINTER(OrEos), ctypes.POINTER(Trc)]
         L.or_destroy.argtypes = [ctypes.c_void_p]
         L.or_ncell.restype = ctypes.c_long
         L.or_ncell.argtypes = [ctypes.c_void_p]
@@ -87,16 +104,18 @@ def lib():
         L.or_run.argtypes = [ctypes.c_void_p, dp, ctypes.c_double, ctypes.c_int]
         L.or_check_dtau.restype = ctypes.c_int
         L.or_check_dtau.argtypes = [ctypes.c_void_p, ctypes.c_double]
-        L.or_c2p.argtypes = [ctypes.c_int, dp, ctypes.c_long, dp,
+        L.or_c2p.argtypes = [ctypes.POINTER(OrEos), dp, ctypes.c_long, dp,
                              ctypes.POINTER(ctypes.c_int)]
         L.or_p2c.argtypes = [dp, ctypes.c_long, dp]
-        L.or_eos.argtypes = [ctypes.c_int, dp, ctypes.c_long, dp]
-        L.or_trcoeff.argtypes = [ctypes.POINTER(Trc), ctypes.c_int, dp, ctypes.c_long, dp]
+        L.or_eos.argtypes = [ctypes.POINTER(OrEos), dp, ctypes.c_long, dp]
+        L.or_eos_spline.argtypes = [ctypes.POINTER(OrEos), ctypes.c_int, dp, ctypes.c_long, dp]
+        L.or_trcoeff.argtypes = [ctypes.POINTER(Trc), ctypes.POINTER(OrEos), dp, ctypes.c_long,
+                                 dp]
         L.or_cornelius.argtypes = [dp, ctypes.c_long, dp, ctypes.c_double, dp, dp,
                                    ctypes.POINTER(ctypes.c_int)]
         L.or_freezeout.argtypes = [ctypes.POINTER(FoGrid), dp, dp, dp, dp, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, dp,
-                                   ctypes.c_long]
+                                   ctypes.c_long, ctypes.POINTER(OrEos)]
         L.or_freezeout.restype = ctypes.c_long
         _LIB = L
     return _LIB
@@ -106,13 +125,14 @@ def _p(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
-def c2p(Q):
+def c2p(Q, table=None):
     """transformPV over rows of Q (n x 7). Returns (prim n x 8, iters n, Q_after)."""
     Q = np.ascontiguousarray(Q, dtype=np.float64).copy()
     n = Q.shape[0]
     out = np.zeros((n, 8))
     it = np.zeros(n, dtype=np.int32)
-    lib().or_c2p(0, _p(Q), n, _p(out), it.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    d, keep = _eos(table)
+    lib().or_c2p(d, _p(Q), n, _p(out), it.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
     return out, it, Q
 
 
@@ -123,27 +143,40 @@ def p2c(prim):
     return Q
 
 
-def eos(e):
+def eos(e, table=None):
+    """n x 4 {p, T, s, cs2} of the ideal gas or of a table"""
     e = np.ascontiguousarray(e, dtype=np.float64)
     out = np.zeros((e.shape[0], 4))
-    lib().or_eos(0, _p(e), e.shape[0], _p(out))
+    d, keep = _eos(table)
+    lib().or_eos(d, _p(e), e.shape[0], _p(out))
     return out
 
 
-def trcoeff(trc, inp):
+def eos_spline(table, f, v):
+    """the table's natural cubic spline of f (0 p, 1 T, 2 mu) at v"""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.zeros(v.shape[0])
+    d, keep = _eos(table)
+    lib().or_eos_spline(d, f, _p(v), v.shape[0], _p(out))
+    return out
+
+
+def trcoeff(trc, inp, table=None):
     inp = np.ascontiguousarray(inp, dtype=np.float64)
     out = np.zeros((inp.shape[0], 4))
-    lib().or_trcoeff(ctypes.byref(trc), 0, _p(inp), inp.shape[0], _p(out))
+    d, keep = _eos(table)
+    lib().or_trcoeff(ctypes.byref(trc), d, _p(inp), inp.shape[0], _p(out))
     return out
 
 
 class Solver:
     """Oracle hydro grid + Hydro::performStep driver (CPU, serial)."""
 
-    def __init__(self, nx, ny, nz, xlim, ylim, zlim, trc=None):
+    def __init__(self, nx, ny, nz, xlim, ylim, zlim, trc=None, table=None):
         self.trc = trc if trc is not None else make_trc()
+        d, keep = _eos(table)
         self.h = lib().or_create(nx, ny, nz, xlim[0], xlim[1], ylim[0], ylim[1],
-                                 zlim[0], zlim[1], 0, ctypes.byref(self.trc))
+                                 zlim[0], zlim[1], d, ctypes.byref(self.trc))
         if not self.h:
             raise ValueError("bad grid")
         self.shape = (nx, ny, nz)
@@ -202,15 +235,16 @@ def cornelius(h, dx, E0):
 
 
 def freezeout(shape, spacing, mins, Q, Qlow, pi, Pi, tau, dt, ecrit, extend=False,
-              max_rows=1000000):
+              max_rows=1000000, table=None):
     """Fluid::outputSurface's element records from SoA state arrays."""
     g = FoGrid(shape[0], shape[1], shape[2], spacing[0], spacing[1], spacing[2], mins[0],
                mins[1], mins[2])
     ncol = 29 if extend else 27
     rows = np.zeros((max_rows, ncol))
     arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (Q, Qlow, pi, Pi)]
+    d, keep = _eos(table)
     n = lib().or_freezeout(ctypes.byref(g), *[_p(a) for a in arrs], tau, dt, ecrit,
-                           int(extend), _p(rows), max_rows)
+                           int(extend), _p(rows), max_rows, d)
     if n < 0:
         raise RuntimeError("or_freezeout: construction failed")
     return rows[:min(n, max_rows)]

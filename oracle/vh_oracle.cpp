@@ -34,19 +34,103 @@ inline double minmod(double a, double b) {
 inline double sgn(double x) { return x > 0 ? 1. : (x < 0. ? -1. : 0.); }
 
 // ------------------------------------------------------------------- EoS
-// Ideal massless gas, the SIMPLE branch of EoSs (src/eos.cpp:87-111).
-struct IdealEos {
+// EoSs (src/eos.h:44-71, src/eos.cpp:27-120).  kind 0: the SIMPLE branch,
+// ideal massless gas p = e/3, T = (e/cnst)^(1/4).  kind 1: the TABLE branch,
+// GSL natural cubic splines (gsl_interp_cspline; GSL is not part of the
+// reference tree and its version is unpinned) of p(e), T(e), mu(e), restated
+// from the published natural-spline construction: c_i = y''(x_i)/2 from
+//   h_i-1 c_i-1 + 2 (h_i-1 + h_i) c_i + h_i c_i+1
+//     = 3 ((y_i+1 - y_i)/h_i - (y_i - y_i-1)/h_i-1),   c_0 = c_n-1 = 0
+// (symmetric tridiagonal LDL^T), b_i = dy/h - h (c_i+1 + 2 c_i)/3,
+// d_i = (c_i+1 - c_i)/(3 h), y = y_i + dx (b_i + dx (c_i + dx d_i)); interval
+// search as gsl_interp_bsearch.  cs2(e) = dpe(e) = (p(1.1 e) - p(e))/(0.1 e).
+// Outside the table GSL's domain error aborts the reference; here the end
+// interval is used and out_of_range counts it.
+struct Spline {
+  std::vector<double> x, y, b, c, d;
+  void init(const double *xs, const double *ys, int n) {
+    x.assign(xs, xs + n);
+    y.assign(ys, ys + n);
+    b.assign(n, 0.);
+    c.assign(n, 0.);
+    d.assign(n, 0.);
+    const int m = n - 2;
+    std::vector<double> diag(m > 0 ? m : 1), off(m > 0 ? m : 1), g(m > 0 ? m : 1);
+    for (int i = 0; i < m; i++) {
+      const double h0 = x[i + 1] - x[i], h1 = x[i + 2] - x[i + 1];
+      const double y0 = y[i + 1] - y[i], y1 = y[i + 2] - y[i + 1];
+      const double g0 = h0 != 0. ? 1. / h0 : 0., g1 = h1 != 0. ? 1. / h1 : 0.;
+      off[i] = h1;
+      diag[i] = 2. * (h1 + h0);
+      g[i] = 3. * (y1 * g1 - y0 * g0);
+    }
+    if (m == 1) {
+      c[1] = g[0] / diag[0];
+    } else if (m > 1) {
+      std::vector<double> gam(m), alp(m), z(m);
+      alp[0] = diag[0];
+      gam[0] = off[0] / alp[0];
+      for (int i = 1; i < m - 1; i++) {
+        alp[i] = diag[i] - off[i - 1] * gam[i - 1];
+        gam[i] = off[i] / alp[i];
+      }
+      alp[m - 1] = diag[m - 1] - off[m - 2] * gam[m - 2];
+      z[0] = g[0];
+      for (int i = 1; i < m; i++) z[i] = g[i] - gam[i - 1] * z[i - 1];
+      for (int i = 0; i < m; i++) z[i] = z[i] / alp[i];
+      c[m] = z[m - 1];
+      for (int i = m - 2; i >= 0; i--) c[i + 1] = z[i] - gam[i] * c[i + 2];
+    }
+    for (int i = 0; i < n - 1; i++) {
+      const double dx = x[i + 1] - x[i], dy = y[i + 1] - y[i];
+      b[i] = (dy / dx) - dx * (c[i + 1] + 2. * c[i]) / 3.;
+      d[i] = (c[i + 1] - c[i]) / (3. * dx);
+    }
+  }
+  double eval(double v, long &out_of_range) const {
+    const int n = (int)x.size();
+    if (!(v >= x[0] && v <= x[n - 1])) out_of_range++;
+    int lo = 0, hi = n - 1;
+    while (hi > lo + 1) {
+      const int mid = (hi + lo) / 2;
+      if (x[mid] > v)
+        hi = mid;
+      else
+        lo = mid;
+    }
+    const double delx = v - x[lo];
+    return y[lo] + delx * (b[lo] + delx * (c[lo] + delx * d[lo]));
+  }
+};
+
+struct Eos {
+  int kind = 0;
   // cnst of EoSs::t (src/eos.cpp:98-99); GCC folds pow(C_PI,2) and
   // pow(0.197326968,3) at compile time with correct rounding.
   double cnst;
-  IdealEos() {
+  Spline sp, sT, smu;
+  mutable long out_of_range = 0;
+  explicit Eos(const or_eos_t *d = nullptr) {
     const double pi_ = 3.14159265358979312;  // C_PI, src/inc.h:18
     cnst = (16 + 0.5 * 21.0 * 2.5) * (pi_ * pi_) / 30.0 /
            (0.197326968 * 0.197326968 * 0.197326968);
+    if (d && d->kind == 1) {
+      kind = 1;
+      const int n = d->n;
+      std::vector<double> p(n), mu(n, 0.);
+      for (int i = 0; i < n; i++) p[i] = d->p[i] < 0. ? 0. : d->p[i];  // src/eos.cpp:49
+      if (d->mu) mu.assign(d->mu, d->mu + n);
+      sp.init(d->e, p.data(), n);
+      sT.init(d->e, d->T, n);
+      smu.init(d->e, mu.data(), n);
+    }
   }
-  double p(double e) const { return e / 3.; }
-  double T(double e) const { return e > 0. ? 1.0 * std::pow(e / cnst, 0.25) : 0.; }
-  // EoS::eos for this EoS: p, T, mu=0
+  double p(double e) const { return kind ? sp.eval(e, out_of_range) : e / 3.; }
+  double T(double e) const {
+    if (kind) return sT.eval(e, out_of_range);
+    return e > 0. ? 1.0 * std::pow(e / cnst, 0.25) : 0.;
+  }
+  // EoS::eos for these EoSs: p, T, mu=0
   void eos(double e, double &T_, double &mub, double &muq, double &mus,
            double &p_) const {
     p_ = p(e);
@@ -60,7 +144,10 @@ struct IdealEos {
     if (T_ > 0.0) return (e + p_ - mub * nb - muq * nq - mus * ns) / T_;
     return 0.;
   }
-  double cs2e(double) const { return 1. / 3.; }  // EoSs::dpe SIMPLE
+  double cs2e(double e) const {  // EoSs::cs2(e) = dpe(e)
+    if (kind) return (p(e * 1.1) - p(e)) / (0.1 * e);
+    return 1. / 3.;
+  }
 };
 
 // (C2P_COUNT) iteration counter hook for or_c2p
@@ -69,7 +156,7 @@ thread_local int g_c2p_iters = 0;
 // ---------------------------------------------------------------- con2prim
 // transformPV (src/rmn.cpp:28-139).  Q is modified in place when |Q_vec| > Q_t
 // (src/rmn.cpp:65-70), like the reference.
-void con2prim(const IdealEos &eos, double Q[7], double &e, double &p,
+void con2prim(const Eos &eos, double Q[7], double &e, double &p,
               double &nb, double &nq, double &ns, double &vx, double &vy,
               double &vz) {
   const int maxit = 100;
@@ -163,7 +250,7 @@ void prim2con(double e, double p, double nb, double nq, double ns, double vx,
 // TransportCoeff (src/trancoeff.h:6-45, src/trancoeff.cpp:19-100)
 struct Trc {
   or_trc_t c;
-  const IdealEos *eos;
+  const Eos *eos;
   double zetaS(double e, double T, double s, double P) const {
     if (c.zetaSparam == 0)
       return c.zetaS0 * (1. / 3. - eos->cs2e(e)) / (std::exp((0.16 - T) / 0.001) + 1.);
@@ -298,7 +385,7 @@ struct Grid {
 // ------------------------------------------------------------------- hydro
 struct Solver {
   Grid g;
-  IdealEos eos;
+  Eos eos;
   Trc trc;
   double dt = 0, tau = 0;
   int status = 0;  // 0 ok, 1 NaN exit(1), 2 e>1e10 exit(0)
@@ -1026,10 +1113,11 @@ struct Solver {
 extern "C" {
 
 void *or_create(int nx, int ny, int nz, double minx, double maxx, double miny,
-                double maxy, double minz, double maxz, int eos_kind,
+                double maxy, double minz, double maxz, const or_eos_t *eos,
                 const or_trc_t *trc) {
-  if (eos_kind != 0 || nx < 1 || ny < 1 || nz < 1) return nullptr;
+  if ((eos && eos->kind != 0 && eos->kind != 1) || nx < 1 || ny < 1 || nz < 1) return nullptr;
   Solver *s = new Solver;
+  s->eos = Eos(eos);
   Grid &g = s->g;
   g.nx = nx;
   g.ny = ny;
@@ -1143,8 +1231,8 @@ int or_check_dtau(void *h, double dt) {
   return (dt > s->g.dx / 2. || dt > s->g.dy / 2.) ? 1 : 0;
 }
 
-void or_c2p(int, double *Q, long n, double *out, int *iters) {
-  IdealEos eos;
+void or_c2p(const or_eos_t *d, double *Q, long n, double *out, int *iters) {
+  const Eos eos(d);
   for (long i = 0; i < n; i++) {
     double *o = out + 8 * i;
     con2prim(eos, Q + 7 * i, o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7]);
@@ -1159,8 +1247,15 @@ void or_p2c(const double *prim, long n, double *Q) {
   }
 }
 
-void or_eos(int, const double *e, long n, double *out) {
-  IdealEos eos;
+void or_eos_spline(const or_eos_t *d, int f, const double *v, long n, double *out) {
+  const Eos eos(d);
+  const Spline &sp = f == 0 ? eos.sp : (f == 1 ? eos.sT : eos.smu);
+  long oor = 0;
+  for (long i = 0; i < n; i++) out[i] = sp.eval(v[i], oor);
+}
+
+void or_eos(const or_eos_t *d, const double *e, long n, double *out) {
+  const Eos eos(d);
   for (long i = 0; i < n; i++) {
     double T, mub, muq, mus, p;
     eos.eos(e[i], T, mub, muq, mus, p);
@@ -1171,8 +1266,8 @@ void or_eos(int, const double *e, long n, double *out) {
   }
 }
 
-void or_trcoeff(const or_trc_t *c, int, const double *in, long n, double *out) {
-  IdealEos eos;
+void or_trcoeff(const or_trc_t *c, const or_eos_t *d, const double *in, long n, double *out) {
+  const Eos eos(d);
   Trc t;
   t.c = *c;
   t.eos = &eos;

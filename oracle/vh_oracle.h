@@ -27,6 +27,16 @@ typedef struct {
   double zetaSPeakT, zetaSWidthMinus, zetaSWidthPlus;
 } or_trc_t;
 
+/* EoS (src/eos.h, src/eos.cpp): kind 0 the ideal gas p=e/3 (SIMPLE branch of
+ * EoSs); kind 1 the TABLE branch: n rows of e (increasing), p (< 0 clamped to
+ * 0), T, mu (may be NULL), natural cubic splines (gsl_interp_cspline).  A NULL
+ * descriptor is the ideal gas. */
+typedef struct {
+  int kind;
+  int n;
+  const double *e, *p, *T, *mu;
+} or_eos_t;
+
 /* Whole-state view, SoA host arrays (any pointer may be NULL on set: field
  * then left unchanged; on get: not written). */
 typedef struct {
@@ -40,7 +50,7 @@ typedef struct {
 } or_state_t;
 
 void *or_create(int nx, int ny, int nz, double minx, double maxx, double miny,
-                double maxy, double minz, double maxz, int eos_kind,
+                double maxy, double minz, double maxz, const or_eos_t *eos,
                 const or_trc_t *trc);
 void or_destroy(void *h);
 long or_ncell(void *h);
@@ -64,15 +74,16 @@ int or_check_dtau(void *h, double dt);
  * modified in place exactly like the reference modifies its argument).
  * out: n x 8 row-major {e, p, nb, nq, ns, vx, vy, vz}.  iters (may be NULL):
  * loop iterations executed per input. */
-void or_c2p(int eos_kind, double *Q, long n, double *out, int *iters);
+void or_c2p(const or_eos_t *eos, double *Q, long n, double *out, int *iters);
 /* transformCV (src/rmn.cpp:253-263): prim n x 8 {e,p,nb,nq,ns,vx,vy,vz} -> Q n x 7 */
 void or_p2c(const double *prim, long n, double *Q);
-/* EoS relations of the ideal gas p=e/3 (src/eos.cpp:87-111 SIMPLE branch):
- * out n x 4 {p, T, s, cs2}. */
-void or_eos(int eos_kind, const double *e, long n, double *out);
+/* EoS relations (EoS::eos, EoS::s, cs2(e)): out n x 4 {p, T, s, cs2}. */
+void or_eos(const or_eos_t *eos, const double *e, long n, double *out);
+/* the table's spline of f (0 p, 1 T, 2 mu) at n points (kind 1 only) */
+void or_eos_spline(const or_eos_t *eos, int f, const double *v, long n, double *out);
 /* TransportCoeff::getEta + getTau (src/trancoeff.cpp:19-100) on n states
  * {e, nb, T, muB, s, p} -> out n x 4 {etaS, zetaS, taupi, tauPi}. */
-void or_trcoeff(const or_trc_t *trc, int eos_kind, const double *in, long n,
+void or_trcoeff(const or_trc_t *trc, const or_eos_t *eos, const double *in, long n,
                 double *out);
 
 /* ---- freeze-out (oracle/fo_oracle.cpp) ---- */
@@ -90,7 +101,7 @@ typedef struct {
  * construction failure. */
 long or_freezeout(const or_fo_grid_t *g, const double *Q, const double *Qlow, const double *pi,
                   const double *Pi, double tau, double dt, double ecrit, int extend,
-                  double *rows, long max_rows);
+                  double *rows, long max_rows, const or_eos_t *eos);
 #ifdef __cplusplus
 }
 #endif

@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libvhlle_gpu.so")
 
 VH_OK, VH_ERR_ARG, VH_ERR_HIP, VH_ERR_NODEVICE, VH_ERR_DTAU = 0, -1, -2, -3, -4
-ST_NAN_FLUX, ST_BIG_E, ST_NAN_SOURCE, ST_NAN_PI = 1, 2, 4, 8
+ST_NAN_FLUX, ST_BIG_E, ST_NAN_SOURCE, ST_NAN_PI, ST_EOS_RANGE = 1, 2, 4, 8, 16
+EOS_IDEAL, EOS_TABLE = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 C2P_REFERENCE, C2P_LEARNED, C2P_CLOSED, C2P_NEWTON = 0, 1, 2, 3
 START_MLP, START_HALF = 0, 1
@@ -30,7 +31,55 @@ class Grid(ctypes.Structure):
 
 
 class Eos(ctypes.Structure):
-    _fields_ = [("kind", ctypes.c_int)]
+    """vh_eos_t: kind (EOS_IDEAL, EOS_TABLE) and the table's columns."""
+    _fields_ = [("kind", ctypes.c_int), ("n", ctypes.c_int)] + [
+        (k, ctypes.POINTER(ctypes.c_double)) for k in ("e", "p", "T", "mu")]
+
+
+class EosTable:
+    """A tabulated EoS (the TABLE branch of EoSs): increasing e (GeV/fm^3),
+    p (GeV/fm^3), T (GeV), mu (GeV, optional).  desc() is the vh_eos_t (the
+    arrays stay alive with this object)."""
+
+    def __init__(self, e, p, T, mu=None):
+        self.e = np.ascontiguousarray(e, dtype=np.float64)
+        self.p = np.ascontiguousarray(p, dtype=np.float64)
+        self.T = np.ascontiguousarray(T, dtype=np.float64)
+        self.mu = None if mu is None else np.ascontiguousarray(mu, dtype=np.float64)
+        self._d = Eos(EOS_TABLE, len(self.e), _p(self.e), _p(self.p), _p(self.T),
+                      None if self.mu is None else _p(self.mu))
+
+    @classmethod
+    def read(cls, path, ncols=3):
+        """Columns e, p, T [, mu] of a whitespace table file (EoSs's format)."""
+        a = np.loadtxt(path, ndmin=2)
+        return cls(a[:, 0], a[:, 1], a[:, 2], a[:, 3] if ncols > 3 else None)
+
+    def desc(self):
+        return self._d
+
+    def spline_table(self):
+        """(13 n) spline table of vh_eos_table: nodes, then y, b, c, d of p, T, mu"""
+        buf = np.zeros(13 * len(self.e))
+        check(lib().vh_eos_table(ctypes.byref(self._d), _p(buf)), "vh_eos_table")
+        return buf
+
+    def spline(self, f, v):
+        """the spline of f (0 p, 1 T, 2 mu) at v, evaluated in numpy from the
+        library's coefficients (same formula as the device EoS)"""
+        n = len(self.e)
+        t = self.spline_table().reshape(13, n)
+        x = t[0]
+        y, b, c, d = t[1 + 4 * f:5 + 4 * f]
+        v = np.asarray(v, dtype=np.float64)
+        i = np.clip(np.searchsorted(x, v, side="right") - 1, 0, n - 2)
+        dx = v - x[i]
+        return y[i] + dx * (b[i] + dx * (c[i] + dx * d[i]))
+
+
+def eos_desc(eos):
+    """vh_eos_t of None (ideal gas) or an EosTable"""
+    return Eos(EOS_IDEAL, 0, None, None, None, None) if eos is None else eos.desc()
 
 
 class Trc(ctypes.Structure):
@@ -52,7 +101,7 @@ class MlpTrainCfg(ctypes.Structure):
 class HarvestCfg(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_ulonglong), ("steps", ctypes.c_int), ("namp", ctypes.c_int),
                 ("amp", ctypes.c_double * 4), ("augment", ctypes.c_double),
-                ("vmax", ctypes.c_double)]
+                ("vmax", ctypes.c_double), ("eos", ctypes.POINTER(Eos))]
 
 
 class HotspotCfg(ctypes.Structure):
@@ -136,6 +185,7 @@ def lib():
                                         ctypes.c_int, ctypes.c_int, _dp, ctypes.c_long,
                                         ctypes.POINTER(ctypes.c_long), ctypes.POINTER(FoStats)]),
         "vh_cornelius": (ctypes.c_int, [_dp, ctypes.c_long, _dp, ctypes.c_double, _dp, _dp, ip]),
+        "vh_eos_table": (ctypes.c_int, [ctypes.POINTER(Eos), _dp]),
         "vh_create_batch": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Grid), ctypes.POINTER(Eos),
                                            ctypes.POINTER(Trc), ctypes.c_int, ctypes.POINTER(vp)]),
     }
@@ -153,7 +203,8 @@ def exported_symbols():
             "vh_kernel_times", "vh_c2p", "vh_p2c", "vh_device_info", "vh_mlp_init",
             "vh_mlp_forward", "vh_mlp_grad", "vh_mlp_train", "vh_c2p_learned",
             "vh_set_c2p_model", "vh_mlp_harvest", "vh_set_stage", "vh_get_stage", "vh_create_batch", "vh_hotspot_ic",
-            "vh_copy_bandwidth", "vh_fo_snapshot", "vh_freezeout", "vh_cornelius"]
+            "vh_copy_bandwidth", "vh_fo_snapshot", "vh_freezeout", "vh_cornelius",
+            "vh_eos_table"]
 
 
 def _p(a):
@@ -178,13 +229,13 @@ def device_info():
     return buf.value.decode()
 
 
-def c2p(Q):
+def c2p(Q, eos=None):
     """transformPV on the GPU over rows of Q (n x 7): (prim n x 8, iters, Q_after)."""
     Q = np.ascontiguousarray(Q, dtype=np.float64).copy()
     n = Q.shape[0]
     out = np.zeros((n, 8))
     it = np.zeros(n, dtype=np.int32)
-    e = Eos(0)
+    e = eos_desc(eos)
     check(lib().vh_c2p(ctypes.byref(e), _p(Q), n, _p(out),
                        it.ctypes.data_as(ctypes.POINTER(ctypes.c_int))), "vh_c2p")
     return out, it, Q
@@ -233,24 +284,27 @@ def mlp_train(params, Q, prim, batch=4096, epochs=4, lr=2e-3, lr_final=1e-5, see
     return p, hist
 
 
-def c2p_learned(params, Q, start=START_MLP):
+def c2p_learned(params, Q, start=START_MLP, eos=None):
     """con2prim from the MLP guess (or v = 0.5): (prim n x 8, iters, Q_after)."""
     params = np.ascontiguousarray(params, dtype=np.float32)
     Q = np.ascontiguousarray(Q, dtype=np.float64).copy()
     n = Q.shape[0]
     out = np.zeros((n, 8))
     it = np.zeros(n, dtype=np.int32)
-    e = Eos(0)
+    e = eos_desc(eos)
     check(lib().vh_c2p_learned(ctypes.byref(e), _f(params), _p(Q), n, start, _p(out),
                                it.ctypes.data_as(ctypes.POINTER(ctypes.c_int))),
           "vh_c2p_learned")
     return out, it, Q
 
 
-def harvest(n, seed=1, steps=200, amps=(10.0, 1.0), augment=0.5, vmax=0.99):
-    """Training pairs (Q n x 7, prim n x 8) from Gubser runs + augmentation."""
+def harvest(n, seed=1, steps=200, amps=(10.0, 1.0), augment=0.5, vmax=0.99, eos=None):
+    """Training pairs (Q n x 7, prim n x 8) from Gubser runs + augmentation,
+    with the ideal gas or an EosTable."""
     a = (ctypes.c_double * 4)(*(list(amps) + [0.0] * (4 - len(amps))))
-    cfg = HarvestCfg(seed, steps, len(amps), a, augment, vmax)
+    d = eos_desc(eos)
+    cfg = HarvestCfg(seed, steps, len(amps), a, augment, vmax,
+                     None if eos is None else ctypes.pointer(d))
     Q = np.zeros((n, 7))
     prim = np.zeros((n, 8))
     check(lib().vh_mlp_harvest(ctypes.byref(cfg), n, _p(Q), _p(prim)), "vh_mlp_harvest")
@@ -304,19 +358,21 @@ class Context:
     of independent grids: a list of (nx, ny, nz, xlim, ylim, zlim))."""
 
     def __init__(self, nx=None, ny=None, nz=None, xlim=None, ylim=None, zlim=None, trc=None,
-                 precision=PREC_FP64, grids=None):
+                 precision=PREC_FP64, grids=None, eos=None):
         self.trc = trc if trc is not None else make_trc()
+        self.eos = eos
         h = ctypes.c_void_p()
         if grids is None:
             g = Grid(nx, ny, nz, xlim[0], xlim[1], ylim[0], ylim[1], zlim[0], zlim[1])
-            check(lib().vh_create(ctypes.byref(g), ctypes.byref(Eos(0)), ctypes.byref(self.trc),
-                                  precision, ctypes.byref(h)), "vh_create")
+            check(lib().vh_create(ctypes.byref(g), ctypes.byref(eos_desc(eos)),
+                                  ctypes.byref(self.trc), precision, ctypes.byref(h)),
+                  "vh_create")
             self.N = nx * ny * nz
             self.shape = (nx, ny, nz)
         else:
             arr = (Grid * len(grids))(*[Grid(a, b, c, x[0], x[1], y[0], y[1], z[0], z[1])
                                         for (a, b, c, x, y, z) in grids])
-            check(lib().vh_create_batch(len(grids), arr, ctypes.byref(Eos(0)),
+            check(lib().vh_create_batch(len(grids), arr, ctypes.byref(eos_desc(eos)),
                                         ctypes.byref(self.trc), precision, ctypes.byref(h)),
                   "vh_create_batch")
             self.N = sum(a * b * c for (a, b, c, _, _, _) in grids)

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/vhlle_gpu.h"
+#include "vh_eos.h"
 #include "vh_kernels.h"
 #include "vh_kernels_mixed.h"
 #include "vh_mlp_host.h"
@@ -28,14 +29,6 @@
 
 namespace {
 
-// EoSs::t constant (src/eos.cpp:98-99), evaluated like the reference's
-// compile-time expression.
-double ideal_cnst() {
-  const double pi_ = 3.14159265358979312;
-  return (16 + 0.5 * 21.0 * 2.5) * (pi_ * pi_) / 30.0 /
-         (0.197326968 * 0.197326968 * 0.197326968);
-}
-
 struct KTime {
   int launches = 0;
   double ms = 0.;
@@ -46,6 +39,7 @@ struct KTime {
 struct vh_ctx {
   vh::Geom g;
   vh::DevEos eos;
+  double *eos_buf = nullptr;  // spline table of a VH_EOS_TABLE EoS (device)
   vh::DevTrc trc;
   vh_grid_t grid;
   int precision = 0, c2p_mode = 0;
@@ -414,7 +408,13 @@ int build_grid_tables(vh_ctx *c) {
 int create_impl(const vh_grid_t *grids, int ngrid, const vh_eos_t *eos, const vh_trcoeff_t *trc,
                 int precision, vh_ctx_t **out) {
   if (!grids || ngrid < 1 || !eos || !trc || !out) return VH_ERR_ARG;
-  if (eos->kind != VH_EOS_IDEAL) return VH_ERR_ARG;
+  if (eos->kind == VH_EOS_TABLE) {  // validate before allocating anything
+    std::vector<double> tab;
+    int n = 0;
+    if (vh::eos_table_host(*eos, tab, n) != VH_OK) return VH_ERR_ARG;
+  } else if (eos->kind != VH_EOS_IDEAL) {
+    return VH_ERR_ARG;
+  }
   if (precision != VH_PREC_FP64 && precision != VH_PREC_MIXED) return VH_ERR_ARG;
   if (ngrid > 1 && precision != VH_PREC_MIXED) return VH_ERR_ARG;
   long ncell = 0;
@@ -455,8 +455,6 @@ int create_impl(const vh_grid_t *grids, int ngrid, const vh_eos_t *eos, const vh
   c->g.base = 0;
   c->g.P = ncell;
   const vh::Geom &g = c->g;
-  c->eos.kind = eos->kind;
-  c->eos.cnst = ideal_cnst();
   c->trc.c = *trc;
   // TransportCoeff::isViscous (src/trancoeff.h:39-44)
   c->viscous = trc->etaS0 > 0. || trc->zetaS0 > 0. || trc->etaSparam > 0 || trc->zetaSparam > 0;
@@ -505,6 +503,10 @@ int create_impl(const vh_grid_t *grids, int ngrid, const vh_eos_t *eos, const vh
   c->gv = reinterpret_cast<float *>(c->F[0]);  // the fused mixed stages use no F
   c->status = reinterpret_cast<int *>(p);
   p += al;
+  if (vh::make_dev_eos(eos, c->status, c->eos, &c->eos_buf) != VH_OK) {
+    vh_destroy(c);
+    return VH_ERR_HIP;
+  }
   for (int d = 0; d < 3; d++) {
     c->fl[d] = reinterpret_cast<unsigned char *>(p);
     p += bN;
@@ -557,6 +559,7 @@ int vh_destroy(vh_ctx_t *c) {
   if (c->gtab) (void)hipFree(c->gtab);
   if (c->fo_snap) (void)hipFree(c->fo_snap);
   if (c->mlp) (void)hipFree(c->mlp);
+  if (c->eos_buf) (void)hipFree(c->eos_buf);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
   return VH_OK;
@@ -698,6 +701,7 @@ int vh_set_c2p_mode(vh_ctx_t *c, int mode) {
       mode != VH_C2P_NEWTON)
     return VH_ERR_ARG;
   if (mode == VH_C2P_LEARNED && !c->mlp) return VH_ERR_ARG;  // no model installed
+  if (mode == VH_C2P_CLOSED && c->eos.kind != VH_EOS_IDEAL) return VH_ERR_ARG;  // p = e/3 only
   c->c2p_mode = mode;
   if (!c->tile_forced) c->tile.kind = 0;  // retune for this con2prim
   drop_graphs(c);
@@ -868,8 +872,16 @@ int vh_c2p(const vh_eos_t *eos, double *Q, long n, double *out, int *iters) {
   if (!eos || !Q || !out || n < 0) return VH_ERR_ARG;
   if (n == 0) return VH_OK;
   vh::DevEos de;
-  de.kind = eos->kind;
-  de.cnst = ideal_cnst();
+  double *tab = nullptr;
+  int *dS = nullptr;
+  VH_HIP(hipMalloc(&dS, sizeof(int)));
+  {
+    const int rc = vh::make_dev_eos(eos, dS, de, &tab);
+    if (rc != VH_OK) {
+      (void)hipFree(dS);
+      return rc;
+    }
+  }
   double *dQ = nullptr, *dO = nullptr;
   int *dI = nullptr;
   VH_HIP(hipMalloc(&dQ, n * 7 * sizeof(double)));
@@ -884,6 +896,8 @@ int vh_c2p(const vh_eos_t *eos, double *Q, long n, double *out, int *iters) {
   (void)hipFree(dQ);
   (void)hipFree(dO);
   if (dI) (void)hipFree(dI);
+  if (tab) (void)hipFree(tab);
+  (void)hipFree(dS);
   return VH_OK;
 }
 

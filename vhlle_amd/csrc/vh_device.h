@@ -32,14 +32,59 @@ __device__ inline double smin(double a, double b) { return (b < a) ? b : a; }
 __device__ inline double smax(double a, double b) { return (a < b) ? b : a; }
 
 // --------------------------------------------------------------------- EoS
-// Device EoS: the ideal massless gas (SIMPLE branch of EoSs,
-// src/eos.cpp:87-111).  cnst is computed on the host exactly as the
-// reference's compile-time constant.
+// Natural cubic spline of a tabulated function (gsl_interp_cspline, the TABLE
+// branch of EoSs, src/eos.cpp:59-69): on [x_i, x_i+1],
+//   y = y_i + dx (b_i + dx (c_i + dx d_i)),  dx = v - x_i,
+// with c_i = y''(x_i)/2 from the natural tridiagonal system and b_i, d_i from
+// c_i, c_i+1 (computed once on the host).  Interval search as gsl_interp_bsearch
+// (x_i <= v < x_i+1; v = x_n-1 falls in the last interval).
+struct SplineTab {
+  const double *x;  // n nodes, then per function: y, b, c, d (n each)
+  int n;
+  __device__ inline int interval(double v) const {
+    int lo = 0, hi = n - 1;
+    while (hi > lo + 1) {
+      const int mid = (hi + lo) / 2;
+      if (x[mid] > v)
+        hi = mid;
+      else
+        lo = mid;
+    }
+    return lo;
+  }
+  // function f (0: p, 1: T, 2: mu) at v; *out_of_range set outside [x_0, x_n-1]
+  __device__ inline double eval(int f, double v, bool &out_of_range) const {
+    out_of_range = !(v >= x[0] && v <= x[n - 1]);
+    const int i = interval(v);
+    const double *y = x + (long)(1 + 4 * f) * n;
+    const double *b = y + n, *c = y + 2 * n, *d = y + 3 * n;
+    const double delx = v - x[i];
+    return y[i] + delx * (b[i] + delx * (c[i] + delx * d[i]));
+  }
+};
+
+// Device EoS (kind VH_EOS_IDEAL or VH_EOS_TABLE, include/vhlle_gpu.h).
+// Ideal massless gas: the SIMPLE branch of EoSs (src/eos.cpp:87-111), cnst
+// computed on the host exactly as the reference's compile-time constant.
+// Table: the TABLE branch (src/eos.cpp:27-120), p, T, mu splines of e; an
+// evaluation outside the table raises VH_ST_EOS_RANGE in *st (the reference
+// aborts there) and uses the end interval's cubic.
 struct DevEos {
   int kind;
   double cnst;
-  __device__ double p(double e) const { return e / 3.; }
-  __device__ double T(double e) const { return e > 0. ? 1.0 * pow(e / cnst, 0.25) : 0.; }
+  SplineTab tab;
+  int *st;  // status word for VH_ST_EOS_RANGE (may be null)
+  __device__ inline double spline(int f, double e) const {
+    bool out;
+    const double v = tab.eval(f, e, out);
+    if (out && st) atomicOr(st, VH_ST_EOS_RANGE);
+    return v;
+  }
+  __device__ double p(double e) const { return kind == VH_EOS_TABLE ? spline(0, e) : e / 3.; }
+  __device__ double T(double e) const {
+    if (kind == VH_EOS_TABLE) return spline(1, e);
+    return e > 0. ? 1.0 * pow(e / cnst, 0.25) : 0.;
+  }
   // EoS::s (src/eos.cpp:18-25) with mu = 0
   __device__ double s(double e, double nb, double nq, double ns) const {
     const double p_ = p(e), T_ = T(e);
@@ -47,12 +92,21 @@ struct DevEos {
     if (T_ > 0.0) return (e + p_ - mub * nb - muq * nq - mus * ns) / T_;
     return 0.;
   }
-  __device__ double cs2e(double) const { return 1. / 3.; }
+  // EoSs::cs2(e) = dpe(e) (src/eos.cpp:89-96; 1/3 for the ideal gas)
+  __device__ double cs2e(double e) const {
+    if (kind == VH_EOS_TABLE) return (spline(0, e * 1.1) - spline(0, e)) / (0.1 * e);
+    return 1. / 3.;
+  }
   // FP32 forms (mixed precision: interface states of the FP32 flux, the
   // thermodynamics of the FP32 relaxation step)
-  __device__ float pf(float e) const { return e * (1.f / 3.f); }
-  __device__ float cs2f(float) const { return 1.f / 3.f; }
+  __device__ float pf(float e) const {
+    return kind == VH_EOS_TABLE ? (float)spline(0, (double)e) : e * (1.f / 3.f);
+  }
+  __device__ float cs2f(float e) const {
+    return kind == VH_EOS_TABLE ? (float)cs2e((double)e) : 1.f / 3.f;
+  }
   __device__ float Tf(float e) const {
+    if (kind == VH_EOS_TABLE) return (float)spline(1, (double)e);
     return e > 0.f ? __builtin_amdgcn_sqrtf(__builtin_amdgcn_sqrtf(e * (float)(1. / cnst))) : 0.f;
   }
   __device__ float sf(float e, float T) const { return T > 0.f ? (e + pf(e)) / T : 0.f; }

@@ -13,6 +13,7 @@
 
 #include "../../include/vhlle_gpu.h"
 #include "vh_mlp.h"
+#include "vh_eos.h"
 #include "vh_mlp_host.h"
 
 namespace vh {
@@ -539,8 +540,19 @@ int vh_c2p_learned(const vh_eos_t *eos, const float *params, double *Q, long n, 
   if (!have_device()) return VH_ERR_NODEVICE;
   if (n == 0) return VH_OK;
   vh::DevEos de;
-  de.kind = eos->kind;
-  de.cnst = vh::ml::ideal_eos_cnst();
+  double *tab = nullptr;
+  DBuf<int> dS;
+  ML_HIP(dS.alloc(1));
+  {
+    const int rc = vh::make_dev_eos(eos, dS.p, de, &tab);
+    if (rc != VH_OK) return rc;
+  }
+  struct FreeTab {
+    double *t;
+    ~FreeTab() {
+      if (t) (void)hipFree(t);
+    }
+  } free_tab{tab};
   DBuf<double> dQ, dO;
   DBuf<int> dI;
   DBuf<Packed> P;

@@ -7,13 +7,6 @@
 namespace vh {
 namespace ml {
 
-// EoSs::t constant of the ideal massless gas (src/eos.cpp:98-99)
-inline double ideal_eos_cnst() {
-  const double pi_ = 3.14159265358979312;
-  return (16 + 0.5 * 21.0 * 2.5) * (pi_ * pi_) / 30.0 /
-         (0.197326968 * 0.197326968 * 0.197326968);
-}
-
 // FP32 parameter vector -> MFMA fragments (BF16, round to nearest even)
 void pack(const float *params, Packed &out);
 

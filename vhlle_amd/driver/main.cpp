@@ -3,8 +3,9 @@
 //
 // Same keys, defaults and CLI (-params, -system, -ISinput, -outputDir) as the
 // reference; keys the reference does not know are echoed "UUU", comment lines
-// ('!') "CCC".  Added keys: eosType 5 = ideal gas p=e/3 (the reference's
-// tabulated EoS files are not part of this build); icModel 10 = smooth
+// ('!') "CCC".  eosType 0 reads the tabulated EoS eosFile (default
+// eos/Laine_nf3.dat, the reference's path; columns e, p, T) as EoSs does.
+// Added keys: eosType 5 = ideal gas p=e/3; eosFile; icModel 10 = smooth
 // Gaussian state (icE0, icSigmaX, icSigmaY, icEtaPlateau, icEtaSigma);
 // gubserScale (energy scale of icModel 4); zetaSPeakT, zetaSWidthMinus,
 // zetaSWidthPlus (zetaSparam 5); dumpState 1 writes the final conserved
@@ -13,8 +14,9 @@
 // 3 Newton from v = 0.5), c2pModel (a file written by -trainC2P);
 // stopOnEmptySurface (1, the reference's loop condition: stop at the first
 // step without freeze-out elements, which for nz < 5 is the first step).
-// `vhlle -trainC2P file [-seed S]` harvests 4 M conserved -> primitive pairs
-// from Gubser runs, trains the con2prim MLP on the GPU and writes it.
+// `vhlle [-params file] -trainC2P file [-seed S]` harvests 4 M conserved ->
+// primitive pairs from Gubser runs with the params' EoS (ideal gas without
+// -params), trains the con2prim MLP on the GPU and writes it.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -36,6 +38,7 @@ using namespace std;
 namespace {
 
 int nx{100}, ny{100}, nz{100}, eosType{1}, etaSparam{0}, zetaSparam{0}, eosTypeHadron{0};
+string eosFile{"eos/Laine_nf3.dat"};
 bool freezeoutOnly{false}, freezeoutExtend{false}, vorticityOn{false};
 double xmin{-5.0}, xmax{5.0}, ymin{-5.0}, ymax{5.0}, etamin{-5.0}, etamax{5.0}, tau0{1.0},
     tauMax{20.0}, tauResize{4.0}, dtau{0.05}, etaS{0.08}, zetaS{0.0}, eCrit{0.5},
@@ -84,6 +87,7 @@ void readParameters(const char *parFile) {
       {"zetaSWidthMinus", D(zetaSWidthMinus)}, {"zetaSWidthPlus", D(zetaSWidthPlus)},
       {"dumpState", I(dumpState)}, {"precision", I(precision)}, {"c2pMode", I(c2pMode)},
       {"c2pModel", [](const string &s) { c2pModel = s; }},
+      {"eosFile", [](const string &s) { eosFile = s; }},
       {"stopOnEmptySurface", I(stopOnEmptySurface)},
   };
   while (fin.good()) {
@@ -166,10 +170,10 @@ double totalEnergy(Fluid *f, double tau) {
 }  // namespace
 
 // harvest + GPU training of the con2prim MLP (vh_mlp_harvest, vh_mlp_train)
-int trainC2P(const char *out, unsigned long long seed) {
+int trainC2P(const char *out, unsigned long long seed, const vh_eos_t *eos) {
   const long n = 4L << 20;
   std::vector<double> Q(7 * n), prim(8 * n);
-  vh_harvest_cfg_t hc{seed, 200, 2, {10.0, 1.0, 0., 0.}, 0.5, 0.99};
+  vh_harvest_cfg_t hc{seed, 200, 2, {10.0, 1.0, 0., 0.}, 0.5, 0.99, eos};
   if (vh_mlp_harvest(&hc, n, Q.data(), prim.data()) != VH_OK) {
     cout << "harvest failed\n";
     return 1;
@@ -192,21 +196,31 @@ int trainC2P(const char *out, unsigned long long seed) {
 }
 
 int main(int argc, char **argv) {
-  for (int i = 1; i < argc - 1; i++)
-    if (strcmp(argv[i], "-trainC2P") == 0) {
-      unsigned long long seed = 1;
-      for (int j = 1; j < argc - 1; j++)
-        if (strcmp(argv[j], "-seed") == 0) seed = strtoull(argv[j + 1], nullptr, 10);
-      return trainC2P(argv[i + 1], seed);
-    }
+  const char *train = nullptr;
+  unsigned long long seed = 1;
+  bool params = false;
+  for (int i = 1; i < argc - 1; i++) {
+    if (strcmp(argv[i], "-trainC2P") == 0) train = argv[i + 1];
+    if (strcmp(argv[i], "-seed") == 0) seed = strtoull(argv[i + 1], nullptr, 10);
+    if (strcmp(argv[i], "-params") == 0) params = true;
+  }
+  if (train && !params) return trainC2P(train, seed, nullptr);
   readCommandLine(argc, argv);
-  if (eosType != 5) {
+  EoS *eos = nullptr;
+  if (eosType == 0) {
+    eos = new EoSs(eosFile, 3);  // src/main.cpp:289-290
+  } else if (eosType == 5) {
+    eos = new EoSIdeal();
+  } else {
     cout << "eosType " << eosType
-         << " needs the reference's EoS tables, which this build does not ship; "
-            "use eosType 5 (ideal gas p=e/3)\n";
+         << " (chiral, AZH, CMF) is not part of this build; use eosType 0 (table) or 5 "
+            "(ideal gas p=e/3)\n";
     return 0;
   }
-  EoS *eos = new EoSIdeal();
+  if (train) {
+    const vh_eos_t d = eos->device();
+    return trainC2P(train, seed, &d);
+  }
   TransportCoeff *trcoeff =
       new TransportCoeff(etaS, zetaS, ah, al, aRho, T0, etaSMin, etaSEpsilonMin, etaSShiftMuB,
                          etaSScaleMuB, zetaSPeakEpsilon, zetaSScaleBeta, zetaSSigmaMinus,

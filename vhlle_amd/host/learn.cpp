@@ -54,7 +54,10 @@ extern "C" int vh_mlp_harvest(const vh_harvest_cfg_t *cfg, long n, double *Q, do
   std::stable_sort(order.begin(), order.end(), [&](long a, long b) { return pick[a] < pick[b]; });
   long cursor = 0;
   vh_grid_t grid{nx, ny, 1, -5.0, 5.0, -5.0, 5.0, -1.0, 1.0};
-  vh_eos_t eos{VH_EOS_IDEAL};
+  vh_eos_t ideal{};
+  ideal.kind = VH_EOS_IDEAL;
+  const vh_eos_t &eos = cfg->eos ? *cfg->eos : ideal;
+  const bool table = eos.kind == VH_EOS_TABLE;
   vh_trcoeff_t trc;
   std::memset(&trc, 0, sizeof trc);
   trc.T0 = 0.15;
@@ -92,12 +95,19 @@ extern "C" int vh_mlp_harvest(const vh_harvest_cfg_t *cfg, long n, double *Q, do
     vh_destroy(ctx);
     if (rc) return rc;
   }
-  // augmented part: random primitives, e log-uniform in [1e-6, 1e3]
+  // augmented part: random primitives, e log-uniform in [1e-6, 1e3] (inside
+  // a table's range), p = p(e) of the EoS
   if (n_aug > 0) {
+    double elo = 1e-6, ehi = 1e3;
+    if (table) {
+      elo = std::max(elo, eos.e[0] > 0. ? eos.e[0] : eos.e[1]);
+      ehi = std::min(ehi, eos.e[eos.n - 1]);
+      if (!(ehi > elo)) return VH_ERR_ARG;
+    }
     std::vector<double> pr(n_aug * 8);
     for (long i = 0; i < n_aug; i++) {
       double *p = &pr[i * 8];
-      const double ee = std::exp(std::log(1e-6) + uni(s) * (std::log(1e3) - std::log(1e-6)));
+      const double ee = std::exp(std::log(elo) + uni(s) * (std::log(ehi) - std::log(elo)));
       const double v = cfg->vmax * uni(s);
       const double cz = 2. * uni(s) - 1., ph = 2. * M_PI * uni(s), sz = std::sqrt(1. - cz * cz);
       p[0] = ee;
@@ -107,6 +117,10 @@ extern "C" int vh_mlp_harvest(const vh_harvest_cfg_t *cfg, long n, double *Q, do
       p[5] = v * sz * std::cos(ph);
       p[6] = v * sz * std::sin(ph);
       p[7] = v * cz;
+    }
+    if (table) {
+      EoSs tab(eos);
+      for (long i = 0; i < n_aug; i++) pr[i * 8 + 1] = tab.p(pr[i * 8]);
     }
     const int rc = vh_p2c(pr.data(), n_aug, Q + n_h * 7);
     if (rc) return rc;

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <iomanip>
 #include <iostream>
 #include <stdexcept>
@@ -40,8 +41,95 @@ void EoSIdeal::eos(double e, double, double, double, double &T, double &mub, dou
 }
 
 vh_eos_t EoSIdeal::device() const {
-  vh_eos_t d;
+  vh_eos_t d{};
   d.kind = VH_EOS_IDEAL;
+  return d;
+}
+
+EoSs::EoSs(std::string fname, int ncols) {
+  std::ifstream finput(fname.c_str(), std::ios::in);
+  if (!finput) {
+    std::cerr << "can't open input file \"" << fname.c_str() << "\"" << std::endl;
+    std::exit(1);
+  }
+  double e, p, T, mu;
+  while (true) {
+    mu = 0.;
+    if (!(finput >> e >> p >> T)) break;
+    if (ncols != 3 && !(finput >> mu)) break;
+    if (p < 0.) p = 0.;
+    if (!te_.empty() && e <= te_.back())
+      std::cout << "WARNING non-monotonic EoS table " << te_.size() << "  " << te_.back() << "  "
+                << e << std::endl;
+    te_.push_back(e);
+    tp_.push_back(p);
+    tT_.push_back(T);
+    tmu_.push_back(mu);
+  }
+  build();
+}
+
+EoSs::EoSs(const vh_eos_t &d) {
+  if (d.kind == VH_EOS_TABLE && d.n > 0 && d.e && d.p && d.T) {
+    te_.assign(d.e, d.e + d.n);
+    tp_.assign(d.p, d.p + d.n);
+    tT_.assign(d.T, d.T + d.n);
+    tmu_.assign(d.n, 0.);
+    if (d.mu) tmu_.assign(d.mu, d.mu + d.n);
+  }
+  build();
+}
+
+void EoSs::build() {
+  n_ = (int)te_.size();
+  vh_eos_t d = device();
+  tab_.assign((size_t)13 * (n_ > 0 ? n_ : 1), 0.);
+  if (vh_eos_table(&d, tab_.data()) != VH_OK) {
+    std::cerr << "EoSs: the table needs at least 3 rows with increasing e" << std::endl;
+    std::exit(1);
+  }
+}
+
+double EoSs::eval(int f, double e) const {
+  const double *x = tab_.data();
+  if (!(e >= x[0] && e <= x[n_ - 1])) {  // gsl_spline_eval's domain error
+    std::cerr << "gsl: interpolation error: e = " << e << " outside the EoS table [" << x[0]
+              << ", " << x[n_ - 1] << "]" << std::endl;
+    std::exit(1);
+  }
+  int lo = 0, hi = n_ - 1;
+  while (hi > lo + 1) {
+    const int mid = (hi + lo) / 2;
+    if (x[mid] > e)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  const double *y = x + (size_t)(1 + 4 * f) * n_, *b = y + n_, *c = y + 2 * n_, *dd = y + 3 * n_;
+  const double delx = e - x[lo];
+  return y[lo] + delx * (b[lo] + delx * (c[lo] + delx * dd[lo]));
+}
+
+double EoSs::p(double e) { return eval(0, e); }
+double EoSs::dpe(double e) { return (eval(0, e * 1.1) - eval(0, e)) / (0.1 * e); }
+double EoSs::t(double e) { return eval(1, e); }
+double EoSs::mu(double e) { return eval(2, e); }
+
+void EoSs::eos(double e, double, double, double, double &T, double &mub, double &muq,
+               double &mus, double &pr) {
+  pr = p(e);
+  T = t(e);
+  mub = muq = mus = 0.;
+}
+
+vh_eos_t EoSs::device() const {
+  vh_eos_t d{};
+  d.kind = VH_EOS_TABLE;
+  d.n = (int)te_.size();
+  d.e = te_.data();
+  d.p = tp_.data();
+  d.T = tT_.data();
+  d.mu = tmu_.data();
   return d;
 }
 

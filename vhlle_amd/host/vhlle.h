@@ -70,6 +70,35 @@ class EoSIdeal : public EoS {
   vh_eos_t device() const override;
 };
 
+// Tabulated EoS: the TABLE branch of EoSs (src/eos.h:44-71, src/eos.cpp:27-120).
+// EoSs(fname, ncols) reads columns e, p, T [, mu] as the reference does
+// (p < 0 clamped to 0, a warning for non-increasing e); natural cubic splines
+// of p(e), T(e), mu(e); cs2(e) = dpe(e) = (p(1.1e) - p(e))/(0.1e); mu_b =
+// mu_q = mu_s = 0 in eos().  The splines are the GPU's (vh_eos_table).
+// Outside the table the reference's GSL call aborts; here the program exits
+// with a message.  EoSs(desc) wraps a descriptor's columns (copied).
+class EoSs : public EoS {
+ public:
+  EoSs(std::string fname, int ncols);
+  explicit EoSs(const vh_eos_t &desc);
+  void eos(double e, double nb, double nq, double ns, double &T, double &mub, double &muq,
+           double &mus, double &p) override;
+  double p(double e, double nb, double ns, double nq) override { return p(e); }
+  double p(double e);
+  double dpe(double e);
+  double t(double e);
+  double mu(double e);
+  double cs2(double e) override { return dpe(e); }
+  vh_eos_t device() const override;
+  int rows() const { return n_; }
+
+ private:
+  void build();
+  double eval(int f, double e) const;
+  int n_ = 0;
+  std::vector<double> te_, tp_, tT_, tmu_, tab_;
+};
+
 // con2prim / prim2con on the host (src/rmn.cpp:28-139, 253-263), used by ICs
 // and by host code reading cells; the hot path runs them on the GPU.
 void transformPV(EoS *eos, double Q[7], double &e, double &p, double &nb, double &nq,

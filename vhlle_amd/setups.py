@@ -112,3 +112,41 @@ def full3d_ic(cfg=FULL_3D_FO):
                               eta_plateau=cfg["eta_plateau"], eta_sigma=cfg["eta_sigma"])
     z = np.zeros_like(e)
     return (e, z, z.copy(), z.copy()), int(npart[0])
+
+
+# A crossover equation of state in the TABLE format of EoSs (columns e, p, T):
+# entropy density s = (2 pi^2/45) g(T) T^3 / (hbar c)^3 with g rising from
+# g_h (a massless pion gas) to g_q = 42.25 (the ideal gas of the SIMPLE
+# branch: e = cnst T^4 at high T) as (1 + tanh((T - Tc)/dT))/2; p = int s dT,
+# e = T s - p (thermodynamically consistent, so s = (e + p)/T as EoS::s
+# computes it).  Rows log-spaced in e from e_min to e_max plus e = 0.
+HBARC = 0.197326968
+CROSSOVER_EOS = dict(Tc=0.170, dT=0.020, g_h=3.0, g_q=42.25, n=800, e_min=1e-8, e_max=1e4)
+
+
+def crossover_eos_columns(cfg=CROSSOVER_EOS):
+    Tc, dT, gh, gq = cfg["Tc"], cfg["dT"], cfg["g_h"], cfg["g_q"]
+    k = 2 * np.pi ** 2 / 45 / HBARC ** 3
+    T = np.geomspace(1e-4, 5.0, 400001)
+    g = gh + (gq - gh) * 0.5 * (1 + np.tanh((T - Tc) / dT))
+    s = k * g * T ** 3
+    # p(T_0) of the pion gas below the grid, then the trapezoid integral of s
+    p = k * gh * T[0] ** 4 / 4 + np.concatenate(([0.0], np.cumsum(0.5 * (s[1:] + s[:-1]) *
+                                                                  np.diff(T))))
+    e = T * s - p
+    et = np.concatenate(([0.0], np.geomspace(cfg["e_min"], cfg["e_max"], cfg["n"] - 1)))
+    Tt = np.interp(et, e, T, left=0.0)
+    pt = np.interp(et, e, p, left=0.0)
+    Tt[0] = pt[0] = 0.0
+    return et, pt, Tt
+
+
+def crossover_eos(cfg=CROSSOVER_EOS):
+    from vhlle_amd import abi
+    return abi.EosTable(*crossover_eos_columns(cfg))
+
+
+def write_eos_table(path, cfg=CROSSOVER_EOS):
+    """The table file EoSs reads (three columns e p T)."""
+    e, p, T = crossover_eos_columns(cfg)
+    np.savetxt(path, np.column_stack([e, p, T]), fmt="%.17e")
