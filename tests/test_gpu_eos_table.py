@@ -125,3 +125,61 @@ def test_out_of_table_raises_status(gpu):
     st, _ = ctx.run(1.0, 0.02, 2)
     ctx.close()
     assert st & abi.ST_EOS_RANGE
+
+
+@pytest.fixture(scope="module")
+def table_model(table):
+    """The con2prim MLP trained on states of the crossover EoS (Gubser-shaped
+    runs with the table, half augmented), on the GPU."""
+    Q, prim = abi.harvest(1 << 20, seed=21, steps=100, eos=table)
+    p, hist = abi.mlp_train(abi.mlp_init(5), Q, prim, batch=512, epochs=6, lr=2e-3,
+                            lr_final=1e-6, seed=3)
+    return p, hist
+
+
+def test_learned_c2p_on_table(gpu, table, table_model):
+    """Without a closed form the Newton start matters: the MLP guess against
+    v = 0.5 on held-out states of the table EoS, both polished to the
+    1e-12 Q_t residual of the table's p(e)."""
+    p, hist = table_model
+    assert hist[-1] < 0.05 * hist[0]
+    Q, _ = abi.harvest(200000, seed=77, steps=100, eos=table)
+    res = {}
+    for name, start in (("mlp", abi.START_MLP), ("half", abi.START_HALF)):
+        out, it, Qa = abi.c2p_learned(p, Q, start, eos=table)
+        m = Qa[:, 0] > 0
+        M = np.linalg.norm(Qa[:, 1:4], axis=1)
+        v = np.linalg.norm(out[:, 5:8], axis=1)
+        pe = table.spline(0, np.maximum(out[:, 0], 0.0))
+        resid = np.abs((Qa[:, 0] + pe) * v - M)
+        assert np.all(resid[m] <= 2e-12 * Qa[m, 0]), name
+        res[name] = (np.mean(np.where(it >= 0, it, 0)), np.mean(it < 0))
+    print(f"table EoS: polish updates / fallback rate: MLP {res['mlp']}, v=0.5 {res['half']}",
+          flush=True)
+    assert res["mlp"][1] <= 1e-3
+    assert res["mlp"][0] < 0.75 * res["half"][0]
+
+
+def test_mixed_learned_with_table(gpu, table, table_model):
+    p, _ = table_model
+    nx, ny, nz = 40, 40, 16
+    xl, yl, zl = (-5.0, 5.0), (-5.0, 5.0), (-1.5, 1.5)
+    e, vx, vy, vz = viscous_ic(nx, ny, nz, xl, yl, zl)
+    out = {}
+    for name, prec, mode in (("fp64", abi.PREC_FP64, abi.C2P_REFERENCE),
+                             ("learned", abi.PREC_MIXED, abi.C2P_LEARNED),
+                             ("newton", abi.PREC_MIXED, abi.C2P_NEWTON)):
+        ctx = abi.Context(nx, ny, nz, xl, yl, zl, trc=abi.make_trc(etaS=0.08), precision=prec,
+                          eos=table)
+        if mode != abi.C2P_REFERENCE:
+            ctx.set_c2p(mode, p if mode == abi.C2P_LEARNED else None)
+        ctx.set_prim(0.6, e, vx, vy, vz)
+        st, _ = ctx.run(0.6, 0.02, 30)
+        assert st == 0
+        out[name] = ctx.download()
+        ctx.close()
+    qs = np.max(np.abs(out["fp64"]["Q"][0]))
+    for k in ("learned", "newton"):
+        eT = np.max(np.abs(out[k]["Q"][:4] - out["fp64"]["Q"][:4])) / qs
+        print(f"table EoS {k}: T^tau-mu rel err {eT:.2e}", flush=True)
+        assert eT < 2e-6

@@ -61,6 +61,16 @@ struct SplineTab {
     const double delx = v - x[i];
     return y[i] + delx * (b[i] + delx * (c[i] + delx * d[i]));
   }
+  // f and its derivative at v (one interval search)
+  __device__ inline double eval_d(int f, double v, double &dy, bool &out_of_range) const {
+    out_of_range = !(v >= x[0] && v <= x[n - 1]);
+    const int i = interval(v);
+    const double *y = x + (long)(1 + 4 * f) * n;
+    const double *b = y + n, *c = y + 2 * n, *d = y + 3 * n;
+    const double delx = v - x[i];
+    dy = b[i] + delx * (2. * c[i] + 3. * d[i] * delx);
+    return y[i] + delx * (b[i] + delx * (c[i] + delx * d[i]));
+  }
 };
 
 // Device EoS (kind VH_EOS_IDEAL or VH_EOS_TABLE, include/vhlle_gpu.h).
@@ -91,6 +101,19 @@ struct DevEos {
     const double mub = 0., muq = 0., mus = 0.;
     if (T_ > 0.0) return (e + p_ - mub * nb - muq * nq - mus * ns) / T_;
     return 0.;
+  }
+  // p and the exact dp/de (the spline's derivative; 1/3 for the ideal gas),
+  // for the Newton updates of the con2prim polish.  (The reference's own
+  // iteration keeps the estimate 1/3 for every EoS, src/rmn.cpp:35.)
+  __device__ double p_dpde(double e, double &dp) const {
+    if (kind == VH_EOS_TABLE) {
+      bool out;
+      const double v = tab.eval_d(0, e, dp, out);
+      if (out && st) atomicOr(st, VH_ST_EOS_RANGE);
+      return v;
+    }
+    dp = 1. / 3.;
+    return e / 3.;
   }
   // EoSs::cs2(e) = dpe(e) (src/eos.cpp:89-96; 1/3 for the ideal gas)
   __device__ double cs2e(double e) const {

@@ -254,9 +254,17 @@ __device__ __forceinline__ PrimF c2p_newton_f32(const DevEos &eos, const double 
 #pragma unroll 1
   for (int it = 0; it < 16; it++) {
     const float eh = y + x * w;
-    const float ph = eos.pf(eh * Qtf) * iQt;
+    float ph, dpde;
+    if (eos.kind == VH_EOS_TABLE) {  // one table search for p and dp/de
+      double dp;
+      ph = (float)eos.p_dpde((double)(eh * Qtf), dp) * iQt;
+      dpde = (float)dp;
+    } else {
+      ph = eos.pf(eh * Qtf) * iQt;
+      dpde = eos.cs2f(eh * Qtf);
+    }
     const float f = y - w + ph * (1.f - w);
-    const float df = -(1.f + ph) + eos.cs2f(eh * Qtf) * x * (1.f - w);
+    const float df = -(1.f + ph) + dpde * x * (1.f - w);
     float wn = w - f * frcp(df);
     wn = fminf(fmaxf(wn, y), 1.f);
     const float dw = fabsf(wn - w);

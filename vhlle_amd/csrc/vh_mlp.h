@@ -126,18 +126,20 @@ __device__ inline float2 mlp_wave(const Packed &w, const float f[5]) {
 }
 
 // Newton polish of the con2prim root.  Same equation f(v) = (Q_t + p) v - M,
-// derivative estimate, bracket and Newton/bisection safeguard as transformPV
-// (src/rmn.cpp:72-117), but started from v0 and stopped on the residual
+// bracket and Newton/bisection safeguard as transformPV (src/rmn.cpp:72-117),
+// but with the exact derivative f' = Q_t + p - M v dp/de (transformPV keeps
+// dp/de = 1/3, which for a non-conformal table EoS makes the iteration
+// linearly convergent), started from v0 and stopped on the residual
 // |f| <= tol Q_t instead of |dv| < 1e-5.  Same vacuum, rest and |Q| > Q_t
 // branches (src/rmn.cpp:47-70).  Returns the number of updates, or -1 if the
 // residual was not reached within maxit (the caller then falls back to the
 // reference iteration).  q is rescaled in place as transformPV does.
 __device__ __forceinline__ int c2p_polish(const DevEos &eos, double q[7], double v0, Prim &r,
                                           double tol, int maxit) {
-  const double dpe = 1. / 3.;
   const double Qt = q[T_];
   double Qx = q[X_], Qy = q[Y_], Qz = q[Z_];
   double M = sqrt(Qx * Qx + Qy * Qy + Qz * Qz);
+  double dpe = 1. / 3.;  // dp/de at the current e (exact for a table EoS)
   double e = 0., p = 0., vx = 0., vy = 0., vz = 0., nb = 0., nq = 0., ns = 0.;
   int it = 0;
   if (Qt <= 0.) {
@@ -158,7 +160,7 @@ __device__ __forceinline__ int c2p_polish(const DevEos &eos, double q[7], double
     double vl = 0., vh = 1.;
     e = Qt - M * v;
     if (e < 0.) e = 0.;
-    p = eos.p(e);
+    p = eos.p_dpde(e, dpe);
     double f = (Qt + p) * v - M;
     double df = (Qt + p) - M * v * dpe;
     if (f > 0.)
@@ -182,7 +184,7 @@ __device__ __forceinline__ int c2p_polish(const DevEos &eos, double q[7], double
       it++;
       e = Qt - M * v;
       if (e < 0.) e = 0.;
-      p = eos.p(e);
+      p = eos.p_dpde(e, dpe);
       f = (Qt + p) * v - M;
       df = (Qt + p) - M * v * dpe;
       if (f > 0.)
