@@ -34,13 +34,17 @@ events recorded on the context's stream around every launch during a second,
 identical pass.
 
 python bench.py [--gpus 1] [--steps 300] [--warmup 5] [--variants a,b] [--no-cpu]
-                [--no-batch] [--no-full3d] [--quick]
+                [--no-batch] [--no-full3d] [--no-table] [--quick]
 
 Also measured (field "batch_ebe"): 256 event-by-event boost-invariant grids of
 sides 100..160 with hot-spot initial states in one batch context, against the
 same events run one context at a time; (field "full_3d_freezeout") the
 300x300x150 hot-spot run, 600 steps with the freeze-out surface every 5
-steps, FP64 against FP32 fluxes (element count and sum of dsigma_mu).
+steps, FP64 against FP32 fluxes (element count and sum of dsigma_mu); (field
+"table_eos") the 200x200x100 workload with a tabulated crossover EoS, where
+con2prim has no closed form: FP64 against mixed precision with the Newton
+from v = 0.5 and from the learned guess, and the guess's Newton-update
+ablation.
 """
 import argparse
 import json
@@ -373,6 +377,70 @@ def full3d_bench(c2p_mixed="closed"):
                 dsigma_rel_diff=[float(x) for x in np.abs(db - da) / np.abs(da).max()])
 
 
+def table_eos_bench(args):
+    """The 3-D viscous workload (200x200x100) with the crossover EoS table
+    (p/e down to 0.10 near T = 170 MeV; no closed-form con2prim): FP64 with the
+    reference iteration against mixed precision with the FP32 Newton from
+    v = 0.5 and from the learned guess (an MLP trained on the GPU on states of
+    the table), the Newton-update ablation of the guess on held-out states,
+    and the T^{tau mu} error of each mixed run against FP64 after the timed
+    steps (over e_fp64 > 0.5)."""
+    from vhlle_amd import abi, setups
+    cfg = setups.VISCOUS_3D
+    table = setups.crossover_eos()
+    N = cfg["nx"] * cfg["ny"] * cfg["nz"]
+    steps = min(args.steps, 100)
+    ic = setups.gaussian3d_ic(cfg["nx"], cfg["ny"], cfg["nz"], cfg["xlim"], cfg["ylim"],
+                              cfg["zlim"], e0=cfg["e0"], sx=cfg["sx"], sy=cfg["sy"],
+                              eta_plateau=cfg["eta_plateau"], eta_sigma=cfg["eta_sigma"])
+    t0 = time.perf_counter()
+    Q, prim = abi.harvest(C2P_PAIRS, seed=2024, steps=200, amps=(10.0, 1.0), augment=0.5,
+                          vmax=0.99, eos=table)
+    params, hist = abi.mlp_train(abi.mlp_init(7), Q, prim, batch=512, epochs=C2P_EPOCHS,
+                                 lr=2e-3, lr_final=1e-6, seed=7)
+    t_train = time.perf_counter() - t0
+    Qh, _ = abi.harvest(1 << 18, seed=4242, steps=200, amps=(10.0, 1.0), augment=0.5,
+                        vmax=0.99, eos=table)
+    ablation = {}
+    for name, start in (("mlp_start", abi.START_MLP), ("v05_start", abi.START_HALF)):
+        _, it, _ = abi.c2p_learned(params, Qh, start, eos=table)
+        ablation[name] = dict(mean_updates=round(float(np.mean(np.where(it >= 0, it, 0))), 4),
+                              fallback_rate=float(np.mean(it < 0)))
+    runs = {}
+    for name, prec, c2p in (("fp64", abi.PREC_FP64, abi.C2P_REFERENCE),
+                            ("mixed_newton", abi.PREC_MIXED, abi.C2P_NEWTON),
+                            ("mixed_learned", abi.PREC_MIXED, abi.C2P_LEARNED)):
+        ctx = abi.Context(cfg["nx"], cfg["ny"], cfg["nz"], cfg["xlim"], cfg["ylim"], cfg["zlim"],
+                          trc=abi.make_trc(etaS=cfg["etaS"]), precision=prec, eos=table)
+        if c2p != abi.C2P_REFERENCE:
+            ctx.set_c2p(c2p, params if c2p == abi.C2P_LEARNED else None)
+        ctx.set_prim(cfg["tau0"], *ic)
+        ctx.run(cfg["tau0"], cfg["dt"], 3)
+        ctx.set_prim(cfg["tau0"], *ic)
+        t0 = time.perf_counter()
+        st, tau = ctx.run(cfg["tau0"], cfg["dt"], steps)
+        el = time.perf_counter() - t0
+        runs[name] = dict(state=ctx.download(), elapsed=el, status=st, tau=tau)
+        ctx.close()
+        print(f"[bench] table EoS {name}: {steps} steps in {el:.3f} s, status {st}",
+              file=sys.stderr, flush=True)
+    ref = runs["fp64"]
+    e_ref = abi.c2p((ref["state"]["Q"].T / ref["tau"]).copy(), eos=table)[0][:, 0]
+    mask = e_ref > E_CRIT
+    out = dict(workload="viscous_3d_200x200x100 with the crossover EoS table (setups.CROSSOVER_EOS)",
+               cells=N, steps=steps, train_s=round(t_train, 2), train_loss=[float(h) for h in hist],
+               ablation=ablation, variants={})
+    for name, r in runs.items():
+        v = N * steps / r["elapsed"]
+        d = dict(value=round(v, 1), unit="cell-updates/s",
+                 ms_per_step=round(r["elapsed"] / steps * 1e3, 4), status=r["status"],
+                 speedup_vs_fp64=round(ref["elapsed"] / r["elapsed"], 3))
+        if name != "fp64":
+            d["err_T_vs_fp64"], d["err_pi_vs_fp64"] = state_errors(r["state"], ref["state"], mask)
+        out["variants"][name] = d
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -382,6 +450,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-full3d", action="store_true")
+    ap.add_argument("--no-table", action="store_true")
     ap.add_argument("--quick", action="store_true",
                     help="skip the error runs (profiling); errors reported as null")
     args = ap.parse_args()
@@ -483,6 +552,8 @@ def main():
         out["batch_ebe"] = [batch_bench(args, c) for c in ("closed", "learned")]
     if not args.no_full3d:
         out["full_3d_freezeout"] = full3d_bench("closed")
+    if not args.no_table:
+        out["table_eos"] = table_eos_bench(args)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(cfg)
     print(json.dumps(out), flush=True)
 

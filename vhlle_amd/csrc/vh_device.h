@@ -38,10 +38,25 @@ __device__ inline double smax(double a, double b) { return (a < b) ? b : a; }
 // with c_i = y''(x_i)/2 from the natural tridiagonal system and b_i, d_i from
 // c_i, c_i+1 (computed once on the host).  Interval search as gsl_interp_bsearch
 // (x_i <= v < x_i+1; v = x_n-1 falls in the last interval).
+// The search starts from a hint: hint[k] is the interval of the k-th of K
+// bins uniform in log2(e) over [2^l0, x_n-1]; a walk of a step or two then
+// lands on the interval the bisection would return (x_i <= v < x_i+1, or the
+// end interval outside the table), so the result does not depend on the hint.
 struct SplineTab {
   const double *x;  // n nodes, then per function: y, b, c, d (n each)
   int n;
+  const int *hint;  // K bins (null: bisection)
+  int K;
+  float l0, inv;
   __device__ inline int interval(double v) const {
+    if (hint) {
+      const float kf = (__log2f((float)v) - l0) * inv;
+      const int k = kf > 0.f ? (kf < (float)(K - 1) ? (int)kf : K - 1) : 0;
+      int i = hint[k];
+      while (i < n - 2 && x[i + 1] <= v) i++;
+      while (i > 0 && x[i] > v) i--;
+      return i;
+    }
     int lo = 0, hi = n - 1;
     while (hi > lo + 1) {
       const int mid = (hi + lo) / 2;

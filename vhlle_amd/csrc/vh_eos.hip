@@ -85,6 +85,9 @@ int make_dev_eos(const vh_eos_t *eos, int *st, DevEos &de, double **dbuf) {
             (0.197326968 * 0.197326968 * 0.197326968);
   de.tab.x = nullptr;
   de.tab.n = 0;
+  de.tab.hint = nullptr;
+  de.tab.K = 0;
+  de.tab.l0 = de.tab.inv = 0.f;
   de.st = st;
   if (eos->kind == VH_EOS_IDEAL) return VH_OK;
   if (eos->kind != VH_EOS_TABLE) return VH_ERR_ARG;
@@ -92,15 +95,39 @@ int make_dev_eos(const vh_eos_t *eos, int *st, DevEos &de, double **dbuf) {
   int n = 0;
   const int rc = eos_table_host(*eos, buf, n);
   if (rc != VH_OK) return rc;
-  if (hipMalloc(dbuf, buf.size() * sizeof(double)) != hipSuccess) return VH_ERR_HIP;
-  if (hipMemcpy(*dbuf, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice) !=
-      hipSuccess) {
+  // search hints: K bins uniform in log2(e) from the first positive node
+  const double *x = buf.data();
+  const int K = 4096;
+  const double xlo = x[0] > 0. ? x[0] : x[1];
+  const double l0 = std::log2(xlo), l1 = std::log2(x[n - 1]);
+  std::vector<int> hint(K);
+  for (int k = 0; k < K; k++) {
+    const double v = std::exp2(l0 + (l1 - l0) * k / K);
+    int lo = 0, hi = n - 1;
+    while (hi > lo + 1) {
+      const int mid = (hi + lo) / 2;
+      if (x[mid] > v)
+        hi = mid;
+      else
+        lo = mid;
+    }
+    hint[k] = lo;
+  }
+  const size_t nb = buf.size() * sizeof(double), bytes = nb + K * sizeof(int);
+  if (hipMalloc(dbuf, bytes) != hipSuccess) return VH_ERR_HIP;
+  if (hipMemcpy(*dbuf, buf.data(), nb, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(reinterpret_cast<char *>(*dbuf) + nb, hint.data(), K * sizeof(int),
+                hipMemcpyHostToDevice) != hipSuccess) {
     (void)hipFree(*dbuf);
     *dbuf = nullptr;
     return VH_ERR_HIP;
   }
   de.tab.x = *dbuf;
   de.tab.n = n;
+  de.tab.hint = reinterpret_cast<const int *>(reinterpret_cast<const char *>(*dbuf) + nb);
+  de.tab.K = K;
+  de.tab.l0 = (float)l0;
+  de.tab.inv = (float)(K / (l1 - l0));
   return VH_OK;
 }
 
