@@ -32,6 +32,16 @@ def test_params_reader_echo_cpu(tmp_path):
     assert "eosType 1" in r.stdout
 
 
+def test_missing_eos_table_exits_1_cpu(tmp_path):
+    """eosType 0 reads the table like EoSs (src/eos.cpp:27-40): a missing file
+    is reported and the run exits with 1 before any device work."""
+    p = _params(tmp_path, eosType=0, eosFile=str(tmp_path / "none.dat"), nx=11)
+    r = subprocess.run([EXE, "-params", p, "-outputDir", str(tmp_path)], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 1
+    assert "can't open input file" in r.stderr
+
+
 def test_no_cli_params_exits_1_cpu():
     r = subprocess.run([EXE], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "no CL params" in r.stdout
@@ -128,3 +138,39 @@ def test_driver_freezeout_file(tmp_path):
     print(f"{len(rows)} rows; last counts {counts[-5:]}", flush=True)
     assert rows.shape[1] == 27 and len(rows) == sum(counts) and counts[-1] == 0
     assert np.all(np.isfinite(rows)) and np.all(rows[:, 8] >= 1.0)
+
+
+@pytest.mark.gpu
+def test_driver_table_eos_matches_oracle(tmp_path, oracle):
+    """eosType 0 with a table file (the crossover EoS written in EoSs's
+    three-column format): a 3-D viscous run through the params file against
+    the oracle with the same table."""
+    tab = tmp_path / "eos.dat"
+    setups.write_eos_table(str(tab))
+    nx, ny, nz = 12, 10, 8
+    xl, yl, zl = (-2.75, 2.75), (-2.25, 2.25), (-0.6, 0.6)
+    p = _params(tmp_path, eosType=0, eosFile=str(tab), icModel=10, icE0=20.0, icSigmaX=1.2,
+                icSigmaY=1.0, icEtaPlateau=0.3, icEtaSigma=0.4, nx=nx, ny=ny, nz=nz,
+                xmin=xl[0], xmax=xl[1], ymin=yl[0], ymax=yl[1], etamin=zl[0], etamax=zl[1],
+                tau0=0.5, tauMax=0.6, dtau=0.02, etaS=0.1, tauGridResize=100.0, dumpState=1,
+                stopOnEmptySurface=0)
+    r = subprocess.run([EXE, "-params", p, "-outputDir", str(tmp_path)], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-600:])
+    assert r.returncode == 0, r.stderr
+    Qg = np.fromfile(tmp_path / "state.bin").reshape(7, nx * ny * nz)
+    a = np.loadtxt(tab)
+
+    class Tab:
+        e, p, T, mu = a[:, 0], a[:, 1], a[:, 2], None
+
+    e, vx, vy, vz = setups.gaussian3d_ic(nx, ny, nz, xl, yl, zl, e0=20.0, sx=1.2, sy=1.0,
+                                         eta_plateau=0.3, eta_sigma=0.4)
+    o = oracle.Solver(nx, ny, nz, xl, yl, zl, trc=oracle.make_trc(etaS=0.1), table=Tab)
+    o.set_prim(0.5, e, vx, vy, vz)
+    tau = 0.5
+    while tau < 0.6 + 0.0001:
+        rc, tau = o.run(tau, 0.02, 1)
+        assert rc == 0
+    Qo = o.get_state()["Q"]
+    assert np.max(np.abs(Qg - Qo)) <= 1e-10 * np.max(np.abs(Qo))
