@@ -174,3 +174,56 @@ def test_driver_table_eos_matches_oracle(tmp_path, oracle):
         assert rc == 0
     Qo = o.get_state()["Q"]
     assert np.max(np.abs(Qg - Qo)) <= 1e-10 * np.max(np.abs(Qo))
+
+
+@pytest.mark.gpu
+def test_driver_grid_resize_matches_oracle(tmp_path, oracle):
+    """tauGridResize: at the first step reaching it the driver replaces the
+    grid by one with twice the transverse extent and the same cell counts,
+    importing every second cell around the centre (getCell clamps at the
+    edges) and doubling dtau (src/main.cpp:246-271, 423-427).  The oracle
+    replays the same schedule with the gather done here in numpy."""
+    nx, ny, nz = 16, 14, 6
+    xl, yl, zl = (-3.75, 3.75), (-3.25, 3.25), (-0.5, 0.5)
+    p = _params(tmp_path, eosType=5, icModel=10, icE0=15.0, icSigmaX=1.0, icSigmaY=0.9,
+                icEtaPlateau=0.3, icEtaSigma=0.4, nx=nx, ny=ny, nz=nz, xmin=xl[0], xmax=xl[1],
+                ymin=yl[0], ymax=yl[1], etamin=zl[0], etamax=zl[1], tau0=0.6, tauMax=1.0,
+                dtau=0.05, etaS=0.1, tauGridResize=0.75, dumpState=1, stopOnEmptySurface=0)
+    r = subprocess.run([EXE, "-params", p, "-outputDir", str(tmp_path)], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-800:])
+    assert r.returncode == 0, r.stderr
+    assert "grid resize" in r.stdout
+    Qg = np.fromfile(tmp_path / "state.bin").reshape(7, nx * ny * nz)
+
+    e, vx, vy, vz = setups.gaussian3d_ic(nx, ny, nz, xl, yl, zl, e0=15.0, sx=1.0, sy=0.9,
+                                         eta_plateau=0.3, eta_sigma=0.4)
+    trc = oracle.make_trc(etaS=0.1)
+    o = oracle.Solver(nx, ny, nz, xl, yl, zl, trc=trc)
+    o.set_prim(0.6, e, vx, vy, vz)
+    deta = (zl[1] - zl[0]) / (nz - 1)
+    tau, hdt, resized = 0.6, 0.05, False
+    while True:
+        nsub = 1
+        while 0.05 / nsub > 1.0 * tau * deta:  # the params' dtau, as the reference
+            nsub *= 2
+        rc, tau = o.run(tau, hdt / nsub, nsub)
+        assert rc == 0
+        if tau >= 0.75 and not resized:
+            s = o.get_state()
+            ix = np.clip(2 * (np.arange(nx) - nx // 2) + nx // 2, 0, nx - 1)
+            iy = np.clip(2 * (np.arange(ny) - ny // 2) + ny // 2, 0, ny - 1)
+            def gather(a, ncomp):
+                a = a.reshape(ncomp, nz, ny, nx)
+                return a[:, :, iy][:, :, :, ix].reshape(ncomp, -1)
+            new = {k: gather(v, v.shape[0]) for k, v in s.items()
+                   if k in ("Q", "Qh", "Qprev", "pi", "piH", "Pi", "PiH", "m", "viscCorrCut")}
+            o = oracle.Solver(nx, ny, nz, (2 * xl[0], 2 * xl[1]), (2 * yl[0], 2 * yl[1]), zl,
+                              trc=trc)
+            o.set_state(new)
+            hdt *= 2
+            resized = True
+        if not tau < 1.0 + 0.0001:
+            break
+    Qo = o.get_state()["Q"]
+    assert np.max(np.abs(Qg - Qo)) <= 1e-10 * np.max(np.abs(Qo))

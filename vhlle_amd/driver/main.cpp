@@ -14,6 +14,8 @@
 // 3 Newton from v = 0.5), c2pModel (a file written by -trainC2P);
 // stopOnEmptySurface (1, the reference's loop condition: stop at the first
 // step without freeze-out elements, which for nz < 5 is the first step).
+// tauGridResize: once tau reaches it the grid doubles its transverse extent
+// (expandGrid2x, src/main.cpp:246-271).
 // `vhlle [-params file] -trainC2P file [-seed S]` harvests 4 M conserved ->
 // primitive pairs from Gubser runs with the params' EoS (ideal gas without
 // -params), trains the con2prim MLP on the GPU and writes it.
@@ -169,6 +171,31 @@ double totalEnergy(Fluid *f, double tau) {
 
 }  // namespace
 
+// The grid with twice the transverse extent and the same cell counts: every
+// second cell around the centre is imported (getCell clamps at the edges),
+// dtau doubles, and Hydro continues on the new Fluid (src/main.cpp:246-271).
+Fluid *expandGrid2x(Hydro *h, EoS *eos, EoS *eosH, TransportCoeff *trcoeff) {
+  Fluid *f = h->getFluid();
+  const int nx = f->getNX(), ny = f->getNY(), nz = f->getNZ();
+  if (f->getX(0) + f->getX(nx - 1) > 0.001 || f->getY(0) + f->getY(ny - 1) > 0.001) {
+    cout << "this grid expansion works only with symmetric min/max ranges\n";
+    return f;
+  }
+  Fluid *fnew = new Fluid(eos, eosH, trcoeff, nx, ny, nz, 2.0 * f->getX(0), 2.0 * f->getX(nx - 1),
+                          2.0 * f->getY(0), 2.0 * f->getY(ny - 1), f->getZ(0), f->getZ(nz - 1),
+                          2.0 * h->getDtau(), f->geteCrit());
+  for (int ix = 0; ix < nx; ix++)
+    for (int iy = 0; iy < ny; iy++)
+      for (int iz = 0; iz < nz; iz++)
+        fnew->getCell(ix, iy, iz)
+            ->importVars(f->getCell(2 * (ix - nx / 2) + nx / 2, 2 * (iy - ny / 2) + ny / 2, iz));
+  fnew->adoptOutput(f);
+  h->setFluid(fnew);
+  h->setDtau(2.0 * h->getDtau());
+  delete f;
+  return fnew;
+}
+
 // harvest + GPU training of the con2prim MLP (vh_mlp_harvest, vh_mlp_train)
 int trainC2P(const char *out, unsigned long long seed, const vh_eos_t *eos) {
   const long n = 4L << 20;
@@ -253,9 +280,8 @@ int main(int argc, char **argv) {
     return 0;
   }
   cout << "IC done\n";
-  if (tauResize < tauMax)
-    cout << "note: tauGridResize (expandGrid2x) is not part of this build; ignored\n";
   Hydro *h = new Hydro(f, eos, trcoeff, tau0, dtau);
+  bool resized = false;
   mkdir(outputDir.c_str(), 0755);
   f->initOutput(outputDir.c_str());
   long steps = 0;
@@ -275,6 +301,12 @@ int main(int argc, char **argv) {
     }
     steps += nSub;
     nelements = f->outputSurface(h->getTau(), freezeoutExtend);
+    if (h->getTau() >= tauResize && !resized) {  // src/main.cpp:423-427
+      cout << "grid resize\n";
+      // (the sub-step test keeps using the params' dtau, as the reference's)
+      f = expandGrid2x(h, eos, eos, trcoeff);
+      resized = true;
+    }
   } while (h->getTau() < tauMax + 0.0001 && (nelements > 0 || !stopOnEmptySurface));
   f->closeOutput();
   const double secs = chrono::duration<double>(chrono::steady_clock::now() - t0).count();

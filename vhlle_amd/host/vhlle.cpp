@@ -554,6 +554,12 @@ void Fluid::initOutput(const char *dir) {
   if (!ffreeze) std::cout << "cannot open " << fn << std::endl;
 }
 
+void Fluid::adoptOutput(Fluid *from) {
+  closeOutput();
+  ffreeze = from->ffreeze;
+  from->ffreeze = nullptr;
+}
+
 void Fluid::closeOutput() {
   if (ffreeze) std::fclose(ffreeze);
   ffreeze = nullptr;

@@ -229,6 +229,9 @@ class Fluid {
   // opens dir/freezeout.dat for outputSurface
   void initOutput(const char *dir);
   void closeOutput();
+  // continue writing into another Fluid's freezeout.dat (the reference's
+  // output streams are global, so a Fluid created by expandGrid2x keeps them)
+  void adoptOutput(Fluid *from);
   // ---- precision / con2prim of the device step (new): recreates the device
   // context before the IC is set (VH_PREC_*, VH_C2P_*; model: the con2prim
   // MLP for VH_C2P_LEARNED, VH_MLP_NPARAM floats)
