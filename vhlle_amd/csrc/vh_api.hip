@@ -40,6 +40,7 @@ struct vh_ctx {
   vh::Geom g;
   vh::DevEos eos;
   double *eos_buf = nullptr;  // spline table of a VH_EOS_TABLE EoS (device)
+  vh::fo::Work fo_work;       // freeze-out buffers, kept between calls
   vh::DevTrc trc;
   vh_grid_t grid;
   int precision = 0, c2p_mode = 0;
@@ -560,6 +561,7 @@ int vh_destroy(vh_ctx_t *c) {
   if (c->fo_snap) (void)hipFree(c->fo_snap);
   if (c->mlp) (void)hipFree(c->mlp);
   if (c->eos_buf) (void)hipFree(c->eos_buf);
+  c->fo_work.release();
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
   return VH_OK;
@@ -987,7 +989,7 @@ int vh_freezeout(vh_ctx_t *c, double tau, double dt, double e_crit, int extend, 
   std::vector<double> r;
   long n = 0;
   int err = 0;
-  const int rc = vh::fo::run(a, r, n, err);
+  const int rc = vh::fo::run(a, c->fo_work, r, n, err);
   if (rc != VH_OK) return rc;
   if (nrows) *nrows = n;
   if (a.want_rows)

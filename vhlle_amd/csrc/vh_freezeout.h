@@ -40,8 +40,20 @@ struct FoArgs {
   double *stats;              // host, NSTAT sums
 };
 
+// Device buffers of run(), kept between calls (a context owns one) and grown
+// when a call needs more; release() frees them.
+struct Work {
+  double *e1 = nullptr, *e0 = nullptr, *rows = nullptr, *dstat = nullptr;
+  unsigned char *flag = nullptr;
+  long *cut = nullptr, *sorted = nullptr;
+  int *ncut = nullptr, *cnt = nullptr, *err = nullptr;
+  void *tmp = nullptr;
+  size_t n_cells = 0, n_cubes = 0, n_rows = 0, n_cnt = 0, n_sorted = 0, tmp_bytes = 0;
+  void release();
+};
+
 // elements in the reference's loop order, rows of P.ncol doubles
-int run(const FoArgs &a, std::vector<double> &rows, long &nrows, int &errbits);
+int run(const FoArgs &a, Work &w, std::vector<double> &rows, long &nrows, int &errbits);
 // the construction alone on n hypercubes h (n x 16, [t][x][y][z]) on the GPU
 int cornelius_batch(const double *h, long n, const double *dx, double E0, double *cen,
                     double *nrm, int *nel);

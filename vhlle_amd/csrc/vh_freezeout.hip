@@ -29,6 +29,11 @@ namespace fo {
 
 namespace {
 
+// clears the error word (a kernel, so that no memset is recorded in a graph)
+__global__ void k_fill_err(int *__restrict__ err) {
+  if (threadIdx.x == 0) *err = 0;
+}
+
 // e of every cell at both levels: c2p of Q/tau and of Qlow/tau_low
 __global__ void __launch_bounds__(256)
 k_fo_energy(long N, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qlow,
@@ -57,13 +62,34 @@ __device__ __forceinline__ void cube_origin(const CubeGeom &g, long k, int &ix, 
   ix = (int)(r / g.my) + 2;
 }
 
+// Hypercubes in memory order (x fastest): j -> origin, and the index of the
+// same hypercube in the reference's loop order (ix, iy, iz outer to inner),
+// by which the cut ones are sorted afterwards.
+__device__ __forceinline__ void cube_origin_mem(const CubeGeom &g, long j, int &ix, int &iy,
+                                                int &iz) {
+  const unsigned u = (unsigned)j, mx = (unsigned)g.mx, my = (unsigned)g.my;
+  const unsigned r = u / mx, q = r / my;
+  ix = (int)(u - r * mx) + 2;
+  iy = (int)(r - q * my) + 2;
+  iz = (int)q + 2;
+}
+struct RefKey {
+  CubeGeom g;
+  __host__ __device__ long operator()(long j) const {
+    const long mx = g.mx, my = g.my, mz = g.mz;
+    const long ix = j % mx, r = j / mx, iy = r % my, iz = r / my;
+    return (ix * my + iy) * mz + iz;
+  }
+};
+
+// cut flag of every hypercube, in memory order (coalesced corner loads)
 __global__ void __launch_bounds__(256)
 k_fo_flag(CubeGeom g, const double *__restrict__ e1, const double *__restrict__ e0, double E0,
           unsigned char *__restrict__ flag) {
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (long)g.mx * g.my * g.mz) return;
   int ix, iy, iz;
-  cube_origin(g, k, ix, iy, iz);
+  cube_origin_mem(g, k, ix, iy, iz);
   int above = 0;
   for (int jx = 0; jx < 2; jx++)
     for (int jy = 0; jy < 2; jy++)
@@ -210,15 +236,20 @@ k_fo_cubes(CubeGeom g, FoParams P, DevEos eos, const long *__restrict__ cut,
   }
 }
 
-// sums over all elements in a fixed order (one block): element count, the
-// suspicious ones, dsigma_mu, dV_eff, the EtotSurf and nbSurf increments
-__global__ void __launch_bounds__(1024)
+// sums over all elements in a fixed order: element count, the suspicious
+// ones, dsigma_mu, dV_eff, the EtotSurf and nbSurf increments.  Pass 1: block
+// b reduces the hypercubes of its chunk [b chunk, (b+1) chunk) into part[b];
+// pass 2 (k_fo_stats_sum) adds the parts in block order.
+constexpr int STAT_BLOCKS = 512;
+__global__ void __launch_bounds__(256)
 k_fo_stats(const int *__restrict__ ncut_p, const int *__restrict__ count,
-           const double *__restrict__ rows, int ncol, double ecrit, double *__restrict__ out) {
-  __shared__ double red[1024];
+           const double *__restrict__ rows, int ncol, double ecrit, double *__restrict__ part) {
+  __shared__ double red[256];
   const int ncut = *ncut_p;
+  const long chunk = (ncut + gridDim.x - 1) / gridDim.x;
+  const long t0 = blockIdx.x * chunk, t1 = t0 + chunk < ncut ? t0 + chunk : ncut;
   double acc[NSTAT] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
-  for (long t = threadIdx.x; t < ncut; t += blockDim.x)
+  for (long t = t0 + threadIdx.x; t < t1; t += blockDim.x)
     for (int el = 0; el < count[t]; el++) {
       const double *row = rows + ((long)t * MAXE + el) * ncol;
       const double *aux = row + ncol - AUX;
@@ -236,14 +267,43 @@ k_fo_stats(const int *__restrict__ ncut_p, const int *__restrict__ count,
       if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
       __syncthreads();
     }
-    if (threadIdx.x == 0) out[q] = red[0];
+    if (threadIdx.x == 0) part[(long)q * gridDim.x + blockIdx.x] = red[0];
     __syncthreads();
   }
 }
 
+__global__ void k_fo_stats_sum(const double *__restrict__ part, int nb, double *__restrict__ out) {
+  const int q = threadIdx.x;
+  if (q >= NSTAT) return;
+  double s = 0.;
+  for (int b = 0; b < nb; b++) s += part[(long)q * nb + b];
+  out[q] = s;
+}
+
 }  // namespace
 
-int run(const FoArgs &a, std::vector<double> &rows_out, long &nrows, int &errbits) {
+void Work::release() {
+  for (void *p : {(void *)e1, (void *)e0, (void *)rows, (void *)dstat, (void *)flag, (void *)cut,
+                  (void *)sorted, (void *)ncut, (void *)cnt, (void *)err, tmp})
+    if (p) (void)hipFree(p);
+  *this = Work();
+}
+
+namespace {
+// (re)allocates *p for n elements of T when the capacity cap is smaller
+template <class T>
+hipError_t grow(T **p, size_t &cap, size_t n) {
+  if (*p && cap >= n) return hipSuccess;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  cap = 0;
+  const hipError_t e = hipMalloc(reinterpret_cast<void **>(p), (n ? n : 1) * sizeof(T));
+  if (e == hipSuccess) cap = n;
+  return e;
+}
+}  // namespace
+
+int run(const FoArgs &a, Work &w, std::vector<double> &rows_out, long &nrows, int &errbits) {
   const CubeGeom g{a.nx, a.ny, a.nz, a.nx - 4, a.ny - 4, a.nz - 4, (long)a.nx * a.ny * a.nz};
   nrows = 0;
   errbits = 0;
@@ -251,12 +311,6 @@ int run(const FoArgs &a, std::vector<double> &rows_out, long &nrows, int &errbit
   for (int q = 0; q < NSTAT; q++) a.stats[q] = 0.;
   if (g.mx < 1 || g.my < 1 || g.mz < 1) return VH_OK;  // no interior hypercubes
   const long ncube = (long)g.mx * g.my * g.mz, N = g.N;
-  double *e1 = nullptr, *e0 = nullptr, *rows = nullptr;
-  unsigned char *flag = nullptr;
-  long *ids = nullptr, *cut = nullptr;
-  int *ncut = nullptr, *cnt = nullptr, *err = nullptr;
-  void *tmp = nullptr;
-  size_t tmpb = 0;
   int rc = VH_OK;
   auto ok = [&](hipError_t e) {
     if (e != hipSuccess && rc == VH_OK) {
@@ -266,83 +320,103 @@ int run(const FoArgs &a, std::vector<double> &rows_out, long &nrows, int &errbit
     return rc == VH_OK;
   };
   hipStream_t st = a.stream;
-  if (ok(hipMalloc(&e1, N * sizeof(double))) && ok(hipMalloc(&e0, N * sizeof(double))) &&
-      ok(hipMalloc(&flag, ncube)) && ok(hipMalloc(&cut, ncube * sizeof(long))) &&
-      ok(hipMalloc(&ncut, 2 * sizeof(int)))) {
-    ok(hipMalloc(&err, sizeof(int)));
+  size_t cap_e0 = w.n_cells, cap_flag = w.n_cubes, cap_small = w.ncut ? 2 : 0,
+         cap_err = w.err ? 1 : 0, cap_stat = w.dstat ? (size_t)NSTAT * (STAT_BLOCKS + 1) : 0;
+  // the two energy arrays share one capacity (allocated together)
+  if (w.n_cells < (size_t)N) {
+    size_t c1 = 0;
+    ok(grow(&w.e1, c1, N));
+    ok(grow(&w.e0, cap_e0, N));
+    w.n_cells = rc == VH_OK ? (size_t)N : 0;
   }
+  if (w.n_cubes < (size_t)ncube) {
+    size_t c2 = 0;
+    ok(grow(&w.flag, cap_flag, ncube));
+    ok(grow(&w.cut, c2, ncube));
+    w.n_cubes = rc == VH_OK ? (size_t)ncube : 0;
+  }
+  if (!w.ncut) ok(grow(&w.ncut, cap_small, 2));
+  if (!w.err) ok(grow(&w.err, cap_err, 1));
+  if (!w.dstat) ok(grow(&w.dstat, cap_stat, (size_t)NSTAT * (STAT_BLOCKS + 1)));
+  // cut hypercubes are selected in memory order as their reference-order
+  // index, then sorted (radix sort over the bits that index needs)
+  int endbit = 1;
+  while ((1L << endbit) < ncube) endbit++;
+  using KeyIt = hipcub::TransformInputIterator<long, RefKey, hipcub::CountingInputIterator<long>>;
+  const KeyIt keys(hipcub::CountingInputIterator<long>(0), RefKey{g});
   if (rc == VH_OK) {
     hipLaunchKernelGGL(k_fo_energy, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, N,
-                       a.eos, a.Q, a.Qlow, a.P.tau, a.P.tau - a.P.dt, e1, e0);
+                       a.eos, a.Q, a.Qlow, a.P.tau, a.P.tau - a.P.dt, w.e1, w.e0);
     hipLaunchKernelGGL(k_fo_flag, dim3((unsigned)((ncube + 255) / 256)), dim3(256), 0, st, g,
-                       e1, e0, a.P.ecrit, flag);
-    ok(hipMemsetAsync(err, 0, sizeof(int), st));
-    hipcub::CountingInputIterator<long> it(0);
-    ok(hipcub::DeviceSelect::Flagged(nullptr, tmpb, it, flag, cut, ncut, ncube, st));
-  }
-  if (rc == VH_OK && ok(hipMalloc(&tmp, tmpb ? tmpb : 1))) {
-    hipcub::CountingInputIterator<long> it(0);
-    ok(hipcub::DeviceSelect::Flagged(tmp, tmpb, it, flag, cut, ncut, ncube, st));
+                       w.e1, w.e0, a.P.ecrit, w.flag);
+    hipLaunchKernelGGL(k_fill_err, dim3(1), dim3(64), 0, st, w.err);
+    size_t tmpb = 0, sortb = 0;
+    ok(hipcub::DeviceSelect::Flagged(nullptr, tmpb, keys, w.flag, w.cut, w.ncut, ncube, st));
+    ok(hipcub::DeviceRadixSort::SortKeys(nullptr, sortb, w.cut, w.cut, (int)ncube, 0, endbit, st));
+    tmpb = std::max(tmpb, sortb);
+    if (w.tmp_bytes < tmpb) {
+      size_t cap = w.tmp_bytes;
+      char *t = static_cast<char *>(w.tmp);
+      ok(grow(&t, cap, tmpb));
+      w.tmp = t;
+      w.tmp_bytes = rc == VH_OK ? tmpb : 0;
+    }
+    if (rc == VH_OK)
+      ok(hipcub::DeviceSelect::Flagged(w.tmp, tmpb, keys, w.flag, w.cut, w.ncut, ncube, st));
   }
   int nc = 0;
   if (rc == VH_OK) {
-    ok(hipMemcpyAsync(&nc, ncut, sizeof(int), hipMemcpyDeviceToHost, st));
+    ok(hipMemcpyAsync(&nc, w.ncut, sizeof(int), hipMemcpyDeviceToHost, st));
     ok(hipStreamSynchronize(st));
   }
   std::vector<int> hcnt;
   if (rc == VH_OK && nc > 0) {
-    const long slots = (long)nc * MAXE;
-    if (ok(hipMalloc(&rows, slots * a.P.ncol * sizeof(double))) &&
-        ok(hipMalloc(&cnt, nc * sizeof(int)))) {
+    const size_t slots = (size_t)nc * MAXE;
+    ok(grow(&w.rows, w.n_rows, slots * a.P.ncol));
+    ok(grow(&w.cnt, w.n_cnt, (size_t)nc));
+    ok(grow(&w.sorted, w.n_sorted, (size_t)nc));
+    if (rc == VH_OK) {
+      size_t sortb = w.tmp_bytes;
+      ok(hipcub::DeviceRadixSort::SortKeys(w.tmp, sortb, w.cut, w.sorted, nc, 0, endbit, st));
+    }
+    if (rc == VH_OK) {
       const unsigned grid = (unsigned)std::min<long>((nc + 63) / 64, 2048);
       if (a.pi_float)
-        hipLaunchKernelGGL(k_fo_cubes<float>, dim3(grid), dim3(64), 0, st, g, a.P, a.eos, cut,
-                           ncut, e1, e0, a.Q, a.Qlow, (const float *)a.pi, (const float *)a.Pi,
-                           rows, cnt, err);
+        hipLaunchKernelGGL(k_fo_cubes<float>, dim3(grid), dim3(64), 0, st, g, a.P, a.eos, w.sorted,
+                           w.ncut, w.e1, w.e0, a.Q, a.Qlow, (const float *)a.pi,
+                           (const float *)a.Pi, w.rows, w.cnt, w.err);
       else
-        hipLaunchKernelGGL(k_fo_cubes<double>, dim3(grid), dim3(64), 0, st, g, a.P, a.eos, cut,
-                           ncut, e1, e0, a.Q, a.Qlow, (const double *)a.pi,
-                           (const double *)a.Pi, rows, cnt, err);
+        hipLaunchKernelGGL(k_fo_cubes<double>, dim3(grid), dim3(64), 0, st, g, a.P, a.eos,
+                           w.sorted, w.ncut, w.e1, w.e0, a.Q, a.Qlow, (const double *)a.pi,
+                           (const double *)a.Pi, w.rows, w.cnt, w.err);
       ok(hipGetLastError());
-      double *dstat = nullptr;
-      if (ok(hipMalloc(&dstat, NSTAT * sizeof(double)))) {
-        hipLaunchKernelGGL(k_fo_stats, dim3(1), dim3(1024), 0, st, ncut, cnt, rows, a.P.ncol,
-                           a.P.ecrit, dstat);
-        ok(hipMemcpyAsync(a.stats, dstat, NSTAT * sizeof(double), hipMemcpyDeviceToHost, st));
-      }
+      hipLaunchKernelGGL(k_fo_stats, dim3(STAT_BLOCKS), dim3(256), 0, st, w.ncut, w.cnt, w.rows,
+                         a.P.ncol, a.P.ecrit, w.dstat);
+      double *sums = w.dstat + (size_t)NSTAT * STAT_BLOCKS;
+      hipLaunchKernelGGL(k_fo_stats_sum, dim3(1), dim3(64), 0, st, w.dstat, STAT_BLOCKS, sums);
+      ok(hipMemcpyAsync(a.stats, sums, NSTAT * sizeof(double), hipMemcpyDeviceToHost, st));
       hcnt.resize(nc);
-      ok(hipMemcpyAsync(hcnt.data(), cnt, nc * sizeof(int), hipMemcpyDeviceToHost, st));
-      ok(hipMemcpyAsync(&errbits, err, sizeof(int), hipMemcpyDeviceToHost, st));
+      ok(hipMemcpyAsync(hcnt.data(), w.cnt, nc * sizeof(int), hipMemcpyDeviceToHost, st));
+      ok(hipMemcpyAsync(&errbits, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
       ok(hipStreamSynchronize(st));
-      if (dstat) (void)hipFree(dstat);
     }
-    if (rc == VH_OK && !a.want_rows) {
+    if (rc == VH_OK) {
       long total = 0;
       for (int k = 0; k < nc; k++) total += hcnt[k];
       nrows = total;
-    } else if (rc == VH_OK) {
-      long total = 0;
-      for (int k = 0; k < nc; k++) total += hcnt[k];
-      rows_out.resize(total * a.P.ncol);
-      std::vector<double> all((size_t)slots * a.P.ncol);
-      ok(hipMemcpy(all.data(), rows, all.size() * sizeof(double), hipMemcpyDeviceToHost));
-      long w = 0;
-      for (int k = 0; k < nc; k++)
-        for (int el = 0; el < hcnt[k]; el++, w++)
-          std::copy(&all[((size_t)k * MAXE + el) * a.P.ncol],
-                    &all[((size_t)k * MAXE + el + 1) * a.P.ncol], &rows_out[w * a.P.ncol]);
-      nrows = total;
+      if (a.want_rows) {
+        rows_out.resize(total * a.P.ncol);
+        std::vector<double> all(slots * a.P.ncol);
+        ok(hipMemcpy(all.data(), w.rows, all.size() * sizeof(double), hipMemcpyDeviceToHost));
+        long wr = 0;
+        for (int k = 0; k < nc; k++)
+          for (int el = 0; el < hcnt[k]; el++, wr++)
+            std::copy(&all[((size_t)k * MAXE + el) * a.P.ncol],
+                      &all[((size_t)k * MAXE + el + 1) * a.P.ncol], &rows_out[wr * a.P.ncol]);
+      }
     }
   }
-  if (e1) (void)hipFree(e1);
-  if (e0) (void)hipFree(e0);
-  if (flag) (void)hipFree(flag);
-  if (cut) (void)hipFree(cut);
-  if (ncut) (void)hipFree(ncut);
-  if (err) (void)hipFree(err);
-  if (tmp) (void)hipFree(tmp);
-  if (rows) (void)hipFree(rows);
-  if (cnt) (void)hipFree(cnt);
+  if (rc != VH_OK) w.release();  // leave no half-grown buffer behind
   return rc;
 }
 
