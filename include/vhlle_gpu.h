@@ -297,6 +297,34 @@ typedef struct {
 int vh_fo_snapshot(vh_ctx_t *ctx, double tau);
 int vh_freezeout(vh_ctx_t *ctx, double tau, double dt, double e_crit, int extend, int lower,
                  double *rows, long max_rows, long *nrows, vh_fo_stats_t *stats);
+
+/* ---- vorticity: the thermal-vorticity gradient dbeta[i][j] = d_i (u^j / T)
+ * (the reference's `vorticity` key: Hydro::enableVorticity, src/hdo.cpp:60-65;
+ * formed in NSquant, src/hdo.cpp:420-611; Cell::setDbeta / getDbeta,
+ * src/cll.h:167-204) ----
+ * vh_set_vorticity(ctx, 1) allocates 16 doubles per cell, sets every cell's
+ * matrix to zero (Cell::enableVorticity) and makes every later viscous step
+ * form it as the reference does, including its sharing of one matrix across
+ * the cells of a step (entries NSquant does not form keep the values of the
+ * cell before in the loop order ix, iy, iz).  0 frees it.  FP64 single-grid
+ * contexts only.  Ideal-fluid steps leave the matrices unchanged, as in the
+ * reference (NSquant runs only in the viscous step). */
+int vh_set_vorticity(vh_ctx_t *ctx, int on);
+/* Cell::getDbeta of every cell after the last step: dbeta, 16 x N SoA, row
+ * 4 i + j = dbeta[i][j] (tau, x, y, eta; the eta row and column as the
+ * reference forms them), clamped to [-100, 100] like Cell::setDbeta. */
+int vh_get_dbeta(vh_ctx_t *ctx, double *dbeta);
+/* vh_freezeout plus, in beta (may be NULL; needs vh_set_vorticity), up to
+ * max_rows records of VH_BETA_COLS doubles in the column order of the
+ * reference's beta.dat (Fluid::outputSurface with vorticity,
+ * src/fld.cpp:663-834): tau, x, y, eta; dsigma_mu (4); u^mu (4); T, mu_B,
+ * mu_Q, mu_S; d_i beta_j in Cartesian coordinates (16, i major: the
+ * corners' dbeta interpolated to the centroid, both indices transformed with
+ * the Milne-to-Cartesian Jacobian, j lowered); e. */
+enum { VH_BETA_COLS = 33 };
+int vh_freezeout_beta(vh_ctx_t *ctx, double tau, double dt, double e_crit, int extend,
+                      int lower, double *rows, double *beta, long max_rows, long *nrows,
+                      vh_fo_stats_t *stats);
 /* The Cornelius construction alone on the GPU (Cornelius::find_surface_4d):
  * n hypercubes h (n x 16 corner values, index [t][x][y][z]), edge lengths
  * dx[4] (t, x, y, z), level E0; cen, nrm: n x 10 x 4 (unused elements zero);

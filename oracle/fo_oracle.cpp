@@ -483,10 +483,13 @@ int or_cornelius(const double *h, long n, const double dx[4], double E0, double 
   return 0;
 }
 
-// Fluid::outputSurface's surface elements (src/fld.cpp:544-830), SoA inputs
+// Fluid::outputSurface's surface elements (src/fld.cpp:544-830), SoA inputs.
+// dbeta (16 x N, may be NULL): the cells' Cell::getDbeta; with it, beta
+// receives the beta.dat records (33 columns, src/fld.cpp:663-834).
 long or_freezeout(const or_fo_grid_t *g, const double *Q, const double *Qlow, const double *pi,
                   const double *Pi, double tau, double dt, double ecrit, int extend,
-                  double *rows, long max_rows, const or_eos_t *eos) {
+                  double *rows, long max_rows, const or_eos_t *eos, const double *dbeta,
+                  double *beta) {
   const int nx = g->nx, ny = g->ny, nz = g->nz;
   const long N = (long)nx * ny * nz;
   const int ncol = extend ? 29 : 27;
@@ -597,6 +600,34 @@ long or_freezeout(const or_fo_grid_t *g, const double *Q, const double *Qlow, co
           if (extend) {
             r[27] = eC;
             r[28] = nbC;
+          }
+          if (dbeta && beta) {
+            // trilinear interpolation of the corners' dbeta (src/fld.cpp:711-727),
+            // then d_i beta_j in Cartesian coordinates with the Milne-to-
+            // Cartesian Jacobian of a covariant vector and g_{jj} lowering the
+            // second index (src/fld.cpp:799-831)
+            double dI[4][4] = {};
+            for (int jx = 0; jx < 2; jx++)
+              for (int jy = 0; jy < 2; jy++)
+                for (int jz = 0; jz < 2; jz++) {
+                  const long c = cell(ix + jx, iy + jy, iz + jz);
+                  for (int a = 0; a < 4; a++)
+                    for (int b = 0; b < 4; b++)
+                      dI[a][b] += dbeta[(4 * a + b) * N + c] * wX[jx] * wY[jy] * wZ[jz];
+                }
+            const double jac[4][4] = {{ch, 0., 0., -sh}, {0., 1., 0., 0.}, {0., 0., 1., 0.},
+                                      {-sh, 0., 0., ch}};
+            const double gmm[4] = {1., -1., -1., -1.};
+            double *b = beta + nrow * 33;
+            for (int k = 0; k < 16; k++) b[k] = r[k];  // tau x y eta, dsigma, u, T mu_B mu_Q mu_S
+            for (int i = 0; i < 4; i++)
+              for (int j = 0; j < 4; j++) {
+                double acc = 0.0;
+                for (int k = 0; k < 4; k++)
+                  for (int l = 0; l < 4; l++) acc += jac[i][k] * jac[j][l] * dI[k][l] * gmm[l];
+                b[16 + 4 * i + j] = acc;
+              }
+            b[32] = eC;
           }
         }
       }

@@ -115,7 +115,9 @@ def lib():
                                    ctypes.POINTER(ctypes.c_int)]
         L.or_freezeout.argtypes = [ctypes.POINTER(FoGrid), dp, dp, dp, dp, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, dp,
-                                   ctypes.c_long, ctypes.POINTER(OrEos)]
+                                   ctypes.c_long, ctypes.POINTER(OrEos), dp, dp]
+        L.or_set_vorticity.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_get_dbeta.argtypes = [ctypes.c_void_p, dp]
         L.or_freezeout.restype = ctypes.c_long
         _LIB = L
     return _LIB
@@ -219,6 +221,15 @@ class Solver:
         rc = lib().or_run(self.h, ctypes.byref(t), dt, nsteps)
         return rc, t.value
 
+    def set_vorticity(self, on=True):
+        lib().or_set_vorticity(self.h, int(on))
+
+    def get_dbeta(self):
+        """Cell::getDbeta of every cell: (16, N), row 4 i + j = d_i (u^j / T)"""
+        out = np.zeros(16 * self.N)
+        lib().or_get_dbeta(self.h, _p(out))
+        return out.reshape(16, self.N)
+
 
 def cornelius(h, dx, E0):
     """Cornelius::find_surface_4d over hypercubes h (n x 2 x 2 x 2 x 2):
@@ -235,16 +246,24 @@ def cornelius(h, dx, E0):
 
 
 def freezeout(shape, spacing, mins, Q, Qlow, pi, Pi, tau, dt, ecrit, extend=False,
-              max_rows=1000000, table=None):
-    """Fluid::outputSurface's element records from SoA state arrays."""
+              max_rows=1000000, table=None, dbeta=None):
+    """Fluid::outputSurface's element records from SoA state arrays; with
+    dbeta (16 x N), also the beta.dat records: (rows, beta)."""
     g = FoGrid(shape[0], shape[1], shape[2], spacing[0], spacing[1], spacing[2], mins[0],
                mins[1], mins[2])
     ncol = 29 if extend else 27
     rows = np.zeros((max_rows, ncol))
     arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (Q, Qlow, pi, Pi)]
     d, keep = _eos(table)
+    db = beta = None
+    if dbeta is not None:
+        db = np.ascontiguousarray(dbeta, dtype=np.float64)
+        beta = np.zeros((max_rows, 33))
     n = lib().or_freezeout(ctypes.byref(g), *[_p(a) for a in arrs], tau, dt, ecrit,
-                           int(extend), _p(rows), max_rows, d)
+                           int(extend), _p(rows), max_rows, d,
+                           None if db is None else _p(db), None if beta is None else _p(beta))
     if n < 0:
         raise RuntimeError("or_freezeout: construction failed")
+    if beta is not None:
+        return rows[:min(n, max_rows)], beta[:min(n, max_rows)]
     return rows[:min(n, max_rows)]

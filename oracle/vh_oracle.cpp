@@ -339,6 +339,7 @@ struct OCell {
   double flux[7];
   double m[3], dm[3];
   double vcc;
+  double db[16];  // Cell::dbeta (src/cll.h:50), [i][j] at 4 i + j, as setDbeta left it
   OCell() {
     std::memset(this, 0, sizeof(*this));
     vcc = 1.;
@@ -389,6 +390,7 @@ struct Solver {
   Trc trc;
   double dt = 0, tau = 0;
   int status = 0;  // 0 ok, 1 NaN exit(1), 2 e>1e10 exit(0)
+  bool vort = false;  // Hydro::enableVorticity (src/hdo.cpp:60-65)
 
   // Cell::getPrimVar{Left,Right,HLeft,HRight} (src/cll.cpp:106-204):
   // MUSCL half-slope reconstruction of Q (or Qh), face state (Q -+ dQ)/tau.
@@ -548,9 +550,14 @@ struct Solver {
     con2prim(eos, q, pr[0], pr[1], pr[2], pr[3], pr[4], pr[5], pr[6], pr[7]);
   }
 
-  // Hydro::NSquant (src/hdo.cpp:381-652), vorticity off.
+  // Hydro::NSquant (src/hdo.cpp:381-652).  db: the matrix dbeta[i][j] =
+  // d_i (u^j / T) shared by all cells of one ISformal pass (NULL: vorticity
+  // off).  Like the reference, the function leaves db untouched when it
+  // returns at the vacuum check, and leaves rows 1-3 untouched when a
+  // neighbour pair has e <= 0 (the Christoffel terms of row 3 are still added),
+  // so such entries keep the values of the cell before in the loop order.
   void nsquant(int ix, int iy, int iz, double piNS[4][4], double &PiNS,
-               double dmu[4][4], double &du) {
+               double dmu[4][4], double &du, double (*db)[4] = nullptr) {
     const double VMIN = 1e-2, UDIFF = 3.0;
     const double gmn[4][4] = {{1, 0, 0, 0}, {0, -1, 0, 0}, {0, 0, -1, 0}, {0, 0, 0, -1}};
     OCell &c = g.at(ix, iy, iz);
@@ -577,6 +584,18 @@ struct Solver {
     double ux0 = ut0 * p0[5], uy0 = ut0 * p0[6], uz0 = ut0 * p0[7];
     double ut1 = 1.0 / std::sqrt(1.0 - p1[5] * p1[5] - p1[6] * p1[6] - p1[7] * p1[7]);
     double ux1 = ut1 * p1[5], uy1 = ut1 * p1[6], uz1 = ut1 * p1[7];
+    double Tv0 = 0., Tv1 = 0., mu_[3], pv;
+    if (db) {  // temperatures of the previous and current states (src/hdo.cpp:420-425)
+      eos.eos(p0[0], Tv0, mu_[0], mu_[1], mu_[2], pv);
+      eos.eos(p1[0], Tv1, mu_[0], mu_[1], mu_[2], pv);
+      // d_tau (u^nu / T) (src/hdo.cpp:451-458)
+      db[0][0] = (ut1 / Tv1 - ut0 / Tv0) / dt;
+      db[0][1] = (ux1 / Tv1 - ux0 / Tv0) / dt;
+      db[0][2] = (uy1 / Tv1 - uy0 / Tv0) / dt;
+      db[0][3] = (uz1 / Tv1 - uz0 / Tv0) / dt;
+      if (e1 <= 0. || e0 <= 0. || Tv1 <= 0. || Tv0 <= 0.)
+        db[0][0] = db[0][1] = db[0][2] = db[0][3] = 0.;
+    }
     double u[4];
     u[0] = 1.0 / std::sqrt(1.0 - ph[5] * ph[5] - ph[6] * ph[6] - ph[7] * ph[7]);
     u[1] = u[0] * ph[5];
@@ -602,6 +621,10 @@ struct Solver {
       prim_of(g.nb(ix, iy, iz, d, -1).Qh, tau, b);
       e1 = a[0];
       e0 = b[0];
+      if (db) {  // neighbour temperatures (src/hdo.cpp:474-481)
+        eos.eos(e1, Tv1, mu_[0], mu_[1], mu_[2], pv);
+        eos.eos(e0, Tv0, mu_[0], mu_[1], mu_[2], pv);
+      }
       if (e1 > 0. && e0 > 0.) {
         ut0 = 1.0 / std::sqrt(1.0 - b[5] * b[5] - b[6] * b[6] - b[7] * b[7]);
         ux0 = ut0 * b[5];
@@ -611,6 +634,15 @@ struct Solver {
         ux1 = ut1 * a[5];
         uy1 = ut1 * a[6];
         uz1 = ut1 * a[7];
+        if (db) {  // d_x, d_y, d_eta (u^nu / T) (src/hdo.cpp:495-503,541-549,587-595)
+          const double den = d == 1 ? g.dx : d == 2 ? g.dy : g.dz * (tau + 0.5 * dt);
+          db[d][0] = 0.5 * (ut1 / Tv1 - ut0 / Tv0) / den;
+          db[d][1] = 0.5 * (ux1 / Tv1 - ux0 / Tv0) / den;
+          db[d][2] = 0.5 * (uy1 / Tv1 - uy0 / Tv0) / den;
+          db[d][3] = 0.5 * (uz1 / Tv1 - uz0 / Tv0) / den;
+          if (e1 <= 0. || e0 <= 0. || Tv1 <= 0. || Tv0 <= 0.)
+            db[d][0] = db[d][1] = db[d][2] = db[d][3] = 0.;
+        }
         if (d < 3) {
           const double h = d == 1 ? g.dx : g.dy;
           dmu[d][0] = 0.25 * (ut1 * ut1 - ut0 * ut0) / u[0] / h;
@@ -645,6 +677,10 @@ struct Solver {
     // Christoffel terms (src/hdo.cpp:606-607)
     dmu[3][0] += u[3] / (tau - 0.5 * dt);
     dmu[3][3] += u[0] / (tau - 0.5 * dt);
+    if (db && T > 0.) {  // src/hdo.cpp:608-611
+      db[3][0] += u[3] / (T * (tau - 0.5 * dt));
+      db[3][3] += u[0] / (T * (tau - 0.5 * dt));
+    }
     // projector Z (src/hdo.cpp:613-628) and sigma contraction (:630-637)
     static thread_local double Z[4][4][4][4];
     for (int i = 0; i < 4; i++)
@@ -676,6 +712,7 @@ struct Solver {
   // Hydro::ISformal (src/hdo.cpp:682-963), vorticity off, no FORMAL_SOLUTION
   void isformal() {
     const double gm[4] = {1., -1., -1., -1.};
+    double dbs[4][4] = {};  // the pass's shared dbeta matrix (src/hdo.cpp:685-687)
     // loop #1: relaxation + sources
     for (int ix = 0; ix < g.nx; ix++)
       for (int iy = 0; iy < g.ny; iy++)
@@ -688,6 +725,8 @@ struct Solver {
           if (e <= 0.) {
             for (int i = 0; i < 10; i++) c.piH0[i] = c.pi0[i] = 0.0;
             c.PiH0 = c.Pi0 = 0.0;
+            if (vort)  // Cell::resetDbeta (src/cll.h:186-196)
+              for (int k = 0; k < 16; k++) c.db[k] = 0.0;
             continue;
           }
           const double gamma = 1.0 / std::sqrt(1.0 - vx * vx - vy * vy - vz * vz);
@@ -697,8 +736,12 @@ struct Solver {
           fl[0] += -(tau - dt) * c.Pi;
           c.add_flux(fl);
           double piNS[4][4], PiNS, dmu[4][4], du;
-          nsquant(ix, iy, iz, piNS, PiNS, dmu, du);
+          nsquant(ix, iy, iz, piNS, PiNS, dmu, du, vort ? dbs : nullptr);
           if (status) return;
+          if (vort)  // Cell::setDbeta: copy clamped to [-100, 100] (src/cll.h:173-183)
+            for (int i = 0; i < 4; i++)
+              for (int j = 0; j < 4; j++)
+                c.db[4 * i + j] = std::max(std::min(dbs[i][j], 100.), -100.);
           double T, mub, muq, mus, p;
           eos.eos(e, T, mub, muq, mus, p);
           double etaS, zetaS;
@@ -1192,6 +1235,20 @@ void or_get_state(void *h, or_state_t *st) {
       if (st->m) st->m[k * N + n] = c.m[k];
     if (st->viscCorrCut) st->viscCorrCut[n] = c.vcc;
   }
+}
+
+void or_set_vorticity(void *h, int on) {
+  Solver *s = static_cast<Solver *>(h);
+  s->vort = on != 0;
+  for (OCell &c : s->g.c)
+    for (int k = 0; k < 16; k++) c.db[k] = 0.;  // Cell::enableVorticity (src/cll.h:167-170)
+}
+
+void or_get_dbeta(void *h, double *out) {
+  Solver *s = static_cast<Solver *>(h);
+  const long N = (long)s->g.c.size();
+  for (long n = 0; n < N; n++)
+    for (int k = 0; k < 16; k++) out[k * N + n] = s->g.c[n].db[k];
 }
 
 void or_set_prim(void *h, double tau, const double *e, const double *nb,

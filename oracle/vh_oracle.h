@@ -66,6 +66,12 @@ void or_set_prim(void *h, double tau, const double *e, const double *nb,
  * 1 = NaN in hlle_flux / source_step / NSquant, 2 = e>1e10 (exit(0) in the
  * reference). */
 int or_run(void *h, double *tau, double dt, int nsteps);
+/* Hydro::enableVorticity (src/hdo.cpp:60-65): from the next viscous step on,
+ * NSquant also forms dbeta[i][j] = d_i (u^j / T) and stores it per cell
+ * (Cell::setDbeta, src/cll.h:173-183); every cell's dbeta is reset to 0. */
+void or_set_vorticity(void *h, int on);
+/* Cell::getDbeta of every cell: out 16 x N, component 4 i + j = dbeta[i][j] */
+void or_get_dbeta(void *h, double *out);
 /* Hydro::setDtau validity test (src/hdo.cpp:67-75): 0 if accepted, 1 if the
  * reference would exit(1). */
 int or_check_dtau(void *h, double dt);
@@ -98,10 +104,12 @@ typedef struct {
 /* Fluid::outputSurface's element records (27 or 29 columns, see
  * fo_oracle.cpp) from SoA Q, Qlow (7 x N), pi (10 x N), Pi (N); returns the
  * element count (rows beyond max_rows are counted, not written), -1 on a
- * construction failure. */
+ * construction failure.  dbeta (16 x N) and beta (max_rows x 33) may be
+ * NULL; with both, beta receives the beta.dat records of the elements. */
 long or_freezeout(const or_fo_grid_t *g, const double *Q, const double *Qlow, const double *pi,
                   const double *Pi, double tau, double dt, double ecrit, int extend,
-                  double *rows, long max_rows, const or_eos_t *eos);
+                  double *rows, long max_rows, const or_eos_t *eos, const double *dbeta,
+                  double *beta);
 #ifdef __cplusplus
 }
 #endif

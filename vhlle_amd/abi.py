@@ -18,6 +18,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 C2P_REFERENCE, C2P_LEARNED, C2P_CLOSED, C2P_NEWTON = 0, 1, 2, 3
 START_MLP, START_HALF = 0, 1
 FO_QPREV, FO_SNAPSHOT = 0, 1
+BETA_COLS = 33  # VH_BETA_COLS: columns of a beta.dat record
 STAGE_AUTO, STAGE_CELLS, STAGE_ROWS = 0, 1, 2
 MLP_NPARAM = 4674
 
@@ -185,6 +186,12 @@ def lib():
                                         ctypes.c_int, ctypes.c_int, _dp, ctypes.c_long,
                                         ctypes.POINTER(ctypes.c_long), ctypes.POINTER(FoStats)]),
         "vh_cornelius": (ctypes.c_int, [_dp, ctypes.c_long, _dp, ctypes.c_double, _dp, _dp, ip]),
+        "vh_set_vorticity": (ctypes.c_int, [vp, ctypes.c_int]),
+        "vh_get_dbeta": (ctypes.c_int, [vp, _dp]),
+        "vh_freezeout_beta": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp,
+                                             _dp, ctypes.c_long, ctypes.POINTER(ctypes.c_long),
+                                             ctypes.POINTER(FoStats)]),
         "vh_eos_table": (ctypes.c_int, [ctypes.POINTER(Eos), _dp]),
         "vh_create_batch": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Grid), ctypes.POINTER(Eos),
                                            ctypes.POINTER(Trc), ctypes.c_int, ctypes.POINTER(vp)]),
@@ -204,7 +211,7 @@ def exported_symbols():
             "vh_mlp_forward", "vh_mlp_grad", "vh_mlp_train", "vh_c2p_learned",
             "vh_set_c2p_model", "vh_mlp_harvest", "vh_set_stage", "vh_get_stage", "vh_create_batch", "vh_hotspot_ic",
             "vh_copy_bandwidth", "vh_fo_snapshot", "vh_freezeout", "vh_cornelius",
-            "vh_eos_table"]
+            "vh_eos_table", "vh_set_vorticity", "vh_get_dbeta", "vh_freezeout_beta"]
 
 
 def _p(a):
@@ -464,6 +471,31 @@ class Context:
         return rows[:min(n.value, max_rows)], dict(
             nelements=st.nelements, nsusp=st.nsusp, dsigma=list(st.dsigma), v_eff=st.v_eff,
             e_surf=st.e_surf, nb_surf=st.nb_surf, err=st.err)
+
+    def set_vorticity(self, on=True):
+        check(lib().vh_set_vorticity(self.h, int(on)), "vh_set_vorticity")
+
+    def get_dbeta(self):
+        """Cell::getDbeta of every cell: (16, N), row 4 i + j = d_i (u^j / T)"""
+        out = np.zeros(16 * self.N)
+        check(lib().vh_get_dbeta(self.h, _p(out)), "vh_get_dbeta")
+        return out.reshape(16, self.N)
+
+    def freezeout_beta(self, tau, dt, ecrit, extend=False, lower=FO_QPREV, max_rows=2000000):
+        """freezeout() plus the beta.dat records (rows of BETA_COLS columns)"""
+        ncol = 29 if extend else 27
+        n = ctypes.c_long(0)
+        st = FoStats()
+        check(lib().vh_freezeout_beta(self.h, tau, dt, ecrit, int(extend), lower, None, None, 0,
+                                      ctypes.byref(n), ctypes.byref(st)), "vh_freezeout_beta")
+        m = max(min(n.value, max_rows), 1)
+        rows, beta = np.zeros((m, ncol)), np.zeros((m, BETA_COLS))
+        if n.value and max_rows > 0:
+            check(lib().vh_freezeout_beta(self.h, tau, dt, ecrit, int(extend), lower, _p(rows),
+                                          _p(beta), min(n.value, max_rows), ctypes.byref(n),
+                                          ctypes.byref(st)), "vh_freezeout_beta")
+        k = min(n.value, max_rows)
+        return rows[:k], beta[:k]
 
     def enable_timing(self, on=True):
         check(lib().vh_enable_timing(self.h, 1 if on else 0), "vh_enable_timing")

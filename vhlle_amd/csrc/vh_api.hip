@@ -16,6 +16,7 @@
 #include "vh_kernels_mixed.h"
 #include "vh_mlp_host.h"
 #include "vh_freezeout.h"
+#include "vh_vort.h"
 
 #define VH_HIP(call)                                                         \
   do {                                                                       \
@@ -62,6 +63,12 @@ struct vh_ctx {
   // guesses (mixed precision only; gv reuses the FP64 path's F[0] storage)
   vh::ml::Packed *mlp = nullptr;
   float *gv = nullptr;
+  // vorticity (vh_set_vorticity): per-cell dbeta entries and op codes written
+  // by the relaxation kernel, resolved on demand (dirty: a step ran since)
+  bool vort = false, vort_dirty = false;
+  double *db = nullptr;
+  unsigned char *dop = nullptr;
+  void *vscr = nullptr;
   // freeze-out snapshot (the lower time level of vh_freezeout)
   double *fo_snap = nullptr;
   double fo_tau = 0.;
@@ -251,7 +258,8 @@ int enqueue_step_fp64(vh_ctx *c, double dt) {
     rc |= timed(c, "visc_prim", [&] { vh::launch_vprim(g, c->eos, c->Qh, c->tp, dt, c->ph, c->pt, st); });
     rc |= timed(c, "visc_relax", [&] {
       vh::launch_visc1(g, c->eos, c->trc, Qo, Qn, c->ph, c->pt, c->m, c->m_full, c->pi, c->Pi,
-                       c->tp, dt, c->pi0, c->piH0, c->Pi0, c->PiH0, c->status, st);
+                       c->tp, dt, c->pi0, c->piH0, c->Pi0, c->PiH0, c->status,
+                       c->vort ? c->db : nullptr, c->dop, st);
     });
     rc |= timed(c, "visc_advect", [&] {
       vh::launch_visc2(g, c->ph, c->pi0, c->piH0, c->Pi0, c->PiH0, c->pi, c->piH, c->Pi, c->PiH,
@@ -559,6 +567,9 @@ int vh_destroy(vh_ctx_t *c) {
   if (c->pool) (void)hipFree(c->pool);
   if (c->gtab) (void)hipFree(c->gtab);
   if (c->fo_snap) (void)hipFree(c->fo_snap);
+  if (c->db) (void)hipFree(c->db);
+  if (c->dop) (void)hipFree(c->dop);
+  if (c->vscr) (void)hipFree(c->vscr);
   if (c->mlp) (void)hipFree(c->mlp);
   if (c->eos_buf) (void)hipFree(c->eos_buf);
   c->fo_work.release();
@@ -829,6 +840,7 @@ static int run_impl(vh_ctx_t *c, double *tau, double dt, int nsteps, int *status
       if (rc != VH_OK) return rc;
     }
   }
+  if (nsteps > 0 && c->vort && c->viscous) c->vort_dirty = true;
   int st = 0;
   VH_HIP(hipMemcpyAsync(tau, c->tp, sizeof(double), hipMemcpyDeviceToHost, c->st));
   VH_HIP(hipMemcpyAsync(&st, c->status, sizeof(int), hipMemcpyDeviceToHost, c->st));
@@ -952,10 +964,64 @@ int vh_fo_snapshot(vh_ctx_t *c, double tau) {
   return VH_OK;
 }
 
+namespace {
+// entries of the last step's vorticity matrix that depend on earlier cells
+int vort_settle(vh_ctx_t *c) {
+  if (!c->vort_dirty) return VH_OK;
+  vh::vort_resolve(c->g, c->db, c->dop, c->vscr, c->st);
+  VH_HIP(hipGetLastError());
+  c->vort_dirty = false;
+  return VH_OK;
+}
+}  // namespace
+
+int vh_set_vorticity(vh_ctx_t *c, int on) {
+  if (!c || c->evh.size() != 1 || c->precision != VH_PREC_FP64) return VH_ERR_ARG;
+  const long N = c->g.N, M = (long)c->g.nx * c->g.ny;
+  if (M > (1L << 28)) return VH_ERR_ARG;  // column indices of the resolver are int
+  VH_HIP(hipStreamSynchronize(c->st));
+  drop_graphs(c);  // the relaxation kernel's arguments change
+  if (c->db) (void)hipFree(c->db);
+  if (c->dop) (void)hipFree(c->dop);
+  if (c->vscr) (void)hipFree(c->vscr);
+  c->db = nullptr;
+  c->dop = nullptr;
+  c->vscr = nullptr;
+  c->vort = c->vort_dirty = false;
+  if (!on) return VH_OK;
+  VH_HIP(hipMalloc(&c->db, 16 * N * sizeof(double)));
+  VH_HIP(hipMalloc(&c->dop, N));
+  VH_HIP(hipMalloc(&c->vscr, vh::vort_scratch_bytes(c->g)));
+  // Cell::enableVorticity: every cell's matrix zero (src/cll.h:167-170)
+  VH_HIP(hipMemsetAsync(c->db, 0, 16 * N * sizeof(double), c->st));
+  VH_HIP(hipStreamSynchronize(c->st));
+  c->vort = true;
+  return VH_OK;
+}
+
+int vh_get_dbeta(vh_ctx_t *c, double *dbeta) {
+  if (!c || !dbeta || !c->vort) return VH_ERR_ARG;
+  int rc = vort_settle(c);
+  if (rc != VH_OK) return rc;
+  const long n = 16 * c->g.N;
+  vh::vort_clamp(c->db, n, c->st);
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipMemcpyAsync(dbeta, c->db, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  VH_HIP(hipStreamSynchronize(c->st));
+  return VH_OK;
+}
+
 int vh_freezeout(vh_ctx_t *c, double tau, double dt, double e_crit, int extend, int lower,
                  double *rows, long max_rows, long *nrows, vh_fo_stats_t *stats) {
+  return vh_freezeout_beta(c, tau, dt, e_crit, extend, lower, rows, nullptr, max_rows, nrows,
+                           stats);
+}
+
+int vh_freezeout_beta(vh_ctx_t *c, double tau, double dt, double e_crit, int extend, int lower,
+                      double *rows, double *beta, long max_rows, long *nrows,
+                      vh_fo_stats_t *stats) {
   if (!c || c->evh.size() != 1 || (lower != VH_FO_QPREV && lower != VH_FO_SNAPSHOT) ||
-      (rows && max_rows < 0))
+      ((rows || beta) && max_rows < 0) || (beta && !c->vort))
     return VH_ERR_ARG;
   if (lower == VH_FO_SNAPSHOT && !c->fo_snap) return VH_ERR_ARG;
   const double edge = lower == VH_FO_SNAPSHOT ? tau - c->fo_tau : dt;
@@ -976,14 +1042,22 @@ int vh_freezeout(vh_ctx_t *c, double tau, double dt, double e_crit, int extend, 
   a.P.zmin = c->grid.zmin;
   a.P.ecrit = e_crit;
   const int rec = extend ? vh::fo::ROW_EXT : vh::fo::ROW_BASE;
-  a.P.ncol = rec + vh::fo::AUX;
+  const bool wb = beta != nullptr && max_rows > 0;
+  a.P.nbeta = wb ? vh::fo::BETA : 0;
+  a.P.ncol = rec + a.P.nbeta + vh::fo::AUX;
+  a.db = nullptr;
+  if (wb) {
+    const int rc = vort_settle(c);
+    if (rc != VH_OK) return rc;
+    a.db = c->db;
+  }
   a.Q = c->Q[c->cur];
   a.Qlow = lower == VH_FO_SNAPSHOT ? c->fo_snap : c->Q[c->cur ^ 1];
   a.pi = c->pi;
   a.Pi = c->Pi;
   a.pi_float = c->precision == VH_PREC_MIXED;
   a.stream = c->st;
-  a.want_rows = rows != nullptr && max_rows > 0;
+  a.want_rows = (rows != nullptr || wb) && max_rows > 0;
   double sums[vh::fo::NSTAT];
   a.stats = sums;
   std::vector<double> r;
@@ -992,9 +1066,16 @@ int vh_freezeout(vh_ctx_t *c, double tau, double dt, double e_crit, int extend, 
   const int rc = vh::fo::run(a, c->fo_work, r, n, err);
   if (rc != VH_OK) return rc;
   if (nrows) *nrows = n;
-  if (a.want_rows)
-    for (long i = 0; i < n && i < max_rows; i++)
-      std::memcpy(rows + i * rec, &r[i * a.P.ncol], rec * sizeof(double));
+  for (long i = 0; a.want_rows && i < n && i < max_rows; i++) {
+    const double *src = &r[i * a.P.ncol];
+    if (rows) std::memcpy(rows + i * rec, src, rec * sizeof(double));
+    if (wb) {  // beta.dat columns (src/fld.cpp:663-834)
+      double *b = beta + i * VH_BETA_COLS;
+      std::memcpy(b, src, 16 * sizeof(double));         // tau x y eta, dsigma, u, T, mu
+      std::memcpy(b + 16, src + rec, 16 * sizeof(double));  // d_i beta_j, Cartesian
+      b[32] = src[a.P.ncol - vh::fo::AUX + 3];              // e at the centroid
+    }
+  }
   if (stats) {
     std::memset(stats, 0, sizeof *stats);
     stats->nelements = (long)sums[0];

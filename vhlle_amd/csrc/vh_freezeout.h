@@ -19,9 +19,14 @@ constexpr int ROW_BASE = 27, ROW_EXT = 29, AUX = 4;
 // EtotSurf and nbSurf increments
 constexpr int NSTAT = 9;
 
+// With the vorticity output, BETA columns follow the record (before AUX):
+// d_i beta_j in Cartesian coordinates, 4 i + j (src/fld.cpp:799-831).
+constexpr int BETA = 16;
+
 struct FoParams {
   double tau, dt, dx, dy, dz, xmin, ymin, zmin, ecrit;
-  int ncol;  // record columns + AUX
+  int ncol;   // record columns + nbeta + AUX
+  int nbeta;  // 0, or BETA with the vorticity output
 };
 
 struct Dx4 {
@@ -35,6 +40,7 @@ struct FoArgs {
   const double *Q, *Qlow;     // upper (tau) and lower (tau - dt) levels, 7 x N
   const void *pi, *Pi;        // pi (10 x N), Pi (N) of the upper level
   int pi_float;               // pi, Pi stored in FP32
+  const double *db;           // the cells' dbeta (16 x N, clamped on read) with nbeta
   hipStream_t stream;
   bool want_rows;             // copy the records to the host
   double *stats;              // host, NSTAT sums

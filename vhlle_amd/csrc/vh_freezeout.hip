@@ -23,6 +23,7 @@
 #include "../../include/vhlle_gpu.h"
 #include "vh_cornelius.h"
 #include "vh_freezeout.h"
+#include "vh_vort.h"
 
 namespace vh {
 namespace fo {
@@ -114,8 +115,8 @@ k_fo_cubes(CubeGeom g, FoParams P, DevEos eos, const long *__restrict__ cut,
            const int *__restrict__ ncut_p, const double *__restrict__ e1,
            const double *__restrict__ e0, const double *__restrict__ Q,
            const double *__restrict__ Qlow, const PT *__restrict__ pi,
-           const PT *__restrict__ Pi, double *__restrict__ rows, int *__restrict__ count,
-           int *__restrict__ err) {
+           const PT *__restrict__ Pi, const double *__restrict__ db, double *__restrict__ rows,
+           int *__restrict__ count, int *__restrict__ err) {
   const int ncut = *ncut_p;
   const long N = g.N;
   const double dxs[4] = {P.dt, P.dx, P.dy, P.dz};
@@ -218,9 +219,35 @@ k_fo_cubes(CubeGeom g, FoParams P, DevEos eos, const long *__restrict__ cut,
                       2. * sh * ch * piC[sym(0, 3)];
       for (int i = 0; i < 10; i++) row[16 + i] = pc[i];
       row[26] = PiC;
-      if (P.ncol > ROW_BASE) {
+      const int rec = P.ncol - AUX - P.nbeta;
+      if (rec > ROW_BASE) {
         row[27] = eC;
         row[28] = nbC;
+      }
+      if (P.nbeta) {
+        // trilinear dbeta of the upper level's cells (Cell::getDbeta: the
+        // clamped copy), then the Milne-to-Cartesian Jacobian of a covariant
+        // vector on both indices, the second index lowered by g
+        // (src/fld.cpp:703-727,799-831)
+        double dI[16];
+        for (int k = 0; k < 16; k++) dI[k] = 0.;
+        for (int jx = 0; jx < 2; jx++)
+          for (int jy = 0; jy < 2; jy++)
+            for (int jz = 0; jz < 2; jz++) {
+              const long c = cc[jx][jy][jz];
+              for (int k = 0; k < 16; k++)
+                dI[k] += vort::clamp100(db[k * N + c]) * wX[jx] * wY[jy] * wZ[jz];
+            }
+        const double jac[4][4] = {{ch, 0., 0., -sh}, {0., 1., 0., 0.}, {0., 0., 1., 0.},
+                                  {-sh, 0., 0., ch}};
+        const double gmm[4] = {1., -1., -1., -1.};
+        for (int i = 0; i < 4; i++)
+          for (int j = 0; j < 4; j++) {
+            double acc = 0.0;
+            for (int k = 0; k < 4; k++)
+              for (int l = 0; l < 4; l++) acc += jac[i][k] * jac[j][l] * dI[4 * k + l] * gmm[l];
+            row[rec + 4 * i + j] = acc;
+          }
       }
       // diagnostics: dVEff, EtotSurf increment, nb flux, e_C
       double dV = 0.0;
@@ -384,11 +411,11 @@ int run(const FoArgs &a, Work &w, std::vector<double> &rows_out, long &nrows, in
       if (a.pi_float)
         hipLaunchKernelGGL(k_fo_cubes<float>, dim3(grid), dim3(64), 0, st, g, a.P, a.eos, w.sorted,
                            w.ncut, w.e1, w.e0, a.Q, a.Qlow, (const float *)a.pi,
-                           (const float *)a.Pi, w.rows, w.cnt, w.err);
+                           (const float *)a.Pi, a.db, w.rows, w.cnt, w.err);
       else
         hipLaunchKernelGGL(k_fo_cubes<double>, dim3(grid), dim3(64), 0, st, g, a.P, a.eos,
                            w.sorted, w.ncut, w.e1, w.e0, a.Q, a.Qlow, (const double *)a.pi,
-                           (const double *)a.Pi, w.rows, w.cnt, w.err);
+                           (const double *)a.Pi, a.db, w.rows, w.cnt, w.err);
       ok(hipGetLastError());
       hipLaunchKernelGGL(k_fo_stats, dim3(STAT_BLOCKS), dim3(256), 0, st, w.ncut, w.cnt, w.rows,
                          a.P.ncol, a.P.ecrit, w.dstat);

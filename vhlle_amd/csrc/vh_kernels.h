@@ -28,7 +28,7 @@ void launch_visc1(const Geom &g, const DevEos &eos, const DevTrc &trc, const dou
                   const double *Qn, const double *ph, const double *pt, const double *m,
                   int m_full, const double *pi, const double *Pi, const double *tp, double dt,
                   double *pi0, double *piH0, double *Pi0, double *PiH0, int *status,
-                  hipStream_t st);
+                  double *db, unsigned char *dop, hipStream_t st);
 void launch_visc2(const Geom &g, const double *ph, const double *pi0, const double *piH0,
                   const double *Pi0, const double *PiH0, double *pi, double *piH, double *Pi,
                   double *PiH, double *vcc, double *vf, const double *tp, double dt,

@@ -13,6 +13,7 @@
 // accumulation order (+x-, -x+, +y-, -y+, +z-, -z+, then the source), which
 // reproduces the reference's sums bit for bit.
 #include "vh_kernels.h"
+#include "vh_vort.h"
 
 namespace vh {
 
@@ -473,13 +474,16 @@ k_vprim(Geom g, DevEos eos, const double *__restrict__ Qh, const double *__restr
 // Navier-Stokes values, relaxation and second-order/cross terms; writes
 // pi0, piH0, Pi0, PiH0.  (The loop's flux contribution (tau-dt)*dQ_n is
 // formed in k_visc2, which reads the same old pi/Pi before replacing them.)
+// With VORT, also the cell's vorticity entries and op code (vh_vort.h).
+template <bool VORT>
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_VISC1)
 k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
         const double *__restrict__ Qn, const double *__restrict__ ph,
         const double *__restrict__ pt, const double *__restrict__ m, int m_full,
         const double *__restrict__ pi, const double *__restrict__ Pi,
         const double *__restrict__ tp, double dt, double *__restrict__ pi0, double *__restrict__ piH0, double *__restrict__ Pi0,
-        double *__restrict__ PiH0, int *__restrict__ status) {
+        double *__restrict__ PiH0, int *__restrict__ status, double *__restrict__ db,
+        unsigned char *__restrict__ dop) {
   const double tau = *tp;
   const long c = xcd_block() * blockDim.x + threadIdx.x;
   if (c >= g.N) return;
@@ -496,6 +500,7 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
     }
     PiH0[c] = 0.0;
     Pi0[c] = 0.0;
+    if (VORT) vort::store(db, dop, N, c, vort::K_EMPTY, nullptr);
     return;
   }
   const double gamma = 1.0 / sqrt(1.0 - vx * vx - vy * vy - vz * vz);
@@ -504,6 +509,8 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
 
   // ---- NSquant
   double dmu[4][4], piNS[10], PiNS, du;
+  double vv[16];             // vorticity entries formed by this cell (VORT)
+  unsigned char vflags = 0;
   bool vac = false;
   if (!m_full) {
     auto maxM = [&](int a, int b) {
@@ -521,6 +528,7 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
 #pragma unroll
     for (int k = 0; k < 10; k++) piNS[k] = 0.;
     PiNS = du = 0.;
+    if (VORT) vort::store(db, dop, N, c, vort::K_STALE, nullptr);
   } else {
     const double VMIN = 1e-2, UDIFF = 3.0;
     double q[7];
@@ -545,12 +553,18 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
     if (fabs(u[2]) < VMIN || fabs(0.5 * (uy1 + uy0) / uy1) > UDIFF) dmu[0][2] = (uy1 - uy0) / dt;
     if (fabs(u[3]) < VMIN || fabs(0.5 * (uz1 + uz0) / uz1) > UDIFF) dmu[0][3] = (uz1 - uz0) / dt;
     if (e1 <= 0. || e0 <= 0.) dmu[0][0] = dmu[0][1] = dmu[0][2] = dmu[0][3] = 0.;
+    if (VORT) {  // d_tau (u^nu / T), src/hdo.cpp:419-458
+      const double a0[4] = {ut0, ux0, uy0, uz0}, a1[4] = {ut1, ux1, uy1, uz1};
+      vort::row_time(vv, a0, a1, eos.T(p0.e), eos.T(p1.e), e1, e0, dt);
+    }
     // space derivatives from the neighbours' Qh/tau_{n+1} states
 #pragma unroll
     for (int d = 1; d <= 3; d++) {
       const long ca = g.cidx(ix + (d == 1), iy + (d == 2), iz + (d == 3));
       const long cb = g.cidx(ix - (d == 1), iy - (d == 2), iz - (d == 3));
       const double ea = pt[ca], eb = pt[cb];
+      // the reference takes both temperatures before its e test (src/hdo.cpp:474-481)
+      const double TA = VORT ? eos.T(ea) : 0., TB = VORT ? eos.T(eb) : 0.;
       if (ea > 0. && eb > 0.) {
         const double avx = pt[N + ca], avy = pt[2 * N + ca], avz = pt[3 * N + ca];
         const double bvx = pt[N + cb], bvy = pt[2 * N + cb], bvz = pt[3 * N + cb];
@@ -562,6 +576,12 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
         ux1 = ut1 * avx;
         uy1 = ut1 * avy;
         uz1 = ut1 * avz;
+        if (VORT) {  // src/hdo.cpp:495-503,541-549,587-595
+          const double A[4] = {ut1, ux1, uy1, uz1}, B[4] = {ut0, ux0, uy0, uz0};
+          const double den = d == 1 ? g.dx : (d == 2 ? g.dy : g.dz * (tau + 0.5 * dt));
+          vort::row_space(vv + 4 * d, A, B, TA, TB, den);
+          vflags |= d == 1 ? vort::F_R1 : (d == 2 ? vort::F_R2 : vort::F_R3);
+        }
         if (d < 3) {
           const double h = d == 1 ? g.dx : g.dy;
           dmu[d][0] = 0.25 * (ut1 * ut1 - ut0 * ut0) / u[0] / h;
@@ -595,6 +615,21 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
     }
     dmu[3][0] += u[3] / (tau - 0.5 * dt);
     dmu[3][3] += u[0] / (tau - 0.5 * dt);
+    if (VORT) {  // Christoffel terms (src/hdo.cpp:608-611), then Cell::setDbeta
+      const double TH = eos.T(e);
+      if (TH > 0.) {
+        const double a30 = u[3] / (TH * (tau - 0.5 * dt)), a33 = u[0] / (TH * (tau - 0.5 * dt));
+        if (vflags & vort::F_R3) {
+          vv[12] += a30;
+          vv[15] += a33;
+        } else {
+          vv[12] = a30;
+          vv[15] = a33;
+        }
+        vflags |= vort::F_ADD;
+      }
+      vort::store(db, dop, N, c, vort::K_FULL | vflags, vv);
+    }
     // transport coefficients at the H-centre state
     const double s = eos.s(e, nb, nq, ns);
     const double T = eos.T(e), p = eos.p(e);
@@ -1084,9 +1119,15 @@ void launch_visc1(const Geom &g, const DevEos &eos, const DevTrc &trc, const dou
                   const double *Qn, const double *ph, const double *pt, const double *m,
                   int m_full, const double *pi, const double *Pi, const double *tp, double dt,
                   double *pi0, double *piH0, double *Pi0, double *PiH0, int *status,
-                  hipStream_t st) {
-  hipLaunchKernelGGL(k_visc1, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, trc, Qprev, Qn, ph,
-                     pt, m, m_full, pi, Pi, tp, dt, pi0, piH0, Pi0, PiH0, status);
+                  double *db, unsigned char *dop, hipStream_t st) {
+  if (db)
+    hipLaunchKernelGGL(k_visc1<true>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, trc, Qprev,
+                       Qn, ph, pt, m, m_full, pi, Pi, tp, dt, pi0, piH0, Pi0, PiH0, status, db,
+                       dop);
+  else
+    hipLaunchKernelGGL(k_visc1<false>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, trc, Qprev,
+                       Qn, ph, pt, m, m_full, pi, Pi, tp, dt, pi0, piH0, Pi0, PiH0, status,
+                       nullptr, nullptr);
 }
 
 void launch_visc2(const Geom &g, const double *ph, const double *pi0, const double *piH0,
