@@ -325,6 +325,16 @@ enum { VH_BETA_COLS = 33 };
 int vh_freezeout_beta(vh_ctx_t *ctx, double tau, double dt, double e_crit, int extend,
                       int lower, double *rows, double *beta, long max_rows, long *nrows,
                       vh_fo_stats_t *stats);
+/* Fluid::outputCorona (src/fld.cpp:876-1072): the elements of the initial
+ * state at tau (called once, before the first step): every interior cube
+ * (ix, iy, iz in [2, n-3)) whose 8 corners have e <= e_crit and whose centre
+ * state, the corners' mean Q / tau, has e >= 0.01, is one element at the cube
+ * centre with dsigma_mu = tau dx dy dz (cosh eta, 0, 0, -sinh eta).  rows,
+ * max_rows, *nrows as vh_freezeout (27 or 29 columns, the reference's
+ * element order); stats as vh_freezeout (nsusp 0).  Its element count is the
+ * "Number of corona cells" of the beta.dat header. */
+int vh_corona(vh_ctx_t *ctx, double tau, double e_crit, int extend, double *rows, long max_rows,
+              long *nrows, vh_fo_stats_t *stats);
 /* The Cornelius construction alone on the GPU (Cornelius::find_surface_4d):
  * n hypercubes h (n x 16 corner values, index [t][x][y][z]), edge lengths
  * dx[4] (t, x, y, z), level E0; cen, nrm: n x 10 x 4 (unused elements zero);

@@ -634,4 +634,120 @@ long or_freezeout(const or_fo_grid_t *g, const double *Q, const double *Qlow, co
   return nrow;
 }
 
+// Fluid::outputCorona (src/fld.cpp:876-1072): at tau, every interior cube
+// (loop order ix, iy, iz in [2, n-3)) whose 8 corners have e <= ecrit and
+// whose centre (the corners' mean Q) has e >= 0.01 is one element at the
+// cube centre with dsigma = tau dx dy dz (cosh eta, 0, 0, -sinh eta).
+// Records as or_freezeout's; stats (may be NULL): element count, sum of
+// dV_eff, the EtotSurf increment, the sum of n_b dV_eff.
+long or_corona(const or_fo_grid_t *g, const double *Q, const double *pi, const double *Pi,
+               double tau, double ecrit, int extend, double *rows, long max_rows,
+               const or_eos_t *eos, double *stats) {
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const long N = (long)nx * ny * nz;
+  const int ncol = extend ? 29 : 27;
+  auto cell = [&](int ix, int iy, int iz) { return ix + (long)nx * (iy + (long)ny * iz); };
+  std::vector<double> el(N);
+  {
+    std::vector<double> q(7 * N), out(8 * N);
+    for (long c = 0; c < N; c++)
+      for (int k = 0; k < 7; k++) q[7 * c + k] = Q[k * N + c] / tau;
+    or_c2p(eos, q.data(), N, out.data(), nullptr);
+    for (long c = 0; c < N; c++) el[c] = out[8 * c];
+  }
+  double acc[4] = {0., 0., 0., 0.};
+  long nrow = 0;
+  for (int ix = 2; ix < nx - 2; ix++)
+    for (int iy = 2; iy < ny - 2; iy++)
+      for (int iz = 2; iz < nz - 2; iz++) {
+        bool corona = true;
+        double QC[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (int jx = 0; jx < 2; jx++)
+          for (int jy = 0; jy < 2; jy++)
+            for (int jz = 0; jz < 2; jz++) {
+              const long c = cell(ix + jx, iy + jy, iz + jz);
+              if (el[c] > ecrit) corona = false;
+              for (int i = 0; i < 7; i++) QC[i] += Q[i * N + c] * 0.125;
+            }
+        for (int i = 0; i < 7; i++) QC[i] = QC[i] / tau;
+        double pr[8], eo[4];
+        or_c2p(eos, QC, 1, pr, nullptr);
+        const double eC = pr[0], nbC = pr[2];
+        if (!(corona && eC >= 0.01)) continue;
+        or_eos(eos, &eC, 1, eo);
+        const double pC = eo[0], TC = eo[1];
+        double piC[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, PiC = 0.;
+        for (int jx = 0; jx < 2; jx++)
+          for (int jy = 0; jy < 2; jy++)
+            for (int jz = 0; jz < 2; jz++) {
+              const long c = cell(ix + jx, iy + jy, iz + jz);
+              for (int ii = 0; ii < 10; ii++) piC[ii] += pi[ii * N + c] * 0.125;
+              PiC += Pi[c] * 0.125;
+            }
+        double vxC = pr[5], vyC = pr[6], vzC = pr[7];
+        const double v2C = vxC * vxC + vyC * vyC + vzC * vzC;
+        if (v2C > 1.) {
+          vxC *= std::sqrt(0.99 / v2C);
+          vyC *= std::sqrt(0.99 / v2C);
+          vzC *= std::sqrt(0.99 / v2C);
+        }
+        const double etaC = g->zmin + iz * g->dz + 0.5 * g->dz;
+        const double Y = etaC + 1. / 2. * std::log((1. + vzC) / (1. - vzC));
+        vxC = vxC * std::cosh(Y - etaC) / std::cosh(Y);
+        vyC = vyC * std::cosh(Y - etaC) / std::cosh(Y);
+        vzC = std::tanh(Y);
+        const double gammaC = 1. / std::sqrt(1. - vxC * vxC - vyC * vyC - vzC * vzC);
+        const double uC[4] = {gammaC, gammaC * vxC, gammaC * vyC, gammaC * vzC};
+        const double ch = std::cosh(etaC), sh = std::sinh(etaC);
+        double ds[4];
+        ds[0] = tau * (ch * g->dx * g->dy * g->dz);
+        ds[3] = tau * (-sh * g->dx * g->dy * g->dz);
+        ds[1] = 0.0;
+        ds[2] = 0.0;
+        double dV = 0.0;
+        for (int ii = 0; ii < 4; ii++) dV += ds[ii] * uC[ii];
+        double pc[10];
+        pc[sym(0, 0)] = ch * ch * piC[sym(0, 0)] + 2. * ch * sh * piC[sym(0, 3)] +
+                        sh * sh * piC[sym(3, 3)];
+        pc[sym(0, 1)] = ch * piC[sym(0, 1)] + sh * piC[sym(3, 1)];
+        pc[sym(0, 2)] = ch * piC[sym(0, 2)] + sh * piC[sym(3, 2)];
+        pc[sym(0, 3)] = ch * sh * (piC[sym(0, 0)] + piC[sym(3, 3)]) +
+                        (ch * ch + sh * sh) * piC[sym(0, 3)];
+        pc[sym(1, 1)] = piC[sym(1, 1)];
+        pc[sym(1, 2)] = piC[sym(1, 2)];
+        pc[sym(1, 3)] = sh * piC[sym(0, 1)] + ch * piC[sym(3, 1)];
+        pc[sym(2, 2)] = piC[sym(2, 2)];
+        pc[sym(2, 3)] = sh * piC[sym(0, 2)] + ch * piC[sym(3, 2)];
+        pc[sym(3, 3)] = sh * sh * piC[sym(0, 0)] + ch * ch * piC[sym(3, 3)] +
+                        2. * sh * ch * piC[sym(0, 3)];
+        double dEv = 0.;
+        for (int i = 0; i < 4; i++) dEv += pc[sym(0, i)] * ds[i];
+        acc[1] += dV;
+        acc[2] += (eC + pC) * uC[0] * dV - pC * ds[0] + dEv;
+        acc[3] += nbC * dV;
+        if (nrow < max_rows) {
+          double *r = rows + nrow * ncol;
+          r[0] = tau;
+          r[1] = g->xmin + ix * g->dx + 0.5 * g->dx;
+          r[2] = g->ymin + iy * g->dy + 0.5 * g->dy;
+          r[3] = g->zmin + iz * g->dz + 0.5 * g->dz;
+          for (int i = 0; i < 4; i++) r[4 + i] = ds[i];
+          for (int i = 0; i < 4; i++) r[8 + i] = uC[i];
+          r[12] = TC;
+          r[13] = r[14] = r[15] = 0.;
+          for (int i = 0; i < 10; i++) r[16 + i] = pc[i];
+          r[26] = PiC;
+          if (extend) {
+            r[27] = eC;
+            r[28] = nbC;
+          }
+        }
+        nrow++;
+      }
+  acc[0] = (double)nrow;
+  if (stats)
+    for (int k = 0; k < 4; k++) stats[k] = acc[k];
+  return nrow;
+}
+
 }  // extern "C"

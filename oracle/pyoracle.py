@@ -116,6 +116,10 @@ def lib():
         L.or_freezeout.argtypes = [ctypes.POINTER(FoGrid), dp, dp, dp, dp, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, dp,
                                    ctypes.c_long, ctypes.POINTER(OrEos), dp, dp]
+        L.or_corona.argtypes = [ctypes.POINTER(FoGrid), dp, dp, dp, ctypes.c_double,
+                                ctypes.c_double, ctypes.c_int, dp, ctypes.c_long,
+                                ctypes.POINTER(OrEos), dp]
+        L.or_corona.restype = ctypes.c_long
         L.or_set_vorticity.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.or_get_dbeta.argtypes = [ctypes.c_void_p, dp]
         L.or_freezeout.restype = ctypes.c_long
@@ -267,3 +271,19 @@ def freezeout(shape, spacing, mins, Q, Qlow, pi, Pi, tau, dt, ecrit, extend=Fals
     if beta is not None:
         return rows[:min(n, max_rows)], beta[:min(n, max_rows)]
     return rows[:min(n, max_rows)]
+
+
+def corona(shape, spacing, mins, Q, pi, Pi, tau, ecrit, extend=False, max_rows=1000000,
+           table=None):
+    """Fluid::outputCorona's element records and stats (count, sum dV_eff,
+    EtotSurf increment, sum n_b dV_eff)."""
+    g = FoGrid(shape[0], shape[1], shape[2], spacing[0], spacing[1], spacing[2], mins[0],
+               mins[1], mins[2])
+    ncol = 29 if extend else 27
+    rows = np.zeros((max_rows, ncol))
+    st = np.zeros(4)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (Q, pi, Pi)]
+    d, keep = _eos(table)
+    n = lib().or_corona(ctypes.byref(g), *[_p(a) for a in arrs], tau, ecrit, int(extend),
+                        _p(rows), max_rows, d, _p(st))
+    return rows[:min(n, max_rows)], st

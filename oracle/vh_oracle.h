@@ -110,6 +110,12 @@ long or_freezeout(const or_fo_grid_t *g, const double *Q, const double *Qlow, co
                   const double *Pi, double tau, double dt, double ecrit, int extend,
                   double *rows, long max_rows, const or_eos_t *eos, const double *dbeta,
                   double *beta);
+/* Fluid::outputCorona's element records (src/fld.cpp:876-1072) at tau from SoA
+ * Q, pi, Pi; stats (may be NULL, 4 doubles): count, sum dV_eff, EtotSurf and
+ * sum n_b dV_eff increments.  Returns the element count. */
+long or_corona(const or_fo_grid_t *g, const double *Q, const double *pi, const double *Pi,
+               double tau, double ecrit, int extend, double *rows, long max_rows,
+               const or_eos_t *eos, double *stats);
 #ifdef __cplusplus
 }
 #endif

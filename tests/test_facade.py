@@ -1,6 +1,7 @@
 """The C++ facade (Fluid/Hydro/Cell/EoS/TransportCoeff/ICGubser) and the
 params-file driver built on it (vhlle_amd/bin/vhlle)."""
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -135,8 +136,10 @@ def test_driver_freezeout_file(tmp_path):
     rows = np.loadtxt(tmp_path / "freezeout.dat", ndmin=2)
     counts = [int(l.split()[3]) for l in r.stdout.splitlines()
               if len(l.split()) == 5 and l.split()[3].isdigit()]
-    print(f"{len(rows)} rows; last counts {counts[-5:]}", flush=True)
-    assert rows.shape[1] == 27 and len(rows) == sum(counts) and counts[-1] == 0
+    corona = int(re.search(r"corona elements : (\d+)", r.stdout).group(1))
+    print(f"{len(rows)} rows, {corona} corona; last counts {counts[-5:]}", flush=True)
+    assert rows.shape[1] == 27 and len(rows) == corona + sum(counts) and counts[-1] == 0
+    assert corona > 0 and np.all(rows[:corona, 5:7] == 0.0)  # corona dsigma has no x, y part
     assert np.all(np.isfinite(rows)) and np.all(rows[:, 8] >= 1.0)
 
 
@@ -227,3 +230,58 @@ def test_driver_grid_resize_matches_oracle(tmp_path, oracle):
             break
     Qo = o.get_state()["Q"]
     assert np.max(np.abs(Qg - Qo)) <= 1e-10 * np.max(np.abs(Qo))
+
+
+@pytest.mark.gpu
+def test_driver_vorticity_beta_file(tmp_path, oracle):
+    """vorticity 1: beta.dat has the reference's three header lines (with the
+    corona count) and one 33-column record per freeze-out element after the
+    corona, equal to the oracle's records of the same run (the oracle replays
+    the driver loop: a step, then the surface between the two levels)."""
+    nx, ny, nz = 14, 12, 9
+    xl, yl, zl = (-3.25, 3.25), (-2.75, 2.75), (-1.0, 1.0)
+    kw = dict(eosType=5, icModel=10, icE0=6.0, icSigmaX=1.0, icSigmaY=0.8, icEtaPlateau=0.3,
+              icEtaSigma=0.4, nx=nx, ny=ny, nz=nz, xmin=xl[0], xmax=xl[1], ymin=yl[0],
+              ymax=yl[1], etamin=zl[0], etamax=zl[1], tau0=1.0, tauMax=1.3, dtau=0.05,
+              etaS=0.12, tauGridResize=100.0, e_crit=0.5, stopOnEmptySurface=0, vorticity=1)
+    p = _params(tmp_path, **kw)
+    r = subprocess.run([EXE, "-params", p, "-outputDir", str(tmp_path)], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-600:])
+    assert r.returncode == 0, r.stderr
+    head = (tmp_path / "beta.dat").read_text(encoding="utf-8").splitlines()[:3]
+    corona = int(re.search(r"corona elements : (\d+)", r.stdout).group(1))
+    assert head[0].startswith("# The derivatives of \u03b2 are given in Cartesian")
+    assert head[1] == f"# Number of corona cells: {corona}"
+    assert head[2].startswith("#  \u03c4  x  y  \u03b7  d\u03a3[0]") and head[2].endswith("\u03f5")
+    beta = np.loadtxt(tmp_path / "beta.dat", ndmin=2)
+    fo = np.loadtxt(tmp_path / "freezeout.dat", ndmin=2)
+    assert beta.shape[1] == 33 and len(beta) == len(fo) - corona > 20
+    assert np.allclose(beta[:, :16], fo[corona:, :16], rtol=1e-14, atol=0)
+
+    e, vx, vy, vz = setups.gaussian3d_ic(nx, ny, nz, xl, yl, zl, e0=6.0, sx=1.0, sy=0.8,
+                                         eta_plateau=0.3, eta_sigma=0.4)
+    o = oracle.Solver(nx, ny, nz, xl, yl, zl, trc=oracle.make_trc(etaS=0.12))
+    o.set_vorticity(True)
+    o.set_prim(1.0, e, vx, vy, vz)
+    sp = tuple((l[1] - l[0]) / (n - 1) for l, n in zip((xl, yl, zl), (nx, ny, nz)))
+    s0 = o.get_state()
+    crow, _ = oracle.corona((nx, ny, nz), sp, (xl[0], yl[0], zl[0]), s0["Q"], s0["pi"],
+                            s0["Pi"], 1.0, 0.5)
+    assert len(crow) == corona
+    assert np.allclose(fo[:corona], crow, rtol=1e-12, atol=1e-15)
+    tau, ref = 1.0, []
+    while True:
+        rc, tau = o.run(tau, 0.05, 1)
+        assert rc == 0
+        s = o.get_state()
+        _, b = oracle.freezeout((nx, ny, nz), sp, (xl[0], yl[0], zl[0]), s["Q"], s["Qprev"],
+                                s["pi"], s["Pi"], tau, 0.05, 0.5, dbeta=o.get_dbeta())
+        ref.append(b)
+        if not tau < 1.3 + 0.0001:
+            break
+    ref = np.concatenate(ref)
+    assert len(ref) == len(beta)
+    scale = np.max(np.abs(ref[:, 16:32]))
+    assert np.max(np.abs(beta[:, 16:32] - ref[:, 16:32])) <= 1e-9 * scale
+    assert np.allclose(beta[:, 32], ref[:, 32], rtol=1e-12)

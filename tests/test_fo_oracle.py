@@ -11,6 +11,9 @@ construction and Fluid::outputSurface's element records).
 * Mirroring a hypercube mirrors its element.
 * Two diagonally opposite corners above the level give two elements.
 * An element record of a uniform state carries that state (u, T, dsigma).
+* The corona (Fluid::outputCorona) of a uniform static state below e_crit is
+  every interior cube, each in closed form (centre, dsigma, u, T, dV_eff,
+  EtotSurf); centres below e = 0.01 and cubes touching a hot cell are left out.
 """
 import numpy as np
 import pytest
@@ -154,3 +157,61 @@ def test_record_of_a_uniform_flow():
         if abs(r[3]) < 1e-12:  # eta = 0: lab frame = Milne frame
             g = 1 / np.sqrt(1 - vx * vx)
             assert np.allclose(r[8:12], [g, g * vx, 0, 0], atol=1e-12)
+
+
+def _uniform_Q(shape, e, tau):
+    N = shape[0] * shape[1] * shape[2]
+    Q = np.zeros((7, N))
+    Q[0] = tau * e  # at rest: Q^tau = tau e
+    return Q
+
+
+def test_corona_of_a_uniform_state_in_closed_form():
+    """Fluid::outputCorona on a static uniform state below e_crit: every
+    interior cube is one element at its centre, dsigma = tau dV (cosh eta, 0,
+    0, -sinh eta), u = (cosh eta, 0, 0, sinh eta) in the lab, T = T(e),
+    dV_eff = tau dV, and the EtotSurf increment e tau dV cosh eta."""
+    shape, sp, mins = (9, 8, 10), (0.3, 0.25, 0.2), (-1.2, -0.875, -0.9)
+    tau, e = 1.3, 0.3
+    N = np.prod(shape)
+    rows, st = po.corona(shape, sp, mins, _uniform_Q(shape, e, tau), np.zeros((10, N)),
+                         np.zeros(N), tau, 0.5, extend=True)
+    n = (shape[0] - 4) * (shape[1] - 4) * (shape[2] - 4)
+    assert len(rows) == n and st[0] == n
+    dV = sp[0] * sp[1] * sp[2]
+    ix, iy, iz = np.meshgrid(*[np.arange(2, s - 2) for s in shape], indexing="ij")
+    x = mins[0] + (ix.ravel() + 0.5) * sp[0]
+    y = mins[1] + (iy.ravel() + 0.5) * sp[1]
+    eta = mins[2] + (iz.ravel() + 0.5) * sp[2]
+    assert np.allclose(rows[:, 1], x, atol=1e-14) and np.allclose(rows[:, 2], y, atol=1e-14)
+    assert np.allclose(rows[:, 3], eta, atol=1e-14) and np.all(rows[:, 0] == tau)
+    ch, sh = np.cosh(eta), np.sinh(eta)
+    assert np.allclose(rows[:, 4], tau * dV * ch, rtol=1e-14)
+    assert np.allclose(rows[:, 7], -tau * dV * sh, rtol=1e-14, atol=1e-16)
+    assert np.all(rows[:, 5:7] == 0.0)
+    assert np.allclose(rows[:, 8], ch, rtol=1e-14) and np.allclose(rows[:, 11], sh, atol=1e-14)
+    T = (e / ((16 + 0.5 * 21.0 * 2.5) * np.pi ** 2 / 30.0 / 0.197326968 ** 3)) ** 0.25
+    assert np.allclose(rows[:, 12], T, rtol=1e-14)
+    assert np.allclose(rows[:, 27], e, rtol=1e-14)
+    assert st[1] == pytest.approx(n * tau * dV, rel=1e-12)
+    assert st[2] == pytest.approx(e * tau * dV * np.sum(ch), rel=1e-12)
+
+
+def test_corona_excludes_tails_and_cubes_touching_hot_cells():
+    shape, sp, mins = (10, 9, 8), (0.3, 0.3, 0.3), (-1.35, -1.2, -1.05)
+    tau = 1.0
+    N = np.prod(shape)
+    z = np.zeros((10, N)), np.zeros(N)
+    tail, _ = po.corona(shape, sp, mins, _uniform_Q(shape, 0.009, tau), *z, tau, 0.5)
+    assert len(tail) == 0  # centre e < 0.01
+    Q = _uniform_Q(shape, 0.2, tau)
+    hot = np.zeros(shape[::-1], dtype=bool)
+    hot[3:5, 4:6, 4:7] = True  # [iz, iy, ix]
+    Q[0, hot.ravel()] = tau * 2.0
+    rows, st = po.corona(shape, sp, mins, Q, *z, tau, 0.5)
+    cubes = 0
+    for ix in range(2, shape[0] - 2):
+        for iy in range(2, shape[1] - 2):
+            for iz in range(2, shape[2] - 2):
+                cubes += not hot[iz:iz + 2, iy:iy + 2, ix:ix + 2].any()
+    assert len(rows) == cubes < (shape[0] - 4) * (shape[1] - 4) * (shape[2] - 4)

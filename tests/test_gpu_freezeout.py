@@ -89,3 +89,25 @@ def test_fp32_fluxes_surface(gpu):
     print(f"elements fp64 {n64} mixed {n32}; sum dsigma rel diff {rel}", flush=True)
     assert abs(n32 - n64) <= max(2, 0.01 * n64)
     assert np.all(rel < 1e-3)
+
+
+def test_corona_matches_oracle(gpu):
+    """Fluid::outputCorona's elements of an initial state (FP64 path)"""
+    nx, ny, nz = 26, 24, 14
+    xl, yl, zl = (-2.5, 2.5), (-2.3, 2.3), (-0.65, 0.65)
+    e, vx, vy, vz = setups.gaussian3d_ic(nx, ny, nz, xl, yl, zl, e0=8.0, sx=1.0, sy=1.2,
+                                         eta_plateau=0.3, eta_sigma=0.3)
+    ctx = abi.Context(nx, ny, nz, xl, yl, zl, trc=abi.make_trc(etaS=0.1))
+    ctx.set_prim(0.6, e, vx, vy, vz)
+    rows, stats = ctx.corona(0.6, 0.5, extend=True)
+    s = ctx.download()
+    ctx.close()
+    sp = ((xl[1] - xl[0]) / (nx - 1), (yl[1] - yl[0]) / (ny - 1), (zl[1] - zl[0]) / (nz - 1))
+    ref, rst = po.corona((nx, ny, nz), sp, (xl[0], yl[0], zl[0]), s["Q"], s["pi"], s["Pi"],
+                         0.6, 0.5, extend=True)
+    print(f"{len(rows)} corona elements", flush=True)
+    assert len(rows) == len(ref) == stats["nelements"] > 100
+    assert np.array_equal(rows[:, :4], ref[:, :4])
+    assert np.allclose(rows, ref, rtol=1e-13, atol=1e-15)
+    assert stats["v_eff"] == pytest.approx(rst[1], rel=1e-12)
+    assert stats["e_surf"] == pytest.approx(rst[2], rel=1e-12)

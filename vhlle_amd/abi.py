@@ -186,6 +186,9 @@ def lib():
                                         ctypes.c_int, ctypes.c_int, _dp, ctypes.c_long,
                                         ctypes.POINTER(ctypes.c_long), ctypes.POINTER(FoStats)]),
         "vh_cornelius": (ctypes.c_int, [_dp, ctypes.c_long, _dp, ctypes.c_double, _dp, _dp, ip]),
+        "vh_corona": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp,
+                                     ctypes.c_long, ctypes.POINTER(ctypes.c_long),
+                                     ctypes.POINTER(FoStats)]),
         "vh_set_vorticity": (ctypes.c_int, [vp, ctypes.c_int]),
         "vh_get_dbeta": (ctypes.c_int, [vp, _dp]),
         "vh_freezeout_beta": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double,
@@ -211,7 +214,8 @@ def exported_symbols():
             "vh_mlp_forward", "vh_mlp_grad", "vh_mlp_train", "vh_c2p_learned",
             "vh_set_c2p_model", "vh_mlp_harvest", "vh_set_stage", "vh_get_stage", "vh_create_batch", "vh_hotspot_ic",
             "vh_copy_bandwidth", "vh_fo_snapshot", "vh_freezeout", "vh_cornelius",
-            "vh_eos_table", "vh_set_vorticity", "vh_get_dbeta", "vh_freezeout_beta"]
+            "vh_eos_table", "vh_set_vorticity", "vh_get_dbeta", "vh_freezeout_beta",
+            "vh_corona"]
 
 
 def _p(a):
@@ -471,6 +475,21 @@ class Context:
         return rows[:min(n.value, max_rows)], dict(
             nelements=st.nelements, nsusp=st.nsusp, dsigma=list(st.dsigma), v_eff=st.v_eff,
             e_surf=st.e_surf, nb_surf=st.nb_surf, err=st.err)
+
+    def corona(self, tau, ecrit, extend=False, max_rows=2000000):
+        """Fluid::outputCorona's elements (rows of 27 or 29 columns) and stats"""
+        ncol = 29 if extend else 27
+        n = ctypes.c_long(0)
+        st = FoStats()
+        check(lib().vh_corona(self.h, tau, ecrit, int(extend), None, 0, ctypes.byref(n),
+                              ctypes.byref(st)), "vh_corona")
+        rows = np.zeros((max(min(n.value, max_rows), 1), ncol))
+        if n.value and max_rows > 0:
+            check(lib().vh_corona(self.h, tau, ecrit, int(extend), _p(rows),
+                                  min(n.value, max_rows), ctypes.byref(n), ctypes.byref(st)),
+                  "vh_corona")
+        return rows[:min(n.value, max_rows)], dict(nelements=st.nelements, v_eff=st.v_eff,
+                                                   e_surf=st.e_surf, nb_surf=st.nb_surf)
 
     def set_vorticity(self, on=True):
         check(lib().vh_set_vorticity(self.h, int(on)), "vh_set_vorticity")

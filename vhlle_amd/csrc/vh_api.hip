@@ -1089,6 +1089,55 @@ int vh_freezeout_beta(vh_ctx_t *c, double tau, double dt, double e_crit, int ext
   return VH_OK;
 }
 
+int vh_corona(vh_ctx_t *c, double tau, double e_crit, int extend, double *rows, long max_rows,
+              long *nrows, vh_fo_stats_t *stats) {
+  if (!c || c->evh.size() != 1 || (rows && max_rows < 0) || !(tau > 0.)) return VH_ERR_ARG;
+  VH_HIP(hipStreamSynchronize(c->st));
+  vh::fo::FoArgs a;
+  a.nx = c->g.nx;
+  a.ny = c->g.ny;
+  a.nz = c->g.nz;
+  a.eos = c->eos;
+  a.P.tau = tau;
+  a.P.dt = 0.;
+  a.P.dx = c->g.dx;
+  a.P.dy = c->g.dy;
+  a.P.dz = c->g.dz;
+  a.P.xmin = c->grid.xmin;
+  a.P.ymin = c->grid.ymin;
+  a.P.zmin = c->grid.zmin;
+  a.P.ecrit = e_crit;
+  const int rec = extend ? vh::fo::ROW_EXT : vh::fo::ROW_BASE;
+  a.P.nbeta = 0;
+  a.P.ncol = rec + vh::fo::AUX;
+  a.Q = a.Qlow = c->Q[c->cur];
+  a.pi = c->pi;
+  a.Pi = c->Pi;
+  a.pi_float = c->precision == VH_PREC_MIXED;
+  a.db = nullptr;
+  a.stream = c->st;
+  a.want_rows = rows != nullptr && max_rows > 0;
+  double sums[vh::fo::NSTAT];
+  a.stats = sums;
+  std::vector<double> r;
+  long n = 0;
+  const int rc = vh::fo::run_corona(a, c->fo_work, r, n);
+  if (rc != VH_OK) return rc;
+  if (nrows) *nrows = n;
+  for (long i = 0; a.want_rows && i < n && i < max_rows; i++)
+    std::memcpy(rows + i * rec, &r[i * a.P.ncol], rec * sizeof(double));
+  if (stats) {
+    std::memset(stats, 0, sizeof *stats);
+    stats->nelements = (long)sums[0];
+    stats->nsusp = 0;
+    for (int k = 0; k < 4; k++) stats->dsigma[k] = sums[2 + k];
+    stats->v_eff = sums[6];
+    stats->e_surf = sums[7];
+    stats->nb_surf = sums[8];
+  }
+  return VH_OK;
+}
+
 int vh_cornelius(const double *h, long n, const double *dx, double E0, double *cen,
                  double *nrm, int *nel) {
   if (!h || n < 0 || !dx || !cen || !nrm || !nel) return VH_ERR_ARG;

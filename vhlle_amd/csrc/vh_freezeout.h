@@ -60,6 +60,8 @@ struct Work {
 
 // elements in the reference's loop order, rows of P.ncol doubles
 int run(const FoArgs &a, Work &w, std::vector<double> &rows, long &nrows, int &errbits);
+// Fluid::outputCorona's elements at P.tau from Q (Qlow = Q, P.dt = 0, nbeta 0)
+int run_corona(const FoArgs &a, Work &w, std::vector<double> &rows, long &nrows);
 // the construction alone on n hypercubes h (n x 16, [t][x][y][z]) on the GPU
 int cornelius_batch(const double *h, long n, const double *dx, double E0, double *cen,
                     double *nrm, int *nel);

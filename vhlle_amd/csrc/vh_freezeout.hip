@@ -109,6 +109,23 @@ __device__ __forceinline__ void transform_to_lab(double eta, double &vx, double 
   vz = tanh(Y);
 }
 
+// pi^{mu nu} at eta from Milne to Cartesian components (src/fld.cpp:765-790)
+__device__ inline void cart_pi(const double piC[10], double ch, double sh, double pc[10]) {
+  pc[sym(0, 0)] = ch * ch * piC[sym(0, 0)] + 2. * ch * sh * piC[sym(0, 3)] +
+                  sh * sh * piC[sym(3, 3)];
+  pc[sym(0, 1)] = ch * piC[sym(0, 1)] + sh * piC[sym(3, 1)];
+  pc[sym(0, 2)] = ch * piC[sym(0, 2)] + sh * piC[sym(3, 2)];
+  pc[sym(0, 3)] = ch * sh * (piC[sym(0, 0)] + piC[sym(3, 3)]) +
+                  (ch * ch + sh * sh) * piC[sym(0, 3)];
+  pc[sym(1, 1)] = piC[sym(1, 1)];
+  pc[sym(1, 2)] = piC[sym(1, 2)];
+  pc[sym(1, 3)] = sh * piC[sym(0, 1)] + ch * piC[sym(3, 1)];
+  pc[sym(2, 2)] = piC[sym(2, 2)];
+  pc[sym(2, 3)] = sh * piC[sym(0, 2)] + ch * piC[sym(3, 2)];
+  pc[sym(3, 3)] = sh * sh * piC[sym(0, 0)] + ch * ch * piC[sym(3, 3)] +
+                  2. * sh * ch * piC[sym(0, 3)];
+}
+
 template <typename PT>
 __global__ void __launch_bounds__(64)
 k_fo_cubes(CubeGeom g, FoParams P, DevEos eos, const long *__restrict__ cut,
@@ -204,19 +221,7 @@ k_fo_cubes(CubeGeom g, FoParams P, DevEos eos, const long *__restrict__ cut,
       row[14] = muqC;
       row[15] = musC;
       double pc[10];
-      pc[sym(0, 0)] = ch * ch * piC[sym(0, 0)] + 2. * ch * sh * piC[sym(0, 3)] +
-                      sh * sh * piC[sym(3, 3)];
-      pc[sym(0, 1)] = ch * piC[sym(0, 1)] + sh * piC[sym(3, 1)];
-      pc[sym(0, 2)] = ch * piC[sym(0, 2)] + sh * piC[sym(3, 2)];
-      pc[sym(0, 3)] = ch * sh * (piC[sym(0, 0)] + piC[sym(3, 3)]) +
-                      (ch * ch + sh * sh) * piC[sym(0, 3)];
-      pc[sym(1, 1)] = piC[sym(1, 1)];
-      pc[sym(1, 2)] = piC[sym(1, 2)];
-      pc[sym(1, 3)] = sh * piC[sym(0, 1)] + ch * piC[sym(3, 1)];
-      pc[sym(2, 2)] = piC[sym(2, 2)];
-      pc[sym(2, 3)] = sh * piC[sym(0, 2)] + ch * piC[sym(3, 2)];
-      pc[sym(3, 3)] = sh * sh * piC[sym(0, 0)] + ch * ch * piC[sym(3, 3)] +
-                      2. * sh * ch * piC[sym(0, 3)];
+      cart_pi(piC, ch, sh, pc);
       for (int i = 0; i < 10; i++) row[16 + i] = pc[i];
       row[26] = PiC;
       const int rec = P.ncol - AUX - P.nbeta;
@@ -263,13 +268,127 @@ k_fo_cubes(CubeGeom g, FoParams P, DevEos eos, const long *__restrict__ cut,
   }
 }
 
+// Fluid::outputCorona's selection (src/fld.cpp:946-985): every corner at
+// tau has e <= ecrit and the centre state (the corners' mean Q) e >= 0.01
+__global__ void __launch_bounds__(256)
+k_corona_flag(CubeGeom g, DevEos eos, const double *__restrict__ e1,
+              const double *__restrict__ Q, double tau, double ecrit,
+              unsigned char *__restrict__ flag) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (long)g.mx * g.my * g.mz) return;
+  int ix, iy, iz;
+  cube_origin_mem(g, k, ix, iy, iz);
+  bool corona = true;
+  for (int jx = 0; jx < 2; jx++)
+    for (int jy = 0; jy < 2; jy++)
+      for (int jz = 0; jz < 2; jz++) {
+        const long c = (ix + jx) + (long)g.nx * ((iy + jy) + (long)g.ny * (iz + jz));
+        corona = corona && !(e1[c] > ecrit);
+      }
+  unsigned char f = 0;
+  if (corona) {
+    double QC[7] = {0., 0., 0., 0., 0., 0., 0.};
+    for (int jx = 0; jx < 2; jx++)
+      for (int jy = 0; jy < 2; jy++)
+        for (int jz = 0; jz < 2; jz++) {
+          const long c = (ix + jx) + (long)g.nx * ((iy + jy) + (long)g.ny * (iz + jz));
+          for (int i = 0; i < 7; i++) QC[i] += Q[i * g.N + c] * 0.125;
+        }
+    for (int i = 0; i < 7; i++) QC[i] = QC[i] / tau;
+    Prim pr;
+    c2p_ref(eos, QC, pr);
+    f = pr.e >= 0.01;
+  }
+  flag[k] = f;
+}
+
+// one corona element per selected cube (src/fld.cpp:986-1066): the record at
+// the cube centre with dsigma = tau dx dy dz (cosh eta, 0, 0, -sinh eta),
+// then the AUX columns
+template <typename PT>
+__global__ void __launch_bounds__(64)
+k_corona_rows(CubeGeom g, FoParams P, DevEos eos, const long *__restrict__ cube,
+              const int *__restrict__ n_p, const double *__restrict__ Q,
+              const PT *__restrict__ pi, const PT *__restrict__ Pi, double *__restrict__ rows) {
+  const int n = *n_p;
+  const long N = g.N;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x) {
+    int ix, iy, iz;
+    cube_origin(g, cube[t], ix, iy, iz);
+    double QC[7] = {0., 0., 0., 0., 0., 0., 0.}, piC[10], PiC = 0.;
+    for (int ii = 0; ii < 10; ii++) piC[ii] = 0.;
+    for (int jx = 0; jx < 2; jx++)
+      for (int jy = 0; jy < 2; jy++)
+        for (int jz = 0; jz < 2; jz++) {
+          const long c = (ix + jx) + (long)g.nx * ((iy + jy) + (long)g.ny * (iz + jz));
+          for (int i = 0; i < 7; i++) QC[i] += Q[i * N + c] * 0.125;
+        }
+    for (int i = 0; i < 7; i++) QC[i] = QC[i] / P.tau;
+    Prim pr;
+    c2p_ref(eos, QC, pr);
+    const double eC = pr.e, nbC = pr.nb;
+    const double TC = eos.T(eC), pC = eos.p(eC);
+    for (int jx = 0; jx < 2; jx++)
+      for (int jy = 0; jy < 2; jy++)
+        for (int jz = 0; jz < 2; jz++) {
+          const long c = (ix + jx) + (long)g.nx * ((iy + jy) + (long)g.ny * (iz + jz));
+          for (int ii = 0; ii < 10; ii++) piC[ii] += (double)pi[ii * N + c] * 0.125;
+          PiC += (double)Pi[c] * 0.125;
+        }
+    double vxC = pr.vx, vyC = pr.vy, vzC = pr.vz;
+    const double v2C = vxC * vxC + vyC * vyC + vzC * vzC;
+    if (v2C > 1.) {
+      vxC *= sqrt(0.99 / v2C);
+      vyC *= sqrt(0.99 / v2C);
+      vzC *= sqrt(0.99 / v2C);
+    }
+    const double etaC = P.zmin + iz * P.dz + 0.5 * P.dz;
+    transform_to_lab(etaC, vxC, vyC, vzC);
+    const double gammaC = 1. / sqrt(1. - vxC * vxC - vyC * vyC - vzC * vzC);
+    const double uC[4] = {gammaC, gammaC * vxC, gammaC * vyC, gammaC * vzC};
+    const double ch = cosh(etaC), sh = sinh(etaC);
+    double ds[4];
+    ds[0] = P.tau * (ch * P.dx * P.dy * P.dz);
+    ds[3] = P.tau * (-sh * P.dx * P.dy * P.dz);
+    ds[1] = 0.0;
+    ds[2] = 0.0;
+    double *row = rows + t * P.ncol;
+    row[0] = P.tau;
+    row[1] = P.xmin + ix * P.dx + 0.5 * P.dx;
+    row[2] = P.ymin + iy * P.dy + 0.5 * P.dy;
+    row[3] = P.zmin + iz * P.dz + 0.5 * P.dz;
+    for (int i = 0; i < 4; i++) row[4 + i] = ds[i];
+    for (int i = 0; i < 4; i++) row[8 + i] = uC[i];
+    row[12] = TC;
+    row[13] = row[14] = row[15] = 0.;
+    double pc[10];
+    cart_pi(piC, ch, sh, pc);
+    for (int i = 0; i < 10; i++) row[16 + i] = pc[i];
+    row[26] = PiC;
+    if (P.ncol - AUX > ROW_BASE) {
+      row[27] = eC;
+      row[28] = nbC;
+    }
+    double dV = 0.0;
+    for (int ii = 0; ii < 4; ii++) dV += ds[ii] * uC[ii];
+    double dEv = 0.;
+    for (int i = 0; i < 4; i++) dEv += pc[sym(0, i)] * ds[i];
+    double *aux = row + P.ncol - AUX;
+    aux[0] = dV;
+    aux[1] = (eC + pC) * uC[0] * dV - pC * ds[0] + dEv;
+    aux[2] = nbC * dV;
+    aux[3] = eC;
+  }
+}
+
 // sums over all elements in a fixed order: element count, the suspicious
 // ones, dsigma_mu, dV_eff, the EtotSurf and nbSurf increments.  Pass 1: block
 // b reduces the hypercubes of its chunk [b chunk, (b+1) chunk) into part[b];
 // pass 2 (k_fo_stats_sum) adds the parts in block order.
 constexpr int STAT_BLOCKS = 512;
 __global__ void __launch_bounds__(256)
-k_fo_stats(const int *__restrict__ ncut_p, const int *__restrict__ count,
+k_fo_stats(const int *__restrict__ ncut_p, const int *__restrict__ count, int slots,
            const double *__restrict__ rows, int ncol, double ecrit, double *__restrict__ part) {
   __shared__ double red[256];
   const int ncut = *ncut_p;
@@ -277,8 +396,8 @@ k_fo_stats(const int *__restrict__ ncut_p, const int *__restrict__ count,
   const long t0 = blockIdx.x * chunk, t1 = t0 + chunk < ncut ? t0 + chunk : ncut;
   double acc[NSTAT] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
   for (long t = t0 + threadIdx.x; t < t1; t += blockDim.x)
-    for (int el = 0; el < count[t]; el++) {
-      const double *row = rows + ((long)t * MAXE + el) * ncol;
+    for (int el = 0; el < (count ? count[t] : 1); el++) {
+      const double *row = rows + ((long)t * slots + el) * ncol;
       const double *aux = row + ncol - AUX;
       acc[0] += 1.;
       acc[1] += (aux[3] > ecrit * 2.0 || aux[3] < ecrit * 0.5) ? 1. : 0.;
@@ -330,7 +449,10 @@ hipError_t grow(T **p, size_t &cap, size_t n) {
 }
 }  // namespace
 
-int run(const FoArgs &a, Work &w, std::vector<double> &rows_out, long &nrows, int &errbits) {
+namespace {
+// the surface (corona false) or the corona (true) elements
+int run_impl(const FoArgs &a, Work &w, std::vector<double> &rows_out, long &nrows, int &errbits,
+             bool corona) {
   const CubeGeom g{a.nx, a.ny, a.nz, a.nx - 4, a.ny - 4, a.nz - 4, (long)a.nx * a.ny * a.nz};
   nrows = 0;
   errbits = 0;
@@ -374,8 +496,12 @@ int run(const FoArgs &a, Work &w, std::vector<double> &rows_out, long &nrows, in
   if (rc == VH_OK) {
     hipLaunchKernelGGL(k_fo_energy, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, N,
                        a.eos, a.Q, a.Qlow, a.P.tau, a.P.tau - a.P.dt, w.e1, w.e0);
-    hipLaunchKernelGGL(k_fo_flag, dim3((unsigned)((ncube + 255) / 256)), dim3(256), 0, st, g,
-                       w.e1, w.e0, a.P.ecrit, w.flag);
+    if (corona)
+      hipLaunchKernelGGL(k_corona_flag, dim3((unsigned)((ncube + 255) / 256)), dim3(256), 0, st,
+                         g, a.eos, w.e1, a.Q, a.P.tau, a.P.ecrit, w.flag);
+    else
+      hipLaunchKernelGGL(k_fo_flag, dim3((unsigned)((ncube + 255) / 256)), dim3(256), 0, st, g,
+                         w.e1, w.e0, a.P.ecrit, w.flag);
     hipLaunchKernelGGL(k_fill_err, dim3(1), dim3(64), 0, st, w.err);
     size_t tmpb = 0, sortb = 0;
     ok(hipcub::DeviceSelect::Flagged(nullptr, tmpb, keys, w.flag, w.cut, w.ncut, ncube, st));
@@ -398,7 +524,8 @@ int run(const FoArgs &a, Work &w, std::vector<double> &rows_out, long &nrows, in
   }
   std::vector<int> hcnt;
   if (rc == VH_OK && nc > 0) {
-    const size_t slots = (size_t)nc * MAXE;
+    const int per = corona ? 1 : MAXE;  // record slots per selected cube
+    const size_t slots = (size_t)nc * per;
     ok(grow(&w.rows, w.n_rows, slots * a.P.ncol));
     ok(grow(&w.cnt, w.n_cnt, (size_t)nc));
     ok(grow(&w.sorted, w.n_sorted, (size_t)nc));
@@ -408,7 +535,16 @@ int run(const FoArgs &a, Work &w, std::vector<double> &rows_out, long &nrows, in
     }
     if (rc == VH_OK) {
       const unsigned grid = (unsigned)std::min<long>((nc + 63) / 64, 2048);
-      if (a.pi_float)
+      if (corona) {
+        if (a.pi_float)
+          hipLaunchKernelGGL(k_corona_rows<float>, dim3(grid), dim3(64), 0, st, g, a.P, a.eos,
+                             w.sorted, w.ncut, a.Q, (const float *)a.pi, (const float *)a.Pi,
+                             w.rows);
+        else
+          hipLaunchKernelGGL(k_corona_rows<double>, dim3(grid), dim3(64), 0, st, g, a.P, a.eos,
+                             w.sorted, w.ncut, a.Q, (const double *)a.pi, (const double *)a.Pi,
+                             w.rows);
+      } else if (a.pi_float)
         hipLaunchKernelGGL(k_fo_cubes<float>, dim3(grid), dim3(64), 0, st, g, a.P, a.eos, w.sorted,
                            w.ncut, w.e1, w.e0, a.Q, a.Qlow, (const float *)a.pi,
                            (const float *)a.Pi, a.db, w.rows, w.cnt, w.err);
@@ -417,13 +553,15 @@ int run(const FoArgs &a, Work &w, std::vector<double> &rows_out, long &nrows, in
                            w.sorted, w.ncut, w.e1, w.e0, a.Q, a.Qlow, (const double *)a.pi,
                            (const double *)a.Pi, a.db, w.rows, w.cnt, w.err);
       ok(hipGetLastError());
-      hipLaunchKernelGGL(k_fo_stats, dim3(STAT_BLOCKS), dim3(256), 0, st, w.ncut, w.cnt, w.rows,
-                         a.P.ncol, a.P.ecrit, w.dstat);
+      hipLaunchKernelGGL(k_fo_stats, dim3(STAT_BLOCKS), dim3(256), 0, st, w.ncut,
+                         corona ? (const int *)nullptr : w.cnt, per, w.rows, a.P.ncol,
+                         a.P.ecrit, w.dstat);
       double *sums = w.dstat + (size_t)NSTAT * STAT_BLOCKS;
       hipLaunchKernelGGL(k_fo_stats_sum, dim3(1), dim3(64), 0, st, w.dstat, STAT_BLOCKS, sums);
       ok(hipMemcpyAsync(a.stats, sums, NSTAT * sizeof(double), hipMemcpyDeviceToHost, st));
-      hcnt.resize(nc);
-      ok(hipMemcpyAsync(hcnt.data(), w.cnt, nc * sizeof(int), hipMemcpyDeviceToHost, st));
+      hcnt.assign(nc, 1);
+      if (!corona)
+        ok(hipMemcpyAsync(hcnt.data(), w.cnt, nc * sizeof(int), hipMemcpyDeviceToHost, st));
       ok(hipMemcpyAsync(&errbits, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
       ok(hipStreamSynchronize(st));
     }
@@ -438,13 +576,23 @@ int run(const FoArgs &a, Work &w, std::vector<double> &rows_out, long &nrows, in
         long wr = 0;
         for (int k = 0; k < nc; k++)
           for (int el = 0; el < hcnt[k]; el++, wr++)
-            std::copy(&all[((size_t)k * MAXE + el) * a.P.ncol],
-                      &all[((size_t)k * MAXE + el + 1) * a.P.ncol], &rows_out[wr * a.P.ncol]);
+            std::copy(&all[((size_t)k * per + el) * a.P.ncol],
+                      &all[((size_t)k * per + el + 1) * a.P.ncol], &rows_out[wr * a.P.ncol]);
       }
     }
   }
   if (rc != VH_OK) w.release();  // leave no half-grown buffer behind
   return rc;
+}
+}  // namespace
+
+int run(const FoArgs &a, Work &w, std::vector<double> &rows, long &nrows, int &errbits) {
+  return run_impl(a, w, rows, nrows, errbits, false);
+}
+
+int run_corona(const FoArgs &a, Work &w, std::vector<double> &rows, long &nrows) {
+  int errbits = 0;
+  return run_impl(a, w, rows, nrows, errbits, true);
 }
 
 namespace {

@@ -14,6 +14,7 @@
 // 3 Newton from v = 0.5), c2pModel (a file written by -trainC2P);
 // stopOnEmptySurface (1, the reference's loop condition: stop at the first
 // step without freeze-out elements, which for nz < 5 is the first step).
+// vorticity 1 (FP64 only) writes beta.dat next to freezeout.dat.
 // tauGridResize: once tau reaches it the grid doubles its transverse extent
 // (expandGrid2x, src/main.cpp:246-271).
 // `vhlle [-params file] -trainC2P file [-seed S]` harvests 4 M conserved ->
@@ -184,6 +185,7 @@ Fluid *expandGrid2x(Hydro *h, EoS *eos, EoS *eosH, TransportCoeff *trcoeff) {
   Fluid *fnew = new Fluid(eos, eosH, trcoeff, nx, ny, nz, 2.0 * f->getX(0), 2.0 * f->getX(nx - 1),
                           2.0 * f->getY(0), 2.0 * f->getY(ny - 1), f->getZ(0), f->getZ(nz - 1),
                           2.0 * h->getDtau(), f->geteCrit());
+  if (vorticityOn) fnew->enableVorticity();  // matrices start at zero (src/main.cpp:257-265)
   for (int ix = 0; ix < nx; ix++)
     for (int iy = 0; iy < ny; iy++)
       for (int iz = 0; iz < nz; iz++)
@@ -281,9 +283,18 @@ int main(int argc, char **argv) {
   }
   cout << "IC done\n";
   Hydro *h = new Hydro(f, eos, trcoeff, tau0, dtau);
+  if (vorticityOn) {  // src/main.cpp:380-383
+    if (precision != VH_PREC_FP64) {
+      cout << "vorticity needs precision 0 (the FP64 step) in this build\n";
+      return 1;
+    }
+    h->enableVorticity();
+  }
   bool resized = false;
   mkdir(outputDir.c_str(), 0755);
   f->initOutput(outputDir.c_str());
+  f->outputCorona(tau0, freezeoutExtend);  // src/main.cpp:389-391
+  if (vorticityOn) f->printDbetaHeader();
   long steps = 0;
   int nelements = 0;
   const auto t0 = chrono::steady_clock::now();

@@ -11,6 +11,8 @@
 #include <stdexcept>
 
 // ------------------------------------------------------------- helpers
+static void vh_or_die(int rc, const char *what);
+
 int index44(const int &i, const int &j) {
   if (i > 3 || j > 3 || i < 0 || j < 0) {
     std::cout << "index44: i j " << i << " " << j << std::endl;
@@ -462,6 +464,16 @@ void Cell::setViscCorrCutFlag(double value) {
   f->hvcc[n] = value;
 }
 double Cell::getViscCorrCutFlag(void) { return CELL_RD(hvcc, 0); }
+void Cell::enableVorticity() { f->enableVorticity(); }
+double Cell::getDbeta(int i, int j) {
+  if (!f->vorticityOn) throw std::runtime_error("Error: dbeta is nullptr. Cannot get values.");
+  if (f->db_stale) {
+    f->hdb.resize(16 * f->N);
+    vh_or_die(vh_get_dbeta(f->ctx, f->hdb.data()), "vh_get_dbeta");
+    f->db_stale = false;
+  }
+  return f->hdb[(4 * i + j) * f->N + n];
+}
 void Cell::Dump(double tau) {
   double q[7], qh[7];
   getQ(q);
@@ -552,17 +564,78 @@ void Fluid::initOutput(const char *dir) {
   const std::string fn = std::string(dir) + "/freezeout.dat";
   ffreeze = std::fopen(fn.c_str(), "w");
   if (!ffreeze) std::cout << "cannot open " << fn << std::endl;
+  if (vorticityOn) {  // src/fld.cpp:131-136
+    const std::string fb = std::string(dir) + "/beta.dat";
+    fbeta = std::fopen(fb.c_str(), "w");
+    if (!fbeta) std::cout << "cannot open " << fb << std::endl;
+  }
 }
 
 void Fluid::adoptOutput(Fluid *from) {
   closeOutput();
   ffreeze = from->ffreeze;
-  from->ffreeze = nullptr;
+  fbeta = from->fbeta;
+  from->ffreeze = from->fbeta = nullptr;
+  num_corona_cells = from->num_corona_cells;
+  EtotSurf = from->EtotSurf;
+  nbSurf = from->nbSurf;
+  vEff = from->vEff;
 }
 
 void Fluid::closeOutput() {
   if (ffreeze) std::fclose(ffreeze);
-  ffreeze = nullptr;
+  if (fbeta) std::fclose(fbeta);
+  ffreeze = fbeta = nullptr;
+}
+
+void Fluid::enableVorticity() {
+  if (!vorticityOn) vh_or_die(vh_set_vorticity(ctx, 1), "vh_set_vorticity");
+  vorticityOn = true;
+  db_stale = true;
+}
+
+void Fluid::printDbetaHeader() {
+  if (num_corona_cells == -1)
+    throw std::runtime_error(
+        "Error: num_corona_cells not set, cannot print header of beta.dat");
+  if (!fbeta) return;
+  std::fprintf(fbeta,
+               "# The derivatives of \u03b2 are given in Cartesian coordinates and include "
+               "a factor of 1/2, such that \u2202\u2098\u03b2\u2099=1/2 * "
+               "\u2202\u2098(u\u2099/T)\n");
+  std::fprintf(fbeta, "# Number of corona cells: %d\n", num_corona_cells);
+  std::fprintf(fbeta,
+               "#  \u03c4  x  y  \u03b7  d\u03a3[0]  d\u03a3[1]  d\u03a3[2]  "
+               "d\u03a3[3]  u[0]  u[1]  u[2]  u[3]  T  \u03bcB  \u03bcQ  \u03bcS  ");
+  static const char *sub[4] = {"\u2080", "\u2081", "\u2082", "\u2083"};
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) std::fprintf(fbeta, "\u2202%s\u03b2%s  ", sub[i], sub[j]);
+  std::fprintf(fbeta, "\u03f5\n");
+  std::fflush(fbeta);
+}
+
+int Fluid::outputCorona(double tau, bool extendFO) {
+  syncToDevice();
+  long n = 0;
+  vh_fo_stats_t st;
+  const int ncol = extendFO ? 29 : 27;
+  vh_or_die(vh_corona(ctx, tau, ecrit, extendFO, nullptr, 0, &n, &st), "vh_corona");
+  if (ffreeze && n > 0) {
+    std::vector<double> rows((size_t)n * ncol);
+    vh_or_die(vh_corona(ctx, tau, ecrit, extendFO, rows.data(), n, &n, &st), "vh_corona");
+    for (long i = 0; i < n; i++) {
+      for (int k = 0; k < ncol; k++) std::fprintf(ffreeze, "%24.15g", rows[i * ncol + k]);
+      std::fprintf(ffreeze, "\n");
+    }
+    std::fflush(ffreeze);
+  }
+  // the reference adds the corona to EtotSurf and vEff; its n_b sum goes to
+  // a function-local static (src/fld.cpp:877), not to the Fluid's nbSurf
+  EtotSurf += st.e_surf;
+  vEff += st.v_eff;
+  num_corona_cells = (int)n;
+  std::cout << "corona elements : " << n << std::endl;
+  return (int)n;
 }
 
 int Fluid::outputSurface(double tau, bool extendFO) {
@@ -572,15 +645,23 @@ int Fluid::outputSurface(double tau, bool extendFO) {
   const int ncol = extendFO ? 29 : 27;
   vh_or_die(vh_freezeout(ctx, tau, dt, ecrit, extendFO, VH_FO_QPREV, nullptr, 0, &n, &st),
             "vh_freezeout");
-  if (ffreeze && n > 0) {
-    std::vector<double> rows((size_t)n * ncol);
-    vh_or_die(vh_freezeout(ctx, tau, dt, ecrit, extendFO, VH_FO_QPREV, rows.data(), n, &n, &st),
+  const bool wb = vorticityOn && fbeta;
+  if ((ffreeze || wb) && n > 0) {
+    std::vector<double> rows((size_t)n * ncol), beta(wb ? (size_t)n * VH_BETA_COLS : 0);
+    vh_or_die(vh_freezeout_beta(ctx, tau, dt, ecrit, extendFO, VH_FO_QPREV, rows.data(),
+                                wb ? beta.data() : nullptr, n, &n, &st),
               "vh_freezeout");
-    for (long i = 0; i < n; i++) {
+    for (long i = 0; ffreeze && i < n; i++) {
       for (int k = 0; k < ncol; k++) std::fprintf(ffreeze, "%24.15g", rows[i * ncol + k]);
       std::fprintf(ffreeze, "\n");
     }
-    std::fflush(ffreeze);
+    if (ffreeze) std::fflush(ffreeze);
+    for (long i = 0; wb && i < n; i++) {  // beta.dat (src/fld.cpp:663-834)
+      for (int k = 0; k < VH_BETA_COLS; k++)
+        std::fprintf(fbeta, "%24.15g", beta[i * VH_BETA_COLS + k]);
+      std::fprintf(fbeta, "\n");
+    }
+    if (wb) std::fflush(fbeta);
   }
   if (st.err) std::cout << "#### Error (surface): hypercube construction failed ####\n";
   EtotSurf += st.e_surf;
@@ -599,6 +680,7 @@ void Fluid::configure(int precision, int c2pMode, const float *model) {
   ctx = nullptr;
   vh_or_die(vh_create(&g, &e, &t, precision, &ctx), "vh_create");
   if (model) vh_or_die(vh_set_c2p_model(ctx, model), "vh_set_c2p_model");
+  if (vorticityOn) vh_or_die(vh_set_vorticity(ctx, 1), "vh_set_vorticity");
   if (c2pMode != VH_C2P_REFERENCE) vh_or_die(vh_set_c2p_mode(ctx, c2pMode), "vh_set_c2p_mode");
   host_dirty = true;  // the host mirror is the state to upload
   dev_newer = false;

@@ -185,6 +185,11 @@ class Cell {
   void setViscCorrCutFlag(double value);
   double getViscCorrCutFlag(void);
   void Dump(double tau);
+  // vorticity (src/cll.h:167-204): the matrices of all cells live on the
+  // device; enableVorticity switches the whole Fluid on, getDbeta reads the
+  // value the last step stored (clamped to [-100, 100])
+  void enableVorticity();
+  double getDbeta(int i, int j);
 
  private:
   friend class Fluid;
@@ -226,7 +231,18 @@ class Fluid {
   // returns the element count and prints the reference's summary line
   // (tau, EtotSurf, nbSurf, elements, suspicious elements).
   int outputSurface(double tau, bool extendFO = false);
-  // opens dir/freezeout.dat for outputSurface
+  // Fluid::outputCorona (src/fld.cpp:876-1072): the initial state's corona
+  // elements into freezeout.dat (call once before the first step; sets the
+  // corona count of the beta.dat header); returns the element count
+  int outputCorona(double tau, bool extendFO = false);
+  // vorticity (src/fld.h:44-52): every later viscous step also forms the
+  // cells' dbeta, and outputSurface writes beta.dat (opened by initOutput)
+  void enableVorticity();
+  bool vorticity() const { return vorticityOn; }
+  // beta.dat header (src/fld.cpp:174-187); throws std::runtime_error like the
+  // reference when outputCorona has not run
+  void printDbetaHeader();
+  // opens dir/freezeout.dat (and dir/beta.dat with vorticity) for outputSurface
   void initOutput(const char *dir);
   void closeOutput();
   // continue writing into another Fluid's freezeout.dat (the reference's
@@ -241,7 +257,10 @@ class Fluid {
   vh_ctx_t *context() { return ctx; }
   void syncToDevice();    // uploads the host mirror if host code changed it
   void syncFromDevice();  // downloads the state if a step changed it
-  void markDeviceNewer() { dev_newer = true; }
+  void markDeviceNewer() {
+    dev_newer = true;
+    db_stale = true;
+  }
   long ncell() const { return N; }
 
  private:
@@ -257,8 +276,11 @@ class Fluid {
   std::vector<double> hQ, hQh, hQprev, hpi, hpiH, hPi, hPiH, hm, hvcc;
   std::vector<Cell> cells;
   bool host_dirty = false, dev_newer = false;
-  std::FILE *ffreeze = nullptr;
+  std::FILE *ffreeze = nullptr, *fbeta = nullptr;
   double EtotSurf = 0., nbSurf = 0., vEff = 0.;
+  bool vorticityOn = false, db_stale = true;
+  int num_corona_cells = -1;  // -1: outputCorona has not run
+  std::vector<double> hdb;    // host copy of the cells' dbeta (16 x N)
   void hostRead();
   void hostWrite();
 };
@@ -278,6 +300,8 @@ class Hydro {
   // nsteps calls of performStep replayed as HIP graphs (no host round trip)
   void performSteps(int nsteps);
   inline double getTau() const { return tau; }
+  // Hydro::enableVorticity (src/hdo.cpp:60-64)
+  void enableVorticity() { f->enableVorticity(); }
 
  private:
   Fluid *f;
