@@ -34,7 +34,7 @@ events recorded on the context's stream around every launch during a second,
 identical pass.
 
 python bench.py [--gpus 1] [--steps 300] [--warmup 5] [--variants a,b] [--no-cpu]
-                [--no-batch] [--no-full3d] [--no-table] [--quick]
+                [--no-batch] [--no-full3d] [--no-table] [--no-vort] [--quick]
 
 Also measured (field "batch_ebe"): 256 event-by-event boost-invariant grids of
 sides 100..160 with hot-spot initial states in one batch context, against the
@@ -44,7 +44,8 @@ steps, FP64 against FP32 fluxes (element count and sum of dsigma_mu); (field
 "table_eos") the 200x200x100 workload with a tabulated crossover EoS, where
 con2prim has no closed form: FP64 against mixed precision with the Newton
 from v = 0.5 and from the learned guess, and the guess's Newton-update
-ablation.
+ablation; (field "vorticity") the FP64 200x200x100 step with and without the
+vorticity output, and the freeze-out with the beta.dat records.
 """
 import argparse
 import json
@@ -441,6 +442,54 @@ def table_eos_bench(args):
     return out
 
 
+def vorticity_bench(args):
+    """The 3-D viscous workload (200x200x100, FP64) with and without the
+    vorticity output (dbeta formed in every step), and the freeze-out with the
+    beta.dat records: the first call after the steps also resolves the entries
+    the reference inherits from earlier cells; a second call on the same state
+    shows the records alone."""
+    from vhlle_amd import abi, setups
+    cfg = setups.VISCOUS_3D
+    N = cfg["nx"] * cfg["ny"] * cfg["nz"]
+    steps = min(args.steps, 100)
+    ic = setups.gaussian3d_ic(cfg["nx"], cfg["ny"], cfg["nz"], cfg["xlim"], cfg["ylim"],
+                              cfg["zlim"], e0=cfg["e0"], sx=cfg["sx"], sy=cfg["sy"],
+                              eta_plateau=cfg["eta_plateau"], eta_sigma=cfg["eta_sigma"])
+    out = dict(workload="viscous_3d_200x200x100, FP64", cells=N, steps=steps)
+    for on in (False, True):
+        ctx = abi.Context(cfg["nx"], cfg["ny"], cfg["nz"], cfg["xlim"], cfg["ylim"], cfg["zlim"],
+                          trc=abi.make_trc(etaS=cfg["etaS"]))
+        ctx.set_vorticity(on)
+        ctx.set_prim(cfg["tau0"], *ic)
+        ctx.run(cfg["tau0"], cfg["dt"], 3)
+        ctx.set_prim(cfg["tau0"], *ic)
+        t0 = time.perf_counter()
+        st, tau = ctx.run(cfg["tau0"], cfg["dt"], steps)
+        el = time.perf_counter() - t0
+        key = "vorticity_on" if on else "vorticity_off"
+        out[key] = dict(ms_per_step=round(el / steps * 1e3, 4), status=st)
+        if on:
+            t0 = time.perf_counter()
+            rows, _ = ctx.freezeout(tau, cfg["dt"], E_CRIT)
+            t_fo = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            _, beta = ctx.freezeout_beta(tau, cfg["dt"], E_CRIT)
+            t_first = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            ctx.freezeout_beta(tau, cfg["dt"], E_CRIT)
+            t_again = time.perf_counter() - t0
+            out["freezeout"] = dict(elements=len(rows), ms_records=round(t_fo * 1e3, 3),
+                                    ms_with_beta_first=round(t_first * 1e3, 3),
+                                    ms_with_beta_again=round(t_again * 1e3, 3),
+                                    max_abs_dbeta_cart=float(np.max(np.abs(beta[:, 16:32])))
+                                    if len(beta) else 0.0)
+        ctx.close()
+        print(f"[bench] vorticity {key}: {out[key]}", file=sys.stderr, flush=True)
+    out["step_overhead"] = round(out["vorticity_on"]["ms_per_step"] /
+                                 out["vorticity_off"]["ms_per_step"] - 1.0, 4)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -451,6 +500,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-full3d", action="store_true")
     ap.add_argument("--no-table", action="store_true")
+    ap.add_argument("--no-vort", action="store_true")
     ap.add_argument("--quick", action="store_true",
                     help="skip the error runs (profiling); errors reported as null")
     args = ap.parse_args()
@@ -554,6 +604,8 @@ def main():
         out["full_3d_freezeout"] = full3d_bench("closed")
     if not args.no_table:
         out["table_eos"] = table_eos_bench(args)
+    if not args.no_vort:
+        out["vorticity"] = vorticity_bench(args)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(cfg)
     print(json.dumps(out), flush=True)
 
