@@ -964,6 +964,8 @@ int vh_fo_snapshot(vh_ctx_t *c, double tau) {
   return VH_OK;
 }
 
+static_assert(vh::fo::BETA_ROW == VH_BETA_COLS, "beta.dat record width");
+
 namespace {
 // entries of the last step's vorticity matrix that depend on earlier cells
 int vort_settle(vh_ctx_t *c) {
@@ -1057,25 +1059,17 @@ int vh_freezeout_beta(vh_ctx_t *c, double tau, double dt, double e_crit, int ext
   a.Pi = c->Pi;
   a.pi_float = c->precision == VH_PREC_MIXED;
   a.stream = c->st;
-  a.want_rows = (rows != nullptr || wb) && max_rows > 0;
+  a.rows_out = max_rows > 0 ? rows : nullptr;
+  a.beta_out = wb ? beta : nullptr;  // beta.dat columns (src/fld.cpp:663-834)
+  a.max_rows = max_rows;
+  a.rec = rec;
   double sums[vh::fo::NSTAT];
   a.stats = sums;
-  std::vector<double> r;
   long n = 0;
   int err = 0;
-  const int rc = vh::fo::run(a, c->fo_work, r, n, err);
+  const int rc = vh::fo::run(a, c->fo_work, n, err);
   if (rc != VH_OK) return rc;
   if (nrows) *nrows = n;
-  for (long i = 0; a.want_rows && i < n && i < max_rows; i++) {
-    const double *src = &r[i * a.P.ncol];
-    if (rows) std::memcpy(rows + i * rec, src, rec * sizeof(double));
-    if (wb) {  // beta.dat columns (src/fld.cpp:663-834)
-      double *b = beta + i * VH_BETA_COLS;
-      std::memcpy(b, src, 16 * sizeof(double));         // tau x y eta, dsigma, u, T, mu
-      std::memcpy(b + 16, src + rec, 16 * sizeof(double));  // d_i beta_j, Cartesian
-      b[32] = src[a.P.ncol - vh::fo::AUX + 3];              // e at the centroid
-    }
-  }
   if (stats) {
     std::memset(stats, 0, sizeof *stats);
     stats->nelements = (long)sums[0];
@@ -1116,16 +1110,16 @@ int vh_corona(vh_ctx_t *c, double tau, double e_crit, int extend, double *rows, 
   a.pi_float = c->precision == VH_PREC_MIXED;
   a.db = nullptr;
   a.stream = c->st;
-  a.want_rows = rows != nullptr && max_rows > 0;
+  a.rows_out = max_rows > 0 ? rows : nullptr;
+  a.beta_out = nullptr;
+  a.max_rows = max_rows;
+  a.rec = rec;
   double sums[vh::fo::NSTAT];
   a.stats = sums;
-  std::vector<double> r;
   long n = 0;
-  const int rc = vh::fo::run_corona(a, c->fo_work, r, n);
+  const int rc = vh::fo::run_corona(a, c->fo_work, n);
   if (rc != VH_OK) return rc;
   if (nrows) *nrows = n;
-  for (long i = 0; a.want_rows && i < n && i < max_rows; i++)
-    std::memcpy(rows + i * rec, &r[i * a.P.ncol], rec * sizeof(double));
   if (stats) {
     std::memset(stats, 0, sizeof *stats);
     stats->nelements = (long)sums[0];

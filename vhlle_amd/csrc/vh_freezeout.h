@@ -42,26 +42,33 @@ struct FoArgs {
   int pi_float;               // pi, Pi stored in FP32
   const double *db;           // the cells' dbeta (16 x N, clamped on read) with nbeta
   hipStream_t stream;
-  bool want_rows;             // copy the records to the host
+  // host outputs (each may be NULL): the first max_rows records' rec columns,
+  // and their beta.dat records (BETA_ROW columns, with nbeta)
+  double *rows_out, *beta_out;
+  long max_rows;
+  int rec;
   double *stats;              // host, NSTAT sums
 };
+constexpr int BETA_ROW = 33;  // a beta.dat record
 
 // Device buffers of run(), kept between calls (a context owns one) and grown
 // when a call needs more; release() frees them.
 struct Work {
   double *e1 = nullptr, *e0 = nullptr, *rows = nullptr, *dstat = nullptr;
+  double *crow = nullptr, *cbeta = nullptr;  // compacted output records
   unsigned char *flag = nullptr;
   long *cut = nullptr, *sorted = nullptr;
-  int *ncut = nullptr, *cnt = nullptr, *err = nullptr;
+  int *ncut = nullptr, *cnt = nullptr, *err = nullptr, *off = nullptr;
   void *tmp = nullptr;
-  size_t n_cells = 0, n_cubes = 0, n_rows = 0, n_cnt = 0, n_sorted = 0, tmp_bytes = 0;
+  size_t n_cells = 0, n_cubes = 0, n_rows = 0, n_cnt = 0, n_sorted = 0, tmp_bytes = 0,
+         n_off = 0, n_crow = 0, n_cbeta = 0;
   void release();
 };
 
-// elements in the reference's loop order, rows of P.ncol doubles
-int run(const FoArgs &a, Work &w, std::vector<double> &rows, long &nrows, int &errbits);
+// elements in the reference's loop order; nrows = their count
+int run(const FoArgs &a, Work &w, long &nrows, int &errbits);
 // Fluid::outputCorona's elements at P.tau from Q (Qlow = Q, P.dt = 0, nbeta 0)
-int run_corona(const FoArgs &a, Work &w, std::vector<double> &rows, long &nrows);
+int run_corona(const FoArgs &a, Work &w, long &nrows);
 // the construction alone on n hypercubes h (n x 16, [t][x][y][z]) on the GPU
 int cornelius_batch(const double *h, long n, const double *dx, double E0, double *cen,
                     double *nrm, int *nel);

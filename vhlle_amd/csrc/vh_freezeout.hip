@@ -382,6 +382,33 @@ k_corona_rows(CubeGeom g, FoParams P, DevEos eos, const long *__restrict__ cube,
   }
 }
 
+// the selected records in element order, cut to max_rows: the first rec
+// columns into out (may be NULL) and, with outb, the beta.dat record (the
+// first 16 columns, the BETA columns, e_C)
+__global__ void __launch_bounds__(256)
+k_fo_compact(const int *__restrict__ ncut_p, const int *__restrict__ count,
+             const int *__restrict__ off, int per, const double *__restrict__ rows, int ncol,
+             int rec, long max_rows, double *__restrict__ out, double *__restrict__ outb) {
+  const int ncut = *ncut_p;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ncut;
+       t += (long)gridDim.x * blockDim.x) {
+    const int ne = count ? count[t] : 1;
+    const long o = count ? (long)off[t] : t;
+    for (int el = 0; el < ne && o + el < max_rows; el++) {
+      const long r = o + el;
+      const double *src = rows + (t * per + el) * ncol;
+      if (out)
+        for (int k = 0; k < rec; k++) out[r * rec + k] = src[k];
+      if (outb) {
+        double *b = outb + r * BETA_ROW;
+        for (int k = 0; k < 16; k++) b[k] = src[k];
+        for (int k = 0; k < BETA; k++) b[16 + k] = src[rec + k];
+        b[32] = src[ncol - AUX + 3];
+      }
+    }
+  }
+}
+
 // sums over all elements in a fixed order: element count, the suspicious
 // ones, dsigma_mu, dV_eff, the EtotSurf and nbSurf increments.  Pass 1: block
 // b reduces the hypercubes of its chunk [b chunk, (b+1) chunk) into part[b];
@@ -430,7 +457,8 @@ __global__ void k_fo_stats_sum(const double *__restrict__ part, int nb, double *
 
 void Work::release() {
   for (void *p : {(void *)e1, (void *)e0, (void *)rows, (void *)dstat, (void *)flag, (void *)cut,
-                  (void *)sorted, (void *)ncut, (void *)cnt, (void *)err, tmp})
+                  (void *)sorted, (void *)ncut, (void *)cnt, (void *)err, tmp, (void *)off,
+                  (void *)crow, (void *)cbeta})
     if (p) (void)hipFree(p);
   *this = Work();
 }
@@ -451,12 +479,10 @@ hipError_t grow(T **p, size_t &cap, size_t n) {
 
 namespace {
 // the surface (corona false) or the corona (true) elements
-int run_impl(const FoArgs &a, Work &w, std::vector<double> &rows_out, long &nrows, int &errbits,
-             bool corona) {
+int run_impl(const FoArgs &a, Work &w, long &nrows, int &errbits, bool corona) {
   const CubeGeom g{a.nx, a.ny, a.nz, a.nx - 4, a.ny - 4, a.nz - 4, (long)a.nx * a.ny * a.nz};
   nrows = 0;
   errbits = 0;
-  rows_out.clear();
   for (int q = 0; q < NSTAT; q++) a.stats[q] = 0.;
   if (g.mx < 1 || g.my < 1 || g.mz < 1) return VH_OK;  // no interior hypercubes
   const long ncube = (long)g.mx * g.my * g.mz, N = g.N;
@@ -522,7 +548,6 @@ int run_impl(const FoArgs &a, Work &w, std::vector<double> &rows_out, long &nrow
     ok(hipMemcpyAsync(&nc, w.ncut, sizeof(int), hipMemcpyDeviceToHost, st));
     ok(hipStreamSynchronize(st));
   }
-  std::vector<int> hcnt;
   if (rc == VH_OK && nc > 0) {
     const int per = corona ? 1 : MAXE;  // record slots per selected cube
     const size_t slots = (size_t)nc * per;
@@ -559,25 +584,45 @@ int run_impl(const FoArgs &a, Work &w, std::vector<double> &rows_out, long &nrow
       double *sums = w.dstat + (size_t)NSTAT * STAT_BLOCKS;
       hipLaunchKernelGGL(k_fo_stats_sum, dim3(1), dim3(64), 0, st, w.dstat, STAT_BLOCKS, sums);
       ok(hipMemcpyAsync(a.stats, sums, NSTAT * sizeof(double), hipMemcpyDeviceToHost, st));
-      hcnt.assign(nc, 1);
-      if (!corona)
-        ok(hipMemcpyAsync(hcnt.data(), w.cnt, nc * sizeof(int), hipMemcpyDeviceToHost, st));
       ok(hipMemcpyAsync(&errbits, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
       ok(hipStreamSynchronize(st));
     }
-    if (rc == VH_OK) {
-      long total = 0;
-      for (int k = 0; k < nc; k++) total += hcnt[k];
-      nrows = total;
-      if (a.want_rows) {
-        rows_out.resize(total * a.P.ncol);
-        std::vector<double> all(slots * a.P.ncol);
-        ok(hipMemcpy(all.data(), w.rows, all.size() * sizeof(double), hipMemcpyDeviceToHost));
-        long wr = 0;
-        for (int k = 0; k < nc; k++)
-          for (int el = 0; el < hcnt[k]; el++, wr++)
-            std::copy(&all[((size_t)k * per + el) * a.P.ncol],
-                      &all[((size_t)k * per + el + 1) * a.P.ncol], &rows_out[wr * a.P.ncol]);
+    if (rc == VH_OK) nrows = (long)a.stats[0];
+    // the requested records, compacted on the device into the caller's layout
+    const bool wr = a.rows_out != nullptr, wb = a.beta_out != nullptr && a.P.nbeta;
+    const long nout = std::min(nrows, a.max_rows);
+    if (rc == VH_OK && (wr || wb) && nout > 0) {
+      if (!corona) {  // element offsets of the cubes (exclusive sum of the counts)
+        ok(grow(&w.off, w.n_off, (size_t)nc));
+        size_t scanb = 0;
+        ok(hipcub::DeviceScan::ExclusiveSum(nullptr, scanb, w.cnt, w.off, nc, st));
+        if (rc == VH_OK && w.tmp_bytes < scanb) {
+          size_t cap = w.tmp_bytes;
+          char *t = static_cast<char *>(w.tmp);
+          ok(grow(&t, cap, scanb));
+          w.tmp = t;
+          w.tmp_bytes = rc == VH_OK ? scanb : 0;
+        }
+        if (rc == VH_OK) {
+          scanb = w.tmp_bytes;
+          ok(hipcub::DeviceScan::ExclusiveSum(w.tmp, scanb, w.cnt, w.off, nc, st));
+        }
+      }
+      if (wr) ok(grow(&w.crow, w.n_crow, (size_t)nout * a.rec));
+      if (wb) ok(grow(&w.cbeta, w.n_cbeta, (size_t)nout * BETA_ROW));
+      if (rc == VH_OK) {
+        const unsigned grid = (unsigned)std::min<long>((nc + 255) / 256, 2048);
+        hipLaunchKernelGGL(k_fo_compact, dim3(grid), dim3(256), 0, st, w.ncut,
+                           corona ? (const int *)nullptr : w.cnt, w.off, per, w.rows, a.P.ncol,
+                           a.rec, nout, wr ? w.crow : nullptr, wb ? w.cbeta : nullptr);
+        ok(hipGetLastError());
+        if (wr)
+          ok(hipMemcpyAsync(a.rows_out, w.crow, nout * a.rec * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+        if (wb)
+          ok(hipMemcpyAsync(a.beta_out, w.cbeta, nout * BETA_ROW * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+        ok(hipStreamSynchronize(st));
       }
     }
   }
@@ -586,13 +631,13 @@ int run_impl(const FoArgs &a, Work &w, std::vector<double> &rows_out, long &nrow
 }
 }  // namespace
 
-int run(const FoArgs &a, Work &w, std::vector<double> &rows, long &nrows, int &errbits) {
-  return run_impl(a, w, rows, nrows, errbits, false);
+int run(const FoArgs &a, Work &w, long &nrows, int &errbits) {
+  return run_impl(a, w, nrows, errbits, false);
 }
 
-int run_corona(const FoArgs &a, Work &w, std::vector<double> &rows, long &nrows) {
+int run_corona(const FoArgs &a, Work &w, long &nrows) {
   int errbits = 0;
-  return run_impl(a, w, rows, nrows, errbits, true);
+  return run_impl(a, w, nrows, errbits, true);
 }
 
 namespace {

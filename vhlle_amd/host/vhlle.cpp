@@ -643,14 +643,22 @@ int Fluid::outputSurface(double tau, bool extendFO) {
   long n = 0;
   vh_fo_stats_t st;
   const int ncol = extendFO ? 29 : 27;
-  vh_or_die(vh_freezeout(ctx, tau, dt, ecrit, extendFO, VH_FO_QPREV, nullptr, 0, &n, &st),
-            "vh_freezeout");
-  const bool wb = vorticityOn && fbeta;
-  if ((ffreeze || wb) && n > 0) {
-    std::vector<double> rows((size_t)n * ncol), beta(wb ? (size_t)n * VH_BETA_COLS : 0);
-    vh_or_die(vh_freezeout_beta(ctx, tau, dt, ecrit, extendFO, VH_FO_QPREV, rows.data(),
-                                wb ? beta.data() : nullptr, n, &n, &st),
+  const bool wb = vorticityOn && fbeta, want = ffreeze || wb;
+  for (;;) {
+    if (want) {
+      fo_rows.resize((size_t)fo_cap * ncol);
+      if (wb) fo_beta.resize((size_t)fo_cap * VH_BETA_COLS);
+    }
+    const bool has = want && fo_cap > 0;
+    vh_or_die(vh_freezeout_beta(ctx, tau, dt, ecrit, extendFO, VH_FO_QPREV,
+                                has ? fo_rows.data() : nullptr,
+                                has && wb ? fo_beta.data() : nullptr, has ? fo_cap : 0, &n, &st),
               "vh_freezeout");
+    if (!want || n <= fo_cap) break;
+    fo_cap = n + n / 4 + 1024;  // room for the surface to grow, then fetch again
+  }
+  const std::vector<double> &rows = fo_rows, &beta = fo_beta;
+  if (want && n > 0) {
     for (long i = 0; ffreeze && i < n; i++) {
       for (int k = 0; k < ncol; k++) std::fprintf(ffreeze, "%24.15g", rows[i * ncol + k]);
       std::fprintf(ffreeze, "\n");

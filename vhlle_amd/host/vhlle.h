@@ -281,6 +281,10 @@ class Fluid {
   bool vorticityOn = false, db_stale = true;
   int num_corona_cells = -1;  // -1: outputCorona has not run
   std::vector<double> hdb;    // host copy of the cells' dbeta (16 x N)
+  // record buffers of outputSurface, grown ahead of the surface so that a
+  // call usually needs one pass
+  std::vector<double> fo_rows, fo_beta;
+  long fo_cap = 0;
   void hostRead();
   void hostWrite();
 };
