@@ -66,7 +66,7 @@ struct vh_ctx {
   // vorticity (vh_set_vorticity): per-cell dbeta entries and op codes written
   // by the relaxation kernel, resolved on demand (dirty: a step ran since)
   bool vort = false, vort_dirty = false;
-  double *db = nullptr;
+  double *db = nullptr, *vT = nullptr;  // vT: T of the neighbour states (k_vprim)
   unsigned char *dop = nullptr;
   void *vscr = nullptr;
   // freeze-out snapshot (the lower time level of vh_freezeout)
@@ -255,11 +255,13 @@ int enqueue_step_fp64(vh_ctx *c, double dt) {
   // Fluid::correctImagCells (src/fld.cpp:211-273): z, y, x
   ghost_fill(c, 0, Qn, st, rc);
   if (c->viscous) {
-    rc |= timed(c, "visc_prim", [&] { vh::launch_vprim(g, c->eos, c->Qh, c->tp, dt, c->ph, c->pt, st); });
+    rc |= timed(c, "visc_prim", [&] {
+      vh::launch_vprim(g, c->eos, c->Qh, c->tp, dt, c->ph, c->pt, c->vort ? c->vT : nullptr, st);
+    });
     rc |= timed(c, "visc_relax", [&] {
       vh::launch_visc1(g, c->eos, c->trc, Qo, Qn, c->ph, c->pt, c->m, c->m_full, c->pi, c->Pi,
                        c->tp, dt, c->pi0, c->piH0, c->Pi0, c->PiH0, c->status,
-                       c->vort ? c->db : nullptr, c->dop, st);
+                       c->vort ? c->db : nullptr, c->dop, c->vT, st);
     });
     rc |= timed(c, "visc_advect", [&] {
       vh::launch_visc2(g, c->ph, c->pi0, c->piH0, c->Pi0, c->PiH0, c->pi, c->piH, c->Pi, c->PiH,
@@ -568,6 +570,7 @@ int vh_destroy(vh_ctx_t *c) {
   if (c->gtab) (void)hipFree(c->gtab);
   if (c->fo_snap) (void)hipFree(c->fo_snap);
   if (c->db) (void)hipFree(c->db);
+  if (c->vT) (void)hipFree(c->vT);
   if (c->dop) (void)hipFree(c->dop);
   if (c->vscr) (void)hipFree(c->vscr);
   if (c->mlp) (void)hipFree(c->mlp);
@@ -984,14 +987,16 @@ int vh_set_vorticity(vh_ctx_t *c, int on) {
   VH_HIP(hipStreamSynchronize(c->st));
   drop_graphs(c);  // the relaxation kernel's arguments change
   if (c->db) (void)hipFree(c->db);
+  if (c->vT) (void)hipFree(c->vT);
   if (c->dop) (void)hipFree(c->dop);
   if (c->vscr) (void)hipFree(c->vscr);
-  c->db = nullptr;
+  c->db = c->vT = nullptr;
   c->dop = nullptr;
   c->vscr = nullptr;
   c->vort = c->vort_dirty = false;
   if (!on) return VH_OK;
   VH_HIP(hipMalloc(&c->db, 16 * N * sizeof(double)));
+  VH_HIP(hipMalloc(&c->vT, N * sizeof(double)));
   VH_HIP(hipMalloc(&c->dop, N));
   VH_HIP(hipMalloc(&c->vscr, vh::vort_scratch_bytes(c->g)));
   // Cell::enableVorticity: every cell's matrix zero (src/cll.h:167-170)

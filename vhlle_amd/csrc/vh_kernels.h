@@ -23,12 +23,12 @@ void launch_ghost1(int full, int pi_float, const Geom &g, double *Q, double *Qh,
 void launch_update_m(int pass, const Geom &g, double *m, double *dm, const double *tp, double dt,
                      hipStream_t st);
 void launch_vprim(const Geom &g, const DevEos &eos, const double *Qh, const double *tp,
-                  double dt, double *ph, double *pt, hipStream_t st);
+                  double dt, double *ph, double *pt, double *Tt, hipStream_t st);
 void launch_visc1(const Geom &g, const DevEos &eos, const DevTrc &trc, const double *Qprev,
                   const double *Qn, const double *ph, const double *pt, const double *m,
                   int m_full, const double *pi, const double *Pi, const double *tp, double dt,
                   double *pi0, double *piH0, double *Pi0, double *PiH0, int *status,
-                  double *db, unsigned char *dop, hipStream_t st);
+                  double *db, unsigned char *dop, const double *Tt, hipStream_t st);
 void launch_visc2(const Geom &g, const double *ph, const double *pi0, const double *piH0,
                   const double *Pi0, const double *PiH0, double *pi, double *piH, double *Pi,
                   double *PiH, double *vcc, double *vf, const double *tp, double dt,

@@ -439,10 +439,11 @@ k_update_m_apply(Geom g, double *__restrict__ m, const double *__restrict__ dm) 
 // Qh/(tau - dt/2) (the cell-centre state of ISformal, NSquant's own u,
 // visc_flux and visc_source_step) -> ph[8][N]; con2prim of Qh/tau (NSquant's
 // neighbour states, src/hdo.cpp:472-568) -> pt[4][N] = {e, vx, vy, vz}.
-// tau here is tau_{n+1}.
+// tau here is tau_{n+1}.  Tt (may be NULL): T of the pt state where e > 0
+// (the neighbour temperatures of the vorticity rows).
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_VPRIM)
 k_vprim(Geom g, DevEos eos, const double *__restrict__ Qh, const double *__restrict__ tp,
-        double dt, double *__restrict__ ph, double *__restrict__ pt) {
+        double dt, double *__restrict__ ph, double *__restrict__ pt, double *__restrict__ Tt) {
   const double tau = *tp;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.N) return;
@@ -468,6 +469,7 @@ k_vprim(Geom g, DevEos eos, const double *__restrict__ Qh, const double *__restr
   pt[N + c] = r.vx;
   pt[2 * N + c] = r.vy;
   pt[3 * N + c] = r.vz;
+  if (Tt) Tt[c] = r.e > 0. ? eos.T(r.e) : 0.;
 }
 
 // ISformal loop #1 (src/hdo.cpp:691-864) with NSquant (src/hdo.cpp:381-652):
@@ -483,7 +485,7 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
         const double *__restrict__ pi, const double *__restrict__ Pi,
         const double *__restrict__ tp, double dt, double *__restrict__ pi0, double *__restrict__ piH0, double *__restrict__ Pi0,
         double *__restrict__ PiH0, int *__restrict__ status, double *__restrict__ db,
-        unsigned char *__restrict__ dop) {
+        unsigned char *__restrict__ dop, const double *__restrict__ Tt) {
   const double tau = *tp;
   const long c = xcd_block() * blockDim.x + threadIdx.x;
   if (c >= g.N) return;
@@ -563,8 +565,10 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
       const long ca = g.cidx(ix + (d == 1), iy + (d == 2), iz + (d == 3));
       const long cb = g.cidx(ix - (d == 1), iy - (d == 2), iz - (d == 3));
       const double ea = pt[ca], eb = pt[cb];
-      // the reference takes both temperatures before its e test (src/hdo.cpp:474-481)
-      const double TA = VORT ? eos.T(ea) : 0., TB = VORT ? eos.T(eb) : 0.;
+      // the reference takes both temperatures before its e test (src/hdo.cpp:474-481);
+      // k_vprim stored them where e > 0 (the same evaluation)
+      const double TA = VORT ? (ea > 0. ? Tt[ca] : eos.T(ea)) : 0.;
+      const double TB = VORT ? (eb > 0. ? Tt[cb] : eos.T(eb)) : 0.;
       if (ea > 0. && eb > 0.) {
         const double avx = pt[N + ca], avy = pt[2 * N + ca], avz = pt[3 * N + ca];
         const double bvx = pt[N + cb], bvy = pt[2 * N + cb], bvz = pt[3 * N + cb];
@@ -1111,23 +1115,24 @@ void launch_update_m(int pass, const Geom &g, double *m, double *dm, const doubl
 }
 
 void launch_vprim(const Geom &g, const DevEos &eos, const double *Qh, const double *tp,
-                  double dt, double *ph, double *pt, hipStream_t st) {
-  hipLaunchKernelGGL(k_vprim, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, Qh, tp, dt, ph, pt);
+                  double dt, double *ph, double *pt, double *Tt, hipStream_t st) {
+  hipLaunchKernelGGL(k_vprim, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, Qh, tp, dt, ph, pt,
+                     Tt);
 }
 
 void launch_visc1(const Geom &g, const DevEos &eos, const DevTrc &trc, const double *Qprev,
                   const double *Qn, const double *ph, const double *pt, const double *m,
                   int m_full, const double *pi, const double *Pi, const double *tp, double dt,
                   double *pi0, double *piH0, double *Pi0, double *PiH0, int *status,
-                  double *db, unsigned char *dop, hipStream_t st) {
+                  double *db, unsigned char *dop, const double *Tt, hipStream_t st) {
   if (db)
     hipLaunchKernelGGL(k_visc1<true>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, trc, Qprev,
                        Qn, ph, pt, m, m_full, pi, Pi, tp, dt, pi0, piH0, Pi0, PiH0, status, db,
-                       dop);
+                       dop, Tt);
   else
     hipLaunchKernelGGL(k_visc1<false>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, trc, Qprev,
                        Qn, ph, pt, m, m_full, pi, Pi, tp, dt, pi0, piH0, Pi0, PiH0, status,
-                       nullptr, nullptr);
+                       nullptr, nullptr, nullptr);
 }
 
 void launch_visc2(const Geom &g, const double *ph, const double *pi0, const double *piH0,
