@@ -152,6 +152,11 @@ int vh_set_prim(vh_ctx_t *ctx, double tau, const double *e, const double *nb,
                 const double *nq, const double *ns, const double *vx,
                 const double *vy, const double *vz);
 
+/* Cell::getPrimVar of every cell at tau (src/cll.cpp:90-96, transformPV of
+ * Q/tau), the inputs of the reference's cell outputs (VtkOutput::write,
+ * src/vtk.cpp:38-120): prim, 8 x N SoA, rows e, p, nb, nq, ns, vx, vy, vz. */
+int vh_prim_fields(vh_ctx_t *ctx, double tau, double *prim);
+
 /* Hydro::performStep (src/hdo.cpp:1016-1134): one predictor-corrector step
  * (+ Israel-Stewart step when viscous) from *tau with timestep dt; *tau is
  * advanced by dt.  *status receives VH_ST_* bits (0 = the reference would

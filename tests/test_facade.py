@@ -285,3 +285,76 @@ def test_driver_vorticity_beta_file(tmp_path, oracle):
     scale = np.max(np.abs(ref[:, 16:32]))
     assert np.max(np.abs(beta[:, 16:32] - ref[:, 16:32])) <= 1e-9 * scale
     assert np.allclose(beta[:, 32], ref[:, 32], rtol=1e-12)
+
+
+def _vtk_blocks(path):
+    """header lines and the numeric blocks of a VTK structured-points file"""
+    lines = path.read_text().splitlines()
+    head, blocks, cur = lines[:8], [], None
+    for l in lines[8:]:
+        if l.startswith(("SCALARS", "VECTORS")):
+            cur = []
+            blocks.append((l, cur))
+        elif l.startswith("LOOKUP_TABLE"):
+            continue
+        else:
+            cur.extend(float(x) for x in l.split())
+    return head, blocks
+
+
+@pytest.mark.gpu
+def test_driver_vtk_files_match_oracle(tmp_path, oracle):
+    """VTK_output_values: before every step the driver writes one VTK file per
+    quantity (src/vtk.cpp), with the reference's header and "%.3f" values of
+    the state at that tau; unknown names are skipped with the reference's
+    message."""
+    nx, ny, nz = 12, 10, 6
+    xl, yl, zl = (-2.75, 2.75), (-2.25, 2.25), (-0.5, 0.5)
+    p = _params(tmp_path, eosType=5, icModel=10, icE0=8.0, icSigmaX=1.2, icSigmaY=1.0,
+                icEtaPlateau=0.3, icEtaSigma=0.4, nx=nx, ny=ny, nz=nz, xmin=xl[0], xmax=xl[1],
+                ymin=yl[0], ymax=yl[1], etamin=zl[0], etamax=zl[1], tau0=1.0, tauMax=1.1,
+                dtau=0.05, etaS=0.1, tauGridResize=100.0, stopOnEmptySurface=0,
+                VTK_output_values="eps,T,bogus,v,pi,Pi,p")
+    r = subprocess.run([EXE, "-params", p, "-outputDir", str(tmp_path)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Given quantity 'bogus' is not an implemented VTK quantity" in r.stdout
+    e, vx, vy, vz = setups.gaussian3d_ic(nx, ny, nz, xl, yl, zl, e0=8.0, sx=1.2, sy=1.0,
+                                         eta_plateau=0.3, eta_sigma=0.4)
+    o = oracle.Solver(nx, ny, nz, xl, yl, zl, trc=oracle.make_trc(etaS=0.1))
+    o.set_prim(1.0, e, vx, vy, vz)
+    cnst = (16 + 0.5 * 21.0 * 2.5) * np.pi ** 2 / 30.0 / 0.197326968 ** 3
+    tau = 1.0
+    sp = [(l[1] - l[0]) / (n - 1) for l, n in zip((xl, yl, zl), (nx, ny, nz))]
+    for k in range(3):
+        s = o.get_state()
+        pr = oracle.c2p((s["Q"] / tau).T.copy())[0]
+        want = {"eps": pr[:, 0], "p": pr[:, 1], "T": (np.maximum(pr[:, 0], 0) / cnst) ** 0.25,
+                "Pi": s["Pi"][0]}
+        for q in ("eps", "T", "p", "Pi", "v", "pi"):
+            head, blocks = _vtk_blocks(tmp_path / f"{q}_taustep{k:05d}.vtk")
+            assert head[:4] == ["# vtk DataFile Version 2.0", q, "ASCII",
+                                "DATASET STRUCTURED_POINTS"]
+            assert head[4] == f"DIMENSIONS {nx} {ny} {nz}"
+            assert head[5] == "SPACING %g %g %g" % tuple(sp)
+            assert head[6] == "ORIGIN %g %g %g" % (xl[0], yl[0], zl[0])
+            assert head[7] == f"POINT_DATA {nx * ny * nz}"
+            if q == "v":
+                assert blocks[0][0] == "VECTORS v double"
+                got = np.array(blocks[0][1]).reshape(-1, 3)
+                ref = pr[:, 5:8]
+            elif q == "pi":
+                assert [b[0] for b in blocks] == [f"SCALARS pi{i}{j} double 1"
+                                                  for i in range(4) for j in range(4)]
+                got = np.array([b[1] for b in blocks]).T
+                ref = np.stack([s["pi"][(i * (i + 1)) // 2 + j if j <= i else
+                                        (j * (j + 1)) // 2 + i]
+                                for i in range(4) for j in range(4)]).T
+            else:
+                assert blocks[0][0] == f"SCALARS {q} double 1"
+                got, ref = np.array(blocks[0][1]), want[q]
+            assert got.shape == ref.shape, q
+            assert np.max(np.abs(got - ref)) <= 5.0001e-4, (k, q)
+        assert not (tmp_path / f"bogus_taustep{k:05d}.vtk").exists()
+        rc, tau = o.run(tau, 0.05, 1)
+        assert rc == 0

@@ -189,6 +189,7 @@ def lib():
         "vh_corona": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp,
                                      ctypes.c_long, ctypes.POINTER(ctypes.c_long),
                                      ctypes.POINTER(FoStats)]),
+        "vh_prim_fields": (ctypes.c_int, [vp, ctypes.c_double, _dp]),
         "vh_set_vorticity": (ctypes.c_int, [vp, ctypes.c_int]),
         "vh_get_dbeta": (ctypes.c_int, [vp, _dp]),
         "vh_freezeout_beta": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double,
@@ -215,7 +216,7 @@ def exported_symbols():
             "vh_set_c2p_model", "vh_mlp_harvest", "vh_set_stage", "vh_get_stage", "vh_create_batch", "vh_hotspot_ic",
             "vh_copy_bandwidth", "vh_fo_snapshot", "vh_freezeout", "vh_cornelius",
             "vh_eos_table", "vh_set_vorticity", "vh_get_dbeta", "vh_freezeout_beta",
-            "vh_corona"]
+            "vh_corona", "vh_prim_fields"]
 
 
 def _p(a):
@@ -490,6 +491,12 @@ class Context:
                   "vh_corona")
         return rows[:min(n.value, max_rows)], dict(nelements=st.nelements, v_eff=st.v_eff,
                                                    e_surf=st.e_surf, nb_surf=st.nb_surf)
+
+    def prim_fields(self, tau):
+        """Cell::getPrimVar of every cell: (8, N) rows e, p, nb, nq, ns, vx, vy, vz"""
+        out = np.zeros(8 * self.N)
+        check(lib().vh_prim_fields(self.h, tau, _p(out)), "vh_prim_fields")
+        return out.reshape(8, self.N)
 
     def set_vorticity(self, on=True):
         check(lib().vh_set_vorticity(self.h, int(on)), "vh_set_vorticity")

@@ -67,6 +67,7 @@ struct vh_ctx {
   // by the relaxation kernel, resolved on demand (dirty: a step ran since)
   bool vort = false, vort_dirty = false;
   double *db = nullptr, *vT = nullptr;  // vT: T of the neighbour states (k_vprim)
+  double *pf = nullptr;                 // primitive fields (vh_prim_fields)
   unsigned char *dop = nullptr;
   void *vscr = nullptr;
   // freeze-out snapshot (the lower time level of vh_freezeout)
@@ -571,6 +572,7 @@ int vh_destroy(vh_ctx_t *c) {
   if (c->fo_snap) (void)hipFree(c->fo_snap);
   if (c->db) (void)hipFree(c->db);
   if (c->vT) (void)hipFree(c->vT);
+  if (c->pf) (void)hipFree(c->pf);
   if (c->dop) (void)hipFree(c->dop);
   if (c->vscr) (void)hipFree(c->vscr);
   if (c->mlp) (void)hipFree(c->mlp);
@@ -1003,6 +1005,17 @@ int vh_set_vorticity(vh_ctx_t *c, int on) {
   VH_HIP(hipMemsetAsync(c->db, 0, 16 * N * sizeof(double), c->st));
   VH_HIP(hipStreamSynchronize(c->st));
   c->vort = true;
+  return VH_OK;
+}
+
+int vh_prim_fields(vh_ctx_t *c, double tau, double *prim) {
+  if (!c || !prim || !(tau > 0.)) return VH_ERR_ARG;
+  const long N = c->g.N;
+  if (!c->pf) VH_HIP(hipMalloc(&c->pf, 8 * N * sizeof(double)));
+  vh::launch_prim_fields(N, c->eos, c->Q[c->cur], tau, c->pf, c->st);
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipMemcpyAsync(prim, c->pf, 8 * N * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  VH_HIP(hipStreamSynchronize(c->st));
   return VH_OK;
 }
 

@@ -43,5 +43,7 @@ void launch_set_prim(const Geom &g, const DevEos &eos, double tau, const double 
 void launch_c2p_batch(const DevEos &eos, double *Q, long n, double *out, int *iters,
                       hipStream_t st);
 void launch_p2c_batch(const double *prim, long n, double *Q, hipStream_t st);
+void launch_prim_fields(long N, const DevEos &eos, const double *Q, double tau, double *prim,
+                        hipStream_t st);
 
 }  // namespace vh

@@ -1164,6 +1164,33 @@ void launch_set_prim(const Geom &g, const DevEos &eos, double tau, const double 
                      m);
 }
 
+// Cell::getPrimVar of every cell (src/cll.cpp:90-96): con2prim of a copy of
+// Q/tau, prim[k * N + cell] = e, p, nb, nq, ns, vx, vy, vz
+__global__ void __launch_bounds__(256)
+k_prim_fields(long N, DevEos eos, const double *__restrict__ Q, double tau,
+              double *__restrict__ prim) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  double q[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) q[k] = Q[k * N + c] / tau;
+  Prim r;
+  c2p_ref(eos, q, r);
+  prim[c] = r.e;
+  prim[N + c] = r.p;
+  prim[2 * N + c] = r.nb;
+  prim[3 * N + c] = r.nq;
+  prim[4 * N + c] = r.ns;
+  prim[5 * N + c] = r.vx;
+  prim[6 * N + c] = r.vy;
+  prim[7 * N + c] = r.vz;
+}
+
+void launch_prim_fields(long N, const DevEos &eos, const double *Q, double tau, double *prim,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_prim_fields, dim3(blocks(N)), dim3(256), 0, st, N, eos, Q, tau, prim);
+}
+
 void launch_c2p_batch(const DevEos &eos, double *Q, long n, double *out, int *iters,
                       hipStream_t st) {
   hipLaunchKernelGGL(k_c2p_batch, dim3(blocks(n)), dim3(256), 0, st, eos, Q, n, out, iters);

@@ -14,7 +14,8 @@
 // 3 Newton from v = 0.5), c2pModel (a file written by -trainC2P);
 // stopOnEmptySurface (1, the reference's loop condition: stop at the first
 // step without freeze-out elements, which for nz < 5 is the first step).
-// vorticity 1 (FP64 only) writes beta.dat next to freezeout.dat.
+// vorticity 1 (FP64 only) writes beta.dat next to freezeout.dat;
+// VTK_output_values (e.g. eps,T,v,pi) writes VTK files before every step.
 // tauGridResize: once tau reaches it the grid doubles its transverse extent
 // (expandGrid2x, src/main.cpp:246-271).
 // `vhlle [-params file] -trainC2P file [-seed S]` harvests 4 M conserved ->
@@ -295,10 +296,12 @@ int main(int argc, char **argv) {
   f->initOutput(outputDir.c_str());
   f->outputCorona(tau0, freezeoutExtend);  // src/main.cpp:389-391
   if (vorticityOn) f->printDbetaHeader();
+  VtkOutput vtk_out(outputDir, eos, xmin, ymin, etamin);  // src/main.cpp:395-396
   long steps = 0;
   int nelements = 0;
   const auto t0 = chrono::steady_clock::now();
   do {
+    if (!vtk_values.empty()) vtk_out.write(*h, vtk_values);  // src/main.cpp:402-404
     // sub-steps while dtau exceeds tau * d(eta) (src/main.cpp:405-419)
     int nSub = 1;
     while (dtau / nSub > 1.0 * h->getTau() * (etamax - etamin) / (nz - 1)) nSub *= 2;

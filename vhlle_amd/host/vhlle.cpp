@@ -8,6 +8,8 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <map>
+#include <sstream>
 #include <stdexcept>
 
 // ------------------------------------------------------------- helpers
@@ -890,4 +892,105 @@ void ICGubser::setIC(Fluid *f, EoS *eos, double tau) {
   std::cout << "average initial flow = " << avv_num / avv_den << std::endl;
   std::cout << "total energy = " << Etotal * f->getDx() * f->getDy() * f->getDz() * tau
             << std::endl;
+}
+
+// ------------------------------------------------------------ VtkOutput
+namespace {
+// kind of each quantity VtkOutput writes (src/vtk.h:33-46)
+const std::map<std::string, char> &vtk_quantities() {
+  static const std::map<std::string, char> q = {
+      {"eps", 's'}, {"mub", 's'}, {"muq", 's'}, {"mus", 's'}, {"nb", 's'}, {"nq", 's'},
+      {"ns", 's'},  {"p", 's'},   {"Pi", 's'},  {"pi", 't'},  {"T", 's'},  {"v", 'v'}};
+  return q;
+}
+}  // namespace
+
+bool VtkOutput::is_quantity_implemented(const std::string &quantity) {
+  return vtk_quantities().count(quantity) != 0;
+}
+
+std::string VtkOutput::make_filename(const std::string &descr, int counter) {
+  char suffix[24];
+  std::snprintf(suffix, sizeof(suffix), "_taustep%05i.vtk", counter);
+  return path_ + "/" + descr + suffix;
+}
+
+void VtkOutput::write(const Hydro &h, const std::string &quantities) {
+  Fluid *f = h.getFluid();
+  const int nx = f->getNX(), ny = f->getNY(), nz = f->getNZ();
+  const long N = f->ncell();
+  std::vector<double> prim;  // e, p, nb, nq, ns, vx, vy, vz (8 x N), fetched once
+  auto fields = [&]() -> const std::vector<double> & {
+    if (prim.empty()) {
+      f->syncToDevice();
+      prim.resize(8 * N);
+      vh_or_die(vh_prim_fields(f->context(), h.getTau(), prim.data()), "vh_prim_fields");
+    }
+    return prim;
+  };
+  std::stringstream ss(quantities);
+  std::string q;
+  while (std::getline(ss, q, ',')) {
+    if (!is_quantity_implemented(q)) {
+      std::cout << "Given quantity '" << q << "' is not an "
+                << "implemented VTK quantity. This entry will be skipped." << std::endl;
+      continue;
+    }
+    const std::string fn = make_filename(q, vtk_output_counter_);
+    std::FILE *out = std::fopen(fn.c_str(), "w");
+    if (!out) {
+      std::cout << "cannot open " << fn << std::endl;
+      continue;
+    }
+    std::fprintf(out,
+                 "# vtk DataFile Version 2.0\n%s\nASCII\nDATASET STRUCTURED_POINTS\n"
+                 "DIMENSIONS %d %d %d\nSPACING %g %g %g\nORIGIN %g %g %g\nPOINT_DATA %d\n",
+                 q.c_str(), nx, ny, nz, f->getDx(), f->getDy(), f->getDz(), xmin_, ymin_,
+                 etamin_, nx * ny * nz);
+    const char kind = vtk_quantities().at(q);
+    if (kind == 's') {
+      std::fprintf(out, "SCALARS %s double 1\nLOOKUP_TABLE default\n", q.c_str());
+      const std::vector<double> &pr = fields();
+      for (int iz = 0; iz < nz; iz++)
+        for (int iy = 0; iy < ny; iy++) {
+          for (int ix = 0; ix < nx; ix++) {
+            const long c = ix + (long)nx * (iy + (long)ny * iz);
+            const double e = pr[c], p = pr[N + c], nb = pr[2 * N + c], nq = pr[3 * N + c],
+                         ns = pr[4 * N + c];
+            double v = 0.;
+            if (q == "eps") v = e;
+            else if (q == "nb") v = nb;
+            else if (q == "nq") v = nq;
+            else if (q == "ns") v = ns;
+            else if (q == "p") v = p;
+            else if (q == "Pi") v = f->getCell(ix, iy, iz)->getPi();
+            else {  // EoS::eos of the cell's state (src/vtk.cpp:64-76)
+              double T, mub, muq, mus, pp;
+              eos_->eos(e, nb, nq, ns, T, mub, muq, mus, pp);
+              v = q == "mub" ? mub : q == "muq" ? muq : q == "mus" ? mus : T;
+            }
+            std::fprintf(out, "%.3f ", v);
+          }
+          std::fprintf(out, "\n");
+        }
+    } else if (kind == 'v') {
+      std::fprintf(out, "VECTORS %s double\n", q.c_str());
+      const std::vector<double> &pr = fields();
+      for (long c = 0; c < N; c++)
+        std::fprintf(out, "%.3f %.3f %.3f\n", pr[5 * N + c], pr[6 * N + c], pr[7 * N + c]);
+    } else {
+      for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+          std::fprintf(out, "SCALARS %s%d%d double 1\nLOOKUP_TABLE default\n", q.c_str(), i, j);
+          for (int iz = 0; iz < nz; iz++)
+            for (int iy = 0; iy < ny; iy++) {
+              for (int ix = 0; ix < nx; ix++)
+                std::fprintf(out, "%.3f ", f->getCell(ix, iy, iz)->getpi(i, j));
+              std::fprintf(out, "\n");
+            }
+        }
+    }
+    std::fclose(out);
+  }
+  vtk_output_counter_++;
 }

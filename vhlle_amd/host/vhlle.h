@@ -315,6 +315,27 @@ class Hydro {
   void check(int rc, int status);
 };
 
+// ------------------------------------------------------------ VtkOutput
+// ASCII VTK structured-points files of cell fields (src/vtk.h:16-77,
+// src/vtk.cpp:1-181): one file per quantity and call, named
+// <path>/<quantity>_taustepNNNNN.vtk, values "%.3f".  Quantities: eps, p,
+// nb, nq, ns, T, mub, muq, mus, Pi (scalars), v (vector), pi (16 scalars
+// pi00..pi33).  The primitive fields come from the device (vh_prim_fields).
+class VtkOutput {
+ public:
+  VtkOutput(std::string path, EoS *eos, double xmin, double ymin, double etamin)
+      : path_(std::move(path)), eos_(eos), xmin_(xmin), ymin_(ymin), etamin_(etamin) {}
+  void write(const Hydro &h, const std::string &quantities);
+  bool is_quantity_implemented(const std::string &quantity);
+  std::string make_filename(const std::string &descr, int counter);
+
+ private:
+  std::string path_;
+  EoS *eos_;
+  double xmin_, ymin_, etamin_;
+  int vtk_output_counter_ = 0;
+};
+
 // Gubser analytic IC (src/icGubser.h:6-12, src/icGubser.cpp:17-62), with an
 // optional energy scale (1 = the reference's normalisation).
 class ICGubser {
