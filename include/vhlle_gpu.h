@@ -157,6 +157,22 @@ int vh_set_prim(vh_ctx_t *ctx, double tau, const double *e, const double *nb,
  * src/vtk.cpp:38-120): prim, 8 x N SoA, rows e, p, nb, nq, ns, vx, vy, vz. */
 int vh_prim_fields(vh_ctx_t *ctx, double tau, double *prim);
 
+/* The grid totals that Fluid::outputSurface and Fluid::outputCorona print
+ * (src/fld.cpp:545-606, 852-857, 889-945, 1071-1084), summed over the
+ * interior cells (ix, iy, iz in [2, n-3]) of the state at tau.  out[16]:
+ * E, Efull, S, Px, Nb1, Nb2 (before the factor dx dy dz), vt_num, vt_den,
+ * vxvy_num, vxvy_den, pi0x_num, pi0x_den, txxyy_num, txxyy_den, the cells
+ * with e > e_crit, and those of them with viscCorrCut < 0.9.  central != 0:
+ * the vt, vxvy and txxyy sums only over nz/2-3 < iz < nz/2+3, as
+ * outputSurface; 0: over all, as outputCorona.  Single-grid contexts. */
+int vh_fluid_stats(vh_ctx_t *ctx, double tau, double e_crit, int central, double *out);
+/* The state of n listed cells (cell indices) at tau, VH_CELL_COLS doubles
+ * each: Cell::getPrimVar (e, p, nb, nq, ns, vx, vy, vz), pi (10, index44),
+ * Pi, viscCorrCut, 0 -- the inputs of Fluid::outputGnuplot
+ * (src/fld.cpp:425-504). */
+enum { VH_CELL_COLS = 21 };
+int vh_cells(vh_ctx_t *ctx, double tau, const long *cells, long n, double *out);
+
 /* Hydro::performStep (src/hdo.cpp:1016-1134): one predictor-corrector step
  * (+ Israel-Stewart step when viscous) from *tau with timestep dt; *tau is
  * advanced by dt.  *status receives VH_ST_* bits (0 = the reference would

@@ -13,6 +13,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "vhlle_amd", "bin", "vhlle")
 
 
+def _surface_counts(out):
+    """element counts of the per-step lines of Fluid::outputSurface (tau, E,
+    Efull, nbSurf, S, EtotSurf, elements, suspicious, cut fraction)"""
+    return [int(l.split()[6]) for l in out.splitlines()
+            if len(l.split()) == 9 and l.split()[6].isdigit() and l.split()[7].isdigit()]
+
+
 def _params(tmp_path, **kw):
     lines = [f"{k} {v}" for k, v in kw.items()]
     lines.insert(0, "! test params")
@@ -134,8 +141,7 @@ def test_driver_freezeout_file(tmp_path):
                        text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     rows = np.loadtxt(tmp_path / "freezeout.dat", ndmin=2)
-    counts = [int(l.split()[3]) for l in r.stdout.splitlines()
-              if len(l.split()) == 5 and l.split()[3].isdigit()]
+    counts = _surface_counts(r.stdout)
     corona = int(re.search(r"corona elements : (\d+)", r.stdout).group(1))
     print(f"{len(rows)} rows, {corona} corona; last counts {counts[-5:]}", flush=True)
     assert rows.shape[1] == 27 and len(rows) == corona + sum(counts) and counts[-1] == 0
@@ -356,5 +362,75 @@ def test_driver_vtk_files_match_oracle(tmp_path, oracle):
             assert got.shape == ref.shape, q
             assert np.max(np.abs(got - ref)) <= 5.0001e-4, (k, q)
         assert not (tmp_path / f"bogus_taustep{k:05d}.vtk").exists()
+        rc, tau = o.run(tau, 0.05, 1)
+        assert rc == 0
+
+
+@pytest.mark.gpu
+def test_driver_gnuplot_files_and_totals(tmp_path, oracle):
+    """The reference's default outputs (freezeoutOnly 0): outx/outy/outz/
+    outdiag.dat hold the lines through the grid centre at tau0 and after every
+    step, out.aniz.dat one line per surface call after its header, and the
+    per-step stdout line the grid totals E, Efull, S; freezeout.dat is renamed
+    from freezeout.dat.unfinished at the end.  Values against the oracle's
+    state at the same tau (6 significant digits in the files)."""
+    nx, ny, nz = 12, 10, 8
+    xl, yl, zl = (-2.75, 2.75), (-2.25, 2.25), (-0.7, 0.7)
+    p = _params(tmp_path, eosType=5, icModel=10, icE0=8.0, icSigmaX=1.2, icSigmaY=1.0,
+                icEtaPlateau=0.3, icEtaSigma=0.4, nx=nx, ny=ny, nz=nz, xmin=xl[0], xmax=xl[1],
+                ymin=yl[0], ymax=yl[1], etamin=zl[0], etamax=zl[1], tau0=1.0, tauMax=1.1,
+                dtau=0.05, etaS=0.1, tauGridResize=100.0, stopOnEmptySurface=0)
+    r = subprocess.run([EXE, "-params", p, "-outputDir", str(tmp_path)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert (tmp_path / "freezeout.dat").exists()
+    assert not (tmp_path / "freezeout.dat.unfinished").exists()
+    for f in ("outx.visc.dat", "outy.visc.dat", "diag.visc.dat"):
+        assert (tmp_path / f).read_text() == ""
+    blocks = (tmp_path / "outx.dat").read_text().strip("\n").split("\n\n")
+    aniz = (tmp_path / "out.aniz.dat").read_text().splitlines()
+    assert aniz[0].startswith("#  tau  <<v_T>>")
+    lines = [l.split() for l in r.stdout.splitlines()
+             if len(l.split()) == 9 and l.split()[6].isdigit() and l.split()[7].isdigit()]
+    assert len(blocks) == 1 + len(lines) == 4 and len(aniz) == 2 + len(lines)
+
+    e, vx, vy, vz = setups.gaussian3d_ic(nx, ny, nz, xl, yl, zl, e0=8.0, sx=1.2, sy=1.0,
+                                         eta_plateau=0.3, eta_sigma=0.4)
+    o = oracle.Solver(nx, ny, nz, xl, yl, zl, trc=oracle.make_trc(etaS=0.1))
+    o.set_prim(1.0, e, vx, vy, vz)
+    cnst = (16 + 0.5 * 21.0 * 2.5) * np.pi ** 2 / 30.0 / 0.197326968 ** 3
+    X, Y, H = setups.axes(nx, ny, nz, xl, yl, zl)
+    dV = np.prod([(l[1] - l[0]) / (n - 1) for l, n in zip((xl, yl, zl), (nx, ny, nz))])
+    tau = 1.0
+    for k in range(4):
+        s = o.get_state()
+        pr = oracle.c2p((s["Q"] / tau).T.copy())[0]
+        ee, pp, nb = pr[:, 0], pr[:, 1], pr[:, 2]
+        eta = H.ravel()
+        rap = eta + 0.5 * np.log((1 + pr[:, 7]) / (1 - pr[:, 7]))
+        cx = pr[:, 5] * np.cosh(rap - eta) / np.cosh(rap)
+        cy = pr[:, 6] * np.cosh(rap - eta) / np.cosh(rap)
+        T = (np.maximum(ee, 0) / cnst) ** 0.25
+        # the x line through (ny/2, nz/2)
+        c = np.arange(nx) + nx * (ny // 2 + ny * (nz // 2))
+        got = np.array([[float(x) for x in l.split()] for l in blocks[k].splitlines()])
+        want = np.column_stack([np.full(nx, tau), X.ravel()[c], cx[c], cy[c], ee[c], nb[c], T[c],
+                                np.zeros(nx)])
+        assert np.allclose(got[:, :8], want, rtol=1e-5, atol=1e-9), k
+        pi = s["pi"]
+        assert np.allclose(got[:, 8], pi[0][c], rtol=1e-5, atol=1e-9)
+        if k > 0:  # the totals of the step's surface call
+            th = np.tanh(rap)
+            den = 1 - cx ** 2 - cy ** 2 - th ** 2
+            dz = (zl[1] - zl[0]) / (nz - 1)
+            ci = (np.sinh(eta + 0.5 * dz) - np.sinh(eta - 0.5 * dz)) / dz
+            si = (np.cosh(eta + 0.5 * dz) - np.cosh(eta - 0.5 * dz)) / dz
+            Ec = tau * (ee + pp) / den * (ci - th * si) - tau * pp * ci
+            inner = np.zeros((nz, ny, nx), dtype=bool)
+            inner[2:-2, 2:-2, 2:-2] = True
+            E = np.sum(Ec[inner.ravel()]) * dV
+            line = lines[k - 1]
+            assert float(line[0]) == pytest.approx(tau, rel=1e-5)
+            assert float(line[1]) == pytest.approx(E, rel=1e-5)
         rc, tau = o.run(tau, 0.05, 1)
         assert rc == 0

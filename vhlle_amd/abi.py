@@ -190,6 +190,10 @@ def lib():
                                      ctypes.c_long, ctypes.POINTER(ctypes.c_long),
                                      ctypes.POINTER(FoStats)]),
         "vh_prim_fields": (ctypes.c_int, [vp, ctypes.c_double, _dp]),
+        "vh_fluid_stats": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                          _dp]),
+        "vh_cells": (ctypes.c_int, [vp, ctypes.c_double, ctypes.POINTER(ctypes.c_long),
+                                    ctypes.c_long, _dp]),
         "vh_set_vorticity": (ctypes.c_int, [vp, ctypes.c_int]),
         "vh_get_dbeta": (ctypes.c_int, [vp, _dp]),
         "vh_freezeout_beta": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double,
@@ -216,7 +220,7 @@ def exported_symbols():
             "vh_set_c2p_model", "vh_mlp_harvest", "vh_set_stage", "vh_get_stage", "vh_create_batch", "vh_hotspot_ic",
             "vh_copy_bandwidth", "vh_fo_snapshot", "vh_freezeout", "vh_cornelius",
             "vh_eos_table", "vh_set_vorticity", "vh_get_dbeta", "vh_freezeout_beta",
-            "vh_corona", "vh_prim_fields"]
+            "vh_corona", "vh_prim_fields", "vh_fluid_stats", "vh_cells"]
 
 
 def _p(a):
@@ -497,6 +501,20 @@ class Context:
         out = np.zeros(8 * self.N)
         check(lib().vh_prim_fields(self.h, tau, _p(out)), "vh_prim_fields")
         return out.reshape(8, self.N)
+
+    def fluid_stats(self, tau, ecrit, central=True):
+        """the 16 grid totals of the reference's output routines"""
+        out = np.zeros(16)
+        check(lib().vh_fluid_stats(self.h, tau, ecrit, int(central), _p(out)), "vh_fluid_stats")
+        return out
+
+    def cells(self, tau, idx):
+        """getPrimVar (8), pi (10), Pi, viscCorrCut, 0 of the listed cells"""
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        out = np.zeros((len(idx), 21))
+        check(lib().vh_cells(self.h, tau, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
+                             len(idx), _p(out)), "vh_cells")
+        return out
 
     def set_vorticity(self, on=True):
         check(lib().vh_set_vorticity(self.h, int(on)), "vh_set_vorticity")

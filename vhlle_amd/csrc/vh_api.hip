@@ -68,6 +68,10 @@ struct vh_ctx {
   bool vort = false, vort_dirty = false;
   double *db = nullptr, *vT = nullptr;  // vT: T of the neighbour states (k_vprim)
   double *pf = nullptr;                 // primitive fields (vh_prim_fields)
+  double *fsb = nullptr;                // grid totals: 16 x 256 partial sums + 16
+  long *cidx = nullptr;                 // vh_cells: cell list and records
+  double *cbuf = nullptr;
+  long ccap = 0;
   unsigned char *dop = nullptr;
   void *vscr = nullptr;
   // freeze-out snapshot (the lower time level of vh_freezeout)
@@ -573,6 +577,9 @@ int vh_destroy(vh_ctx_t *c) {
   if (c->db) (void)hipFree(c->db);
   if (c->vT) (void)hipFree(c->vT);
   if (c->pf) (void)hipFree(c->pf);
+  if (c->fsb) (void)hipFree(c->fsb);
+  if (c->cidx) (void)hipFree(c->cidx);
+  if (c->cbuf) (void)hipFree(c->cbuf);
   if (c->dop) (void)hipFree(c->dop);
   if (c->vscr) (void)hipFree(c->vscr);
   if (c->mlp) (void)hipFree(c->mlp);
@@ -1015,6 +1022,45 @@ int vh_prim_fields(vh_ctx_t *c, double tau, double *prim) {
   vh::launch_prim_fields(N, c->eos, c->Q[c->cur], tau, c->pf, c->st);
   VH_HIP(hipGetLastError());
   VH_HIP(hipMemcpyAsync(prim, c->pf, 8 * N * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  VH_HIP(hipStreamSynchronize(c->st));
+  return VH_OK;
+}
+
+int vh_fluid_stats(vh_ctx_t *c, double tau, double e_crit, int central, double *out) {
+  if (!c || !out || c->evh.size() != 1 || !(tau > 0.)) return VH_ERR_ARG;
+  if (!c->fsb) VH_HIP(hipMalloc(&c->fsb, (16 * 256 + 16) * sizeof(double)));
+  vh::launch_fluid_stats(c->g, c->eos, c->Q[c->cur], c->pi, c->precision == VH_PREC_MIXED,
+                         c->vcc, tau, c->grid.zmin, e_crit, c->viscous ? 1 : 0, central ? 1 : 0,
+                         c->fsb, c->fsb + 16 * 256, c->st);
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipMemcpyAsync(out, c->fsb + 16 * 256, 16 * sizeof(double), hipMemcpyDeviceToHost,
+                        c->st));
+  VH_HIP(hipStreamSynchronize(c->st));
+  return VH_OK;
+}
+
+int vh_cells(vh_ctx_t *c, double tau, const long *cells, long n, double *out) {
+  if (!c || n < 0 || (n && (!cells || !out)) || !(tau > 0.)) return VH_ERR_ARG;
+  const long N = vh_ncell(c);
+  for (long i = 0; i < n; i++)
+    if (cells[i] < 0 || cells[i] >= N) return VH_ERR_ARG;
+  if (n == 0) return VH_OK;
+  if (c->ccap < n) {
+    if (c->cidx) (void)hipFree(c->cidx);
+    if (c->cbuf) (void)hipFree(c->cbuf);
+    c->cidx = nullptr;
+    c->cbuf = nullptr;
+    c->ccap = 0;
+    VH_HIP(hipMalloc(&c->cidx, n * sizeof(long)));
+    VH_HIP(hipMalloc(&c->cbuf, n * VH_CELL_COLS * sizeof(double)));
+    c->ccap = n;
+  }
+  VH_HIP(hipMemcpyAsync(c->cidx, cells, n * sizeof(long), hipMemcpyHostToDevice, c->st));
+  vh::launch_cells(N, c->eos, c->Q[c->cur], c->pi, c->Pi, c->precision == VH_PREC_MIXED, c->vcc,
+                   tau, c->cidx, n, c->cbuf, c->st);
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipMemcpyAsync(out, c->cbuf, n * VH_CELL_COLS * sizeof(double), hipMemcpyDeviceToHost,
+                        c->st));
   VH_HIP(hipStreamSynchronize(c->st));
   return VH_OK;
 }

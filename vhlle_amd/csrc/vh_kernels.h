@@ -45,5 +45,13 @@ void launch_c2p_batch(const DevEos &eos, double *Q, long n, double *out, int *it
 void launch_p2c_batch(const double *prim, long n, double *Q, hipStream_t st);
 void launch_prim_fields(long N, const DevEos &eos, const double *Q, double tau, double *prim,
                         hipStream_t st);
+// grid totals of the reference's output routines (16 sums, see vh_kernels.hip);
+// part: 16 * 256 doubles of scratch, out: 16 doubles (device)
+void launch_fluid_stats(const Geom &g, const DevEos &eos, const double *Q, const void *pi,
+                        int pi_float, const double *vcc, double tau, double zmin, double ecrit,
+                        int viscous, int central, double *part, double *out, hipStream_t st);
+void launch_cells(long N, const DevEos &eos, const double *Q, const void *pi, const void *Pi,
+                  int pi_float, const double *vcc, double tau, const long *idx, long n,
+                  double *out, hipStream_t st);
 
 }  // namespace vh

@@ -1186,6 +1186,160 @@ k_prim_fields(long N, DevEos eos, const double *__restrict__ Q, double tau,
   prim[7 * N + c] = r.vz;
 }
 
+// Fluid::getCMFvariables (src/fld.cpp:50-60): vx, vy to the lab frame and
+// the longitudinal rapidity Y of a cell at eta
+__device__ inline void cmf(double eta, double &vx, double &vy, double vz, double &Y) {
+  Y = eta + 1. / 2. * log((1. + vz) / (1. - vz));
+  vx = vx * cosh(Y - eta) / cosh(Y);
+  vy = vy * cosh(Y - eta) / cosh(Y);
+}
+
+// The grid totals of outputSurface / outputCorona (src/fld.cpp:545-606,
+// 889-945) over the interior cells (ix, iy, iz in [2, n-3]), one partial
+// sum per block (FS sums): E, Efull, S, Px, Nb1, Nb2, vt_num, vt_den,
+// vxvy_num, vxvy_den, pi0x_num, pi0x_den, txxyy_num, txxyy_den, core cells,
+// core cells with the viscous correction cut.  central: the v_T, vx-vy and
+// T^xx-T^yy sums only over nz/2-3 < iz < nz/2+3 (outputSurface).
+constexpr int FS = 16;
+template <typename PT>
+__global__ void __launch_bounds__(256)
+k_fluid_stats(Geom g, DevEos eos, const double *__restrict__ Q, const PT *__restrict__ pi,
+              const double *__restrict__ vcc, double tau, double zmin, double ecrit,
+              int viscous, int central, double *__restrict__ part) {
+  __shared__ double red[256];
+  const long N = g.N;
+  double acc[FS];
+  for (int k = 0; k < FS; k++) acc[k] = 0.;
+  const int mx = g.nx - 4, my = g.ny - 4, mz = g.nz - 4;
+  const long M = (mx > 0 && my > 0 && mz > 0) ? (long)mx * my * mz : 0;
+  for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < M;
+       j += (long)gridDim.x * blockDim.x) {
+    const int ix = (int)(j % mx) + 2, iy = (int)((j / mx) % my) + 2, iz = (int)(j / ((long)mx * my)) + 2;
+    const long c = ix + (long)g.nx * (iy + (long)g.ny * iz);
+    double q[7];
+    for (int k = 0; k < 7; k++) q[k] = Q[k * N + c] / tau;
+    Prim r;
+    c2p_ref(eos, q, r);
+    const double e = r.e, nb = r.nb;
+    double vx = r.vx, vy = r.vy, Y;
+    const double eta = zmin + iz * g.dz;
+    cmf(eta, vx, vy, r.vz, Y);
+    const double t = eos.T(e), p = eos.p(e);
+    double s = eos.s(e, nb, r.nq, r.ns);
+    const double dz = g.dz;
+    const double cosh_int = (sinh(eta + 0.5 * dz) - sinh(eta - 0.5 * dz)) / dz;
+    const double sinh_int = (cosh(eta + 0.5 * dz) - cosh(eta - 0.5 * dz)) / dz;
+    const double th = tanh(Y);
+    const double den = 1. - vx * vx - vy * vy - th * th;
+    acc[0] += tau * (e + p) / den * (cosh_int - th * sinh_int) - tau * p * cosh_int;
+    acc[4] += Q[4 * N + c];
+    acc[5] += tau * nb * (cosh_int - th * sinh_int) / sqrt(den);
+    double ef = tau * (e + p) / den * (cosh(eta) - th * sinh(eta)) - tau * p * cosh(eta);
+    double pic[10];
+    for (int k = 0; k < 10; k++) pic[k] = (double)pi[k * N + c];
+    if (viscous) ef += tau * pic[sym(0, 0)] * cosh(eta) + tau * pic[sym(0, 3)] * sinh(eta);
+    acc[1] += ef;
+    if (e > ecrit) {
+      acc[14] += 1.;
+      if (vcc[c] < 0.9) acc[15] += 1.;
+    }
+    const double gmm[4] = {1., -1., -1., -1.};
+    double deltas = 0.;
+    if (viscous)
+      for (int i = 0; i < 4; i++)
+        for (int k = 0; k < 4; k++) {
+          const double x = pic[sym(i, k)];
+          deltas += x * x * gmm[i] * gmm[k];
+        }
+    if (t > 0.02) {
+      s += 1.5 * deltas / ((e + p) * t);
+      acc[2] += tau * s * (cosh_int - th * sinh_int) / sqrt(den);
+    }
+    acc[3] += tau * (e + p) * vx / den;
+    if (!central || (iz > g.nz / 2 - 3 && iz < g.nz / 2 + 3)) {
+      acc[8] += e * (fabs(vx) - fabs(vy));
+      acc[9] += e;
+      acc[7] += e / sqrt(1. - vx * vx - vy * vy);
+      acc[6] += e / sqrt(1. - vx * vx - vy * vy) * sqrt(vx * vx + vy * vy);
+      acc[12] += (e + p) / den * (vx * vx - vy * vy);
+      acc[13] += (e + p) / den * (vx * vx + vy * vy) + 2. * p;
+    }
+    acc[10] += e / den * fabs(pic[sym(0, 1)]);
+    acc[11] += e / den;
+  }
+  for (int k = 0; k < FS; k++) {
+    red[threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(long)k * gridDim.x + blockIdx.x] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void k_fluid_stats_sum(const double *__restrict__ part, int nb, double *__restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= FS) return;
+  double a = 0.;
+  for (int b = 0; b < nb; b++) a += part[(long)k * nb + b];
+  out[k] = a;
+}
+
+// selected cells' getPrimVar, pi (10), Pi, viscCorrCut (21 doubles each)
+template <typename PT>
+__global__ void __launch_bounds__(256)
+k_cells(long N, DevEos eos, const double *__restrict__ Q, const PT *__restrict__ pi,
+        const PT *__restrict__ Pi, const double *__restrict__ vcc, double tau,
+        const long *__restrict__ idx, long n, double *__restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long c = idx[i];
+  double q[7];
+  for (int k = 0; k < 7; k++) q[k] = Q[k * N + c] / tau;
+  Prim r;
+  c2p_ref(eos, q, r);
+  double *o = out + 21 * i;
+  o[0] = r.e;
+  o[1] = r.p;
+  o[2] = r.nb;
+  o[3] = r.nq;
+  o[4] = r.ns;
+  o[5] = r.vx;
+  o[6] = r.vy;
+  o[7] = r.vz;
+  for (int k = 0; k < 10; k++) o[8 + k] = (double)pi[k * N + c];
+  o[18] = (double)Pi[c];
+  o[19] = vcc[c];
+  o[20] = 0.;
+}
+
+constexpr int FS_BLOCKS = 256;
+
+void launch_fluid_stats(const Geom &g, const DevEos &eos, const double *Q, const void *pi,
+                        int pi_float, const double *vcc, double tau, double zmin, double ecrit,
+                        int viscous, int central, double *part, double *out, hipStream_t st) {
+  if (pi_float)
+    hipLaunchKernelGGL(k_fluid_stats<float>, dim3(FS_BLOCKS), dim3(256), 0, st, g, eos, Q,
+                       (const float *)pi, vcc, tau, zmin, ecrit, viscous, central, part);
+  else
+    hipLaunchKernelGGL(k_fluid_stats<double>, dim3(FS_BLOCKS), dim3(256), 0, st, g, eos, Q,
+                       (const double *)pi, vcc, tau, zmin, ecrit, viscous, central, part);
+  hipLaunchKernelGGL(k_fluid_stats_sum, dim3(1), dim3(64), 0, st, part, FS_BLOCKS, out);
+}
+
+void launch_cells(long N, const DevEos &eos, const double *Q, const void *pi, const void *Pi,
+                  int pi_float, const double *vcc, double tau, const long *idx, long n,
+                  double *out, hipStream_t st) {
+  if (pi_float)
+    hipLaunchKernelGGL(k_cells<float>, dim3(blocks(n)), dim3(256), 0, st, N, eos, Q,
+                       (const float *)pi, (const float *)Pi, vcc, tau, idx, n, out);
+  else
+    hipLaunchKernelGGL(k_cells<double>, dim3(blocks(n)), dim3(256), 0, st, N, eos, Q,
+                       (const double *)pi, (const double *)Pi, vcc, tau, idx, n, out);
+}
+
 void launch_prim_fields(long N, const DevEos &eos, const double *Q, double tau, double *prim,
                         hipStream_t st) {
   hipLaunchKernelGGL(k_prim_fields, dim3(blocks(N)), dim3(256), 0, st, N, eos, Q, tau, prim);

@@ -292,8 +292,7 @@ int main(int argc, char **argv) {
     h->enableVorticity();
   }
   bool resized = false;
-  mkdir(outputDir.c_str(), 0755);
-  f->initOutput(outputDir.c_str());
+  f->initOutput(outputDir.c_str(), tau0, freezeoutOnly);  // src/main.cpp:389
   f->outputCorona(tau0, freezeoutExtend);  // src/main.cpp:389-391
   if (vorticityOn) f->printDbetaHeader();
   VtkOutput vtk_out(outputDir, eos, xmin, ymin, etamin);  // src/main.cpp:395-396
@@ -315,6 +314,7 @@ int main(int argc, char **argv) {
     }
     steps += nSub;
     nelements = f->outputSurface(h->getTau(), freezeoutExtend);
+    if (!freezeoutOnly) f->outputGnuplot(h->getTau());  // src/main.cpp:421-422
     if (h->getTau() >= tauResize && !resized) {  // src/main.cpp:423-427
       cout << "grid resize\n";
       // (the sub-step test keeps using the params' dtau, as the reference's)
@@ -322,6 +322,7 @@ int main(int argc, char **argv) {
       resized = true;
     }
   } while (h->getTau() < tauMax + 0.0001 && (nelements > 0 || !stopOnEmptySurface));
+  f->renameOutput(outputDir.c_str());  // src/main.cpp:435
   f->closeOutput();
   const double secs = chrono::duration<double>(chrono::steady_clock::now() - t0).count();
   cout << "tau = " << h->getTau() << "  steps = " << steps

@@ -11,6 +11,7 @@
 #include <map>
 #include <sstream>
 #include <stdexcept>
+#include <sys/stat.h>
 
 // ------------------------------------------------------------- helpers
 static void vh_or_die(int rc, const char *what);
@@ -561,16 +562,144 @@ void Fluid::hostWrite() {
   host_dirty = true;
 }
 
-void Fluid::initOutput(const char *dir) {
+namespace {
+// Fluid::checkOutputDirectory (src/fld.cpp:198-209): an old file is removed
+void remove_old(const std::string &path) {
+  std::FILE *f = std::fopen(path.c_str(), "r");
+  if (!f) return;
+  std::fclose(f);
+  const std::string name = path.substr(path.find_last_of('/') + 1);
+  std::cout << "\033[33mWarning! A '" << name << "' is present in your output directory.\n"
+            << "         It will be deleted automatically.\n\033[0m";
+  std::remove(path.c_str());
+}
+}  // namespace
+
+void Fluid::initOutput(const char *dir, double tau0, bool hsOnly) {
   closeOutput();
-  const std::string fn = std::string(dir) + "/freezeout.dat";
+  const std::string d(dir);
+  std::cout << "mkdir returns: " << (mkdir(dir, 0755) == 0) << std::endl;
+  remove_old(d + "/freezeout.dat");
+  remove_old(d + "/freezeout.dat.unfinished");
+  const std::string fn = d + "/freezeout.dat.unfinished";
   ffreeze = std::fopen(fn.c_str(), "w");
   if (!ffreeze) std::cout << "cannot open " << fn << std::endl;
   if (vorticityOn) {  // src/fld.cpp:131-136
-    const std::string fb = std::string(dir) + "/beta.dat";
+    const std::string fb = d + "/beta.dat";
     fbeta = std::fopen(fb.c_str(), "w");
     if (!fbeta) std::cout << "cannot open " << fb << std::endl;
   }
+  if (hsOnly) return;
+  static const char *names[9] = {"outx.dat",   "outy.dat",      "outz.dat",
+                                 "outdiag.dat", "out2D.dat",     "out.aniz.dat",
+                                 "outx.visc.dat", "outy.visc.dat", "diag.visc.dat"};
+  for (int k = 0; k < 9; k++) fgp[k] = std::fopen((d + "/" + names[k]).c_str(), "w");
+  outputGnuplot(tau0);
+  if (fgp[5]) std::fprintf(fgp[5], "#  tau  <<v_T>>  e_p  e'_p  (to compare with SongHeinz)\n");
+}
+
+void Fluid::renameOutput(const char *dir) {
+  const std::string d(dir);
+  if (ffreeze) std::fflush(ffreeze);
+  if (std::rename((d + "/freezeout.dat.unfinished").c_str(), (d + "/freezeout.dat").c_str()) != 0)
+    std::perror("Error renaming freezeout.dat.unfinished!");
+}
+
+void Fluid::outputGnuplot(double tau) {
+  if (!fgp[0]) return;
+  syncToDevice();
+  // the cells of the four lines: x, y, diagonal (getCell clamps iy), eta
+  std::vector<long> idx;
+  auto at = [&](int ix, int iy, int iz) {
+    ix = std::min(std::max(ix, 0), nx - 1);
+    iy = std::min(std::max(iy, 0), ny - 1);
+    iz = std::min(std::max(iz, 0), nz - 1);
+    return ix + (long)nx * (iy + (long)ny * iz);
+  };
+  for (int ix = 0; ix < nx; ix++) idx.push_back(at(ix, ny / 2, nz / 2));
+  for (int iy = 0; iy < ny; iy++) idx.push_back(at(nx / 2, iy, nz / 2));
+  for (int ix = 0; ix < nx; ix++) idx.push_back(at(ix, ix, nz / 2));
+  for (int iz = 0; iz < nz; iz++) idx.push_back(at(nx / 2, ny / 2, iz));
+  std::vector<double> v(idx.size() * VH_CELL_COLS);
+  vh_or_die(vh_cells(ctx, tau, idx.data(), (long)idx.size(), v.data()), "vh_cells");
+  // getCMFvariables (src/fld.cpp:50-60) and EoS::eos of one listed cell
+  struct Row {
+    double e, nb, t, mub, vx, vy, Y;
+    const double *pi;
+    double Pi, vcc;
+  };
+  auto row = [&](size_t k) {
+    const double *c = &v[k * VH_CELL_COLS];
+    const long cell = idx[k];
+    const double eta = getZ((int)(cell / ((long)nx * ny)));
+    Row r;
+    r.e = c[0];
+    r.nb = c[2];
+    double vx = c[5], vy = c[6], vz = c[7];
+    r.Y = eta + 1. / 2. * std::log((1. + vz) / (1. - vz));
+    r.vx = vx * std::cosh(r.Y - eta) / std::cosh(r.Y);
+    r.vy = vy * std::cosh(r.Y - eta) / std::cosh(r.Y);
+    double muq, mus, p;
+    eos->eos(r.e, r.nb, c[3], c[4], r.t, r.mub, muq, mus, p);
+    r.pi = c + 8;
+    r.Pi = c[18];
+    r.vcc = c[19];
+    return r;
+  };
+  auto tail = [&](std::FILE *f, const Row &r) {  // pi^{00..33}, Pi, viscCorrCut
+    static const int order[10][2] = {{0, 0}, {0, 1}, {0, 2}, {0, 3}, {1, 1},
+                                     {1, 2}, {1, 3}, {2, 2}, {2, 3}, {3, 3}};
+    for (auto &ij : order) std::fprintf(f, "%14g", r.pi[index44(ij[0], ij[1])]);
+    std::fprintf(f, "%14g%14g\n", r.Pi, r.vcc);
+  };
+  size_t k = 0;
+  for (int ix = 0; ix < nx; ix++, k++) {
+    const Row r = row(k);
+    std::fprintf(fgp[0], "%14g%14g%14g%14g%14g%14g%14g%14g", tau, getX(ix), r.vx, r.vy, r.e, r.nb,
+                 r.t, r.mub);
+    tail(fgp[0], r);
+  }
+  std::fprintf(fgp[0], "\n");
+  for (int iy = 0; iy < ny; iy++, k++) {
+    const Row r = row(k);
+    std::fprintf(fgp[1], "%14g%14g%14g%14g%14g%14g%14g%14g", tau, getY(iy), r.vy, r.vx, r.e, r.nb,
+                 r.t, r.mub);
+    tail(fgp[1], r);
+  }
+  std::fprintf(fgp[1], "\n");
+  for (int ix = 0; ix < nx; ix++, k++) {
+    const Row r = row(k);
+    std::fprintf(fgp[3], "%14g%14g%14g%14g%14g%14g%14g%14g\n", tau, std::sqrt(2.) * getY(ix), r.vx,
+                 r.vy, r.e, r.nb, r.t, r.mub);
+    tail(fgp[3], r);
+  }
+  std::fprintf(fgp[3], "\n");
+  for (int iz = 0; iz < nz; iz++, k++) {
+    const Row r = row(k);
+    std::fprintf(fgp[2], "%14g%14g%14g%14g%14g%14g%14g%14g", tau, getZ(iz), r.Y, r.vx, r.e, r.nb,
+                 r.t, r.mub);
+    tail(fgp[2], r);
+  }
+  std::fprintf(fgp[2], "\n");
+  for (int q = 0; q < 4; q++) std::fflush(fgp[q]);
+}
+
+void Fluid::gridTotals(double tau, bool central, double out[16]) {
+  vh_or_die(vh_fluid_stats(ctx, tau, ecrit, central ? 1 : 0, out), "vh_fluid_stats");
+  if (std::isinf(out[0])) {  // src/fld.cpp:562-566
+    std::cout << "EEinf" << std::endl;
+    std::exit(1);
+  }
+  const double dV = dx * dy * dz;
+  out[0] *= dV;  // E
+  out[1] *= dV;  // Efull
+  out[2] *= dV;  // S
+  out[4] *= dV;  // Nb1
+  out[5] *= dV;  // Nb2
+  if (fgp[5])
+    std::fprintf(fgp[5], "%12g%14g%14g%14g\n", tau, out[6] / out[7], out[8] / out[9],
+                 out[10] / out[11]);
+  if (fgp[5]) std::fflush(fgp[5]);
 }
 
 void Fluid::adoptOutput(Fluid *from) {
@@ -578,6 +707,10 @@ void Fluid::adoptOutput(Fluid *from) {
   ffreeze = from->ffreeze;
   fbeta = from->fbeta;
   from->ffreeze = from->fbeta = nullptr;
+  for (int k = 0; k < 9; k++) {
+    fgp[k] = from->fgp[k];
+    from->fgp[k] = nullptr;
+  }
   num_corona_cells = from->num_corona_cells;
   EtotSurf = from->EtotSurf;
   nbSurf = from->nbSurf;
@@ -588,6 +721,10 @@ void Fluid::closeOutput() {
   if (ffreeze) std::fclose(ffreeze);
   if (fbeta) std::fclose(fbeta);
   ffreeze = fbeta = nullptr;
+  for (auto &f : fgp) {
+    if (f) std::fclose(f);
+    f = nullptr;
+  }
 }
 
 void Fluid::enableVorticity() {
@@ -618,6 +755,7 @@ void Fluid::printDbetaHeader() {
 
 int Fluid::outputCorona(double tau, bool extendFO) {
   syncToDevice();
+  if (fgp[4]) std::fprintf(fgp[4], "\n");  // out2D.dat (src/fld.cpp:888)
   long n = 0;
   vh_fo_stats_t st;
   const int ncol = extendFO ? 29 : 27;
@@ -636,12 +774,23 @@ int Fluid::outputCorona(double tau, bool extendFO) {
   EtotSurf += st.e_surf;
   vEff += st.v_eff;
   num_corona_cells = (int)n;
+  double T[16];
+  gridTotals(tau, false, T);
+  std::fflush(stdout);
+  std::cout << std::setw(10) << "tau" << std::setw(13) << "E" << std::setw(13) << "Efull"
+            << std::setw(13) << "Nb" << std::setw(13) << "Sfull" << std::setw(13) << "EtotSurf"
+            << std::setw(13) << "elements" << std::setw(10) << "susp." << std::setw(13) << "%cut"
+            << std::endl;
+  std::cout << std::setw(10) << tau << std::setw(13) << T[0] << std::setw(13) << T[1]
+            << std::setw(13) << st.nb_surf << std::setw(13) << T[2] << std::setw(13) << EtotSurf
+            << std::endl;
   std::cout << "corona elements : " << n << std::endl;
   return (int)n;
 }
 
 int Fluid::outputSurface(double tau, bool extendFO) {
   syncToDevice();
+  if (fgp[4]) std::fprintf(fgp[4], "\n");  // out2D.dat (src/fld.cpp:543)
   long n = 0;
   vh_fo_stats_t st;
   const int ncol = extendFO ? 29 : 27;
@@ -677,8 +826,12 @@ int Fluid::outputSurface(double tau, bool extendFO) {
   EtotSurf += st.e_surf;
   nbSurf += st.nb_surf;
   vEff += st.v_eff;
-  std::cout << std::setw(10) << tau << std::setw(13) << nbSurf << std::setw(13) << EtotSurf
-            << std::setw(10) << n << std::setw(10) << st.nsusp << std::endl;
+  double T[16];
+  gridTotals(tau, true, T);
+  // src/fld.cpp:852-857
+  std::printf("%10g%13g%13g%13g%13g%13g%10ld%10ld%13g\n", tau, T[0], T[1], nbSurf, T[2],
+              EtotSurf, n, st.nsusp, (double)((float)T[15] / (float)T[14]));
+  std::fflush(stdout);
   return (int)n;
 }
 

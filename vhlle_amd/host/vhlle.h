@@ -242,8 +242,20 @@ class Fluid {
   // beta.dat header (src/fld.cpp:174-187); throws std::runtime_error like the
   // reference when outputCorona has not run
   void printDbetaHeader();
-  // opens dir/freezeout.dat (and dir/beta.dat with vorticity) for outputSurface
-  void initOutput(const char *dir);
+  // Fluid::initOutput (src/fld.cpp:119-172): creates dir, opens
+  // freezeout.dat.unfinished (removing old freezeout.dat files), beta.dat with
+  // vorticity, and unless hsOnly the gnuplot files outx/outy/outz/outdiag/
+  // out2D/out.aniz.dat (and the empty outx.visc/outy.visc/diag.visc.dat);
+  // writes outputGnuplot(tau0)
+  void initOutput(const char *dir, double tau0, bool hsOnly = false);
+  // the hypersurface files only (initOutput(dir, tau, true))
+  void initOutput(const char *dir) { initOutput(dir, 0., true); }
+  // Fluid::outputGnuplot (src/fld.cpp:425-504): the x, y, diagonal and eta
+  // lines through the grid centre at tau
+  void outputGnuplot(double tau);
+  // Fluid::renameOutput (src/fld.cpp:189-196): freezeout.dat.unfinished ->
+  // freezeout.dat after a clean run
+  void renameOutput(const char *dir);
   void closeOutput();
   // continue writing into another Fluid's freezeout.dat (the reference's
   // output streams are global, so a Fluid created by expandGrid2x keeps them)
@@ -277,6 +289,11 @@ class Fluid {
   std::vector<Cell> cells;
   bool host_dirty = false, dev_newer = false;
   std::FILE *ffreeze = nullptr, *fbeta = nullptr;
+  // gnuplot-style outputs: x, y, z, diag, 2D, aniz, x.visc, y.visc, diag.visc
+  std::FILE *fgp[9] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                       nullptr};
+  // prints the grid totals line of outputSurface / outputCorona
+  void gridTotals(double tau, bool central, double out[16]);
   double EtotSurf = 0., nbSurf = 0., vEff = 0.;
   bool vorticityOn = false, db_stale = true;
   int num_corona_cells = -1;  // -1: outputCorona has not run
