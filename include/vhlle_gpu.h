@@ -327,9 +327,11 @@ int vh_freezeout(vh_ctx_t *ctx, double tau, double dt, double e_crit, int extend
  * matrix to zero (Cell::enableVorticity) and makes every later viscous step
  * form it as the reference does, including its sharing of one matrix across
  * the cells of a step (entries NSquant does not form keep the values of the
- * cell before in the loop order ix, iy, iz).  0 frees it.  FP64 single-grid
- * contexts only.  Ideal-fluid steps leave the matrices unchanged, as in the
- * reference (NSquant runs only in the viscous step). */
+ * cell before in the loop order ix, iy, iz).  0 frees it.  Single-grid
+ * contexts; with VH_PREC_MIXED the matrix is formed in FP64 from the mixed
+ * step's states (the same states as its velocity gradients).  Ideal-fluid
+ * steps leave the matrices unchanged, as in the reference (NSquant runs only
+ * in the viscous step). */
 int vh_set_vorticity(vh_ctx_t *ctx, int on);
 /* Cell::getDbeta of every cell after the last step: dbeta, 16 x N SoA, row
  * 4 i + j = dbeta[i][j] (tau, x, y, eta; the eta row and column as the

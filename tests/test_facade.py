@@ -434,3 +434,20 @@ def test_driver_gnuplot_files_and_totals(tmp_path, oracle):
             assert float(line[1]) == pytest.approx(E, rel=1e-5)
         rc, tau = o.run(tau, 0.05, 1)
         assert rc == 0
+
+
+@pytest.mark.gpu
+def test_driver_big_e_exits_0_and_keeps_unfinished_file(tmp_path):
+    """e > 1e10 in a face state: the reference prints its message and exits
+    with 0 (src/hdo.cpp:118-130) before renameOutput, so the hypersurface
+    file stays freezeout.dat.unfinished (src/main.cpp:435)."""
+    p = _params(tmp_path, eosType=5, icModel=10, icE0=3e10, icSigmaX=1.0, icSigmaY=1.0,
+                icEtaPlateau=0.3, icEtaSigma=0.4, nx=12, ny=10, nz=6, xmin=-2.75, xmax=2.75,
+                ymin=-2.25, ymax=2.25, etamin=-0.5, etamax=0.5, tau0=1.0, tauMax=1.5,
+                dtau=0.05, etaS=0.1, tauGridResize=100.0)
+    r = subprocess.run([EXE, "-params", p, "-outputDir", str(tmp_path)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "e>1e10" in r.stdout
+    assert (tmp_path / "freezeout.dat.unfinished").exists()
+    assert not (tmp_path / "freezeout.dat").exists()

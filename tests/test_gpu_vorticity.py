@@ -146,3 +146,29 @@ def test_ideal_fluid_keeps_zero_and_switch_off(gpu):
     g.set_vorticity(False)
     with pytest.raises(RuntimeError):
         g.get_dbeta()
+
+
+@pytest.mark.parametrize("c2p", [abi.C2P_REFERENCE, abi.C2P_NEWTON])
+def test_mixed_precision_close_to_fp64(gpu, c2p):
+    """VH_PREC_MIXED forms the matrix in FP64 from its own states: within the
+    mixed step's accuracy of the FP64 path's matrix, the entries inherited at
+    the vacuum front included"""
+    nx, ny, nz = 18, 16, 9
+    lims = ((-4.25, 4.25), (-3.75, 3.75), (-1.0, 1.0))
+    e, vx, vy, vz = vacuum_ic(nx, ny, nz, lims, 7)
+    out = {}
+    for prec in (abi.PREC_FP64, abi.PREC_MIXED):
+        g = abi.Context(nx, ny, nz, *lims, trc=abi.make_trc(etaS=0.12), precision=prec)
+        if prec == abi.PREC_MIXED and c2p != abi.C2P_REFERENCE:
+            g.set_c2p(c2p)
+        g.set_vorticity(True)
+        g.set_prim(0.8, e, vx, vy, vz)
+        st, tau = g.run(0.8, 0.05, 6)
+        assert st == 0
+        out[prec] = g.get_dbeta()
+        g.close()
+    a, b = out[abi.PREC_MIXED], out[abi.PREC_FP64]
+    scale = np.max(np.abs(b))
+    rel = np.max(np.abs(a - b)) / scale
+    print(f"mixed vs FP64: max diff / scale = {rel:.3e}", flush=True)
+    assert rel <= 1e-4

@@ -212,7 +212,8 @@ int enqueue_step_mixed(vh_ctx *c, double dt) {
     rc |= timed(c, "visc_prim", [&] { vh::mx::launch_vprim(cm, nc, g.N, c->eos, c->Qh, c->gv, c->tp, dt, c->ph, c->pt, st); });
     rc |= timed(c, "visc_relax", [&] {
       vh::mx::launch_visc1(cm, nc, G, c->eos, c->trc, Qo, Qn, c->gv, c->ph, c->pt, c->m, c->m_full,
-                           fp(c->pi), fp(c->Pi), c->tp, dt, rec(c), c->status, st);
+                           fp(c->pi), fp(c->Pi), c->tp, dt, rec(c), c->status,
+                           c->vort ? c->db : nullptr, c->dop, st);
     });
     rc |= timed(c, "visc_advect", [&] {
       vh::mx::launch_visc2(G, c->ph, rec(c), fp(c->pi), fp(c->piH), fp(c->Pi), fp(c->PiH), c->vcc,
@@ -990,7 +991,7 @@ int vort_settle(vh_ctx_t *c) {
 }  // namespace
 
 int vh_set_vorticity(vh_ctx_t *c, int on) {
-  if (!c || c->evh.size() != 1 || c->precision != VH_PREC_FP64) return VH_ERR_ARG;
+  if (!c || c->evh.size() != 1) return VH_ERR_ARG;
   const long N = c->g.N, M = (long)c->g.nx * c->g.ny;
   if (M > (1L << 28)) return VH_ERR_ARG;  // column indices of the resolver are int
   VH_HIP(hipStreamSynchronize(c->st));

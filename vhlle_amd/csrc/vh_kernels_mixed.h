@@ -53,7 +53,7 @@ void launch_visc1(int cm, int nc, const Grids &G, const DevEos &eos, const DevTr
                   const double *Qprev, const double *Qn, const float *gv, const double *ph,
                   const double *pt, const double *m, int m_full, const float *pi,
                   const float *Pi, const double *tp, double dt, float4 *rec, int *status,
-                  hipStream_t st);
+                  double *db, unsigned char *dop, hipStream_t st);
 // rec: the Israel-Stewart auxiliaries pi0, Pi0, piH0, PiH0 as six planes of
 // P float4: (pi0 0-3) (pi0 4-7) (pi0 8-9, Pi0, 0) and the same for piH0
 void launch_visc2(const Grids &G, const double *ph, const float4 *rec, float *pi, float *piH,

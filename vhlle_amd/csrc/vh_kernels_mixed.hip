@@ -16,6 +16,7 @@
 // from a per-cell guess array gv (C2P_LEARNED: the MLP guess of the cell that
 // owns the state; C2P_NEWTON: v = 0.5 everywhere, the ablation).
 #include "vh_kernels_mixed.h"
+#include "vh_vort.h"
 #include "vh_mlp.h"
 
 #include <algorithm>
@@ -717,7 +718,9 @@ __device__ __forceinline__ void is_cross(const float *X, const float *Pm, const 
     }
 }
 
-template <int CM, int NC>
+// With VORT, also the cell's vorticity entries and op code (vh_vort.h), formed
+// in FP64 from the same states as NSquant's velocity gradients.
+template <int CM, int NC, bool VORT>
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_MX_VISC1)
 k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
         const double *__restrict__ Qn, const float *__restrict__ gv,
@@ -725,7 +728,7 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
         const double *__restrict__ pt, const double *__restrict__ m, int m_full,
         const float *__restrict__ pi, const float *__restrict__ Pi,
         const double *__restrict__ tp, double dt, float4 *__restrict__ rec,
-        int *__restrict__ status) {
+        int *__restrict__ status, double *__restrict__ db, unsigned char *__restrict__ dop) {
   const double tau = *tp;
   Geom g;
   long c;
@@ -738,6 +741,7 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
     const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
     for (int q = 0; q < 6; q++) rec[q * N + c] = z;
+    if (VORT) vort::store(db, dop, N, c, vort::K_EMPTY, nullptr);
     return;
   }
   const double gamma = 1.0 / sqrt(1.0 - vx * vx - vy * vy - vz * vz);
@@ -763,6 +767,9 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
   // |u^k| >= VMIN, or k = 0 with u^0 >= 1)
   const double ru[4] = {1. / u[0], 1. / u[1], 1. / u[2], 1. / u[3]};
   const double ihx = 1. / g.dx, ihy = 1. / g.dy, ihz = 1. / (g.dz * (tau + 0.5 * dt));
+  double vv[16];  // vorticity entries formed by this cell (VORT)
+  unsigned char vflags = 0;
+  if (VORT && vac) vort::store(db, dop, N, c, vort::K_STALE, nullptr);
   if (!vac) {
     double q[7];
     Prim p0, p1;
@@ -778,6 +785,7 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
     const double ut1 = 1.0 / sqrt(1.0 - p1.vx * p1.vx - p1.vy * p1.vy - p1.vz * p1.vz);
     const double a0[4] = {ut0, ut0 * p0.vx, ut0 * p0.vy, ut0 * p0.vz};
     const double a1[4] = {ut1, ut1 * p1.vx, ut1 * p1.vy, ut1 * p1.vz};
+    if (VORT) vort::row_time(vv, a0, a1, eos.T(p0.e), eos.T(p1.e), e, p0.e, dt);
     if (e > 0. && p0.e > 0.) {
 #pragma unroll
       for (int k = 0; k < 4; k++) {
@@ -792,6 +800,8 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
     for (int d = 1; d <= 3; d++) {
       const long ca = g.cidx(ix + (d == 1), iy + (d == 2), iz + (d == 3));
       const long cb = g.cidx(ix - (d == 1), iy - (d == 2), iz - (d == 3));
+      // the reference takes both temperatures before its e test (src/hdo.cpp:474-481)
+      const double TA = VORT ? eos.T(pt[ca]) : 0., TB = VORT ? eos.T(pt[cb]) : 0.;
       if (pt[ca] > 0. && pt[cb] > 0.) {
         const double avx = pt[N + ca], avy = pt[2 * N + ca], avz = pt[3 * N + ca];
         const double bvx = pt[N + cb], bvy = pt[2 * N + cb], bvz = pt[3 * N + cb];
@@ -800,6 +810,11 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
         const double B[4] = {gb, gb * bvx, gb * bvy, gb * bvz};
         const double A[4] = {ga, ga * avx, ga * avy, ga * avz};
         const double ih = d == 1 ? ihx : (d == 2 ? ihy : ihz);
+        if (VORT) {
+          const double den = d == 1 ? g.dx : (d == 2 ? g.dy : g.dz * (tau + 0.5 * dt));
+          vort::row_space(vv + 4 * d, A, B, TA, TB, den);
+          vflags |= d == 1 ? vort::F_R1 : (d == 2 ? vort::F_R2 : vort::F_R3);
+        }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
           const bool fast = fabs(0.5 * (A[k] + B[k])) > UDIFF * fabs(u[k]);
@@ -812,6 +827,21 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
     const double itm = 1. / (tau - 0.5 * dt);
     dmu[3][0] += u[3] * itm;
     dmu[3][3] += u[0] * itm;
+    if (VORT) {  // Christoffel terms (src/hdo.cpp:608-611), then Cell::setDbeta
+      const double TH = eos.T(e);
+      if (TH > 0.) {
+        const double a30 = u[3] / (TH * (tau - 0.5 * dt)), a33 = u[0] / (TH * (tau - 0.5 * dt));
+        if (vflags & vort::F_R3) {
+          vv[12] += a30;
+          vv[15] += a33;
+        } else {
+          vv[12] = a30;
+          vv[15] = a33;
+        }
+        vflags |= vort::F_ADD;
+      }
+      vort::store(db, dop, N, c, vort::K_FULL | vflags, vv);
+    }
   }
   const double du = dmu[0][0] + dmu[1][1] + dmu[2][2] + dmu[3][3];
   // sigma^{ij} = 1/2 (D^{ik} dmu_kj + D^{jk} dmu_ki) - 1/3 D^{ij} theta, D = g - uu
@@ -1304,21 +1334,20 @@ void launch_vprim(int cm, int nc, long N, const DevEos &eos, const double *Qh, c
     hipLaunchKernelGGL((k_vprim<CC, 4>), gr, b, 0, st, N, eos, Qh, gv, tp, dt, ph, pt);
   VH_V(VH_C2P_REFERENCE) VH_V(VH_C2P_CLOSED) VH_V(VH_C2P_LEARNED) VH_V(VH_C2P_NEWTON)
 #undef VH_V
+#undef VH_VV
 }
 
 void launch_visc1(int cm, int nc, const Grids &G, const DevEos &eos, const DevTrc &trc,
                   const double *Qprev, const double *Qn, const float *gv, const double *ph,
                   const double *pt, const double *m, int m_full, const float *pi,
                   const float *Pi, const double *tp, double dt, float4 *rec, int *status,
-                  hipStream_t st) {
+                  double *db, unsigned char *dop, hipStream_t st) {
   const dim3 gr((unsigned)G.nblk), b(256);
-#define VH_V(CC)                                                                               \
-  if (cm == CC && nc == 7)                                                                     \
-    hipLaunchKernelGGL((k_visc1<CC, 7>), gr, b, 0, st, G, eos, trc, Qprev, Qn, gv, ph, pt, m,    \
-                       m_full, pi, Pi, tp, dt, rec, status);                                   \
-  if (cm == CC && nc == 4)                                                                     \
-    hipLaunchKernelGGL((k_visc1<CC, 4>), gr, b, 0, st, G, eos, trc, Qprev, Qn, gv, ph, pt, m,    \
-                       m_full, pi, Pi, tp, dt, rec, status);
+#define VH_VV(CC, NN, VV)                                                                      \
+  if (cm == CC && nc == NN && (db != nullptr) == VV)                                           \
+    hipLaunchKernelGGL((k_visc1<CC, NN, VV>), gr, b, 0, st, G, eos, trc, Qprev, Qn, gv, ph, pt,  \
+                       m, m_full, pi, Pi, tp, dt, rec, status, db, dop);
+#define VH_V(CC) VH_VV(CC, 7, false) VH_VV(CC, 4, false) VH_VV(CC, 7, true) VH_VV(CC, 4, true)
   VH_V(VH_C2P_REFERENCE) VH_V(VH_C2P_CLOSED) VH_V(VH_C2P_LEARNED) VH_V(VH_C2P_NEWTON)
 #undef VH_V
 }

@@ -14,7 +14,7 @@
 // 3 Newton from v = 0.5), c2pModel (a file written by -trainC2P);
 // stopOnEmptySurface (1, the reference's loop condition: stop at the first
 // step without freeze-out elements, which for nz < 5 is the first step).
-// vorticity 1 (FP64 only) writes beta.dat next to freezeout.dat;
+// vorticity 1 writes beta.dat next to freezeout.dat;
 // VTK_output_values (e.g. eps,T,v,pi) writes VTK files before every step.
 // tauGridResize: once tau reaches it the grid doubles its transverse extent
 // (expandGrid2x, src/main.cpp:246-271).
@@ -284,13 +284,7 @@ int main(int argc, char **argv) {
   }
   cout << "IC done\n";
   Hydro *h = new Hydro(f, eos, trcoeff, tau0, dtau);
-  if (vorticityOn) {  // src/main.cpp:380-383
-    if (precision != VH_PREC_FP64) {
-      cout << "vorticity needs precision 0 (the FP64 step) in this build\n";
-      return 1;
-    }
-    h->enableVorticity();
-  }
+  if (vorticityOn) h->enableVorticity();  // src/main.cpp:380-383
   bool resized = false;
   f->initOutput(outputDir.c_str(), tau0, freezeoutOnly);  // src/main.cpp:389
   f->outputCorona(tau0, freezeoutExtend);  // src/main.cpp:389-391
