@@ -2,6 +2,7 @@
 #include "vhlle.h"
 
 #include <algorithm>
+#include <charconv>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -11,10 +12,50 @@
 #include <map>
 #include <sstream>
 #include <stdexcept>
+#include <string>
+#include <thread>
 #include <sys/stat.h>
 
 // ------------------------------------------------------------- helpers
 static void vh_or_die(int rc, const char *what);
+
+// Writes n records of ncol doubles as "%24.15g" columns, one record a line
+// (the reference's freezeout.dat / beta.dat format).  Each number goes
+// through std::to_chars (general format, precision 15: by definition the
+// digits of printf's %.15g in the C locale), right-aligned to 24 characters.
+// Up to 16 host threads format separate record ranges into their own buffers,
+// written in record order, so the file is the same as one thread's.
+// Formatting is the cost (0.67 s for 53 000 records with fprintf), not the
+// write.
+static void write_records(std::FILE *f, const double *rows, long n, int ncol) {
+  if (!f || n <= 0) return;
+  const long per = 4096;
+  const int nt = (int)std::max(1L, std::min<long>({16L, (long)std::thread::hardware_concurrency(),
+                                                    (n + per - 1) / per}));
+  std::vector<std::string> out(nt);
+  auto work = [&](int t) {
+    const long a = n * t / nt, b = n * (t + 1) / nt;
+    std::string &o = out[t];
+    o.reserve((size_t)(b - a) * (ncol * 24 + 1));
+    char buf[40];
+    for (long i = a; i < b; i++) {
+      for (int k = 0; k < ncol; k++) {
+        const auto r = std::to_chars(buf, buf + sizeof buf, rows[i * ncol + k],
+                                     std::chars_format::general, 15);
+        const size_t len = (size_t)(r.ptr - buf);
+        if (len < 24) o.append(24 - len, ' ');
+        o.append(buf, len);
+      }
+      o.push_back('\n');
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; t++) th.emplace_back(work, t);
+  work(0);
+  for (auto &x : th) x.join();
+  for (auto &o : out) std::fwrite(o.data(), 1, o.size(), f);
+  std::fflush(f);
+}
 
 int index44(const int &i, const int &j) {
   if (i > 3 || j > 3 || i < 0 || j < 0) {
@@ -763,11 +804,7 @@ int Fluid::outputCorona(double tau, bool extendFO) {
   if (ffreeze && n > 0) {
     std::vector<double> rows((size_t)n * ncol);
     vh_or_die(vh_corona(ctx, tau, ecrit, extendFO, rows.data(), n, &n, &st), "vh_corona");
-    for (long i = 0; i < n; i++) {
-      for (int k = 0; k < ncol; k++) std::fprintf(ffreeze, "%24.15g", rows[i * ncol + k]);
-      std::fprintf(ffreeze, "\n");
-    }
-    std::fflush(ffreeze);
+    write_records(ffreeze, rows.data(), n, ncol);
   }
   // the reference adds the corona to EtotSurf and vEff; its n_b sum goes to
   // a function-local static (src/fld.cpp:877), not to the Fluid's nbSurf
@@ -810,17 +847,8 @@ int Fluid::outputSurface(double tau, bool extendFO) {
   }
   const std::vector<double> &rows = fo_rows, &beta = fo_beta;
   if (want && n > 0) {
-    for (long i = 0; ffreeze && i < n; i++) {
-      for (int k = 0; k < ncol; k++) std::fprintf(ffreeze, "%24.15g", rows[i * ncol + k]);
-      std::fprintf(ffreeze, "\n");
-    }
-    if (ffreeze) std::fflush(ffreeze);
-    for (long i = 0; wb && i < n; i++) {  // beta.dat (src/fld.cpp:663-834)
-      for (int k = 0; k < VH_BETA_COLS; k++)
-        std::fprintf(fbeta, "%24.15g", beta[i * VH_BETA_COLS + k]);
-      std::fprintf(fbeta, "\n");
-    }
-    if (wb) std::fflush(fbeta);
+    write_records(ffreeze, rows.data(), n, ncol);
+    if (wb) write_records(fbeta, beta.data(), n, VH_BETA_COLS);  // src/fld.cpp:663-834
   }
   if (st.err) std::cout << "#### Error (surface): hypercube construction failed ####\n";
   EtotSurf += st.e_surf;
