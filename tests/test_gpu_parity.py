@@ -209,3 +209,26 @@ def test_dtau_check(gpu):
     ctx = gpu.Context(11, 11, 1, (-1, 1), (-1, 1), (-1, 1))
     assert ctx.check_dtau(0.1) == gpu.VH_OK
     assert ctx.check_dtau(0.11) == gpu.VH_ERR_DTAU
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (2, 1, 1), (3, 3, 1), (4, 2, 3), (5, 1, 2),
+                                   (6, 5, 4)])
+def test_tiny_and_ragged_grids(gpu, oracle, shape):
+    """extents 1 to 6, where the ghost copies (2 layers from index 2 and n-3)
+    and the clamped neighbours overlap: the step still equals the oracle's"""
+    nx, ny, nz = shape
+    lim = lambda n, h: (-(n - 1) * h / 2, (n - 1) * h / 2) if n > 1 else (-1.0, 1.0)
+    xl, yl, zl = lim(nx, 0.5), lim(ny, 0.5), lim(nz, 0.2)
+    rng = np.random.default_rng(sum(shape))
+    e = 5.0 + 5.0 * rng.uniform(size=(nz, ny, nx))
+    vx, vy, vz = (0.2 * rng.uniform(-1, 1, e.shape) for _ in range(3))
+    ctx, orc = _pair(gpu, oracle, nx, ny, nz, xl, yl, zl, dict(etaS=0.1, zetaS=0.03,
+                                                                   zetaSparam=2))
+    ctx.set_prim(1.0, e, vx, vy, vz)
+    orc.set_prim(1.0, e, vx, vy, vz)
+    tg = to = 1.0
+    for s in range(4):
+        st, tg = ctx.step(tg, 0.02)
+        rc, to = orc.run(to, 0.02, 1)
+        assert st == 0 and rc == 0
+    assert_state_close(ctx.download(), orc.get_state(), label=f"grid {shape}")
