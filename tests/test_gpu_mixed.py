@@ -230,3 +230,35 @@ def test_charge_free_kernels_match_charged(gpu, c2p):
     a, b = out
     assert np.array_equal(a["Q"][:4], b["Q"][:4]) and np.array_equal(a["pi"], b["pi"])
     assert not a["Q"][4:].any() and not a["Qh"][4:].any() and b["Q"][4].any()
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (3, 3, 1), (4, 2, 3), (5, 1, 2), (6, 5, 4),
+                                   (9, 7, 3), (70, 3, 2)])
+def test_tiny_and_ragged_grids_mixed(gpu, shape):
+    """The mixed kernels (y-strip cell order, stage tiles) on extents 1 to 9
+    and a long thin grid: status clean, the same state as the per-cell stage
+    kernel forced, and close to the FP64 step"""
+    nx, ny, nz = shape
+    lim = lambda n, h: (-(n - 1) * h / 2, (n - 1) * h / 2) if n > 1 else (-1.0, 1.0)
+    xl, yl, zl = lim(nx, 0.5), lim(ny, 0.5), lim(nz, 0.2)
+    rng = np.random.default_rng(sum(shape))
+    e = 5.0 + 5.0 * rng.uniform(size=(nz, ny, nx))
+    vx, vy, vz = (0.2 * rng.uniform(-1, 1, e.shape) for _ in range(3))
+    out = {}
+    for name, prec, stage in (("fp64", abi.PREC_FP64, None),
+                              ("mixed", abi.PREC_MIXED, None),
+                              ("mixed_cells", abi.PREC_MIXED, (abi.STAGE_CELLS, 0))):
+        ctx = abi.Context(nx, ny, nz, xl, yl, zl, trc=abi.make_trc(etaS=0.1), precision=prec)
+        if stage:
+            ctx.set_stage(*stage)
+        ctx.set_prim(1.0, e, vx, vy, vz)
+        st, _ = ctx.run(1.0, 0.02, 4)
+        assert st == 0, (name, st)
+        out[name] = ctx.download()
+        ctx.close()
+    a, b = out["mixed"]["Q"], out["fp64"]["Q"]
+    assert np.all(np.isfinite(a))
+    assert np.max(np.abs(a - b)) <= 1e-5 * np.max(np.abs(b)), shape
+    if out["mixed"]["Q"].shape == out["mixed_cells"]["Q"].shape:
+        assert np.max(np.abs(out["mixed"]["Q"] - out["mixed_cells"]["Q"])) <= 1e-12 * np.max(
+            np.abs(b))
