@@ -434,6 +434,12 @@ __device__ __forceinline__ long xcd_block() {
 #define VH_OCC_4 __attribute__((amdgpu_waves_per_eu(1, 4)))
 #define VH_OCC_5 __attribute__((amdgpu_waves_per_eu(1, 5)))
 #define VH_OCC_6 __attribute__((amdgpu_waves_per_eu(1, 6)))
+// M<n>: at least n waves per SIMD (registers capped accordingly, spilling
+// what does not fit)
+#define VH_OCC_M2 __attribute__((amdgpu_waves_per_eu(2)))
+#define VH_OCC_M3 __attribute__((amdgpu_waves_per_eu(3)))
+#define VH_OCC_M4 __attribute__((amdgpu_waves_per_eu(4)))
+#define VH_OCC_M5 __attribute__((amdgpu_waves_per_eu(5)))
 #define VH_OCC_EXP(n) VH_OCC_##n
 #define VH_OCC(n) VH_OCC_EXP(n)
 #ifndef VH_W_FACES
@@ -446,7 +452,9 @@ __device__ __forceinline__ long xcd_block() {
 #define VH_W_VPRIM 0
 #endif
 #ifndef VH_W_VISC1
-#define VH_W_VISC1 0
+// FP64 relaxation: 255 VGPRs + 76 AGPRs at the default (one wave per SIMD);
+// two waves with 108 registers spilled to scratch run 12 % faster
+#define VH_W_VISC1 M2
 #endif
 #ifndef VH_W_VISC2
 #define VH_W_VISC2 0
@@ -464,7 +472,9 @@ __device__ __forceinline__ long xcd_block() {
 #define VH_W_MX_VPRIM 0
 #endif
 #ifndef VH_W_MX_VISC1
-#define VH_W_MX_VISC1 0
+// mixed relaxation: 131 VGPRs (three waves) at the default, 127 with no
+// spill at four waves, 13 % faster
+#define VH_W_MX_VISC1 M4
 #endif
 #ifndef VH_W_MX_VISC2
 #define VH_W_MX_VISC2 2
@@ -502,5 +512,38 @@ struct Geom {
     return idx(ix, iy, iz);
   }
 };
+
+// The ghost cells (the two outer layers of every face, src/fld.cpp:211-382)
+// of a grid whose extents are all >= 5, enumerated without the interior: the
+// four eta layers iz = 0, 1, nz-2, nz-1 (x fastest), then the four y rows of
+// the other eta slices, then the four x columns of the remaining rows.
+// ghost_count of them; item t is cell (ix, iy, iz).
+__host__ __device__ inline long ghost_count(int nx, int ny, int nz) {
+  return 4L * nx * ny + 4L * nx * (nz - 4) + 4L * (ny - 4) * (nz - 4);
+}
+__host__ __device__ inline int ghost_outer(int k, int n) { return k < 2 ? k : n - 4 + k; }
+__host__ __device__ inline void ghost_cell(int nx, int ny, int nz, long t, int &ix, int &iy, int &iz) {
+  const unsigned nxu = nx, nxy = (unsigned)nx * (unsigned)ny;
+  const unsigned n1 = 4u * nxy, n2 = 4u * nxu * (unsigned)(nz - 4);
+  unsigned u = (unsigned)t;
+  if (u < n1) {
+    const unsigned l = u / nxy, r = u - l * nxy, y = r / nxu;
+    iz = ghost_outer((int)l, nz);
+    iy = (int)y;
+    ix = (int)(r - y * nxu);
+  } else if (u < n1 + n2) {
+    u -= n1;
+    const unsigned per = 4u * nxu, z = u / per, r = u - z * per, row = r / nxu;
+    iz = (int)z + 2;
+    iy = ghost_outer((int)row, ny);
+    ix = (int)(r - row * nxu);
+  } else {
+    u -= n1 + n2;
+    const unsigned per = 4u * (unsigned)(ny - 4), z = u / per, r = u - z * per, y = r >> 2;
+    iz = (int)z + 2;
+    iy = (int)y + 2;
+    ix = ghost_outer((int)(r & 3u), nx);
+  }
+}
 
 }  // namespace vh

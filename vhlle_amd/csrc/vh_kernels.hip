@@ -365,6 +365,32 @@ k_ghost1(Geom g, double *__restrict__ Q, double *__restrict__ Qh, PT *__restrict
   }
 }
 
+// k_ghost1 over the ghost cells only (every extent >= 5; ghost_cell)
+template <int FULL, typename PT>
+__global__ void __launch_bounds__(256)
+k_ghost_list(Geom g, long n, double *__restrict__ Q, double *__restrict__ Qh,
+             PT *__restrict__ pi, PT *__restrict__ Pi) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int ix, iy, iz;
+  ghost_cell(g.nx, g.ny, g.nz, t, ix, iy, iz);
+  const long N = g.N, c = g.idx(ix, iy, iz);
+  const long src = g.idx(ghost_map(ix, g.nx), ghost_map(iy, g.ny), ghost_map(iz, g.nz));
+  double v[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) v[k] = Q[k * N + src];
+  set_clamped(Q, N, c, v);
+  if (!FULL) {
+#pragma unroll
+    for (int k = 0; k < 7; k++) v[k] = Qh[k * N + src];
+    set_clamped(Qh, N, c, v);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 10; k++) pi[k * N + c] = pi[k * N + src];
+    Pi[c] = Pi[src];
+  }
+}
+
 bool ghost_fusable(const Geom &g) {
   auto ok = [](int n) { return n == 1 || n >= 5; };
   return ok(g.nx) && ok(g.ny) && ok(g.nz);
@@ -373,6 +399,20 @@ bool ghost_fusable(const Geom &g) {
 void launch_ghost1(int full, int pi_float, const Geom &g, double *Q, double *Qh, void *pi,
                    void *Pi, hipStream_t st) {
   const dim3 gr((unsigned)((g.N + 255) / 256)), b(256);
+  if (g.nx >= 5 && g.ny >= 5 && g.nz >= 5) {  // launch over the ghost cells only
+    const long n = ghost_count(g.nx, g.ny, g.nz);
+    const dim3 gl((unsigned)((n + 255) / 256));
+    if (!full)
+      hipLaunchKernelGGL((k_ghost_list<0, double>), gl, b, 0, st, g, n, Q, Qh, (double *)pi,
+                         (double *)Pi);
+    else if (pi_float)
+      hipLaunchKernelGGL((k_ghost_list<1, float>), gl, b, 0, st, g, n, Q, Qh, (float *)pi,
+                         (float *)Pi);
+    else
+      hipLaunchKernelGGL((k_ghost_list<1, double>), gl, b, 0, st, g, n, Q, Qh, (double *)pi,
+                         (double *)Pi);
+    return;
+  }
   if (g.nx == 1 || g.ny == 1 || g.nz == 1) {
     if (full) hipLaunchKernelGGL((k_ghost_zero<1>), gr, b, 0, st, g, Q, Qh);
     else hipLaunchKernelGGL((k_ghost_zero<0>), gr, b, 0, st, g, Q, Qh);

@@ -1249,6 +1249,37 @@ k_ghost_b(Grids G, double *__restrict__ Q, double *__restrict__ Qh, float *__res
   }
 }
 
+// k_ghost_b of a single grid over its ghost cells only (every extent >= 5;
+// ghost_cell)
+template <int FULL>
+__global__ void __launch_bounds__(256)
+k_ghost_list(Geom g, long n, double *__restrict__ Q, double *__restrict__ Qh,
+             float *__restrict__ pi, float *__restrict__ Pi) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int ix, iy, iz;
+  ghost_cell(g.nx, g.ny, g.nz, t, ix, iy, iz);
+  const long N = g.P, c = g.idx(ix, iy, iz);
+  const long src = g.idx(ghost_src(ix, g.nx), ghost_src(iy, g.ny), ghost_src(iz, g.nz));
+  double v[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) v[k] = Q[k * N + src];
+  const bool z = v[0] < 0.;
+#pragma unroll
+  for (int k = 0; k < 7; k++) Q[k * N + c] = z ? 0. : v[k];
+  if (!FULL) {
+#pragma unroll
+    for (int k = 0; k < 7; k++) v[k] = Qh[k * N + src];
+    const bool zh = v[0] < 0.;
+#pragma unroll
+    for (int k = 0; k < 7; k++) Qh[k * N + c] = zh ? 0. : v[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 10; k++) pi[k * N + c] = pi[k * N + src];
+    Pi[c] = Pi[src];
+  }
+}
+
 // ---------------------------------------------------------- launchers
 static inline unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
 
@@ -1369,6 +1400,13 @@ void launch_visc3(const Grids &G, const double *ph, const float *piH, const floa
 void launch_ghost_cells(int full, const Grids &G, bool degenerate, double *Q, double *Qh,
                         float *pi, float *Pi, hipStream_t st) {
   const dim3 gr((unsigned)G.nblk), b(256);
+  const Geom &g = G.g0;
+  if (G.nev == 1 && g.nx >= 5 && g.ny >= 5 && g.nz >= 5) {  // over the ghost cells only
+    const long n = ghost_count(g.nx, g.ny, g.nz);
+    if (full) hipLaunchKernelGGL(k_ghost_list<1>, dim3(blocks(n)), b, 0, st, g, n, Q, Qh, pi, Pi);
+    else hipLaunchKernelGGL(k_ghost_list<0>, dim3(blocks(n)), b, 0, st, g, n, Q, Qh, pi, Pi);
+    return;
+  }
   if (degenerate) {
     if (full) hipLaunchKernelGGL(k_ghost_zero_b<1>, gr, b, 0, st, G, Q, Qh);
     else hipLaunchKernelGGL(k_ghost_zero_b<0>, gr, b, 0, st, G, Q, Qh);
