@@ -85,11 +85,13 @@ def sheet_state(o, shape, lims):
     return st
 
 
-def test_thin_sheet_running_sums(gpu):
+@pytest.mark.parametrize("nx, ny", [(20, 17), (48, 45)])
+def test_thin_sheet_running_sums(gpu, nx, ny):
     """one running sum over the whole grid in loop order (the resolver's
-    chains of columns that only add)"""
-    nx, ny, nz = 20, 17, 9
-    lims = ((-2.375, 2.375), (-2.0, 2.0), (-1.0, 1.0))
+    chains of columns that only add); 48 x 45 columns span three scan tiles
+    of 1024"""
+    nz = 9
+    lims = ((-0.125 * (nx - 1), 0.125 * (nx - 1)), (-0.125 * (ny - 1), 0.125 * (ny - 1)), (-1.0, 1.0))
     g = abi.Context(nx, ny, nz, *lims, trc=abi.make_trc(etaS=0.08))
     o = po.Solver(nx, ny, nz, *lims, trc=po.make_trc(etaS=0.08))
     g.set_vorticity(True)
@@ -105,6 +107,34 @@ def test_thin_sheet_running_sums(gpu):
         close(got, want, f"step {s}")
         if s == 0:  # the sums grew along the loop order and reached the clamp
             assert np.max(want[15]) == 100.0 and np.all(want[13:15] == 0.0)
+
+
+def test_vacuum_fronts_over_several_scan_tiles(gpu):
+    """64 x 40 columns (three scan tiles of 1024): stale cells and partial
+    rows next to vacuum take the matrix of the cell before them across tile
+    boundaries"""
+    nx, ny, nz = 64, 40, 7
+    lims = ((-7.875, 7.875), (-4.875, 4.875), (-0.75, 0.75))
+    X, Y, H = setups.axes(nx, ny, nz, *lims)
+    rng = np.random.default_rng(11)
+    # a row of blobs along x: vacuum gaps between them in every tile
+    e = np.zeros_like(X)
+    for x0 in (-6.0, -2.0, 2.0, 6.0):
+        r2 = (X - x0) ** 2 / 2.0 + Y ** 2 / 9.0 + H ** 2 / 0.5
+        e += np.where(r2 < 1.0, 20.0 * (1.0 - r2) ** 2 + 0.3, 0.0)
+    e *= 1 + 0.05 * rng.uniform(-1, 1, e.shape)
+    vx = 0.2 * np.sin(X)
+    vy = 0.3 * np.tanh(Y / 2)
+    vz = 0.1 * np.tanh(H)
+    g, o = pair((nx, ny, nz), lims, dict(etaS=0.1), 0.8, e, vx, vy, vz)
+    tg = to = 0.8
+    for s in range(3):
+        st, tg = g.step(tg, 0.05)
+        rc, to = o.run(to, 0.05, 1)
+        assert st == 0 and rc == 0
+        close(g.get_dbeta(), o.get_dbeta(), f"step {s}")
+    op_stale = o.get_dbeta()
+    assert np.any(op_stale != 0.0)
 
 
 def test_graph_run_and_records(gpu):

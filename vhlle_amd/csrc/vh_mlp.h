@@ -72,6 +72,14 @@ __device__ inline void load_packed(Packed &dst, const Packed *__restrict__ src) 
   for (int i = threadIdx.x; i < (int)(sizeof(Packed) / 16); i += blockDim.x) d[i] = s[i];
 }
 
+// ReLU as an integer max on the bit pattern: negative values (sign bit set)
+// become +0, the rest pass unchanged.  Equal to fmaxf(x, 0) for every finite
+// x (the activations are finite: finite features and weights), in one
+// instruction (fmaxf adds a NaN-quieting max before each).
+__device__ __forceinline__ float relu(float x) {
+  return __int_as_float(max(__float_as_int(x), 0));
+}
+
 // MLP on the 64 lanes' feature vectors; returns (|v| guess, e/Q^tau guess) of
 // the calling lane's sample.  Every lane of the wave must call it.
 __device__ inline float2 mlp_wave(const Packed &w, const float f[5]) {
@@ -105,7 +113,7 @@ __device__ inline float2 mlp_wave(const Packed &w, const float f[5]) {
 #pragma unroll
       for (int t = 0; t < 8; t++) {
         const float z = kb < 2 ? d1a[8 * (kb & 1) + t] : d1b[8 * (kb & 1) + t];
-        hb[t] = (__bf16)fmaxf(z, 0.f);
+        hb[t] = (__bf16)relu(z);
       }
       d2a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w.w2[0][kb][l], hb, d2a, 0, 0, 0);
       d2b = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w.w2[1][kb][l], hb, d2b, 0, 0, 0);
@@ -114,7 +122,7 @@ __device__ inline float2 mlp_wave(const Packed &w, const float f[5]) {
 #pragma unroll
     for (int r = 0; r < 16; r++) {
       const int h = 8 * (r >> 2) + 4 * half + (r & 3);
-      const float a = fmaxf(d2a[r], 0.f), c = fmaxf(d2b[r], 0.f);
+      const float a = relu(d2a[r]), c = relu(d2b[r]);
       o0 += w.w3[0][h] * a + w.w3[0][32 + h] * c;
       o1 += w.w3[1][h] * a + w.w3[1][32 + h] * c;
     }

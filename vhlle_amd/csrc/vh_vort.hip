@@ -7,9 +7,11 @@
 //           entries; records per group whether the column sets it, whether a
 //           cell needs the value that enters the column, and the value that
 //           leaves it.
-//  k_scan   one block: for every column and group, the last earlier column
-//           that changes the group (prefix max) and, for [3][0]/[3][3], the
-//           next one (suffix min).
+//  k_tile, k_carry, k_tile
+//           for every column and group, the last earlier column that changes
+//           the group (prefix max) and, for [3][0]/[3][3], the next one
+//           (suffix min): scans within tiles of 1024 columns, a scan of the
+//           tiles' aggregates, then the tiles again with their carries.
 //  k_chain  a column that only ADDS to [3][0]/[3][3] passes its input on plus
 //           its adds: each chain of such columns is folded by one thread, in
 //           order, from the column that set the entries before it.
@@ -23,13 +25,17 @@ namespace {
 
 constexpr int INFO_NEED = 5;      // bits 0-4: the column sets group g; 5-9: a cell needs its input
 constexpr int INFO_ADDONLY = 10;  // [3][0], [3][3] added to but never set in the column
-constexpr int SCAN_THREADS = 1024;
+constexpr int TILE = 1024;         // columns per scan tile (one thread each)
 constexpr int PREF = 8;           // op codes fetched ahead per column walk
 
 struct Scratch {
   int *info, *prev, *next4;
-  double *out;  // [K][16]: the matrix leaving column K (groups it changes)
+  int *agg, *carry;  // [NGROUP + 1][tiles]: per tile last changing columns and first
+                     // [3][0]/[3][3] change; then the carries from the other tiles
+  double *out;       // [K][16]: the matrix leaving column K (groups it changes)
 };
+
+inline long ntiles(long M) { return (M + TILE - 1) / TILE; }
 
 inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
@@ -42,6 +48,10 @@ Scratch carve(void *p, long M) {
   b += al256(NGROUP * M * sizeof(int));
   s.next4 = reinterpret_cast<int *>(b);
   b += al256(M * sizeof(int));
+  s.agg = reinterpret_cast<int *>(b);
+  b += al256((NGROUP + 1) * ntiles(M) * sizeof(int));
+  s.carry = reinterpret_cast<int *>(b);
+  b += al256((NGROUP + 1) * ntiles(M) * sizeof(int));
   s.out = reinterpret_cast<double *>(b);
   return s;
 }
@@ -131,54 +141,99 @@ __global__ void __launch_bounds__(256) k_col(Geom g, double *__restrict__ db,
   s.info[K] = info;
 }
 
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan(int M, Scratch s) {
-  __shared__ int lastv[NGROUP][SCAN_THREADS];
-  __shared__ int first4[SCAN_THREADS];
+// inclusive prefix max of pm[g][.] and suffix min of sm[.] over the TILE
+// threads of the block (LDS, Hillis-Steele); every thread calls it
+__device__ inline void block_scan(int (*pm)[TILE], int *sm, int M) {
   const int i = threadIdx.x;
-  const int ch = (M + SCAN_THREADS - 1) / SCAN_THREADS;
-  const int k0 = i * ch < M ? i * ch : M;
-  const int k1 = k0 + ch < M ? k0 + ch : M;
-  int last[NGROUP], f4 = M;
-#pragma unroll
-  for (int gi = 0; gi < NGROUP; gi++) last[gi] = -1;
-  for (int K = k0; K < k1; K++) {
-    const int inf = s.info[K];
-#pragma unroll
-    for (int gi = 0; gi < NGROUP; gi++)
-      if (changes(inf, gi)) last[gi] = K;
-    if (f4 == M && changes(inf, 4)) f4 = K;
-  }
-#pragma unroll
-  for (int gi = 0; gi < NGROUP; gi++) lastv[gi][i] = last[gi];
-  first4[i] = f4;
   __syncthreads();
-  // inclusive prefix max (lastv) and suffix min (first4) over the threads
-  for (int d = 1; d < SCAN_THREADS; d <<= 1) {
+  for (int d = 1; d < TILE; d <<= 1) {
     int a[NGROUP];
 #pragma unroll
-    for (int gi = 0; gi < NGROUP; gi++) a[gi] = i >= d ? lastv[gi][i - d] : -1;
-    const int b = i + d < SCAN_THREADS ? first4[i + d] : M;
+    for (int gi = 0; gi < NGROUP; gi++) a[gi] = i >= d ? pm[gi][i - d] : -1;
+    const int b = i + d < TILE ? sm[i + d] : M;
     __syncthreads();
 #pragma unroll
-    for (int gi = 0; gi < NGROUP; gi++) lastv[gi][i] = lastv[gi][i] > a[gi] ? lastv[gi][i] : a[gi];
-    first4[i] = first4[i] < b ? first4[i] : b;
+    for (int gi = 0; gi < NGROUP; gi++) pm[gi][i] = pm[gi][i] > a[gi] ? pm[gi][i] : a[gi];
+    sm[i] = sm[i] < b ? sm[i] : b;
     __syncthreads();
   }
-  int run[NGROUP];
+}
+
+// one thread per column K of tile blockIdx.x.  PASS 0: the tile's aggregates
+// (its last column changing each group, its first changing [3][0]/[3][3]).
+// PASS 1: prev and next4 of every column from the tile's scans and carries.
+template <int PASS>
+__global__ void __launch_bounds__(TILE) k_tile(int M, Scratch s) {
+  __shared__ int pm[NGROUP][TILE];
+  __shared__ int sm[TILE];
+  const int i = threadIdx.x, b = blockIdx.x, nb = gridDim.x;
+  const int K = b * TILE + i;
+  const int inf = K < M ? s.info[K] : 0;
 #pragma unroll
-  for (int gi = 0; gi < NGROUP; gi++) run[gi] = i > 0 ? lastv[gi][i - 1] : -1;
-  for (int K = k0; K < k1; K++) {
-    const int inf = s.info[K];
+  for (int gi = 0; gi < NGROUP; gi++) pm[gi][i] = K < M && changes(inf, gi) ? K : -1;
+  sm[i] = K < M && changes(inf, 4) ? K : M;
+  block_scan(pm, sm, M);
+  if (PASS == 0) {
+    if (i == TILE - 1)
+#pragma unroll
+      for (int gi = 0; gi < NGROUP; gi++) s.agg[gi * nb + b] = pm[gi][i];
+    if (i == 0) s.agg[NGROUP * nb + b] = sm[0];
+    return;
+  }
+  if (K >= M) return;
+#pragma unroll
+  for (int gi = 0; gi < NGROUP; gi++) {
+    int p = s.carry[gi * nb + b];
+    if (i > 0 && pm[gi][i - 1] > p) p = pm[gi][i - 1];
+    s.prev[(long)gi * M + K] = p;
+  }
+  int q = s.carry[NGROUP * nb + b];
+  if (i + 1 < TILE && sm[i + 1] < q) q = sm[i + 1];
+  s.next4[K] = q;
+}
+
+// one block over the nb tile aggregates: carry[g][b] = the last column
+// changing group g in the tiles before b (-1: none), carry[NGROUP][b] = the
+// first column changing [3][0]/[3][3] in the tiles after b (M: none)
+__global__ void __launch_bounds__(TILE) k_carry(int nb, int M, Scratch s) {
+  __shared__ int pm[NGROUP][TILE];
+  __shared__ int sm[TILE];
+  const int i = threadIdx.x;
+  const int ch = (nb + TILE - 1) / TILE;
+  const int b0 = i * ch < nb ? i * ch : nb;
+  const int b1 = b0 + ch < nb ? b0 + ch : nb;
+  int last[NGROUP], f = M;
+#pragma unroll
+  for (int gi = 0; gi < NGROUP; gi++) last[gi] = -1;
+  for (int b = b0; b < b1; b++)
 #pragma unroll
     for (int gi = 0; gi < NGROUP; gi++) {
-      s.prev[(long)gi * M + K] = run[gi];
-      if (changes(inf, gi)) run[gi] = K;
+      const int a = s.agg[gi * nb + b];
+      last[gi] = a > last[gi] ? a : last[gi];
     }
+  for (int b = b0; b < b1; b++) {
+    const int a = s.agg[NGROUP * nb + b];
+    f = a < f ? a : f;
   }
-  int nx4 = i + 1 < SCAN_THREADS ? first4[i + 1] : M;
-  for (int K = k1 - 1; K >= k0; K--) {
-    s.next4[K] = nx4;
-    if (changes(s.info[K], 4)) nx4 = K;
+#pragma unroll
+  for (int gi = 0; gi < NGROUP; gi++) pm[gi][i] = last[gi];
+  sm[i] = f;
+  block_scan(pm, sm, M);
+  int run[NGROUP];
+#pragma unroll
+  for (int gi = 0; gi < NGROUP; gi++) run[gi] = i > 0 ? pm[gi][i - 1] : -1;
+  for (int b = b0; b < b1; b++)
+#pragma unroll
+    for (int gi = 0; gi < NGROUP; gi++) {
+      s.carry[gi * nb + b] = run[gi];
+      const int a = s.agg[gi * nb + b];
+      run[gi] = a > run[gi] ? a : run[gi];
+    }
+  int nf = i + 1 < TILE ? sm[i + 1] : M;
+  for (int b = b1 - 1; b >= b0; b--) {
+    s.carry[NGROUP * nb + b] = nf;
+    const int a = s.agg[NGROUP * nb + b];
+    nf = a < nf ? a : nf;
   }
 }
 
@@ -267,8 +322,9 @@ inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
 size_t vort_scratch_bytes(const Geom &g) {
   const long M = (long)g.nx * g.ny;
+  const size_t aggb = vort::al256((vort::NGROUP + 1) * vort::ntiles(M) * sizeof(int));
   return vort::al256(M * sizeof(int)) + vort::al256(vort::NGROUP * M * sizeof(int)) +
-         vort::al256(M * sizeof(int)) + vort::al256(M * 16 * sizeof(double));
+         vort::al256(M * sizeof(int)) + 2 * aggb + vort::al256(M * 16 * sizeof(double));
 }
 
 void vort_resolve(const Geom &g, double *db, const unsigned char *op, void *scratch,
@@ -277,7 +333,10 @@ void vort_resolve(const Geom &g, double *db, const unsigned char *op, void *scra
   const long M = (long)g.nx * g.ny;
   const Scratch s = carve(scratch, M);
   hipLaunchKernelGGL(k_col, dim3(nblk(M)), dim3(256), 0, st, g, db, op, s);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_THREADS), 0, st, (int)M, s);
+  const unsigned nb = (unsigned)ntiles(M);
+  hipLaunchKernelGGL(k_tile<0>, dim3(nb), dim3(TILE), 0, st, (int)M, s);
+  hipLaunchKernelGGL(k_carry, dim3(1), dim3(TILE), 0, st, (int)nb, (int)M, s);
+  hipLaunchKernelGGL(k_tile<1>, dim3(nb), dim3(TILE), 0, st, (int)M, s);
   hipLaunchKernelGGL(k_chain, dim3(nblk(M)), dim3(256), 0, st, g, db, op, s);
   hipLaunchKernelGGL(k_fill, dim3(nblk(M)), dim3(256), 0, st, g, db, op, s);
 }
