@@ -60,6 +60,21 @@ def test_c2p_bitwise(gpu, oracle):
     assert np.array_equal(og, oo)
 
 
+def test_pressure_of_rest_states_bitwise(gpu):
+    """p = e/3 of the ideal gas (the device forms it with a multiply and two
+    FMAs): a state at rest returns e = Q^tau and p = Q^tau / 3, bit for bit
+    IEEE division, over the whole exponent range (subnormal, zero, huge)"""
+    rng = np.random.default_rng(99)
+    n = 200_000
+    qt = np.ldexp(1.0 + rng.random(n), rng.integers(-1074, 1023, n))
+    qt[:8] = [0.0, 5e-324, 2.2e-308, 2.0 ** -1000, 2.0 ** 1000, 1.7e308, 1.0, 3.0]
+    Q = np.zeros((n, 7))
+    Q[:, 0] = qt
+    out, _, _ = gpu.c2p(Q)
+    assert np.array_equal(out[:, 0], qt)
+    assert np.array_equal(out[:, 1], qt / 3.0)
+
+
 def test_p2c_bitwise(gpu, oracle):
     rng = np.random.default_rng(7)
     n = 50_000

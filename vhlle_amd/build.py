@@ -29,6 +29,9 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contr
 # kernels that are not bound to the reference's rounding: FMA contraction within
 # expressions allowed
 FAST_SOURCES = {"vh_kernels_mixed.hip", "vh_mlp.hip", "vh_batch.hip"}
+# sources with MFMA: accumulators in VGPRs (the ReLU and BF16 packing between
+# layers then read them without a copy out of the AGPRs)
+MFMA_SOURCES = {"vh_mlp.hip"}
 
 
 def _run(cmd, cwd=None):
@@ -68,6 +71,8 @@ def build_lib(verbose=False, defines=(), out=None, objdir=None):
                 # FMA within an expression only, so that the same inlined code
                 # rounds the same way in every kernel that uses it
                 flags[flags.index("-ffp-contract=off")] = "-ffp-contract=on"
+            if os.path.basename(src) in MFMA_SOURCES:
+                flags += ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]
             cmd = [HIPCC] + flags + ["-D" + d for d in defines] + ["-I", ROOT, "-c", src, "-o", obj]
         else:  # host-only C++ (the facade): no HIP headers needed
             cmd = ["g++", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",

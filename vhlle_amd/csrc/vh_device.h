@@ -88,6 +88,22 @@ struct SplineTab {
   }
 };
 
+// e / 3, correctly rounded, without a division.  y = RN(1/3) = (1 - 2^-54)/3,
+// so q = RN(e y) lies within one ulp of e/3; then the remainder e - 3q is
+// exact (one FMA) and RN(q + (e - 3q) y) is the correctly rounded quotient
+// (Markstein's correction theorem; no quotient is a rounding midpoint, since
+// 3 times an odd 54-bit integer does not fit 53 bits).  Outside
+// [2^-1000, 2^1000] in magnitude (subnormal quotients, 0, inf, NaN) it is the
+// division itself.  Bit-identical to e / 3. on every input.
+__device__ inline double div3(double e) {
+  const double y = 0.33333333333333331483;  // RN(1/3)
+  const double a = fabs(e);
+  if (!(a >= 0x1p-1000 && a <= 0x1p1000)) return e / 3.;
+  const double q = e * y;
+  const double r = fma(-q, 3., e);
+  return fma(r, y, q);
+}
+
 // Device EoS (kind VH_EOS_IDEAL or VH_EOS_TABLE, include/vhlle_gpu.h).
 // Ideal massless gas: the SIMPLE branch of EoSs (src/eos.cpp:87-111), cnst
 // computed on the host exactly as the reference's compile-time constant.
@@ -105,7 +121,7 @@ struct DevEos {
     if (out && st) atomicOr(st, VH_ST_EOS_RANGE);
     return v;
   }
-  __device__ double p(double e) const { return kind == VH_EOS_TABLE ? spline(0, e) : e / 3.; }
+  __device__ double p(double e) const { return kind == VH_EOS_TABLE ? spline(0, e) : div3(e); }
   __device__ double T(double e) const {
     if (kind == VH_EOS_TABLE) return spline(1, e);
     return e > 0. ? 1.0 * pow(e / cnst, 0.25) : 0.;
@@ -128,7 +144,7 @@ struct DevEos {
       return v;
     }
     dp = 1. / 3.;
-    return e / 3.;
+    return div3(e);
   }
   // EoSs::cs2(e) = dpe(e) (src/eos.cpp:89-96; 1/3 for the ideal gas)
   __device__ double cs2e(double e) const {
