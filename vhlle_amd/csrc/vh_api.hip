@@ -155,7 +155,7 @@ void ghost_fill(vh_ctx *c, int full, double *Qn, hipStream_t st, int &rc) {
   const char *name = full ? "ghost_full" : "ghost_ideal";
   if (pf && c->fusable) {
     rc |= timed(c, name, [&] {
-      vh::mx::launch_ghost_cells(full, c->G, c->degenerate, Qn, c->Qh,
+      vh::mx::launch_ghost_cells(full, c->charged ? 7 : 4, c->G, c->degenerate, Qn, c->Qh,
                                  reinterpret_cast<float *>(c->pi),
                                  reinterpret_cast<float *>(c->Pi), st);
     });

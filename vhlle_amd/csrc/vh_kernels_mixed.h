@@ -63,8 +63,10 @@ void launch_visc3(const Grids &G, const double *ph, const float *piH, const floa
                   const float *vf, const double *m, int m_full, const double *tp, double dt,
                   double *Q, hipStream_t st);
 // single-pass ghost fill of every grid (each extent 1 or >= 5); degenerate:
-// some grid has an extent 1.  full = 0: Q, Qh; full = 1: Q, pi, Pi.
-void launch_ghost_cells(int full, const Grids &G, bool degenerate, double *Q, double *Qh,
+// some grid has an extent 1.  full = 0: Q, Qh; full = 1: Q, pi, Pi.  nc: the
+// conserved components the states carry (4: the charge components are zero
+// and are not copied on a single grid)
+void launch_ghost_cells(int full, int nc, const Grids &G, bool degenerate, double *Q, double *Qh,
                         float *pi, float *Pi, hipStream_t st);
 // the three-pass fill of one grid (any extents): Q, pi, Pi
 void launch_ghost_full(int axis, const Geom &g, double *Q, float *pi, float *Pi, hipStream_t st);

@@ -1251,7 +1251,7 @@ k_ghost_b(Grids G, double *__restrict__ Q, double *__restrict__ Qh, float *__res
 
 // k_ghost_b of a single grid over its ghost cells only (every extent >= 5;
 // ghost_cell)
-template <int FULL>
+template <int FULL, int NC>
 __global__ void __launch_bounds__(256)
 k_ghost_list(Geom g, long n, double *__restrict__ Q, double *__restrict__ Qh,
              float *__restrict__ pi, float *__restrict__ Pi) {
@@ -1261,18 +1261,18 @@ k_ghost_list(Geom g, long n, double *__restrict__ Q, double *__restrict__ Qh,
   ghost_cell(g.nx, g.ny, g.nz, t, ix, iy, iz);
   const long N = g.P, c = g.idx(ix, iy, iz);
   const long src = g.idx(ghost_src(ix, g.nx), ghost_src(iy, g.ny), ghost_src(iz, g.nz));
-  double v[7];
+  double v[NC];
 #pragma unroll
-  for (int k = 0; k < 7; k++) v[k] = Q[k * N + src];
+  for (int k = 0; k < NC; k++) v[k] = Q[k * N + src];
   const bool z = v[0] < 0.;
 #pragma unroll
-  for (int k = 0; k < 7; k++) Q[k * N + c] = z ? 0. : v[k];
+  for (int k = 0; k < NC; k++) Q[k * N + c] = z ? 0. : v[k];
   if (!FULL) {
 #pragma unroll
-    for (int k = 0; k < 7; k++) v[k] = Qh[k * N + src];
+    for (int k = 0; k < NC; k++) v[k] = Qh[k * N + src];
     const bool zh = v[0] < 0.;
 #pragma unroll
-    for (int k = 0; k < 7; k++) Qh[k * N + c] = zh ? 0. : v[k];
+    for (int k = 0; k < NC; k++) Qh[k * N + c] = zh ? 0. : v[k];
   } else {
 #pragma unroll
     for (int k = 0; k < 10; k++) pi[k * N + c] = pi[k * N + src];
@@ -1397,14 +1397,17 @@ void launch_visc3(const Grids &G, const double *ph, const float *piH, const floa
                      m_full, tp, dt, Q);
 }
 
-void launch_ghost_cells(int full, const Grids &G, bool degenerate, double *Q, double *Qh,
+void launch_ghost_cells(int full, int nc, const Grids &G, bool degenerate, double *Q, double *Qh,
                         float *pi, float *Pi, hipStream_t st) {
   const dim3 gr((unsigned)G.nblk), b(256);
   const Geom &g = G.g0;
   if (G.nev == 1 && g.nx >= 5 && g.ny >= 5 && g.nz >= 5) {  // over the ghost cells only
     const long n = ghost_count(g.nx, g.ny, g.nz);
-    if (full) hipLaunchKernelGGL(k_ghost_list<1>, dim3(blocks(n)), b, 0, st, g, n, Q, Qh, pi, Pi);
-    else hipLaunchKernelGGL(k_ghost_list<0>, dim3(blocks(n)), b, 0, st, g, n, Q, Qh, pi, Pi);
+    const dim3 gl(blocks(n));
+    if (full && nc == 4) hipLaunchKernelGGL((k_ghost_list<1, 4>), gl, b, 0, st, g, n, Q, Qh, pi, Pi);
+    else if (full) hipLaunchKernelGGL((k_ghost_list<1, 7>), gl, b, 0, st, g, n, Q, Qh, pi, Pi);
+    else if (nc == 4) hipLaunchKernelGGL((k_ghost_list<0, 4>), gl, b, 0, st, g, n, Q, Qh, pi, Pi);
+    else hipLaunchKernelGGL((k_ghost_list<0, 7>), gl, b, 0, st, g, n, Q, Qh, pi, Pi);
     return;
   }
   if (degenerate) {
