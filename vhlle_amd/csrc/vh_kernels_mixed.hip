@@ -97,6 +97,24 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float fmin_(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float fmax_(float a, float b) { return (a < b) ? b : a; }
+// FP64 1/x and 1/sqrt(x) to about an ulp, not correctly rounded: the hardware
+// estimate refined by two Newton steps (each doubles the correct bits).  For
+// the per-cell Lorentz factors and 1/u^k of the mixed relaxation kernel,
+// whose results are scaled into FP32.  x must be finite and nonzero (1/inf
+// and 1/0 come out NaN, not 0 and inf): the uniform factors of the grid
+// spacing, where a grid of one cell along an axis has spacing inf, keep the
+// division.
+__device__ __forceinline__ double drcp(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(y, fma(-x, y, 1.), y);
+  return fma(y, fma(-x, y, 1.), y);
+}
+__device__ __forceinline__ double drsqrt(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = fma(y, fma(-h * y, y, 0.5), y);
+  return fma(y, fma(-h * y, y, 0.5), y);
+}
 
 // FP32 copy of a primitive state; gamma^2 is formed in FP64 first so that
 // |v| close to 1 does not round to 1 in FP32
@@ -744,7 +762,7 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
     if (VORT) vort::store(db, dop, N, c, vort::K_EMPTY, nullptr);
     return;
   }
-  const double gamma = 1.0 / sqrt(1.0 - vx * vx - vy * vy - vz * vz);
+  const double gamma = drsqrt(1.0 - vx * vx - vy * vy - vz * vz);
   const double u[4] = {gamma, gamma * vx, gamma * vy, gamma * vz};
   // ---- NSquant in FP64 (src/hdo.cpp:381-652)
   double dmu[4][4];
@@ -765,7 +783,7 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
   // the reference's tests |0.5 (a + b) / u| > UDIFF as |0.5 (a + b)| > UDIFF |u|,
   // and its divisions by u^k as products with 1/u^k (used only where
   // |u^k| >= VMIN, or k = 0 with u^0 >= 1)
-  const double ru[4] = {1. / u[0], 1. / u[1], 1. / u[2], 1. / u[3]};
+  const double ru[4] = {drcp(u[0]), drcp(u[1]), drcp(u[2]), drcp(u[3])};
   const double ihx = 1. / g.dx, ihy = 1. / g.dy, ihz = 1. / (g.dz * (tau + 0.5 * dt));
   double vv[16];  // vorticity entries formed by this cell (VORT)
   unsigned char vflags = 0;
@@ -781,8 +799,8 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
     for (int k = 0; k < 7; k++) q[k] = k < NC ? Qn[k * N + c] * it1 : 0.;
     c2p_mode<CM>(eos, q, p1, guess_of<CM>(gv, c));
     const double idt = 1. / dt;
-    const double ut0 = 1.0 / sqrt(1.0 - p0.vx * p0.vx - p0.vy * p0.vy - p0.vz * p0.vz);
-    const double ut1 = 1.0 / sqrt(1.0 - p1.vx * p1.vx - p1.vy * p1.vy - p1.vz * p1.vz);
+    const double ut0 = drsqrt(1.0 - p0.vx * p0.vx - p0.vy * p0.vy - p0.vz * p0.vz);
+    const double ut1 = drsqrt(1.0 - p1.vx * p1.vx - p1.vy * p1.vy - p1.vz * p1.vz);
     const double a0[4] = {ut0, ut0 * p0.vx, ut0 * p0.vy, ut0 * p0.vz};
     const double a1[4] = {ut1, ut1 * p1.vx, ut1 * p1.vy, ut1 * p1.vz};
     if (VORT) vort::row_time(vv, a0, a1, eos.T(p0.e), eos.T(p1.e), e, p0.e, dt);
@@ -805,8 +823,8 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
       if (pt[ca] > 0. && pt[cb] > 0.) {
         const double avx = pt[N + ca], avy = pt[2 * N + ca], avz = pt[3 * N + ca];
         const double bvx = pt[N + cb], bvy = pt[2 * N + cb], bvz = pt[3 * N + cb];
-        const double gb = 1.0 / sqrt(1.0 - bvx * bvx - bvy * bvy - bvz * bvz);
-        const double ga = 1.0 / sqrt(1.0 - avx * avx - avy * avy - avz * avz);
+        const double gb = drsqrt(1.0 - bvx * bvx - bvy * bvy - bvz * bvz);
+        const double ga = drsqrt(1.0 - avx * avx - avy * avy - avz * avz);
         const double B[4] = {gb, gb * bvx, gb * bvy, gb * bvz};
         const double A[4] = {ga, ga * avx, ga * avy, ga * avz};
         const double ih = d == 1 ? ihx : (d == 2 ? ihy : ihz);
@@ -885,7 +903,7 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
   if (taupi < hdt) dpp = tpp = lpP = phi7 = 0.f;
   float dPP = 2.f / 3.f, lPp = 1.6f * (1.f / 3.f - eos.cs2f(ef));
   if (tauPi < hdt) dPP = lPp = 0.f;
-  const float ig = (float)(1. / gamma);
+  const float ig = (float)drcp(gamma);
   const float uf[4] = {(float)u[0], (float)u[1], (float)u[2], (float)u[3]};
   float af[4];
 #pragma unroll
