@@ -214,7 +214,7 @@ def run_variant(args, cfg, prec, c2p, ic):
     if p == abi.PREC_MIXED:
         k, cz, ms = ctx.get_stage()
         stage = dict(kernel={1: "cells", 2: "rows"}.get(k, "auto"), CZ=cz,
-                     tuned_pred_ms=round(ms, 4))
+                     tuned_pred_corr_ms=round(ms, 4))
     ctx.close()
     print(f"[bench] {prec}/{c2p}: {args.steps} steps in {elapsed:.3f} s, status {st | st2}",
           file=sys.stderr, flush=True)

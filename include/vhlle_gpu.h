@@ -195,8 +195,9 @@ int vh_set_c2p_mode(vh_ctx_t *ctx, int mode);
  * Both give identical results.  VH_STAGE_AUTO (default): an autotuner times
  * the candidates (cells; rows with CZ = 1..32) on the first vh_step/vh_run
  * (and after vh_set_c2p_mode) and keeps the fastest.  vh_get_stage reports
- * the kernel in use (VH_STAGE_AUTO: not tuned yet), CZ and the tuned stage
- * time in ms.  Mixed precision only (VH_ERR_ARG otherwise). */
+ * the kernel in use (VH_STAGE_AUTO: not tuned yet), CZ and the tuned time of
+ * a predictor and a corrector stage in ms.  Mixed precision only (VH_ERR_ARG
+ * otherwise). */
 enum { VH_STAGE_AUTO = 0, VH_STAGE_CELLS = 1, VH_STAGE_ROWS = 2 };
 int vh_set_stage(vh_ctx_t *ctx, int kind, int CZ);
 int vh_get_stage(const vh_ctx_t *ctx, int *kind, int *CZ, double *ms);
