@@ -338,7 +338,9 @@ __device__ __forceinline__ PrimF face_state(const DevEos &eos, const double *__r
       continue;
     }
     const double a = Qs[k * N + c0];
-    const double dq = minmod((Qs[k * N + cp] - a) * 0.5, (a - Qs[k * N + cm]) * 0.5);
+    // minmod(x/2, y/2) = minmod(x, y)/2 (the halving is exact; only products
+    // x y below ~1e-308 can decide the sign test differently)
+    const double dq = 0.5 * minmod(Qs[k * N + cp] - a, a - Qs[k * N + cm]);
     q[k] = (SIDE < 0 ? a - dq : a + dq) * itau;
   }
   if (CM == VH_C2P_CLOSED) return c2p_fast(q);
@@ -447,12 +449,12 @@ __device__ __forceinline__ void axis_fluxes(const Geom &g, const DevEos &eos,
                                             double acc[7], int *status) {
   if (n < 2) return;
   const long N = g.P;
-  const double itau = 1. / tau;
+  const double itau = drcp(tau);  // tau > 0 and finite
   const double tauFactor = MODE == 0 ? tau + 0.25 * dt : tau + 0.5 * dt;
   const double dta = MODE == 0 ? 0.5 * dt : dt;
   const double dx = D == 1 ? g.dx : (D == 2 ? g.dy : g.dz);
-  const float pre = (float)(tauFactor * dta / dx);
-  const float itf = (float)(1. / tauFactor), itauf = (float)itau;
+  const float pre = (float)(tauFactor * dta / dx);  // dx = inf on a one-cell axis
+  const float itf = (float)drcp(tauFactor), itauf = (float)itau;
   const long cm = i > 0 ? c - s : c, cp = i < n - 1 ? c + s : c;
   const PrimF L0 = face_state<-1, CM, NC>(eos, Qs, gv, N, cm, c, cp, itau);
   const PrimF R0 = face_state<+1, CM, NC>(eos, Qs, gv, N, cm, c, cp, itau);
@@ -485,7 +487,7 @@ __device__ __forceinline__ void finish_cell(const DevEos &eos, const double *__r
                                             double *__restrict__ out) {
   const double tau1 = MODE == 0 ? tau : tau + 0.5 * dt;
   const double sdt = MODE == 0 ? 0.5 * dt : dt;
-  const double it1 = 1. / tau1;
+  const double it1 = drcp(tau1);
   double q[7];
 #pragma unroll
   for (int k = 0; k < 7; k++) q[k] = k < NC ? Qs[k * N + c] * it1 : 0.;
@@ -589,12 +591,12 @@ k_stage_rows(Grids G, DevEos eos, const double *__restrict__ Q, const double *__
   const long N = g.P, sy = g.nx, sz = nxy;
   const double tau = *tp;
   const double *Qs = MODE == 0 ? Q : Qh;
-  const double itau = 1. / tau;
+  const double itau = drcp(tau);  // as in axis_fluxes: identical interface states
   const double tauFactor = MODE == 0 ? tau + 0.25 * dt : tau + 0.5 * dt;
   const double dta = MODE == 0 ? 0.5 * dt : dt;
   const float prx = (float)(tauFactor * dta / g.dx), pry = (float)(tauFactor * dta / g.dy),
               prz = (float)(tauFactor * dta / g.dz);
-  const float itf = (float)(1. / tauFactor), itauf = (float)itau;
+  const float itf = (float)drcp(tauFactor), itauf = (float)itau;
   float fzl[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   float fzu[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   bool hzl = false;
