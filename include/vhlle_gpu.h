@@ -184,6 +184,18 @@ int vh_step(vh_ctx_t *ctx, double *tau, double dt, int *status);
 int vh_run(vh_ctx_t *ctx, double *tau, double dt, int nsteps, int *status);
 /* Hydro::setDtau validity test (src/hdo.cpp:67-75): VH_OK or VH_ERR_DTAU. */
 int vh_check_dtau(const vh_ctx_t *ctx, double dt);
+/* Largest HLLE signal speeds max(-b_l, b_r) over the interfaces that formed a
+ * flux: the wave-speed estimates of Hydro::hlle_flux (src/hdo.cpp:181-188,
+ * 210-227, 255-264; 1 or 1/tau at a vacuum boundary), in units of dx/dtau
+ * (eta: deta/dtau, as the reference divides them by tau).  The reference
+ * keeps its timestep fixed and never forms this maximum; here every flux
+ * kernel reduces it per wave, so a caller can check the Courant numbers
+ * dt * out[k] / dx_k without another pass over the grid.  Steps are not
+ * changed by it.  out[9]: the last step's predictor (x, y, eta) and
+ * corrector (x, y, eta) maxima, then the per-axis maxima over every step
+ * since the last vh_set_prim / vh_upload or reset call; reset != 0 clears the
+ * latter after reading. */
+int vh_signal_speed(vh_ctx_t *ctx, int reset, double *out);
 
 /* Selects the con2prim algorithm for later steps (VH_C2P_*). */
 int vh_set_c2p_mode(vh_ctx_t *ctx, int mode);

@@ -103,6 +103,8 @@ def lib():
         L.or_run.restype = ctypes.c_int
         L.or_run.argtypes = [ctypes.c_void_p, dp, ctypes.c_double, ctypes.c_int]
         L.or_check_dtau.restype = ctypes.c_int
+        L.or_branch_counts.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long)]
+        L.or_signal_speed.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
         L.or_check_dtau.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.or_c2p.argtypes = [ctypes.POINTER(OrEos), dp, ctypes.c_long, dp,
                              ctypes.POINTER(ctypes.c_int)]
@@ -227,6 +229,21 @@ class Solver:
 
     def set_vorticity(self, on=True):
         lib().or_set_vorticity(self.h, int(on))
+
+    def branch_counts(self):
+        """(negative-pressure re-evaluations of hlle_flux in the predictor, in
+        the corrector, limiter rescales, Pi clamps) since creation"""
+        out = (ctypes.c_long * 4)()
+        lib().or_branch_counts(self.h, out)
+        return tuple(int(v) for v in out)
+
+    def signal_speed(self, reset=False):
+        """9 values: the last step's predictor and corrector maxima of
+        max(-bl, br) per axis (x, y, eta), then the per-axis maxima since the
+        state was set (or the last reset)"""
+        out = np.zeros(9)
+        lib().or_signal_speed(self.h, int(reset), _p(out))
+        return out
 
     def get_dbeta(self):
         """Cell::getDbeta of every cell: (16, N), row 4 i + j = d_i (u^j / T)"""

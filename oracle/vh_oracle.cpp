@@ -391,6 +391,17 @@ struct Solver {
   double dt = 0, tau = 0;
   int status = 0;  // 0 ok, 1 NaN exit(1), 2 e>1e10 exit(0)
   bool vort = false;  // Hydro::enableVorticity (src/hdo.cpp:60-65)
+  // Branch counters (test instrumentation, no effect on the results):
+  // [0], [1] negative-pressure re-evaluations of hlle_flux in the predictor
+  // and the corrector (src/hdo.cpp:134-140), [2] cells whose pi was rescaled
+  // by the ISformal limiter (src/hdo.cpp:925-932), [3] cells whose Pi was
+  // clamped to +-p (:933-937).
+  long counts[4] = {0, 0, 0, 0};
+  // HLLE signal speeds max(-bl, br) of the interfaces that formed a flux
+  // (src/hdo.cpp:181-188, 210-227, 255-264): [0..5] the running step's
+  // predictor/corrector maxima per axis, [6..11] the last step's, [12..14]
+  // the per-axis maxima since the last reset.
+  double sig[15] = {0};
 
   // Cell::getPrimVar{Left,Right,HLeft,HRight} (src/cll.cpp:106-204):
   // MUSCL half-slope reconstruction of Q (or Qh), face state (Q -+ dQ)/tau.
@@ -442,6 +453,7 @@ struct Solver {
     if (el == 0. && er == 0.) return;
     if (pr < 0.) {  // the reference re-evaluates both states from Q (not Qh)
                     // in either mode (src/hdo.cpp:134-140)
+      counts[mode == kPredict ? 0 : 1]++;
       face_state(ix, iy, iz, d, false, +1, tau, l);
       face_state(rx, ry, rz, d, false, -1, tau, r);
     }
@@ -517,6 +529,9 @@ struct Solver {
     for (int i = 0; i < 7; i++) mF[i] = -F[i];
     L.add_flux(mF);
     R.add_flux(F);
+    const double sp = std::max(-bl, br);
+    double &slot = sig[3 * mode + d - 1];
+    if (sp > slot) slot = sp;
   }
 
   // Hydro::source + source_step (src/hdo.cpp:307-353)
@@ -900,8 +915,10 @@ struct Solver {
                 piH[i][j] = 0.1 * piH[i][j] * maxT0 / maxpi;
               }
             rescaled = true;
+            counts[2]++;
           }
           if (std::fabs(Pi) > p) {
+            counts[3]++;
             if (Pi != 0.) Pi = 1 * Pi / std::fabs(Pi) * p;
             if (PiH != 0.) PiH = 1 * PiH / std::fabs(PiH) * p;
             rescaled = true;
@@ -1118,6 +1135,11 @@ struct Solver {
           }
     }
     tau += dt;
+    for (int k = 0; k < 6; k++) {
+      sig[6 + k] = sig[k];
+      if (sig[k] > sig[12 + k % 3]) sig[12 + k % 3] = sig[k];
+      sig[k] = 0.;
+    }
     ghost_ideal();
     if (trc.isViscous()) {
       isformal();
@@ -1187,6 +1209,7 @@ long or_ncell(void *h) { return (long)static_cast<Solver *>(h)->g.c.size(); }
 
 void or_set_state(void *h, const or_state_t *st) {
   Solver *s = static_cast<Solver *>(h);
+  for (double &v : s->sig) v = 0.;  // a new state: no signal speeds yet
   const long N = (long)s->g.c.size();
   for (long n = 0; n < N; n++) {
     OCell &c = s->g.c[n];
@@ -1255,6 +1278,7 @@ void or_set_prim(void *h, double tau, const double *e, const double *nb,
                  const double *nq, const double *ns, const double *vx,
                  const double *vy, const double *vz) {
   Solver *s = static_cast<Solver *>(h);
+  for (double &v : s->sig) v = 0.;
   const long N = (long)s->g.c.size();
   for (long n = 0; n < N; n++) {
     OCell &c = s->g.c[n];
@@ -1281,6 +1305,18 @@ int or_run(void *h, double *tau, double dt, int nsteps) {
   for (int i = 0; i < nsteps && !s->status; i++) s->step();
   *tau = s->tau;
   return s->status;
+}
+
+void or_branch_counts(void *h, long *out) {
+  const Solver *sv = static_cast<Solver *>(h);
+  for (int k = 0; k < 4; k++) out[k] = sv->counts[k];
+}
+
+void or_signal_speed(void *h, int reset, double *out) {
+  Solver *sv = static_cast<Solver *>(h);
+  for (int k = 0; k < 9; k++) out[k] = sv->sig[6 + k];
+  if (reset)
+    for (int k = 12; k < 15; k++) sv->sig[k] = 0.;
 }
 
 int or_check_dtau(void *h, double dt) {

@@ -72,6 +72,16 @@ int or_run(void *h, double *tau, double dt, int nsteps);
 void or_set_vorticity(void *h, int on);
 /* Cell::getDbeta of every cell: out 16 x N, component 4 i + j = dbeta[i][j] */
 void or_get_dbeta(void *h, double *out);
+/* Branch counters since creation: out[4] = negative-pressure re-evaluations
+ * of hlle_flux in the predictor and in the corrector (src/hdo.cpp:134-140),
+ * cells rescaled by the ISformal limiter (src/hdo.cpp:925-932), cells with Pi
+ * clamped to +-p (src/hdo.cpp:933-937). */
+void or_branch_counts(void *h, long *out);
+/* Largest HLLE signal speeds max(-bl, br) over the interfaces that formed a
+ * flux: out[9] = the last step's predictor (x, y, eta) and corrector (x, y,
+ * eta) maxima, then the per-axis maxima since creation or the last reset
+ * (reset != 0 clears them after reading). */
+void or_signal_speed(void *h, int reset, double *out);
 /* Hydro::setDtau validity test (src/hdo.cpp:67-75): 0 if accepted, 1 if the
  * reference would exit(1). */
 int or_check_dtau(void *h, double dt);

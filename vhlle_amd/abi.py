@@ -161,6 +161,7 @@ def lib():
         "vh_step": (ctypes.c_int, [vp, _dp, ctypes.c_double, ip]),
         "vh_run": (ctypes.c_int, [vp, _dp, ctypes.c_double, ctypes.c_int, ip]),
         "vh_check_dtau": (ctypes.c_int, [vp, ctypes.c_double]),
+        "vh_signal_speed": (ctypes.c_int, [vp, ctypes.c_int, _dp]),
         "vh_set_c2p_mode": (ctypes.c_int, [vp, ctypes.c_int]),
         "vh_enable_timing": (ctypes.c_int, [vp, ctypes.c_int]),
         "vh_kernel_times": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_void_p, ip, _dp]),
@@ -220,7 +221,7 @@ def exported_symbols():
             "vh_set_c2p_model", "vh_mlp_harvest", "vh_set_stage", "vh_get_stage", "vh_create_batch", "vh_hotspot_ic",
             "vh_copy_bandwidth", "vh_fo_snapshot", "vh_freezeout", "vh_cornelius",
             "vh_eos_table", "vh_set_vorticity", "vh_get_dbeta", "vh_freezeout_beta",
-            "vh_corona", "vh_prim_fields", "vh_fluid_stats", "vh_cells"]
+            "vh_corona", "vh_prim_fields", "vh_fluid_stats", "vh_cells", "vh_signal_speed"]
 
 
 def _p(a):
@@ -555,3 +556,11 @@ class Context:
 
     def check_dtau(self, dt):
         return lib().vh_check_dtau(self.h, dt)
+
+    def signal_speed(self, reset=False):
+        """9 values: the last step's predictor and corrector maxima of the
+        HLLE signal speed max(-b_l, b_r) per axis (x, y, eta), then the per-axis
+        maxima since the state was set (or the last reset)"""
+        out = np.zeros(9)
+        check(lib().vh_signal_speed(self.h, int(reset), _p(out)), "vh_signal_speed")
+        return out

@@ -10,6 +10,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -62,7 +63,7 @@ def build_lib(verbose=False, defines=(), out=None, objdir=None):
     lib = out or LIB
     objdir = objdir or os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    jobs = []
     for src in hip_sources() + host_sources():
         obj = os.path.join(objdir, os.path.basename(src) + ".o")
         if src.endswith(".hip"):
@@ -77,10 +78,16 @@ def build_lib(verbose=False, defines=(), out=None, objdir=None):
         else:  # host-only C++ (the facade): no HIP headers needed
             cmd = ["g++", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
                    "-I", ROOT, "-c", src, "-o", obj]
-        log = _run(cmd)
-        if verbose and log.strip():
-            print(log)
-        objs.append(obj)
+        jobs.append((cmd, obj))
+    # the sources compile independently: one compiler process per source
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        logs = list(ex.map(lambda j: _run(j[0]), jobs))
+    if verbose:
+        for log in logs:
+            if log.strip():
+                print(log)
+    objs = [obj for _, obj in jobs]
     tmp = lib + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs)
     os.replace(tmp, lib)

@@ -59,6 +59,11 @@ struct vh_ctx {
   double *ph = nullptr, *pt = nullptr, *vf = nullptr;
   double *tp = nullptr;
   int *status = nullptr;
+  // HLLE signal speeds (bit patterns of non-negative doubles): [0..5] the
+  // running step's predictor and corrector maxima (x, y, eta), [6..11] the
+  // last completed step's, [12..14] the maxima since the last reset,
+  // [16..18] the autotuner's scratch slots
+  unsigned long long *sig = nullptr;
   // learned con2prim: MFMA fragments of the installed MLP and the per-cell
   // guesses (mixed precision only; gv reuses the FP64 path's F[0] storage)
   vh::ml::Packed *mlp = nullptr;
@@ -202,11 +207,11 @@ int enqueue_step_mixed(vh_ctx *c, double dt) {
     rc |= timed(c, "c2p_guess", [&] { vh::ml::launch_guess(Qo, g.N, nc, c->tp, 0., c->mlp, c->gv, st); });
   auto stage = [&](int mode, double *out) {
     vh::mx::launch_stage_tile(mode, cm, nc, G, c->eos, Qo, c->Qh, c->gv, c->m, c->m_full, c->tp,
-                              dt, out, c->status, c->tile, st);
+                              dt, out, c->status, c->sig + 3 * mode, c->tile, st);
   };
   rc |= timed(c, "stage_pred", [&] { stage(0, c->Qh); });
   rc |= timed(c, "stage_corr", [&] { stage(1, Qn); });
-  rc |= timed(c, "advance_tau", [&] { vh::launch_advance_tau(c->tp, dt, st); });
+  rc |= timed(c, "advance_tau", [&] { vh::launch_advance_tau(c->tp, dt, c->sig, st); });
   ghost_fill(c, 0, Qn, st, rc);
   if (c->viscous) {
     rc |= timed(c, "visc_prim", [&] { vh::mx::launch_vprim(cm, nc, g.N, c->eos, c->Qh, c->gv, c->tp, dt, c->ph, c->pt, st); });
@@ -246,8 +251,8 @@ int enqueue_step_fp64(vh_ctx *c, double dt) {
     for (int D = 1; D <= 3; D++) {
       if (ext[D - 1] < 2) continue;  // no interfaces along a 1-cell axis
       rc |= timed(c, face_name(D, mode), [&] {
-        vh::launch_faces(D, mode, g, c->eos, src, c->m, c->m_full, c->tp, dt, c->F[D - 1],
-                         c->fl[D - 1], c->status, st);
+        vh::launch_faces(D, mode, g, c->eos, src, Qo, c->m, c->m_full, c->tp, dt, c->F[D - 1],
+                         c->fl[D - 1], c->status, c->sig + 3 * mode + D - 1, st);
       });
     }
     const double *Fs[3] = {c->F[0], c->F[1], c->F[2]};
@@ -257,7 +262,7 @@ int enqueue_step_fp64(vh_ctx *c, double dt) {
                         c->status, st);
     });
   }
-  rc |= timed(c, "advance_tau", [&] { vh::launch_advance_tau(c->tp, dt, st); });
+  rc |= timed(c, "advance_tau", [&] { vh::launch_advance_tau(c->tp, dt, c->sig, st); });
   // Fluid::correctImagCells (src/fld.cpp:211-273): z, y, x
   ghost_fill(c, 0, Qn, st, rc);
   if (c->viscous) {
@@ -485,7 +490,7 @@ int create_impl(const vh_grid_t *grids, int ngrid, const vh_eos_t *eos, const vh
   auto rnd = [&](size_t b) { return (b + al - 1) / al * al; };
   const size_t dN = rnd(N * sizeof(double)), bN = rnd(N);
   const size_t ndoubles_arrays = 14 + 7 + 20 + 2 + 20 + 2 + 3 + 3 + 1 + 21 + 8 + 4 + 4 + 1;
-  const size_t total = ndoubles_arrays * dN + 3 * bN + 2 * al;
+  const size_t total = ndoubles_arrays * dN + 3 * bN + 3 * al;
   if (hipMalloc(&c->pool, total) != hipSuccess) {
     (void)hipStreamDestroy(c->st);
     delete c;
@@ -519,6 +524,8 @@ int create_impl(const vh_grid_t *grids, int ngrid, const vh_eos_t *eos, const vh
   c->tp = takef(1);
   c->gv = reinterpret_cast<float *>(c->F[0]);  // the fused mixed stages use no F
   c->status = reinterpret_cast<int *>(p);
+  p += al;
+  c->sig = reinterpret_cast<unsigned long long *>(p);
   p += al;
   if (vh::make_dev_eos(eos, c->status, c->eos, &c->eos_buf) != VH_OK) {
     vh_destroy(c);
@@ -637,6 +644,7 @@ int vh_upload(vh_ctx_t *c, const vh_state_t *h) {
   }
   rc |= put(c->m, h->m, 3, 0.);
   rc |= put(c->vcc, h->viscCorrCut, 1, 1.);
+  vh::launch_clear_u64(c->sig, 32, c->st);  // a new state: no signal speeds yet
   VH_HIP(hipStreamSynchronize(c->st));
   if (rc != VH_OK) return VH_ERR_HIP;
   return refresh_m_full(c);
@@ -704,6 +712,7 @@ int vh_set_prim(vh_ctx_t *c, double tau, const double *e, const double *nb, cons
   }
   if (rc == VH_OK) {
     vh::launch_set_prim(c->g, c->eos, tau, d, c->Q[c->cur], c->Q[c->cur ^ 1], c->m, c->st);
+    vh::launch_clear_u64(c->sig, 32, c->st);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->st) != hipSuccess)
       rc = VH_ERR_HIP;
   }
@@ -802,9 +811,9 @@ static int ensure_tile(vh_ctx_t *c, double tau, double dt) {
     // predictor gains)
     auto go = [&] {
       vh::mx::launch_stage_tile(0, c->c2p_mode, nc, c->G, c->eos, c->Q[c->cur], c->Qh, c->gv,
-                                c->m, c->m_full, c->tp, dt, so, ss, t, st);
+                                c->m, c->m_full, c->tp, dt, so, ss, c->sig + 16, t, st);
       vh::mx::launch_stage_tile(1, c->c2p_mode, nc, c->G, c->eos, c->Q[c->cur], so, c->gv,
-                                c->m, c->m_full, c->tp, dt, sc, ss, t, st);
+                                c->m, c->m_full, c->tp, dt, sc, ss, c->sig + 16, t, st);
     };
     go();
     go();
@@ -880,6 +889,16 @@ int vh_step(vh_ctx_t *c, double *tau, double dt, int *status) {
 
 int vh_run(vh_ctx_t *c, double *tau, double dt, int nsteps, int *status) {
   return run_impl(c, tau, dt, nsteps, status, true);
+}
+
+int vh_signal_speed(vh_ctx_t *c, int reset, double *out) {
+  if (!c || !out) return VH_ERR_ARG;
+  unsigned long long h[9];
+  VH_HIP(hipMemcpyAsync(h, c->sig + 6, sizeof h, hipMemcpyDeviceToHost, c->st));
+  if (reset) vh::launch_clear_u64(c->sig + 12, 3, c->st);
+  VH_HIP(hipStreamSynchronize(c->st));
+  for (int k = 0; k < 9; k++) std::memcpy(&out[k], &h[k], sizeof(double));
+  return VH_OK;
 }
 
 int vh_enable_timing(vh_ctx_t *c, int enable) {

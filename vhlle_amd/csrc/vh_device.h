@@ -438,6 +438,23 @@ __device__ __forceinline__ long xcd_block() {
   return (long)x * q + (x < r ? x : r) + i;
 }
 
+// Maximum over the 64 lanes of a wave (every lane must be active); the result
+// is in every lane.  Butterfly of cross-lane exchanges.
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+// Maximum signal speed of a wave into slot *s (a non-negative double kept as
+// its bit pattern: for non-negative doubles the integer order is the numeric
+// order, so an integer atomic max is a floating-point max).  One atomic per
+// wave; every lane of the wave must call it.
+__device__ __forceinline__ void wave_speed_max(unsigned long long *s, double v) {
+  v = wave_max(v);
+  if ((threadIdx.x & 63) == 0 && v > 0.)
+    atomicMax(s, (unsigned long long)__double_as_longlong(v));
+}
+
 // Occupancy target of a kernel: VH_OCC(n) caps it at n waves per SIMD (0: the
 // compiler's default).  The stencil gathers are latency bound; a lower target
 // lets the scheduler keep more loads in flight per wave, which pays more than

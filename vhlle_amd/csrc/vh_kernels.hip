@@ -58,18 +58,24 @@ __device__ __forceinline__ void face_state(const DevEos &eos, const double *__re
 // Hydro::hlle_flux (src/hdo.cpp:77-305) for the interface between cell c and
 // its +1 neighbour along axis D.  Writes F (7 x N, slot c) and flag[c]
 // (1 = flux present, 0 = the reference returned early without a flux).
+// Qs: the stage's state (Q in the predictor, Qh in the corrector); Qr: Q, from
+// which the reference re-evaluates both face states when the right state's
+// pressure is negative (src/hdo.cpp:134-140), in either stage.  Returns the
+// interface's signal speed max(-b_l, b_r) (the wave-speed estimates of the
+// HLLE flux, in units of dx/dtau, deta/dtau along eta), 0 without a flux.
 template <int D, int MODE>
-__global__ void __launch_bounds__(256) VH_OCC(VH_W_FACES)
-k_faces(Geom g, DevEos eos, const double *__restrict__ Qs, const double *__restrict__ m,
-        int m_full, const double *__restrict__ tp, double dt, double *__restrict__ F,
-        unsigned char *__restrict__ flag, int *__restrict__ status) {
-  const double tau = *tp;
-  const long c = xcd_block() * blockDim.x + threadIdx.x;
-  if (c >= g.N) return;
+__device__ __forceinline__ double face_flux(const Geom &g, const DevEos &eos,
+                                            const double *__restrict__ Qs,
+                                            const double *__restrict__ Qr,
+                                            const double *__restrict__ m, int m_full,
+                                            double tau, double dt, long c,
+                                            double *__restrict__ F,
+                                            unsigned char *__restrict__ flag,
+                                            int *__restrict__ status) {
   int ix, iy, iz;
   decompose(g, c, ix, iy, iz);
   const int i = coord<D>(ix, iy, iz), n = extent<D>(g);
-  if (i >= n - 1) return;
+  if (i >= n - 1) return 0.;
   const long s = stride<D>(g);
   const long cl = c, cr = c + s;
   const long cll = i > 0 ? c - s : c;          // clamped prev of the left cell
@@ -91,14 +97,21 @@ k_faces(Geom g, DevEos eos, const double *__restrict__ Qs, const double *__restr
     R.e = 0.;
     R.p = 0.;
   }
-  // (the reference's "pr < 0" re-evaluation, src/hdo.cpp:134-140, cannot
-  // trigger for p = e/3 once e >= 0 and is therefore absent here)
+  double speed = 0.;
   do {
     if (L.e > 1e10) {
       atomicOr(status, VH_ST_BIG_E);
       break;
     }
     if (L.e == 0. && R.e == 0.) break;
+    // negative pressure on the right (a table EoS whose spline undershoots
+    // below its clamped nodes): the reference evaluates both face states again
+    // from Q -- not Qh, even in the corrector -- and goes on with them, without
+    // clamping e again and with El, Er of the first states (src/hdo.cpp:134-140)
+    if (R.p < 0.) {
+      face_state<+1>(eos, Qr, N, cll, cl, cr, tau, L);
+      face_state<-1>(eos, Qr, N, cl, cr, crr, tau, R);
+    }
     if (!m_full && m[(D - 1) * N + cl] < 1. && m[(D - 1) * N + cr] < 1.) break;
     const double gl = 1. / sqrt(1 - L.vx * L.vx - L.vy * L.vy - L.vz * L.vz);
     const double gr = 1. / sqrt(1 - R.vx * R.vx - R.vy * R.vy - R.vz * R.vz);
@@ -177,8 +190,25 @@ k_faces(Geom g, DevEos eos, const double *__restrict__ Qs, const double *__restr
 #pragma unroll
     for (int k = 0; k < 7; k++) F[k * N + c] = f[k];
     ok = 1;
+    speed = smax(-bl, br);
   } while (0);
   flag[c] = ok;
+  return speed;
+}
+
+// One thread per interface (slot = left cell); the wave's largest signal
+// speed goes to *sig (the stage's slot of this axis).
+template <int D, int MODE>
+__global__ void __launch_bounds__(256) VH_OCC(VH_W_FACES)
+k_faces(Geom g, DevEos eos, const double *__restrict__ Qs, const double *__restrict__ Qr,
+        const double *__restrict__ m, int m_full, const double *__restrict__ tp, double dt,
+        double *__restrict__ F, unsigned char *__restrict__ flag, int *__restrict__ status,
+        unsigned long long *__restrict__ sig) {
+  const double tau = *tp;
+  const long c = xcd_block() * blockDim.x + threadIdx.x;
+  double speed = 0.;
+  if (c < g.N) speed = face_flux<D, MODE>(g, eos, Qs, Qr, m, m_full, tau, dt, c, F, flag, status);
+  wave_speed_max(sig, speed);
 }
 
 // ------------------------------------------------------ stage update
@@ -1100,8 +1130,21 @@ k_p2c_batch(const double *__restrict__ prim, long n, double *__restrict__ Q) {
 }
 
 // tau_{n+1} = tau_n + dt on the device (Hydro::performStep, src/hdo.cpp:1095)
-__global__ void k_advance_tau(double *__restrict__ tp, double dt) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) tp[0] = tp[0] + dt;
+__global__ void k_advance_tau(double *__restrict__ tp, double dt,
+                              unsigned long long *__restrict__ sig) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    tp[0] = tp[0] + dt;
+    for (int k = 0; k < 6; k++) {
+      const unsigned long long v = sig[k];
+      sig[6 + k] = v;
+      if (v > sig[12 + k % 3]) sig[12 + k % 3] = v;  // non-negative: integer order
+      sig[k] = 0ull;
+    }
+  }
+}
+
+__global__ void k_clear_u64(unsigned long long *__restrict__ p, int n) {
+  if (blockIdx.x == 0 && (int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
 __global__ void k_fill_int(int *__restrict__ p, int v) {
@@ -1112,13 +1155,14 @@ __global__ void k_fill_int(int *__restrict__ p, int v) {
 static inline unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
 
 void launch_faces(int D, int mode, const Geom &g, const DevEos &eos, const double *Qs,
-                  const double *m, int m_full, const double *tp, double dt, double *F,
-                  unsigned char *flag, int *status, hipStream_t st) {
+                  const double *Qr, const double *m, int m_full, const double *tp, double dt,
+                  double *F, unsigned char *flag, int *status, unsigned long long *sig,
+                  hipStream_t st) {
   const dim3 b(256), gr(blocks(g.N));
-#define VH_F(DD, MM)                                                                      \
-  if (D == DD && mode == MM)                                                              \
-    hipLaunchKernelGGL((k_faces<DD, MM>), gr, b, 0, st, g, eos, Qs, m, m_full, tp, dt, F, \
-                       flag, status);
+#define VH_F(DD, MM)                                                                          \
+  if (D == DD && mode == MM)                                                                  \
+    hipLaunchKernelGGL((k_faces<DD, MM>), gr, b, 0, st, g, eos, Qs, Qr, m, m_full, tp, dt, F, \
+                       flag, status, sig);
   VH_F(1, 0) VH_F(1, 1) VH_F(2, 0) VH_F(2, 1) VH_F(3, 0) VH_F(3, 1)
 #undef VH_F
 }
@@ -1190,8 +1234,12 @@ void launch_visc3(const Geom &g, const double *ph, const double *piH, const doub
                      m_full, tp, dt, Q);
 }
 
-void launch_advance_tau(double *tp, double dt, hipStream_t st) {
-  hipLaunchKernelGGL(k_advance_tau, dim3(1), dim3(64), 0, st, tp, dt);
+void launch_advance_tau(double *tp, double dt, unsigned long long *sig, hipStream_t st) {
+  hipLaunchKernelGGL(k_advance_tau, dim3(1), dim3(64), 0, st, tp, dt, sig);
+}
+
+void launch_clear_u64(unsigned long long *sig, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_clear_u64, dim3(1), dim3(64), 0, st, sig, n);
 }
 
 void launch_fill_int(int *p, int v, hipStream_t st) {

@@ -27,9 +27,12 @@ struct Grids {
 };
 
 // gv: per-cell con2prim guesses (used by VH_C2P_LEARNED only, may be null otherwise)
+// sig: the stage's signal-speed slots (x, y, eta; maxima of max(-b_l, b_r)
+// kept as bit patterns, see wave_speed_max)
 void launch_stage(int mode, int cm, int nc, const Grids &G, const DevEos &eos, const double *Q,
                   const double *Qh, const float *gv, const double *m, int m_full,
-                  const double *tp, double dt, double *out, int *status, hipStream_t st);
+                  const double *tp, double dt, double *out, int *status,
+                  unsigned long long *sig, hipStream_t st);
 // Stage kernel shapes (same results for both): STAGE_CELLS one thread per
 // cell evaluating its six interfaces; STAGE_ROWS waves of 64 flattened (x, y)
 // cells marching CZ cells in eta, every x- and eta-interface evaluated once.
@@ -43,8 +46,8 @@ Tile make_cells();
 std::vector<Tile> tile_candidates(const Grids &G);
 void launch_stage_tile(int mode, int cm, int nc, const Grids &G, const DevEos &eos, const double *Q,
                        const double *Qh, const float *gv, const double *m, int m_full,
-                       const double *tp, double dt, double *out, int *status, const Tile &tl,
-                       hipStream_t st);
+                       const double *tp, double dt, double *out, int *status,
+                       unsigned long long *sig, const Tile &tl, hipStream_t st);
 // con2prim of every cell (N cells, pitch N): ph = prims of Qh at tau - dt/2,
 // pt = (e, v) of Qh at tau
 void launch_vprim(int cm, int nc, long N, const DevEos &eos, const double *Qh, const float *gv,

@@ -352,10 +352,16 @@ __device__ __forceinline__ PrimF face_state(const DevEos &eos, const double *__r
 
 // HLLE interface flux in FP32 (Hydro::hlle_flux, src/hdo.cpp:77-305: same
 // wave speeds, vacuum rules and skips).  Returns false if the reference would
-// return without a flux.  pre = tauFactor * dta / dx.
-template <int D, int NC>
+// return without a flux.  pre = tauFactor * dta / dx.  A negative pressure of
+// the right state (a table EoS whose spline undershoots below its clamped
+// nodes) makes the reference evaluate both face states again from Q, in either
+// stage, and go on with them and with El, Er of the first states
+// (src/hdo.cpp:134-140): reeval(L, R) does that.  *sp receives the interface's
+// signal speed max(-b_l, b_r) when a flux is formed.
+template <int D, int NC, class RE>
 __device__ __forceinline__ bool hlle_f32(PrimF L, PrimF R, float itau, float itf, float pre,
-                                         bool mskip, float F[7], int *status) {
+                                         bool mskip, float F[7], int *status, float &sp,
+                                         RE &&reeval) {
   if (L.e < 0.f) L.e = L.p = 0.f;
   if (R.e < 0.f) R.e = R.p = 0.f;
   if (L.e > 1e10f) {
@@ -363,9 +369,11 @@ __device__ __forceinline__ bool hlle_f32(PrimF L, PrimF R, float itau, float itf
     return false;
   }
   if (L.e == 0.f && R.e == 0.f) return false;
+  const float El = (L.e + L.p) * L.g2, Er = (R.e + R.p) * R.g2;
+  if (R.p < 0.f) reeval(L, R);  // table EoS only: p = e/3 >= 0 here
   if (mskip) return false;
   const float g2l = L.g2, g2r = R.g2;
-  const float wl = (L.e + L.p) * g2l, wr = (R.e + R.p) * g2r;  // El, Er of the reference
+  const float wl = (L.e + L.p) * g2l, wr = (R.e + R.p) * g2r;
   const float gl = fsqrt(g2l), gr = fsqrt(g2r);
   float Ul[7], Ur[7];
   Ul[X_] = wl * L.vx;
@@ -406,7 +414,7 @@ __device__ __forceinline__ bool hlle_f32(PrimF L, PrimF R, float itau, float itf
     }
   }
   const float cs = 0.57735026918962576f;
-  const float sEl = fsqrt(wl), sEr = fsqrt(wr);
+  const float sEl = fsqrt(El), sEr = fsqrt(Er);
   // both sides below the FP32 range (e ~ 1e-40 in the FP64 state, flushed
   // here): vacuum to FP32 precision, no flux (instead of 0 * inf)
   if (!(sEl + sEr > 0.f)) return false;
@@ -435,18 +443,21 @@ __device__ __forceinline__ bool hlle_f32(PrimF L, PrimF R, float itau, float itf
     atomicOr(status, VH_ST_NAN_FLUX);
     return false;
   }
+  sp = fmax_(sp, fmax_(-bl, br));
   return true;
 }
 
 // the two interfaces of cell c along axis D, added into acc in the order
-// +F(i-1/2), -F(i+1/2)
+// +F(i-1/2), -F(i+1/2).  Qs: the stage's state; Qr: Q (the states of the
+// negative-pressure re-evaluation); sp: the largest signal speed so far.
 template <int D, int MODE, int CM, int NC>
 __device__ __forceinline__ void axis_fluxes(const Geom &g, const DevEos &eos,
                                             const double *__restrict__ Qs,
+                                            const double *__restrict__ Qr,
                                             const float *__restrict__ gv,
                                             const double *__restrict__ m, int m_full, int i,
                                             int n, long c, long s, double tau, double dt,
-                                            double acc[7], int *status) {
+                                            double acc[7], int *status, float &sp) {
   if (n < 2) return;
   const long N = g.P;
   const double itau = drcp(tau);  // tau > 0 and finite
@@ -463,7 +474,11 @@ __device__ __forceinline__ void axis_fluxes(const Geom &g, const DevEos &eos,
     const long cmm = i > 1 ? cm - s : cm;
     const PrimF Rm = face_state<+1, CM, NC>(eos, Qs, gv, N, cmm, cm, c, itau);
     const bool mk = !m_full && m[(D - 1) * N + cm] < 1. && m[(D - 1) * N + c] < 1.;
-    if (hlle_f32<D, NC>(Rm, L0, itauf, itf, pre, mk, F, status))
+    auto re = [&](PrimF &a, PrimF &b) {
+      a = face_state<+1, CM, NC>(eos, Qr, gv, N, cmm, cm, c, itau);
+      b = face_state<-1, CM, NC>(eos, Qr, gv, N, cm, c, cp, itau);
+    };
+    if (hlle_f32<D, NC>(Rm, L0, itauf, itf, pre, mk, F, status, sp, re))
 #pragma unroll
       for (int k = 0; k < NC; k++) acc[k] += (double)F[k];
   }
@@ -471,7 +486,11 @@ __device__ __forceinline__ void axis_fluxes(const Geom &g, const DevEos &eos,
     const long cpp = i + 2 < n ? cp + s : cp;
     const PrimF Lp = face_state<-1, CM, NC>(eos, Qs, gv, N, c, cp, cpp, itau);
     const bool mk = !m_full && m[(D - 1) * N + c] < 1. && m[(D - 1) * N + cp] < 1.;
-    if (hlle_f32<D, NC>(R0, Lp, itauf, itf, pre, mk, F, status))
+    auto re = [&](PrimF &a, PrimF &b) {
+      a = face_state<+1, CM, NC>(eos, Qr, gv, N, cm, c, cp, itau);
+      b = face_state<-1, CM, NC>(eos, Qr, gv, N, c, cp, cpp, itau);
+    };
+    if (hlle_f32<D, NC>(R0, Lp, itauf, itf, pre, mk, F, status, sp, re))
 #pragma unroll
       for (int k = 0; k < NC; k++) acc[k] -= (double)F[k];
   }
@@ -522,27 +541,35 @@ __device__ __forceinline__ void finish_cell(const DevEos &eos, const double *__r
 // MODE 0 (predictor): reads Q, writes Qh = Q + flux.
 // MODE 1 (corrector): reads Q and Qh, writes Qnew = Q + flux unless
 // Q0 + flux0 < 0 (src/cll.cpp:70-74).
+// sig: the stage's three signal-speed slots (x, y, eta), one atomic per wave.
 template <int MODE, int CM, int NC>
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_MX_STAGE)
 k_stage(Grids G, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
         const float *__restrict__ gv, const double *__restrict__ m, int m_full,
         const double *__restrict__ tp, double dt, double *__restrict__ out,
-        int *__restrict__ status) {
+        int *__restrict__ status, unsigned long long *__restrict__ sig) {
   const double tau = *tp;
   Geom g;
   long c;
   int ix, iy, iz;
-  if (!work_cell(G, g, c, ix, iy, iz)) return;
-  const long N = g.P;
-  const double *Qs = MODE == 0 ? Q : Qh;
-  double acc[7];
+  float sx = 0.f, sy = 0.f, sz = 0.f;
+  if (work_cell(G, g, c, ix, iy, iz)) {
+    const long N = g.P;
+    const double *Qs = MODE == 0 ? Q : Qh;
+    double acc[7];
 #pragma unroll
-  for (int k = 0; k < 7; k++) acc[k] = 0.;
-  axis_fluxes<1, MODE, CM, NC>(g, eos, Qs, gv, m, m_full, ix, g.nx, c, 1, tau, dt, acc, status);
-  axis_fluxes<2, MODE, CM, NC>(g, eos, Qs, gv, m, m_full, iy, g.ny, c, g.nx, tau, dt, acc, status);
-  axis_fluxes<3, MODE, CM, NC>(g, eos, Qs, gv, m, m_full, iz, g.nz, c, (long)g.nx * g.ny, tau, dt,
-                               acc, status);
-  finish_cell<MODE, CM, NC>(eos, Q, Qs, gv, N, c, tau, dt, acc, out);
+    for (int k = 0; k < 7; k++) acc[k] = 0.;
+    axis_fluxes<1, MODE, CM, NC>(g, eos, Qs, Q, gv, m, m_full, ix, g.nx, c, 1, tau, dt, acc,
+                                 status, sx);
+    axis_fluxes<2, MODE, CM, NC>(g, eos, Qs, Q, gv, m, m_full, iy, g.ny, c, g.nx, tau, dt, acc,
+                                 status, sy);
+    axis_fluxes<3, MODE, CM, NC>(g, eos, Qs, Q, gv, m, m_full, iz, g.nz, c, (long)g.nx * g.ny,
+                                 tau, dt, acc, status, sz);
+    finish_cell<MODE, CM, NC>(eos, Q, Qs, gv, N, c, tau, dt, acc, out);
+  }
+  wave_speed_max(sig, (double)sx);
+  wave_speed_max(sig + 1, (double)sy);
+  wave_speed_max(sig + 2, (double)sz);
 }
 
 // Interface flux between cells a (left) and b (right) along axis D: MUSCL
@@ -550,14 +577,19 @@ k_stage(Grids G, DevEos eos, const double *__restrict__ Q, const double *__restr
 // Same states, mask and flux as the interfaces of axis_fluxes.
 template <int D, int CM, int NC>
 __device__ __forceinline__ bool iface(const DevEos &eos, const double *__restrict__ Qs,
+                                      const double *__restrict__ Qr,
                                       const float *__restrict__ gv,
                                       const double *__restrict__ m, int m_full, long N, long amm,
                                       long a, long b, long bp, double itau, float itauf,
-                                      float itf, float pre, float F[7], int *status) {
+                                      float itf, float pre, float F[7], int *status, float &sp) {
   const PrimF L = face_state<+1, CM, NC>(eos, Qs, gv, N, amm, a, b, itau);
   const PrimF R = face_state<-1, CM, NC>(eos, Qs, gv, N, a, b, bp, itau);
   const bool mk = !m_full && m[(D - 1) * N + a] < 1. && m[(D - 1) * N + b] < 1.;
-  return hlle_f32<D, NC>(L, R, itauf, itf, pre, mk, F, status);
+  auto re = [&](PrimF &l, PrimF &r) {
+    l = face_state<+1, CM, NC>(eos, Qr, gv, N, amm, a, b, itau);
+    r = face_state<-1, CM, NC>(eos, Qr, gv, N, a, b, bp, itau);
+  };
+  return hlle_f32<D, NC>(L, R, itauf, itf, pre, mk, F, status, sp, re);
 }
 
 // Row stage: every x- and eta-interface is evaluated once, without barriers.
@@ -573,7 +605,7 @@ __global__ void __launch_bounds__(256) VH_OCC(VH_W_MX_ROWS)
 k_stage_rows(Grids G, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
              const float *__restrict__ gv, const double *__restrict__ m, int m_full,
              const double *__restrict__ tp, double dt, double *__restrict__ out,
-             int *__restrict__ status, int CZ) {
+             int *__restrict__ status, unsigned long long *__restrict__ sig, int CZ) {
   const int lane = threadIdx.x & 63;
   const long w = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const long sgl = w % G.nseg, zc = w / G.nseg;  // row segment over all grids, eta chunk
@@ -600,6 +632,7 @@ k_stage_rows(Grids G, DevEos eos, const double *__restrict__ Q, const double *__
   float fzl[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   float fzu[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   bool hzl = false;
+  float spx = 0.f, spy = 0.f, spz = 0.f;
   for (int iz = z0; iz < z1; iz++) {
     const long c = cx ? g.base + f + iz * sz : 0;
     float fx[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -607,26 +640,28 @@ k_stage_rows(Grids G, DevEos eos, const double *__restrict__ Q, const double *__
     float fyu[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     bool hx = false, hyl = false, hyu = false, hzu = false;
     if (cx && ix >= 1)
-      hx = iface<1, CM, NC>(eos, Qs, gv, m, m_full, N, ix >= 2 ? c - 2 : c - 1, c - 1, c,
-                        ix + 1 < g.nx ? c + 1 : c, itau, itauf, itf, prx, fx, status);
+      hx = iface<1, CM, NC>(eos, Qs, Q, gv, m, m_full, N, ix >= 2 ? c - 2 : c - 1, c - 1, c,
+                            ix + 1 < g.nx ? c + 1 : c, itau, itauf, itf, prx, fx, status, spx);
     float fxu[7];
 #pragma unroll
     for (int k = 0; k < NC; k++) fxu[k] = __shfl_down(fx[k], 1, 64);
     const bool hxu = __shfl_down(hx ? 1 : 0, 1, 64) != 0 && ix + 1 < g.nx;
     if (cx && iy >= 1)
-      hyl = iface<2, CM, NC>(eos, Qs, gv, m, m_full, N, iy >= 2 ? c - 2 * sy : c - sy, c - sy, c,
-                         iy + 1 < g.ny ? c + sy : c, itau, itauf, itf, pry, fyl, status);
+      hyl = iface<2, CM, NC>(eos, Qs, Q, gv, m, m_full, N, iy >= 2 ? c - 2 * sy : c - sy, c - sy,
+                             c, iy + 1 < g.ny ? c + sy : c, itau, itauf, itf, pry, fyl, status,
+                             spy);
     if (cx && iy + 1 < g.ny)
-      hyu = iface<2, CM, NC>(eos, Qs, gv, m, m_full, N, iy >= 1 ? c - sy : c, c, c + sy,
-                         iy + 2 < g.ny ? c + 2 * sy : c + sy, itau, itauf, itf, pry, fyu,
-                         status);
+      hyu = iface<2, CM, NC>(eos, Qs, Q, gv, m, m_full, N, iy >= 1 ? c - sy : c, c, c + sy,
+                             iy + 2 < g.ny ? c + 2 * sy : c + sy, itau, itauf, itf, pry, fyu,
+                             status, spy);
     if (iz == z0 && cx && iz >= 1)
-      hzl = iface<3, CM, NC>(eos, Qs, gv, m, m_full, N, iz >= 2 ? c - 2 * sz : c - sz, c - sz, c,
-                         iz + 1 < g.nz ? c + sz : c, itau, itauf, itf, prz, fzl, status);
+      hzl = iface<3, CM, NC>(eos, Qs, Q, gv, m, m_full, N, iz >= 2 ? c - 2 * sz : c - sz, c - sz,
+                             c, iz + 1 < g.nz ? c + sz : c, itau, itauf, itf, prz, fzl, status,
+                             spz);
     if (cx && iz + 1 < g.nz)
-      hzu = iface<3, CM, NC>(eos, Qs, gv, m, m_full, N, iz >= 1 ? c - sz : c, c, c + sz,
-                         iz + 2 < g.nz ? c + 2 * sz : c + sz, itau, itauf, itf, prz, fzu,
-                         status);
+      hzu = iface<3, CM, NC>(eos, Qs, Q, gv, m, m_full, N, iz >= 1 ? c - sz : c, c, c + sz,
+                             iz + 2 < g.nz ? c + 2 * sz : c + sz, itau, itauf, itf, prz, fzu,
+                             status, spz);
     if (upd) {
       double acc[7];
 #pragma unroll
@@ -655,6 +690,9 @@ k_stage_rows(Grids G, DevEos eos, const double *__restrict__ Q, const double *__
     for (int k = 0; k < NC; k++) fzl[k] = fzu[k];
     hzl = hzu;
   }
+  wave_speed_max(sig, (double)spx);
+  wave_speed_max(sig + 1, (double)spy);
+  wave_speed_max(sig + 2, (double)spz);
 }
 
 // ---------------------------------------------------------------- viscous
@@ -901,7 +939,17 @@ k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
   trc.getTauf(eos, ef, nbf, T, s, p, etaS, zetaS, taupi, tauPi);
   const float hdt = (float)(0.5 * dt), fdt = (float)dt;
   float dpp = 4.f / 3.f, tpp = 10.f / 7.f, lpP = 6.f / 5.f;
-  float phi7 = p > 0.f ? (9.f / 70.f) / (p * taupi) : 0.f;
+  // phi7 = 9/(70 p) / tau_pi for every p != 0, negative p included; 0 for
+  // p = 0, where the reference's quotient is inf (src/hdo.cpp:744-746).  A
+  // table EoS whose spline rings next to a clamped region gives |p| down to
+  // 1e-40: there the FP64 reference forms cross terms phi7 pi pi of 1e20 and
+  // more, which its limiter then scales back to 0.1 T^{mu nu}; in FP32 the
+  // full step's phi7 pi piH would overflow to inf (and the limiter's inf * 0
+  // to NaN).  |phi7| is capped at 1e10, far above the 9/(70 p tau_pi) < 1e4
+  // of any p > 1e-3 GeV/fm^3 and tau_pi > 0.01 fm, so that such cells stay
+  // finite and keep the sign and the dominance of the term.
+  float phi7 = 0.f;
+  if (p != 0.f) phi7 = fminf(fmaxf((9.f / 70.f) / (p * taupi), -1e10f), 1e10f);
   if (taupi < hdt) dpp = tpp = lpP = phi7 = 0.f;
   float dPP = 2.f / 3.f, lPp = 1.6f * (1.f / 3.f - eos.cs2f(ef));
   if (tauPi < hdt) dPP = lPp = 0.f;
@@ -1305,15 +1353,16 @@ static inline unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
 
 void launch_stage(int mode, int cm, int nc, const Grids &G, const DevEos &eos, const double *Q,
                   const double *Qh, const float *gv, const double *m, int m_full,
-                  const double *tp, double dt, double *out, int *status, hipStream_t st) {
+                  const double *tp, double dt, double *out, int *status,
+                  unsigned long long *sig, hipStream_t st) {
   const dim3 gr((unsigned)G.nblk), b(256);
 #define VH_S(MM, CC)                                                                            \
   if (mode == MM && cm == CC && nc == 7)                                                        \
     hipLaunchKernelGGL((k_stage<MM, CC, 7>), gr, b, 0, st, G, eos, Q, Qh, gv, m, m_full, tp, dt, \
-                       out, status);                                                            \
+                       out, status, sig);                                                       \
   if (mode == MM && cm == CC && nc == 4)                                                        \
     hipLaunchKernelGGL((k_stage<MM, CC, 4>), gr, b, 0, st, G, eos, Q, Qh, gv, m, m_full, tp, dt, \
-                       out, status);
+                       out, status, sig);
   VH_S(0, VH_C2P_REFERENCE) VH_S(1, VH_C2P_REFERENCE) VH_S(0, VH_C2P_CLOSED)
   VH_S(1, VH_C2P_CLOSED) VH_S(0, VH_C2P_LEARNED) VH_S(1, VH_C2P_LEARNED)
   VH_S(0, VH_C2P_NEWTON) VH_S(1, VH_C2P_NEWTON)
@@ -1353,10 +1402,10 @@ std::vector<Tile> tile_candidates(const Grids &G) {
 
 void launch_stage_tile(int mode, int cm, int nc, const Grids &G, const DevEos &eos, const double *Q,
                        const double *Qh, const float *gv, const double *m, int m_full,
-                       const double *tp, double dt, double *out, int *status, const Tile &tl,
-                       hipStream_t st) {
+                       const double *tp, double dt, double *out, int *status,
+                       unsigned long long *sig, const Tile &tl, hipStream_t st) {
   if (tl.kind != STAGE_ROWS) {
-    launch_stage(mode, cm, nc, G, eos, Q, Qh, gv, m, m_full, tp, dt, out, status, st);
+    launch_stage(mode, cm, nc, G, eos, Q, Qh, gv, m, m_full, tp, dt, out, status, sig, st);
     return;
   }
   const long ntz = (G.maxnz + tl.CZ - 1) / tl.CZ;
@@ -1365,10 +1414,10 @@ void launch_stage_tile(int mode, int cm, int nc, const Grids &G, const DevEos &e
 #define VH_S(MM, CC)                                                                          \
   if (mode == MM && cm == CC && nc == 7)                                                      \
     hipLaunchKernelGGL((k_stage_rows<MM, CC, 7>), gr, b, 0, st, G, eos, Q, Qh, gv, m, m_full, \
-                       tp, dt, out, status, tl.CZ);                                           \
+                       tp, dt, out, status, sig, tl.CZ);                                      \
   if (mode == MM && cm == CC && nc == 4)                                                      \
     hipLaunchKernelGGL((k_stage_rows<MM, CC, 4>), gr, b, 0, st, G, eos, Q, Qh, gv, m, m_full, \
-                       tp, dt, out, status, tl.CZ);
+                       tp, dt, out, status, sig, tl.CZ);
   VH_S(0, VH_C2P_REFERENCE) VH_S(1, VH_C2P_REFERENCE) VH_S(0, VH_C2P_CLOSED)
   VH_S(1, VH_C2P_CLOSED) VH_S(0, VH_C2P_LEARNED) VH_S(1, VH_C2P_LEARNED)
   VH_S(0, VH_C2P_NEWTON) VH_S(1, VH_C2P_NEWTON)
