@@ -172,6 +172,15 @@ int vh_fluid_stats(vh_ctx_t *ctx, double tau, double e_crit, int central, double
  * (src/fld.cpp:425-504). */
 enum { VH_CELL_COLS = 21 };
 int vh_cells(vh_ctx_t *ctx, double tau, const long *cells, long n, double *out);
+/* vh_cells for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): no
+ * counterpart in the reference.  Same records as vh_cells, but the call
+ * allocates nothing: the cell list and the records go through one fixed
+ * device array per GPU (1.4 MB, held until the process ends), so at most
+ * VH_CELLS_NOALLOC_MAX cells per call (more: VH_ERR_ARG).  Calls that
+ * overlap on one GPU share that array and race; the caller serialises them. */
+enum { VH_CELLS_NOALLOC_MAX = 8192 };
+int vh_cells_noalloc(vh_ctx_t *ctx, double tau, const long *cells, long n, double *out);
 
 /* Hydro::performStep (src/hdo.cpp:1016-1134): one predictor-corrector step
  * (+ Israel-Stewart step when viscous) from *tau with timestep dt; *tau is

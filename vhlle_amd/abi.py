@@ -20,6 +20,7 @@ START_MLP, START_HALF = 0, 1
 FO_QPREV, FO_SNAPSHOT = 0, 1
 BETA_COLS = 33  # VH_BETA_COLS: columns of a beta.dat record
 STAGE_AUTO, STAGE_CELLS, STAGE_ROWS = 0, 1, 2
+CELLS_NOALLOC_MAX = 8192
 MLP_NPARAM = 4674
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -195,6 +196,8 @@ def lib():
                                           _dp]),
         "vh_cells": (ctypes.c_int, [vp, ctypes.c_double, ctypes.POINTER(ctypes.c_long),
                                     ctypes.c_long, _dp]),
+        "vh_cells_noalloc": (ctypes.c_int, [vp, ctypes.c_double, ctypes.POINTER(ctypes.c_long),
+                                            ctypes.c_long, _dp]),
         "vh_set_vorticity": (ctypes.c_int, [vp, ctypes.c_int]),
         "vh_get_dbeta": (ctypes.c_int, [vp, _dp]),
         "vh_freezeout_beta": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double,
@@ -221,7 +224,8 @@ def exported_symbols():
             "vh_set_c2p_model", "vh_mlp_harvest", "vh_set_stage", "vh_get_stage", "vh_create_batch", "vh_hotspot_ic",
             "vh_copy_bandwidth", "vh_fo_snapshot", "vh_freezeout", "vh_cornelius",
             "vh_eos_table", "vh_set_vorticity", "vh_get_dbeta", "vh_freezeout_beta",
-            "vh_corona", "vh_prim_fields", "vh_fluid_stats", "vh_cells", "vh_signal_speed"]
+            "vh_corona", "vh_prim_fields", "vh_fluid_stats", "vh_cells", "vh_signal_speed",
+            "vh_cells_noalloc"]
 
 
 def _p(a):
@@ -516,6 +520,15 @@ class Context:
         check(lib().vh_cells(self.h, tau, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
                              len(idx), _p(out)), "vh_cells")
         return out
+
+    def cells_noalloc(self, tau, idx):
+        """cells() through vh_cells_noalloc (no device allocation; at most
+        CELLS_NOALLOC_MAX cells); returns (status code, records)"""
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        out = np.zeros((len(idx), 21))
+        rc = lib().vh_cells_noalloc(self.h, tau, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
+                                    len(idx), _p(out))
+        return rc, out
 
     def set_vorticity(self, on=True):
         check(lib().vh_set_vorticity(self.h, int(on)), "vh_set_vorticity")

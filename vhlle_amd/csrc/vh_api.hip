@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -29,6 +30,35 @@
   } while (0)
 
 namespace {
+
+// Device scratch of vh_cells_noalloc: the cell list (VH_CELLS_NOALLOC_MAX
+// indices) and the records (VH_CELL_COLS doubles per cell), 1.4 MB held on
+// each GPU the library runs on until the process ends.  Host code never uses
+// the name `scratch` as an address (in host code it is a stand-in, not the
+// device array): it uses the address hipGetSymbolAddress returns for each
+// device, kept in scratch_ptr.
+constexpr long kScratchDoubles = (long)VH_CELLS_NOALLOC_MAX * (1 + VH_CELL_COLS);
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+double *scratch_ptr[kMaxDevices] = {};
+std::mutex scratch_mu;
+
+// Looks up this device's scratch address once (the device current); called
+// at context creation, so that vh_cells_noalloc itself allocates nothing.
+int scratch_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return VH_ERR_ARG;
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (scratch_ptr[dev]) return VH_OK;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    std::fprintf(stderr, "vhlle_gpu: hipGetSymbolAddress(scratch) failed on device %d: %s\n",
+                 dev, hipGetErrorString(e));
+    return VH_ERR_HIP;
+  }
+  scratch_ptr[dev] = static_cast<double *>(ptr);
+  return VH_OK;
+}
 
 struct KTime {
   int launches = 0;
@@ -481,6 +511,11 @@ int create_impl(const vh_grid_t *grids, int ngrid, const vh_eos_t *eos, const vh
   // TransportCoeff::isViscous (src/trancoeff.h:39-44)
   c->viscous = trc->etaS0 > 0. || trc->zetaS0 > 0. || trc->etaSparam > 0 || trc->zetaSparam > 0;
   if (hipGetDevice(&c->dev) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess) {
+    delete c;
+    return VH_ERR_HIP;
+  }
+  if (scratch_setup(c->dev) != VH_OK) {
+    (void)hipStreamDestroy(c->st);
     delete c;
     return VH_ERR_HIP;
   }
@@ -1087,6 +1122,36 @@ int vh_cells(vh_ctx_t *c, double tau, const long *cells, long n, double *out) {
                    tau, c->cidx, n, c->cbuf, c->st);
   VH_HIP(hipGetLastError());
   VH_HIP(hipMemcpyAsync(out, c->cbuf, n * VH_CELL_COLS * sizeof(double), hipMemcpyDeviceToHost,
+                        c->st));
+  VH_HIP(hipStreamSynchronize(c->st));
+  return VH_OK;
+}
+
+int vh_cells_noalloc(vh_ctx_t *c, double tau, const long *cells, long n, double *out) {
+  if (!c || n < 0 || (n && (!cells || !out)) || !(tau > 0.)) return VH_ERR_ARG;
+  if (n > VH_CELLS_NOALLOC_MAX) {
+    std::fprintf(stderr,
+                 "vhlle_gpu: vh_cells_noalloc: %ld cells requested, at most %d per call "
+                 "(use vh_cells for more)\n",
+                 n, VH_CELLS_NOALLOC_MAX);
+    return VH_ERR_ARG;
+  }
+  const long N = vh_ncell(c);
+  for (long i = 0; i < n; i++)
+    if (cells[i] < 0 || cells[i] >= N) return VH_ERR_ARG;
+  if (n == 0) return VH_OK;
+  double *base = (c->dev >= 0 && c->dev < kMaxDevices) ? scratch_ptr[c->dev] : nullptr;
+  if (!base) return VH_ERR_HIP;  // set up at vh_create
+  // [0, MAX): the cell list (8-byte indices); then the records.  Every index
+  // and record read in this call is written in it first, so nothing needs
+  // clearing; calls that overlap on one GPU share the array and race.
+  long *idx = reinterpret_cast<long *>(base);
+  double *rec = base + VH_CELLS_NOALLOC_MAX;
+  VH_HIP(hipMemcpyAsync(idx, cells, n * sizeof(long), hipMemcpyHostToDevice, c->st));
+  vh::launch_cells(N, c->eos, c->Q[c->cur], c->pi, c->Pi, c->precision == VH_PREC_MIXED, c->vcc,
+                   tau, idx, n, rec, c->st);
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipMemcpyAsync(out, rec, n * VH_CELL_COLS * sizeof(double), hipMemcpyDeviceToHost,
                         c->st));
   VH_HIP(hipStreamSynchronize(c->st));
   return VH_OK;
