@@ -187,9 +187,9 @@ int vh_cells_noalloc(vh_ctx_t *ctx, double tau, const long *cells, long n, doubl
  * advanced by dt.  *status receives VH_ST_* bits (0 = the reference would
  * have continued). */
 int vh_step(vh_ctx_t *ctx, double *tau, double dt, int *status);
-/* nsteps of vh_step without host synchronisation in between (the step
- * sequence is replayed as a HIP graph); stops early only on the host side
- * after all steps if *status != 0. */
+/* nsteps of vh_step without host synchronisation in between: the step
+ * sequence is replayed from HIP graphs, one graph launch per 8 steps (and
+ * one per step for the rest); *status after all steps. */
 int vh_run(vh_ctx_t *ctx, double *tau, double dt, int nsteps, int *status);
 /* Hydro::setDtau validity test (src/hdo.cpp:67-75): VH_OK or VH_ERR_DTAU. */
 int vh_check_dtau(const vh_ctx_t *ctx, double dt);
@@ -215,7 +215,10 @@ int vh_set_c2p_mode(vh_ctx_t *ctx, int mode);
  * plane marching CZ cells in eta, every x- and eta-interface evaluated once.
  * Both give identical results.  VH_STAGE_AUTO (default): an autotuner times
  * the candidates (cells; rows with CZ = 1..32) on the first vh_step/vh_run
- * (and after vh_set_c2p_mode) and keeps the fastest.  vh_get_stage reports
+ * (and after vh_set_c2p_mode) and keeps the fastest; 32 steps later it times
+ * that shape against the best of the other kind again, on the developed
+ * flow, and keeps the faster (within a vh_run if one is going on: the
+ * results do not depend on the shape).  vh_get_stage reports
  * the kernel in use (VH_STAGE_AUTO: not tuned yet), CZ and the tuned time of
  * a predictor and a corrector stage in ms.  Mixed precision only (VH_ERR_ARG
  * otherwise). */

@@ -114,3 +114,37 @@ def test_batch_workload_slice(gpu):
     b.close()
     print(f"12 events, npart {npart.tolist()}, e_max {e.max():.1f}, status {st}", flush=True)
     assert st == 0 and np.all(np.isfinite(s["Q"])) and np.all(np.isfinite(s["pi"]))
+
+
+def test_long_runs_multistep_graphs_and_retuning(gpu):
+    """Runs long enough for the 8-step graphs and the second tuning of the
+    stage shape (after 32 steps): a batch run of 45 steps in one vh_run, in
+    uneven chunks, and a forced cell-kernel run all give the same states, bit
+    for bit."""
+    trc = setups.batch_trc()
+    e, _ = abi.hotspot_ic(GRIDS, seed=5, amp=5.2, eta_plateau=0.2, eta_sigma=0.3)
+    z = np.zeros_like(e)
+    out = []
+    for how in ("one_run", "chunks", "cells"):
+        b = abi.Context(grids=GRIDS, trc=trc, precision=abi.PREC_MIXED)
+        b.set_c2p(abi.C2P_CLOSED)
+        if how == "cells":
+            b.set_stage(abi.STAGE_CELLS)
+        b.set_prim(0.5, e, z, z, z)
+        if how == "chunks":
+            tau = 0.5
+            for n in (1, 9, 16, 3, 16):
+                st, tau = b.run(tau, 0.02, n)
+                assert st == 0
+        else:
+            st, tau = b.run(0.5, 0.02, 45)
+            assert st == 0
+        k, cz, ms = b.get_stage()
+        print(f"{how}: stage kernel {k} CZ {cz} ({ms:.4f} ms per stage pair), tau {tau}",
+              flush=True)
+        assert k in (abi.STAGE_CELLS, abi.STAGE_ROWS)
+        out.append(b.download())
+        b.close()
+    for f in ("Q", "Qh", "pi", "Pi"):
+        assert np.array_equal(out[0][f], out[1][f]), f
+        assert np.array_equal(out[0][f], out[2][f]), f

@@ -187,12 +187,12 @@ def test_graph_run_equals_steps(gpu, oracle):
     orc.set_prim(0.7, e, vx, vy, vz, nb=nb)
     ta = tb = to = 0.7
     dt = 0.02
-    for rep in range(3):
-        st, ta = a.run(ta, dt, 5)
+    for rep, n in enumerate((5, 17, 3)):  # 17: two 8-step graphs and a single step
+        st, ta = a.run(ta, dt, n)
         assert st == 0
-        for _ in range(5):
+        for _ in range(n):
             st, tb = b.step(tb, dt)
-        rc, to = orc.run(to, dt, 5)
+        rc, to = orc.run(to, dt, n)
         da, db = a.download(), b.download()
         for f in FIELDS:
             assert np.array_equal(da[f], db[f]), (rep, f)

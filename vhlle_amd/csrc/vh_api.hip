@@ -3,6 +3,7 @@
 // kernel launches on one HIP stream, and its replay as HIP graphs.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -121,14 +122,20 @@ struct vh_ctx {
   // some n_b, n_q or n_s nonzero: the mixed kernels carry 7 conserved
   // components, else 4 (charges that are zero everywhere stay zero)
   bool charged = true;
-  // stage kernel shape (mixed precision): kind 0 until the autotuner ran
-  vh::mx::Tile tile = {0, 0};
+  // stage kernel shape (mixed precision): kind 0 until the autotuner ran.
+  // tune_stage 1: tuned on the state of the first run (tile_alt: the best
+  // shape of the other kind); 2: tuned again on the developed flow
+  vh::mx::Tile tile = {0, 0}, tile_alt = {0, 0};
   bool tile_forced = false;
   double tile_ms = 0.;
+  int tune_stage = 0;
+  long steps_since_tune = 0;
   int cur = 0;
   int m_full = 0;
-  // graph cache: one executable per parity of cur
+  // graph cache: one executable per parity of cur for a single step, and
+  // one per parity for kGraphSteps steps in a row
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  hipGraphExec_t gmulti[2] = {nullptr, nullptr};
   double g_dt = 0.;
   int g_mfull = -1;
   // timing
@@ -324,13 +331,22 @@ int enqueue_step(vh_ctx *c, double dt) {
 }
 
 void drop_graphs(vh_ctx *c) {
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < 2; i++) {
     if (c->gexec[i]) {
       (void)hipGraphExecDestroy(c->gexec[i]);
       c->gexec[i] = nullptr;
     }
+    if (c->gmulti[i]) {
+      (void)hipGraphExecDestroy(c->gmulti[i]);
+      c->gmulti[i] = nullptr;
+    }
+  }
   c->g_mfull = -1;
 }
+
+// Steps per multi-step graph: a run replays one graph launch per kGraphSteps
+// steps (even, so that the graph ends on the parity of Q it starts from).
+constexpr int kGraphSteps = 8;
 
 bool any_nonzero(const double *a, long n) {
   for (long i = 0; i < n; i++)
@@ -347,17 +363,21 @@ void set_charged(vh_ctx *c, bool ch) {
 }
 
 int build_graphs(vh_ctx *c, double dt) {
-  if (c->gexec[0] && c->gexec[1] && c->g_dt == dt && c->g_mfull == c->m_full) return VH_OK;
+  if (c->gexec[0] && c->gexec[1] && c->gmulti[0] && c->gmulti[1] && c->g_dt == dt &&
+      c->g_mfull == c->m_full)
+    return VH_OK;
   drop_graphs(c);
   const int cur0 = c->cur;
-  for (int k = 0; k < 2; k++) {
-    const int par = cur0 ^ k;  // graph for steps that start from Q[par]
+  for (int k = 0; k < 4; k++) {
+    const int par = cur0 ^ (k & 1);  // graphs for steps that start from Q[par]
+    const int nst = k < 2 ? 1 : kGraphSteps;
     c->cur = par;
     hipGraph_t graph;
     VH_HIP(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
     const bool t = c->timing;
     c->timing = false;
-    int rc = enqueue_step(c, dt);
+    int rc = VH_OK;
+    for (int s = 0; s < nst && rc == VH_OK; s++) rc = enqueue_step(c, dt);
     c->timing = t;
     VH_HIP(hipStreamEndCapture(c->st, &graph));
     if (rc != VH_OK) {
@@ -365,7 +385,8 @@ int build_graphs(vh_ctx *c, double dt) {
       c->cur = cur0;
       return rc;
     }
-    VH_HIP(hipGraphInstantiate(&c->gexec[par], graph, nullptr, nullptr, 0));
+    VH_HIP(hipGraphInstantiate(k < 2 ? &c->gexec[par] : &c->gmulti[par], graph, nullptr,
+                               nullptr, 0));
     VH_HIP(hipGraphDestroy(graph));
   }
   c->cur = cur0;
@@ -820,49 +841,63 @@ int vh_set_c2p_model(vh_ctx_t *c, const float *params) {
   return VH_OK;
 }
 
-// Autotuner of the stage kernel: times the predictor stage of every candidate
-// shape (vh::mx::tile_candidates) on the current state, writing into scratch
-// memory, and keeps the fastest.  Every shape gives the same result.
-static int ensure_tile(vh_ctx_t *c, double tau, double dt) {
-  if (c->precision != VH_PREC_MIXED || c->tile.kind || c->tile_forced) return VH_OK;
-  const std::vector<vh::mx::Tile> cand = vh::mx::tile_candidates(c->G);
-  const vh::Geom &g = c->g;
+// Time of `reps` predictor + corrector pairs of stage shape t (ms per pair)
+// on the current state at the device tau, written into scratch memory (the
+// FP64 path's flux buffers, a scratch status word and signal-speed slots):
+// the context's state is left as it was.
+static int time_tile(vh_ctx_t *c, const vh::mx::Tile &t, double dt, int reps, float &ms) {
   hipStream_t st = c->st;
   double *so = c->F[1], *sc = c->F[2];               // unused by the mixed path
   int *ss = reinterpret_cast<int *>(c->fl[0]);        // scratch status word
-  VH_HIP(hipMemcpyAsync(c->tp, &tau, sizeof(double), hipMemcpyHostToDevice, st));
   const int nc = c->charged ? 7 : 4;
-  if (c->c2p_mode == VH_C2P_LEARNED)
-    vh::ml::launch_guess(c->Q[c->cur], g.N, nc, c->tp, 0., c->mlp, c->gv, st);
+  auto go = [&] {
+    vh::mx::launch_stage_tile(0, c->c2p_mode, nc, c->G, c->eos, c->Q[c->cur], c->Qh, c->gv,
+                              c->m, c->m_full, c->tp, dt, so, ss, c->sig + 16, t, st);
+    vh::mx::launch_stage_tile(1, c->c2p_mode, nc, c->G, c->eos, c->Q[c->cur], so, c->gv,
+                              c->m, c->m_full, c->tp, dt, sc, ss, c->sig + 16, t, st);
+  };
   hipEvent_t e0, e1;
   VH_HIP(hipEventCreate(&e0));
   VH_HIP(hipEventCreate(&e1));
-  float best = 3.0e38f;
-  vh::mx::Tile bt = cand[0];
+  go();
+  go();
   int rc = VH_OK;
+  if (hipEventRecord(e0, st) != hipSuccess) rc = VH_ERR_HIP;
+  for (int r = 0; r < reps; r++) go();
+  if (hipEventRecord(e1, st) != hipSuccess) rc = VH_ERR_HIP;
+  ms = 0.f;
+  if (rc == VH_OK &&
+      (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+    rc = VH_ERR_HIP;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != VH_OK) return rc;
+  VH_HIP(hipGetLastError());
+  ms /= (float)reps;
+  return VH_OK;
+}
+
+// Autotuner of the stage kernel: times a predictor and the corrector from
+// its state for every candidate shape (vh::mx::tile_candidates) on the
+// current state, and keeps the fastest.  Every shape gives the same result.
+static int ensure_tile(vh_ctx_t *c, double tau, double dt) {
+  if (c->precision != VH_PREC_MIXED || c->tile.kind || c->tile_forced) return VH_OK;
+  const std::vector<vh::mx::Tile> cand = vh::mx::tile_candidates(c->G);
+  VH_HIP(hipMemcpyAsync(c->tp, &tau, sizeof(double), hipMemcpyHostToDevice, c->st));
+  if (c->c2p_mode == VH_C2P_LEARNED)
+    vh::ml::launch_guess(c->Q[c->cur], c->g.N, c->charged ? 7 : 4, c->tp, 0., c->mlp, c->gv,
+                         c->st);
+  float best = 3.0e38f, best_alt = 3.0e38f;
+  vh::mx::Tile bt = cand[0], alt = cand[0];
   for (const vh::mx::Tile &t : cand) {
-    // a predictor and the corrector from its state (the two stages' costs
-    // differ per shape: the row kernel's corrector can lose what its
-    // predictor gains)
-    auto go = [&] {
-      vh::mx::launch_stage_tile(0, c->c2p_mode, nc, c->G, c->eos, c->Q[c->cur], c->Qh, c->gv,
-                                c->m, c->m_full, c->tp, dt, so, ss, c->sig + 16, t, st);
-      vh::mx::launch_stage_tile(1, c->c2p_mode, nc, c->G, c->eos, c->Q[c->cur], so, c->gv,
-                                c->m, c->m_full, c->tp, dt, sc, ss, c->sig + 16, t, st);
-    };
-    go();
-    go();
-    if (hipEventRecord(e0, st) != hipSuccess) rc = VH_ERR_HIP;
-    for (int r = 0; r < 6; r++) go();
-    if (hipEventRecord(e1, st) != hipSuccess) rc = VH_ERR_HIP;
     float ms = 0.f;
-    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
-      rc = VH_ERR_HIP;
-    if (rc != VH_OK) break;
-    // the tuning runs on the initial state; as the flow develops, the row
+    const int rc = time_tile(c, t, dt, 6, ms);
+    if (rc != VH_OK) return rc;
+    // the first tuning sees the initial state; as the flow develops, the row
     // kernel's waves (64 cells along x) diverge more in the con2prim
     // iterations than the cell kernel's, so a row shape is kept only when it
-    // is clearly faster here
+    // is clearly faster here (the second tuning, on the developed flow,
+    // decides without this margin)
     const bool rows = t.kind == vh::mx::STAGE_ROWS, brows = bt.kind == vh::mx::STAGE_ROWS;
     const float m = rows == brows ? 1.f : (rows ? 0.95f : 1.05f);
     if (ms < best * m) {
@@ -870,13 +905,50 @@ static int ensure_tile(vh_ctx_t *c, double tau, double dt) {
       bt = t;
     }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (rc != VH_OK) return rc;
-  VH_HIP(hipGetLastError());
+  // the best shape of the other kind, for the second tuning
+  for (const vh::mx::Tile &t : cand) {
+    if (t.kind == bt.kind) continue;
+    float ms = 0.f;
+    const int rc = time_tile(c, t, dt, 2, ms);
+    if (rc != VH_OK) return rc;
+    if (ms < best_alt) {
+      best_alt = ms;
+      alt = t;
+    }
+  }
   c->tile = bt;
-  c->tile_ms = best / 6.;
+  c->tile_alt = alt.kind != bt.kind ? alt : bt;
+  c->tile_ms = best;
+  c->tune_stage = 1;
+  c->steps_since_tune = 0;
   drop_graphs(c);
+  return VH_OK;
+}
+
+// Steps after the first tuning at which the stage shape is tuned again, on
+// the developed flow: the current shape against the best of the other kind.
+constexpr long kRetuneAfter = 32;
+
+static int retune(vh_ctx_t *c, double dt) {
+  c->tune_stage = 2;
+  if (c->tile_forced || c->tile_alt.kind == c->tile.kind || c->tile_alt.kind == 0) return VH_OK;
+  VH_HIP(hipStreamSynchronize(c->st));
+  // the learned con2prim's guesses of the current state (each step forms its
+  // own before its stages, so the replayed steps are unaffected)
+  if (c->c2p_mode == VH_C2P_LEARNED)
+    vh::ml::launch_guess(c->Q[c->cur], c->g.N, c->charged ? 7 : 4, c->tp, 0., c->mlp, c->gv,
+                         c->st);
+  float a = 0.f, b = 0.f;
+  int rc = time_tile(c, c->tile, dt, 4, a);
+  if (rc == VH_OK) rc = time_tile(c, c->tile_alt, dt, 4, b);
+  if (rc != VH_OK) return rc;
+  if (b < a) {
+    std::swap(c->tile, c->tile_alt);
+    c->tile_ms = b;
+    drop_graphs(c);
+  } else {
+    c->tile_ms = a;
+  }
   return VH_OK;
 }
 
@@ -893,14 +965,31 @@ static int run_impl(vh_ctx_t *c, double *tau, double dt, int nsteps, int *status
   }
   vh::launch_fill_int(c->status, 0, c->st);
   VH_HIP(hipMemcpyAsync(c->tp, tau, sizeof(double), hipMemcpyHostToDevice, c->st));
-  if (use_graph) {
-    for (int s = 0; s < nsteps; s++) {
-      VH_HIP(hipGraphLaunch(c->gexec[c->cur], c->st));
-      c->cur ^= 1;
+  int done = 0;
+  while (done < nsteps) {
+    // the steps up to the second tuning of the stage shape, if it is due
+    int chunk = nsteps - done;
+    const bool due = c->tune_stage == 1;
+    if (due) chunk = (int)std::min<long>(chunk, std::max<long>(1, kRetuneAfter - c->steps_since_tune));
+    if (use_graph) {
+      int left = chunk;
+      for (; left >= kGraphSteps; left -= kGraphSteps)  // kGraphSteps is even: same parity
+        VH_HIP(hipGraphLaunch(c->gmulti[c->cur], c->st));
+      for (; left > 0; left--) {
+        VH_HIP(hipGraphLaunch(c->gexec[c->cur], c->st));
+        c->cur ^= 1;
+      }
+    } else {
+      for (int s = 0; s < chunk; s++) {
+        int rc = enqueue_step(c, dt);
+        if (rc != VH_OK) return rc;
+      }
     }
-  } else {
-    for (int s = 0; s < nsteps; s++) {
-      int rc = enqueue_step(c, dt);
+    done += chunk;
+    c->steps_since_tune += chunk;
+    if (due && c->steps_since_tune >= kRetuneAfter) {
+      int rc = retune(c, dt);
+      if (rc == VH_OK && use_graph) rc = build_graphs(c, dt);
       if (rc != VH_OK) return rc;
     }
   }
