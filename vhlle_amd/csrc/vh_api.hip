@@ -90,11 +90,12 @@ struct vh_ctx {
   double *ph = nullptr, *pt = nullptr, *vf = nullptr;
   double *tp = nullptr;
   int *status = nullptr;
-  // HLLE signal speeds (bit patterns of non-negative doubles): [0..5] the
-  // running step's predictor and corrector maxima (x, y, eta), [6..11] the
-  // last completed step's, [12..14] the maxima since the last reset,
-  // [16..18] the autotuner's scratch slots
-  unsigned long long *sig = nullptr;
+  // HLLE signal speeds (bit patterns of non-negative doubles): sig_acc, nine
+  // slot groups (vh::kSigGroup words each) the flux kernels add to: the
+  // running step's predictor and corrector (x, y, eta), then the autotuner's
+  // scratch; sig: [6..11] the last completed step's maxima, [12..14] the
+  // per-axis maxima since the last reset
+  unsigned long long *sig = nullptr, *sig_acc = nullptr;
   // learned con2prim: MFMA fragments of the installed MLP and the per-cell
   // guesses (mixed precision only; gv reuses the FP64 path's F[0] storage)
   vh::ml::Packed *mlp = nullptr;
@@ -244,11 +245,11 @@ int enqueue_step_mixed(vh_ctx *c, double dt) {
     rc |= timed(c, "c2p_guess", [&] { vh::ml::launch_guess(Qo, g.N, nc, c->tp, 0., c->mlp, c->gv, st); });
   auto stage = [&](int mode, double *out) {
     vh::mx::launch_stage_tile(mode, cm, nc, G, c->eos, Qo, c->Qh, c->gv, c->m, c->m_full, c->tp,
-                              dt, out, c->status, c->sig + 3 * mode, c->tile, st);
+                              dt, out, c->status, c->sig_acc + 3 * mode * vh::kSigGroup, c->tile, st);
   };
   rc |= timed(c, "stage_pred", [&] { stage(0, c->Qh); });
   rc |= timed(c, "stage_corr", [&] { stage(1, Qn); });
-  rc |= timed(c, "advance_tau", [&] { vh::launch_advance_tau(c->tp, dt, c->sig, st); });
+  rc |= timed(c, "advance_tau", [&] { vh::launch_advance_tau(c->tp, dt, c->sig_acc, c->sig, st); });
   ghost_fill(c, 0, Qn, st, rc);
   if (c->viscous) {
     rc |= timed(c, "visc_prim", [&] { vh::mx::launch_vprim(cm, nc, g.N, c->eos, c->Qh, c->gv, c->tp, dt, c->ph, c->pt, st); });
@@ -289,7 +290,8 @@ int enqueue_step_fp64(vh_ctx *c, double dt) {
       if (ext[D - 1] < 2) continue;  // no interfaces along a 1-cell axis
       rc |= timed(c, face_name(D, mode), [&] {
         vh::launch_faces(D, mode, g, c->eos, src, Qo, c->m, c->m_full, c->tp, dt, c->F[D - 1],
-                         c->fl[D - 1], c->status, c->sig + 3 * mode + D - 1, st);
+                         c->fl[D - 1], c->status,
+                         c->sig_acc + (3 * mode + D - 1) * vh::kSigGroup, st);
       });
     }
     const double *Fs[3] = {c->F[0], c->F[1], c->F[2]};
@@ -299,7 +301,7 @@ int enqueue_step_fp64(vh_ctx *c, double dt) {
                         c->status, st);
     });
   }
-  rc |= timed(c, "advance_tau", [&] { vh::launch_advance_tau(c->tp, dt, c->sig, st); });
+  rc |= timed(c, "advance_tau", [&] { vh::launch_advance_tau(c->tp, dt, c->sig_acc, c->sig, st); });
   // Fluid::correctImagCells (src/fld.cpp:211-273): z, y, x
   ghost_fill(c, 0, Qn, st, rc);
   if (c->viscous) {
@@ -347,6 +349,12 @@ void drop_graphs(vh_ctx *c) {
 // Steps per multi-step graph: a run replays one graph launch per kGraphSteps
 // steps (even, so that the graph ends on the parity of Q it starts from).
 constexpr int kGraphSteps = 8;
+
+// clears the signal-speed maxima and slot groups (kernels, on the stream)
+void clear_sig(vh_ctx *c) {
+  vh::launch_clear_u64(c->sig, 32, c->st);
+  vh::launch_clear_u64(c->sig_acc, 9L * vh::kSigGroup, c->st);
+}
 
 bool any_nonzero(const double *a, long n) {
   for (long i = 0; i < n; i++)
@@ -546,7 +554,8 @@ int create_impl(const vh_grid_t *grids, int ngrid, const vh_eos_t *eos, const vh
   auto rnd = [&](size_t b) { return (b + al - 1) / al * al; };
   const size_t dN = rnd(N * sizeof(double)), bN = rnd(N);
   const size_t ndoubles_arrays = 14 + 7 + 20 + 2 + 20 + 2 + 3 + 3 + 1 + 21 + 8 + 4 + 4 + 1;
-  const size_t total = ndoubles_arrays * dN + 3 * bN + 3 * al;
+  const size_t total = ndoubles_arrays * dN + 3 * bN + 3 * al +
+                       rnd(9 * vh::kSigGroup * sizeof(unsigned long long));
   if (hipMalloc(&c->pool, total) != hipSuccess) {
     (void)hipStreamDestroy(c->st);
     delete c;
@@ -583,6 +592,8 @@ int create_impl(const vh_grid_t *grids, int ngrid, const vh_eos_t *eos, const vh
   p += al;
   c->sig = reinterpret_cast<unsigned long long *>(p);
   p += al;
+  c->sig_acc = reinterpret_cast<unsigned long long *>(p);
+  p += rnd(9 * vh::kSigGroup * sizeof(unsigned long long));
   if (vh::make_dev_eos(eos, c->status, c->eos, &c->eos_buf) != VH_OK) {
     vh_destroy(c);
     return VH_ERR_HIP;
@@ -700,7 +711,7 @@ int vh_upload(vh_ctx_t *c, const vh_state_t *h) {
   }
   rc |= put(c->m, h->m, 3, 0.);
   rc |= put(c->vcc, h->viscCorrCut, 1, 1.);
-  vh::launch_clear_u64(c->sig, 32, c->st);  // a new state: no signal speeds yet
+  clear_sig(c);  // a new state: no signal speeds yet
   VH_HIP(hipStreamSynchronize(c->st));
   if (rc != VH_OK) return VH_ERR_HIP;
   return refresh_m_full(c);
@@ -768,7 +779,7 @@ int vh_set_prim(vh_ctx_t *c, double tau, const double *e, const double *nb, cons
   }
   if (rc == VH_OK) {
     vh::launch_set_prim(c->g, c->eos, tau, d, c->Q[c->cur], c->Q[c->cur ^ 1], c->m, c->st);
-    vh::launch_clear_u64(c->sig, 32, c->st);
+    clear_sig(c);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->st) != hipSuccess)
       rc = VH_ERR_HIP;
   }
@@ -850,11 +861,12 @@ static int time_tile(vh_ctx_t *c, const vh::mx::Tile &t, double dt, int reps, fl
   double *so = c->F[1], *sc = c->F[2];               // unused by the mixed path
   int *ss = reinterpret_cast<int *>(c->fl[0]);        // scratch status word
   const int nc = c->charged ? 7 : 4;
+  unsigned long long *tune_sig = c->sig_acc + 6 * vh::kSigGroup;  // scratch slot groups
   auto go = [&] {
     vh::mx::launch_stage_tile(0, c->c2p_mode, nc, c->G, c->eos, c->Q[c->cur], c->Qh, c->gv,
-                              c->m, c->m_full, c->tp, dt, so, ss, c->sig + 16, t, st);
+                              c->m, c->m_full, c->tp, dt, so, ss, tune_sig, t, st);
     vh::mx::launch_stage_tile(1, c->c2p_mode, nc, c->G, c->eos, c->Q[c->cur], so, c->gv,
-                              c->m, c->m_full, c->tp, dt, sc, ss, c->sig + 16, t, st);
+                              c->m, c->m_full, c->tp, dt, sc, ss, tune_sig, t, st);
   };
   hipEvent_t e0, e1;
   VH_HIP(hipEventCreate(&e0));

@@ -445,14 +445,19 @@ __device__ __forceinline__ double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   return v;
 }
-// Maximum signal speed of a wave into slot *s (a non-negative double kept as
-// its bit pattern: for non-negative doubles the integer order is the numeric
-// order, so an integer atomic max is a floating-point max).  One atomic per
-// wave; every lane of the wave must call it.
-__device__ __forceinline__ void wave_speed_max(unsigned long long *s, double v) {
+// Signal-speed maxima are gathered in groups of kSigSlots slots, one 128-byte
+// line apart: a wave adds its maximum to the slot of its index, so the
+// atomics of a launch spread over 64 lines instead of serialising on one
+// address; k_advance_tau folds each group.  A slot holds a non-negative
+// double as its bit pattern (for those the integer order is the numeric
+// order, so an integer atomic max is a floating-point max).
+constexpr int kSigSlots = 64, kSigStride = 16, kSigGroup = kSigSlots * kSigStride;
+// Every lane of the wave must call it.
+__device__ __forceinline__ void wave_speed_max(unsigned long long *grp, double v) {
   v = wave_max(v);
+  const unsigned w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if ((threadIdx.x & 63) == 0 && v > 0.)
-    atomicMax(s, (unsigned long long)__double_as_longlong(v));
+    atomicMax(grp + (w % kSigSlots) * kSigStride, (unsigned long long)__double_as_longlong(v));
 }
 
 // Occupancy target of a kernel: VH_OCC(n) caps it at n waves per SIMD (0: the

@@ -7,8 +7,8 @@
 namespace vh {
 
 // Qs: the stage's state; Qr: Q (states of the negative-pressure
-// re-evaluation, src/hdo.cpp:134-140); sig: the signal-speed slot of this
-// stage and axis (see wave_speed_max)
+// re-evaluation, src/hdo.cpp:134-140); sig: the signal-speed slot group of
+// this stage and axis (see wave_speed_max)
 void launch_faces(int D, int mode, const Geom &g, const DevEos &eos, const double *Qs,
                   const double *Qr, const double *m, int m_full, const double *tp, double dt,
                   double *F, unsigned char *flag, int *status, unsigned long long *sig,
@@ -40,12 +40,14 @@ void launch_visc2(const Geom &g, const double *ph, const double *pi0, const doub
 void launch_visc3(const Geom &g, const double *ph, const double *piH, const double *PiH,
                   const double *vf, const double *m, int m_full, const double *tp, double dt,
                   double *Q, hipStream_t st);
-// tau += dt on the device, and the step's signal speeds: sig[0..5] (this
-// step's predictor and corrector maxima, x, y, eta) are copied to sig[6..11],
-// folded into the running maxima sig[12..14] (per axis) and cleared
-void launch_advance_tau(double *tp, double dt, unsigned long long *sig, hipStream_t st);
-// clears n signal-speed slots from sig (a kernel: graph-capturable)
-void launch_clear_u64(unsigned long long *sig, int n, hipStream_t st);
+// tau += dt on the device, and the step's signal speeds: the six slot
+// groups of acc (kSigGroup each: predictor x, y, eta, corrector x, y, eta)
+// are folded into sig[6..11] and the running maxima sig[12..14] (per axis),
+// and cleared
+void launch_advance_tau(double *tp, double dt, unsigned long long *acc, unsigned long long *sig,
+                        hipStream_t st);
+// clears n 64-bit words from p (a kernel: graph-capturable)
+void launch_clear_u64(unsigned long long *p, long n, hipStream_t st);
 void launch_fill_int(int *p, int v, hipStream_t st);
 void launch_set_prim(const Geom &g, const DevEos &eos, double tau, const double *pr, double *Q,
                      double *Qprev, double *m, hipStream_t st);

@@ -12,6 +12,8 @@
 // and stored; each cell then gathers its interfaces in the reference's
 // accumulation order (+x-, -x+, +y-, -y+, +z-, -z+, then the source), which
 // reproduces the reference's sums bit for bit.
+#include <algorithm>
+
 #include "vh_kernels.h"
 #include "vh_vort.h"
 
@@ -55,63 +57,26 @@ __device__ __forceinline__ void face_state(const DevEos &eos, const double *__re
 }
 
 // ------------------------------------------------------- interface flux
-// Hydro::hlle_flux (src/hdo.cpp:77-305) for the interface between cell c and
-// its +1 neighbour along axis D.  Writes F (7 x N, slot c) and flag[c]
-// (1 = flux present, 0 = the reference returned early without a flux).
-// Qs: the stage's state (Q in the predictor, Qh in the corrector); Qr: Q, from
-// which the reference re-evaluates both face states when the right state's
-// pressure is negative (src/hdo.cpp:134-140), in either stage.  Returns the
-// interface's signal speed max(-b_l, b_r) (the wave-speed estimates of the
-// HLLE flux, in units of dx/dtau, deta/dtau along eta), 0 without a flux.
+// Hydro::hlle_flux (src/hdo.cpp:77-305) for the interface between cell cl
+// and its +1 neighbour cr along axis D, in two parts.
+//
+// flux_tail: everything after the face states are final (src/hdo.cpp:142-
+// 304): the partial-vacuum skip, the HLLE wave speeds from El, Er, the flux.
+// Writes F (7 x N, slot cl) and returns the interface's signal speed
+// max(-b_l, b_r) (in units of dx/dtau, deta/dtau along eta); *ok = 1 when a
+// flux is formed, 0 when the reference returns without one.
 template <int D, int MODE>
-__device__ __forceinline__ double face_flux(const Geom &g, const DevEos &eos,
-                                            const double *__restrict__ Qs,
-                                            const double *__restrict__ Qr,
-                                            const double *__restrict__ m, int m_full,
-                                            double tau, double dt, long c,
-                                            double *__restrict__ F,
-                                            unsigned char *__restrict__ flag,
+__device__ __forceinline__ double flux_tail(const Geom &g, const double *__restrict__ m,
+                                            int m_full, Prim &L, Prim &R, double El, double Er,
+                                            double tau, double dt, long cl, long cr,
+                                            double *__restrict__ F, unsigned char &ok,
                                             int *__restrict__ status) {
-  int ix, iy, iz;
-  decompose(g, c, ix, iy, iz);
-  const int i = coord<D>(ix, iy, iz), n = extent<D>(g);
-  if (i >= n - 1) return 0.;
-  const long s = stride<D>(g);
-  const long cl = c, cr = c + s;
-  const long cll = i > 0 ? c - s : c;          // clamped prev of the left cell
-  const long crr = i + 2 < n ? cr + s : cr;    // clamped next of the right cell
-  const long N = g.N;
-  Prim L, R;
-  face_state<+1>(eos, Qs, N, cll, cl, cr, tau, L);
-  face_state<-1>(eos, Qs, N, cl, cr, crr, tau, R);
-  unsigned char ok = 0;
+  const long N = g.N, c = cl;
   const double dta = MODE == 0 ? dt / 2. : dt;
-  const double El = (L.e + L.p) / (1 - L.vx * L.vx - L.vy * L.vy - L.vz * L.vz);
-  const double Er = (R.e + R.p) / (1 - R.vx * R.vx - R.vy * R.vy - R.vz * R.vz);
   const double tauFactor = MODE == 0 ? tau + 0.25 * dt : tau + 0.5 * dt;
-  if (L.e < 0.) {
-    L.e = 0.;
-    L.p = 0.;
-  }
-  if (R.e < 0.) {
-    R.e = 0.;
-    R.p = 0.;
-  }
   double speed = 0.;
+  ok = 0;
   do {
-    if (L.e > 1e10) {
-      atomicOr(status, VH_ST_BIG_E);
-      break;
-    }
-    if (L.e == 0. && R.e == 0.) break;
-    // negative pressure on the right (a table EoS whose spline undershoots
-    // below its clamped nodes): the reference evaluates both face states again
-    // from Q -- not Qh, even in the corrector -- and goes on with them, without
-    // clamping e again and with El, Er of the first states (src/hdo.cpp:134-140)
-    if (R.p < 0.) {
-      face_state<+1>(eos, Qr, N, cll, cl, cr, tau, L);
-      face_state<-1>(eos, Qr, N, cl, cr, crr, tau, R);
-    }
     if (!m_full && m[(D - 1) * N + cl] < 1. && m[(D - 1) * N + cr] < 1.) break;
     const double gl = 1. / sqrt(1 - L.vx * L.vx - L.vy * L.vy - L.vz * L.vz);
     const double gr = 1. / sqrt(1 - R.vx * R.vx - R.vy * R.vy - R.vz * R.vz);
@@ -192,22 +157,107 @@ __device__ __forceinline__ double face_flux(const Geom &g, const DevEos &eos,
     ok = 1;
     speed = smax(-bl, br);
   } while (0);
+  return speed;
+}
+
+// The face states of the stage's state Qs (Q in the predictor, Qh in the
+// corrector), the reference's early returns, then flux_tail.  A negative
+// pressure of the right state (a table EoS whose spline undershoots below
+// its clamped nodes) makes the reference evaluate both face states again
+// from Q -- not Qh, even in the corrector -- and go on with them, without
+// clamping e again and with El, Er of the first states (src/hdo.cpp:134-140):
+// such an interface is left to k_faces_fix with flag 2 and El, Er in its F
+// slot, so that this kernel carries one copy of the face-state code.
+template <int D, int MODE>
+__device__ __forceinline__ double face_flux(const Geom &g, const DevEos &eos,
+                                            const double *__restrict__ Qs,
+                                            const double *__restrict__ m, int m_full,
+                                            double tau, double dt, long c,
+                                            double *__restrict__ F,
+                                            unsigned char *__restrict__ flag,
+                                            int *__restrict__ status) {
+  int ix, iy, iz;
+  decompose(g, c, ix, iy, iz);
+  const int i = coord<D>(ix, iy, iz), n = extent<D>(g);
+  if (i >= n - 1) return 0.;
+  const long s = stride<D>(g);
+  const long cl = c, cr = c + s;
+  const long cll = i > 0 ? c - s : c;          // clamped prev of the left cell
+  const long crr = i + 2 < n ? cr + s : cr;    // clamped next of the right cell
+  const long N = g.N;
+  Prim L, R;
+  face_state<+1>(eos, Qs, N, cll, cl, cr, tau, L);
+  face_state<-1>(eos, Qs, N, cl, cr, crr, tau, R);
+  const double El = (L.e + L.p) / (1 - L.vx * L.vx - L.vy * L.vy - L.vz * L.vz);
+  const double Er = (R.e + R.p) / (1 - R.vx * R.vx - R.vy * R.vy - R.vz * R.vz);
+  if (L.e < 0.) {
+    L.e = 0.;
+    L.p = 0.;
+  }
+  if (R.e < 0.) {
+    R.e = 0.;
+    R.p = 0.;
+  }
+  unsigned char ok = 0;
+  double speed = 0.;
+  if (L.e > 1e10) {
+    atomicOr(status, VH_ST_BIG_E);
+  } else if (!(L.e == 0. && R.e == 0.)) {
+    if (R.p < 0.) {
+      F[c] = El;
+      F[N + c] = Er;
+      ok = 2;
+    } else {
+      speed = flux_tail<D, MODE>(g, m, m_full, L, R, El, Er, tau, dt, cl, cr, F, ok, status);
+    }
+  }
   flag[c] = ok;
   return speed;
 }
 
 // One thread per interface (slot = left cell); the wave's largest signal
-// speed goes to *sig (the stage's slot of this axis).
+// speed goes to the slot group sig of this stage and axis.
 template <int D, int MODE>
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_FACES)
-k_faces(Geom g, DevEos eos, const double *__restrict__ Qs, const double *__restrict__ Qr,
-        const double *__restrict__ m, int m_full, const double *__restrict__ tp, double dt,
-        double *__restrict__ F, unsigned char *__restrict__ flag, int *__restrict__ status,
+k_faces(Geom g, DevEos eos, const double *__restrict__ Qs, const double *__restrict__ m,
+        int m_full, const double *__restrict__ tp, double dt, double *__restrict__ F,
+        unsigned char *__restrict__ flag, int *__restrict__ status,
         unsigned long long *__restrict__ sig) {
   const double tau = *tp;
   const long c = xcd_block() * blockDim.x + threadIdx.x;
   double speed = 0.;
-  if (c < g.N) speed = face_flux<D, MODE>(g, eos, Qs, Qr, m, m_full, tau, dt, c, F, flag, status);
+  if (c < g.N) speed = face_flux<D, MODE>(g, eos, Qs, m, m_full, tau, dt, c, F, flag, status);
+  wave_speed_max(sig, speed);
+}
+
+// The interfaces k_faces left with flag 2: both face states again from Q
+// (Qr), then flux_tail with the El, Er of the first states.  Launched after
+// k_faces for table EoSs only (p = e/3 >= 0 never leaves one).
+template <int D, int MODE>
+__global__ void __launch_bounds__(256)
+k_faces_fix(Geom g, DevEos eos, const double *__restrict__ Qr, const double *__restrict__ m,
+            int m_full, const double *__restrict__ tp, double dt, double *__restrict__ F,
+            unsigned char *__restrict__ flag, int *__restrict__ status,
+            unsigned long long *__restrict__ sig) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double speed = 0.;
+  if (c < g.N && flag[c] == 2) {
+    const double tau = *tp;
+    int ix, iy, iz;
+    decompose(g, c, ix, iy, iz);
+    const int i = coord<D>(ix, iy, iz), n = extent<D>(g);
+    const long s = stride<D>(g), N = g.N;
+    const long cl = c, cr = c + s;
+    const long cll = i > 0 ? c - s : c;
+    const long crr = i + 2 < n ? cr + s : cr;
+    const double El = F[c], Er = F[N + c];
+    Prim L, R;
+    face_state<+1>(eos, Qr, N, cll, cl, cr, tau, L);
+    face_state<-1>(eos, Qr, N, cl, cr, crr, tau, R);
+    unsigned char ok = 0;
+    speed = flux_tail<D, MODE>(g, m, m_full, L, R, El, Er, tau, dt, cl, cr, F, ok, status);
+    flag[c] = ok;
+  }
   wave_speed_max(sig, speed);
 }
 
@@ -1130,21 +1180,29 @@ k_p2c_batch(const double *__restrict__ prim, long n, double *__restrict__ Q) {
 }
 
 // tau_{n+1} = tau_n + dt on the device (Hydro::performStep, src/hdo.cpp:1095)
-__global__ void k_advance_tau(double *__restrict__ tp, double dt,
-                              unsigned long long *__restrict__ sig) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    tp[0] = tp[0] + dt;
-    for (int k = 0; k < 6; k++) {
-      const unsigned long long v = sig[k];
-      sig[6 + k] = v;
-      if (v > sig[12 + k % 3]) sig[12 + k % 3] = v;  // non-negative: integer order
-      sig[k] = 0ull;
+// one wave: tau += dt; each of the six signal-speed groups of the step
+// (predictor x, y, eta; corrector x, y, eta) is folded over its slots into
+// sig[6 + k] and the running maximum sig[12 + axis], and cleared
+__global__ void __launch_bounds__(64) k_advance_tau(double *__restrict__ tp, double dt,
+                                                    unsigned long long *__restrict__ acc,
+                                                    unsigned long long *__restrict__ sig) {
+  const int t = threadIdx.x;
+  if (t == 0) tp[0] = tp[0] + dt;
+  for (int k = 0; k < 6; k++) {
+    unsigned long long *slot = acc + k * kSigGroup + t * kSigStride;
+    const double v = wave_max(__longlong_as_double((long long)*slot));
+    *slot = 0ull;
+    if (t == 0) {
+      const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+      sig[6 + k] = u;
+      if (u > sig[12 + k % 3]) sig[12 + k % 3] = u;
     }
   }
 }
 
-__global__ void k_clear_u64(unsigned long long *__restrict__ p, int n) {
-  if (blockIdx.x == 0 && (int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+__global__ void __launch_bounds__(256) k_clear_u64(unsigned long long *__restrict__ p, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    p[i] = 0ull;
 }
 
 __global__ void k_fill_int(int *__restrict__ p, int v) {
@@ -1159,10 +1217,14 @@ void launch_faces(int D, int mode, const Geom &g, const DevEos &eos, const doubl
                   double *F, unsigned char *flag, int *status, unsigned long long *sig,
                   hipStream_t st) {
   const dim3 b(256), gr(blocks(g.N));
-#define VH_F(DD, MM)                                                                          \
-  if (D == DD && mode == MM)                                                                  \
-    hipLaunchKernelGGL((k_faces<DD, MM>), gr, b, 0, st, g, eos, Qs, Qr, m, m_full, tp, dt, F, \
-                       flag, status, sig);
+#define VH_F(DD, MM)                                                                        \
+  if (D == DD && mode == MM) {                                                              \
+    hipLaunchKernelGGL((k_faces<DD, MM>), gr, b, 0, st, g, eos, Qs, m, m_full, tp, dt, F,   \
+                       flag, status, sig);                                                  \
+    if (eos.kind == VH_EOS_TABLE)                                                           \
+      hipLaunchKernelGGL((k_faces_fix<DD, MM>), gr, b, 0, st, g, eos, Qr, m, m_full, tp, dt, \
+                         F, flag, status, sig);                                             \
+  }
   VH_F(1, 0) VH_F(1, 1) VH_F(2, 0) VH_F(2, 1) VH_F(3, 0) VH_F(3, 1)
 #undef VH_F
 }
@@ -1234,12 +1296,15 @@ void launch_visc3(const Geom &g, const double *ph, const double *piH, const doub
                      m_full, tp, dt, Q);
 }
 
-void launch_advance_tau(double *tp, double dt, unsigned long long *sig, hipStream_t st) {
-  hipLaunchKernelGGL(k_advance_tau, dim3(1), dim3(64), 0, st, tp, dt, sig);
+void launch_advance_tau(double *tp, double dt, unsigned long long *acc, unsigned long long *sig,
+                        hipStream_t st) {
+  static_assert(kSigSlots == 64, "one lane per slot");
+  hipLaunchKernelGGL(k_advance_tau, dim3(1), dim3(64), 0, st, tp, dt, acc, sig);
 }
 
-void launch_clear_u64(unsigned long long *sig, int n, hipStream_t st) {
-  hipLaunchKernelGGL(k_clear_u64, dim3(1), dim3(64), 0, st, sig, n);
+void launch_clear_u64(unsigned long long *p, long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_clear_u64, dim3((unsigned)std::min<long>((n + 255) / 256, 64)), dim3(256),
+                     0, st, p, n);
 }
 
 void launch_fill_int(int *p, int v, hipStream_t st) {

@@ -27,8 +27,8 @@ struct Grids {
 };
 
 // gv: per-cell con2prim guesses (used by VH_C2P_LEARNED only, may be null otherwise)
-// sig: the stage's signal-speed slots (x, y, eta; maxima of max(-b_l, b_r)
-// kept as bit patterns, see wave_speed_max)
+// sig: the stage's signal-speed slot groups (x, y, eta, kSigGroup words
+// apart; maxima of max(-b_l, b_r) kept as bit patterns, see wave_speed_max)
 void launch_stage(int mode, int cm, int nc, const Grids &G, const DevEos &eos, const double *Q,
                   const double *Qh, const float *gv, const double *m, int m_full,
                   const double *tp, double dt, double *out, int *status,

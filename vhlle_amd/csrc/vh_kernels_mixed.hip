@@ -541,7 +541,8 @@ __device__ __forceinline__ void finish_cell(const DevEos &eos, const double *__r
 // MODE 0 (predictor): reads Q, writes Qh = Q + flux.
 // MODE 1 (corrector): reads Q and Qh, writes Qnew = Q + flux unless
 // Q0 + flux0 < 0 (src/cll.cpp:70-74).
-// sig: the stage's three signal-speed slots (x, y, eta), one atomic per wave.
+// sig: the stage's three signal-speed slot groups (x, y, eta; kSigGroup
+// words apart), one atomic per wave and group.
 template <int MODE, int CM, int NC>
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_MX_STAGE)
 k_stage(Grids G, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
@@ -568,8 +569,8 @@ k_stage(Grids G, DevEos eos, const double *__restrict__ Q, const double *__restr
     finish_cell<MODE, CM, NC>(eos, Q, Qs, gv, N, c, tau, dt, acc, out);
   }
   wave_speed_max(sig, (double)sx);
-  wave_speed_max(sig + 1, (double)sy);
-  wave_speed_max(sig + 2, (double)sz);
+  wave_speed_max(sig + kSigGroup, (double)sy);
+  wave_speed_max(sig + 2 * kSigGroup, (double)sz);
 }
 
 // Interface flux between cells a (left) and b (right) along axis D: MUSCL
@@ -691,8 +692,8 @@ k_stage_rows(Grids G, DevEos eos, const double *__restrict__ Q, const double *__
     hzl = hzu;
   }
   wave_speed_max(sig, (double)spx);
-  wave_speed_max(sig + 1, (double)spy);
-  wave_speed_max(sig + 2, (double)spz);
+  wave_speed_max(sig + kSigGroup, (double)spy);
+  wave_speed_max(sig + 2 * kSigGroup, (double)spz);
 }
 
 // ---------------------------------------------------------------- viscous
