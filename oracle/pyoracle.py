@@ -104,6 +104,11 @@ def lib():
         L.or_run.argtypes = [ctypes.c_void_p, dp, ctypes.c_double, ctypes.c_int]
         L.or_check_dtau.restype = ctypes.c_int
         L.or_branch_counts.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long)]
+        L.or_ghost.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_update_m.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.or_advect.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.or_hlle.restype = ctypes.c_int
+        L.or_hlle.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5 + [ctypes.c_double] * 2 + [dp]
         L.or_signal_speed.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
         L.or_check_dtau.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.or_c2p.argtypes = [ctypes.POINTER(OrEos), dp, ctypes.c_long, dp,
@@ -229,6 +234,24 @@ class Solver:
 
     def set_vorticity(self, on=True):
         lib().or_set_vorticity(self.h, int(on))
+
+    def ghost(self, full=False):
+        """the ghost fill alone (correctImagCells / correctImagCellsFull)"""
+        lib().or_ghost(self.h, int(full))
+
+    def update_m(self, tau, dt):
+        """Fluid::updateM alone"""
+        lib().or_update_m(self.h, tau, dt)
+
+    def advect(self, tau, dt):
+        """ISformal loop #2 alone (advection of pi0, piH0, Pi0, PiH0, limiter)"""
+        lib().or_advect(self.h, tau, dt)
+
+    def hlle(self, ix, iy, iz, d, mode, tau, dt):
+        """(formed, flux[7]) of one interface's hlle_flux"""
+        f = np.zeros(7)
+        rc = lib().or_hlle(self.h, ix, iy, iz, d, mode, tau, dt, _p(f))
+        return rc, f
 
     def branch_counts(self):
         """(negative-pressure re-evaluations of hlle_flux in the predictor, in

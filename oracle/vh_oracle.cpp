@@ -863,7 +863,13 @@ struct Solver {
               c.Pi0 += lPp * pH[sym(k, l)] * sig[k][l] * gm[k] * gm[l] / gamma * dt;
           c.Pi0 += -dPP * c.PiH0 * du / gamma * dt;
         }
-    // loop #2: advection + limiter
+    advect();
+  }
+
+  // ISformal loop #2 (src/hdo.cpp:866-962): upwind advection of pi0, piH0,
+  // Pi0, PiH0 from the 8 cells toward -sign(v), the limiter, the bulk clamp
+  // and the -tau dQ_{n+1} flux term
+  void advect() {
     for (int ix = 0; ix < g.nx; ix++)
       for (int iy = 0; iy < g.ny; iy++)
         for (int iz = 0; iz < g.nz; iz++) {
@@ -1305,6 +1311,47 @@ int or_run(void *h, double *tau, double dt, int nsteps) {
   for (int i = 0; i < nsteps && !s->status; i++) s->step();
   *tau = s->tau;
   return s->status;
+}
+
+// ---- single functions of the step, for unit tests of the restatement
+void or_ghost(void *h, int full) {
+  Solver *sv = static_cast<Solver *>(h);
+  if (full)
+    sv->ghost_full();
+  else
+    sv->ghost_ideal();
+}
+
+void or_update_m(void *h, double tau, double dt) {
+  Solver *sv = static_cast<Solver *>(h);
+  sv->tau = tau;
+  sv->dt = dt;
+  sv->update_m();
+}
+
+void or_advect(void *h, double tau, double dt) {
+  Solver *sv = static_cast<Solver *>(h);
+  sv->tau = tau;
+  sv->dt = dt;
+  for (auto &c : sv->g.c) c.clear_flux();
+  sv->advect();
+}
+
+int or_hlle(void *h, int ix, int iy, int iz, int d, int mode, double tau, double dt,
+            double *flux) {
+  Solver *sv = static_cast<Solver *>(h);
+  sv->tau = tau;
+  sv->dt = dt;
+  sv->status = 0;
+  for (auto &c : sv->g.c) c.clear_flux();
+  sv->hlle(ix, iy, iz, d, mode);
+  const OCell &r = sv->g.at(ix + (d == 1), iy + (d == 2), iz + (d == 3));
+  int any = 0;
+  for (int k = 0; k < 7; k++) {
+    flux[k] = r.flux[k];
+    any |= r.flux[k] != 0.;
+  }
+  return sv->status ? -sv->status : any;
 }
 
 void or_branch_counts(void *h, long *out) {

@@ -72,6 +72,21 @@ int or_run(void *h, double *tau, double dt, int nsteps);
 void or_set_vorticity(void *h, int on);
 /* Cell::getDbeta of every cell: out 16 x N, component 4 i + j = dbeta[i][j] */
 void or_get_dbeta(void *h, double *out);
+/* Single functions of the step on the current state, for unit tests of the
+ * restatement: the ghost fill (full = 0: correctImagCells, Q and Qh; 1:
+ * correctImagCellsFull, Q, pi, Pi; src/fld.cpp:211-382); Fluid::updateM
+ * (src/fld.cpp:384-423); ISformal loop #2 (advection, limiter, bulk clamp,
+ * src/hdo.cpp:866-962) from the auxiliary arrays pi0, piH0, Pi0, PiH0 and the
+ * Qh prims at tau - dt/2; hlle_flux of one interface (src/hdo.cpp:77-305) from
+ * (ix, iy, iz) to its +1 neighbour along d (1..3) in mode 0 (predictor, Q) or
+ * 1 (corrector, Qh): flux[7] receives the flux added to the right cell; the
+ * return value is 1 if a flux was formed, 0 if the reference returned early,
+ * negative on its error exits. */
+void or_ghost(void *h, int full);
+void or_update_m(void *h, double tau, double dt);
+void or_advect(void *h, double tau, double dt);
+int or_hlle(void *h, int ix, int iy, int iz, int d, int mode, double tau, double dt,
+            double *flux);
 /* Branch counters since creation: out[4] = negative-pressure re-evaluations
  * of hlle_flux in the predictor and in the corrector (src/hdo.cpp:134-140),
  * cells rescaled by the ISformal limiter (src/hdo.cpp:925-932), cells with Pi
