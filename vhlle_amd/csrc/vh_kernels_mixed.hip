@@ -608,7 +608,10 @@ k_stage_rows(Grids G, DevEos eos, const double *__restrict__ Q, const double *__
              const double *__restrict__ tp, double dt, double *__restrict__ out,
              int *__restrict__ status, unsigned long long *__restrict__ sig, int CZ) {
   const int lane = threadIdx.x & 63;
-  const long w = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  // XCD-contiguous workgroups: the segments an XCD processes at once are
+  // consecutive, so the y-neighbour rows of a segment (nx cells away) were
+  // fetched into that XCD's L2 by the waves just before it
+  const long w = xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const long sgl = w % G.nseg, zc = w / G.nseg;  // row segment over all grids, eta chunk
   long seg;
   const Geom g = seg_grid(G, sgl, seg);
