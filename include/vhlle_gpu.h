@@ -77,12 +77,48 @@ typedef struct {
  * increasing, GeV/fm^3), p (negative values clamped to 0 as the reference
  * does), T (GeV) and mu (GeV, may be NULL = 0); natural cubic splines of p(e),
  * T(e), mu(e) (gsl_interp_cspline); cs2(e) = dpe(e) = (p(1.1e) - p(e))/(0.1e);
- * mu_b = mu_q = mu_s = 0 in eos().  The arrays are copied (host memory). */
-enum { VH_EOS_IDEAL = 0, VH_EOS_TABLE = 1 };
+ * mu_b = mu_q = mu_s = 0 in eos().
+ * kind 2: EoSChiral (src/eoChiral.cpp:99-154): bilinear lookups in two
+ * uniform (e, n_b) tables -- `small` where e < 1.46 and n_b < 0.3, else `big`
+ * where e < 146 and n_b < 6, else p = 0.2964 e, T = 0.15120476935 e^(1/4),
+ * mu_b = mu_s = 0; mu_q = 0 always; cs2(e) = 1/3 (src/eos.h:37).
+ * kind 3: EoSHadron (src/eoHadron.cpp:78-208): trilinear lookups in a table
+ * uniform in log(e/e0) and in the log-mapped n_b, n_q; cs2(e) = 1/3.
+ * Kinds 2 and 3 depend on the charge densities: con2prim refreshes n_b, n_q,
+ * n_s every iteration as transformPV does (src/rmn.cpp:74-104).  They are
+ * evaluated by the FP64 path only; VH_PREC_MIXED and the learned con2prim
+ * reject them with VH_ERR_ARG.  All arrays are copied (host memory). */
+enum { VH_EOS_IDEAL = 0, VH_EOS_TABLE = 1, VH_EOS_CHIRAL = 2, VH_EOS_HADRON = 3 };
+
+/* One table of EoSChiral (the auxiliary class EoSaux, src/eoChiral.cpp:38-84):
+ * ne x nn nodes uniform over [emin, emax] x [nmin, nmax] (GeV/fm^3, 1/fm^3 --
+ * the units after the reference's conversion of the file columns), each field
+ * stored at [ie * nn + in].  ne, nn >= 2. */
+typedef struct {
+  int ne, nn;
+  double emin, emax, nmin, nmax;
+  const double *p, *T, *mub, *mus;
+} vh_eos_chiral_tab_t;
+
+/* The table of EoSHadron (src/eoHadron.cpp:13-65): ne x nnb x nnq nodes,
+ * field index inq + nnq * (inb + nnb * ie) (index3, src/eoHadron.h:9-11);
+ * nodes uniform in x_e = log(e/e0) over [logemin, logemax] and in the mapped
+ * x_nb, x_nq over [-lognmax, lognmax].  status 1 marks a node outside the
+ * EoS (lookups whose base node has it return zeros).  ne, nnb, nnq >= 2, and
+ * nnb, nnq >= 3 so that the linear region near n = 0 exists. */
+typedef struct {
+  int ne, nnb, nnq;
+  double e0, n0, logemin, logemax, lognmax;
+  const double *p, *T, *mub, *muq, *mus;
+  const int *status;
+} vh_eos_hadron_tab_t;
+
 typedef struct {
   int kind;
   int n;                          /* table rows (kind 1), >= 3 */
   const double *e, *p, *T, *mu;   /* table columns (kind 1) */
+  const vh_eos_chiral_tab_t *small, *big;  /* kind 2 */
+  const vh_eos_hadron_tab_t *hadron;       /* kind 3 */
 } vh_eos_t;
 /* The natural cubic splines of a table EoS (gsl_spline_init of EoSs,
  * src/eos.cpp:59-69), computed on the host (no GPU needed) into buf[13 n]:
@@ -90,6 +126,13 @@ typedef struct {
  * (n each; y = y_i + dx (b_i + dx (c_i + dx d_i)) on [e_i, e_i+1], dx = e - e_i).
  * Host code (the C++ facade's EoSs) evaluates the same splines as the GPU. */
 int vh_eos_table(const vh_eos_t *eos, double *buf);
+/* EoS::eos and EoS::s (src/eos.h:24-30, src/eos.cpp:18-25) at n points:
+ * in[4 i .. 4 i + 3] = e, n_b, n_q, n_s; out[6 i .. 6 i + 5] = p, T, mu_b,
+ * mu_q, mu_s, s.  where = 1 evaluates with the GPU kernels' device EoS (needs
+ * a GPU); where = 0 on the host, with the same lookups compiled for the CPU
+ * (kinds 2 and 3; kinds 0 and 1 evaluate e/3 and the splines).  The C++
+ * facade's EoSChiral and EoSHadron call it with where = 0. */
+int vh_eos_eval(const vh_eos_t *eos, const double *in, long n, double *out, int where);
 
 /* TransportCoeff (src/trancoeff.h:8-11, ctor :14-20) as a POD, plus the
  * T-peaked bulk parametrisation zetaSparam = 5 (this build):
@@ -156,6 +199,15 @@ int vh_set_prim(vh_ctx_t *ctx, double tau, const double *e, const double *nb,
  * Q/tau), the inputs of the reference's cell outputs (VtkOutput::write,
  * src/vtk.cpp:38-120): prim, 8 x N SoA, rows e, p, nb, nq, ns, vx, vy, vz. */
 int vh_prim_fields(vh_ctx_t *ctx, double tau, double *prim);
+
+/* The EoS of the freeze-out outputs: Fluid's eosH (src/fld.cpp:62-66), which
+ * the reference swaps in for all of outputSurface and outputCorona
+ * (SWAP_EOS, src/fld.cpp:39,540-542,870-872,885-887,1085-1087) -- the grid
+ * totals, the con2prim at the element centroids and the T, mu columns.  It
+ * applies to vh_fluid_stats, vh_freezeout, vh_freezeout_beta and vh_corona.
+ * NULL restores the hydro EoS (the default).  The arrays are copied.  A
+ * charge-dependent EoS (kinds 2, 3) needs a VH_PREC_FP64 context. */
+int vh_set_surface_eos(vh_ctx_t *ctx, const vh_eos_t *eos);
 
 /* The grid totals that Fluid::outputSurface and Fluid::outputCorona print
  * (src/fld.cpp:545-606, 852-857, 889-945, 1071-1084), summed over the

@@ -545,12 +545,13 @@ long or_freezeout(const or_fo_grid_t *g, const double *Q, const double *Qlow, co
                     QC[i] += (jt ? Q : Qlow)[i * N + cell(ix + jx, iy + jy, iz + jz)] * wT[jt] *
                              wX[jx] * wY[jy] * wZ[jz];
           for (int i = 0; i < 7; i++) QC[i] = QC[i] / (tau + c4[0]);
-          double pr[8], eo[4];
+          double pr[8], eo[6];
           or_c2p(eos, QC, 1, pr, nullptr);
           const double eC = pr[0], nbC = pr[2];
           double vxC = pr[5], vyC = pr[6], vzC = pr[7];
-          or_eos(eos, &eC, 1, eo);
-          const double TC = eo[1];
+          const double in4[4] = {eC, nbC, pr[3], pr[4]};  // src/fld.cpp:695-697
+          or_eos_eval(eos, in4, 1, eo);
+          const double TC = eo[1], mubC = eo[2], muqC = eo[3], musC = eo[4];
           double piC[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, PiC = 0.;
           for (int jx = 0; jx < 2; jx++)
             for (int jy = 0; jy < 2; jy++)
@@ -581,7 +582,9 @@ long or_freezeout(const or_fo_grid_t *g, const double *Q, const double *Qlow, co
           r[6] = tauC * n0[2];
           for (int i = 0; i < 4; i++) r[8 + i] = uC[i];
           r[12] = TC;
-          r[13] = r[14] = r[15] = 0.;
+          r[13] = mubC;
+          r[14] = muqC;
+          r[15] = musC;
           double *pc = r + 16;
           pc[sym(0, 0)] = ch * ch * piC[sym(0, 0)] + 2. * ch * sh * piC[sym(0, 3)] +
                           sh * sh * piC[sym(3, 3)];
@@ -670,12 +673,13 @@ long or_corona(const or_fo_grid_t *g, const double *Q, const double *pi, const d
               for (int i = 0; i < 7; i++) QC[i] += Q[i * N + c] * 0.125;
             }
         for (int i = 0; i < 7; i++) QC[i] = QC[i] / tau;
-        double pr[8], eo[4];
+        double pr[8], eo[6];
         or_c2p(eos, QC, 1, pr, nullptr);
         const double eC = pr[0], nbC = pr[2];
         if (!(corona && eC >= 0.01)) continue;
-        or_eos(eos, &eC, 1, eo);
-        const double pC = eo[0], TC = eo[1];
+        const double in4[4] = {eC, nbC, pr[3], pr[4]};  // src/fld.cpp:979-988
+        or_eos_eval(eos, in4, 1, eo);
+        const double pC = eo[0], TC = eo[1], mubC = eo[2], muqC = eo[3], musC = eo[4];
         double piC[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, PiC = 0.;
         for (int jx = 0; jx < 2; jx++)
           for (int jy = 0; jy < 2; jy++)
@@ -734,7 +738,9 @@ long or_corona(const or_fo_grid_t *g, const double *Q, const double *pi, const d
           for (int i = 0; i < 4; i++) r[4 + i] = ds[i];
           for (int i = 0; i < 4; i++) r[8 + i] = uC[i];
           r[12] = TC;
-          r[13] = r[14] = r[15] = 0.;
+          r[13] = mubC;
+          r[14] = muqC;
+          r[15] = musC;
           for (int i = 0; i < 10; i++) r[16 + i] = pc[i];
           r[26] = PiC;
           if (extend) {

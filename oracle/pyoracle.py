@@ -66,19 +66,63 @@ def make_trc(etaS=0.0, zetaS=0.0, etaSparam=0, zetaSparam=0, **kw):
     return Trc(**d)
 
 
+class OrChiral(ctypes.Structure):
+    _fields_ = [("ne", ctypes.c_int), ("nn", ctypes.c_int)] + [
+        (k, ctypes.c_double) for k in ("emin", "emax", "nmin", "nmax")] + [
+        (k, ctypes.POINTER(ctypes.c_double)) for k in ("p", "T", "mub", "mus")]
+
+
+class OrHadron(ctypes.Structure):
+    _fields_ = [("ne", ctypes.c_int), ("nnb", ctypes.c_int), ("nnq", ctypes.c_int)] + [
+        (k, ctypes.c_double) for k in ("e0", "n0", "logemin", "logemax", "lognmax")] + [
+        (k, ctypes.POINTER(ctypes.c_double)) for k in ("p", "T", "mub", "muq", "mus")] + [
+        ("status", ctypes.POINTER(ctypes.c_int))]
+
+
 class OrEos(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("n", ctypes.c_int)] + [
-        (k, ctypes.POINTER(ctypes.c_double)) for k in ("e", "p", "T", "mu")]
+        (k, ctypes.POINTER(ctypes.c_double)) for k in ("e", "p", "T", "mu")] + [
+        ("small", ctypes.POINTER(OrChiral)), ("big", ctypes.POINTER(OrChiral)),
+        ("hadron", ctypes.POINTER(OrHadron))]
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
 
 
 def _eos(table):
-    """(pointer to an or_eos_t or None, arrays kept alive) of a table (any
-    object with arrays e, p, T and optionally mu) or None (the ideal gas)."""
+    """(pointer to an or_eos_t or None, arrays kept alive) of a table or None
+    (the ideal gas).  A table is any object with arrays e, p, T [, mu] (the
+    spline EoS), or with `small` and `big` (the chiral EoS: each with ne, nn,
+    emin, emax, nmin, nmax and (ne, nn) arrays p, T, mub, mus), or with
+    e0, n0, logemin, logemax, lognmax and (ne, nnb, nnq) arrays p, T, mub,
+    muq, mus, status (the hadron EoS)."""
     if table is None:
         return None, []
-    arrs = [np.ascontiguousarray(getattr(table, k), dtype=np.float64) for k in ("e", "p", "T")]
+    if hasattr(table, "small"):
+        keep, tabs = [], []
+        for t in (table.small, table.big):
+            arrs = [_f64(getattr(t, k)) for k in ("p", "T", "mub", "mus")]
+            ne, nn = arrs[0].shape
+            c = OrChiral(ne, nn, t.emin, t.emax, t.nmin, t.nmax, *[_p(a) for a in arrs])
+            keep += arrs + [c]
+            tabs.append(c)
+        d = OrEos(2, 0, None, None, None, None, ctypes.pointer(tabs[0]),
+                  ctypes.pointer(tabs[1]), None)
+        return ctypes.byref(d), [d] + keep
+    if hasattr(table, "lognmax"):
+        arrs = [_f64(getattr(table, k)) for k in ("p", "T", "mub", "muq", "mus")]
+        st = np.ascontiguousarray(getattr(table, "status", np.zeros(arrs[0].shape)),
+                                  dtype=np.int32)
+        ne, nnb, nnq = arrs[0].shape
+        h = OrHadron(ne, nnb, nnq, table.e0, table.n0, table.logemin, table.logemax,
+                     table.lognmax, *[_p(a) for a in arrs],
+                     st.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        d = OrEos(3, 0, None, None, None, None, None, None, ctypes.pointer(h))
+        return ctypes.byref(d), [d, h, st] + arrs
+    arrs = [_f64(getattr(table, k)) for k in ("e", "p", "T")]
     mu = getattr(table, "mu", None)
-    arrs.append(None if mu is None else np.ascontiguousarray(mu, dtype=np.float64))
+    arrs.append(None if mu is None else _f64(mu))
     d = OrEos(1, len(arrs[0]), *[None if a is None else _p(a) for a in arrs])
     return ctypes.byref(d), [d] + arrs
 
@@ -115,6 +159,7 @@ def lib():
                              ctypes.POINTER(ctypes.c_int)]
         L.or_p2c.argtypes = [dp, ctypes.c_long, dp]
         L.or_eos.argtypes = [ctypes.POINTER(OrEos), dp, ctypes.c_long, dp]
+        L.or_eos_eval.argtypes = [ctypes.POINTER(OrEos), dp, ctypes.c_long, dp]
         L.or_eos_spline.argtypes = [ctypes.POINTER(OrEos), ctypes.c_int, dp, ctypes.c_long, dp]
         L.or_trcoeff.argtypes = [ctypes.POINTER(Trc), ctypes.POINTER(OrEos), dp, ctypes.c_long,
                                  dp]
@@ -162,6 +207,18 @@ def eos(e, table=None):
     out = np.zeros((e.shape[0], 4))
     d, keep = _eos(table)
     lib().or_eos(d, _p(e), e.shape[0], _p(out))
+    return out
+
+
+def eos_eval(table, e, nb=None, nq=None, ns=None):
+    """EoS::eos and EoS::s: n x 6 {p, T, mub, muq, mus, s} at (e, nb, nq, ns)"""
+    e = _f64(e)
+    z = np.zeros_like(e)
+    inp = _f64(np.stack([e, z if nb is None else nb, z if nq is None else nq,
+                         z if ns is None else ns], axis=1))
+    out = np.zeros((e.shape[0], 6))
+    d, keep = _eos(table)
+    lib().or_eos_eval(d, _p(inp), e.shape[0], _p(out))
     return out
 
 

@@ -103,12 +103,163 @@ struct Spline {
   }
 };
 
+// EoSaux (src/eoChiral.cpp:38-154): one uniform (e, nb) table of p, T, mu_b,
+// mu_s with bilinear interpolation; indices clamped to the table (so outside
+// it the end cell's bilinear form extrapolates); p < 0 clamped to 0; e < 0
+// gives zeros.
+struct ChiralAux {
+  int ne = 0, nn = 0;
+  double emin = 0., emax = 0., nmin = 0., nmax = 0.;
+  std::vector<double> ptab, Ttab, mubtab, mustab;  // [ie * nn + in]
+  void init(const or_eos_chiral_tab_t &t) {
+    ne = t.ne;
+    nn = t.nn;
+    emin = t.emin;
+    emax = t.emax;
+    nmin = t.nmin;
+    nmax = t.nmax;
+    const size_t M = (size_t)ne * nn;
+    ptab.assign(t.p, t.p + M);
+    Ttab.assign(t.T, t.T + M);
+    mubtab.assign(t.mub, t.mub + M);
+    mustab.assign(t.mus, t.mus + M);
+  }
+  // EoSaux::get (src/eoChiral.cpp:99-129); with only_p, EoSaux::p (:131-154)
+  void get(double e, double nb, double &p, double &T, double &mub, double &mus,
+           bool only_p) const {
+    if (e < 0.) {
+      T = mub = mus = p = 0.;
+      return;
+    }
+    const double de = (emax - emin) / (ne - 1);
+    const double dn = (nmax - nmin) / (nn - 1);
+    int ie = (int)((e - emin) / de);
+    int in = (int)((nb - nmin) / dn);
+    if (ie < 0) ie = 0;
+    if (in < 0) in = 0;
+    if (ie > ne - 2) ie = ne - 2;
+    if (in > nn - 2) in = nn - 2;
+    const double em = e - emin - ie * de;
+    const double nm = nb - nmin - in * dn;
+    const double we[2] = {1. - em / de, em / de};
+    const double wn[2] = {1. - nm / dn, nm / dn};
+    T = mub = mus = p = 0.0;
+    for (int je = 0; je < 2; je++)
+      for (int jn = 0; jn < 2; jn++) {
+        const size_t k = (size_t)(ie + je) * nn + (in + jn);
+        p += we[je] * wn[jn] * ptab[k];
+        if (only_p) continue;
+        T += we[je] * wn[jn] * Ttab[k];
+        mub += we[je] * wn[jn] * mubtab[k];
+        mus += we[je] * wn[jn] * mustab[k];
+      }
+    if (p < 0.0) p = 0.0;
+  }
+};
+
+// EoSHadron (src/eoHadron.cpp:13-208): a table of p, T, mu_b, mu_q, mu_s on
+// nodes uniform in log(e/e0) and in log-mapped nb, nq (linear near n = 0).
+struct HadronTable {
+  double e0 = 1., n0 = 1., logemin = 0., logemax = 0., lognmax = 0.;
+  int ne = 0, nnb = 0, nnq = 0;
+  double nb_abs_min = 0., nq_abs_min = 0., e_min = 0.;
+  std::vector<double> ptab, Ttab, mubtab, muqtab, mustab;
+  std::vector<int> statustab;
+  int index3(int ie, int inb, int inq) const { return inq + nnq * inb + nnq * nnb * ie; }
+  void init(const or_eos_hadron_tab_t &t) {
+    ne = t.ne;
+    nnb = t.nnb;
+    nnq = t.nnq;
+    e0 = t.e0;
+    n0 = t.n0;
+    logemin = t.logemin;
+    logemax = t.logemax;
+    lognmax = t.lognmax;
+    const size_t M = (size_t)ne * nnb * nnq;
+    ptab.assign(t.p, t.p + M);
+    Ttab.assign(t.T, t.T + M);
+    mubtab.assign(t.mub, t.mub + M);
+    muqtab.assign(t.muq, t.muq + M);
+    mustab.assign(t.mus, t.mus + M);
+    statustab.assign(M, 0);
+    if (t.status) statustab.assign(t.status, t.status + M);
+    // the constructor's derived constants (src/eoHadron.cpp:50-54)
+    const double dxnb = (2.0 * lognmax) / (nnb - 1);
+    const double dxnq = (2.0 * lognmax) / (nnq - 1);
+    nb_abs_min = n0 * std::exp(2.0 * dxnb - lognmax);
+    nq_abs_min = n0 * std::exp(2.0 * dxnq - lognmax);
+    e_min = e0 * std::exp(logemin);
+  }
+  // position of a charge density on its mapped axis (src/eoHadron.cpp:98-111)
+  static double xpos(double n, double n0, double lognmax, double dxn, double n_abs_min) {
+    if (n == 0.0) return 0.;
+    const double zn = std::log(std::fabs(n) / n0);
+    if (zn >= -lognmax + 2.0 * dxn) return n / std::fabs(n) * 0.5 * (zn + lognmax);
+    return n / n_abs_min * dxn;
+  }
+  // EoSHadron::eos (src/eoHadron.cpp:78-152); with only_p, EoSHadron::p (:154-208)
+  void get(double e, double nb, double nq, double &T, double &mub, double &muq, double &mus,
+           double &p, bool only_p) const {
+    if (e <= 0.) {
+      T = mub = muq = mus = p = 0.;
+      return;
+    }
+    if (e < e_min) {
+      p = e / e_min * ptab[index3(0, nnb / 2, nnq / 2)];
+      T = e / e_min * Ttab[index3(0, nnb / 2, nnq / 2)];
+      mub = muq = mus = 0.;
+      return;
+    }
+    const double xe = std::log(e / e0);
+    const double dxe = (logemax - logemin) / (ne - 1);
+    const double dxnb = (2.0 * lognmax) / (nnb - 1);
+    const double dxnq = (2.0 * lognmax) / (nnq - 1);
+    const double xnb = xpos(nb, n0, lognmax, dxnb, nb_abs_min);
+    const double xnq = xpos(nq, n0, lognmax, dxnq, nq_abs_min);
+    int ie = (int)((xe - logemin) / dxe);
+    int inb = (int)((xnb + lognmax) / dxnb);
+    int inq = (int)((xnq + lognmax) / dxnq);
+    if (ie < 0) ie = 0;
+    if (inb < 0) inb = 0;
+    if (inq < 0) inq = 0;
+    if (ie > ne - 2) ie = ne - 2;
+    if (inb > nnb - 2) inb = nnb - 2;
+    if (inq > nnq - 2) inq = nnq - 2;
+    const double em = xe - logemin - ie * dxe;
+    const double nbm = xnb + lognmax - inb * dxnb;
+    const double nqm = xnq + lognmax - inq * dxnq;
+    if (statustab[index3(ie, inb, inq)] == 1) {
+      T = mub = muq = mus = p = 0.;
+      return;
+    }
+    const double we[2] = {1. - em / dxe, em / dxe};
+    const double wnb[2] = {1. - nbm / dxnb, nbm / dxnb};
+    const double wnq[2] = {1. - nqm / dxnq, nqm / dxnq};
+    T = mub = muq = mus = p = 0.0;
+    for (int je = 0; je < 2; je++)
+      for (int jnb = 0; jnb < 2; jnb++)
+        for (int jnq = 0; jnq < 2; jnq++) {
+          const int k = index3(ie + je, inb + jnb, inq + jnq);
+          const double w = we[je] * wnb[jnb] * wnq[jnq];
+          p += w * ptab[k];
+          if (only_p) continue;
+          T += w * Ttab[k];
+          mub += w * mubtab[k];
+          muq += w * muqtab[k];
+          mus += w * mustab[k];
+        }
+    if (p < 0.0) p = 0.0;
+  }
+};
+
 struct Eos {
   int kind = 0;
   // cnst of EoSs::t (src/eos.cpp:98-99); GCC folds pow(C_PI,2) and
   // pow(0.197326968,3) at compile time with correct rounding.
   double cnst;
   Spline sp, sT, smu;
+  ChiralAux small, big;
+  HadronTable had;
   mutable long out_of_range = 0;
   explicit Eos(const or_eos_t *d = nullptr) {
     const double pi_ = 3.14159265358979312;  // C_PI, src/inc.h:18
@@ -123,29 +274,72 @@ struct Eos {
       sp.init(d->e, p.data(), n);
       sT.init(d->e, d->T, n);
       smu.init(d->e, mu.data(), n);
+    } else if (d && d->kind == 2) {
+      kind = 2;
+      small.init(*d->small);
+      big.init(*d->big);
+    } else if (d && d->kind == 3) {
+      kind = 3;
+      had.init(*d->hadron);
     }
   }
-  double p(double e) const { return kind ? sp.eval(e, out_of_range) : e / 3.; }
+  // EoSs::p(e) / EoSs::t(e): the kinds whose p does not depend on the charges
+  double p(double e) const { return kind == 1 ? sp.eval(e, out_of_range) : e / 3.; }
   double T(double e) const {
-    if (kind) return sT.eval(e, out_of_range);
+    if (kind == 1) return sT.eval(e, out_of_range);
     return e > 0. ? 1.0 * std::pow(e / cnst, 0.25) : 0.;
   }
-  // EoS::eos for these EoSs: p, T, mu=0
-  void eos(double e, double &T_, double &mub, double &muq, double &mus,
-           double &p_) const {
-    p_ = p(e);
-    T_ = T(e);
-    mub = muq = mus = 0.;
+  // EoSChiral::eos (src/eoChiral.cpp:131-145) and ::p (:147-154)
+  void chiral(double e, double nb, double &T_, double &mub, double &muq, double &mus,
+              double &p_, bool only_p) const {
+    if (e < 1.46 && nb < 0.3)
+      small.get(e, nb, p_, T_, mub, mus, only_p);
+    else if (e < 146. && nb < 6.)
+      big.get(e, nb, p_, T_, mub, mus, only_p);
+    else {
+      p_ = 0.2964 * e;
+      T_ = only_p ? 0. : 0.15120476935 * std::pow(e, 0.25);
+      mub = mus = 0.0;
+    }
+    muq = 0.0;
+  }
+  // EoS::p(e, nb, nq, ns)
+  double p(double e, double nb, double nq, double ns) const {
+    double T_, mub, muq, mus, p_;
+    if (kind == 2) {
+      chiral(e, nb, T_, mub, muq, mus, p_, true);
+      return p_;
+    }
+    if (kind == 3) {
+      had.get(e, nb, nq, T_, mub, muq, mus, p_, true);
+      return p_;
+    }
+    return p(e);
+  }
+  // EoS::eos(e, nb, nq, ns, T, mub, muq, mus, p)
+  void eos(double e, double nb, double nq, double ns, double &T_, double &mub, double &muq,
+           double &mus, double &p_) const {
+    if (kind == 2) {
+      chiral(e, nb, T_, mub, muq, mus, p_, false);
+    } else if (kind == 3) {
+      had.get(e, nb, nq, T_, mub, muq, mus, p_, false);
+    } else {
+      p_ = p(e);
+      T_ = T(e);
+      mub = muq = mus = 0.;
+    }
   }
   // EoS::s (src/eos.cpp:18-25)
   double s(double e, double nb, double nq, double ns) const {
     double T_, mub, muq, mus, p_;
-    eos(e, T_, mub, muq, mus, p_);
+    eos(e, nb, nq, ns, T_, mub, muq, mus, p_);
     if (T_ > 0.0) return (e + p_ - mub * nb - muq * nq - mus * ns) / T_;
     return 0.;
   }
-  double cs2e(double e) const {  // EoSs::cs2(e) = dpe(e)
-    if (kind) return (p(e * 1.1) - p(e)) / (0.1 * e);
+  // EoSs::cs2(e) = dpe(e); the chiral and hadron EoSs keep EoS::cs2(e) = 1/3
+  // (src/eos.h:37)
+  double cs2e(double e) const {
+    if (kind == 1) return (p(e * 1.1) - p(e)) / (0.1 * e);
     return 1. / 3.;
   }
 };
@@ -177,7 +371,7 @@ void con2prim(const Eos &eos, double Q[7], double &e, double &p,
     nb = Q[kNB];
     nq = Q[kNQ];
     ns = Q[kNS];
-    p = eos.p(e);
+    p = eos.p(e, nb, nq, ns);
     return;
   }
   if (M > Q[kT]) {
@@ -192,7 +386,7 @@ void con2prim(const Eos &eos, double Q[7], double &e, double &p,
   nb = Q[kNB] * std::sqrt(1 - v * v);
   nq = Q[kNQ] * std::sqrt(1 - v * v);
   ns = Q[kNS] * std::sqrt(1 - v * v);
-  p = eos.p(e);
+  p = eos.p(e, nb, nq, ns);
   double f = (Q[kT] + p) * v - M;
   double df = (Q[kT] + p) - M * v * dpe;
   double dvold = vh - vl;
@@ -215,7 +409,7 @@ void con2prim(const Eos &eos, double Q[7], double &e, double &p,
     nb = Q[kNB] * std::sqrt(1 - v * v);
     nq = Q[kNQ] * std::sqrt(1 - v * v);
     ns = Q[kNS] * std::sqrt(1 - v * v);
-    p = eos.p(e);
+    p = eos.p(e, nb, nq, ns);
     f = (Q[kT] + p) * v - M;
     df = (Q[kT] + p) - M * v * dpe;
     if (f > 0.)
@@ -227,7 +421,7 @@ void con2prim(const Eos &eos, double Q[7], double &e, double &p,
   vy = v * Q[kY] / M;
   vz = v * Q[kZ] / M;
   e = Q[kT] - M * v;
-  p = eos.p(e);  // uses the previous iterate's nb (src/rmn.cpp:125-126)
+  p = eos.p(e, nb, nq, ns);  // the previous iterate's nb (src/rmn.cpp:125-126)
   nb = Q[kNB] * std::sqrt(1 - vx * vx - vy * vy - vz * vz);
   nq = Q[kNQ] * std::sqrt(1 - vx * vx - vy * vy - vz * vz);
   ns = Q[kNS] * std::sqrt(1 - vx * vx - vy * vy - vz * vz);
@@ -592,7 +786,7 @@ struct Solver {
     double e0 = p0[0], e1 = ph[0];  // e1 is overwritten by the H-centre state
     const double s = eos.s(ph[0], ph[2], ph[3], ph[4]);
     double T, mub, muq, mus, p;
-    eos.eos(ph[0], T, mub, muq, mus, p);
+    eos.eos(ph[0], ph[2], ph[3], ph[4], T, mub, muq, mus, p);
     double etaS, zetaS;
     trc.getEta(ph[0], ph[2], T, mub, s, p, etaS, zetaS);
     double ut0 = 1.0 / std::sqrt(1.0 - p0[5] * p0[5] - p0[6] * p0[6] - p0[7] * p0[7]);
@@ -601,8 +795,8 @@ struct Solver {
     double ux1 = ut1 * p1[5], uy1 = ut1 * p1[6], uz1 = ut1 * p1[7];
     double Tv0 = 0., Tv1 = 0., mu_[3], pv;
     if (db) {  // temperatures of the previous and current states (src/hdo.cpp:420-425)
-      eos.eos(p0[0], Tv0, mu_[0], mu_[1], mu_[2], pv);
-      eos.eos(p1[0], Tv1, mu_[0], mu_[1], mu_[2], pv);
+      eos.eos(p0[0], p0[2], p0[3], p0[4], Tv0, mu_[0], mu_[1], mu_[2], pv);
+      eos.eos(p1[0], p1[2], p1[3], p1[4], Tv1, mu_[0], mu_[1], mu_[2], pv);
       // d_tau (u^nu / T) (src/hdo.cpp:451-458)
       db[0][0] = (ut1 / Tv1 - ut0 / Tv0) / dt;
       db[0][1] = (ux1 / Tv1 - ux0 / Tv0) / dt;
@@ -637,8 +831,8 @@ struct Solver {
       e1 = a[0];
       e0 = b[0];
       if (db) {  // neighbour temperatures (src/hdo.cpp:474-481)
-        eos.eos(e1, Tv1, mu_[0], mu_[1], mu_[2], pv);
-        eos.eos(e0, Tv0, mu_[0], mu_[1], mu_[2], pv);
+        eos.eos(e1, a[2], a[3], a[4], Tv1, mu_[0], mu_[1], mu_[2], pv);
+        eos.eos(e0, b[2], b[3], b[4], Tv0, mu_[0], mu_[1], mu_[2], pv);
       }
       if (e1 > 0. && e0 > 0.) {
         ut0 = 1.0 / std::sqrt(1.0 - b[5] * b[5] - b[6] * b[6] - b[7] * b[7]);
@@ -758,7 +952,7 @@ struct Solver {
               for (int j = 0; j < 4; j++)
                 c.db[4 * i + j] = std::max(std::min(dbs[i][j], 100.), -100.);
           double T, mub, muq, mus, p;
-          eos.eos(e, T, mub, muq, mus, p);
+          eos.eos(e, nb, nq, ns, T, mub, muq, mus, p);
           double etaS, zetaS;
           const double s = eos.s(e, nb, nq, ns);
           trc.getEta(e, nb, T, mub, s, p, etaS, zetaS);
@@ -773,7 +967,7 @@ struct Solver {
           trc.getTau(e, nb, T, mub, s, p, taupi, tauPi);
           // getOther / getOtherBulk (src/trancoeff.h:28-37)
           double dpp = 4. / 3., tpp = 10. / 7., lpP = 6. / 5.;
-          double phi7 = 9. / 70. / eos.p(e);
+          double phi7 = 9. / 70. / eos.p(e, nb, nq, ns);
           if (std::isinf(phi7)) phi7 = 0.0;
           phi7 = phi7 / taupi;
           if (taupi < 0.5 * dt) dpp = tpp = lpP = phi7 = 0.0;
@@ -1186,7 +1380,7 @@ extern "C" {
 void *or_create(int nx, int ny, int nz, double minx, double maxx, double miny,
                 double maxy, double minz, double maxz, const or_eos_t *eos,
                 const or_trc_t *trc) {
-  if ((eos && eos->kind != 0 && eos->kind != 1) || nx < 1 || ny < 1 || nz < 1) return nullptr;
+  if ((eos && (eos->kind < 0 || eos->kind > 3)) || nx < 1 || ny < 1 || nz < 1) return nullptr;
   Solver *s = new Solver;
   s->eos = Eos(eos);
   Grid &g = s->g;
@@ -1290,7 +1484,7 @@ void or_set_prim(void *h, double tau, const double *e, const double *nb,
     OCell &c = s->g.c[n];
     // Cell::setPrimVar (src/cll.cpp:222-241)
     const double g2 = 1. / (1 - vx[n] * vx[n] - vy[n] * vy[n] - vz[n] * vz[n]);
-    const double p = s->eos.p(e[n]);
+    const double p = s->eos.p(e[n], nb[n], nq[n], ns[n]);
     c.Q[kT] = tau * (e[n] + p * (vx[n] * vx[n] + vy[n] * vy[n] + vz[n] * vz[n])) * g2;
     c.Q[kX] = tau * (e[n] + p) * vx[n] * g2;
     c.Q[kY] = tau * (e[n] + p) * vy[n] * g2;
@@ -1398,11 +1592,21 @@ void or_eos(const or_eos_t *d, const double *e, long n, double *out) {
   const Eos eos(d);
   for (long i = 0; i < n; i++) {
     double T, mub, muq, mus, p;
-    eos.eos(e[i], T, mub, muq, mus, p);
+    eos.eos(e[i], 0., 0., 0., T, mub, muq, mus, p);
     out[4 * i + 0] = p;
     out[4 * i + 1] = T;
     out[4 * i + 2] = eos.s(e[i], 0., 0., 0.);
     out[4 * i + 3] = eos.cs2e(e[i]);
+  }
+}
+
+void or_eos_eval(const or_eos_t *d, const double *in, long n, double *out) {
+  const Eos eos(d);
+  for (long i = 0; i < n; i++) {
+    const double *a = in + 4 * i;
+    double *o = out + 6 * i;
+    eos.eos(a[0], a[1], a[2], a[3], o[1], o[2], o[3], o[4], o[0]);
+    o[5] = eos.s(a[0], a[1], a[2], a[3]);
   }
 }
 

@@ -29,12 +29,28 @@ typedef struct {
 
 /* EoS (src/eos.h, src/eos.cpp): kind 0 the ideal gas p=e/3 (SIMPLE branch of
  * EoSs); kind 1 the TABLE branch: n rows of e (increasing), p (< 0 clamped to
- * 0), T, mu (may be NULL), natural cubic splines (gsl_interp_cspline).  A NULL
- * descriptor is the ideal gas. */
+ * 0), T, mu (may be NULL), natural cubic splines (gsl_interp_cspline); kind 2
+ * EoSChiral (src/eoChiral.cpp): two bilinear (e, n_b) tables and a power law;
+ * kind 3 EoSHadron (src/eoHadron.cpp): a trilinear table in log(e/e0) and
+ * the log-mapped n_b, n_q.  A NULL descriptor is the ideal gas.  The table
+ * layouts are those of vh_eos_chiral_tab_t / vh_eos_hadron_tab_t. */
+typedef struct {
+  int ne, nn;
+  double emin, emax, nmin, nmax;
+  const double *p, *T, *mub, *mus; /* [ie * nn + in] */
+} or_eos_chiral_tab_t;
+typedef struct {
+  int ne, nnb, nnq;
+  double e0, n0, logemin, logemax, lognmax;
+  const double *p, *T, *mub, *muq, *mus; /* [inq + nnq * (inb + nnb * ie)] */
+  const int *status;                     /* may be NULL (all 0) */
+} or_eos_hadron_tab_t;
 typedef struct {
   int kind;
   int n;
   const double *e, *p, *T, *mu;
+  const or_eos_chiral_tab_t *small, *big;
+  const or_eos_hadron_tab_t *hadron;
 } or_eos_t;
 
 /* Whole-state view, SoA host arrays (any pointer may be NULL on set: field
@@ -110,6 +126,9 @@ void or_c2p(const or_eos_t *eos, double *Q, long n, double *out, int *iters);
 void or_p2c(const double *prim, long n, double *Q);
 /* EoS relations (EoS::eos, EoS::s, cs2(e)): out n x 4 {p, T, s, cs2}. */
 void or_eos(const or_eos_t *eos, const double *e, long n, double *out);
+/* EoS::eos and EoS::s at n points: in n x 4 {e, nb, nq, ns} ->
+ * out n x 6 {p, T, mub, muq, mus, s} */
+void or_eos_eval(const or_eos_t *eos, const double *in, long n, double *out);
 /* the table's spline of f (0 p, 1 T, 2 mu) at n points (kind 1 only) */
 void or_eos_spline(const or_eos_t *eos, int f, const double *v, long n, double *out);
 /* TransportCoeff::getEta + getTau (src/trancoeff.cpp:19-100) on n states

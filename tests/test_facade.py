@@ -21,6 +21,7 @@ def _surface_counts(out):
 
 
 def _params(tmp_path, **kw):
+    kw.setdefault("eosTypeHadron", -1)  # the hydro EoS for the surface (no eos/ tables)
     lines = [f"{k} {v}" for k, v in kw.items()]
     lines.insert(0, "! test params")
     p = tmp_path / "params.txt"
@@ -29,15 +30,29 @@ def _params(tmp_path, **kw):
 
 
 def test_params_reader_echo_cpu(tmp_path):
-    """Unknown keys echo 'UUU', comments 'CCC' (src/main.cpp:139-152); a
-    tabulated eosType is refused before any device work."""
-    p = _params(tmp_path, eosType=1, nx=11)
+    """Unknown keys echo 'UUU', comments 'CCC' (src/main.cpp:139-152); an
+    eosType outside this build (2, AZH) is refused before any device work."""
+    p = _params(tmp_path, eosType=2, nx=11)
     r = subprocess.run([EXE, "-params", p, "-outputDir", str(tmp_path)], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode == 0
     assert "UUU unknownKey 3" in r.stdout
     assert "CCC ! test params" in r.stdout
-    assert "eosType 1" in r.stdout
+    assert "eosType 2" in r.stdout
+
+
+@pytest.mark.parametrize("kw,missing", [(dict(eosType=1), "eos/chiraleos.dat"),
+                                        (dict(eosType=5, eosTypeHadron=0),
+                                         "eos/eosHadronLog.dat")])
+def test_missing_chiral_or_hadron_tables_exit_1_cpu(tmp_path, kw, missing):
+    """EoSChiral() and EoSHadron(file) read their tables from eos/ under the
+    working directory; a missing one is reported and the run exits with 1
+    (src/eoChiral.cpp:55-59, src/eoHadron.cpp:15-18), before any device work."""
+    p = _params(tmp_path, nx=11, **kw)
+    r = subprocess.run([EXE, "-params", p, "-outputDir", str(tmp_path)], capture_output=True,
+                       text=True, timeout=60, cwd=str(tmp_path))
+    assert r.returncode == 1
+    assert "I/O error with " + missing in r.stdout
 
 
 def test_missing_eos_table_exits_1_cpu(tmp_path):

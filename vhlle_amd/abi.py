@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libvhlle_gpu.so")
 
 VH_OK, VH_ERR_ARG, VH_ERR_HIP, VH_ERR_NODEVICE, VH_ERR_DTAU = 0, -1, -2, -3, -4
 ST_NAN_FLUX, ST_BIG_E, ST_NAN_SOURCE, ST_NAN_PI, ST_EOS_RANGE = 1, 2, 4, 8, 16
-EOS_IDEAL, EOS_TABLE = 0, 1
+EOS_IDEAL, EOS_TABLE, EOS_CHIRAL, EOS_HADRON = 0, 1, 2, 3
 PREC_FP64, PREC_MIXED = 0, 1
 C2P_REFERENCE, C2P_LEARNED, C2P_CLOSED, C2P_NEWTON = 0, 1, 2, 3
 START_MLP, START_HALF = 0, 1
@@ -32,10 +32,28 @@ class Grid(ctypes.Structure):
         (n, ctypes.c_double) for n in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")]
 
 
+class ChiralTab(ctypes.Structure):
+    """vh_eos_chiral_tab_t: one EoSaux table of EoSChiral"""
+    _fields_ = [("ne", ctypes.c_int), ("nn", ctypes.c_int)] + [
+        (k, ctypes.c_double) for k in ("emin", "emax", "nmin", "nmax")] + [
+        (k, ctypes.POINTER(ctypes.c_double)) for k in ("p", "T", "mub", "mus")]
+
+
+class HadronTab(ctypes.Structure):
+    """vh_eos_hadron_tab_t: the EoSHadron table"""
+    _fields_ = [("ne", ctypes.c_int), ("nnb", ctypes.c_int), ("nnq", ctypes.c_int)] + [
+        (k, ctypes.c_double) for k in ("e0", "n0", "logemin", "logemax", "lognmax")] + [
+        (k, ctypes.POINTER(ctypes.c_double)) for k in ("p", "T", "mub", "muq", "mus")] + [
+        ("status", ctypes.POINTER(ctypes.c_int))]
+
+
 class Eos(ctypes.Structure):
-    """vh_eos_t: kind (EOS_IDEAL, EOS_TABLE) and the table's columns."""
+    """vh_eos_t: kind (EOS_IDEAL, EOS_TABLE, EOS_CHIRAL, EOS_HADRON) and the
+    kind's tables."""
     _fields_ = [("kind", ctypes.c_int), ("n", ctypes.c_int)] + [
-        (k, ctypes.POINTER(ctypes.c_double)) for k in ("e", "p", "T", "mu")]
+        (k, ctypes.POINTER(ctypes.c_double)) for k in ("e", "p", "T", "mu")] + [
+        ("small", ctypes.POINTER(ChiralTab)), ("big", ctypes.POINTER(ChiralTab)),
+        ("hadron", ctypes.POINTER(HadronTab))]
 
 
 class EosTable:
@@ -79,9 +97,111 @@ class EosTable:
         return y[i] + dx * (b[i] + dx * (c[i] + dx * d[i]))
 
 
+class ChiralTable:
+    """One EoSaux table of EoSChiral (src/eoChiral.cpp:38-84): fields p, T,
+    mub, mus as (ne, nn) arrays over the uniform nodes e in [emin, emax], n_b
+    in [nmin, nmax] (GeV/fm^3, 1/fm^3, GeV)."""
+
+    def __init__(self, emin, emax, nmin, nmax, p, T, mub, mus):
+        self.p, self.T, self.mub, self.mus = (np.ascontiguousarray(a, dtype=np.float64)
+                                              for a in (p, T, mub, mus))
+        self.ne, self.nn = self.p.shape
+        self.emin, self.emax, self.nmin, self.nmax = emin, emax, nmin, nmax
+        self._d = ChiralTab(self.ne, self.nn, emin, emax, nmin, nmax,
+                            *(_p(a) for a in (self.p, self.T, self.mub, self.mus)))
+
+    @classmethod
+    def read(cls, path, ne, nn):
+        """The reference's file format (EoSaux::EoSaux, src/eoChiral.cpp:38-84):
+        nn blocks of ne rows of T [MeV], mu_q [MeV], e [e_0], p [e_0], n [n_0],
+        s [n_0], mu_S [MeV], a placeholder; converted to GeV, GeV/fm^3, 1/fm^3
+        with the reference's factors (mu_b = 3 mu_q, e_0 = 0.146 GeV/fm^3,
+        n_0 = 0.15 /fm^3)."""
+        a = np.loadtxt(path).reshape(nn, ne, 8)
+        T = a[:, :, 0].T / 1000.0
+        mub = a[:, :, 1].T * 3.0 / 1000.0
+        p = a[:, :, 3].T * 0.146
+        mus = a[:, :, 6].T / 1000.0
+        e = a[0, :, 2] * 0.146
+        n = a[:, 0, 4] * 0.15
+        return cls(e[0], e[-1], n[0], n[-1], p, T, mub, mus)
+
+
+class EosChiral:
+    """EoSChiral (src/eoChiral.cpp:131-154): `small` where e < 1.46 and
+    n_b < 0.3, `big` where e < 146 and n_b < 6, else p = 0.2964 e.  FP64
+    contexts only."""
+
+    def __init__(self, small, big):
+        self.small, self.big = small, big
+        self._d = Eos(EOS_CHIRAL, 0, None, None, None, None,
+                      ctypes.pointer(small._d), ctypes.pointer(big._d), None)
+
+    @classmethod
+    def read(cls, big_path, small_path):
+        """EoSChiral() reads eos/chiraleos.dat (2001 x 401) and
+        eos/chiralsmall.dat (201 x 201)."""
+        return cls(ChiralTable.read(small_path, 201, 201), ChiralTable.read(big_path, 2001, 401))
+
+    def desc(self):
+        return self._d
+
+
+class EosHadron:
+    """EoSHadron (src/eoHadron.cpp): fields p, T, mub, muq, mus and the node
+    status as (ne, nnb, nnq) arrays; nodes uniform in log(e/e0) over [logemin,
+    logemax] and in the log-mapped n_b, n_q over [-lognmax, lognmax].  FP64
+    contexts only."""
+
+    def __init__(self, e0, n0, logemin, logemax, lognmax, p, T, mub, muq, mus, status=None):
+        self.p, self.T, self.mub, self.muq, self.mus = (
+            np.ascontiguousarray(a, dtype=np.float64) for a in (p, T, mub, muq, mus))
+        self.ne, self.nnb, self.nnq = self.p.shape
+        self.status = np.ascontiguousarray(np.zeros(self.p.shape) if status is None else status,
+                                           dtype=np.int32)
+        self.e0, self.n0, self.logemin, self.logemax, self.lognmax = (
+            e0, n0, logemin, logemax, lognmax)
+        self._t = HadronTab(self.ne, self.nnb, self.nnq, e0, n0, logemin, logemax, lognmax,
+                            *(_p(a) for a in (self.p, self.T, self.mub, self.muq, self.mus)),
+                            self.status.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        self._d = Eos(EOS_HADRON, 0, None, None, None, None, None, None, ctypes.pointer(self._t))
+
+    @classmethod
+    def read(cls, path):
+        """The reference's file format (EoSHadron::EoSHadron,
+        src/eoHadron.cpp:13-65): a header ne nnb nnq e0 n0 logemin logemax
+        lognmax, then rows e nb nq p T mub muq mus status in (ie, inb, inq)
+        order."""
+        with open(path) as f:
+            head = f.readline().split()
+            while len(head) < 8:
+                head += f.readline().split()
+            ne, nnb, nnq = (int(v) for v in head[:3])
+            e0, n0, logemin, logemax, lognmax = (float(v) for v in head[3:8])
+            a = np.loadtxt(f, ndmin=2).reshape(ne, nnb, nnq, 9)
+        return cls(e0, n0, logemin, logemax, lognmax, *(a[..., k] for k in range(3, 8)),
+                   status=a[..., 8].astype(np.int32))
+
+    def desc(self):
+        return self._d
+
+
 def eos_desc(eos):
-    """vh_eos_t of None (ideal gas) or an EosTable"""
+    """vh_eos_t of None (ideal gas), an EosTable, an EosChiral or an EosHadron"""
     return Eos(EOS_IDEAL, 0, None, None, None, None) if eos is None else eos.desc()
+
+
+def eos_eval(eos, e, nb=None, nq=None, ns=None, on_gpu=True):
+    """vh_eos_eval: n x 6 {p, T, mu_b, mu_q, mu_s, s} at (e, nb, nq, ns)"""
+    e = np.asarray(e, dtype=np.float64)
+    z = np.zeros_like(e)
+    inp = np.ascontiguousarray(np.stack([e, z if nb is None else nb, z if nq is None else nq,
+                                         z if ns is None else ns], axis=1), dtype=np.float64)
+    out = np.zeros((len(e), 6))
+    d = eos_desc(eos)
+    check(lib().vh_eos_eval(ctypes.byref(d), _p(inp), len(e), _p(out), 1 if on_gpu else 0),
+          "vh_eos_eval")
+    return out
 
 
 class Trc(ctypes.Structure):
@@ -205,6 +325,8 @@ def lib():
                                              _dp, ctypes.c_long, ctypes.POINTER(ctypes.c_long),
                                              ctypes.POINTER(FoStats)]),
         "vh_eos_table": (ctypes.c_int, [ctypes.POINTER(Eos), _dp]),
+        "vh_eos_eval": (ctypes.c_int, [ctypes.POINTER(Eos), _dp, ctypes.c_long, _dp, ctypes.c_int]),
+        "vh_set_surface_eos": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Eos)]),
         "vh_create_batch": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Grid), ctypes.POINTER(Eos),
                                            ctypes.POINTER(Trc), ctypes.c_int, ctypes.POINTER(vp)]),
     }
@@ -223,7 +345,7 @@ def exported_symbols():
             "vh_mlp_forward", "vh_mlp_grad", "vh_mlp_train", "vh_c2p_learned",
             "vh_set_c2p_model", "vh_mlp_harvest", "vh_set_stage", "vh_get_stage", "vh_create_batch", "vh_hotspot_ic",
             "vh_copy_bandwidth", "vh_fo_snapshot", "vh_freezeout", "vh_cornelius",
-            "vh_eos_table", "vh_set_vorticity", "vh_get_dbeta", "vh_freezeout_beta",
+            "vh_eos_table", "vh_eos_eval", "vh_set_surface_eos", "vh_set_vorticity", "vh_get_dbeta", "vh_freezeout_beta",
             "vh_corona", "vh_prim_fields", "vh_fluid_stats", "vh_cells", "vh_signal_speed",
             "vh_cells_noalloc"]
 
@@ -532,6 +654,13 @@ class Context:
 
     def set_vorticity(self, on=True):
         check(lib().vh_set_vorticity(self.h, int(on)), "vh_set_vorticity")
+
+    def set_surface_eos(self, eos):
+        """vh_set_surface_eos: the EoS of the freeze-out outputs (None: the
+        hydro EoS)"""
+        self._surface_eos = eos  # keeps the tables alive
+        d = None if eos is None else ctypes.byref(eos.desc())
+        check(lib().vh_set_surface_eos(self.h, d), "vh_set_surface_eos")
 
     def get_dbeta(self):
         """Cell::getDbeta of every cell: (16, N), row 4 i + j = d_i (u^j / T)"""

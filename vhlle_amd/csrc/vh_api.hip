@@ -71,7 +71,10 @@ struct KTime {
 struct vh_ctx {
   vh::Geom g;
   vh::DevEos eos;
-  double *eos_buf = nullptr;  // spline table of a VH_EOS_TABLE EoS (device)
+  double *eos_buf = nullptr;  // tables of a VH_EOS_TABLE/CHIRAL/HADRON EoS (device)
+  vh::DevEos eos_fo;          // surface EoS of the freeze-out outputs (vh_set_surface_eos)
+  double *eos_fo_buf = nullptr;
+  bool has_eos_fo = false;
   vh::fo::Work fo_work;       // freeze-out buffers, kept between calls
   vh::DevTrc trc;
   vh_grid_t grid;
@@ -493,6 +496,12 @@ int create_impl(const vh_grid_t *grids, int ngrid, const vh_eos_t *eos, const vh
     std::vector<double> tab;
     int n = 0;
     if (vh::eos_table_host(*eos, tab, n) != VH_OK) return VH_ERR_ARG;
+  } else if (eos->kind == VH_EOS_CHIRAL || eos->kind == VH_EOS_HADRON) {
+    // charge-dependent EoSs: the FP64 path only
+    if (precision != VH_PREC_FP64) return VH_ERR_ARG;
+    vh::DevEos de = vh::DevEos();
+    de.kind = eos->kind;
+    if (vh::charged_tables_host(*eos, de, nullptr, nullptr) != VH_OK) return VH_ERR_ARG;
   } else if (eos->kind != VH_EOS_IDEAL) {
     return VH_ERR_ARG;
   }
@@ -659,6 +668,7 @@ int vh_destroy(vh_ctx_t *c) {
   if (c->vscr) (void)hipFree(c->vscr);
   if (c->mlp) (void)hipFree(c->mlp);
   if (c->eos_buf) (void)hipFree(c->eos_buf);
+  if (c->eos_fo_buf) (void)hipFree(c->eos_fo_buf);
   c->fo_work.release();
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
@@ -1189,10 +1199,38 @@ int vh_prim_fields(vh_ctx_t *c, double tau, double *prim) {
   return VH_OK;
 }
 
+namespace {
+// the EoS of outputSurface / outputCorona: Fluid's eosH when set
+const vh::DevEos &surface_eos(const vh_ctx_t *c) { return c->has_eos_fo ? c->eos_fo : c->eos; }
+}  // namespace
+
+int vh_set_surface_eos(vh_ctx_t *c, const vh_eos_t *eos) {
+  if (!c) return VH_ERR_ARG;
+  if (eos && eos->kind != VH_EOS_IDEAL && eos->kind != VH_EOS_TABLE &&
+      eos->kind != VH_EOS_CHIRAL && eos->kind != VH_EOS_HADRON)
+    return VH_ERR_ARG;
+  // the mixed path's FP32 pi goes through the charge-free kernels only
+  if (eos && (eos->kind == VH_EOS_CHIRAL || eos->kind == VH_EOS_HADRON) &&
+      c->precision != VH_PREC_FP64)
+    return VH_ERR_ARG;
+  VH_HIP(hipStreamSynchronize(c->st));
+  vh::DevEos de;
+  double *buf = nullptr;
+  if (eos) {
+    const int rc = vh::make_dev_eos(eos, c->status, de, &buf);
+    if (rc != VH_OK) return rc;
+  }
+  if (c->eos_fo_buf) (void)hipFree(c->eos_fo_buf);
+  c->eos_fo_buf = buf;
+  c->eos_fo = de;
+  c->has_eos_fo = eos != nullptr;
+  return VH_OK;
+}
+
 int vh_fluid_stats(vh_ctx_t *c, double tau, double e_crit, int central, double *out) {
   if (!c || !out || c->evh.size() != 1 || !(tau > 0.)) return VH_ERR_ARG;
   if (!c->fsb) VH_HIP(hipMalloc(&c->fsb, (16 * 256 + 16) * sizeof(double)));
-  vh::launch_fluid_stats(c->g, c->eos, c->Q[c->cur], c->pi, c->precision == VH_PREC_MIXED,
+  vh::launch_fluid_stats(c->g, surface_eos(c), c->Q[c->cur], c->pi, c->precision == VH_PREC_MIXED,
                          c->vcc, tau, c->grid.zmin, e_crit, c->viscous ? 1 : 0, central ? 1 : 0,
                          c->fsb, c->fsb + 16 * 256, c->st);
   VH_HIP(hipGetLastError());
@@ -1290,7 +1328,7 @@ int vh_freezeout_beta(vh_ctx_t *c, double tau, double dt, double e_crit, int ext
   a.nx = c->g.nx;
   a.ny = c->g.ny;
   a.nz = c->g.nz;
-  a.eos = c->eos;
+  a.eos = surface_eos(c);
   a.P.tau = tau;
   a.P.dt = edge;
   a.P.dx = c->g.dx;
@@ -1348,7 +1386,7 @@ int vh_corona(vh_ctx_t *c, double tau, double e_crit, int extend, double *rows, 
   a.nx = c->g.nx;
   a.ny = c->g.ny;
   a.nz = c->g.nz;
-  a.eos = c->eos;
+  a.eos = surface_eos(c);
   a.P.tau = tau;
   a.P.dt = 0.;
   a.P.dx = c->g.dx;

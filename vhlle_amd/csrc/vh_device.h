@@ -104,17 +104,172 @@ __device__ inline double div3(double e) {
   return fma(r, y, q);
 }
 
-// Device EoS (kind VH_EOS_IDEAL or VH_EOS_TABLE, include/vhlle_gpu.h).
+// One table of EoSChiral (the auxiliary class EoSaux, src/eoChiral.cpp:38-154):
+// ne x nn nodes uniform in e and n_b; the fields p, T, mu_b, mu_s are each
+// ne*nn long, at [ie*nn + in].  de and dn are (emax-emin)/(ne-1) and
+// (nmax-nmin)/(nn-1), formed on the host as the reference forms them per call.
+// Written for host and device, so the C++ facade evaluates the same lookups.
+struct ChiralTab {
+  const double *p, *T, *mub, *mus;
+  int ne, nn;
+  double emin, nmin, de, dn;
+  // EoSaux::get (ALL) and EoSaux::p (src/eoChiral.cpp:99-154), in the
+  // reference's operation order: the bilinear sums run over (je, jn) from 0.0.
+  template <bool ALL>
+  __host__ __device__ __forceinline__ void get(double e, double nb, double &p_, double &T_, double &mub_,
+                                      double &mus_) const {
+    p_ = T_ = mub_ = mus_ = 0.;
+    if (e < 0.) return;
+    int ie = (int)((e - emin) / de);
+    int in = (int)((nb - nmin) / dn);
+    if (ie < 0) ie = 0;
+    if (in < 0) in = 0;
+    if (ie > ne - 2) ie = ne - 2;
+    if (in > nn - 2) in = nn - 2;
+    const double em = e - emin - ie * de;
+    const double nm = nb - nmin - in * dn;
+    const double we[2] = {1. - em / de, em / de};
+    const double wn[2] = {1. - nm / dn, nm / dn};
+    double sp = 0.0, sT = 0.0, sb = 0.0, ss = 0.0;
+#pragma unroll
+    for (int je = 0; je < 2; je++)
+#pragma unroll
+      for (int jn = 0; jn < 2; jn++) {
+        const long k = (long)(ie + je) * nn + (in + jn);
+        const double w = we[je] * wn[jn];
+        sp += w * p[k];
+        if (ALL) {
+          sT += w * T[k];
+          sb += w * mub[k];
+          ss += w * mus[k];
+        }
+      }
+    if (sp < 0.0) sp = 0.0;
+    p_ = sp;
+    T_ = sT;
+    mub_ = sb;
+    mus_ = ss;
+  }
+};
+
+// The table of EoSHadron (src/eoHadron.cpp:13-208): ne x nnb x nnq nodes
+// uniform in x_e = log(e/e0) and in the log-mapped x_nb, x_nq; the fields p,
+// T, mu_b, mu_q, mu_s are each ne*nnb*nnq long, in index3 order inq +
+// nnq*(inb + nnb*ie); status holds the node flags (null: none set).  dxe, dxnb, dxnq, nb_abs_min, nq_abs_min
+// and e_min are formed on the host as the reference's constructor does.
+struct HadronTab {
+  const double *p, *T, *mub, *muq, *mus;
+  const int *status;
+  int ne, nnb, nnq;
+  double e0, n0, logemin, lognmax, dxe, dxnb, dxnq, nb_abs_min, nq_abs_min, e_min;
+  __host__ __device__ inline long index3(int ie, int inb, int inq) const {
+    return inq + (long)nnq * inb + (long)nnq * nnb * ie;
+  }
+  // inverse of the log mapping of a charge density (src/eoHadron.cpp:98-111)
+  __host__ __device__ static inline double xmap(double n, double n0, double lognmax, double dx,
+                                                double n_abs_min) {
+    if (n != 0.0) {
+      const double z = log(fabs(n) / n0);
+      if (z >= -lognmax + 2.0 * dx) return n / fabs(n) * 0.5 * (z + lognmax);
+      return n / n_abs_min * dx;
+    }
+    return 0.;
+  }
+  // EoSHadron::eos (ALL) and EoSHadron::p (src/eoHadron.cpp:78-208)
+  template <bool ALL>
+  __host__ __device__ __forceinline__ void get(double e, double nb, double nq, double &p_, double &T_,
+                                      double &mub_, double &muq_, double &mus_) const {
+    p_ = T_ = mub_ = muq_ = mus_ = 0.;
+    if (e <= 0.) return;
+    if (e < e_min) {
+      const long k0 = index3(0, nnb / 2, nnq / 2);
+      p_ = e / e_min * p[k0];
+      if (ALL) T_ = e / e_min * T[k0];
+      return;
+    }
+    const double xe = log(e / e0);
+    const double xnb = xmap(nb, n0, lognmax, dxnb, nb_abs_min);
+    const double xnq = xmap(nq, n0, lognmax, dxnq, nq_abs_min);
+    int ie = (int)((xe - logemin) / dxe);
+    int inb = (int)((xnb + lognmax) / dxnb);
+    int inq = (int)((xnq + lognmax) / dxnq);
+    if (ie < 0) ie = 0;
+    if (inb < 0) inb = 0;
+    if (inq < 0) inq = 0;
+    if (ie > ne - 2) ie = ne - 2;
+    if (inb > nnb - 2) inb = nnb - 2;
+    if (inq > nnq - 2) inq = nnq - 2;
+    const double em = xe - logemin - ie * dxe;
+    const double nbm = xnb + lognmax - inb * dxnb;
+    const double nqm = xnq + lognmax - inq * dxnq;
+    if (status && status[index3(ie, inb, inq)] == 1) return;
+    const double we[2] = {1. - em / dxe, em / dxe};
+    const double wnb[2] = {1. - nbm / dxnb, nbm / dxnb};
+    const double wnq[2] = {1. - nqm / dxnq, nqm / dxnq};
+    double sp = 0.0, sT = 0.0, sb = 0.0, sq = 0.0, ss = 0.0;
+#pragma unroll
+    for (int je = 0; je < 2; je++)
+#pragma unroll
+      for (int jnb = 0; jnb < 2; jnb++)
+#pragma unroll
+        for (int jnq = 0; jnq < 2; jnq++) {
+          const long k = index3(ie + je, inb + jnb, inq + jnq);
+          const double w = we[je] * wnb[jnb] * wnq[jnq];
+          sp += w * p[k];
+          if (ALL) {
+            sT += w * T[k];
+            sb += w * mub[k];
+            sq += w * muq[k];
+            ss += w * mus[k];
+          }
+        }
+    if (sp < 0.0) sp = 0.0;
+    p_ = sp;
+    T_ = sT;
+    mub_ = sb;
+    muq_ = sq;
+    mus_ = ss;
+  }
+};
+
+// EoSChiral::eos / EoSChiral::p (src/eoChiral.cpp:131-154): the table by
+// region, else the power law; mu_q = 0.
+template <bool ALL>
+__host__ __device__ __forceinline__ void chiral_eos(const ChiralTab &small, const ChiralTab &big,
+                                                   double e, double nb, double &p, double &T,
+                                                   double &mub, double &muq, double &mus) {
+  if (e < 1.46 && nb < 0.3) {
+    small.get<ALL>(e, nb, p, T, mub, mus);
+  } else if (e < 146. && nb < 6.) {
+    big.get<ALL>(e, nb, p, T, mub, mus);
+  } else {
+    p = 0.2964 * e;
+    T = ALL ? 0.15120476935 * pow(e, 0.25) : 0.;
+    mub = mus = 0.0;
+  }
+  muq = 0.0;
+}
+
+// Device EoS (kinds VH_EOS_IDEAL, VH_EOS_TABLE, VH_EOS_CHIRAL, VH_EOS_HADRON,
+// include/vhlle_gpu.h).
 // Ideal massless gas: the SIMPLE branch of EoSs (src/eos.cpp:87-111), cnst
 // computed on the host exactly as the reference's compile-time constant.
 // Table: the TABLE branch (src/eos.cpp:27-120), p, T, mu splines of e; an
 // evaluation outside the table raises VH_ST_EOS_RANGE in *st (the reference
 // aborts there) and uses the end interval's cubic.
+// Chiral / hadron: the bilinear and log-trilinear lookups above; these
+// depend on the charge densities.
+// p(e), T(e) and the FP32 forms are the charge-free evaluations of kinds 0
+// and 1 (the mixed-precision kernels, which reject kinds 2 and 3); the FP64
+// path calls p(e, nb, nq, ns), eos(...) and s(...), which hold for every kind.
 struct DevEos {
   int kind;
   double cnst;
   SplineTab tab;
+  ChiralTab small, big;
+  HadronTab had;
   int *st;  // status word for VH_ST_EOS_RANGE (may be null)
+  __host__ __device__ inline bool charged() const { return kind >= VH_EOS_CHIRAL; }
   __device__ inline double spline(int f, double e) const {
     bool out;
     const double v = tab.eval(f, e, out);
@@ -126,10 +281,50 @@ struct DevEos {
     if (kind == VH_EOS_TABLE) return spline(1, e);
     return e > 0. ? 1.0 * pow(e / cnst, 0.25) : 0.;
   }
-  // EoS::s (src/eos.cpp:18-25) with mu = 0
+  // The charge-aware forms, compiled per class of EoS: CH = false for kinds 0
+  // and 1 (p, T of e alone, mu = 0), CH = true for kinds 2 and 3.  A kernel
+  // that evaluates them is instantiated for both classes and launched with
+  // CH = charged(), so that the lookups of kinds 2 and 3 cost the other
+  // kinds' kernels no registers.
+  // EoS::p(e, nb, nq, ns)
+  template <bool CH>
+  __device__ double p(double e, double nb, double nq, double ns) const {
+    if (!CH) return p(e);
+    if (kind == VH_EOS_CHIRAL) {
+      double p_, T_, mub, muq, mus;
+      chiral_eos<false>(small, big, e, nb, p_, T_, mub, muq, mus);
+      return p_;
+    }
+    double p_, T_, mub, muq, mus;
+    had.get<false>(e, nb, nq, p_, T_, mub, muq, mus);
+    return p_;
+  }
+  // EoS::eos(e, nb, nq, ns, T, mub, muq, mus, p)
+  template <bool CH>
+  __device__ void eos(double e, double nb, double nq, double ns, double &T_, double &mub,
+                      double &muq, double &mus, double &p_) const {
+    if (!CH) {
+      p_ = p(e);
+      T_ = T(e);
+      mub = muq = mus = 0.;
+    } else if (kind == VH_EOS_CHIRAL) {
+      chiral_eos<true>(small, big, e, nb, p_, T_, mub, muq, mus);
+    } else {
+      had.get<true>(e, nb, nq, p_, T_, mub, muq, mus);
+    }
+  }
+  template <bool CH>
+  __device__ double T(double e, double nb, double nq, double ns) const {
+    if (!CH) return T(e);
+    double T_, mub, muq, mus, p_;
+    eos<CH>(e, nb, nq, ns, T_, mub, muq, mus, p_);
+    return T_;
+  }
+  // EoS::s (src/eos.cpp:18-25)
+  template <bool CH>
   __device__ double s(double e, double nb, double nq, double ns) const {
-    const double p_ = p(e), T_ = T(e);
-    const double mub = 0., muq = 0., mus = 0.;
+    double T_, mub, muq, mus, p_;
+    eos<CH>(e, nb, nq, ns, T_, mub, muq, mus, p_);
     if (T_ > 0.0) return (e + p_ - mub * nb - muq * nq - mus * ns) / T_;
     return 0.;
   }
@@ -146,7 +341,9 @@ struct DevEos {
     dp = 1. / 3.;
     return div3(e);
   }
-  // EoSs::cs2(e) = dpe(e) (src/eos.cpp:89-96; 1/3 for the ideal gas)
+  // EoSs::cs2(e) = dpe(e) (src/eos.cpp:89-96); 1/3 for the ideal gas and for
+  // the chiral and hadron EoSs, which keep the base class's cs2(e)
+  // (src/eos.h:37)
   __device__ double cs2e(double e) const {
     if (kind == VH_EOS_TABLE) return (spline(0, e * 1.1) - spline(0, e)) / (0.1 * e);
     return 1. / 3.;
@@ -174,7 +371,12 @@ struct Prim {
 // transformPV (src/rmn.cpp:28-139), FP64, reference operation order.  The
 // input q is rescaled in place when |q_vec| > q_t (src/rmn.cpp:65-70).
 // Returns the number of loop iterations executed.  Written on scalar locals
-// with one exit so that nothing is spilled to scratch memory.
+// with one exit so that nothing is spilled to scratch memory.  For an EoS
+// that depends on the charges, n = N sqrt(1 - v^2) is refreshed with every
+// pressure evaluation, and the final pressure uses the last iterate's n, as
+// the reference does (src/rmn.cpp:74-78,99-103,122-126); for the others p(e)
+// alone is evaluated (n does not enter).  CH: the class of the EoS (DevEos).
+template <bool CH>
 __device__ __forceinline__ int c2p_ref(const DevEos &eos, double q[7], Prim &r) {
   const double dpe = 1. / 3.;
   const double corrf = 0.9999;
@@ -190,7 +392,7 @@ __device__ __forceinline__ int c2p_ref(const DevEos &eos, double q[7], Prim &r) 
     nb = Qb;
     nq = Qq;
     ns = Qs;
-    p = eos.p(e);
+    p = eos.p<CH>(e, nb, nq, ns);
   } else {
     if (M > Qt) {
       Qx *= corrf * Qt / M;
@@ -198,11 +400,18 @@ __device__ __forceinline__ int c2p_ref(const DevEos &eos, double q[7], Prim &r) 
       Qz *= corrf * Qt / M;
       M = Qt * corrf;
     }
+    constexpr bool chg = CH;
     double vl = 0., vh = 1.;
     double v = 0.5 * (vl + vh);
+    double vn = v;  // the iterate of the last pressure evaluation
     e = Qt - M * v;
     if (e < 0.) e = 0.;
-    p = eos.p(e);
+    if (chg) {
+      const double w = sqrt(1 - v * v);
+      p = eos.p<CH>(e, Qb * w, Qq * w, Qs * w);
+    } else {
+      p = eos.p(e);
+    }
     double f = (Qt + p) * v - M;
     double df = (Qt + p) - M * v * dpe;
     double dvold = vh - vl;
@@ -222,7 +431,13 @@ __device__ __forceinline__ int c2p_ref(const DevEos &eos, double q[7], Prim &r) 
       if (fabs(dv) < 0.00001) break;
       e = Qt - M * v;
       if (e < 0.) e = 0.;
-      p = eos.p(e);  // p(e, n) of this EoS does not depend on n
+      if (chg) {
+        const double w = sqrt(1 - v * v);
+        p = eos.p<CH>(e, Qb * w, Qq * w, Qs * w);
+        vn = v;
+      } else {
+        p = eos.p(e);
+      }
       f = (Qt + p) * v - M;
       df = (Qt + p) - M * v * dpe;
       if (f > 0.)
@@ -234,7 +449,12 @@ __device__ __forceinline__ int c2p_ref(const DevEos &eos, double q[7], Prim &r) 
     vy = v * Qy / M;
     vz = v * Qz / M;
     e = Qt - M * v;
-    p = eos.p(e);
+    if (chg) {
+      const double w = sqrt(1 - vn * vn);
+      p = eos.p<CH>(e, Qb * w, Qq * w, Qs * w);
+    } else {
+      p = eos.p(e);
+    }
     const double s2 = sqrt(1 - vx * vx - vy * vy - vz * vz);
     nb = Qb * s2;
     nq = Qq * s2;

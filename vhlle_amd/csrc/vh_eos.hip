@@ -1,8 +1,13 @@
-// Tabulated EoS on the host: the TABLE branch of EoSs (src/eos.cpp:27-74).
-// The reference reads columns e, p, T [, mu], clamps p < 0 to 0, and builds
-// GSL natural cubic splines (gsl_interp_cspline) of p(e), T(e), mu(e).  The
-// spline coefficients below follow the published natural-spline construction
-// of that routine: c_i = y''(x_i)/2 solves, for i = 1 .. n-2,
+// Host side of the tabulated EoSs: validation, the natural cubic spline
+// coefficients of EoSs (VH_EOS_TABLE), the device copies of the EoSChiral and
+// EoSHadron tables (VH_EOS_CHIRAL, VH_EOS_HADRON) used by DevEos, and
+// vh_eos_eval.
+//
+// The TABLE branch of EoSs (src/eos.cpp:27-74) reads columns e, p, T [, mu],
+// clamps p < 0 to 0, and builds GSL natural cubic splines (gsl_interp_cspline)
+// of p(e), T(e), mu(e).  The spline coefficients below follow the published
+// natural-spline construction of that routine: c_i = y''(x_i)/2 solves, for
+// i = 1 .. n-2,
 //   h_i-1 c_i-1 + 2 (h_i-1 + h_i) c_i + h_i c_i+1
 //     = 3 ((y_i+1 - y_i)/h_i - (y_i - y_i-1)/h_i-1),   c_0 = c_n-1 = 0,
 // by the symmetric tridiagonal LDL^T recurrence, and on each interval
@@ -75,6 +80,115 @@ int eos_table_host(const vh_eos_t &eos, std::vector<double> &buf, int &n) {
   return VH_OK;
 }
 
+namespace {
+
+bool chiral_ok(const vh_eos_chiral_tab_t *t) {
+  return t && t->ne >= 2 && t->nn >= 2 && t->emax > t->emin && t->nmax > t->nmin && t->p &&
+         t->T && t->mub && t->mus;
+}
+
+// the scalars of one EoSaux table, its fields pointing at the descriptor's
+void chiral_scalars(const vh_eos_chiral_tab_t &t, ChiralTab &c) {
+  c.ne = t.ne;
+  c.nn = t.nn;
+  c.emin = t.emin;
+  c.nmin = t.nmin;
+  c.de = (t.emax - t.emin) / (t.ne - 1);  // src/eoChiral.cpp:104-105
+  c.dn = (t.nmax - t.nmin) / (t.nn - 1);
+  c.p = t.p;
+  c.T = t.T;
+  c.mub = t.mub;
+  c.mus = t.mus;
+}
+
+void chiral_append(const vh_eos_chiral_tab_t &t, std::vector<double> &v) {
+  const size_t M = (size_t)t.ne * t.nn;
+  for (const double *f : {t.p, t.T, t.mub, t.mus}) v.insert(v.end(), f, f + M);
+}
+
+}  // namespace
+
+// The scalars of a VH_EOS_CHIRAL or VH_EOS_HADRON EoS into de, whose table
+// pointers are left at the descriptor's host arrays; with pack, also the
+// concatenated field arrays for the device copy: chiral small then big (p, T,
+// mu_b, mu_s each), hadron p, T, mu_b, mu_q, mu_s; stat: the hadron status
+// flags (zeros when the descriptor has none).
+int charged_tables_host(const vh_eos_t &eos, DevEos &de, std::vector<double> *v,
+                        std::vector<int> *stat) {
+  if (v) v->clear();
+  if (stat) stat->clear();
+  if (eos.kind == VH_EOS_CHIRAL) {
+    if (!chiral_ok(eos.small) || !chiral_ok(eos.big)) return VH_ERR_ARG;
+    chiral_scalars(*eos.small, de.small);
+    chiral_scalars(*eos.big, de.big);
+    if (v) {
+      chiral_append(*eos.small, *v);
+      chiral_append(*eos.big, *v);
+    }
+    return VH_OK;
+  }
+  if (eos.kind != VH_EOS_HADRON) return VH_ERR_ARG;
+  const vh_eos_hadron_tab_t *t = eos.hadron;
+  if (!t || t->ne < 2 || t->nnb < 3 || t->nnq < 3 || !(t->logemax > t->logemin) ||
+      !(t->lognmax > 0.) || !(t->e0 > 0.) || !(t->n0 > 0.) || !t->p || !t->T || !t->mub ||
+      !t->muq || !t->mus)
+    return VH_ERR_ARG;
+  HadronTab &h = de.had;
+  h.ne = t->ne;
+  h.nnb = t->nnb;
+  h.nnq = t->nnq;
+  h.e0 = t->e0;
+  h.n0 = t->n0;
+  h.logemin = t->logemin;
+  h.lognmax = t->lognmax;
+  // the constructor's derived constants (src/eoHadron.cpp:50-54) and the
+  // per-call steps (src/eoHadron.cpp:93-96)
+  h.dxe = (t->logemax - t->logemin) / (t->ne - 1);
+  h.dxnb = (2.0 * t->lognmax) / (t->nnb - 1);
+  h.dxnq = (2.0 * t->lognmax) / (t->nnq - 1);
+  h.nb_abs_min = t->n0 * std::exp(2.0 * h.dxnb - t->lognmax);
+  h.nq_abs_min = t->n0 * std::exp(2.0 * h.dxnq - t->lognmax);
+  h.e_min = t->e0 * std::exp(t->logemin);
+  h.p = t->p;
+  h.T = t->T;
+  h.mub = t->mub;
+  h.muq = t->muq;
+  h.mus = t->mus;
+  h.status = t->status;
+  const size_t M = (size_t)t->ne * t->nnb * t->nnq;
+  if (v)
+    for (const double *f : {t->p, t->T, t->mub, t->muq, t->mus}) v->insert(v->end(), f, f + M);
+  if (stat) {
+    if (t->status)
+      stat->assign(t->status, t->status + M);
+    else
+      stat->assign(M, 0);
+  }
+  return VH_OK;
+}
+
+// points the tables of de into the concatenated arrays at v (and st)
+void point_tables(DevEos &de, const double *v, const int *st) {
+  if (de.kind == VH_EOS_CHIRAL) {
+    for (ChiralTab *t : {&de.small, &de.big}) {
+      const size_t M = (size_t)t->ne * t->nn;
+      t->p = v;
+      t->T = v + M;
+      t->mub = v + 2 * M;
+      t->mus = v + 3 * M;
+      v += 4 * M;
+    }
+  } else if (de.kind == VH_EOS_HADRON) {
+    const size_t M = (size_t)de.had.ne * de.had.nnb * de.had.nnq;
+    de.had.p = v;
+    de.had.T = v + M;
+    de.had.mub = v + 2 * M;
+    de.had.muq = v + 3 * M;
+    de.had.mus = v + 4 * M;
+    de.had.status = st;
+  }
+}
+
 int make_dev_eos(const vh_eos_t *eos, int *st, DevEos &de, double **dbuf) {
   *dbuf = nullptr;
   if (!eos) return VH_ERR_ARG;
@@ -90,6 +204,23 @@ int make_dev_eos(const vh_eos_t *eos, int *st, DevEos &de, double **dbuf) {
   de.tab.l0 = de.tab.inv = 0.f;
   de.st = st;
   if (eos->kind == VH_EOS_IDEAL) return VH_OK;
+  if (eos->kind == VH_EOS_CHIRAL || eos->kind == VH_EOS_HADRON) {
+    std::vector<double> v;
+    std::vector<int> stat;
+    const int rc = charged_tables_host(*eos, de, &v, &stat);
+    if (rc != VH_OK) return rc;
+    const size_t nb = v.size() * sizeof(double), bytes = nb + stat.size() * sizeof(int);
+    if (hipMalloc(dbuf, bytes) != hipSuccess) return VH_ERR_HIP;
+    if (hipMemcpy(*dbuf, v.data(), nb, hipMemcpyHostToDevice) != hipSuccess ||
+        (!stat.empty() && hipMemcpy(reinterpret_cast<char *>(*dbuf) + nb, stat.data(),
+                                    stat.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)) {
+      (void)hipFree(*dbuf);
+      *dbuf = nullptr;
+      return VH_ERR_HIP;
+    }
+    point_tables(de, *dbuf, reinterpret_cast<const int *>(reinterpret_cast<const char *>(*dbuf) + nb));
+    return VH_OK;
+  }
   if (eos->kind != VH_EOS_TABLE) return VH_ERR_ARG;
   std::vector<double> buf;
   int n = 0;
@@ -142,4 +273,119 @@ extern "C" int vh_eos_table(const vh_eos_t *eos, double *buf) {
   if (rc != VH_OK) return rc;
   std::memcpy(buf, t.data(), t.size() * sizeof(double));
   return VH_OK;
+}
+
+namespace {
+
+// EoS::eos and EoS::s (src/eos.cpp:18-25) at n points on the GPU
+template <bool CH>
+__global__ void __launch_bounds__(256)
+k_eos_eval(vh::DevEos eos, const double *__restrict__ in, long n, double *__restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double e = in[4 * i], nb = in[4 * i + 1], nq = in[4 * i + 2], ns = in[4 * i + 3];
+  double T, mub, muq, mus, p;
+  eos.eos<CH>(e, nb, nq, ns, T, mub, muq, mus, p);
+  double *o = out + 6 * i;
+  o[0] = p;
+  o[1] = T;
+  o[2] = mub;
+  o[3] = muq;
+  o[4] = mus;
+  o[5] = T > 0.0 ? (e + p - mub * nb - muq * nq - mus * ns) / T : 0.;
+}
+
+// the same on the host: the lookups of kinds 2 and 3 are the device code
+// compiled for the CPU, reading the descriptor's arrays in place (no copy, so
+// a call per point is cheap); kinds 0 and 1 evaluate p = e/3 and the splines
+int eos_eval_host(const vh_eos_t &d, const double *in, long n, double *out) {
+  vh::DevEos de = vh::DevEos();
+  de.kind = d.kind;
+  std::vector<double> tab;
+  int nt = 0;
+  if (d.kind == VH_EOS_CHIRAL || d.kind == VH_EOS_HADRON) {
+    // the lookups read the descriptor's arrays in place
+    const int rc = vh::charged_tables_host(d, de, nullptr, nullptr);
+    if (rc != VH_OK) return rc;
+  } else if (d.kind == VH_EOS_TABLE) {
+    const int rc = vh::eos_table_host(d, tab, nt);
+    if (rc != VH_OK) return rc;
+  } else if (d.kind != VH_EOS_IDEAL) {
+    return VH_ERR_ARG;
+  }
+  const double pi_ = 3.14159265358979312;  // C_PI, src/inc.h
+  const double cnst = (16 + 0.5 * 21.0 * 2.5) * (pi_ * pi_) / 30.0 /
+                      (0.197326968 * 0.197326968 * 0.197326968);
+  auto spline = [&](int f, double x) {  // gsl_interp_bsearch + the cubic
+    const double *xs = tab.data();
+    int lo = 0, hi = nt - 1;
+    while (hi > lo + 1) {
+      const int mid = (hi + lo) / 2;
+      if (xs[mid] > x)
+        hi = mid;
+      else
+        lo = mid;
+    }
+    const double *y = xs + (size_t)(1 + 4 * f) * nt, *b = y + nt, *c = y + 2 * nt, *dd = y + 3 * nt;
+    const double delx = x - xs[lo];
+    return y[lo] + delx * (b[lo] + delx * (c[lo] + delx * dd[lo]));
+  };
+  for (long i = 0; i < n; i++) {
+    const double e = in[4 * i], nb = in[4 * i + 1], nq = in[4 * i + 2], ns = in[4 * i + 3];
+    double T = 0., mub = 0., muq = 0., mus = 0., p = 0.;
+    if (d.kind == VH_EOS_CHIRAL) {
+      vh::chiral_eos<true>(de.small, de.big, e, nb, p, T, mub, muq, mus);
+    } else if (d.kind == VH_EOS_HADRON) {
+      de.had.get<true>(e, nb, nq, p, T, mub, muq, mus);
+    } else if (d.kind == VH_EOS_TABLE) {
+      p = spline(0, e);
+      T = spline(1, e);
+    } else {
+      p = e / 3.;
+      T = e > 0. ? 1.0 * std::pow(e / cnst, 0.25) : 0.;
+    }
+    double *o = out + 6 * i;
+    o[0] = p;
+    o[1] = T;
+    o[2] = mub;
+    o[3] = muq;
+    o[4] = mus;
+    o[5] = T > 0.0 ? (e + p - mub * nb - muq * nq - mus * ns) / T : 0.;
+  }
+  return VH_OK;
+}
+
+}  // namespace
+
+extern "C" int vh_eos_eval(const vh_eos_t *eos, const double *in, long n, double *out,
+                           int where) {
+  if (!eos || n < 0 || (n > 0 && (!in || !out)) || (where != 0 && where != 1)) return VH_ERR_ARG;
+  if (where == 0) return eos_eval_host(*eos, in, n, out);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return VH_ERR_NODEVICE;
+  vh::DevEos de;
+  double *tab = nullptr, *buf = nullptr;
+  int rc = vh::make_dev_eos(eos, nullptr, de, &tab);
+  if (rc != VH_OK) return rc;
+  if (n > 0) {
+    const size_t bi = (size_t)4 * n * sizeof(double), bo = (size_t)6 * n * sizeof(double);
+    if (hipMalloc(&buf, bi + bo) != hipSuccess) {
+      rc = VH_ERR_HIP;
+    } else if (hipMemcpy(buf, in, bi, hipMemcpyHostToDevice) != hipSuccess) {
+      rc = VH_ERR_HIP;
+    } else {
+      if (de.charged())
+        hipLaunchKernelGGL(k_eos_eval<true>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, de,
+                           buf, n, buf + 4 * n);
+      else
+        hipLaunchKernelGGL(k_eos_eval<false>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0,
+                           de, buf, n, buf + 4 * n);
+      if (hipGetLastError() != hipSuccess ||
+          hipMemcpy(out, buf + 4 * n, bo, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = VH_ERR_HIP;
+    }
+  }
+  if (buf) (void)hipFree(buf);
+  if (tab) (void)hipFree(tab);
+  return rc;
 }

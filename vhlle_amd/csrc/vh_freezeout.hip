@@ -36,6 +36,7 @@ __global__ void k_fill_err(int *__restrict__ err) {
 }
 
 // e of every cell at both levels: c2p of Q/tau and of Qlow/tau_low
+template <bool CH>
 __global__ void __launch_bounds__(256)
 k_fo_energy(long N, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qlow,
             double tau, double tlow, double *__restrict__ e1, double *__restrict__ e0) {
@@ -44,10 +45,10 @@ k_fo_energy(long N, DevEos eos, const double *__restrict__ Q, const double *__re
   double q[7];
   Prim r;
   for (int k = 0; k < 7; k++) q[k] = Q[k * N + c] / tau;
-  c2p_ref(eos, q, r);
+  c2p_ref<CH>(eos, q, r);
   e1[c] = r.e;
   for (int k = 0; k < 7; k++) q[k] = Qlow[k * N + c] / tlow;
-  c2p_ref(eos, q, r);
+  c2p_ref<CH>(eos, q, r);
   e0[c] = r.e;
 }
 
@@ -126,7 +127,7 @@ __device__ inline void cart_pi(const double piC[10], double ch, double sh, doubl
                   2. * sh * ch * piC[sym(0, 3)];
 }
 
-template <typename PT>
+template <typename PT, bool CH>
 __global__ void __launch_bounds__(64)
 k_fo_cubes(CubeGeom g, FoParams P, DevEos eos, const long *__restrict__ cut,
            const int *__restrict__ ncut_p, const double *__restrict__ e1,
@@ -180,11 +181,11 @@ k_fo_cubes(CubeGeom g, FoParams P, DevEos eos, const long *__restrict__ cut,
             }
       for (int i = 0; i < 7; i++) QC[i] = QC[i] / (P.tau + c4[0]);
       Prim pr;
-      c2p_ref(eos, QC, pr);
+      c2p_ref<CH>(eos, QC, pr);
       const double eC = pr.e, nbC = pr.nb;
       double vxC = pr.vx, vyC = pr.vy, vzC = pr.vz;
-      const double TC = eos.T(eC), mubC = 0., muqC = 0., musC = 0.;
-      const double pC = eos.p(eC);
+      double TC, mubC, muqC, musC, pC;
+      eos.eos<CH>(eC, nbC, pr.nq, pr.ns, TC, mubC, muqC, musC, pC);  // src/fld.cpp:695-697
       // trilinear pi, Pi of the upper level (src/fld.cpp:709-728)
       double piC[10] = {0., 0., 0., 0., 0., 0., 0., 0., 0., 0.}, PiC = 0.;
       for (int jx = 0; jx < 2; jx++)
@@ -270,6 +271,7 @@ k_fo_cubes(CubeGeom g, FoParams P, DevEos eos, const long *__restrict__ cut,
 
 // Fluid::outputCorona's selection (src/fld.cpp:946-985): every corner at
 // tau has e <= ecrit and the centre state (the corners' mean Q) e >= 0.01
+template <bool CH>
 __global__ void __launch_bounds__(256)
 k_corona_flag(CubeGeom g, DevEos eos, const double *__restrict__ e1,
               const double *__restrict__ Q, double tau, double ecrit,
@@ -296,7 +298,7 @@ k_corona_flag(CubeGeom g, DevEos eos, const double *__restrict__ e1,
         }
     for (int i = 0; i < 7; i++) QC[i] = QC[i] / tau;
     Prim pr;
-    c2p_ref(eos, QC, pr);
+    c2p_ref<CH>(eos, QC, pr);
     f = pr.e >= 0.01;
   }
   flag[k] = f;
@@ -305,7 +307,7 @@ k_corona_flag(CubeGeom g, DevEos eos, const double *__restrict__ e1,
 // one corona element per selected cube (src/fld.cpp:986-1066): the record at
 // the cube centre with dsigma = tau dx dy dz (cosh eta, 0, 0, -sinh eta),
 // then the AUX columns
-template <typename PT>
+template <typename PT, bool CH>
 __global__ void __launch_bounds__(64)
 k_corona_rows(CubeGeom g, FoParams P, DevEos eos, const long *__restrict__ cube,
               const int *__restrict__ n_p, const double *__restrict__ Q,
@@ -326,9 +328,10 @@ k_corona_rows(CubeGeom g, FoParams P, DevEos eos, const long *__restrict__ cube,
         }
     for (int i = 0; i < 7; i++) QC[i] = QC[i] / P.tau;
     Prim pr;
-    c2p_ref(eos, QC, pr);
+    c2p_ref<CH>(eos, QC, pr);
     const double eC = pr.e, nbC = pr.nb;
-    const double TC = eos.T(eC), pC = eos.p(eC);
+    double TC, mubC, muqC, musC, pC;
+    eos.eos<CH>(eC, nbC, pr.nq, pr.ns, TC, mubC, muqC, musC, pC);  // src/fld.cpp:979-988
     for (int jx = 0; jx < 2; jx++)
       for (int jy = 0; jy < 2; jy++)
         for (int jz = 0; jz < 2; jz++) {
@@ -361,7 +364,9 @@ k_corona_rows(CubeGeom g, FoParams P, DevEos eos, const long *__restrict__ cube,
     for (int i = 0; i < 4; i++) row[4 + i] = ds[i];
     for (int i = 0; i < 4; i++) row[8 + i] = uC[i];
     row[12] = TC;
-    row[13] = row[14] = row[15] = 0.;
+    row[13] = mubC;
+    row[14] = muqC;
+    row[15] = musC;
     double pc[10];
     cart_pi(piC, ch, sh, pc);
     for (int i = 0; i < 10; i++) row[16 + i] = pc[i];
@@ -520,11 +525,19 @@ int run_impl(const FoArgs &a, Work &w, long &nrows, int &errbits, bool corona) {
   using KeyIt = hipcub::TransformInputIterator<long, RefKey, hipcub::CountingInputIterator<long>>;
   const KeyIt keys(hipcub::CountingInputIterator<long>(0), RefKey{g});
   if (rc == VH_OK) {
-    hipLaunchKernelGGL(k_fo_energy, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, N,
-                       a.eos, a.Q, a.Qlow, a.P.tau, a.P.tau - a.P.dt, w.e1, w.e0);
-    if (corona)
-      hipLaunchKernelGGL(k_corona_flag, dim3((unsigned)((ncube + 255) / 256)), dim3(256), 0, st,
-                         g, a.eos, w.e1, a.Q, a.P.tau, a.P.ecrit, w.flag);
+    const bool ch = a.eos.charged();
+    if (ch)
+      hipLaunchKernelGGL(k_fo_energy<true>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st,
+                         N, a.eos, a.Q, a.Qlow, a.P.tau, a.P.tau - a.P.dt, w.e1, w.e0);
+    else
+      hipLaunchKernelGGL(k_fo_energy<false>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st,
+                         N, a.eos, a.Q, a.Qlow, a.P.tau, a.P.tau - a.P.dt, w.e1, w.e0);
+    if (corona && ch)
+      hipLaunchKernelGGL(k_corona_flag<true>, dim3((unsigned)((ncube + 255) / 256)), dim3(256), 0,
+                         st, g, a.eos, w.e1, a.Q, a.P.tau, a.P.ecrit, w.flag);
+    else if (corona)
+      hipLaunchKernelGGL(k_corona_flag<false>, dim3((unsigned)((ncube + 255) / 256)), dim3(256),
+                         0, st, g, a.eos, w.e1, a.Q, a.P.tau, a.P.ecrit, w.flag);
     else
       hipLaunchKernelGGL(k_fo_flag, dim3((unsigned)((ncube + 255) / 256)), dim3(256), 0, st, g,
                          w.e1, w.e0, a.P.ecrit, w.flag);
@@ -560,23 +573,34 @@ int run_impl(const FoArgs &a, Work &w, long &nrows, int &errbits, bool corona) {
     }
     if (rc == VH_OK) {
       const unsigned grid = (unsigned)std::min<long>((nc + 63) / 64, 2048);
+      // FP32 pi (the mixed path) comes with a charge-free EoS only
+      const bool ch = a.eos.charged();
       if (corona) {
         if (a.pi_float)
-          hipLaunchKernelGGL(k_corona_rows<float>, dim3(grid), dim3(64), 0, st, g, a.P, a.eos,
-                             w.sorted, w.ncut, a.Q, (const float *)a.pi, (const float *)a.Pi,
-                             w.rows);
+          hipLaunchKernelGGL((k_corona_rows<float, false>), dim3(grid), dim3(64), 0, st, g, a.P,
+                             a.eos, w.sorted, w.ncut, a.Q, (const float *)a.pi,
+                             (const float *)a.Pi, w.rows);
+        else if (ch)
+          hipLaunchKernelGGL((k_corona_rows<double, true>), dim3(grid), dim3(64), 0, st, g, a.P,
+                             a.eos, w.sorted, w.ncut, a.Q, (const double *)a.pi,
+                             (const double *)a.Pi, w.rows);
         else
-          hipLaunchKernelGGL(k_corona_rows<double>, dim3(grid), dim3(64), 0, st, g, a.P, a.eos,
-                             w.sorted, w.ncut, a.Q, (const double *)a.pi, (const double *)a.Pi,
-                             w.rows);
-      } else if (a.pi_float)
-        hipLaunchKernelGGL(k_fo_cubes<float>, dim3(grid), dim3(64), 0, st, g, a.P, a.eos, w.sorted,
-                           w.ncut, w.e1, w.e0, a.Q, a.Qlow, (const float *)a.pi,
+          hipLaunchKernelGGL((k_corona_rows<double, false>), dim3(grid), dim3(64), 0, st, g, a.P,
+                             a.eos, w.sorted, w.ncut, a.Q, (const double *)a.pi,
+                             (const double *)a.Pi, w.rows);
+      } else if (a.pi_float) {
+        hipLaunchKernelGGL((k_fo_cubes<float, false>), dim3(grid), dim3(64), 0, st, g, a.P, a.eos,
+                           w.sorted, w.ncut, w.e1, w.e0, a.Q, a.Qlow, (const float *)a.pi,
                            (const float *)a.Pi, a.db, w.rows, w.cnt, w.err);
-      else
-        hipLaunchKernelGGL(k_fo_cubes<double>, dim3(grid), dim3(64), 0, st, g, a.P, a.eos,
+      } else if (ch) {
+        hipLaunchKernelGGL((k_fo_cubes<double, true>), dim3(grid), dim3(64), 0, st, g, a.P, a.eos,
                            w.sorted, w.ncut, w.e1, w.e0, a.Q, a.Qlow, (const double *)a.pi,
                            (const double *)a.Pi, a.db, w.rows, w.cnt, w.err);
+      } else {
+        hipLaunchKernelGGL((k_fo_cubes<double, false>), dim3(grid), dim3(64), 0, st, g, a.P, a.eos,
+                           w.sorted, w.ncut, w.e1, w.e0, a.Q, a.Qlow, (const double *)a.pi,
+                           (const double *)a.Pi, a.db, w.rows, w.cnt, w.err);
+      }
       ok(hipGetLastError());
       hipLaunchKernelGGL(k_fo_stats, dim3(STAT_BLOCKS), dim3(256), 0, st, w.ncut,
                          corona ? (const int *)nullptr : w.cnt, per, w.rows, a.P.ncol,

@@ -43,7 +43,7 @@ __device__ inline long stride(const Geom &g) {
 // MUSCL face state of a cell (Cell::getPrimVar{Left,Right}[H],
 // src/cll.cpp:106-204): half slopes limited by minmod, state (q -+ dq)/tau,
 // then con2prim.  cm/c0/cp: the cell and its clamped neighbours.
-template <int SIDE>
+template <int SIDE, bool CH>
 __device__ __forceinline__ void face_state(const DevEos &eos, const double *__restrict__ Qs, long N,
                                   long cm, long c0, long cp, double tau, Prim &r) {
   double q[7];
@@ -53,7 +53,7 @@ __device__ __forceinline__ void face_state(const DevEos &eos, const double *__re
     const double dq = minmod((Qs[k * N + cp] - a) / 2., (a - Qs[k * N + cm]) / 2.);
     q[k] = SIDE < 0 ? (a - dq) / tau : (a + dq) / tau;
   }
-  c2p_ref(eos, q, r);
+  c2p_ref<CH>(eos, q, r);
 }
 
 // ------------------------------------------------------- interface flux
@@ -168,7 +168,7 @@ __device__ __forceinline__ double flux_tail(const Geom &g, const double *__restr
 // clamping e again and with El, Er of the first states (src/hdo.cpp:134-140):
 // such an interface is left to k_faces_fix with flag 2 and El, Er in its F
 // slot, so that this kernel carries one copy of the face-state code.
-template <int D, int MODE>
+template <int D, int MODE, bool CH>
 __device__ __forceinline__ double face_flux(const Geom &g, const DevEos &eos,
                                             const double *__restrict__ Qs,
                                             const double *__restrict__ m, int m_full,
@@ -186,8 +186,8 @@ __device__ __forceinline__ double face_flux(const Geom &g, const DevEos &eos,
   const long crr = i + 2 < n ? cr + s : cr;    // clamped next of the right cell
   const long N = g.N;
   Prim L, R;
-  face_state<+1>(eos, Qs, N, cll, cl, cr, tau, L);
-  face_state<-1>(eos, Qs, N, cl, cr, crr, tau, R);
+  face_state<+1, CH>(eos, Qs, N, cll, cl, cr, tau, L);
+  face_state<-1, CH>(eos, Qs, N, cl, cr, crr, tau, R);
   const double El = (L.e + L.p) / (1 - L.vx * L.vx - L.vy * L.vy - L.vz * L.vz);
   const double Er = (R.e + R.p) / (1 - R.vx * R.vx - R.vy * R.vy - R.vz * R.vz);
   if (L.e < 0.) {
@@ -217,7 +217,7 @@ __device__ __forceinline__ double face_flux(const Geom &g, const DevEos &eos,
 
 // One thread per interface (slot = left cell); the wave's largest signal
 // speed goes to the slot group sig of this stage and axis.
-template <int D, int MODE>
+template <int D, int MODE, bool CH>
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_FACES)
 k_faces(Geom g, DevEos eos, const double *__restrict__ Qs, const double *__restrict__ m,
         int m_full, const double *__restrict__ tp, double dt, double *__restrict__ F,
@@ -226,14 +226,14 @@ k_faces(Geom g, DevEos eos, const double *__restrict__ Qs, const double *__restr
   const double tau = *tp;
   const long c = xcd_block() * blockDim.x + threadIdx.x;
   double speed = 0.;
-  if (c < g.N) speed = face_flux<D, MODE>(g, eos, Qs, m, m_full, tau, dt, c, F, flag, status);
+  if (c < g.N) speed = face_flux<D, MODE, CH>(g, eos, Qs, m, m_full, tau, dt, c, F, flag, status);
   wave_speed_max(sig, speed);
 }
 
 // The interfaces k_faces left with flag 2: both face states again from Q
 // (Qr), then flux_tail with the El, Er of the first states.  Launched after
 // k_faces for table EoSs only (p = e/3 >= 0 never leaves one).
-template <int D, int MODE>
+template <int D, int MODE, bool CH>
 __global__ void __launch_bounds__(256)
 k_faces_fix(Geom g, DevEos eos, const double *__restrict__ Qr, const double *__restrict__ m,
             int m_full, const double *__restrict__ tp, double dt, double *__restrict__ F,
@@ -252,8 +252,8 @@ k_faces_fix(Geom g, DevEos eos, const double *__restrict__ Qr, const double *__r
     const long crr = i + 2 < n ? cr + s : cr;
     const double El = F[c], Er = F[N + c];
     Prim L, R;
-    face_state<+1>(eos, Qr, N, cll, cl, cr, tau, L);
-    face_state<-1>(eos, Qr, N, cl, cr, crr, tau, R);
+    face_state<+1, CH>(eos, Qr, N, cll, cl, cr, tau, L);
+    face_state<-1, CH>(eos, Qr, N, cl, cr, crr, tau, R);
     unsigned char ok = 0;
     speed = flux_tail<D, MODE>(g, m, m_full, L, R, El, Er, tau, dt, cl, cr, F, ok, status);
     flag[c] = ok;
@@ -267,7 +267,7 @@ k_faces_fix(Geom g, DevEos eos, const double *__restrict__ Qr, const double *__r
 // applies the stage update: predictor Qh = Q + flux (Cell::updateQtoQhByFlux,
 // src/cll.cpp:86-88); corrector Q += flux unless Q0 + flux0 < 0
 // (Cell::updateByFlux, src/cll.cpp:70-74).
-template <int MODE>
+template <int MODE, bool CH>
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_UPDATE)
 k_update(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
          const double *__restrict__ Fx, const unsigned char *__restrict__ fx,
@@ -311,7 +311,7 @@ k_update(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restr
 #pragma unroll
   for (int k = 0; k < 7; k++) q[k] = Qsrc[k * N + c] / tau1;
   Prim r;
-  c2p_ref(eos, q, r);
+  c2p_ref<CH>(eos, q, r);
   double S[7];
   S[T_] = -q[T_] * r.vz * r.vz - r.p * (1. + r.vz * r.vz);
   S[X_] = 0.;
@@ -559,8 +559,10 @@ k_update_m_apply(Geom g, double *__restrict__ m, const double *__restrict__ dm) 
 // Qh/(tau - dt/2) (the cell-centre state of ISformal, NSquant's own u,
 // visc_flux and visc_source_step) -> ph[8][N]; con2prim of Qh/tau (NSquant's
 // neighbour states, src/hdo.cpp:472-568) -> pt[4][N] = {e, vx, vy, vz}.
-// tau here is tau_{n+1}.  Tt (may be NULL): T of the pt state where e > 0
-// (the neighbour temperatures of the vorticity rows).
+// tau here is tau_{n+1}.  Tt (may be NULL): T of the pt state where e > 0,
+// and everywhere for an EoS that depends on the charges (whose T at e <= 0
+// needs the cell's n) -- the neighbour temperatures of the vorticity rows.
+template <bool CH>
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_VPRIM)
 k_vprim(Geom g, DevEos eos, const double *__restrict__ Qh, const double *__restrict__ tp,
         double dt, double *__restrict__ ph, double *__restrict__ pt, double *__restrict__ Tt) {
@@ -573,7 +575,7 @@ k_vprim(Geom g, DevEos eos, const double *__restrict__ Qh, const double *__restr
   const double th = tau - dt / 2.;
 #pragma unroll
   for (int k = 0; k < 7; k++) q[k] = Qh[k * N + c] / th;
-  c2p_ref(eos, q, r);
+  c2p_ref<CH>(eos, q, r);
   ph[c] = r.e;
   ph[N + c] = r.p;
   ph[2 * N + c] = r.nb;
@@ -584,12 +586,12 @@ k_vprim(Geom g, DevEos eos, const double *__restrict__ Qh, const double *__restr
   ph[7 * N + c] = r.vz;
 #pragma unroll
   for (int k = 0; k < 7; k++) q[k] = Qh[k * N + c] / tau;
-  c2p_ref(eos, q, r);
+  c2p_ref<CH>(eos, q, r);
   pt[c] = r.e;
   pt[N + c] = r.vx;
   pt[2 * N + c] = r.vy;
   pt[3 * N + c] = r.vz;
-  if (Tt) Tt[c] = r.e > 0. ? eos.T(r.e) : 0.;
+  if (Tt) Tt[c] = (r.e > 0. || CH) ? eos.T<CH>(r.e, r.nb, r.nq, r.ns) : 0.;
 }
 
 // ISformal loop #1 (src/hdo.cpp:691-864) with NSquant (src/hdo.cpp:381-652):
@@ -597,7 +599,7 @@ k_vprim(Geom g, DevEos eos, const double *__restrict__ Qh, const double *__restr
 // pi0, piH0, Pi0, PiH0.  (The loop's flux contribution (tau-dt)*dQ_n is
 // formed in k_visc2, which reads the same old pi/Pi before replacing them.)
 // With VORT, also the cell's vorticity entries and op code (vh_vort.h).
-template <bool VORT>
+template <bool VORT, bool CH>
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_VISC1)
 k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
         const double *__restrict__ Qn, const double *__restrict__ ph,
@@ -657,10 +659,10 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
     Prim p0, p1;
 #pragma unroll
     for (int k = 0; k < 7; k++) q[k] = Qprev[k * N + c] / (tau - dt);
-    c2p_ref(eos, q, p0);
+    c2p_ref<CH>(eos, q, p0);
 #pragma unroll
     for (int k = 0; k < 7; k++) q[k] = Qn[k * N + c] / tau;
-    c2p_ref(eos, q, p1);
+    c2p_ref<CH>(eos, q, p1);
     const double e0 = p0.e, e1 = e;  // e1: the H-centre state (src/hdo.cpp:427)
     double ut0 = 1.0 / sqrt(1.0 - p0.vx * p0.vx - p0.vy * p0.vy - p0.vz * p0.vz);
     double ux0 = ut0 * p0.vx, uy0 = ut0 * p0.vy, uz0 = ut0 * p0.vz;
@@ -677,7 +679,8 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
     if (e1 <= 0. || e0 <= 0.) dmu[0][0] = dmu[0][1] = dmu[0][2] = dmu[0][3] = 0.;
     if (VORT) {  // d_tau (u^nu / T), src/hdo.cpp:419-458
       const double a0[4] = {ut0, ux0, uy0, uz0}, a1[4] = {ut1, ux1, uy1, uz1};
-      vort::row_time(vv, a0, a1, eos.T(p0.e), eos.T(p1.e), e1, e0, dt);
+      vort::row_time(vv, a0, a1, eos.T<CH>(p0.e, p0.nb, p0.nq, p0.ns), eos.T<CH>(p1.e, p1.nb, p1.nq, p1.ns),
+                     e1, e0, dt);
     }
     // space derivatives from the neighbours' Qh/tau_{n+1} states
 #pragma unroll
@@ -687,8 +690,8 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
       const double ea = pt[ca], eb = pt[cb];
       // the reference takes both temperatures before its e test (src/hdo.cpp:474-481);
       // k_vprim stored them where e > 0 (the same evaluation)
-      const double TA = VORT ? (ea > 0. ? Tt[ca] : eos.T(ea)) : 0.;
-      const double TB = VORT ? (eb > 0. ? Tt[cb] : eos.T(eb)) : 0.;
+      const double TA = VORT ? ((ea > 0. || CH) ? Tt[ca] : eos.T(ea)) : 0.;
+      const double TB = VORT ? ((eb > 0. || CH) ? Tt[cb] : eos.T(eb)) : 0.;
       if (ea > 0. && eb > 0.) {
         const double avx = pt[N + ca], avy = pt[2 * N + ca], avz = pt[3 * N + ca];
         const double bvx = pt[N + cb], bvy = pt[2 * N + cb], bvz = pt[3 * N + cb];
@@ -740,7 +743,7 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
     dmu[3][0] += u[3] / (tau - 0.5 * dt);
     dmu[3][3] += u[0] / (tau - 0.5 * dt);
     if (VORT) {  // Christoffel terms (src/hdo.cpp:608-611), then Cell::setDbeta
-      const double TH = eos.T(e);
+      const double TH = eos.T<CH>(e, nb, nq, ns);
       if (TH > 0.) {
         const double a30 = u[3] / (TH * (tau - 0.5 * dt)), a33 = u[0] / (TH * (tau - 0.5 * dt));
         if (vflags & vort::F_R3) {
@@ -755,9 +758,10 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
       vort::store(db, dop, N, c, vort::K_FULL | vflags, vv);
     }
     // transport coefficients at the H-centre state
-    const double s = eos.s(e, nb, nq, ns);
-    const double T = eos.T(e), p = eos.p(e);
-    const double etaS = trc.etaS(e, nb, T, 0., s, p);
+    double T, mub, muq, mus, p;
+    eos.eos<CH>(e, nb, nq, ns, T, mub, muq, mus, p);
+    const double s = T > 0.0 ? (e + p - mub * nb - muq * nq - mus * ns) / T : 0.;  // EoS::s
+    const double etaS = trc.etaS(e, nb, T, mub, s, p);
     const double zetaS = trc.zetaS(eos, e, T, s, p);
     // sigma^{mu nu} through the projector Z (src/hdo.cpp:613-637), evaluated
     // term by term; the terms where Z vanishes identically add +-0 and are
@@ -789,17 +793,18 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
   }
 
   // ---- relaxation + sources
-  const double s = eos.s(e, nb, nq, ns);
-  const double T = eos.T(e), p = eos.p(e);
-  const double etaS = trc.etaS(e, nb, T, 0., s, p);
+  double T, mub, muq, mus, p;
+  eos.eos<CH>(e, nb, nq, ns, T, mub, muq, mus, p);
+  const double s = T > 0.0 ? (e + p - mub * nb - muq * nq - mus * ns) / T : 0.;  // EoS::s
+  const double etaS = trc.etaS(e, nb, T, mub, s, p);
   const double eta = etaS * s;
   double sig[10];
 #pragma unroll
   for (int k = 0; k < 10; k++) sig[k] = eta <= 0.0 ? 0.0 : 0.5 * piNS[k] / eta * 5.068;
   double taupi, tauPi;
-  trc.getTau(eos, e, nb, T, 0., s, p, taupi, tauPi);
+  trc.getTau(eos, e, nb, T, mub, s, p, taupi, tauPi);
   double dpp = 4. / 3., tpp = 10. / 7., lpP = 6. / 5.;
-  double phi7 = 9. / 70. / eos.p(e);
+  double phi7 = 9. / 70. / eos.p<CH>(e, nb, nq, ns);
   if (isinf(phi7)) phi7 = 0.0;
   phi7 = phi7 / taupi;
   if (taupi < 0.5 * dt) dpp = tpp = lpP = phi7 = 0.0;
@@ -1117,6 +1122,7 @@ k_visc3(Geom g, const double *__restrict__ ph, const double *__restrict__ piH,
 // ------------------------------------------------------- IC and batches
 // Cell::setPrimVar + saveQprev + setAllM (src/cll.cpp:222-241,
 // src/icGubser.cpp:47-57) on all cells.  prim: 7 arrays e,nb,nq,ns,vx,vy,vz.
+template <bool CH>
 __global__ void __launch_bounds__(256)
 k_set_prim(Geom g, DevEos eos, double tau, const double *__restrict__ pr,
            double *__restrict__ Q, double *__restrict__ Qprev, double *__restrict__ m) {
@@ -1126,7 +1132,7 @@ k_set_prim(Geom g, DevEos eos, double tau, const double *__restrict__ pr,
   const double e = pr[c], nb = pr[N + c], nq = pr[2 * N + c], ns = pr[3 * N + c];
   const double vx = pr[4 * N + c], vy = pr[5 * N + c], vz = pr[6 * N + c];
   const double g2 = 1. / (1 - vx * vx - vy * vy - vz * vz);
-  const double p = eos.p(e);
+  const double p = eos.p<CH>(e, nb, nq, ns);
   double q[7];
   q[T_] = tau * (e + p * (vx * vx + vy * vy + vz * vz)) * g2;
   q[X_] = tau * (e + p) * vx * g2;
@@ -1143,6 +1149,7 @@ k_set_prim(Geom g, DevEos eos, double tau, const double *__restrict__ pr,
   if (e > 0.) m[c] = m[N + c] = m[2 * N + c] = 1.;
 }
 
+template <bool CH>
 __global__ void __launch_bounds__(256)
 k_c2p_batch(DevEos eos, double *__restrict__ Q, long n, double *__restrict__ out,
             int *__restrict__ iters) {
@@ -1152,7 +1159,7 @@ k_c2p_batch(DevEos eos, double *__restrict__ Q, long n, double *__restrict__ out
 #pragma unroll
   for (int k = 0; k < 7; k++) q[k] = Q[7 * i + k];
   Prim r;
-  const int it = c2p_ref(eos, q, r);
+  const int it = c2p_ref<CH>(eos, q, r);
 #pragma unroll
   for (int k = 0; k < 7; k++) Q[7 * i + k] = q[k];
   double *o = out + 8 * i;
@@ -1217,15 +1224,20 @@ void launch_faces(int D, int mode, const Geom &g, const DevEos &eos, const doubl
                   double *F, unsigned char *flag, int *status, unsigned long long *sig,
                   hipStream_t st) {
   const dim3 b(256), gr(blocks(g.N));
-#define VH_F(DD, MM)                                                                        \
-  if (D == DD && mode == MM) {                                                              \
-    hipLaunchKernelGGL((k_faces<DD, MM>), gr, b, 0, st, g, eos, Qs, m, m_full, tp, dt, F,   \
-                       flag, status, sig);                                                  \
-    if (eos.kind == VH_EOS_TABLE)                                                           \
-      hipLaunchKernelGGL((k_faces_fix<DD, MM>), gr, b, 0, st, g, eos, Qr, m, m_full, tp, dt, \
-                         F, flag, status, sig);                                             \
+  // k_faces_fix re-evaluates interfaces with a negative face pressure: only
+  // the spline table and the hadron table (below e_min) can produce one
+  const bool fix = eos.kind == VH_EOS_TABLE || eos.kind == VH_EOS_HADRON;
+#define VH_F(DD, MM, CC)                                                                      \
+  if (D == DD && mode == MM && eos.charged() == CC) {                                         \
+    hipLaunchKernelGGL((k_faces<DD, MM, CC>), gr, b, 0, st, g, eos, Qs, m, m_full, tp, dt, F, \
+                       flag, status, sig);                                                    \
+    if (fix)                                                                                  \
+      hipLaunchKernelGGL((k_faces_fix<DD, MM, CC>), gr, b, 0, st, g, eos, Qr, m, m_full, tp,  \
+                         dt, F, flag, status, sig);                                           \
   }
-  VH_F(1, 0) VH_F(1, 1) VH_F(2, 0) VH_F(2, 1) VH_F(3, 0) VH_F(3, 1)
+  VH_F(1, 0, false) VH_F(1, 1, false) VH_F(2, 0, false) VH_F(2, 1, false) VH_F(3, 0, false)
+  VH_F(3, 1, false) VH_F(1, 0, true) VH_F(1, 1, true) VH_F(2, 0, true) VH_F(2, 1, true)
+  VH_F(3, 0, true) VH_F(3, 1, true)
 #undef VH_F
 }
 
@@ -1233,12 +1245,12 @@ void launch_update(int mode, const Geom &g, const DevEos &eos, const double *Q,
                    const double *Qh, const double *const F[3],
                    const unsigned char *const fl[3], const double *tp, double dt, double *out,
                    int *status, hipStream_t st) {
-  if (mode == 0)
-    hipLaunchKernelGGL(k_update<0>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, Q, Qh, F[0],
-                       fl[0], F[1], fl[1], F[2], fl[2], tp, dt, out, status);
-  else
-    hipLaunchKernelGGL(k_update<1>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, Q, Qh, F[0],
-                       fl[0], F[1], fl[1], F[2], fl[2], tp, dt, out, status);
+#define VH_U(MM, CC)                                                                          \
+  if (mode == MM && eos.charged() == CC)                                                      \
+    hipLaunchKernelGGL((k_update<MM, CC>), dim3(blocks(g.N)), dim3(256), 0, st, g, eos, Q, Qh, \
+                       F[0], fl[0], F[1], fl[1], F[2], fl[2], tp, dt, out, status);
+  VH_U(0, false) VH_U(1, false) VH_U(0, true) VH_U(1, true)
+#undef VH_U
 }
 
 void launch_ghost(int full, int axis, const Geom &g, double *Q, double *Qh, double *pi,
@@ -1262,8 +1274,12 @@ void launch_update_m(int pass, const Geom &g, double *m, double *dm, const doubl
 
 void launch_vprim(const Geom &g, const DevEos &eos, const double *Qh, const double *tp,
                   double dt, double *ph, double *pt, double *Tt, hipStream_t st) {
-  hipLaunchKernelGGL(k_vprim, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, Qh, tp, dt, ph, pt,
-                     Tt);
+  if (eos.charged())
+    hipLaunchKernelGGL(k_vprim<true>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, Qh, tp, dt, ph,
+                       pt, Tt);
+  else
+    hipLaunchKernelGGL(k_vprim<false>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, Qh, tp, dt, ph,
+                       pt, Tt);
 }
 
 void launch_visc1(const Geom &g, const DevEos &eos, const DevTrc &trc, const double *Qprev,
@@ -1271,14 +1287,13 @@ void launch_visc1(const Geom &g, const DevEos &eos, const DevTrc &trc, const dou
                   int m_full, const double *pi, const double *Pi, const double *tp, double dt,
                   double *pi0, double *piH0, double *Pi0, double *PiH0, int *status,
                   double *db, unsigned char *dop, const double *Tt, hipStream_t st) {
-  if (db)
-    hipLaunchKernelGGL(k_visc1<true>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, trc, Qprev,
-                       Qn, ph, pt, m, m_full, pi, Pi, tp, dt, pi0, piH0, Pi0, PiH0, status, db,
-                       dop, Tt);
-  else
-    hipLaunchKernelGGL(k_visc1<false>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, trc, Qprev,
-                       Qn, ph, pt, m, m_full, pi, Pi, tp, dt, pi0, piH0, Pi0, PiH0, status,
-                       nullptr, nullptr, nullptr);
+#define VH_V(VV, CC)                                                                          \
+  if (!!db == VV && eos.charged() == CC)                                                      \
+    hipLaunchKernelGGL((k_visc1<VV, CC>), dim3(blocks(g.N)), dim3(256), 0, st, g, eos, trc,     \
+                       Qprev, Qn, ph, pt, m, m_full, pi, Pi, tp, dt, pi0, piH0, Pi0, PiH0,     \
+                       status, VV ? db : nullptr, VV ? dop : nullptr, VV ? Tt : nullptr);
+  VH_V(true, false) VH_V(false, false) VH_V(true, true) VH_V(false, true)
+#undef VH_V
 }
 
 void launch_visc2(const Geom &g, const double *ph, const double *pi0, const double *piH0,
@@ -1313,12 +1328,17 @@ void launch_fill_int(int *p, int v, hipStream_t st) {
 
 void launch_set_prim(const Geom &g, const DevEos &eos, double tau, const double *pr, double *Q,
                      double *Qprev, double *m, hipStream_t st) {
-  hipLaunchKernelGGL(k_set_prim, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, tau, pr, Q, Qprev,
-                     m);
+  if (eos.charged())
+    hipLaunchKernelGGL(k_set_prim<true>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, tau, pr, Q,
+                       Qprev, m);
+  else
+    hipLaunchKernelGGL(k_set_prim<false>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, tau, pr, Q,
+                       Qprev, m);
 }
 
 // Cell::getPrimVar of every cell (src/cll.cpp:90-96): con2prim of a copy of
 // Q/tau, prim[k * N + cell] = e, p, nb, nq, ns, vx, vy, vz
+template <bool CH>
 __global__ void __launch_bounds__(256)
 k_prim_fields(long N, DevEos eos, const double *__restrict__ Q, double tau,
               double *__restrict__ prim) {
@@ -1328,7 +1348,7 @@ k_prim_fields(long N, DevEos eos, const double *__restrict__ Q, double tau,
 #pragma unroll
   for (int k = 0; k < 7; k++) q[k] = Q[k * N + c] / tau;
   Prim r;
-  c2p_ref(eos, q, r);
+  c2p_ref<CH>(eos, q, r);
   prim[c] = r.e;
   prim[N + c] = r.p;
   prim[2 * N + c] = r.nb;
@@ -1354,7 +1374,7 @@ __device__ inline void cmf(double eta, double &vx, double &vy, double vz, double
 // core cells with the viscous correction cut.  central: the v_T, vx-vy and
 // T^xx-T^yy sums only over nz/2-3 < iz < nz/2+3 (outputSurface).
 constexpr int FS = 16;
-template <typename PT>
+template <typename PT, bool CH>
 __global__ void __launch_bounds__(256)
 k_fluid_stats(Geom g, DevEos eos, const double *__restrict__ Q, const PT *__restrict__ pi,
               const double *__restrict__ vcc, double tau, double zmin, double ecrit,
@@ -1372,13 +1392,14 @@ k_fluid_stats(Geom g, DevEos eos, const double *__restrict__ Q, const PT *__rest
     double q[7];
     for (int k = 0; k < 7; k++) q[k] = Q[k * N + c] / tau;
     Prim r;
-    c2p_ref(eos, q, r);
+    c2p_ref<CH>(eos, q, r);
     const double e = r.e, nb = r.nb;
     double vx = r.vx, vy = r.vy, Y;
     const double eta = zmin + iz * g.dz;
     cmf(eta, vx, vy, r.vz, Y);
-    const double t = eos.T(e), p = eos.p(e);
-    double s = eos.s(e, nb, r.nq, r.ns);
+    double t, mub, muq, mus, p;
+    eos.eos<CH>(e, nb, r.nq, r.ns, t, mub, muq, mus, p);
+    double s = t > 0.0 ? (e + p - mub * nb - muq * r.nq - mus * r.ns) / t : 0.;  // EoS::s
     const double dz = g.dz;
     const double cosh_int = (sinh(eta + 0.5 * dz) - sinh(eta - 0.5 * dz)) / dz;
     const double sinh_int = (cosh(eta + 0.5 * dz) - cosh(eta - 0.5 * dz)) / dz;
@@ -1441,7 +1462,7 @@ __global__ void k_fluid_stats_sum(const double *__restrict__ part, int nb, doubl
 }
 
 // selected cells' getPrimVar, pi (10), Pi, viscCorrCut (21 doubles each)
-template <typename PT>
+template <typename PT, bool CH>
 __global__ void __launch_bounds__(256)
 k_cells(long N, DevEos eos, const double *__restrict__ Q, const PT *__restrict__ pi,
         const PT *__restrict__ Pi, const double *__restrict__ vcc, double tau,
@@ -1452,7 +1473,7 @@ k_cells(long N, DevEos eos, const double *__restrict__ Q, const PT *__restrict__
   double q[7];
   for (int k = 0; k < 7; k++) q[k] = Q[k * N + c] / tau;
   Prim r;
-  c2p_ref(eos, q, r);
+  c2p_ref<CH>(eos, q, r);
   double *o = out + 21 * i;
   o[0] = r.e;
   o[1] = r.p;
@@ -1473,34 +1494,50 @@ constexpr int FS_BLOCKS = 256;
 void launch_fluid_stats(const Geom &g, const DevEos &eos, const double *Q, const void *pi,
                         int pi_float, const double *vcc, double tau, double zmin, double ecrit,
                         int viscous, int central, double *part, double *out, hipStream_t st) {
-  if (pi_float)
-    hipLaunchKernelGGL(k_fluid_stats<float>, dim3(FS_BLOCKS), dim3(256), 0, st, g, eos, Q,
-                       (const float *)pi, vcc, tau, zmin, ecrit, viscous, central, part);
+  if (pi_float)  // FP32 pi: the mixed path, charge-free EoSs only
+    hipLaunchKernelGGL((k_fluid_stats<float, false>), dim3(FS_BLOCKS), dim3(256), 0, st, g, eos,
+                       Q, (const float *)pi, vcc, tau, zmin, ecrit, viscous, central, part);
+  else if (eos.charged())
+    hipLaunchKernelGGL((k_fluid_stats<double, true>), dim3(FS_BLOCKS), dim3(256), 0, st, g, eos,
+                       Q, (const double *)pi, vcc, tau, zmin, ecrit, viscous, central, part);
   else
-    hipLaunchKernelGGL(k_fluid_stats<double>, dim3(FS_BLOCKS), dim3(256), 0, st, g, eos, Q,
-                       (const double *)pi, vcc, tau, zmin, ecrit, viscous, central, part);
+    hipLaunchKernelGGL((k_fluid_stats<double, false>), dim3(FS_BLOCKS), dim3(256), 0, st, g, eos,
+                       Q, (const double *)pi, vcc, tau, zmin, ecrit, viscous, central, part);
   hipLaunchKernelGGL(k_fluid_stats_sum, dim3(1), dim3(64), 0, st, part, FS_BLOCKS, out);
 }
 
 void launch_cells(long N, const DevEos &eos, const double *Q, const void *pi, const void *Pi,
                   int pi_float, const double *vcc, double tau, const long *idx, long n,
                   double *out, hipStream_t st) {
-  if (pi_float)
-    hipLaunchKernelGGL(k_cells<float>, dim3(blocks(n)), dim3(256), 0, st, N, eos, Q,
+  if (pi_float)  // FP32 pi: the mixed path, charge-free EoSs only
+    hipLaunchKernelGGL((k_cells<float, false>), dim3(blocks(n)), dim3(256), 0, st, N, eos, Q,
                        (const float *)pi, (const float *)Pi, vcc, tau, idx, n, out);
+  else if (eos.charged())
+    hipLaunchKernelGGL((k_cells<double, true>), dim3(blocks(n)), dim3(256), 0, st, N, eos, Q,
+                       (const double *)pi, (const double *)Pi, vcc, tau, idx, n, out);
   else
-    hipLaunchKernelGGL(k_cells<double>, dim3(blocks(n)), dim3(256), 0, st, N, eos, Q,
+    hipLaunchKernelGGL((k_cells<double, false>), dim3(blocks(n)), dim3(256), 0, st, N, eos, Q,
                        (const double *)pi, (const double *)Pi, vcc, tau, idx, n, out);
 }
 
 void launch_prim_fields(long N, const DevEos &eos, const double *Q, double tau, double *prim,
                         hipStream_t st) {
-  hipLaunchKernelGGL(k_prim_fields, dim3(blocks(N)), dim3(256), 0, st, N, eos, Q, tau, prim);
+  if (eos.charged())
+    hipLaunchKernelGGL(k_prim_fields<true>, dim3(blocks(N)), dim3(256), 0, st, N, eos, Q, tau,
+                       prim);
+  else
+    hipLaunchKernelGGL(k_prim_fields<false>, dim3(blocks(N)), dim3(256), 0, st, N, eos, Q, tau,
+                       prim);
 }
 
 void launch_c2p_batch(const DevEos &eos, double *Q, long n, double *out, int *iters,
                       hipStream_t st) {
-  hipLaunchKernelGGL(k_c2p_batch, dim3(blocks(n)), dim3(256), 0, st, eos, Q, n, out, iters);
+  if (eos.charged())
+    hipLaunchKernelGGL(k_c2p_batch<true>, dim3(blocks(n)), dim3(256), 0, st, eos, Q, n, out,
+                       iters);
+  else
+    hipLaunchKernelGGL(k_c2p_batch<false>, dim3(blocks(n)), dim3(256), 0, st, eos, Q, n, out,
+                       iters);
 }
 
 void launch_p2c_batch(const double *prim, long n, double *Q, hipStream_t st) {

@@ -316,7 +316,7 @@ __device__ __forceinline__ void c2p_mode(const DevEos &eos, double q[7], Prim &r
   else if (CM == VH_C2P_NEWTON)
     ml::c2p_guess(eos, q, 0.5, r);
   else
-    c2p_ref(eos, q, r);
+    c2p_ref<false>(eos, q, r);
 }
 template <int CM>
 __device__ __forceinline__ float guess_of(const float *__restrict__ gv, long c) {

@@ -229,7 +229,7 @@ constexpr int POLISH_MAXIT = 30;
 __device__ __forceinline__ int c2p_guess(const DevEos &eos, double q[7], double v0, Prim &r) {
   const int it = c2p_polish(eos, q, v0, r, POLISH_TOL, POLISH_MAXIT);
   if (it >= 0) return it;
-  return -1 - c2p_ref(eos, q, r);
+  return -1 - c2p_ref<false>(eos, q, r);
 }
 
 }  // namespace ml

@@ -537,6 +537,8 @@ int vh_c2p_learned(const vh_eos_t *eos, const float *params, double *Q, long n, 
                    double *out, int *iters) {
   if (!eos || !params || !Q || !out || n < 0 || (start != VH_START_MLP && start != VH_START_HALF))
     return VH_ERR_ARG;
+  // the polish evaluates p(e) alone: EoSs with charge dependence are refused
+  if (eos->kind != VH_EOS_IDEAL && eos->kind != VH_EOS_TABLE) return VH_ERR_ARG;
   if (!have_device()) return VH_ERR_NODEVICE;
   if (n == 0) return VH_OK;
   vh::DevEos de;

@@ -5,7 +5,11 @@
 // reference; keys the reference does not know are echoed "UUU", comment lines
 // ('!') "CCC".  eosType 0 reads the tabulated EoS eosFile (default
 // eos/Laine_nf3.dat, the reference's path; columns e, p, T) as EoSs does.
-// Added keys: eosType 5 = ideal gas p=e/3; eosFile; icModel 10 = smooth
+// eosType 1 reads eos/chiraleos.dat and eos/chiralsmall.dat (EoSChiral);
+// eosTypeHadron 0 reads eos/eosHadronLog.dat (EoSHadron) as the EoS of the
+// freeze-out outputs, as the reference does.
+// Added keys: eosType 5 = ideal gas p=e/3; eosTypeHadron -1 = the hydro EoS
+// also for the freeze-out outputs; eosFile; icModel 10 = smooth
 // Gaussian state (icE0, icSigmaX, icSigmaY, icEtaPlateau, icEtaSigma);
 // gubserScale (energy scale of icModel 4); zetaSPeakT, zetaSWidthMinus,
 // zetaSWidthPlus (zetaSparam 5); dumpState 1 writes the final conserved
@@ -239,12 +243,26 @@ int main(int argc, char **argv) {
   EoS *eos = nullptr;
   if (eosType == 0) {
     eos = new EoSs(eosFile, 3);  // src/main.cpp:289-290
+  } else if (eosType == 1) {
+    eos = new EoSChiral();  // src/main.cpp:291-292
   } else if (eosType == 5) {
     eos = new EoSIdeal();
   } else {
     cout << "eosType " << eosType
-         << " (chiral, AZH, CMF) is not part of this build; use eosType 0 (table) or 5 "
+         << " (AZH, CMF) is not part of this build; use eosType 0 (table), 1 (chiral) or 5 "
             "(ideal gas p=e/3)\n";
+    return 0;
+  }
+  // hadronic EoS for the freeze-out surface (src/main.cpp:304-314)
+  EoS *eosH = nullptr;
+  if (eosTypeHadron == 0) {
+    eosH = new EoSHadron("eos/eosHadronLog.dat");
+  } else if (eosTypeHadron == -1) {
+    eosH = eos;
+  } else {
+    cout << "eosTypeHadron " << eosTypeHadron
+         << " (SMASH) is not part of this build; use 0 (PDG hadronic EoS, "
+            "eos/eosHadronLog.dat) or -1 (the hydro EoS)\n";
     return 0;
   }
   if (train) {
@@ -258,7 +276,7 @@ int main(int argc, char **argv) {
   trcoeff->zetaSPeakT = zetaSPeakT;
   trcoeff->zetaSWidthMinus = zetaSWidthMinus;
   trcoeff->zetaSWidthPlus = zetaSWidthPlus;
-  Fluid *f = new Fluid(eos, eos, trcoeff, nx, ny, nz, xmin, xmax, ymin, ymax, etamin, etamax,
+  Fluid *f = new Fluid(eos, eosH, trcoeff, nx, ny, nz, xmin, xmax, ymin, ymax, etamin, etamax,
                        dtau, eCrit);
   if (precision != 0 || c2pMode != 0) {
     std::vector<float> model;
@@ -312,7 +330,7 @@ int main(int argc, char **argv) {
     if (h->getTau() >= tauResize && !resized) {  // src/main.cpp:423-427
       cout << "grid resize\n";
       // (the sub-step test keeps using the params' dtau, as the reference's)
-      f = expandGrid2x(h, eos, eos, trcoeff);
+      f = expandGrid2x(h, eos, eosH, trcoeff);
       resized = true;
     }
   } while (h->getTau() < tauMax + 0.0001 && (nelements > 0 || !stopOnEmptySurface));
@@ -345,6 +363,7 @@ int main(int argc, char **argv) {
   delete h;
   delete f;
   delete trcoeff;
+  if (eosH != eos) delete eosH;
   delete eos;
   return 0;
 }

@@ -57,6 +57,7 @@ extern "C" int vh_mlp_harvest(const vh_harvest_cfg_t *cfg, long n, double *Q, do
   vh_eos_t ideal{};
   ideal.kind = VH_EOS_IDEAL;
   const vh_eos_t &eos = cfg->eos ? *cfg->eos : ideal;
+  if (eos.kind != VH_EOS_IDEAL && eos.kind != VH_EOS_TABLE) return VH_ERR_ARG;  // p(e) EoSs only
   const bool table = eos.kind == VH_EOS_TABLE;
   vh_trcoeff_t trc;
   std::memset(&trc, 0, sizeof trc);

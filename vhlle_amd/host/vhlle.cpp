@@ -179,6 +179,174 @@ vh_eos_t EoSs::device() const {
   return d;
 }
 
+// -------------------------------------------------- EoSChiral, EoSHadron
+namespace {
+// EoS::eos of a charge-dependent EoS at one point, on the host
+void eval_host(const vh_eos_t &d, double e, double nb, double nq, double ns, double &T,
+               double &mub, double &muq, double &mus, double &p) {
+  const double in[4] = {e, nb, nq, ns};
+  double out[6];
+  vh_or_die(vh_eos_eval(&d, in, 1, out, 0), "vh_eos_eval");
+  p = out[0];
+  T = out[1];
+  mub = out[2];
+  muq = out[3];
+  mus = out[4];
+}
+}  // namespace
+
+// EoSaux::EoSaux (src/eoChiral.cpp:38-84): nn blocks of ne rows
+void EoSChiral::Tab::read(const char *fname, int ne, int nn) {
+  std::ifstream fin(fname);
+  if (!fin.good()) {
+    std::cout << "I/O error with " << fname << std::endl;
+    std::exit(1);
+  }
+  const size_t M = (size_t)ne * nn;
+  p.assign(M, 0.);
+  T.assign(M, 0.);
+  mub.assign(M, 0.);
+  mus.assign(M, 0.);
+  std::vector<double> e(ne), n(nn);
+  double s, a;
+  for (int in = 0; in < nn; in++)
+    for (int ie = 0; ie < ne; ie++) {
+      const size_t k = (size_t)ie * nn + in;
+      fin >> T[k] >> mub[k] >> e[ie] >> p[k] >> n[in] >> s >> mus[k] >> a;
+      T[k] /= 1000.0;
+      mub[k] *= 3.0 / 1000.0;
+      mus[k] /= 1000.0;
+      p[k] *= 0.146;
+    }
+  d.ne = ne;
+  d.nn = nn;
+  d.emin = e[0] * 0.146;
+  d.emax = e[ne - 1] * 0.146;
+  d.nmin = n[0] * 0.15;
+  d.nmax = n[nn - 1] * 0.15;
+  d.p = p.data();
+  d.T = T.data();
+  d.mub = mub.data();
+  d.mus = mus.data();
+  std::cout << "EoSaux: table " << fname << " read, [emin,emax,nmin,nmax] = " << d.emin << "  "
+            << d.emax << "  " << d.nmin << "  " << d.nmax << std::endl;
+}
+
+void EoSChiral::Tab::copy(const vh_eos_chiral_tab_t &t) {
+  const size_t M = (size_t)t.ne * t.nn;
+  p.assign(t.p, t.p + M);
+  T.assign(t.T, t.T + M);
+  mub.assign(t.mub, t.mub + M);
+  mus.assign(t.mus, t.mus + M);
+  d = t;
+  d.p = p.data();
+  d.T = T.data();
+  d.mub = mub.data();
+  d.mus = mus.data();
+}
+
+EoSChiral::EoSChiral() {  // src/eoChiral.cpp:156-159
+  big_.read("eos/chiraleos.dat", 2001, 401);
+  small_.read("eos/chiralsmall.dat", 201, 201);
+}
+
+EoSChiral::EoSChiral(const vh_eos_chiral_tab_t &small, const vh_eos_chiral_tab_t &big) {
+  small_.copy(small);
+  big_.copy(big);
+}
+
+vh_eos_t EoSChiral::device() const {
+  vh_eos_t d{};
+  d.kind = VH_EOS_CHIRAL;
+  d.small = &small_.d;
+  d.big = &big_.d;
+  return d;
+}
+
+void EoSChiral::eos(double e, double nb, double nq, double ns, double &T, double &mub,
+                    double &muq, double &mus, double &p_) {
+  eval_host(device(), e, nb, nq, ns, T, mub, muq, mus, p_);
+}
+
+double EoSChiral::p(double e, double nb, double nq, double ns) {
+  double T, mub, muq, mus, p_;
+  eval_host(device(), e, nb, nq, ns, T, mub, muq, mus, p_);
+  return p_;
+}
+
+// EoSHadron::EoSHadron (src/eoHadron.cpp:13-65)
+EoSHadron::EoSHadron(const char *filename) {
+  std::ifstream fin(filename);
+  if (!fin.good()) {
+    std::cout << "I/O error with " << filename << std::endl;
+    std::exit(1);
+  }
+  fin >> d_.ne >> d_.nnb >> d_.nnq >> d_.e0 >> d_.n0 >> d_.logemin >> d_.logemax >> d_.lognmax;
+  const size_t M = (size_t)d_.ne * d_.nnb * d_.nnq;
+  for (auto *v : {&hp_, &hT_, &hmub_, &hmuq_, &hmus_}) v->assign(M, 0.);
+  hstatus_.assign(M, 0);
+  std::vector<double> e(d_.ne), nb(d_.nnb), nq(d_.nnq);
+  for (int ie = 0; ie < d_.ne; ie++)
+    for (int inb = 0; inb < d_.nnb; inb++)
+      for (int inq = 0; inq < d_.nnq; inq++) {
+        const size_t k = inq + (size_t)d_.nnq * (inb + (size_t)d_.nnb * ie);
+        fin >> e[ie] >> nb[inb] >> nq[inq] >> hp_[k] >> hT_[k] >> hmub_[k] >> hmuq_[k] >> hmus_[k] >>
+            hstatus_[k];
+      }
+  const double emin = e[0], emax = e[d_.ne - 1], nbmax = nb[d_.nnb - 1];
+  if (std::fabs((d_.e0 * std::exp(d_.logemin) - emin) / emin) > 1e-5 ||
+      std::fabs((d_.e0 * std::exp(d_.logemax) - emax) / emax) > 1e-5 ||
+      std::fabs(d_.n0 * std::exp(d_.lognmax) - nbmax) > 1e-4) {
+    std::cout << "EoSHadron: wrong eps or nb range: " << emin << " " << emax << " " << nb[0]
+              << " " << nbmax << std::endl;
+    std::exit(1);
+  }
+  std::cout << "EoHadron: table " << filename << " read, [emin,emax,nmin,nmax] = " << emin
+            << "  " << emax << "  " << nb[0] << "  " << nbmax << "  " << nq[0] << "  "
+            << nq[d_.nnq - 1] << std::endl;
+  point();
+}
+
+EoSHadron::EoSHadron(const vh_eos_hadron_tab_t &t) {
+  d_ = t;
+  const size_t M = (size_t)t.ne * t.nnb * t.nnq;
+  hp_.assign(t.p, t.p + M);
+  hT_.assign(t.T, t.T + M);
+  hmub_.assign(t.mub, t.mub + M);
+  hmuq_.assign(t.muq, t.muq + M);
+  hmus_.assign(t.mus, t.mus + M);
+  hstatus_.assign(M, 0);
+  if (t.status) hstatus_.assign(t.status, t.status + M);
+  point();
+}
+
+void EoSHadron::point() {
+  d_.p = hp_.data();
+  d_.T = hT_.data();
+  d_.mub = hmub_.data();
+  d_.muq = hmuq_.data();
+  d_.mus = hmus_.data();
+  d_.status = hstatus_.data();
+}
+
+vh_eos_t EoSHadron::device() const {
+  vh_eos_t d{};
+  d.kind = VH_EOS_HADRON;
+  d.hadron = &d_;
+  return d;
+}
+
+void EoSHadron::eos(double e, double nb, double nq, double ns, double &T, double &mub,
+                    double &muq, double &mus, double &p) {
+  eval_host(device(), e, nb, nq, ns, T, mub, muq, mus, p);
+}
+
+double EoSHadron::p(double e, double nb, double nq, double ns) {
+  double T, mub, muq, mus, pr;
+  eval_host(device(), e, nb, nq, ns, T, mub, muq, mus, pr);
+  return pr;
+}
+
 // ----------------------------------------------------- con2prim (host)
 // Bisection/Newton root of v = M / (Q_t + p(Q_t - M v, n)), from v = 0.5,
 // stopping at |dv| < 1e-5 (src/rmn.cpp:28-139).
@@ -553,6 +721,7 @@ Fluid::Fluid(EoS *_eos, EoS *_eosH, TransportCoeff *_trcoeff, int _nx, int _ny, 
   vh_eos_t e = eos->device();
   vh_trcoeff_t t = trcoeff->device();
   vh_or_die(vh_create(&g, &e, &t, VH_PREC_FP64, &ctx), "vh_create");
+  setSurfaceEos();
   // Cell ctor defaults (src/cll.cpp:33-52)
   hQ.assign(7 * N, 0.);
   hQh.assign(7 * N, 0.);
@@ -564,6 +733,13 @@ Fluid::Fluid(EoS *_eos, EoS *_eosH, TransportCoeff *_trcoeff, int _nx, int _ny, 
   hm.assign(3 * N, 0.);
   hvcc.assign(N, 1.);
   host_dirty = true;
+}
+
+// eosH, the EoS of outputSurface / outputCorona (SWAP_EOS, src/fld.cpp:39)
+void Fluid::setSurfaceEos() {
+  if (!eosH || eosH == eos) return;
+  const vh_eos_t d = eosH->device();
+  vh_or_die(vh_set_surface_eos(ctx, &d), "vh_set_surface_eos");
 }
 
 Fluid::~Fluid() {
@@ -870,6 +1046,7 @@ void Fluid::configure(int precision, int c2pMode, const float *model) {
   if (ctx) vh_destroy(ctx);
   ctx = nullptr;
   vh_or_die(vh_create(&g, &e, &t, precision, &ctx), "vh_create");
+  setSurfaceEos();
   if (model) vh_or_die(vh_set_c2p_model(ctx, model), "vh_set_c2p_model");
   if (vorticityOn) vh_or_die(vh_set_vorticity(ctx, 1), "vh_set_vorticity");
   if (c2pMode != VH_C2P_REFERENCE) vh_or_die(vh_set_c2p_mode(ctx, c2pMode), "vh_set_c2p_mode");

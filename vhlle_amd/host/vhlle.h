@@ -99,6 +99,57 @@ class EoSs : public EoS {
   std::vector<double> te_, tp_, tT_, tmu_, tab_;
 };
 
+// EoSChiral (src/eoChiral.h, src/eoChiral.cpp): bilinear lookups in two
+// (e, n_b) tables -- eos/chiralsmall.dat (201 x 201) where e < 1.46 and
+// n_b < 0.3, eos/chiraleos.dat (2001 x 401) where e < 146 and n_b < 6 -- and
+// p = 0.2964 e, T = 0.15120476935 e^(1/4) beyond; mu_q = 0; cs2(e) = 1/3.
+// EoSChiral() reads the two files as the reference does (columns T [MeV],
+// mu_q [MeV], e [e_0], p [e_0], n [n_0], s [n_0], mu_S [MeV], unused; unit
+// factors 1/1000, 3/1000, 0.146, 0.146, 0.15); the second constructor wraps
+// tables already in GeV, GeV/fm^3, 1/fm^3 (copied).  Host evaluations use the
+// GPU kernels' lookups compiled for the CPU (vh_eos_eval).  The GPU path
+// evaluates it in FP64 contexts only.
+class EoSChiral : public EoS {
+ public:
+  EoSChiral();
+  EoSChiral(const vh_eos_chiral_tab_t &small, const vh_eos_chiral_tab_t &big);
+  void eos(double e, double nb, double nq, double ns, double &T, double &mub, double &muq,
+           double &mus, double &p) override;
+  double p(double e, double nb, double nq, double ns) override;
+  vh_eos_t device() const override;
+
+ private:
+  struct Tab {
+    vh_eos_chiral_tab_t d{};
+    std::vector<double> p, T, mub, mus;
+    void read(const char *fname, int ne, int nn);
+    void copy(const vh_eos_chiral_tab_t &t);
+  } small_, big_;
+};
+
+// EoSHadron (src/eoHadron.h, src/eoHadron.cpp): trilinear lookups in a table
+// uniform in log(e/e0) and in the log-mapped n_b, n_q; cs2(e) = 1/3.
+// EoSHadron(filename) reads the reference's format (a header ne nnb nnq e0 n0
+// logemin logemax lognmax, then rows e nb nq p T mub muq mus status) and runs
+// its range check; the second constructor wraps a table (copied).  The
+// reference uses it as the hadronic EoS of the freeze-out surface (Fluid's
+// eosH); the GPU path evaluates it in FP64 contexts only.
+class EoSHadron : public EoS {
+ public:
+  explicit EoSHadron(const char *filename);
+  explicit EoSHadron(const vh_eos_hadron_tab_t &t);
+  void eos(double e, double nb, double nq, double ns, double &T, double &mub, double &muq,
+           double &mus, double &p) override;
+  double p(double e, double nb, double nq, double ns) override;
+  vh_eos_t device() const override;
+
+ private:
+  vh_eos_hadron_tab_t d_{};
+  std::vector<double> hp_, hT_, hmub_, hmuq_, hmus_;
+  std::vector<int> hstatus_;
+  void point();
+};
+
 // con2prim / prim2con on the host (src/rmn.cpp:28-139, 253-263), used by ICs
 // and by host code reading cells; the hot path runs them on the GPU.
 void transformPV(EoS *eos, double Q[7], double &e, double &p, double &nb, double &nq,
@@ -277,6 +328,7 @@ class Fluid {
 
  private:
   friend class Cell;
+  void setSurfaceEos();
   EoS *eos, *eosH;
   TransportCoeff *trcoeff;
   int nx, ny, nz;
