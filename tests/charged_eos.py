@@ -123,3 +123,29 @@ def to_abi(t):
                                                x.mus) for x in (t.small, t.big)))
     return abi.EosHadron(t.e0, t.n0, t.logemin, t.logemax, t.lognmax, t.p, t.T, t.mub, t.muq,
                          t.mus, t.status)
+
+
+def write_chiral_files(eos_dir, t):
+    """chiraleos.dat (big) and chiralsmall.dat (small) in the reference's
+    format (src/eoChiral.cpp:60-72): nn blocks of ne rows of T [MeV],
+    mu_q [MeV], e [e_0], p [e_0], n [n_0], s [n_0], mu_S [MeV], unused"""
+    for name, x in (("chiraleos.dat", t.big), ("chiralsmall.dat", t.small)):
+        E, N = x.nodes()
+        cols = [x.T * 1000.0, x.mub * 1000.0 / 3.0, E / 0.146, x.p / 0.146, N / 0.15,
+                np.ones_like(E), x.mus * 1000.0, np.zeros_like(E)]
+        rows = np.stack([c.T.ravel() for c in cols], axis=1)  # in-major, ie-minor
+        np.savetxt(f"{eos_dir}/{name}", rows, fmt="%.17g")
+
+
+def write_hadron_file(path, h):
+    """eosHadronLog.dat in the reference's format (src/eoHadron.cpp:19-40): a
+    header ne nnb nnq e0 n0 logemin logemax lognmax, then rows e nb nq p T
+    mub muq mus status in (ie, inb, inq) order"""
+    X, Y, Z = np.meshgrid(*h.axes(), indexing="ij")
+    _, dxb, dxq = h.dx()
+    cols = [h.e0 * np.exp(X), mapped_to_density(Y, h.n0, h.lognmax, dxb),
+            mapped_to_density(Z, h.n0, h.lognmax, dxq), h.p, h.T, h.mub, h.muq, h.mus]
+    rows = np.column_stack([c.ravel() for c in cols] + [h.status.ravel()])
+    head = f"{h.ne} {h.nnb} {h.nnq} " + " ".join(
+        repr(float(v)) for v in (h.e0, h.n0, h.logemin, h.logemax, h.lognmax))
+    np.savetxt(path, rows, fmt=["%.17g"] * 8 + ["%d"], header=head, comments="")

@@ -466,3 +466,54 @@ def test_driver_big_e_exits_0_and_keeps_unfinished_file(tmp_path):
     assert "e>1e10" in r.stdout
     assert (tmp_path / "freezeout.dat.unfinished").exists()
     assert not (tmp_path / "freezeout.dat").exists()
+
+
+@pytest.mark.gpu
+def test_driver_chiral_and_hadron_tables(tmp_path, oracle):
+    """eosType 1 and eosTypeHadron 0: the driver reads eos/chiraleos.dat,
+    eos/chiralsmall.dat and eos/eosHadronLog.dat (synthetic tables in the
+    reference's formats) from its working directory.  The 3-D viscous state
+    equals the oracle's with the same chiral tables, and the corona records
+    of the IC at tau0 (freezeout.dat's first rows) equal the oracle's corona
+    with the hadron EoS, which the reference swaps in for the surface."""
+    import charged_eos as ce
+    from vhlle_amd import abi
+    eos_dir = tmp_path / "eos"
+    eos_dir.mkdir()
+    ce.write_chiral_files(str(eos_dir), ce.chiral())
+    ce.write_hadron_file(str(eos_dir / "eosHadronLog.dat"), ce.hadron())
+    chiral = abi.EosChiral.read(str(eos_dir / "chiraleos.dat"), str(eos_dir / "chiralsmall.dat"))
+    hadron = abi.EosHadron.read(str(eos_dir / "eosHadronLog.dat"))
+    nx, ny, nz = 14, 12, 9
+    xl, yl, zl = (-2.6, 2.6), (-2.2, 2.2), (-0.6, 0.6)
+    p = _params(tmp_path, eosType=1, eosTypeHadron=0, icModel=10, icE0=1.2, icSigmaX=1.0,
+                icSigmaY=0.9, icEtaPlateau=0.3, icEtaSigma=0.4, nx=nx, ny=ny, nz=nz,
+                xmin=xl[0], xmax=xl[1], ymin=yl[0], ymax=yl[1], etamin=zl[0], etamax=zl[1],
+                tau0=0.5, tauMax=0.6, dtau=0.02, etaS=0.1, tauGridResize=100.0, dumpState=1,
+                stopOnEmptySurface=0, e_crit=0.3)
+    r = subprocess.run([EXE, "-params", p, "-outputDir", str(tmp_path)], capture_output=True,
+                       text=True, timeout=300, cwd=str(tmp_path))
+    print(r.stdout[-600:])
+    assert r.returncode == 0, r.stderr
+    assert "EoSaux: table eos/chiraleos.dat read" in r.stdout
+    assert "EoHadron: table eos/eosHadronLog.dat read" in r.stdout
+    e, vx, vy, vz = setups.gaussian3d_ic(nx, ny, nz, xl, yl, zl, e0=1.2, sx=1.0, sy=0.9,
+                                         eta_plateau=0.3, eta_sigma=0.4)
+    o = oracle.Solver(nx, ny, nz, xl, yl, zl, trc=oracle.make_trc(etaS=0.1), table=chiral)
+    o.set_prim(0.5, e, vx, vy, vz)
+    s0 = o.get_state()
+    tau = 0.5
+    while tau < 0.6 + 0.0001:
+        rc, tau = o.run(tau, 0.02, 1)
+        assert rc == 0
+    Qg = np.fromfile(tmp_path / "state.bin").reshape(7, nx * ny * nz)
+    Qo = o.get_state()["Q"]
+    assert np.max(np.abs(Qg - Qo)) <= 1e-10 * np.max(np.abs(Qo))
+    rows = np.loadtxt(tmp_path / "freezeout.dat", ndmin=2)
+    ncor = int(re.search(r"corona elements : (\d+)", r.stdout).group(1))
+    spacing = ((xl[1] - xl[0]) / (nx - 1), (yl[1] - yl[0]) / (ny - 1), (zl[1] - zl[0]) / (nz - 1))
+    ref, _ = oracle.corona((nx, ny, nz), spacing, (xl[0], yl[0], zl[0]), s0["Q"], s0["pi"],
+                           s0["Pi"], 0.5, 0.3, table=hadron)
+    print(f"corona {ncor} rows, T {ref[:, 12].min():.4f}..{ref[:, 12].max():.4f}", flush=True)
+    assert ncor == len(ref) > 0
+    assert np.allclose(rows[:ncor], ref, rtol=1e-13, atol=1e-14 * np.abs(ref).max())

@@ -45,13 +45,13 @@ __device__ inline long stride(const Geom &g) {
 // then con2prim.  cm/c0/cp: the cell and its clamped neighbours.
 template <int SIDE, bool CH>
 __device__ __forceinline__ void face_state(const DevEos &eos, const double *__restrict__ Qs, long N,
-                                  long cm, long c0, long cp, double tau, Prim &r) {
+                                  long cm, long c0, long cp, const Divider &dtau, Prim &r) {
   double q[7];
 #pragma unroll
   for (int k = 0; k < 7; k++) {
     const double a = Qs[k * N + c0];
     const double dq = minmod((Qs[k * N + cp] - a) / 2., (a - Qs[k * N + cm]) / 2.);
-    q[k] = SIDE < 0 ? (a - dq) / tau : (a + dq) / tau;
+    q[k] = SIDE < 0 ? dtau(a - dq) : dtau(a + dq);  // (a -+ dq) / tau
   }
   c2p_ref<CH>(eos, q, r);
 }
@@ -186,8 +186,9 @@ __device__ __forceinline__ double face_flux(const Geom &g, const DevEos &eos,
   const long crr = i + 2 < n ? cr + s : cr;    // clamped next of the right cell
   const long N = g.N;
   Prim L, R;
-  face_state<+1, CH>(eos, Qs, N, cll, cl, cr, tau, L);
-  face_state<-1, CH>(eos, Qs, N, cl, cr, crr, tau, R);
+  const Divider dtau(tau);
+  face_state<+1, CH>(eos, Qs, N, cll, cl, cr, dtau, L);
+  face_state<-1, CH>(eos, Qs, N, cl, cr, crr, dtau, R);
   const double El = (L.e + L.p) / (1 - L.vx * L.vx - L.vy * L.vy - L.vz * L.vz);
   const double Er = (R.e + R.p) / (1 - R.vx * R.vx - R.vy * R.vy - R.vz * R.vz);
   if (L.e < 0.) {
@@ -252,8 +253,9 @@ k_faces_fix(Geom g, DevEos eos, const double *__restrict__ Qr, const double *__r
     const long crr = i + 2 < n ? cr + s : cr;
     const double El = F[c], Er = F[N + c];
     Prim L, R;
-    face_state<+1, CH>(eos, Qr, N, cll, cl, cr, tau, L);
-    face_state<-1, CH>(eos, Qr, N, cl, cr, crr, tau, R);
+    const Divider dtau(tau);
+    face_state<+1, CH>(eos, Qr, N, cll, cl, cr, dtau, L);
+    face_state<-1, CH>(eos, Qr, N, cl, cr, crr, dtau, R);
     unsigned char ok = 0;
     speed = flux_tail<D, MODE>(g, m, m_full, L, R, El, Er, tau, dt, cl, cr, F, ok, status);
     flag[c] = ok;
@@ -308,8 +310,9 @@ k_update(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restr
   const double tau1 = MODE == 0 ? tau : tau + 0.5 * dt;
   const double sdt = MODE == 0 ? dt / 2. : dt;
   double q[7];
+  const Divider d1(tau1);
 #pragma unroll
-  for (int k = 0; k < 7; k++) q[k] = Qsrc[k * N + c] / tau1;
+  for (int k = 0; k < 7; k++) q[k] = d1(Qsrc[k * N + c]);  // Q / tau1
   Prim r;
   c2p_ref<CH>(eos, q, r);
   double S[7];
@@ -573,8 +576,9 @@ k_vprim(Geom g, DevEos eos, const double *__restrict__ Qh, const double *__restr
   double q[7];
   Prim r;
   const double th = tau - dt / 2.;
+  const Divider dth(th), dtau(tau);
 #pragma unroll
-  for (int k = 0; k < 7; k++) q[k] = Qh[k * N + c] / th;
+  for (int k = 0; k < 7; k++) q[k] = dth(Qh[k * N + c]);  // Qh / th
   c2p_ref<CH>(eos, q, r);
   ph[c] = r.e;
   ph[N + c] = r.p;
@@ -585,7 +589,7 @@ k_vprim(Geom g, DevEos eos, const double *__restrict__ Qh, const double *__restr
   ph[6 * N + c] = r.vy;
   ph[7 * N + c] = r.vz;
 #pragma unroll
-  for (int k = 0; k < 7; k++) q[k] = Qh[k * N + c] / tau;
+  for (int k = 0; k < 7; k++) q[k] = dtau(Qh[k * N + c]);  // Qh / tau
   c2p_ref<CH>(eos, q, r);
   pt[c] = r.e;
   pt[N + c] = r.vx;
@@ -630,6 +634,8 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
   const double gamma = 1.0 / sqrt(1.0 - vx * vx - vy * vy - vz * vz);
   const double u[4] = {gamma, gamma * vx, gamma * vy, gamma * vz};
   const double gm[4] = {1., -1., -1., -1.};
+  // the quotients x / gamma and x / 5.068 of the IEEE division (Divider)
+  const Divider dg(gamma), d5068(5.068);
 
   // ---- NSquant
   double dmu[4][4], piNS[10], PiNS, du;
@@ -658,24 +664,25 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
     double q[7];
     Prim p0, p1;
 #pragma unroll
-    for (int k = 0; k < 7; k++) q[k] = Qprev[k * N + c] / (tau - dt);
+    for (int k = 0; k < 7; k++) q[k] = Divider(tau - dt)(Qprev[k * N + c]);
     c2p_ref<CH>(eos, q, p0);
 #pragma unroll
-    for (int k = 0; k < 7; k++) q[k] = Qn[k * N + c] / tau;
+    for (int k = 0; k < 7; k++) q[k] = Divider(tau)(Qn[k * N + c]);
     c2p_ref<CH>(eos, q, p1);
     const double e0 = p0.e, e1 = e;  // e1: the H-centre state (src/hdo.cpp:427)
     double ut0 = 1.0 / sqrt(1.0 - p0.vx * p0.vx - p0.vy * p0.vy - p0.vz * p0.vz);
     double ux0 = ut0 * p0.vx, uy0 = ut0 * p0.vy, uz0 = ut0 * p0.vz;
     double ut1 = 1.0 / sqrt(1.0 - p1.vx * p1.vx - p1.vy * p1.vy - p1.vz * p1.vz);
     double ux1 = ut1 * p1.vx, uy1 = ut1 * p1.vy, uz1 = ut1 * p1.vz;
-    dmu[0][0] = (ut1 * ut1 - ut0 * ut0) / 2. / u[0] / dt;
-    dmu[0][1] = (ux1 * ux1 - ux0 * ux0) / 2. / u[1] / dt;
-    dmu[0][2] = (uy1 * uy1 - uy0 * uy0) / 2. / u[2] / dt;
-    dmu[0][3] = (uz1 * uz1 - uz0 * uz0) / 2. / u[3] / dt;
-    if (fabs(0.5 * (ut1 + ut0) / ut1) > UDIFF) dmu[0][0] = (ut1 - ut0) / dt;
-    if (fabs(u[1]) < VMIN || fabs(0.5 * (ux1 + ux0) / ux1) > UDIFF) dmu[0][1] = (ux1 - ux0) / dt;
-    if (fabs(u[2]) < VMIN || fabs(0.5 * (uy1 + uy0) / uy1) > UDIFF) dmu[0][2] = (uy1 - uy0) / dt;
-    if (fabs(u[3]) < VMIN || fabs(0.5 * (uz1 + uz0) / uz1) > UDIFF) dmu[0][3] = (uz1 - uz0) / dt;
+    const Divider ddt(dt);  // x / dt
+    dmu[0][0] = ddt((ut1 * ut1 - ut0 * ut0) / 2. / u[0]);
+    dmu[0][1] = ddt((ux1 * ux1 - ux0 * ux0) / 2. / u[1]);
+    dmu[0][2] = ddt((uy1 * uy1 - uy0 * uy0) / 2. / u[2]);
+    dmu[0][3] = ddt((uz1 * uz1 - uz0 * uz0) / 2. / u[3]);
+    if (fabs(0.5 * (ut1 + ut0) / ut1) > UDIFF) dmu[0][0] = ddt(ut1 - ut0);
+    if (fabs(u[1]) < VMIN || fabs(0.5 * (ux1 + ux0) / ux1) > UDIFF) dmu[0][1] = ddt(ux1 - ux0);
+    if (fabs(u[2]) < VMIN || fabs(0.5 * (uy1 + uy0) / uy1) > UDIFF) dmu[0][2] = ddt(uy1 - uy0);
+    if (fabs(u[3]) < VMIN || fabs(0.5 * (uz1 + uz0) / uz1) > UDIFF) dmu[0][3] = ddt(uz1 - uz0);
     if (e1 <= 0. || e0 <= 0.) dmu[0][0] = dmu[0][1] = dmu[0][2] = dmu[0][3] = 0.;
     if (VORT) {  // d_tau (u^nu / T), src/hdo.cpp:419-458
       const double a0[4] = {ut0, ux0, uy0, uz0}, a1[4] = {ut1, ux1, uy1, uz1};
@@ -780,11 +787,11 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
             if (j == l) z += 0.5 * ((i == k ? gm[i] : 0.) - u[i] * u[k]);
             if (i == l) z += 0.5 * ((j == k ? gm[j] : 0.) - u[j] * u[k]);
             if (k == l) z -= ((i == j ? gm[i] : 0.) - u[i] * u[j]) / 3.0;
-            a += z * dmu[k][l] * 2.0 * etaS * s / 5.068;
+            a += d5068(z * dmu[k][l] * 2.0 * etaS * s);
           }
         piNS[sym(i, j)] = a;
       }
-    PiNS = -zetaS * s * (dmu[0][0] + dmu[1][1] + dmu[2][2] + dmu[3][3]) / 5.068;
+    PiNS = d5068(-zetaS * s * (dmu[0][0] + dmu[1][1] + dmu[2][2] + dmu[3][3]));
     du = dmu[0][0] + dmu[1][1] + dmu[2][2] + dmu[3][3];
     bool bad = false;
 #pragma unroll
@@ -808,6 +815,7 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
   if (isinf(phi7)) phi7 = 0.0;
   phi7 = phi7 / taupi;
   if (taupi < 0.5 * dt) dpp = tpp = lpP = phi7 = 0.0;
+  const Divider dtp(taupi);  // used where taupi > dt/2
   double dPP = 2. / 3., lPp = 8. / 5. * (1. / 3. - eos.cs2e(e));
   if (tauPi < 0.5 * dt) dPP = lPp = 0.0;
 
@@ -822,34 +830,34 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
       const int ij = sym(i, j);
       D[ij] = -u[i] * u[j];
       if (i == j) D[ij] += gm[i];
-      H[ij] = taupi > 0.5 * dt ? P[ij] - (P[ij] - piNS[ij]) * dt / 2.0 / gamma / taupi : piNS[ij];
+      H[ij] = taupi > 0.5 * dt ? P[ij] - dtp(dg((P[ij] - piNS[ij]) * dt / 2.0)) : piNS[ij];
     }
-  double PH = tauPi > 0.5 * dt ? Pc - (Pc - PiNS) * dt / 2.0 / gamma / tauPi : PiNS;
-  double tau1 = tau - dt * 0.75;
-  H[sym(0, 0)] += -2. * vz * P[sym(0, 3)] / tau1 * dt / 2.;
-  H[sym(3, 3)] += -(2. * vz / tau1 * P[sym(0, 3)]) * dt / 2.;
-  H[sym(3, 0)] += -(vz / tau1 * P[sym(0, 0)] + vz / tau1 * P[sym(3, 3)]) * dt / 2.;
-  H[sym(1, 0)] += -vz / tau1 * P[sym(1, 3)] * dt / 2.;
-  H[sym(2, 0)] += -vz / tau1 * P[sym(2, 3)] * dt / 2.;
-  H[sym(3, 1)] += -(vz / tau1 * P[sym(0, 1)]) * dt / 2.;
-  H[sym(3, 2)] += -(vz / tau1 * P[sym(0, 2)]) * dt / 2.;
+  double PH = tauPi > 0.5 * dt ? Pc - dg((Pc - PiNS) * dt / 2.0) / tauPi : PiNS;
+  const Divider dh(tau - dt * 0.75);  // x / tau1 of the half step
+  H[sym(0, 0)] += dh(-2. * vz * P[sym(0, 3)]) * dt / 2.;
+  H[sym(3, 3)] += -(dh(2. * vz) * P[sym(0, 3)]) * dt / 2.;
+  H[sym(3, 0)] += -(dh(vz) * P[sym(0, 0)] + dh(vz) * P[sym(3, 3)]) * dt / 2.;
+  H[sym(1, 0)] += dh(-vz) * P[sym(1, 3)] * dt / 2.;
+  H[sym(2, 0)] += dh(-vz) * P[sym(2, 3)] * dt / 2.;
+  H[sym(3, 1)] += -(dh(vz) * P[sym(0, 1)]) * dt / 2.;
+  H[sym(3, 2)] += -(dh(vz) * P[sym(0, 2)]) * dt / 2.;
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int j = 0; j <= i; j++) {
       const int ij = sym(i, j);
       double t = H[ij];
-      t += (-dpp * P[ij] * du + lpP * Pc * sig[ij]) / gamma * 0.5 * dt;
+      t += dg(-dpp * P[ij] * du + lpP * Pc * sig[ij]) * 0.5 * dt;
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        t += (phi7 * P[sym(i, k)] * P[sym(j, k)] * gm[k] -
-              tpp * 0.5 * (P[sym(i, k)] * sig[sym(j, k)] * gm[k] + P[sym(j, k)] * sig[sym(i, k)] * gm[k])) /
-             gamma * 0.5 * dt;
+        t += dg(phi7 * P[sym(i, k)] * P[sym(j, k)] * gm[k] -
+                tpp * 0.5 * (P[sym(i, k)] * sig[sym(j, k)] * gm[k] + P[sym(j, k)] * sig[sym(i, k)] * gm[k])) *
+             0.5 * dt;
 #pragma unroll
         for (int l = 0; l < 4; l++)
-          t += (-P[sym(i, k)] * u[j] - P[sym(j, k)] * u[i]) * u[l] * dmu[l][k] * gm[k] / gamma * 0.5 * dt -
-               1. / 3. * D[ij] * P[sym(k, l)] * (phi7 * P[sym(k, l)] - tpp * sig[sym(k, l)]) * gm[k] * gm[l] /
-                   gamma * 0.5 * dt;
+          t += dg((-P[sym(i, k)] * u[j] - P[sym(j, k)] * u[i]) * u[l] * dmu[l][k] * gm[k]) * 0.5 * dt -
+               dg(1. / 3. * D[ij] * P[sym(k, l)] * (phi7 * P[sym(k, l)] - tpp * sig[sym(k, l)]) * gm[k] *
+                  gm[l]) * 0.5 * dt;
       }
       H[ij] = t;
     }
@@ -857,42 +865,43 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
   for (int k = 0; k < 4; k++)
 #pragma unroll
     for (int l = 0; l < 4; l++)
-      PH += lPp * P[sym(k, l)] * sig[sym(k, l)] * gm[k] * gm[l] / gamma * 0.5 * dt;
-  PH += -dPP * Pc * du / gamma * 0.5 * dt;
+      PH += dg(lPp * P[sym(k, l)] * sig[sym(k, l)] * gm[k] * gm[l]) * 0.5 * dt;
+  PH += dg(-dPP * Pc * du) * 0.5 * dt;
   // full step
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int j = 0; j <= i; j++) {
       const int ij = sym(i, j);
-      F[ij] = taupi > 0.5 * dt ? P[ij] - (H[ij] - piNS[ij]) * dt / gamma / taupi : piNS[ij];
+      F[ij] = taupi > 0.5 * dt ? P[ij] - dtp(dg((H[ij] - piNS[ij]) * dt)) : piNS[ij];
     }
-  double PF = tauPi > 0.5 * dt ? Pc - (PH - PiNS) * dt / gamma / tauPi : PiNS;
-  tau1 = tau - dt * 0.5;
-  F[sym(0, 0)] += -2. * vz / tau1 * H[sym(0, 3)] * dt;
-  F[sym(3, 3)] += -(2. * vz / tau1 * H[sym(0, 3)]) * dt;
-  F[sym(3, 0)] += -(vz / tau1 * H[sym(0, 0)] + vz / tau1 * H[sym(3, 3)]) * dt;
-  F[sym(1, 0)] += -vz / tau1 * H[sym(1, 3)] * dt;
-  F[sym(2, 0)] += -vz / tau1 * H[sym(2, 3)] * dt;
-  F[sym(3, 1)] += -(vz / tau1 * H[sym(0, 1)]) * dt;
-  F[sym(3, 2)] += -(vz / tau1 * H[sym(0, 2)]) * dt;
+  double PF = tauPi > 0.5 * dt ? Pc - dg((PH - PiNS) * dt) / tauPi : PiNS;
+  const Divider df(tau - dt * 0.5);  // x / tau1 of the full step
+  F[sym(0, 0)] += df(-2. * vz) * H[sym(0, 3)] * dt;
+  F[sym(3, 3)] += -(df(2. * vz) * H[sym(0, 3)]) * dt;
+  F[sym(3, 0)] += -(df(vz) * H[sym(0, 0)] + df(vz) * H[sym(3, 3)]) * dt;
+  F[sym(1, 0)] += df(-vz) * H[sym(1, 3)] * dt;
+  F[sym(2, 0)] += df(-vz) * H[sym(2, 3)] * dt;
+  F[sym(3, 1)] += -(df(vz) * H[sym(0, 1)]) * dt;
+  F[sym(3, 2)] += -(df(vz) * H[sym(0, 2)]) * dt;
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int j = 0; j <= i; j++) {
       const int ij = sym(i, j);
       double t = F[ij];
-      t += (-dpp * H[ij] * du + lpP * PH * sig[ij]) / gamma * dt;
+      t += dg(-dpp * H[ij] * du + lpP * PH * sig[ij]) * dt;
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        t += (phi7 * P[sym(i, k)] * H[sym(j, k)] * gm[k] -
-              tpp * 0.5 * (H[sym(i, k)] * sig[sym(j, k)] * gm[k] + H[sym(j, k)] * sig[sym(i, k)] * gm[k])) /
-             gamma * dt;
+        t += dg(phi7 * P[sym(i, k)] * H[sym(j, k)] * gm[k] -
+                tpp * 0.5 * (H[sym(i, k)] * sig[sym(j, k)] * gm[k] + H[sym(j, k)] * sig[sym(i, k)] * gm[k])) *
+             dt;
 #pragma unroll
         for (int l = 0; l < 4; l++)
-          t += ((-H[sym(i, k)] * u[j] - H[sym(j, k)] * u[i]) * u[l] * dmu[l][k] * gm[k] -
-                1. / 3. * D[ij] * H[sym(k, l)] * (phi7 * H[sym(k, l)] - tpp * sig[sym(k, l)]) * gm[k] * gm[l]) /
-               gamma * dt;
+          t += dg((-H[sym(i, k)] * u[j] - H[sym(j, k)] * u[i]) * u[l] * dmu[l][k] * gm[k] -
+                  1. / 3. * D[ij] * H[sym(k, l)] * (phi7 * H[sym(k, l)] - tpp * sig[sym(k, l)]) * gm[k] *
+                      gm[l]) *
+               dt;
       }
       F[ij] = t;
     }
@@ -900,8 +909,8 @@ k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
   for (int k = 0; k < 4; k++)
 #pragma unroll
     for (int l = 0; l < 4; l++)
-      PF += lPp * H[sym(k, l)] * sig[sym(k, l)] * gm[k] * gm[l] / gamma * dt;
-  PF += -dPP * PH * du / gamma * dt;
+      PF += dg(lPp * H[sym(k, l)] * sig[sym(k, l)] * gm[k] * gm[l]) * dt;
+  PF += dg(-dPP * PH * du) * dt;
 #pragma unroll
   for (int k = 0; k < 10; k++) {
     piH0[k * N + c] = H[k];
