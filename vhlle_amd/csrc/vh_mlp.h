@@ -55,11 +55,12 @@ __host__ __device__ inline void features(const double q[7], float f[5]) {
   }
   double M = sqrt(q[X_] * q[X_] + q[Y_] * q[Y_] + q[Z_] * q[Z_]);
   if (M > Qt) M = 0.9999 * Qt;
-  const float x = (float)(M / Qt);
+  const double iq = 1. / Qt;  // one division for the three ratios
+  const float x = (float)(M * iq);
   f[0] = 0.1f * logf((float)Qt);
   f[1] = x;
-  f[2] = (float)(q[Z_] / Qt);
-  f[3] = (float)(q[NB_] / Qt);
+  f[2] = (float)(q[Z_] * iq);
+  f[3] = (float)(q[NB_] * iq);
   const float g = 1.f - x * x;
   f[4] = sqrtf(g > 0.f ? g : 0.f);
 }
@@ -88,49 +89,60 @@ __device__ inline float2 mlp_wave(const Packed &w, const float f[5]) {
   float xp[6];
 #pragma unroll
   for (int k = 0; k < 6; k++) xp[k] = __shfl_xor(x[k], 32, 64);
-  float2 res = make_float2(0.f, 0.f);
-#pragma unroll 1
-  for (int bt = 0; bt < 2; bt++) {
-    // batch bt = the samples of lanes 32 bt .. 32 bt + 31; lanes of the low
-    // half carry input rows 0..7 of sample l % 32, the high half rows 8..15 (zero)
-    bf16x8 b;
+  // the two batches (samples of lanes 0..31 and 32..63) side by side, so
+  // that four independent MFMA chains are in flight; lanes of the low half
+  // carry input rows 0..7 of sample l % 32, the high half rows 8..15 (zero)
+  bf16x8 b0, b1;
 #pragma unroll
-    for (int t = 0; t < 8; t++)
-      b[t] = (__bf16)((half == 0 && t < 6) ? (bt == 0 ? x[t] : xp[t]) : 0.f);
-    const floatx16 zero = {};
-    const floatx16 d1a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w.w1[0][l], b, zero, 0, 0, 0);
-    const floatx16 d1b = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w.w1[1][l], b, zero, 0, 0, 0);
-    floatx16 d2a, d2b;
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-      const int h = 8 * (r >> 2) + 4 * half + (r & 3);
-      d2a[r] = w.b2[h];
-      d2b[r] = w.b2[32 + h];
-    }
-#pragma unroll
-    for (int kb = 0; kb < 4; kb++) {
-      bf16x8 hb;
-#pragma unroll
-      for (int t = 0; t < 8; t++) {
-        const float z = kb < 2 ? d1a[8 * (kb & 1) + t] : d1b[8 * (kb & 1) + t];
-        hb[t] = (__bf16)relu(z);
-      }
-      d2a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w.w2[0][kb][l], hb, d2a, 0, 0, 0);
-      d2b = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w.w2[1][kb][l], hb, d2b, 0, 0, 0);
-    }
-    float o0 = 0.f, o1 = 0.f;
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-      const int h = 8 * (r >> 2) + 4 * half + (r & 3);
-      const float a = relu(d2a[r]), c = relu(d2b[r]);
-      o0 += w.w3[0][h] * a + w.w3[0][32 + h] * c;
-      o1 += w.w3[1][h] * a + w.w3[1][32 + h] * c;
-    }
-    o0 += __shfl_xor(o0, 32, 64);
-    o1 += __shfl_xor(o1, 32, 64);
-    if (half == bt) res = make_float2(o0 + w.b3[0], o1 + w.b3[1]);
+  for (int t = 0; t < 8; t++) {
+    b0[t] = (__bf16)((half == 0 && t < 6) ? x[t] : 0.f);
+    b1[t] = (__bf16)((half == 0 && t < 6) ? xp[t] : 0.f);
   }
-  return res;
+  const floatx16 zero = {};
+  const floatx16 d1a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w.w1[0][l], b0, zero, 0, 0, 0);
+  const floatx16 d1b0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w.w1[1][l], b0, zero, 0, 0, 0);
+  const floatx16 d1a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w.w1[0][l], b1, zero, 0, 0, 0);
+  const floatx16 d1b1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w.w1[1][l], b1, zero, 0, 0, 0);
+  floatx16 d2a0, d2b0, d2a1, d2b1;
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int h = 8 * (r >> 2) + 4 * half + (r & 3);
+    d2a0[r] = d2a1[r] = w.b2[h];
+    d2b0[r] = d2b1[r] = w.b2[32 + h];
+  }
+#pragma unroll
+  for (int kb = 0; kb < 4; kb++) {
+    bf16x8 h0, h1;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      const float z0 = kb < 2 ? d1a0[8 * (kb & 1) + t] : d1b0[8 * (kb & 1) + t];
+      const float z1 = kb < 2 ? d1a1[8 * (kb & 1) + t] : d1b1[8 * (kb & 1) + t];
+      h0[t] = (__bf16)relu(z0);
+      h1[t] = (__bf16)relu(z1);
+    }
+    const bf16x8 wa = w.w2[0][kb][l], wb = w.w2[1][kb][l];
+    d2a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa, h0, d2a0, 0, 0, 0);
+    d2b0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wb, h0, d2b0, 0, 0, 0);
+    d2a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa, h1, d2a1, 0, 0, 0);
+    d2b1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wb, h1, d2b1, 0, 0, 0);
+  }
+  float o00 = 0.f, o10 = 0.f, o01 = 0.f, o11 = 0.f;  // o<output><batch>
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int h = 8 * (r >> 2) + 4 * half + (r & 3);
+    const float wa0 = w.w3[0][h], wb0 = w.w3[0][32 + h], wa1 = w.w3[1][h], wb1 = w.w3[1][32 + h];
+    const float a0 = relu(d2a0[r]), c0 = relu(d2b0[r]), a1 = relu(d2a1[r]), c1 = relu(d2b1[r]);
+    o00 += wa0 * a0 + wb0 * c0;
+    o10 += wa1 * a0 + wb1 * c0;
+    o01 += wa0 * a1 + wb0 * c1;
+    o11 += wa1 * a1 + wb1 * c1;
+  }
+  o00 += __shfl_xor(o00, 32, 64);
+  o10 += __shfl_xor(o10, 32, 64);
+  o01 += __shfl_xor(o01, 32, 64);
+  o11 += __shfl_xor(o11, 32, 64);
+  return half == 0 ? make_float2(o00 + w.b3[0], o10 + w.b3[1])
+                   : make_float2(o01 + w.b3[0], o11 + w.b3[1]);
 }
 
 // Newton polish of the con2prim root.  Same equation f(v) = (Q_t + p) v - M,
