@@ -98,21 +98,30 @@ ALG_BYTES = {
 # shape, con2prim); None where not profiled.
 KIB = 1024
 TRAFFIC = {
-    ("fp64", "visc_relax", None, "reference"): (2 * 1194322 + 2073466) * KIB,
-    ("fp64", "visc_advect", None, "reference"): (2 * 736083 + 843750) * KIB,
-    ("fp64", "visc_flux_update", None, "reference"): (2 * 764654 + 218772) * KIB,
-    ("mixed", "stage_pred", "cells", "closed"): (2 * 93396 + 125000) * KIB,
-    ("mixed", "stage_corr", "cells", "closed"): (2 * 155875 + 125000) * KIB,
-    ("mixed", "stage_pred", "cells", "learned"): (2 * 101350 + 125000) * KIB,
-    ("mixed", "stage_corr", "cells", "learned"): (2 * 166231 + 125000) * KIB,
-    ("mixed", "stage_pred", "rows", None): (2 * 390718 + 126674) * KIB,
-    ("mixed", "visc_relax", None, "closed"): (2 * 353028 + 375000) * KIB,
-    ("mixed", "visc_relax", None, "learned"): (2 * 360648 + 375000) * KIB,
-    ("mixed", "visc_advect", None, None): (2 * 330518 + 437500) * KIB,
-    ("mixed", "visc_flux_update", None, None): (2 * 263193 + 125000) * KIB,
-    ("mixed", "visc_prim", None, "closed"): (2 * 62530 + 281250) * KIB,
-    ("mixed", "visc_prim", None, "learned"): (2 * 70428 + 281250) * KIB,
-    ("mixed", "c2p_guess", None, "learned"): (2 * 62613 + 15625) * KIB,
+    ("fp64", "visc_relax", None, "reference"): (2 * 890037 + 1570517) * KIB,
+    ("fp64", "visc_advect", None, "reference"): (2 * 735496 + 843750) * KIB,
+    ("fp64", "visc_flux_update", None, "reference"): (2 * 764741 + 218772) * KIB,
+    ("mixed", "stage_pred", "cells", "closed"): (2 * 93440 + 130859) * KIB,
+    ("mixed", "stage_corr", "cells", "closed"): (2 * 155957 + 130859) * KIB,
+    ("mixed", "stage_pred", "cells", "newton"): (2 * 93587 + 130859) * KIB,
+    ("mixed", "stage_corr", "cells", "newton"): (2 * 156336 + 130859) * KIB,
+    ("mixed", "stage_pred", "cells", "learned"): (2 * 103567 + 130859) * KIB,
+    ("mixed", "stage_corr", "cells", "learned"): (2 * 166414 + 130859) * KIB,
+    ("mixed", "stage_pred", "rows", "closed"): (2 * 133627 + 127709) * KIB,
+    ("mixed", "stage_corr", "rows", "closed"): (2 * 211703 + 127177) * KIB,
+    ("mixed", "stage_pred", "rows", "newton"): (2 * 132546 + 127060) * KIB,
+    ("mixed", "stage_corr", "rows", "newton"): (2 * 216336 + 126909) * KIB,
+    ("mixed", "stage_pred", "rows", "learned"): (2 * 148043 + 127316) * KIB,
+    ("mixed", "stage_corr", "rows", "learned"): (2 * 238204 + 127058) * KIB,
+    ("mixed", "visc_relax", None, "closed"): (2 * 353639 + 375000) * KIB,
+    ("mixed", "visc_relax", None, "newton"): (2 * 352869 + 375000) * KIB,
+    ("mixed", "visc_relax", None, "learned"): (2 * 360788 + 375000) * KIB,
+    ("mixed", "visc_advect", None, None): (2 * 330505 + 437500) * KIB,
+    ("mixed", "visc_flux_update", None, None): (2 * 262489 + 125000) * KIB,
+    ("mixed", "visc_prim", None, "closed"): (2 * 62524 + 281250) * KIB,
+    ("mixed", "visc_prim", None, "newton"): (2 * 62614 + 281250) * KIB,
+    ("mixed", "visc_prim", None, "learned"): (2 * 70417 + 281250) * KIB,
+    ("mixed", "c2p_guess", None, "learned"): (2 * 62615 + 15625) * KIB,
 }
 
 
