@@ -237,8 +237,8 @@ def run_variant(args, cfg, prec, c2p, ic):
     stage = None
     if p == abi.PREC_MIXED:
         k, cz, ms = ctx.get_stage()
-        stage = dict(kernel={1: "cells", 2: "rows"}.get(k, "auto"), CZ=cz,
-                     tuned_pred_corr_ms=round(ms, 4))
+        stage = dict(kernel={1: "cells", 2: "rows", 3: "bands"}.get(k, "auto"), CZ=cz,
+                     CY=ctx.get_stage_bands(), tuned_pred_corr_ms=round(ms, 4))
     ctx.close()
     print(f"[bench] {prec}/{c2p}: {args.steps} steps in {elapsed:.3f} s, status {st | st2}",
           file=sys.stderr, flush=True)
@@ -381,6 +381,7 @@ def batch_bench(args, c2p):
     st, _ = ctx.run(cfg["tau0"], cfg["dt"], steps)
     el = time.perf_counter() - t0
     k, cz, _ = ctx.get_stage()
+    cy = ctx.get_stage_bands()
     ctx.close()
     # the first 16 events one context each, same steps
     nseq = 16
@@ -407,7 +408,7 @@ def batch_bench(args, c2p):
                 # vh_run replays one graph of 8 steps per 8 steps, single-step
                 # graphs for the rest: every event advances in the same launches
                 graph_launches_per_step=round((steps // 8 + steps % 8) / steps, 4),
-                stage_kernel={1: "cells", 2: "rows"}.get(k, "auto"), CZ=cz,
+                stage_kernel={1: "cells", 2: "rows", 3: "bands"}.get(k, "auto"), CZ=cz, CY=cy,
                 npart_mean=float(npart.mean()), e_max=float(e.max()),
                 one_context_per_event=dict(events=nseq, value=round(seq_cells * steps / seq_t, 1),
                                            batch_speedup=round(v / (seq_cells * steps / seq_t), 2)))

@@ -264,9 +264,13 @@ int vh_set_c2p_mode(vh_ctx_t *ctx, int mode);
 /* Shape of the mixed-precision stage kernel (no counterpart in the
  * reference).  VH_STAGE_CELLS: one thread per cell evaluating its six
  * interfaces; VH_STAGE_ROWS: waves of 64 consecutive cells of an (x, y)
- * plane marching CZ cells in eta, every x- and eta-interface evaluated once.
- * Both give identical results.  VH_STAGE_AUTO (default): an autotuner times
- * the candidates (cells; rows with CZ = 1..32) on the first vh_step/vh_run
+ * plane marching CZ cells in eta, every x- and eta-interface evaluated once;
+ * VH_STAGE_BANDS: waves of 64 cells of the first rows of bands of CY rows,
+ * marching CZ cells in eta and walking the CY rows of their bands, the
+ * y-interfaces inside a band evaluated once as well (vh_set_stage takes
+ * CY = 4 for it; vh_set_stage_bands sets CY = 2, 4 or 8).  All give identical
+ * results.  VH_STAGE_AUTO (default): an autotuner times
+ * the candidates (cells; rows with CZ = 1..32; bands) on the first vh_step/vh_run
  * (and after vh_set_c2p_mode) and keeps the fastest; 32 steps later it times
  * that shape against the best of the other kind again, on the developed
  * flow, and keeps the faster (within a vh_run if one is going on: the
@@ -274,9 +278,14 @@ int vh_set_c2p_mode(vh_ctx_t *ctx, int mode);
  * the kernel in use (VH_STAGE_AUTO: not tuned yet), CZ and the tuned time of
  * a predictor and a corrector stage in ms.  Mixed precision only (VH_ERR_ARG
  * otherwise). */
-enum { VH_STAGE_AUTO = 0, VH_STAGE_CELLS = 1, VH_STAGE_ROWS = 2 };
+enum { VH_STAGE_AUTO = 0, VH_STAGE_CELLS = 1, VH_STAGE_ROWS = 2, VH_STAGE_BANDS = 3 };
 int vh_set_stage(vh_ctx_t *ctx, int kind, int CZ);
 int vh_get_stage(const vh_ctx_t *ctx, int *kind, int *CZ, double *ms);
+/* The band shape with CY rows per band (2, 4 or 8) and CZ eta cells per wave
+ * (no counterpart in the reference); vh_get_stage_bands reports CY of the
+ * shape in use (0 when it is not a band shape). */
+int vh_set_stage_bands(vh_ctx_t *ctx, int CY, int CZ);
+int vh_get_stage_bands(const vh_ctx_t *ctx, int *CY);
 /* Per-kernel timing over the next vh_run: enable=1 records HIP events around
  * every launch on the context's stream; vh_kernel_times returns, for up to
  * max kernels, the name, launch count and summed milliseconds. */

@@ -68,14 +68,17 @@ GRIDS = [(23, 23, 1, (-1.1, 1.1), (-1.1, 1.1), (-0.5, 0.5)),
 
 @pytest.mark.parametrize("c2p,stage", [(abi.C2P_CLOSED, abi.STAGE_AUTO),
                                        (abi.C2P_CLOSED, abi.STAGE_ROWS),
-                                       (abi.C2P_NEWTON, abi.STAGE_CELLS)])
+                                       (abi.C2P_NEWTON, abi.STAGE_CELLS),
+                                       (abi.C2P_NEWTON, abi.STAGE_BANDS)])
 def test_batch_equals_single_grids(gpu, c2p, stage):
     trc = setups.batch_trc()
     e, _ = abi.hotspot_ic(GRIDS, seed=3, amp=5.2, eta_plateau=0.2, eta_sigma=0.3)
     z = np.zeros_like(e)
     b = abi.Context(grids=GRIDS, trc=trc, precision=abi.PREC_MIXED)
     b.set_c2p(c2p)
-    if stage != abi.STAGE_AUTO:
+    if stage == abi.STAGE_BANDS:  # bands of 4 rows: ragged against ny = 23, 27, 19
+        b.set_stage_bands(4, 3)
+    elif stage != abi.STAGE_AUTO:
         b.set_stage(stage, 3)
     b.set_prim(0.5, e, z, z, z)
     st, _ = b.run(0.5, 0.02, 12)

@@ -269,7 +269,7 @@ def test_signal_speed_mixed_close_to_oracle(gpu, c2p):
     # the row stage kernel (each x and eta interface evaluated once) reduces
     # the same maxima as the cell kernel, on a fresh context (set_prim does
     # not reset pi, as Cell::setPrimVar does not)
-    for kind, cz in ((abi.STAGE_CELLS, 0), (abi.STAGE_ROWS, 2)):
+    for kind, cz in ((abi.STAGE_CELLS, 0), (abi.STAGE_ROWS, 2), (abi.STAGE_BANDS, 2)):
         c2, _ = _viscous_pair(abi.PREC_MIXED, c2p=c2p)
         c2.set_stage(kind, cz)
         st, _ = c2.run(0.8, 0.04, 4)

@@ -147,21 +147,29 @@ def test_fp64_path_refuses_non_reference_c2p(gpu):
                                       ((24, 20, 9), abi.C2P_LEARNED)])
 def test_tile_shapes_identical(gpu, dims, c2p):
     """The row stage kernel (each x- and eta-interface evaluated once; any eta
-    chunk, autotuned or forced) gives bit-identical states to the per-cell
+    chunk, autotuned or forced) and the band kernel (the y-interfaces inside
+    a band evaluated once as well) give bit-identical states to the per-cell
     kernel."""
     nx, ny, nz = dims
     xl, yl, zl = (-4.0, 4.0), (-3.5, 3.5), (-1.2, 1.2)
     ic = setups.gaussian3d_ic(nx, ny, nz, xl, yl, zl, e0=30.0, sx=1.5, sy=1.2, eta_plateau=0.4,
                               eta_sigma=0.4)
+    # (kind, CZ) or (STAGE_BANDS, CZ, CY): bands of 2, 4 and 8 rows, ragged
+    # against ny = 29, 41 and 20, with eta chunks of 1, 3 and 13 cells
     shapes = [(abi.STAGE_CELLS, 0), (abi.STAGE_AUTO, 0), (abi.STAGE_ROWS, 1),
-              (abi.STAGE_ROWS, 5), (abi.STAGE_ROWS, 32)]
+              (abi.STAGE_ROWS, 5), (abi.STAGE_ROWS, 32), (abi.STAGE_BANDS, 1, 2),
+              (abi.STAGE_BANDS, 3, 4), (abi.STAGE_BANDS, 13, 8)]
     model = abi.mlp_init(9) if c2p == abi.C2P_LEARNED else None
     ref = None
     for sh in shapes:
         ctx = abi.Context(nx, ny, nz, xl, yl, zl, trc=abi.make_trc(etaS=0.1),
                           precision=abi.PREC_MIXED)
         ctx.set_c2p(c2p, model)
-        ctx.set_stage(*sh)
+        if sh[0] == abi.STAGE_BANDS:
+            ctx.set_stage_bands(sh[2], sh[1])
+            assert ctx.get_stage_bands() == sh[2]
+        else:
+            ctx.set_stage(*sh)
         ctx.set_prim(0.6, *ic)
         st, _ = ctx.run(0.6, 0.02, 6)
         assert st == 0

@@ -19,7 +19,7 @@ C2P_REFERENCE, C2P_LEARNED, C2P_CLOSED, C2P_NEWTON = 0, 1, 2, 3
 START_MLP, START_HALF = 0, 1
 FO_QPREV, FO_SNAPSHOT = 0, 1
 BETA_COLS = 33  # VH_BETA_COLS: columns of a beta.dat record
-STAGE_AUTO, STAGE_CELLS, STAGE_ROWS = 0, 1, 2
+STAGE_AUTO, STAGE_CELLS, STAGE_ROWS, STAGE_BANDS = 0, 1, 2, 3
 CELLS_NOALLOC_MAX = 8192
 MLP_NPARAM = 4674
 
@@ -300,6 +300,8 @@ def lib():
         "vh_mlp_harvest": (ctypes.c_int, [ctypes.POINTER(HarvestCfg), ctypes.c_long, _dp, _dp]),
         "vh_set_stage": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
         "vh_get_stage": (ctypes.c_int, [vp, ip, ip, _dp]),
+        "vh_set_stage_bands": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
+        "vh_get_stage_bands": (ctypes.c_int, [vp, ip]),
         "vh_hotspot_ic": (ctypes.c_int, [ctypes.POINTER(HotspotCfg), ctypes.c_int,
                                          ctypes.POINTER(Grid), _dp, ip, _dp, ctypes.c_long]),
         "vh_copy_bandwidth": (ctypes.c_int, [ctypes.c_long, ctypes.c_int, _dp]),
@@ -347,7 +349,7 @@ def exported_symbols():
             "vh_copy_bandwidth", "vh_fo_snapshot", "vh_freezeout", "vh_cornelius",
             "vh_eos_table", "vh_eos_eval", "vh_set_surface_eos", "vh_set_vorticity", "vh_get_dbeta", "vh_freezeout_beta",
             "vh_corona", "vh_prim_fields", "vh_fluid_stats", "vh_cells", "vh_signal_speed",
-            "vh_cells_noalloc"]
+            "vh_cells_noalloc", "vh_set_stage_bands", "vh_get_stage_bands"]
 
 
 def _p(a):
@@ -582,12 +584,20 @@ class Context:
     def set_stage(self, kind=0, CZ=0):
         check(lib().vh_set_stage(self.h, kind, CZ), "vh_set_stage")
 
+    def set_stage_bands(self, CY, CZ):
+        check(lib().vh_set_stage_bands(self.h, CY, CZ), "vh_set_stage_bands")
+
     def get_stage(self):
         k, cz = ctypes.c_int(), ctypes.c_int()
         ms = ctypes.c_double()
         check(lib().vh_get_stage(self.h, ctypes.byref(k), ctypes.byref(cz), ctypes.byref(ms)),
               "vh_get_stage")
         return k.value, cz.value, ms.value
+
+    def get_stage_bands(self):
+        cy = ctypes.c_int()
+        check(lib().vh_get_stage_bands(self.h, ctypes.byref(cy)), "vh_get_stage_bands")
+        return cy.value
 
     def fo_snapshot(self, tau):
         check(lib().vh_fo_snapshot(self.h, tau), "vh_fo_snapshot")

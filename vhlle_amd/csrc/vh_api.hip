@@ -129,7 +129,7 @@ struct vh_ctx {
   // stage kernel shape (mixed precision): kind 0 until the autotuner ran.
   // tune_stage 1: tuned on the state of the first run (tile_alt: the best
   // shape of the other kind); 2: tuned again on the developed flow
-  vh::mx::Tile tile = {0, 0}, tile_alt = {0, 0};
+  vh::mx::Tile tile = {0, 0, 0}, tile_alt = {0, 0, 0};
   bool tile_forced = false;
   double tile_ms = 0.;
   int tune_stage = 0;
@@ -447,6 +447,7 @@ bool fusable_extent(int n) { return n == 1 || n >= 5; }
 int build_grid_tables(vh_ctx *c) {
   const int nev = (int)c->evh.size();
   std::vector<int> bev, bfirst(nev), sev, sfirst(nev);
+  std::vector<int> bsev[3], bsfirst[3];  // band segments of CY = 2, 4, 8
   int maxnz = 1;
   for (int e = 0; e < nev; e++) {
     const vh::Geom &g = c->evh[e];
@@ -455,12 +456,18 @@ int build_grid_tables(vh_ctx *c) {
     sfirst[e] = (int)sev.size();
     const long nxy = (long)g.nx * g.ny;
     for (long q = 0; q < (nxy + 62) / 63; q++) sev.push_back(e);
+    for (int h = 0; h < 3; h++) {
+      const long CY = 2L << h, nbp = (g.ny + CY - 1) / CY * g.nx;
+      bsfirst[h].push_back((int)bsev[h].size());
+      for (long q = 0; q < (nbp + 62) / 63; q++) bsev[h].push_back(e);
+    }
     maxnz = std::max(maxnz, g.nz);
   }
   const size_t al = 256;
   auto rnd = [&](size_t b) { return (b + al - 1) / al * al; };
-  const size_t sz = rnd(nev * sizeof(vh::Geom)) + rnd(bev.size() * 4) + rnd(nev * 4) +
-                    rnd(sev.size() * 4) + rnd(nev * 4);
+  size_t sz = rnd(nev * sizeof(vh::Geom)) + rnd(bev.size() * 4) + rnd(nev * 4) +
+              rnd(sev.size() * 4) + rnd(nev * 4);
+  for (int h = 0; h < 3; h++) sz += rnd(bsev[h].size() * 4) + rnd(nev * 4);
   VH_HIP(hipMalloc(&c->gtab, sz));
   char *p = c->gtab;
   auto put = [&](const void *src, size_t bytes) -> char * {
@@ -474,6 +481,13 @@ int build_grid_tables(vh_ctx *c) {
   char *ps = put(sev.data(), sev.size() * 4), *psf = put(sfirst.data(), nev * 4);
   if (!ev || !pb || !pbf || !ps || !psf) return VH_ERR_HIP;
   vh::mx::Grids &G = c->G;
+  for (int h = 0; h < 3; h++) {
+    char *a = put(bsev[h].data(), bsev[h].size() * 4), *b = put(bsfirst[h].data(), nev * 4);
+    if (!a || !b) return VH_ERR_HIP;
+    G.bsev[h] = reinterpret_cast<const int *>(a);
+    G.bsfirst[h] = reinterpret_cast<const int *>(b);
+    G.nbseg[h] = (int)bsev[h].size();
+  }
   G.ev = reinterpret_cast<const vh::Geom *>(ev);
   G.bev = reinterpret_cast<const int *>(pb);
   G.bfirst = reinterpret_cast<const int *>(pbf);
@@ -833,10 +847,22 @@ int vh_set_stage(vh_ctx_t *c, int kind, int CZ) {
   } else if (kind == VH_STAGE_ROWS) {
     if (CZ < 1) return VH_ERR_ARG;
     t = vh::mx::make_rows(c->G, CZ);
+  } else if (kind == VH_STAGE_BANDS) {
+    return vh_set_stage_bands(c, 4, CZ);
   } else {
     return VH_ERR_ARG;
   }
   c->tile = t;
+  c->tile_forced = true;
+  c->tile_ms = 0.;
+  drop_graphs(c);
+  return VH_OK;
+}
+
+int vh_set_stage_bands(vh_ctx_t *c, int CY, int CZ) {
+  if (!c || c->precision != VH_PREC_MIXED || CZ < 1 || vh::mx::band_table(CY) < 0)
+    return VH_ERR_ARG;
+  c->tile = vh::mx::make_bands(c->G, CY, CZ);
   c->tile_forced = true;
   c->tile_ms = 0.;
   drop_graphs(c);
@@ -848,6 +874,12 @@ int vh_get_stage(const vh_ctx_t *c, int *kind, int *CZ, double *ms) {
   *kind = c->tile.kind;
   *CZ = c->tile.CZ;
   if (ms) *ms = c->tile_ms;
+  return VH_OK;
+}
+
+int vh_get_stage_bands(const vh_ctx_t *c, int *CY) {
+  if (!c || !CY) return VH_ERR_ARG;
+  *CY = c->tile.kind == VH_STAGE_BANDS ? c->tile.CY : 0;
   return VH_OK;
 }
 
@@ -920,7 +952,7 @@ static int ensure_tile(vh_ctx_t *c, double tau, double dt) {
     // iterations than the cell kernel's, so a row shape is kept only when it
     // is clearly faster here (the second tuning, on the developed flow,
     // decides without this margin)
-    const bool rows = t.kind == vh::mx::STAGE_ROWS, brows = bt.kind == vh::mx::STAGE_ROWS;
+    const bool rows = t.kind != vh::mx::STAGE_CELLS, brows = bt.kind != vh::mx::STAGE_CELLS;
     const float m = rows == brows ? 1.f : (rows ? 0.95f : 1.05f);
     if (ms < best * m) {
       best = ms;

@@ -746,6 +746,9 @@ __device__ __forceinline__ void wave_speed_max(unsigned long long *grp, double v
 #ifndef VH_W_MX_ROWS
 #define VH_W_MX_ROWS 0
 #endif
+#ifndef VH_W_MX_BANDS
+#define VH_W_MX_BANDS 0
+#endif
 #ifndef VH_W_MX_VPRIM
 #define VH_W_MX_VPRIM 0
 #endif

@@ -24,7 +24,15 @@ struct Grids {
   // cell order of the per-cell kernels: y-strips of sy rows swept over all
   // eta, XCD-contiguous workgroups (0: plain x-fastest order)
   int sy;
+  // Band launches (bands of CY = 2, 4, 8 rows, table h = 0, 1, 2): nbseg[h]
+  // segments of 63 cells of the band planes (the first rows of the bands laid
+  // end to end); segment s belongs to grid bsev[h][s] and is its segment
+  // s - bsfirst[h][grid].  Device arrays.
+  const int *bsev[3], *bsfirst[3];
+  int nbseg[3];
 };
+// table index of a band height CY (2, 4, 8), -1 for any other
+inline int band_table(int CY) { return CY == 2 ? 0 : (CY == 4 ? 1 : (CY == 8 ? 2 : -1)); }
 
 // gv: per-cell con2prim guesses (used by VH_C2P_LEARNED only, may be null otherwise)
 // sig: the stage's signal-speed slot groups (x, y, eta, kSigGroup words
@@ -33,16 +41,21 @@ void launch_stage(int mode, int cm, int nc, const Grids &G, const DevEos &eos, c
                   const double *Qh, const float *gv, const double *m, int m_full,
                   const double *tp, double dt, double *out, int *status,
                   unsigned long long *sig, hipStream_t st);
-// Stage kernel shapes (same results for both): STAGE_CELLS one thread per
+// Stage kernel shapes (same results for all): STAGE_CELLS one thread per
 // cell evaluating its six interfaces; STAGE_ROWS waves of 64 flattened (x, y)
-// cells marching CZ cells in eta, every x- and eta-interface evaluated once.
-enum { STAGE_CELLS = VH_STAGE_CELLS, STAGE_ROWS = VH_STAGE_ROWS };
+// cells marching CZ cells in eta, every x- and eta-interface evaluated once;
+// STAGE_BANDS waves of 64 cells of the band plane marching CZ cells in eta
+// and walking the CY rows of their bands, every interface evaluated once but
+// the y-interfaces between bands and the eta-interfaces between chunks.
+enum { STAGE_CELLS = VH_STAGE_CELLS, STAGE_ROWS = VH_STAGE_ROWS, STAGE_BANDS = VH_STAGE_BANDS };
 struct Tile {
-  int kind, CZ;
+  int kind, CZ, CY;
 };
 Tile make_rows(const Grids &G, int CZ);
+Tile make_bands(const Grids &G, int CY, int CZ);
 Tile make_cells();
-// candidates of the autotuner: the cell kernel and rows with several eta chunks
+// candidates of the autotuner: the cell kernel, rows and bands with several
+// eta chunks
 std::vector<Tile> tile_candidates(const Grids &G);
 void launch_stage_tile(int mode, int cm, int nc, const Grids &G, const DevEos &eos, const double *Q,
                        const double *Qh, const float *gv, const double *m, int m_full,

@@ -93,6 +93,19 @@ __device__ __forceinline__ Geom seg_grid(const Grids &G, long sgl, long &seg) {
   seg = sgl - G.sfirst[gi];
   return G.ev[gi];
 }
+// Element c of an array whose base is the same for the whole wave: a context
+// holds fewer than 2^29 cells (checked at creation), so the byte offset fits
+// 32 bits and the load takes the base in scalar registers and a 32-bit lane
+// offset (no 64-bit address arithmetic per lane).
+template <class T>
+__device__ __forceinline__ T ldu(const T *__restrict__ a, long c) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(a) +
+                                      ((unsigned)c * (unsigned)sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ void stu(T *__restrict__ a, long c, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(a) + ((unsigned)c * (unsigned)sizeof(T))) = v;
+}
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float fmin_(float a, float b) { return (b < a) ? b : a; }
@@ -320,7 +333,19 @@ __device__ __forceinline__ void c2p_mode(const DevEos &eos, double q[7], Prim &r
 }
 template <int CM>
 __device__ __forceinline__ float guess_of(const float *__restrict__ gv, long c) {
-  return CM == VH_C2P_LEARNED ? gv[c] : 0.5f;
+  return CM == VH_C2P_LEARNED ? ldu(gv, c) : 0.5f;
+}
+
+// The kernels that evaluate the EoS are compiled per EoS kind EK (VH_EOS_IDEAL
+// or VH_EOS_TABLE, the kinds the mixed path takes; kinds 2 and 3 are refused
+// when the precision is set).  The kind is fixed at kernel entry, so an
+// ideal-gas kernel carries none of the spline code inlined at every EoS call
+// (the corrector's row kernel shrinks from 77 to 29 KB of code).
+template <int EK>
+__device__ __forceinline__ DevEos eos_of_kind(const DevEos &a) {
+  DevEos r = a;
+  r.kind = EK;
+  return r;
 }
 
 // MUSCL face state (src/cll.cpp:106-204) in FP64, con2prim by mode, as the
@@ -337,11 +362,13 @@ __device__ __forceinline__ PrimF face_state(const DevEos &eos, const double *__r
       q[k] = 0.;
       continue;
     }
-    const double a = Qs[k * N + c0];
+    const double *Qk = Qs + k * N;
+    const double a = ldu(Qk, c0);
     // minmod(x/2, y/2) = minmod(x, y)/2 (the halving is exact; only products
-    // x y below ~1e-308 can decide the sign test differently)
-    const double dq = 0.5 * minmod(Qs[k * N + cp] - a, a - Qs[k * N + cm]);
-    q[k] = (SIDE < 0 ? a - dq : a + dq) * itau;
+    // x y below ~1e-308 can decide the sign test differently), and a -+ mm/2
+    // as one fused operation rounds as a -+ (mm/2) does
+    const double mm = minmod(ldu(Qk, cp) - a, a - ldu(Qk, cm));
+    q[k] = fma(SIDE < 0 ? -0.5 : 0.5, mm, a) * itau;
   }
   if (CM == VH_C2P_CLOSED) return c2p_fast(q);
   if (CM == VH_C2P_LEARNED || CM == VH_C2P_NEWTON) return c2p_newton_f32(eos, q, guess_of<CM>(gv, c0));
@@ -361,7 +388,7 @@ __device__ __forceinline__ PrimF face_state(const DevEos &eos, const double *__r
 template <int D, int NC, class RE>
 __device__ __forceinline__ bool hlle_f32(PrimF L, PrimF R, float itau, float itf, float pre,
                                          bool mskip, float F[7], int *status, float &sp,
-                                         RE &&reeval) {
+                                         bool table, RE &&reeval) {
   if (L.e < 0.f) L.e = L.p = 0.f;
   if (R.e < 0.f) R.e = R.p = 0.f;
   if (L.e > 1e10f) {
@@ -370,7 +397,9 @@ __device__ __forceinline__ bool hlle_f32(PrimF L, PrimF R, float itau, float itf
   }
   if (L.e == 0.f && R.e == 0.f) return false;
   const float El = (L.e + L.p) * L.g2, Er = (R.e + R.p) * R.g2;
-  if (R.p < 0.f) reeval(L, R);  // table EoS only: p = e/3 >= 0 here
+  // table EoS only: p = e/3 >= 0 for e >= 0 (a NaN fails the test as well), so
+  // an ideal-gas kernel carries no re-evaluation code
+  if (table && R.p < 0.f) reeval(L, R);
   if (mskip) return false;
   const float g2l = L.g2, g2r = R.g2;
   const float wl = (L.e + L.p) * g2l, wr = (R.e + R.p) * g2r;
@@ -473,24 +502,24 @@ __device__ __forceinline__ void axis_fluxes(const Geom &g, const DevEos &eos,
   if (i > 0) {
     const long cmm = i > 1 ? cm - s : cm;
     const PrimF Rm = face_state<+1, CM, NC>(eos, Qs, gv, N, cmm, cm, c, itau);
-    const bool mk = !m_full && m[(D - 1) * N + cm] < 1. && m[(D - 1) * N + c] < 1.;
+    const bool mk = !m_full && ldu(m + (D - 1) * N, cm) < 1. && ldu(m + (D - 1) * N, c) < 1.;
     auto re = [&](PrimF &a, PrimF &b) {
       a = face_state<+1, CM, NC>(eos, Qr, gv, N, cmm, cm, c, itau);
       b = face_state<-1, CM, NC>(eos, Qr, gv, N, cm, c, cp, itau);
     };
-    if (hlle_f32<D, NC>(Rm, L0, itauf, itf, pre, mk, F, status, sp, re))
+    if (hlle_f32<D, NC>(Rm, L0, itauf, itf, pre, mk, F, status, sp, eos.kind == VH_EOS_TABLE, re))
 #pragma unroll
       for (int k = 0; k < NC; k++) acc[k] += (double)F[k];
   }
   if (i < n - 1) {
     const long cpp = i + 2 < n ? cp + s : cp;
     const PrimF Lp = face_state<-1, CM, NC>(eos, Qs, gv, N, c, cp, cpp, itau);
-    const bool mk = !m_full && m[(D - 1) * N + c] < 1. && m[(D - 1) * N + cp] < 1.;
+    const bool mk = !m_full && ldu(m + (D - 1) * N, c) < 1. && ldu(m + (D - 1) * N, cp) < 1.;
     auto re = [&](PrimF &a, PrimF &b) {
       a = face_state<+1, CM, NC>(eos, Qr, gv, N, cm, c, cp, itau);
       b = face_state<-1, CM, NC>(eos, Qr, gv, N, c, cp, cpp, itau);
     };
-    if (hlle_f32<D, NC>(R0, Lp, itauf, itf, pre, mk, F, status, sp, re))
+    if (hlle_f32<D, NC>(R0, Lp, itauf, itf, pre, mk, F, status, sp, eos.kind == VH_EOS_TABLE, re))
 #pragma unroll
       for (int k = 0; k < NC; k++) acc[k] -= (double)F[k];
   }
@@ -509,7 +538,7 @@ __device__ __forceinline__ void finish_cell(const DevEos &eos, const double *__r
   const double it1 = drcp(tau1);
   double q[7];
 #pragma unroll
-  for (int k = 0; k < 7; k++) q[k] = k < NC ? Qs[k * N + c] * it1 : 0.;
+  for (int k = 0; k < 7; k++) q[k] = k < NC ? ldu(Qs + k * N, c) * it1 : 0.;
   double vz, pr;
   if (CM == VH_C2P_CLOSED) {
     const PrimF r = c2p_fast(q);
@@ -529,11 +558,14 @@ __device__ __forceinline__ void finish_cell(const DevEos &eos, const double *__r
   acc[Z_] += -q[Z_] * sdt;
   if (MODE == 0) {
 #pragma unroll
-    for (int k = 0; k < NC; k++) out[k * N + c] = Q[k * N + c] + acc[k];
+    for (int k = 0; k < NC; k++) stu(out + k * N, c, ldu(Q + k * N, c) + acc[k]);
   } else {
-    const bool skip = Q[c] + acc[0] < 0.;
+    const bool skip = ldu(Q, c) + acc[0] < 0.;
 #pragma unroll
-    for (int k = 0; k < NC; k++) out[k * N + c] = skip ? Q[k * N + c] : Q[k * N + c] + acc[k];
+    for (int k = 0; k < NC; k++) {
+      const double q0 = ldu(Q + k * N, c);
+      stu(out + k * N, c, skip ? q0 : q0 + acc[k]);
+    }
   }
 }
 
@@ -543,12 +575,13 @@ __device__ __forceinline__ void finish_cell(const DevEos &eos, const double *__r
 // Q0 + flux0 < 0 (src/cll.cpp:70-74).
 // sig: the stage's three signal-speed slot groups (x, y, eta; kSigGroup
 // words apart), one atomic per wave and group.
-template <int MODE, int CM, int NC>
+template <int MODE, int CM, int NC, int EK>
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_MX_STAGE)
-k_stage(Grids G, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
+k_stage(Grids G, DevEos eos_a, const double *__restrict__ Q, const double *__restrict__ Qh,
         const float *__restrict__ gv, const double *__restrict__ m, int m_full,
         const double *__restrict__ tp, double dt, double *__restrict__ out,
         int *__restrict__ status, unsigned long long *__restrict__ sig) {
+  const DevEos eos = eos_of_kind<EK>(eos_a);
   const double tau = *tp;
   Geom g;
   long c;
@@ -585,12 +618,13 @@ __device__ __forceinline__ bool iface(const DevEos &eos, const double *__restric
                                       float itf, float pre, float F[7], int *status, float &sp) {
   const PrimF L = face_state<+1, CM, NC>(eos, Qs, gv, N, amm, a, b, itau);
   const PrimF R = face_state<-1, CM, NC>(eos, Qs, gv, N, a, b, bp, itau);
-  const bool mk = !m_full && m[(D - 1) * N + a] < 1. && m[(D - 1) * N + b] < 1.;
+  const bool mk = !m_full && ldu(m + (D - 1) * N, a) < 1. && ldu(m + (D - 1) * N, b) < 1.;
   auto re = [&](PrimF &l, PrimF &r) {
     l = face_state<+1, CM, NC>(eos, Qr, gv, N, amm, a, b, itau);
     r = face_state<-1, CM, NC>(eos, Qr, gv, N, a, b, bp, itau);
   };
-  return hlle_f32<D, NC>(L, R, itauf, itf, pre, mk, F, status, sp, re);
+  return hlle_f32<D, NC>(L, R, itauf, itf, pre, mk, F, status, sp, eos.kind == VH_EOS_TABLE,
+                         re);
 }
 
 // Row stage: every x- and eta-interface is evaluated once, without barriers.
@@ -601,12 +635,13 @@ __device__ __forceinline__ bool iface(const DevEos &eos, const double *__restric
 // upper eta-interface stays in registers as the next cell's lower one; the
 // two y-interfaces are evaluated per lane.  Interface states, fluxes and the
 // accumulation order are those of k_stage: identical results.
-template <int MODE, int CM, int NC>
+template <int MODE, int CM, int NC, int EK>
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_MX_ROWS)
-k_stage_rows(Grids G, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
+k_stage_rows(Grids G, DevEos eos_a, const double *__restrict__ Q, const double *__restrict__ Qh,
              const float *__restrict__ gv, const double *__restrict__ m, int m_full,
              const double *__restrict__ tp, double dt, double *__restrict__ out,
              int *__restrict__ status, unsigned long long *__restrict__ sig, int CZ) {
+  const DevEos eos = eos_of_kind<EK>(eos_a);
   const int lane = threadIdx.x & 63;
   // XCD-contiguous workgroups: the segments an XCD processes at once are
   // consecutive, so the y-neighbour rows of a segment (nx cells away) were
@@ -699,12 +734,172 @@ k_stage_rows(Grids G, DevEos eos, const double *__restrict__ Q, const double *__
   wave_speed_max(sig + 2 * kSigGroup, (double)spz);
 }
 
+// Band stage: the y-interfaces shared as well.  The band plane of a grid
+// lays the first rows of its bands of CY rows end to end (cell (ix, band) at
+// band * nx + ix).  A wave owns 64 consecutive cells of it (waves overlap by
+// one cell, so 63 of 64 lanes update cells whatever nx and CY are) and
+// marches CZ cells in eta; at each eta cell every lane walks the CY rows of
+// its band.  Lane l evaluates the lower x-interface of its cell and receives
+// the upper one from lane l + 1 by a cross-lane shuffle; the upper
+// y-interface of a row is carried in registers to the next row of the band
+// (the first row of a band evaluates its lower one); the upper eta-interface
+// of each row is kept in LDS, in the lane's own slots (no barrier), for the
+// same row at the next eta cell.  Per cell 3 + (1/CY + 1/CZ) interfaces
+// instead of 4 + 1/CZ in k_stage_rows.  Interface states, fluxes and the
+// accumulation order are those of k_stage: identical results.
+// bsev, bsfirst, nseg: the band segments of this CY over all grids.
+template <int MODE, int CM, int NC, int EK>
+__global__ void __launch_bounds__(256) VH_OCC(VH_W_MX_BANDS)
+k_stage_bands(Grids G, const int *__restrict__ bsev, const int *__restrict__ bsfirst, long nseg,
+              DevEos eos_a, const double *__restrict__ Q, const double *__restrict__ Qh,
+              const float *__restrict__ gv, const double *__restrict__ m, int m_full,
+              const double *__restrict__ tp, double dt, double *__restrict__ out,
+              int *__restrict__ status, unsigned long long *__restrict__ sig, int CY, int CZ) {
+  constexpr int NV = NC > 4 ? 2 : 1;  // float4 slots of a carried flux
+  extern __shared__ float4 fzs[];     // [CY][NV][blockDim.x]
+  const DevEos eos = eos_of_kind<EK>(eos_a);
+  const int lane = threadIdx.x & 63;
+  const long w = xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long sgl = w % nseg, zc = w / nseg;  // band segment over all grids, eta chunk
+  long seg = sgl;
+  Geom g = G.g0;
+  if (G.nev > 1) {
+    const int gi = bsev[sgl];
+    seg = sgl - bsfirst[gi];
+    g = G.ev[gi];
+  }
+  const int z0 = (int)zc * CZ;
+  if (z0 >= g.nz) return;  // whole wave
+  const int z1 = min(g.nz, z0 + CZ);
+  const unsigned nxu = (unsigned)g.nx, nbu = ((unsigned)g.ny + (unsigned)CY - 1u) / (unsigned)CY;
+  const long f = seg * 63 + lane;
+  const bool cv = f < (long)nbu * nxu;
+  const unsigned fu = (unsigned)f;
+  const int band = cv ? (int)(fu / nxu) : 0, ix = cv ? (int)(fu - (unsigned)band * nxu) : 0;
+  const int y0 = band * CY, nrow = cv ? min(CY, g.ny - y0) : 0;
+  const long N = g.P, sy = g.nx, sz = (long)g.nx * g.ny;
+  const double tau = *tp;
+  const double *Qs = MODE == 0 ? Q : Qh;
+  const double itau = drcp(tau);  // as in axis_fluxes: identical interface states
+  const double tauFactor = MODE == 0 ? tau + 0.25 * dt : tau + 0.5 * dt;
+  const double dta = MODE == 0 ? 0.5 * dt : dt;
+  const float prx = (float)(tauFactor * dta / g.dx), pry = (float)(tauFactor * dta / g.dy),
+              prz = (float)(tauFactor * dta / g.dz);
+  const float itf = (float)drcp(tauFactor), itauf = (float)itau;
+  float4 *slot = fzs + threadIdx.x;  // row j, part v: slot[(j * NV + v) * blockDim.x]
+  unsigned hzm = 0u;                  // bit j: row j carries an eta flux
+  float spx = 0.f, spy = 0.f, spz = 0.f;
+  for (int iz = z0; iz < z1; iz++) {
+    float fyc[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    bool hyc = false;
+#pragma unroll 1
+    for (int j = 0; j < CY; j++) {  // uniform trip count: every lane shuffles
+      const bool cx = j < nrow;
+      const int iy = y0 + j;
+      const long c = cx ? g.base + ix + iy * sy + iz * sz : 0;
+      float fx[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      bool hx = false;
+      if (cx && ix >= 1)
+        hx = iface<1, CM, NC>(eos, Qs, Q, gv, m, m_full, N, ix >= 2 ? c - 2 : c - 1, c - 1, c,
+                              ix + 1 < g.nx ? c + 1 : c, itau, itauf, itf, prx, fx, status, spx);
+      float fxu[7];
+#pragma unroll
+      for (int k = 0; k < NC; k++) fxu[k] = __shfl_down(fx[k], 1, 64);
+      const bool hxu = __shfl_down(hx ? 1 : 0, 1, 64) != 0 && ix + 1 < g.nx;
+      float fyl[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      bool hyl = false;
+      if (j == 0) {
+        if (cx && iy >= 1)
+          hyl = iface<2, CM, NC>(eos, Qs, Q, gv, m, m_full, N, iy >= 2 ? c - 2 * sy : c - sy,
+                                 c - sy, c, iy + 1 < g.ny ? c + sy : c, itau, itauf, itf, pry,
+                                 fyl, status, spy);
+      } else {
+#pragma unroll
+        for (int k = 0; k < NC; k++) fyl[k] = fyc[k];
+        hyl = hyc;
+      }
+      float fyu[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      bool hyu = false;
+      if (cx && iy + 1 < g.ny)
+        hyu = iface<2, CM, NC>(eos, Qs, Q, gv, m, m_full, N, iy >= 1 ? c - sy : c, c, c + sy,
+                               iy + 2 < g.ny ? c + 2 * sy : c + sy, itau, itauf, itf, pry, fyu,
+                               status, spy);
+      float fzl[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      bool hzl = false;
+      if (iz == z0) {
+        if (cx && iz >= 1)
+          hzl = iface<3, CM, NC>(eos, Qs, Q, gv, m, m_full, N, iz >= 2 ? c - 2 * sz : c - sz,
+                                 c - sz, c, iz + 1 < g.nz ? c + sz : c, itau, itauf, itf, prz,
+                                 fzl, status, spz);
+      } else {
+        hzl = (hzm >> j) & 1u;
+        if (hzl) {
+#pragma unroll
+          for (int v = 0; v < NV; v++) {
+            const float4 t = slot[(j * NV + v) * blockDim.x];
+            fzl[4 * v] = t.x;
+            fzl[4 * v + 1] = t.y;
+            fzl[4 * v + 2] = t.z;
+            fzl[4 * v + 3] = t.w;
+          }
+        }
+      }
+      float fzu[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      bool hzu = false;
+      if (cx && iz + 1 < g.nz)
+        hzu = iface<3, CM, NC>(eos, Qs, Q, gv, m, m_full, N, iz >= 1 ? c - sz : c, c, c + sz,
+                               iz + 2 < g.nz ? c + 2 * sz : c + sz, itau, itauf, itf, prz, fzu,
+                               status, spz);
+      if (hzu) {
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+          slot[(j * NV + v) * blockDim.x] =
+              make_float4(fzu[4 * v], fzu[4 * v + 1], fzu[4 * v + 2], fzu[4 * v + 3]);
+        hzm |= 1u << j;
+      } else {
+        hzm &= ~(1u << j);
+      }
+      if (cx && lane < 63) {
+        double acc[7];
+#pragma unroll
+        for (int k = 0; k < 7; k++) acc[k] = 0.;
+        if (hx)
+#pragma unroll
+          for (int k = 0; k < NC; k++) acc[k] += (double)fx[k];
+        if (hxu)
+#pragma unroll
+          for (int k = 0; k < NC; k++) acc[k] -= (double)fxu[k];
+        if (hyl)
+#pragma unroll
+          for (int k = 0; k < NC; k++) acc[k] += (double)fyl[k];
+        if (hyu)
+#pragma unroll
+          for (int k = 0; k < NC; k++) acc[k] -= (double)fyu[k];
+        if (hzl)
+#pragma unroll
+          for (int k = 0; k < NC; k++) acc[k] += (double)fzl[k];
+        if (hzu)
+#pragma unroll
+          for (int k = 0; k < NC; k++) acc[k] -= (double)fzu[k];
+        finish_cell<MODE, CM, NC>(eos, Q, Qs, gv, N, c, tau, dt, acc, out);
+      }
+#pragma unroll
+      for (int k = 0; k < NC; k++) fyc[k] = fyu[k];
+      hyc = hyu;
+    }
+  }
+  wave_speed_max(sig, (double)spx);
+  wave_speed_max(sig + kSigGroup, (double)spy);
+  wave_speed_max(sig + 2 * kSigGroup, (double)spz);
+}
+
 // ---------------------------------------------------------------- viscous
-template <int CM, int NC>
+template <int CM, int NC, int EK>
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_MX_VPRIM)
-k_vprim(long N, DevEos eos, const double *__restrict__ Qh, const float *__restrict__ gv,
+k_vprim(long N, DevEos eos_a, const double *__restrict__ Qh, const float *__restrict__ gv,
         const double *__restrict__ tp, double dt, double *__restrict__ ph,
         double *__restrict__ pt) {
+  const DevEos eos = eos_of_kind<EK>(eos_a);
   const double tau = *tp;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
@@ -782,15 +977,16 @@ __device__ __forceinline__ void is_cross(const float *X, const float *Pm, const 
 
 // With VORT, also the cell's vorticity entries and op code (vh_vort.h), formed
 // in FP64 from the same states as NSquant's velocity gradients.
-template <int CM, int NC, bool VORT>
+template <int CM, int NC, bool VORT, int EK>
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_MX_VISC1)
-k_visc1(Grids G, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
+k_visc1(Grids G, DevEos eos_a, DevTrc trc, const double *__restrict__ Qprev,
         const double *__restrict__ Qn, const float *__restrict__ gv,
         const double *__restrict__ ph,
         const double *__restrict__ pt, const double *__restrict__ m, int m_full,
         const float *__restrict__ pi, const float *__restrict__ Pi,
         const double *__restrict__ tp, double dt, float4 *__restrict__ rec,
         int *__restrict__ status, double *__restrict__ db, unsigned char *__restrict__ dop) {
+  const DevEos eos = eos_of_kind<EK>(eos_a);
   const double tau = *tp;
   Geom g;
   long c;
@@ -1355,28 +1551,45 @@ k_ghost_list(Geom g, long n, double *__restrict__ Q, double *__restrict__ Qh,
 // ---------------------------------------------------------- launchers
 static inline unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
 
+// the kernel instance of an EoS: the spline table, else the ideal gas (the
+// mixed path takes no other kind)
+static inline int eos_kind_class(const DevEos &e) {
+  return e.kind == VH_EOS_TABLE ? VH_EOS_TABLE : VH_EOS_IDEAL;
+}
+
 void launch_stage(int mode, int cm, int nc, const Grids &G, const DevEos &eos, const double *Q,
                   const double *Qh, const float *gv, const double *m, int m_full,
                   const double *tp, double dt, double *out, int *status,
                   unsigned long long *sig, hipStream_t st) {
   const dim3 gr((unsigned)G.nblk), b(256);
-#define VH_S(MM, CC)                                                                            \
-  if (mode == MM && cm == CC && nc == 7)                                                        \
-    hipLaunchKernelGGL((k_stage<MM, CC, 7>), gr, b, 0, st, G, eos, Q, Qh, gv, m, m_full, tp, dt, \
-                       out, status, sig);                                                       \
-  if (mode == MM && cm == CC && nc == 4)                                                        \
-    hipLaunchKernelGGL((k_stage<MM, CC, 4>), gr, b, 0, st, G, eos, Q, Qh, gv, m, m_full, tp, dt, \
-                       out, status, sig);
+  const int ek = eos_kind_class(eos);
+#define VH_SK(MM, CC, NN, EE)                                                                  \
+  if (mode == MM && cm == CC && nc == NN && ek == EE)                                          \
+    hipLaunchKernelGGL((k_stage<MM, CC, NN, EE>), gr, b, 0, st, G, eos, Q, Qh, gv, m, m_full,   \
+                       tp, dt, out, status, sig);
+#define VH_S(MM, CC)                                                                           \
+  VH_SK(MM, CC, 7, VH_EOS_IDEAL) VH_SK(MM, CC, 4, VH_EOS_IDEAL) VH_SK(MM, CC, 7, VH_EOS_TABLE) \
+  VH_SK(MM, CC, 4, VH_EOS_TABLE)
   VH_S(0, VH_C2P_REFERENCE) VH_S(1, VH_C2P_REFERENCE) VH_S(0, VH_C2P_CLOSED)
   VH_S(1, VH_C2P_CLOSED) VH_S(0, VH_C2P_LEARNED) VH_S(1, VH_C2P_LEARNED)
   VH_S(0, VH_C2P_NEWTON) VH_S(1, VH_C2P_NEWTON)
 #undef VH_S
+#undef VH_SK
 }
 
 Tile make_rows(const Grids &G, int CZ) {
   Tile t;
   t.kind = STAGE_ROWS;
   t.CZ = CZ < 1 ? 1 : (CZ > G.maxnz ? G.maxnz : CZ);
+  t.CY = 1;
+  return t;
+}
+
+Tile make_bands(const Grids &G, int CY, int CZ) {
+  Tile t;
+  t.kind = STAGE_BANDS;
+  t.CZ = CZ < 1 ? 1 : (CZ > G.maxnz ? G.maxnz : CZ);
+  t.CY = CY;
   return t;
 }
 
@@ -1384,7 +1597,14 @@ Tile make_cells() {
   Tile t;
   t.kind = STAGE_CELLS;
   t.CZ = 0;
+  t.CY = 0;
   return t;
+}
+
+// waves of a row or band launch
+static long tile_waves(const Grids &G, const Tile &t) {
+  const long nseg = t.kind == STAGE_BANDS ? G.nbseg[band_table(t.CY)] : G.nseg;
+  return nseg * ((G.maxnz + t.CZ - 1) / t.CZ);
 }
 
 std::vector<Tile> tile_candidates(const Grids &G) {
@@ -1395,12 +1615,22 @@ std::vector<Tile> tile_candidates(const Grids &G) {
   const int czs[] = {1, 2, 4, 8, 16, 32};
   for (int cz : czs) {
     const Tile t = make_rows(G, cz);
-    const long waves = (long)G.nseg * ((G.maxnz + t.CZ - 1) / t.CZ);
-    if (t.CZ > 1 && waves < 4096) continue;
+    if (t.CZ > 1 && tile_waves(G, t) < 4096) continue;
     bool dup = false;
     for (auto &o : out) dup |= o.kind == STAGE_ROWS && o.CZ == t.CZ;
     if (!dup) out.push_back(t);
   }
+  // bands: fewer, longer waves (each updates CY cells per eta cell), so at
+  // least 2048 of them
+  const int cys[] = {2, 4, 8};
+  for (int cy : cys)
+    for (int cz : czs) {
+      const Tile t = make_bands(G, cy, cz);
+      if ((t.CZ > 1 || cy > 2) && tile_waves(G, t) < 2048) continue;
+      bool dup = false;
+      for (auto &o : out) dup |= o.kind == STAGE_BANDS && o.CZ == t.CZ && o.CY == t.CY;
+      if (!dup) out.push_back(t);
+    }
   return out;
 }
 
@@ -1408,37 +1638,59 @@ void launch_stage_tile(int mode, int cm, int nc, const Grids &G, const DevEos &e
                        const double *Qh, const float *gv, const double *m, int m_full,
                        const double *tp, double dt, double *out, int *status,
                        unsigned long long *sig, const Tile &tl, hipStream_t st) {
-  if (tl.kind != STAGE_ROWS) {
+  if (tl.kind == STAGE_CELLS) {
     launch_stage(mode, cm, nc, G, eos, Q, Qh, gv, m, m_full, tp, dt, out, status, sig, st);
     return;
   }
-  const long ntz = (G.maxnz + tl.CZ - 1) / tl.CZ;
-  const long waves = (long)G.nseg * ntz;
+  const long waves = tile_waves(G, tl);
   const dim3 gr((unsigned)((waves + 3) / 4)), b(256);
-#define VH_S(MM, CC)                                                                          \
-  if (mode == MM && cm == CC && nc == 7)                                                      \
-    hipLaunchKernelGGL((k_stage_rows<MM, CC, 7>), gr, b, 0, st, G, eos, Q, Qh, gv, m, m_full, \
-                       tp, dt, out, status, sig, tl.CZ);                                      \
-  if (mode == MM && cm == CC && nc == 4)                                                      \
-    hipLaunchKernelGGL((k_stage_rows<MM, CC, 4>), gr, b, 0, st, G, eos, Q, Qh, gv, m, m_full, \
-                       tp, dt, out, status, sig, tl.CZ);
+  const int ek = eos_kind_class(eos);
+  if (tl.kind == STAGE_BANDS) {
+    const int h = band_table(tl.CY);
+    const int *bsev = G.bsev[h], *bsfirst = G.bsfirst[h];
+    const long nseg = G.nbseg[h];
+    const size_t lds = (size_t)tl.CY * (nc > 4 ? 2 : 1) * 256 * sizeof(float4);
+#define VH_BK(MM, CC, NN, EE)                                                                  \
+  if (mode == MM && cm == CC && nc == NN && ek == EE)                                          \
+    hipLaunchKernelGGL((k_stage_bands<MM, CC, NN, EE>), gr, b, lds, st, G, bsev, bsfirst, nseg, \
+                       eos, Q, Qh, gv, m, m_full, tp, dt, out, status, sig, tl.CY, tl.CZ);
+#define VH_B(MM, CC)                                                                           \
+  VH_BK(MM, CC, 7, VH_EOS_IDEAL) VH_BK(MM, CC, 4, VH_EOS_IDEAL) VH_BK(MM, CC, 7, VH_EOS_TABLE) \
+  VH_BK(MM, CC, 4, VH_EOS_TABLE)
+    VH_B(0, VH_C2P_REFERENCE) VH_B(1, VH_C2P_REFERENCE) VH_B(0, VH_C2P_CLOSED)
+    VH_B(1, VH_C2P_CLOSED) VH_B(0, VH_C2P_LEARNED) VH_B(1, VH_C2P_LEARNED)
+    VH_B(0, VH_C2P_NEWTON) VH_B(1, VH_C2P_NEWTON)
+#undef VH_B
+#undef VH_BK
+    return;
+  }
+#define VH_SK(MM, CC, NN, EE)                                                                  \
+  if (mode == MM && cm == CC && nc == NN && ek == EE)                                          \
+    hipLaunchKernelGGL((k_stage_rows<MM, CC, NN, EE>), gr, b, 0, st, G, eos, Q, Qh, gv, m,      \
+                       m_full, tp, dt, out, status, sig, tl.CZ);
+#define VH_S(MM, CC)                                                                           \
+  VH_SK(MM, CC, 7, VH_EOS_IDEAL) VH_SK(MM, CC, 4, VH_EOS_IDEAL) VH_SK(MM, CC, 7, VH_EOS_TABLE) \
+  VH_SK(MM, CC, 4, VH_EOS_TABLE)
   VH_S(0, VH_C2P_REFERENCE) VH_S(1, VH_C2P_REFERENCE) VH_S(0, VH_C2P_CLOSED)
   VH_S(1, VH_C2P_CLOSED) VH_S(0, VH_C2P_LEARNED) VH_S(1, VH_C2P_LEARNED)
   VH_S(0, VH_C2P_NEWTON) VH_S(1, VH_C2P_NEWTON)
 #undef VH_S
+#undef VH_SK
 }
 
 void launch_vprim(int cm, int nc, long N, const DevEos &eos, const double *Qh, const float *gv,
                   const double *tp, double dt, double *ph, double *pt, hipStream_t st) {
   const dim3 gr(blocks(N)), b(256);
-#define VH_V(CC)                                                                              \
-  if (cm == CC && nc == 7)                                                                    \
-    hipLaunchKernelGGL((k_vprim<CC, 7>), gr, b, 0, st, N, eos, Qh, gv, tp, dt, ph, pt);        \
-  if (cm == CC && nc == 4)                                                                    \
-    hipLaunchKernelGGL((k_vprim<CC, 4>), gr, b, 0, st, N, eos, Qh, gv, tp, dt, ph, pt);
+  const int ek = eos_kind_class(eos);
+#define VH_VK(CC, NN, EE)                                                                      \
+  if (cm == CC && nc == NN && ek == EE)                                                        \
+    hipLaunchKernelGGL((k_vprim<CC, NN, EE>), gr, b, 0, st, N, eos, Qh, gv, tp, dt, ph, pt);
+#define VH_V(CC)                                                                               \
+  VH_VK(CC, 7, VH_EOS_IDEAL) VH_VK(CC, 4, VH_EOS_IDEAL) VH_VK(CC, 7, VH_EOS_TABLE)            \
+  VH_VK(CC, 4, VH_EOS_TABLE)
   VH_V(VH_C2P_REFERENCE) VH_V(VH_C2P_CLOSED) VH_V(VH_C2P_LEARNED) VH_V(VH_C2P_NEWTON)
 #undef VH_V
-#undef VH_VV
+#undef VH_VK
 }
 
 void launch_visc1(int cm, int nc, const Grids &G, const DevEos &eos, const DevTrc &trc,
@@ -1447,13 +1699,18 @@ void launch_visc1(int cm, int nc, const Grids &G, const DevEos &eos, const DevTr
                   const float *Pi, const double *tp, double dt, float4 *rec, int *status,
                   double *db, unsigned char *dop, hipStream_t st) {
   const dim3 gr((unsigned)G.nblk), b(256);
-#define VH_VV(CC, NN, VV)                                                                      \
-  if (cm == CC && nc == NN && (db != nullptr) == VV)                                           \
-    hipLaunchKernelGGL((k_visc1<CC, NN, VV>), gr, b, 0, st, G, eos, trc, Qprev, Qn, gv, ph, pt,  \
-                       m, m_full, pi, Pi, tp, dt, rec, status, db, dop);
-#define VH_V(CC) VH_VV(CC, 7, false) VH_VV(CC, 4, false) VH_VV(CC, 7, true) VH_VV(CC, 4, true)
+  const int ek = eos_kind_class(eos);
+#define VH_VV(CC, NN, VV, EE)                                                                  \
+  if (cm == CC && nc == NN && (db != nullptr) == VV && ek == EE)                               \
+    hipLaunchKernelGGL((k_visc1<CC, NN, VV, EE>), gr, b, 0, st, G, eos, trc, Qprev, Qn, gv, ph, \
+                       pt, m, m_full, pi, Pi, tp, dt, rec, status, db, dop);
+#define VH_VE(CC, EE)                                                                          \
+  VH_VV(CC, 7, false, EE) VH_VV(CC, 4, false, EE) VH_VV(CC, 7, true, EE) VH_VV(CC, 4, true, EE)
+#define VH_V(CC) VH_VE(CC, VH_EOS_IDEAL) VH_VE(CC, VH_EOS_TABLE)
   VH_V(VH_C2P_REFERENCE) VH_V(VH_C2P_CLOSED) VH_V(VH_C2P_LEARNED) VH_V(VH_C2P_NEWTON)
 #undef VH_V
+#undef VH_VE
+#undef VH_VV
 }
 
 void launch_visc2(const Grids &G, const double *ph, const float4 *rec, float *pi, float *piH,
