@@ -388,6 +388,22 @@ struct Prim {
   double e, p, nb, nq, ns, vx, vy, vz;
 };
 
+// The EoS of a kernel compiled for one EoS kind EK (VH_EOS_IDEAL or
+// VH_EOS_TABLE; EK < 0: the kind given at run time, for the charge-dependent
+// EoSs).  The kind is fixed at kernel entry, so that an ideal-gas kernel
+// carries none of the spline code inlined at every EoS call (the mixed
+// corrector's row kernel shrinks from 77 to 29 KB of code).
+template <int EK>
+__device__ __forceinline__ DevEos eos_of_kind(const DevEos &a) {
+  DevEos r = a;
+  if (EK >= 0) r.kind = EK;
+  return r;
+}
+// the kernel instance of an EoS (host side): EK of eos_of_kind
+inline int eos_kind_class(const DevEos &e) {
+  return e.charged() ? -1 : (e.kind == VH_EOS_TABLE ? VH_EOS_TABLE : VH_EOS_IDEAL);
+}
+
 // transformPV (src/rmn.cpp:28-139), FP64, reference operation order.  The
 // input q is rescaled in place when |q_vec| > q_t (src/rmn.cpp:65-70).
 // Returns the number of loop iterations executed.  Written on scalar locals
@@ -766,10 +782,42 @@ __device__ __forceinline__ void wave_speed_max(unsigned long long *grp, double v
 #ifndef VH_W_GUESS
 #define VH_W_GUESS 0
 #endif
-// rows per y-strip of the mixed per-cell kernels' cell order (0: x-fastest)
+// rows per y-strip of the per-cell kernels' cell order (0: x-fastest)
 #ifndef VH_STRIP_ROWS
 #define VH_STRIP_ROWS 8
 #endif
+
+// Cell (ix, iy, iz) of work item u (0 <= u < nx ny nz) in y-strip order: the
+// grid's rows are taken in strips of sy rows (the last strip shorter), each
+// strip swept over all eta in contiguous chunks of sy*nx cells.  With
+// XCD-contiguous workgroups, the eta- and y-neighbours a workgroup gathers
+// were fetched into the same L2 by the workgroups just before it (in
+// x-fastest order an eta-neighbour is a whole plane away).  A bijection of
+// the cells, so results do not depend on it.  32-bit arithmetic: a context
+// holds fewer than 2^29 cells (checked at creation).
+__device__ __forceinline__ void strip_cell(int nx_, int ny_, int nz_, int sy_, long u_, int &ix,
+                                           int &iy, int &iz) {
+  const unsigned nx = nx_, ny = ny_, u = (unsigned)u_;
+  if (sy_ <= 0 || ny_ <= sy_) {  // x-fastest
+    const unsigned r = u / nx, q = r / ny;
+    ix = (int)(u - r * nx);
+    iy = (int)(r - q * ny);
+    iz = (int)q;
+    return;
+  }
+  const unsigned sy = sy_, nz = nz_;
+  const unsigned nfull = ny / sy, plane = sy * nx, strip = plane * nz;
+  unsigned s = u / strip, r = u - s * strip, pl = plane;
+  if (s >= nfull) {  // the last, shorter strip
+    s = nfull;
+    r = u - nfull * strip;
+    pl = (ny - nfull * sy) * nx;
+  }
+  const unsigned z = r / pl, f = r - z * pl, yy = f / nx;
+  iz = (int)z;
+  iy = (int)(s * sy + yy);
+  ix = (int)(f - yy * nx);
+}
 
 // Grid geometry.  N: cells of this grid; base: index of its cell 0 in the
 // state arrays; P: the arrays' component pitch (cells of all grids stored

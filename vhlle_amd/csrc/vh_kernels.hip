@@ -172,12 +172,10 @@ template <int D, int MODE, bool CH>
 __device__ __forceinline__ double face_flux(const Geom &g, const DevEos &eos,
                                             const double *__restrict__ Qs,
                                             const double *__restrict__ m, int m_full,
-                                            double tau, double dt, long c,
-                                            double *__restrict__ F,
+                                            double tau, double dt, long c, int ix, int iy,
+                                            int iz, double *__restrict__ F,
                                             unsigned char *__restrict__ flag,
                                             int *__restrict__ status) {
-  int ix, iy, iz;
-  decompose(g, c, ix, iy, iz);
   const int i = coord<D>(ix, iy, iz), n = extent<D>(g);
   if (i >= n - 1) return 0.;
   const long s = stride<D>(g);
@@ -218,16 +216,22 @@ __device__ __forceinline__ double face_flux(const Geom &g, const DevEos &eos,
 
 // One thread per interface (slot = left cell); the wave's largest signal
 // speed goes to the slot group sig of this stage and axis.
-template <int D, int MODE, bool CH>
+template <int D, int MODE, bool CH, int EK>
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_FACES)
-k_faces(Geom g, DevEos eos, const double *__restrict__ Qs, const double *__restrict__ m,
+k_faces(Geom g, DevEos eos_a, const double *__restrict__ Qs, const double *__restrict__ m,
         int m_full, const double *__restrict__ tp, double dt, double *__restrict__ F,
         unsigned char *__restrict__ flag, int *__restrict__ status,
         unsigned long long *__restrict__ sig) {
+  const DevEos eos = eos_of_kind<EK>(eos_a);
   const double tau = *tp;
-  const long c = xcd_block() * blockDim.x + threadIdx.x;
+  const long t = xcd_block() * blockDim.x + threadIdx.x;  // interfaces in y-strip order
   double speed = 0.;
-  if (c < g.N) speed = face_flux<D, MODE, CH>(g, eos, Qs, m, m_full, tau, dt, c, F, flag, status);
+  if (t < g.N) {
+    int ix, iy, iz;
+    strip_cell(g.nx, g.ny, g.nz, VH_STRIP_ROWS, t, ix, iy, iz);
+    speed = face_flux<D, MODE, CH>(g, eos, Qs, m, m_full, tau, dt, g.idx(ix, iy, iz), ix, iy, iz,
+                                   F, flag, status);
+  }
   wave_speed_max(sig, speed);
 }
 
@@ -269,19 +273,21 @@ k_faces_fix(Geom g, DevEos eos, const double *__restrict__ Qr, const double *__r
 // applies the stage update: predictor Qh = Q + flux (Cell::updateQtoQhByFlux,
 // src/cll.cpp:86-88); corrector Q += flux unless Q0 + flux0 < 0
 // (Cell::updateByFlux, src/cll.cpp:70-74).
-template <int MODE, bool CH>
+template <int MODE, bool CH, int EK>
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_UPDATE)
-k_update(Geom g, DevEos eos, const double *__restrict__ Q, const double *__restrict__ Qh,
+k_update(Geom g, DevEos eos_a, const double *__restrict__ Q, const double *__restrict__ Qh,
          const double *__restrict__ Fx, const unsigned char *__restrict__ fx,
          const double *__restrict__ Fy, const unsigned char *__restrict__ fy,
          const double *__restrict__ Fz, const unsigned char *__restrict__ fz,
          const double *__restrict__ tp, double dt, double *__restrict__ out,
          int *__restrict__ status) {
+  const DevEos eos = eos_of_kind<EK>(eos_a);
   const double tau = *tp;
-  const long c = xcd_block() * blockDim.x + threadIdx.x;
-  if (c >= g.N) return;
+  const long t = xcd_block() * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
   int ix, iy, iz;
-  decompose(g, c, ix, iy, iz);
+  strip_cell(g.nx, g.ny, g.nz, VH_STRIP_ROWS, t, ix, iy, iz);
+  const long c = g.idx(ix, iy, iz);
   const long N = g.N;
   double acc[7];
 #pragma unroll
@@ -565,10 +571,11 @@ k_update_m_apply(Geom g, double *__restrict__ m, const double *__restrict__ dm) 
 // tau here is tau_{n+1}.  Tt (may be NULL): T of the pt state where e > 0,
 // and everywhere for an EoS that depends on the charges (whose T at e <= 0
 // needs the cell's n) -- the neighbour temperatures of the vorticity rows.
-template <bool CH>
+template <bool CH, int EK>
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_VPRIM)
-k_vprim(Geom g, DevEos eos, const double *__restrict__ Qh, const double *__restrict__ tp,
+k_vprim(Geom g, DevEos eos_a, const double *__restrict__ Qh, const double *__restrict__ tp,
         double dt, double *__restrict__ ph, double *__restrict__ pt, double *__restrict__ Tt) {
+  const DevEos eos = eos_of_kind<EK>(eos_a);
   const double tau = *tp;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.N) return;
@@ -603,20 +610,22 @@ k_vprim(Geom g, DevEos eos, const double *__restrict__ Qh, const double *__restr
 // pi0, piH0, Pi0, PiH0.  (The loop's flux contribution (tau-dt)*dQ_n is
 // formed in k_visc2, which reads the same old pi/Pi before replacing them.)
 // With VORT, also the cell's vorticity entries and op code (vh_vort.h).
-template <bool VORT, bool CH>
+template <bool VORT, bool CH, int EK>
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_VISC1)
-k_visc1(Geom g, DevEos eos, DevTrc trc, const double *__restrict__ Qprev,
+k_visc1(Geom g, DevEos eos_a, DevTrc trc, const double *__restrict__ Qprev,
         const double *__restrict__ Qn, const double *__restrict__ ph,
         const double *__restrict__ pt, const double *__restrict__ m, int m_full,
         const double *__restrict__ pi, const double *__restrict__ Pi,
         const double *__restrict__ tp, double dt, double *__restrict__ pi0, double *__restrict__ piH0, double *__restrict__ Pi0,
         double *__restrict__ PiH0, int *__restrict__ status, double *__restrict__ db,
         unsigned char *__restrict__ dop, const double *__restrict__ Tt) {
+  const DevEos eos = eos_of_kind<EK>(eos_a);
   const double tau = *tp;
-  const long c = xcd_block() * blockDim.x + threadIdx.x;
-  if (c >= g.N) return;
+  const long t = xcd_block() * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
   int ix, iy, iz;
-  decompose(g, c, ix, iy, iz);
+  strip_cell(g.nx, g.ny, g.nz, VH_STRIP_ROWS, t, ix, iy, iz);
+  const long c = g.idx(ix, iy, iz);
   const long N = g.N;
   const double e = ph[c], nb = ph[2 * N + c], nq = ph[3 * N + c], ns = ph[4 * N + c];
   const double vx = ph[5 * N + c], vy = ph[6 * N + c], vz = ph[7 * N + c];
@@ -931,10 +940,11 @@ k_visc2(Geom g, const double *__restrict__ ph, const double *__restrict__ pi0,
         double *__restrict__ Pi, double *__restrict__ PiH, double *__restrict__ vcc,
         double *__restrict__ vf, const double *__restrict__ tp, double dt) {
   const double tau = *tp;
-  const long c = xcd_block() * blockDim.x + threadIdx.x;
-  if (c >= g.N) return;
+  const long t = xcd_block() * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
   int ix, iy, iz;
-  decompose(g, c, ix, iy, iz);
+  strip_cell(g.nx, g.ny, g.nz, VH_STRIP_ROWS, t, ix, iy, iz);
+  const long c = g.idx(ix, iy, iz);
   const long N = g.N;
   const double e = ph[c], p = ph[N + c];
   const double vx = ph[5 * N + c], vy = ph[6 * N + c], vz = ph[7 * N + c];
@@ -1072,10 +1082,11 @@ k_visc3(Geom g, const double *__restrict__ ph, const double *__restrict__ piH,
         const double *__restrict__ m, int m_full, const double *__restrict__ tp, double dt,
         double *__restrict__ Q) {
   const double tau = *tp;
-  const long c = xcd_block() * blockDim.x + threadIdx.x;
-  if (c >= g.N) return;
+  const long t = xcd_block() * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
   int ix, iy, iz;
-  decompose(g, c, ix, iy, iz);
+  strip_cell(g.nx, g.ny, g.nz, VH_STRIP_ROWS, t, ix, iy, iz);
+  const long c = g.idx(ix, iy, iz);
   const long N = g.N;
   double acc[4];
 #pragma unroll
@@ -1236,29 +1247,33 @@ void launch_faces(int D, int mode, const Geom &g, const DevEos &eos, const doubl
   // k_faces_fix re-evaluates interfaces with a negative face pressure: only
   // the spline table and the hadron table (below e_min) can produce one
   const bool fix = eos.kind == VH_EOS_TABLE || eos.kind == VH_EOS_HADRON;
-#define VH_F(DD, MM, CC)                                                                      \
-  if (D == DD && mode == MM && eos.charged() == CC) {                                         \
-    hipLaunchKernelGGL((k_faces<DD, MM, CC>), gr, b, 0, st, g, eos, Qs, m, m_full, tp, dt, F, \
-                       flag, status, sig);                                                    \
+  const int ek = eos_kind_class(eos);
+#define VH_FK(DD, MM, CC, EE)                                                                 \
+  if (D == DD && mode == MM && eos.charged() == CC && ek == EE) {                             \
+    hipLaunchKernelGGL((k_faces<DD, MM, CC, EE>), gr, b, 0, st, g, eos, Qs, m, m_full, tp,     \
+                       dt, F, flag, status, sig);                                             \
     if (fix)                                                                                  \
       hipLaunchKernelGGL((k_faces_fix<DD, MM, CC>), gr, b, 0, st, g, eos, Qr, m, m_full, tp,  \
                          dt, F, flag, status, sig);                                           \
   }
-  VH_F(1, 0, false) VH_F(1, 1, false) VH_F(2, 0, false) VH_F(2, 1, false) VH_F(3, 0, false)
-  VH_F(3, 1, false) VH_F(1, 0, true) VH_F(1, 1, true) VH_F(2, 0, true) VH_F(2, 1, true)
-  VH_F(3, 0, true) VH_F(3, 1, true)
+#define VH_F(DD, MM)                                                                          \
+  VH_FK(DD, MM, false, VH_EOS_IDEAL) VH_FK(DD, MM, false, VH_EOS_TABLE) VH_FK(DD, MM, true, -1)
+  VH_F(1, 0) VH_F(1, 1) VH_F(2, 0) VH_F(2, 1) VH_F(3, 0) VH_F(3, 1)
 #undef VH_F
+#undef VH_FK
 }
 
 void launch_update(int mode, const Geom &g, const DevEos &eos, const double *Q,
                    const double *Qh, const double *const F[3],
                    const unsigned char *const fl[3], const double *tp, double dt, double *out,
                    int *status, hipStream_t st) {
-#define VH_U(MM, CC)                                                                          \
-  if (mode == MM && eos.charged() == CC)                                                      \
-    hipLaunchKernelGGL((k_update<MM, CC>), dim3(blocks(g.N)), dim3(256), 0, st, g, eos, Q, Qh, \
-                       F[0], fl[0], F[1], fl[1], F[2], fl[2], tp, dt, out, status);
-  VH_U(0, false) VH_U(1, false) VH_U(0, true) VH_U(1, true)
+  const int ek = eos_kind_class(eos);
+#define VH_U(MM, CC, EE)                                                                      \
+  if (mode == MM && eos.charged() == CC && ek == EE)                                          \
+    hipLaunchKernelGGL((k_update<MM, CC, EE>), dim3(blocks(g.N)), dim3(256), 0, st, g, eos, Q, \
+                       Qh, F[0], fl[0], F[1], fl[1], F[2], fl[2], tp, dt, out, status);
+  VH_U(0, false, VH_EOS_IDEAL) VH_U(1, false, VH_EOS_IDEAL) VH_U(0, false, VH_EOS_TABLE)
+  VH_U(1, false, VH_EOS_TABLE) VH_U(0, true, -1) VH_U(1, true, -1)
 #undef VH_U
 }
 
@@ -1283,12 +1298,16 @@ void launch_update_m(int pass, const Geom &g, double *m, double *dm, const doubl
 
 void launch_vprim(const Geom &g, const DevEos &eos, const double *Qh, const double *tp,
                   double dt, double *ph, double *pt, double *Tt, hipStream_t st) {
+  const int ek = eos_kind_class(eos);
   if (eos.charged())
-    hipLaunchKernelGGL(k_vprim<true>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, Qh, tp, dt, ph,
-                       pt, Tt);
+    hipLaunchKernelGGL((k_vprim<true, -1>), dim3(blocks(g.N)), dim3(256), 0, st, g, eos, Qh, tp,
+                       dt, ph, pt, Tt);
+  else if (ek == VH_EOS_TABLE)
+    hipLaunchKernelGGL((k_vprim<false, VH_EOS_TABLE>), dim3(blocks(g.N)), dim3(256), 0, st, g, eos,
+                       Qh, tp, dt, ph, pt, Tt);
   else
-    hipLaunchKernelGGL(k_vprim<false>, dim3(blocks(g.N)), dim3(256), 0, st, g, eos, Qh, tp, dt, ph,
-                       pt, Tt);
+    hipLaunchKernelGGL((k_vprim<false, VH_EOS_IDEAL>), dim3(blocks(g.N)), dim3(256), 0, st, g, eos,
+                       Qh, tp, dt, ph, pt, Tt);
 }
 
 void launch_visc1(const Geom &g, const DevEos &eos, const DevTrc &trc, const double *Qprev,
@@ -1296,12 +1315,14 @@ void launch_visc1(const Geom &g, const DevEos &eos, const DevTrc &trc, const dou
                   int m_full, const double *pi, const double *Pi, const double *tp, double dt,
                   double *pi0, double *piH0, double *Pi0, double *PiH0, int *status,
                   double *db, unsigned char *dop, const double *Tt, hipStream_t st) {
-#define VH_V(VV, CC)                                                                          \
-  if (!!db == VV && eos.charged() == CC)                                                      \
-    hipLaunchKernelGGL((k_visc1<VV, CC>), dim3(blocks(g.N)), dim3(256), 0, st, g, eos, trc,     \
+  const int ek = eos_kind_class(eos);
+#define VH_V(VV, CC, EE)                                                                      \
+  if (!!db == VV && eos.charged() == CC && ek == EE)                                          \
+    hipLaunchKernelGGL((k_visc1<VV, CC, EE>), dim3(blocks(g.N)), dim3(256), 0, st, g, eos, trc, \
                        Qprev, Qn, ph, pt, m, m_full, pi, Pi, tp, dt, pi0, piH0, Pi0, PiH0,     \
                        status, VV ? db : nullptr, VV ? dop : nullptr, VV ? Tt : nullptr);
-  VH_V(true, false) VH_V(false, false) VH_V(true, true) VH_V(false, true)
+  VH_V(true, false, VH_EOS_IDEAL) VH_V(false, false, VH_EOS_IDEAL) VH_V(true, false, VH_EOS_TABLE)
+  VH_V(false, false, VH_EOS_TABLE) VH_V(true, true, -1) VH_V(false, true, -1)
 #undef VH_V
 }
 

@@ -61,25 +61,8 @@ __device__ __forceinline__ bool work_cell(const Grids &G, Geom &g, long &c, int 
   long cl;
   g = block_grid(G, G.sy > 0 ? xcd_block() : (long)blockIdx.x, cl);
   if (cl >= g.N) return false;
-  if (G.sy > 0 && g.ny > G.sy) {
-    const unsigned nx = g.nx, ny = g.ny, nz = g.nz, sy = G.sy;
-    const unsigned nfull = ny / sy, plane = sy * nx, strip = plane * nz;
-    const unsigned u = (unsigned)cl;
-    unsigned s = u / strip, r = u - s * strip, pl = plane;
-    if (s >= nfull) {  // the last, shorter strip
-      s = nfull;
-      r = u - nfull * strip;
-      pl = (ny - nfull * sy) * nx;
-    }
-    const unsigned z = r / pl, f = r - z * pl, yy = f / nx;
-    iz = (int)z;
-    iy = (int)(s * sy + yy);
-    ix = (int)(f - yy * nx);
-    c = g.idx(ix, iy, iz);
-  } else {
-    coords_of(g, cl, ix, iy, iz);
-    c = g.base + cl;
-  }
+  strip_cell(g.nx, g.ny, g.nz, G.sy, cl, ix, iy, iz);
+  c = g.idx(ix, iy, iz);
   return true;
 }
 
@@ -338,15 +321,7 @@ __device__ __forceinline__ float guess_of(const float *__restrict__ gv, long c) 
 
 // The kernels that evaluate the EoS are compiled per EoS kind EK (VH_EOS_IDEAL
 // or VH_EOS_TABLE, the kinds the mixed path takes; kinds 2 and 3 are refused
-// when the precision is set).  The kind is fixed at kernel entry, so an
-// ideal-gas kernel carries none of the spline code inlined at every EoS call
-// (the corrector's row kernel shrinks from 77 to 29 KB of code).
-template <int EK>
-__device__ __forceinline__ DevEos eos_of_kind(const DevEos &a) {
-  DevEos r = a;
-  r.kind = EK;
-  return r;
-}
+// when the precision is set), see eos_of_kind.
 
 // MUSCL face state (src/cll.cpp:106-204) in FP64, con2prim by mode, as the
 // FP32 state the flux takes.  NC: conserved components carried (7, or 4 for
@@ -1551,11 +1526,6 @@ k_ghost_list(Geom g, long n, double *__restrict__ Q, double *__restrict__ Qh,
 // ---------------------------------------------------------- launchers
 static inline unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
 
-// the kernel instance of an EoS: the spline table, else the ideal gas (the
-// mixed path takes no other kind)
-static inline int eos_kind_class(const DevEos &e) {
-  return e.kind == VH_EOS_TABLE ? VH_EOS_TABLE : VH_EOS_IDEAL;
-}
 
 void launch_stage(int mode, int cm, int nc, const Grids &G, const DevEos &eos, const double *Q,
                   const double *Qh, const float *gv, const double *m, int m_full,
