@@ -828,7 +828,7 @@ k_visc1(Geom g, DevEos eos_a, DevTrc trc, const double *__restrict__ Qprev,
   double dPP = 2. / 3., lPp = 8. / 5. * (1. / 3. - eos.cs2e(e));
   if (tauPi < 0.5 * dt) dPP = lPp = 0.0;
 
-  double P[10], D[10], H[10], F[10];
+  double P[10], H[10], F[10];
   const double Pc = Pi[c];
 #pragma unroll
   for (int k = 0; k < 10; k++) P[k] = pi[k * N + c];
@@ -837,8 +837,6 @@ k_visc1(Geom g, DevEos eos_a, DevTrc trc, const double *__restrict__ Qprev,
 #pragma unroll
     for (int j = 0; j <= i; j++) {
       const int ij = sym(i, j);
-      D[ij] = -u[i] * u[j];
-      if (i == j) D[ij] += gm[i];
       H[ij] = taupi > 0.5 * dt ? P[ij] - dtp(dg((P[ij] - piNS[ij]) * dt / 2.0)) : piNS[ij];
     }
   double PH = tauPi > 0.5 * dt ? Pc - dg((Pc - PiNS) * dt / 2.0) / tauPi : PiNS;
@@ -855,6 +853,8 @@ k_visc1(Geom g, DevEos eos_a, DevTrc trc, const double *__restrict__ Qprev,
 #pragma unroll
     for (int j = 0; j <= i; j++) {
       const int ij = sym(i, j);
+      double Dij = -u[i] * u[j];  // the projector g^{ij} - u^i u^j, formed where used
+      if (i == j) Dij += gm[i];
       double t = H[ij];
       t += dg(-dpp * P[ij] * du + lpP * Pc * sig[ij]) * 0.5 * dt;
 #pragma unroll
@@ -865,10 +865,11 @@ k_visc1(Geom g, DevEos eos_a, DevTrc trc, const double *__restrict__ Qprev,
 #pragma unroll
         for (int l = 0; l < 4; l++)
           t += dg((-P[sym(i, k)] * u[j] - P[sym(j, k)] * u[i]) * u[l] * dmu[l][k] * gm[k]) * 0.5 * dt -
-               dg(1. / 3. * D[ij] * P[sym(k, l)] * (phi7 * P[sym(k, l)] - tpp * sig[sym(k, l)]) * gm[k] *
+               dg(1. / 3. * Dij * P[sym(k, l)] * (phi7 * P[sym(k, l)] - tpp * sig[sym(k, l)]) * gm[k] *
                   gm[l]) * 0.5 * dt;
       }
       H[ij] = t;
+      piH0[ij * N + c] = t;
     }
 #pragma unroll
   for (int k = 0; k < 4; k++)
@@ -898,6 +899,8 @@ k_visc1(Geom g, DevEos eos_a, DevTrc trc, const double *__restrict__ Qprev,
 #pragma unroll
     for (int j = 0; j <= i; j++) {
       const int ij = sym(i, j);
+      double Dij = -u[i] * u[j];
+      if (i == j) Dij += gm[i];
       double t = F[ij];
       t += dg(-dpp * H[ij] * du + lpP * PH * sig[ij]) * dt;
 #pragma unroll
@@ -908,11 +911,11 @@ k_visc1(Geom g, DevEos eos_a, DevTrc trc, const double *__restrict__ Qprev,
 #pragma unroll
         for (int l = 0; l < 4; l++)
           t += dg((-H[sym(i, k)] * u[j] - H[sym(j, k)] * u[i]) * u[l] * dmu[l][k] * gm[k] -
-                  1. / 3. * D[ij] * H[sym(k, l)] * (phi7 * H[sym(k, l)] - tpp * sig[sym(k, l)]) * gm[k] *
+                  1. / 3. * Dij * H[sym(k, l)] * (phi7 * H[sym(k, l)] - tpp * sig[sym(k, l)]) * gm[k] *
                       gm[l]) *
                dt;
       }
-      F[ij] = t;
+      pi0[ij * N + c] = t;  // stored at once: F is not read again
     }
 #pragma unroll
   for (int k = 0; k < 4; k++)
@@ -920,11 +923,6 @@ k_visc1(Geom g, DevEos eos_a, DevTrc trc, const double *__restrict__ Qprev,
     for (int l = 0; l < 4; l++)
       PF += dg(lPp * H[sym(k, l)] * sig[sym(k, l)] * gm[k] * gm[l]) * dt;
   PF += dg(-dPP * PH * du) * dt;
-#pragma unroll
-  for (int k = 0; k < 10; k++) {
-    piH0[k * N + c] = H[k];
-    pi0[k * N + c] = F[k];
-  }
   PiH0[c] = PH;
   Pi0[c] = PF;
 }

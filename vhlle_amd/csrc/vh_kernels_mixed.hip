@@ -883,27 +883,27 @@ k_vprim(long N, DevEos eos_a, const double *__restrict__ Qh, const float *__rest
   const double ith = 1. / (tau - 0.5 * dt), it = 1. / tau;
   double qs[NC];  // Qh is read once for both time levels
 #pragma unroll
-  for (int k = 0; k < NC; k++) qs[k] = Qh[k * N + c];
+  for (int k = 0; k < NC; k++) qs[k] = ldu(Qh + k * N, c);
 #pragma unroll
   for (int k = 0; k < 7; k++) q[k] = k < NC ? qs[k] * ith : 0.;
   c2p_mode<CM>(eos, q, r, guess_of<CM>(gv, c));
-  ph[c] = r.e;
-  ph[N + c] = r.p;
+  stu(ph, c, r.e);
+  stu(ph + N, c, r.p);
   if (NC == 7) {  // without charges n_b, n_q, n_s are zero and not stored
-    ph[2 * N + c] = r.nb;
-    ph[3 * N + c] = r.nq;
-    ph[4 * N + c] = r.ns;
+    stu(ph + 2 * N, c, r.nb);
+    stu(ph + 3 * N, c, r.nq);
+    stu(ph + 4 * N, c, r.ns);
   }
-  ph[5 * N + c] = r.vx;
-  ph[6 * N + c] = r.vy;
-  ph[7 * N + c] = r.vz;
+  stu(ph + 5 * N, c, r.vx);
+  stu(ph + 6 * N, c, r.vy);
+  stu(ph + 7 * N, c, r.vz);
 #pragma unroll
   for (int k = 0; k < 7; k++) q[k] = k < NC ? qs[k] * it : 0.;
   c2p_mode<CM>(eos, q, r, guess_of<CM>(gv, c));
-  pt[c] = r.e;
-  pt[N + c] = r.vx;
-  pt[2 * N + c] = r.vy;
-  pt[3 * N + c] = r.vz;
+  stu(pt, c, r.e);
+  stu(pt + N, c, r.vx);
+  stu(pt + 2 * N, c, r.vy);
+  stu(pt + 3 * N, c, r.vz);
 }
 
 // one Israel-Stewart update of the packed tensor (FP32), in the algebraically
@@ -968,12 +968,12 @@ k_visc1(Grids G, DevEos eos_a, DevTrc trc, const double *__restrict__ Qprev,
   int ix, iy, iz;
   if (!work_cell(G, g, c, ix, iy, iz)) return;
   const long N = g.P;
-  const double e = ph[c], nb = NC == 7 ? ph[2 * N + c] : 0.;
-  const double vx = ph[5 * N + c], vy = ph[6 * N + c], vz = ph[7 * N + c];
+  const double e = ldu(ph, c), nb = NC == 7 ? ldu(ph + 2 * N, c) : 0.;
+  const double vx = ldu(ph + 5 * N, c), vy = ldu(ph + 6 * N, c), vz = ldu(ph + 7 * N, c);
   if (e <= 0.) {
     const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-    for (int q = 0; q < 6; q++) rec[q * N + c] = z;
+    for (int q = 0; q < 6; q++) stu(rec + q * N, c, z);
     if (VORT) vort::store(db, dop, N, c, vort::K_EMPTY, nullptr);
     return;
   }
@@ -989,7 +989,7 @@ k_visc1(Grids G, DevEos eos_a, DevTrc trc, const double *__restrict__ Qprev,
   if (!m_full) {
     auto maxM = [&](int a, int b) {
       const long cc = g.cidx(ix + a, iy + b, iz);
-      return smax(m[cc], smax(m[N + cc], m[2 * N + cc]));
+      return smax(ldu(m, cc), smax(ldu(m + N, cc), ldu(m + 2 * N, cc)));
     };
     vac = maxM(1, 0) <= 0.9 || maxM(0, 1) <= 0.9 || maxM(-1, 0) <= 0.9 || maxM(0, -1) <= 0.9 ||
           maxM(1, 1) <= 0.9 || maxM(1, -1) <= 0.9 || maxM(-1, 1) <= 0.9 || maxM(-1, -1) <= 0.9;
@@ -1008,10 +1008,10 @@ k_visc1(Grids G, DevEos eos_a, DevTrc trc, const double *__restrict__ Qprev,
     Prim p0, p1;
     const double itn = 1. / (tau - dt), it1 = 1. / tau;
 #pragma unroll
-    for (int k = 0; k < 7; k++) q[k] = k < NC ? Qprev[k * N + c] * itn : 0.;
+    for (int k = 0; k < 7; k++) q[k] = k < NC ? ldu(Qprev + k * N, c) * itn : 0.;
     c2p_mode<CM>(eos, q, p0, guess_of<CM>(gv, c));
 #pragma unroll
-    for (int k = 0; k < 7; k++) q[k] = k < NC ? Qn[k * N + c] * it1 : 0.;
+    for (int k = 0; k < 7; k++) q[k] = k < NC ? ldu(Qn + k * N, c) * it1 : 0.;
     c2p_mode<CM>(eos, q, p1, guess_of<CM>(gv, c));
     const double idt = 1. / dt;
     const double ut0 = drsqrt(1.0 - p0.vx * p0.vx - p0.vy * p0.vy - p0.vz * p0.vz);
@@ -1034,10 +1034,11 @@ k_visc1(Grids G, DevEos eos_a, DevTrc trc, const double *__restrict__ Qprev,
       const long ca = g.cidx(ix + (d == 1), iy + (d == 2), iz + (d == 3));
       const long cb = g.cidx(ix - (d == 1), iy - (d == 2), iz - (d == 3));
       // the reference takes both temperatures before its e test (src/hdo.cpp:474-481)
-      const double TA = VORT ? eos.T(pt[ca]) : 0., TB = VORT ? eos.T(pt[cb]) : 0.;
-      if (pt[ca] > 0. && pt[cb] > 0.) {
-        const double avx = pt[N + ca], avy = pt[2 * N + ca], avz = pt[3 * N + ca];
-        const double bvx = pt[N + cb], bvy = pt[2 * N + cb], bvz = pt[3 * N + cb];
+      const double ea = ldu(pt, ca), eb = ldu(pt, cb);
+      const double TA = VORT ? eos.T(ea) : 0., TB = VORT ? eos.T(eb) : 0.;
+      if (ea > 0. && eb > 0.) {
+        const double avx = ldu(pt + N, ca), avy = ldu(pt + 2 * N, ca), avz = ldu(pt + 3 * N, ca);
+        const double bvx = ldu(pt + N, cb), bvy = ldu(pt + 2 * N, cb), bvz = ldu(pt + 3 * N, cb);
         const double gb = drsqrt(1.0 - bvx * bvx - bvy * bvy - bvz * bvz);
         const double ga = drsqrt(1.0 - avx * avx - avy * avy - avz * avz);
         const double B[4] = {gb, gb * bvx, gb * bvy, gb * bvz};
@@ -1137,8 +1138,8 @@ k_visc1(Grids G, DevEos eos_a, DevTrc trc, const double *__restrict__ Qprev,
   const float duf = (float)du, vzf = (float)vz;
   float P[10], H[10], F[10], cr[10];
 #pragma unroll
-  for (int k = 0; k < 10; k++) P[k] = pi[k * N + c];
-  const float Pc = Pi[c];
+  for (int k = 0; k < 10; k++) P[k] = ldu(pi + k * N, c);
+  const float Pc = ldu(Pi, c);
   const float gmf[4] = {1.f, -1.f, -1.f, -1.f};
   // half step
   const bool rel = taupi > hdt, relB = tauPi > hdt;
@@ -1194,12 +1195,12 @@ k_visc1(Grids G, DevEos eos_a, DevTrc trc, const double *__restrict__ Qprev,
     PF += (lPp * t - dPP * PH * duf) * ig * fdt;
   }
   // six planes of float4: (pi0 0-3) (pi0 4-7) (pi0 8-9, Pi0, 0) and the same for piH0
-  rec[c] = make_float4(F[0], F[1], F[2], F[3]);
-  rec[N + c] = make_float4(F[4], F[5], F[6], F[7]);
-  rec[2 * N + c] = make_float4(F[8], F[9], PF, 0.f);
-  rec[3 * N + c] = make_float4(H[0], H[1], H[2], H[3]);
-  rec[4 * N + c] = make_float4(H[4], H[5], H[6], H[7]);
-  rec[5 * N + c] = make_float4(H[8], H[9], PH, 0.f);
+  stu(rec, c, make_float4(F[0], F[1], F[2], F[3]));
+  stu(rec + N, c, make_float4(F[4], F[5], F[6], F[7]));
+  stu(rec + 2 * N, c, make_float4(F[8], F[9], PF, 0.f));
+  stu(rec + 3 * N, c, make_float4(H[0], H[1], H[2], H[3]));
+  stu(rec + 4 * N, c, make_float4(H[4], H[5], H[6], H[7]));
+  stu(rec + 5 * N, c, make_float4(H[8], H[9], PH, 0.f));
 }
 
 // ISformal loop #2 (src/hdo.cpp:866-962) in FP32 with FP32 pi storage, for
