@@ -1213,13 +1213,12 @@ __device__ __forceinline__ void visc2_cell(const Geom &g, long c, int ix, int iy
                                            double *__restrict__ vcc, float *__restrict__ vf,
                                            double tau, double dt) {
   const long N = g.P;
+  // The vacuum test (e <= 0: no advection, the old pi kept) comes after the
+  // gathers, so that the cell's own loads and the eight neighbours' loads
+  // form two round trips to memory, not three; a vacuum cell's gathers are
+  // of clamped, valid cells and their results are not stored.
   const float e = (float)ph[c], p = (float)ph[N + c];
   const float vx = (float)ph[5 * N + c], vy = (float)ph[6 * N + c], vz = (float)ph[7 * N + c];
-  if (!(e > 0.f)) {
-#pragma unroll
-    for (int k = 0; k < 4; k++) vf[k * N + c] = 0.f;
-    return;
-  }
   const double v2d = ph[5 * N + c] * ph[5 * N + c] + ph[6 * N + c] * ph[6 * N + c] +
                      ph[7 * N + c] * ph[7 * N + c];
   const float v2 = (float)v2d;
@@ -1264,6 +1263,7 @@ __device__ __forceinline__ void visc2_cell(const Geom &g, long c, int ix, int iy
         aH[4] += wH * r4.x; aH[5] += wH * r4.y; aH[6] += wH * r4.z; aH[7] += wH * r4.w;
         aH[8] += wH * r5.x; aH[9] += wH * r5.y; AH += wH * r5.z;
       }
+  const bool live = e > 0.f;
   const float maxT0 = fmax_((e + p) * g2 - p, (e + p) * v2 * g2 + p);
   float maxpi = 0.f;
 #pragma unroll
@@ -1283,19 +1283,21 @@ __device__ __forceinline__ void visc2_cell(const Geom &g, long c, int ix, int iy
     if (AH != 0.f) AH = copysignf(p, AH);
     rescaled = true;
   }
-  vcc[c] = rescaled ? (double)(maxT0 / maxpi) : 1.;
+  if (live) {
+    vcc[c] = rescaled ? (double)(maxT0 / maxpi) : 1.;
 #pragma unroll
-  for (int k = 0; k < 10; k++) {
-    pi[k * N + c] = a[k];
-    piH[k * N + c] = aH[k];
+    for (int k = 0; k < 10; k++) {
+      pi[k * N + c] = a[k];
+      piH[k * N + c] = aH[k];
+    }
+    Pi[c] = A;
+    PiH[c] = AH;
   }
-  Pi[c] = A;
-  PiH[c] = AH;
 #pragma unroll
   for (int i = 0; i < 4; i++) fl[i] += -tf * (a[sym(0, i)] + A * u[0] * u[i]);
   fl[0] += tf * A;
 #pragma unroll
-  for (int i = 0; i < 4; i++) vf[i * N + c] = fl[i];
+  for (int i = 0; i < 4; i++) vf[i * N + c] = live ? fl[i] : 0.f;
 }
 
 // one thread per cell
