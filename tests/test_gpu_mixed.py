@@ -143,17 +143,23 @@ def test_fp64_path_refuses_non_reference_c2p(gpu):
     assert abi.lib().vh_set_c2p_mode(ctx.h, abi.C2P_CLOSED) == abi.VH_ERR_ARG
 
 
-@pytest.mark.parametrize("dims,c2p", [((37, 29, 13), abi.C2P_CLOSED), ((41, 41, 1), abi.C2P_CLOSED),
-                                      ((24, 20, 9), abi.C2P_LEARNED)])
-def test_tile_shapes_identical(gpu, dims, c2p):
+@pytest.mark.parametrize("dims,c2p,kind", [((37, 29, 13), abi.C2P_CLOSED, "ideal"),
+                                           ((41, 41, 1), abi.C2P_CLOSED, "ideal"),
+                                           ((24, 20, 9), abi.C2P_LEARNED, "ideal"),
+                                           ((22, 19, 7), abi.C2P_NEWTON, "table"),
+                                           ((19, 23, 8), abi.C2P_NEWTON, "charged")])
+def test_tile_shapes_identical(gpu, dims, c2p, kind):
     """The row stage kernel (each x- and eta-interface evaluated once; any eta
     chunk, autotuned or forced) and the band kernel (the y-interfaces inside
     a band evaluated once as well) give bit-identical states to the per-cell
-    kernel."""
+    kernel; with the ideal gas, the spline-table instances (kind "table", the
+    crossover EoS) and 7-component states (kind "charged", n_b = 0.05 e)."""
     nx, ny, nz = dims
     xl, yl, zl = (-4.0, 4.0), (-3.5, 3.5), (-1.2, 1.2)
     ic = setups.gaussian3d_ic(nx, ny, nz, xl, yl, zl, e0=30.0, sx=1.5, sy=1.2, eta_plateau=0.4,
                               eta_sigma=0.4)
+    eos = setups.crossover_eos() if kind == "table" else None
+    nb = 0.05 * ic[0] if kind == "charged" else None
     # (kind, CZ) or (STAGE_BANDS, CZ, CY): bands of 2, 4 and 8 rows, ragged
     # against ny = 29, 41 and 20, with eta chunks of 1, 3 and 13 cells
     shapes = [(abi.STAGE_CELLS, 0), (abi.STAGE_AUTO, 0), (abi.STAGE_ROWS, 1),
@@ -163,14 +169,14 @@ def test_tile_shapes_identical(gpu, dims, c2p):
     ref = None
     for sh in shapes:
         ctx = abi.Context(nx, ny, nz, xl, yl, zl, trc=abi.make_trc(etaS=0.1),
-                          precision=abi.PREC_MIXED)
+                          precision=abi.PREC_MIXED, eos=eos)
         ctx.set_c2p(c2p, model)
         if sh[0] == abi.STAGE_BANDS:
             ctx.set_stage_bands(sh[2], sh[1])
             assert ctx.get_stage_bands() == sh[2]
         else:
             ctx.set_stage(*sh)
-        ctx.set_prim(0.6, *ic)
+        ctx.set_prim(0.6, *ic, nb=nb)
         st, _ = ctx.run(0.6, 0.02, 6)
         assert st == 0
         tile = ctx.get_stage()

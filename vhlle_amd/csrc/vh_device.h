@@ -76,6 +76,34 @@ struct SplineTab {
     const double delx = v - x[i];
     return y[i] + delx * (b[i] + delx * (c[i] + delx * d[i]));
   }
+  // FP32 bin records of p (f = 0) and T (f = 1) for the mixed path (fb null:
+  // none; the FP64 lookups serve it then).  fK bins uniform in log2(e) over
+  // [fxlo, x_n-1] (fl0 = log2 fxlo, finv = fK / log2 range), each with at most
+  // one node (with a relative margin of 1e-5 on both sides, which covers the
+  // error of the hardware log2): the node xs and the cubics on either side,
+  // three float4 per record, (xs, xa, ya, ba) (ca, da, xb, yb) (bb, cb, db, 0).
+  // Record 0 of a function is the first interval, for v < fxlo.  Records of
+  // function f start at fb + f (fK + 1) 3.  The cubics are the spline's own,
+  // rounded to FP32, so the value is the spline's to FP32 accuracy.
+  const float4 *fb;
+  int fK;
+  float fl0, finv, fxlo, fxmin, fxmax;
+  __device__ inline float evalf(int f, float v, float &dy, bool &out_of_range) const {
+    out_of_range = !(v >= fxmin && v <= fxmax);
+    int k = 0;
+    if (v >= fxlo) {
+      const float kf = (__log2f(v) - fl0) * finv;
+      k = 1 + (kf > 0.f ? (kf < (float)(fK - 1) ? (int)kf : fK - 1) : 0);
+    }
+    const float4 *R = fb + (long)(f * (fK + 1) + k) * 3;
+    const float4 a = R[0], b = R[1], c = R[2];
+    const bool hi = v >= a.x;
+    const float x0 = hi ? b.z : a.y, y0 = hi ? b.w : a.z, b0 = hi ? c.x : a.w,
+                c0 = hi ? c.y : b.x, d0 = hi ? c.z : b.y;
+    const float dx = v - x0;
+    dy = b0 + dx * (2.f * c0 + 3.f * d0 * dx);
+    return y0 + dx * (b0 + dx * (c0 + dx * d0));
+  }
   // f and its derivative at v (one interval search)
   __device__ inline double eval_d(int f, double v, double &dy, bool &out_of_range) const {
     out_of_range = !(v >= x[0] && v <= x[n - 1]);
@@ -370,14 +398,38 @@ struct DevEos {
   }
   // FP32 forms (mixed precision: interface states of the FP32 flux, the
   // thermodynamics of the FP32 relaxation step)
+  // table function f in FP32: the FP32 records when the table has them
+  __device__ inline float splinef(int f, float e, float &dy) const {
+    if (!tab.fb) {
+      double d;
+      const double v = (double)e;
+      bool out;
+      const double r = tab.eval_d(f, v, d, out);
+      if (out && st) atomicOr(st, VH_ST_EOS_RANGE);
+      dy = (float)d;
+      return (float)r;
+    }
+    bool out;
+    const float r = tab.evalf(f, e, dy, out);
+    if (out && st) atomicOr(st, VH_ST_EOS_RANGE);
+    return r;
+  }
   __device__ float pf(float e) const {
-    return kind == VH_EOS_TABLE ? (float)spline(0, (double)e) : e * (1.f / 3.f);
+    float d;
+    return kind == VH_EOS_TABLE ? splinef(0, e, d) : e * (1.f / 3.f);
+  }
+  // p and dp/de in FP32 (one lookup)
+  __device__ float p_dpdef(float e, float &dp) const {
+    if (kind == VH_EOS_TABLE) return splinef(0, e, dp);
+    dp = 1.f / 3.f;
+    return e * (1.f / 3.f);
   }
   __device__ float cs2f(float e) const {
     return kind == VH_EOS_TABLE ? (float)cs2e((double)e) : 1.f / 3.f;
   }
   __device__ float Tf(float e) const {
-    if (kind == VH_EOS_TABLE) return (float)spline(1, (double)e);
+    float d;
+    if (kind == VH_EOS_TABLE) return splinef(1, e, d);
     return e > 0.f ? __builtin_amdgcn_sqrtf(__builtin_amdgcn_sqrtf(e * (float)(1. / cnst))) : 0.f;
   }
   __device__ float sf(float e, float T) const { return T > 0.f ? (e + pf(e)) / T : 0.f; }

@@ -58,6 +58,95 @@ void natural_spline(const double *x, const double *y, int n, double *b, double *
   b[n - 1] = d[n - 1] = 0.;
 }
 
+// the interval SplineTab::interval returns for v (x_i <= v < x_i+1, the end
+// intervals outside the table)
+int interval_of(const double *x, int n, double v) {
+  if (!(v >= x[1])) return 0;
+  if (v >= x[n - 2]) return n - 2;
+  int lo = 1, hi = n - 2;  // x_lo <= v < x_hi
+  while (hi > lo + 1) {
+    const int mid = (hi + lo) / 2;
+    if (x[mid] > v)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  return lo;
+}
+
+// The cubic of interval i of function f re-centred on xf = (float)x_i: the
+// Taylor coefficients of the same polynomial at xf, in double, then rounded.
+// False if one is not a finite float.
+bool recentred(const double *x, int n, int f, int i, float out[5]) {
+  const double *y = x + (size_t)(1 + 4 * f) * n, *b = y + n, *c = y + 2 * n, *d = y + 3 * n;
+  const float xf = (float)x[i];
+  const double t = (double)xf - x[i];
+  const double v[5] = {(double)xf, y[i] + t * (b[i] + t * (c[i] + t * d[i])),
+                       b[i] + t * (2. * c[i] + 3. * d[i] * t), c[i] + 3. * d[i] * t, d[i]};
+  for (int k = 0; k < 5; k++) {
+    out[k] = (float)v[k];
+    if (!std::isfinite(out[k])) return false;
+  }
+  return true;
+}
+
+// The FP32 bin records of SplineTab::evalf for p and T (functions 0, 1),
+// tried with fK = 4096, 8192, ... 2^18 bins; false if no bin count leaves
+// every bin with at most one node.  A bin's record serves every v the device
+// may map to it: its log2 range widened by a relative 1e-4 on both sides,
+// far above the error of the FP32 bin index.
+bool float_records(const double *x, int n, double xlo, SplineTab &t, std::vector<float> &rec) {
+  const float fxlo = (float)xlo;
+  const double l1 = std::log2(x[n - 1]);
+  for (int fK = 4096; fK <= (1 << 18); fK *= 2) {
+    const float fl0 = (float)std::log2((double)fxlo);
+    const float finv = (float)(fK / (l1 - (double)fl0));
+    if (!(finv > 0.f) || !std::isfinite(finv)) return false;
+    rec.assign((size_t)2 * (fK + 1) * 12, 0.f);
+    bool ok = true;
+    for (int k = -1; k < fK && ok; k++) {
+      double a, b;  // the v that map to record k + 1
+      if (k < 0) {
+        a = -INFINITY;
+        b = (double)std::nextafter(fxlo, -INFINITY);
+      } else {
+        a = k == 0 ? (double)fxlo : std::exp2((double)fl0 + k / (double)finv) * (1. - 1e-4);
+        b = k == fK - 1 ? INFINITY : std::exp2((double)fl0 + (k + 1) / (double)finv) * (1. + 1e-4);
+        if (a < (double)fxlo) a = (double)fxlo;
+      }
+      const int ia = interval_of(x, n, a), ib = interval_of(x, n, b);
+      if (ib > ia + 1) {
+        ok = false;
+        break;
+      }
+      for (int f = 0; f < 2 && ok; f++) {
+        float A[5], B[5];
+        ok = recentred(x, n, f, ia, A) && recentred(x, n, f, ib, B);
+        float *r = rec.data() + ((size_t)f * (fK + 1) + (k + 1)) * 12;
+        r[0] = ib > ia ? (float)x[ib] : 3.0e38f;  // xs: v >= xs takes cubic B
+        for (int j = 0; j < 5; j++) {
+          r[1 + j] = A[j];
+          r[6 + j] = B[j];
+        }
+      }
+    }
+    if (!ok) continue;
+    t.fK = fK;
+    t.fl0 = fl0;
+    t.finv = finv;
+    t.fxlo = fxlo;
+    // v >= fxmin and v <= fxmax hold for a float v exactly when v >= x_0 and
+    // v <= x_n-1 hold in double (the range test of SplineTab::eval)
+    float lo = (float)x[0], hi = (float)x[n - 1];
+    if ((double)lo < x[0]) lo = std::nextafter(lo, INFINITY);
+    if ((double)hi > x[n - 1]) hi = std::nextafter(hi, -INFINITY);
+    t.fxmin = lo;
+    t.fxmax = hi;
+    return true;
+  }
+  return false;
+}
+
 }  // namespace
 
 int eos_table_host(const vh_eos_t &eos, std::vector<double> &buf, int &n) {
@@ -202,6 +291,8 @@ int make_dev_eos(const vh_eos_t *eos, int *st, DevEos &de, double **dbuf) {
   de.tab.hint = nullptr;
   de.tab.K = 0;
   de.tab.l0 = de.tab.inv = 0.f;
+  de.tab.fb = nullptr;
+  de.tab.fK = 0;
   de.st = st;
   if (eos->kind == VH_EOS_IDEAL) return VH_OK;
   if (eos->kind == VH_EOS_CHIRAL || eos->kind == VH_EOS_HADRON) {
@@ -244,15 +335,21 @@ int make_dev_eos(const vh_eos_t *eos, int *st, DevEos &de, double **dbuf) {
     }
     hint[k] = lo;
   }
-  const size_t nb = buf.size() * sizeof(double), bytes = nb + K * sizeof(int);
+  std::vector<float> rec;
+  if (!float_records(x, n, xlo, de.tab, rec)) rec.clear();
+  const size_t nb = buf.size() * sizeof(double), nh = (nb + K * sizeof(int) + 255) / 256 * 256;
+  const size_t bytes = nh + rec.size() * sizeof(float);
   if (hipMalloc(dbuf, bytes) != hipSuccess) return VH_ERR_HIP;
+  char *base = reinterpret_cast<char *>(*dbuf);
   if (hipMemcpy(*dbuf, buf.data(), nb, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(reinterpret_cast<char *>(*dbuf) + nb, hint.data(), K * sizeof(int),
-                hipMemcpyHostToDevice) != hipSuccess) {
+      hipMemcpy(base + nb, hint.data(), K * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      (!rec.empty() && hipMemcpy(base + nh, rec.data(), rec.size() * sizeof(float),
+                                 hipMemcpyHostToDevice) != hipSuccess)) {
     (void)hipFree(*dbuf);
     *dbuf = nullptr;
     return VH_ERR_HIP;
   }
+  de.tab.fb = rec.empty() ? nullptr : reinterpret_cast<const float4 *>(base + nh);
   de.tab.x = *dbuf;
   de.tab.n = n;
   de.tab.hint = reinterpret_cast<const int *>(reinterpret_cast<const char *>(*dbuf) + nb);

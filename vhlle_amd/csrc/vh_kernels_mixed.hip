@@ -270,10 +270,8 @@ __device__ __forceinline__ PrimF c2p_newton_f32(const DevEos &eos, const double 
   for (int it = 0; it < 16; it++) {
     const float eh = y + x * w;
     float ph, dpde;
-    if (eos.kind == VH_EOS_TABLE) {  // one table search for p and dp/de
-      double dp;
-      ph = (float)eos.p_dpde((double)(eh * Qtf), dp) * iQt;
-      dpde = (float)dp;
+    if (eos.kind == VH_EOS_TABLE) {  // one table lookup for p and dp/de
+      ph = eos.p_dpdef(eh * Qtf, dpde) * iQt;
     } else {
       ph = eos.pf(eh * Qtf) * iQt;
       dpde = eos.cs2f(eh * Qtf);
