@@ -42,6 +42,11 @@ __device__ inline double smax(double a, double b) { return (a < b) ? b : a; }
 // bins uniform in log2(e) over [2^l0, x_n-1]; a walk of a step or two then
 // lands on the interval the bisection would return (x_i <= v < x_i+1, or the
 // end interval outside the table), so the result does not depend on the hint.
+struct __align__(16) SplineBin {
+  double xs;  // the node inside the bin's range (NaN: none)
+  int ia;     // the interval at the start of the range
+  float xsf;  // xs rounded to FP32 (NaN: none)
+};
 struct SplineTab {
   const double *x;  // n nodes, then per function: y, b, c, d (n each)
   int n;
@@ -67,35 +72,69 @@ struct SplineTab {
     }
     return lo;
   }
+  // Direct lookup (bins non-null when the host could build it): fK bins
+  // uniform in log2(e) over [fxlo, x_n-1] (fl0 = log2 fxlo, finv = fK / log2
+  // range), each holding at most one node, with a relative margin of 1e-4 on
+  // both sides that covers the error of the FP32 bin index.  bins[0] serves
+  // v < fxlo, bins[k + 1] bin k: the first interval ia of the bin's range and
+  // the node xs that ends it (NaN when none does), so that the interval of v
+  // is ia + (v >= xs), with no search.  The coefficients of interval i follow
+  // interleaved, one cache line apart at most:
+  //   c64[f n + i]: three double2 (x_i, y_i) (b_i, c_i) (d_i, 0), functions
+  //       0-2, the spline's own (eval and eval_d return exactly the values of
+  //       the interval search);
+  // The FP32 lookup of the mixed-precision kernels reads one record per bin
+  // instead (a lookup inside a Newton iteration waits on one load, not two):
+  //   c32[f (fK + 1) + k]: three float4 (xs, xa, ya, ba) (ca, da, xb, yb)
+  //       (bb, cb, db, 0), functions 0-1, the cubics of the intervals ia and
+  //       ia + 1, each re-centred on its node rounded to FP32 and rounded to
+  //       FP32 (null when a coefficient does not fit a float: evalf is then
+  //       not used).
+  const SplineBin *bins;
+  const double2 *c64;
+  const float4 *c32;
+  int fK;
+  float fl0, finv, fxlo, fxmin, fxmax;
+  __device__ inline int bin(float vf) const {
+    int k = 0;
+    if (vf >= fxlo) {
+      const float kf = (__log2f(vf) - fl0) * finv;
+      k = 1 + (kf > 0.f ? (kf < (float)(fK - 1) ? (int)kf : fK - 1) : 0);
+    }
+    return k;
+  }
+  // the node and coefficients of the interval of v, from the direct lookup
+  __device__ inline void cubic(int f, double v, double &x0, double &y0, double &b0, double &c0,
+                               double &d0) const {
+    const SplineBin B = bins[bin((float)v)];
+    const int i = B.ia + (v >= B.xs ? 1 : 0);
+    const double2 *C = c64 + ((long)f * n + i) * 3;
+    const double2 r0 = C[0], r1 = C[1], r2 = C[2];
+    x0 = r0.x;
+    y0 = r0.y;
+    b0 = r1.x;
+    c0 = r1.y;
+    d0 = r2.x;
+  }
   // function f (0: p, 1: T, 2: mu) at v; *out_of_range set outside [x_0, x_n-1]
   __device__ inline double eval(int f, double v, bool &out_of_range) const {
     out_of_range = !(v >= x[0] && v <= x[n - 1]);
+    if (bins) {
+      double x0, y0, b0, c0, d0;
+      cubic(f, v, x0, y0, b0, c0, d0);
+      const double delx = v - x0;
+      return y0 + delx * (b0 + delx * (c0 + delx * d0));
+    }
     const int i = interval(v);
     const double *y = x + (long)(1 + 4 * f) * n;
     const double *b = y + n, *c = y + 2 * n, *d = y + 3 * n;
     const double delx = v - x[i];
     return y[i] + delx * (b[i] + delx * (c[i] + delx * d[i]));
   }
-  // FP32 bin records of p (f = 0) and T (f = 1) for the mixed path (fb null:
-  // none; the FP64 lookups serve it then).  fK bins uniform in log2(e) over
-  // [fxlo, x_n-1] (fl0 = log2 fxlo, finv = fK / log2 range), each with at most
-  // one node (with a relative margin of 1e-5 on both sides, which covers the
-  // error of the hardware log2): the node xs and the cubics on either side,
-  // three float4 per record, (xs, xa, ya, ba) (ca, da, xb, yb) (bb, cb, db, 0).
-  // Record 0 of a function is the first interval, for v < fxlo.  Records of
-  // function f start at fb + f (fK + 1) 3.  The cubics are the spline's own,
-  // rounded to FP32, so the value is the spline's to FP32 accuracy.
-  const float4 *fb;
-  int fK;
-  float fl0, finv, fxlo, fxmin, fxmax;
+  // f and its derivative in FP32 (c32 non-null)
   __device__ inline float evalf(int f, float v, float &dy, bool &out_of_range) const {
     out_of_range = !(v >= fxmin && v <= fxmax);
-    int k = 0;
-    if (v >= fxlo) {
-      const float kf = (__log2f(v) - fl0) * finv;
-      k = 1 + (kf > 0.f ? (kf < (float)(fK - 1) ? (int)kf : fK - 1) : 0);
-    }
-    const float4 *R = fb + (long)(f * (fK + 1) + k) * 3;
+    const float4 *R = c32 + ((long)f * (fK + 1) + bin(v)) * 3;
     const float4 a = R[0], b = R[1], c = R[2];
     const bool hi = v >= a.x;
     const float x0 = hi ? b.z : a.y, y0 = hi ? b.w : a.z, b0 = hi ? c.x : a.w,
@@ -107,6 +146,13 @@ struct SplineTab {
   // f and its derivative at v (one interval search)
   __device__ inline double eval_d(int f, double v, double &dy, bool &out_of_range) const {
     out_of_range = !(v >= x[0] && v <= x[n - 1]);
+    if (bins) {
+      double x0, y0, b0, c0, d0;
+      cubic(f, v, x0, y0, b0, c0, d0);
+      const double delx = v - x0;
+      dy = b0 + delx * (2. * c0 + 3. * d0 * delx);
+      return y0 + delx * (b0 + delx * (c0 + delx * d0));
+    }
     const int i = interval(v);
     const double *y = x + (long)(1 + 4 * f) * n;
     const double *b = y + n, *c = y + 2 * n, *d = y + 3 * n;
@@ -400,7 +446,7 @@ struct DevEos {
   // thermodynamics of the FP32 relaxation step)
   // table function f in FP32: the FP32 records when the table has them
   __device__ inline float splinef(int f, float e, float &dy) const {
-    if (!tab.fb) {
+    if (!tab.c32) {
       double d;
       const double v = (double)e;
       bool out;

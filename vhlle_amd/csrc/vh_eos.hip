@@ -14,6 +14,7 @@
 //   b_i = (y_i+1 - y_i)/h_i - h_i (c_i+1 + 2 c_i)/3,  d_i = (c_i+1 - c_i)/(3 h_i).
 #include "vh_eos.h"
 
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 
@@ -90,61 +91,86 @@ bool recentred(const double *x, int n, int f, int i, float out[5]) {
   return true;
 }
 
-// The FP32 bin records of SplineTab::evalf for p and T (functions 0, 1),
-// tried with fK = 4096, 8192, ... 2^18 bins; false if no bin count leaves
-// every bin with at most one node.  A bin's record serves every v the device
-// may map to it: its log2 range widened by a relative 1e-4 on both sides,
-// far above the error of the FP32 bin index.
-bool float_records(const double *x, int n, double xlo, SplineTab &t, std::vector<float> &rec) {
+// The direct lookup of SplineTab: the bins, tried with fK = 4096, 8192, ...
+// 2^18; false if no bin count leaves every bin with at most one node.  A
+// bin's entry serves every v the device may map to it: its range widened by a
+// relative 1e-4 on both sides, far above the error of the FP32 bin index (the
+// rounding of v to FP32 and of the log2).  c64: the interleaved FP64
+// coefficients per interval; c32: the FP32 records per bin (empty when a
+// coefficient does not fit a float).
+bool direct_lookup(const double *x, int n, double xlo, SplineTab &t, std::vector<SplineBin> &bins,
+                   std::vector<double> &c64, std::vector<float> &c32) {
   const float fxlo = (float)xlo;
   const double l1 = std::log2(x[n - 1]);
-  for (int fK = 4096; fK <= (1 << 18); fK *= 2) {
-    const float fl0 = (float)std::log2((double)fxlo);
+  const float fl0 = (float)std::log2((double)fxlo);
+  int fK = 4096;
+  for (; fK <= (1 << 18); fK *= 2) {
     const float finv = (float)(fK / (l1 - (double)fl0));
     if (!(finv > 0.f) || !std::isfinite(finv)) return false;
-    rec.assign((size_t)2 * (fK + 1) * 12, 0.f);
+    bins.assign((size_t)fK + 1, SplineBin{0., 0, 0.f});
     bool ok = true;
-    for (int k = -1; k < fK && ok; k++) {
-      double a, b;  // the v that map to record k + 1
+    for (int k = -1; k < fK; k++) {
+      double a, b;  // the v that map to entry k + 1
       if (k < 0) {
         a = -INFINITY;
-        b = (double)std::nextafter(fxlo, -INFINITY);
+        b = (double)fxlo * (1. + 1e-4);
       } else {
-        a = k == 0 ? (double)fxlo : std::exp2((double)fl0 + k / (double)finv) * (1. - 1e-4);
+        a = std::exp2((double)fl0 + k / (double)finv) * (1. - 1e-4);
         b = k == fK - 1 ? INFINITY : std::exp2((double)fl0 + (k + 1) / (double)finv) * (1. + 1e-4);
-        if (a < (double)fxlo) a = (double)fxlo;
+        if (k == 0) a = std::fmin(a, (double)fxlo * (1. - 1e-4));
       }
       const int ia = interval_of(x, n, a), ib = interval_of(x, n, b);
       if (ib > ia + 1) {
         ok = false;
         break;
       }
-      for (int f = 0; f < 2 && ok; f++) {
-        float A[5], B[5];
-        ok = recentred(x, n, f, ia, A) && recentred(x, n, f, ib, B);
-        float *r = rec.data() + ((size_t)f * (fK + 1) + (k + 1)) * 12;
-        r[0] = ib > ia ? (float)x[ib] : 3.0e38f;  // xs: v >= xs takes cubic B
-        for (int j = 0; j < 5; j++) {
-          r[1 + j] = A[j];
-          r[6 + j] = B[j];
-        }
-      }
+      bins[k + 1] = SplineBin{ib > ia ? x[ib] : NAN, ia, ib > ia ? (float)x[ib] : NAN};
     }
     if (!ok) continue;
-    t.fK = fK;
-    t.fl0 = fl0;
     t.finv = finv;
-    t.fxlo = fxlo;
-    // v >= fxmin and v <= fxmax hold for a float v exactly when v >= x_0 and
-    // v <= x_n-1 hold in double (the range test of SplineTab::eval)
-    float lo = (float)x[0], hi = (float)x[n - 1];
-    if ((double)lo < x[0]) lo = std::nextafter(lo, INFINITY);
-    if ((double)hi > x[n - 1]) hi = std::nextafter(hi, -INFINITY);
-    t.fxmin = lo;
-    t.fxmax = hi;
-    return true;
+    break;
   }
-  return false;
+  if (fK > (1 << 18)) return false;
+  t.fK = fK;
+  t.fl0 = fl0;
+  t.fxlo = fxlo;
+  // v >= fxmin and v <= fxmax hold for a float v exactly when v >= x_0 and
+  // v <= x_n-1 hold in double (the range test of SplineTab::eval)
+  float lo = (float)x[0], hi = (float)x[n - 1];
+  if ((double)lo < x[0]) lo = std::nextafter(lo, INFINITY);
+  if ((double)hi > x[n - 1]) hi = std::nextafter(hi, -INFINITY);
+  t.fxmin = lo;
+  t.fxmax = hi;
+  c64.assign((size_t)3 * n * 6, 0.);
+  for (int f = 0; f < 3; f++) {
+    const double *y = x + (size_t)(1 + 4 * f) * n, *b = y + n, *c = y + 2 * n, *d = y + 3 * n;
+    for (int i = 0; i < n - 1; i++) {
+      double *r = c64.data() + ((size_t)f * n + i) * 6;
+      r[0] = x[i];
+      r[1] = y[i];
+      r[2] = b[i];
+      r[3] = c[i];
+      r[4] = d[i];
+    }
+  }
+  c32.assign((size_t)2 * (fK + 1) * 12, 0.f);
+  for (int f = 0; f < 2; f++)
+    for (int k = 0; k <= fK; k++) {
+      const SplineBin &e = bins[k];
+      const bool two = !std::isnan(e.xs);
+      float A[5], B[5];
+      if (!recentred(x, n, f, e.ia, A) || !recentred(x, n, f, e.ia + (two ? 1 : 0), B)) {
+        c32.clear();
+        return true;
+      }
+      float *r = c32.data() + ((size_t)f * (fK + 1) + k) * 12;
+      r[0] = two ? e.xsf : FLT_MAX;  // v >= xs takes cubic B
+      for (int j = 0; j < 5; j++) {
+        r[1 + j] = A[j];
+        r[6 + j] = B[j];
+      }
+    }
+  return true;
 }
 
 }  // namespace
@@ -291,7 +317,9 @@ int make_dev_eos(const vh_eos_t *eos, int *st, DevEos &de, double **dbuf) {
   de.tab.hint = nullptr;
   de.tab.K = 0;
   de.tab.l0 = de.tab.inv = 0.f;
-  de.tab.fb = nullptr;
+  de.tab.bins = nullptr;
+  de.tab.c64 = nullptr;
+  de.tab.c32 = nullptr;
   de.tab.fK = 0;
   de.st = st;
   if (eos->kind == VH_EOS_IDEAL) return VH_OK;
@@ -335,21 +363,33 @@ int make_dev_eos(const vh_eos_t *eos, int *st, DevEos &de, double **dbuf) {
     }
     hint[k] = lo;
   }
-  std::vector<float> rec;
-  if (!float_records(x, n, xlo, de.tab, rec)) rec.clear();
+  std::vector<SplineBin> bins;
+  std::vector<double> c64;
+  std::vector<float> c32;
+  if (!direct_lookup(x, n, xlo, de.tab, bins, c64, c32)) {
+    bins.clear();
+    c64.clear();
+    c32.clear();
+  }
   const size_t nb = buf.size() * sizeof(double), nh = (nb + K * sizeof(int) + 255) / 256 * 256;
-  const size_t bytes = nh + rec.size() * sizeof(float);
+  const size_t nbin = bins.size() * sizeof(SplineBin), n64 = c64.size() * sizeof(double);
+  const size_t bytes = nh + nbin + n64 + c32.size() * sizeof(float);
   if (hipMalloc(dbuf, bytes) != hipSuccess) return VH_ERR_HIP;
   char *base = reinterpret_cast<char *>(*dbuf);
   if (hipMemcpy(*dbuf, buf.data(), nb, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(base + nb, hint.data(), K * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
-      (!rec.empty() && hipMemcpy(base + nh, rec.data(), rec.size() * sizeof(float),
+      (!bins.empty() &&
+       (hipMemcpy(base + nh, bins.data(), nbin, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(base + nh + nbin, c64.data(), n64, hipMemcpyHostToDevice) != hipSuccess)) ||
+      (!c32.empty() && hipMemcpy(base + nh + nbin + n64, c32.data(), c32.size() * sizeof(float),
                                  hipMemcpyHostToDevice) != hipSuccess)) {
     (void)hipFree(*dbuf);
     *dbuf = nullptr;
     return VH_ERR_HIP;
   }
-  de.tab.fb = rec.empty() ? nullptr : reinterpret_cast<const float4 *>(base + nh);
+  de.tab.bins = bins.empty() ? nullptr : reinterpret_cast<const SplineBin *>(base + nh);
+  de.tab.c64 = bins.empty() ? nullptr : reinterpret_cast<const double2 *>(base + nh + nbin);
+  de.tab.c32 = c32.empty() ? nullptr : reinterpret_cast<const float4 *>(base + nh + nbin + n64);
   de.tab.x = *dbuf;
   de.tab.n = n;
   de.tab.hint = reinterpret_cast<const int *>(reinterpret_cast<const char *>(*dbuf) + nb);
