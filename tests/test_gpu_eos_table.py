@@ -105,6 +105,46 @@ def test_mixed_with_table(gpu, table, c2p):
     assert eT < 2e-6
 
 
+def _probe_points(e_nodes, rng):
+    """Every node and its 1-3 ulp neighbours, random log-uniform points across
+    and beyond the table, and values below the first positive node."""
+    pos = e_nodes[e_nodes > 0]
+    pts = [e_nodes]
+    for k in (1, 2, 3):
+        up = dn = pos.copy()
+        for _ in range(k):
+            up = np.nextafter(up, np.inf)
+            dn = np.nextafter(dn, -np.inf)
+        pts += [up, dn]
+    lo, hi = np.log10(pos[0]), np.log10(e_nodes[-1])
+    pts.append(10 ** rng.uniform(lo - 1, hi + 0.5, 200_000))
+    pts.append(rng.uniform(0, pos[0], 1000))
+    return np.concatenate(pts)
+
+
+@pytest.mark.parametrize("kind", ["log_uniform", "uniform_2000", "uniform_20000"])
+def test_direct_lookup_matches_search(gpu, table, kind):
+    """The device spline lookup by log2 bins (each bin holds at most one node,
+    so the interval is the bin's first interval or the next) returns the
+    values of the interval search bit for bit: the host evaluation bisects.
+    Tables: the log-uniform crossover table; 2 000 uniformly spaced nodes (the
+    bins must be finer than the top nodes' spacing: 32 768 bins); 20 000
+    uniformly spaced nodes (no bin count up to 2^18 separates the top nodes:
+    the device falls back to the hint search)."""
+    rng = np.random.default_rng(41)
+    if kind == "log_uniform":
+        tab = table
+    else:
+        n = int(kind.split("_")[1])
+        e_t = np.linspace(0.0, 50.0, n)
+        tab = abi.EosTable(e_t, e_t / 3.0 - 0.01 * np.sin(e_t), 0.2 * e_t ** 0.25 + 0.001 * e_t)
+    e = _probe_points(tab.e, rng)
+    gpu_out = abi.eos_eval(tab, e, on_gpu=True)
+    host_out = abi.eos_eval(tab, e, on_gpu=False)
+    assert np.array_equal(gpu_out[:, :2], host_out[:, :2])
+    assert np.array_equal(gpu_out[:, 5], host_out[:, 5])
+
+
 def test_closed_form_refused_with_table(gpu, table):
     ctx = abi.Context(10, 10, 1, (-1, 1), (-1, 1), (-1, 1), precision=abi.PREC_MIXED, eos=table)
     assert abi.lib().vh_set_c2p_mode(ctx.h, abi.C2P_CLOSED) == abi.VH_ERR_ARG

@@ -94,7 +94,10 @@ struct SplineTab {
   const double2 *c64;
   const float4 *c32;
   int fK;
-  float fl0, finv, fxlo, fxmin, fxmax;
+  float fl0, finv, fxlo;
+  // the table's range as FP32 bounds, set for every table: a float v has
+  // v >= fxmin and v <= fxmax exactly when x_0 <= v <= x_n-1
+  float fxmin, fxmax;
   __device__ inline int bin(float vf) const {
     int k = 0;
     if (vf >= fxlo) {

@@ -134,13 +134,6 @@ bool direct_lookup(const double *x, int n, double xlo, SplineTab &t, std::vector
   t.fK = fK;
   t.fl0 = fl0;
   t.fxlo = fxlo;
-  // v >= fxmin and v <= fxmax hold for a float v exactly when v >= x_0 and
-  // v <= x_n-1 hold in double (the range test of SplineTab::eval)
-  float lo = (float)x[0], hi = (float)x[n - 1];
-  if ((double)lo < x[0]) lo = std::nextafter(lo, INFINITY);
-  if ((double)hi > x[n - 1]) hi = std::nextafter(hi, -INFINITY);
-  t.fxmin = lo;
-  t.fxmax = hi;
   c64.assign((size_t)3 * n * 6, 0.);
   for (int f = 0; f < 3; f++) {
     const double *y = x + (size_t)(1 + 4 * f) * n, *b = y + n, *c = y + 2 * n, *d = y + 3 * n;
@@ -363,6 +356,13 @@ int make_dev_eos(const vh_eos_t *eos, int *st, DevEos &de, double **dbuf) {
     }
     hint[k] = lo;
   }
+  // v >= fxmin and v <= fxmax hold for a float v exactly when v >= x_0 and
+  // v <= x_n-1 hold in double (the range test of SplineTab::eval)
+  float lo = (float)x[0], hi = (float)x[n - 1];
+  if ((double)lo < x[0]) lo = std::nextafter(lo, INFINITY);
+  if ((double)hi > x[n - 1]) hi = std::nextafter(hi, -INFINITY);
+  de.tab.fxmin = lo;
+  de.tab.fxmax = hi;
   std::vector<SplineBin> bins;
   std::vector<double> c64;
   std::vector<float> c32;
