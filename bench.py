@@ -98,30 +98,36 @@ ALG_BYTES = {
 # shape, con2prim); None where not profiled.
 KIB = 1024
 TRAFFIC = {
-    ("fp64", "visc_relax", None, "reference"): (2 * 890037 + 1570517) * KIB,
-    ("fp64", "visc_advect", None, "reference"): (2 * 735496 + 843750) * KIB,
-    ("fp64", "visc_flux_update", None, "reference"): (2 * 764741 + 218772) * KIB,
-    ("mixed", "stage_pred", "cells", "closed"): (2 * 93440 + 130859) * KIB,
-    ("mixed", "stage_corr", "cells", "closed"): (2 * 155957 + 130859) * KIB,
-    ("mixed", "stage_pred", "cells", "newton"): (2 * 93587 + 130859) * KIB,
-    ("mixed", "stage_corr", "cells", "newton"): (2 * 156336 + 130859) * KIB,
-    ("mixed", "stage_pred", "cells", "learned"): (2 * 103567 + 130859) * KIB,
-    ("mixed", "stage_corr", "cells", "learned"): (2 * 166414 + 130859) * KIB,
-    ("mixed", "stage_pred", "rows", "closed"): (2 * 133627 + 127709) * KIB,
-    ("mixed", "stage_corr", "rows", "closed"): (2 * 211703 + 127177) * KIB,
-    ("mixed", "stage_pred", "rows", "newton"): (2 * 132546 + 127060) * KIB,
-    ("mixed", "stage_corr", "rows", "newton"): (2 * 216336 + 126909) * KIB,
-    ("mixed", "stage_pred", "rows", "learned"): (2 * 148043 + 127316) * KIB,
-    ("mixed", "stage_corr", "rows", "learned"): (2 * 238204 + 127058) * KIB,
-    ("mixed", "visc_relax", None, "closed"): (2 * 353639 + 375000) * KIB,
-    ("mixed", "visc_relax", None, "newton"): (2 * 352869 + 375000) * KIB,
-    ("mixed", "visc_relax", None, "learned"): (2 * 360788 + 375000) * KIB,
-    ("mixed", "visc_advect", None, None): (2 * 330505 + 437500) * KIB,
-    ("mixed", "visc_flux_update", None, None): (2 * 262489 + 125000) * KIB,
-    ("mixed", "visc_prim", None, "closed"): (2 * 62524 + 281250) * KIB,
-    ("mixed", "visc_prim", None, "newton"): (2 * 62614 + 281250) * KIB,
-    ("mixed", "visc_prim", None, "learned"): (2 * 70417 + 281250) * KIB,
-    ("mixed", "c2p_guess", None, "learned"): (2 * 62615 + 15625) * KIB,
+    ("fp64", "visc_relax", None, "reference"): (2 * 536501 + 843248) * KIB,
+    ("fp64", "visc_advect", None, "reference"): (2 * 550075 + 843750) * KIB,
+    ("fp64", "visc_flux_update", None, "reference"): (2 * 544709 + 218781) * KIB,
+    ("mixed", "stage_pred", "cells", "closed"): (2 * 93176 + 130859) * KIB,
+    ("mixed", "stage_corr", "cells", "closed"): (2 * 155869 + 130859) * KIB,
+    ("mixed", "stage_pred", "cells", "newton"): (2 * 93179 + 130859) * KIB,
+    ("mixed", "stage_corr", "cells", "newton"): (2 * 155871 + 130859) * KIB,
+    ("mixed", "stage_pred", "cells", "learned"): (2 * 101129 + 130859) * KIB,
+    ("mixed", "stage_corr", "cells", "learned"): (2 * 165840 + 130859) * KIB,
+    ("mixed", "stage_pred", "rows", "closed"): (2 * 104874 + 137782) * KIB,
+    ("mixed", "stage_corr", "rows", "closed"): (2 * 209914 + 136216) * KIB,
+    ("mixed", "stage_pred", "rows", "newton"): (2 * 104757 + 137837) * KIB,
+    ("mixed", "stage_corr", "rows", "newton"): (2 * 209103 + 136391) * KIB,
+    ("mixed", "stage_pred", "rows", "learned"): (2 * 104818 + 137815) * KIB,
+    ("mixed", "stage_corr", "rows", "learned"): (2 * 235516 + 136097) * KIB,
+    ("mixed", "stage_pred", "bands", "closed"): (2 * 423035 + 135078) * KIB,
+    ("mixed", "stage_corr", "bands", "closed"): (2 * 517782 + 134676) * KIB,
+    ("mixed", "stage_pred", "bands", "newton"): (2 * 427191 + 135047) * KIB,
+    ("mixed", "stage_corr", "bands", "newton"): (2 * 517296 + 134701) * KIB,
+    ("mixed", "stage_pred", "bands", "learned"): (2 * 480048 + 135445) * KIB,
+    ("mixed", "stage_corr", "bands", "learned"): (2 * 582918 + 134369) * KIB,
+    ("mixed", "visc_relax", None, "closed"): (2 * 352326 + 375000) * KIB,
+    ("mixed", "visc_relax", None, "newton"): (2 * 352267 + 375000) * KIB,
+    ("mixed", "visc_relax", None, "learned"): (2 * 360090 + 375000) * KIB,
+    ("mixed", "visc_advect", None, None): (2 * 330191 + 437500) * KIB,
+    ("mixed", "visc_flux_update", None, None): (2 * 263348 + 125000) * KIB,
+    ("mixed", "visc_prim", None, "closed"): (2 * 62530 + 281250) * KIB,
+    ("mixed", "visc_prim", None, "newton"): (2 * 62561 + 281250) * KIB,
+    ("mixed", "visc_prim", None, "learned"): (2 * 70372 + 281250) * KIB,
+    ("mixed", "c2p_guess", None, "learned"): (2 * 62617 + 15625) * KIB,
 }
 
 
