@@ -107,7 +107,8 @@ def test_mixed_with_table(gpu, table, c2p):
 
 def _probe_points(e_nodes, rng):
     """Every node and its 1-3 ulp neighbours, random log-uniform points across
-    and beyond the table, and values below the first positive node."""
+    and beyond the table, values below the first positive node and negative
+    values."""
     pos = e_nodes[e_nodes > 0]
     pts = [e_nodes]
     for k in (1, 2, 3):
@@ -119,6 +120,7 @@ def _probe_points(e_nodes, rng):
     lo, hi = np.log10(pos[0]), np.log10(e_nodes[-1])
     pts.append(10 ** rng.uniform(lo - 1, hi + 0.5, 200_000))
     pts.append(rng.uniform(0, pos[0], 1000))
+    pts.append(-10 ** rng.uniform(lo - 1, hi, 1000))  # below the table: the first cubic
     return np.concatenate(pts)
 
 

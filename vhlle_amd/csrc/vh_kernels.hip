@@ -19,6 +19,16 @@
 
 namespace vh {
 
+// A store of a result that the kernel does not read back and that the next
+// kernel reads after it has left the L2 (the arrays are hundreds of MB):
+// nontemporal, so that it does not evict the lines the kernel's own gathers
+// reuse (the interface fluxes, the stage update, the relaxed tensors and the
+// advected ones; the FP64 step 2-3 % faster).
+template <class T>
+__device__ __forceinline__ void st_nt(T *p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+
 // ------------------------------------------------------------- helpers
 __device__ inline void decompose(const Geom &g, long c, int &ix, int &iy, int &iz) {
   ix = (int)(c % g.nx);
@@ -153,7 +163,7 @@ __device__ __forceinline__ double flux_tail(const Geom &g, const double *__restr
       break;
     }
 #pragma unroll
-    for (int k = 0; k < 7; k++) F[k * N + c] = f[k];
+    for (int k = 0; k < 7; k++) st_nt(F + k * N + c, f[k]);
     ok = 1;
     speed = smax(-bl, br);
   } while (0);
@@ -334,11 +344,11 @@ k_update(Geom g, DevEos eos_a, const double *__restrict__ Q, const double *__res
   for (int k = 0; k < 7; k++) acc[k] += S[k];
   if (MODE == 0) {
 #pragma unroll
-    for (int k = 0; k < 7; k++) out[k * N + c] = Q[k * N + c] + acc[k];
+    for (int k = 0; k < 7; k++) st_nt(out + k * N + c, Q[k * N + c] + acc[k]);
   } else {
     const bool skip = Q[c] + acc[0] < 0.;
 #pragma unroll
-    for (int k = 0; k < 7; k++) out[k * N + c] = skip ? Q[k * N + c] : Q[k * N + c] + acc[k];
+    for (int k = 0; k < 7; k++) st_nt(out + k * N + c, skip ? Q[k * N + c] : Q[k * N + c] + acc[k]);
   }
 }
 
@@ -869,7 +879,7 @@ k_visc1(Geom g, DevEos eos_a, DevTrc trc, const double *__restrict__ Qprev,
                   gm[l]) * 0.5 * dt;
       }
       H[ij] = t;
-      piH0[ij * N + c] = t;
+      st_nt(piH0 + ij * N + c, t);
     }
 #pragma unroll
   for (int k = 0; k < 4; k++)
@@ -915,7 +925,7 @@ k_visc1(Geom g, DevEos eos_a, DevTrc trc, const double *__restrict__ Qprev,
                       gm[l]) *
                dt;
       }
-      pi0[ij * N + c] = t;  // stored at once: F is not read again
+      st_nt(pi0 + ij * N + c, t);  // stored at once: F is not read again
     }
 #pragma unroll
   for (int k = 0; k < 4; k++)
@@ -923,8 +933,8 @@ k_visc1(Geom g, DevEos eos_a, DevTrc trc, const double *__restrict__ Qprev,
     for (int l = 0; l < 4; l++)
       PF += dg(lPp * H[sym(k, l)] * sig[sym(k, l)] * gm[k] * gm[l]) * dt;
   PF += dg(-dPP * PH * du) * dt;
-  PiH0[c] = PH;
-  Pi0[c] = PF;
+  st_nt(PiH0 + c, PH);
+  st_nt(Pi0 + c, PF);
 }
 
 // ISformal loop #2 (src/hdo.cpp:866-962) plus loop #1's flux term
@@ -1015,14 +1025,14 @@ k_visc2(Geom g, const double *__restrict__ ph, const double *__restrict__ pi0,
     if (AH != 0.) AH = AH / fabs(AH) * p;
     rescaled = true;
   }
-  vcc[c] = rescaled ? maxT0 / maxpi : 1.;
+  st_nt(vcc + c, rescaled ? maxT0 / maxpi : 1.);
 #pragma unroll
   for (int k = 0; k < 10; k++) {
-    pi[k * N + c] = a[k];
-    piH[k * N + c] = aH[k];
+    st_nt(pi + k * N + c, a[k]);
+    st_nt(piH + k * N + c, aH[k]);
   }
-  Pi[c] = A;
-  PiH[c] = AH;
+  st_nt(Pi + c, A);
+  st_nt(PiH + c, AH);
   // loop #2 flux with the new pi, Pi (src/hdo.cpp:950-961)
   double f2[4];
 #pragma unroll
@@ -1033,7 +1043,7 @@ k_visc2(Geom g, const double *__restrict__ ph, const double *__restrict__ pi0,
     double t = 0.;
     t += fl[i];
     t += f2[i];
-    vf[i * N + c] = t;
+    st_nt(vf + i * N + c, t);
   }
 }
 
