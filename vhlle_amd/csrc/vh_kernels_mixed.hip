@@ -1232,10 +1232,26 @@ __device__ __forceinline__ void visc2_cell(const Geom &g, long c, int ix, int iy
   // gathers, so that the cell's own loads and the eight neighbours' loads
   // form two round trips to memory, not three; a vacuum cell's gathers are
   // of clamped, valid cells and their results are not stored.
+  const double vxd = ph[5 * N + c], vyd = ph[6 * N + c], vzd = ph[7 * N + c];
+  // the upwind directions, the signs of xm, ym, zm below, taken from the
+  // velocity itself (dt/dx, dt/dy, dt/(dz (tau - dt/2)) > 0; where xm
+  // underflows to 0 in FP32 the extra corner has weight 0), so that the
+  // gathers are issued before the weights' divisions and overlap them
+  const int sx = vxd < 0. ? 1 : (vxd > 0. ? -1 : 0);
+  const int sy = vyd < 0. ? 1 : (vyd > 0. ? -1 : 0);
+  const int sz = vzd < 0. ? 1 : (vzd > 0. ? -1 : 0);
+  float4 R[8][6];
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int jx = j >> 2, jy = (j >> 1) & 1, jz = j & 1;
+    const long c1 = g.cidx(ix + jx * sx, iy + jy * sy, iz + jz * sz);
+#pragma unroll
+    for (int q = 0; q < 6; q++) R[j][q] = rec[q * N + c1];
+  }
+  __builtin_amdgcn_sched_barrier(0);
   const float e = (float)ph[c], p = (float)ph[N + c];
-  const float vx = (float)ph[5 * N + c], vy = (float)ph[6 * N + c], vz = (float)ph[7 * N + c];
-  const double v2d = ph[5 * N + c] * ph[5 * N + c] + ph[6 * N + c] * ph[6 * N + c] +
-                     ph[7 * N + c] * ph[7 * N + c];
+  const float vx = (float)vxd, vy = (float)vyd, vz = (float)vzd;
+  const double v2d = vxd * vxd + vyd * vyd + vzd * vzd;
   const float v2 = (float)v2d;
   const float g2 = (float)(1. / (1. - v2d)), gamma = fsqrt(g2);
   const float u[4] = {gamma, gamma * vx, gamma * vy, gamma * vz};
@@ -1247,37 +1263,30 @@ __device__ __forceinline__ void visc2_cell(const Geom &g, long c, int ix, int iy
     for (int i = 0; i < 4; i++) fl[i] = (tf - dtf) * (pi[sym(0, i) * N + c] + Po * u[0] * u[i]);
     fl[0] -= (tf - dtf) * Po;
   }
-  const float xm = (float)(-ph[5 * N + c] * dt / g.dx);
-  const float ym = (float)(-ph[6 * N + c] * dt / g.dy);
-  const float zm = (float)(-ph[7 * N + c] * dt / g.dz / (tau - 0.5 * dt));
+  const float xm = (float)(-vxd * dt / g.dx);
+  const float ym = (float)(-vyd * dt / g.dy);
+  const float zm = (float)(-vzd * dt / g.dz / (tau - 0.5 * dt));
   const float wx[2] = {1.f - fabsf(xm), fabsf(xm)}, wy[2] = {1.f - fabsf(ym), fabsf(ym)},
               wz[2] = {1.f - fabsf(zm), fabsf(zm)};
   const float wxH[2] = {1.f - 0.5f * fabsf(xm), 0.5f * fabsf(xm)},
               wyH[2] = {1.f - 0.5f * fabsf(ym), 0.5f * fabsf(ym)},
               wzH[2] = {1.f - 0.5f * fabsf(zm), 0.5f * fabsf(zm)};
-  const int sx = xm > 0.f ? 1 : (xm < 0.f ? -1 : 0);
-  const int sy = ym > 0.f ? 1 : (ym < 0.f ? -1 : 0);
-  const int sz = zm > 0.f ? 1 : (zm < 0.f ? -1 : 0);
   float a[10], aH[10], A = 0.f, AH = 0.f;
 #pragma unroll
   for (int k = 0; k < 10; k++) a[k] = aH[k] = 0.f;
 #pragma unroll
-  for (int jx = 0; jx < 2; jx++)
-#pragma unroll
-    for (int jy = 0; jy < 2; jy++)
-#pragma unroll
-      for (int jz = 0; jz < 2; jz++) {
-        const long c1 = g.cidx(ix + jx * sx, iy + jy * sy, iz + jz * sz);
-        const float w = wx[jx] * wy[jy] * wz[jz], wH = wxH[jx] * wyH[jy] * wzH[jz];
-        const float4 r0 = rec[c1], r1 = rec[N + c1], r2 = rec[2 * N + c1];
-        const float4 r3 = rec[3 * N + c1], r4 = rec[4 * N + c1], r5 = rec[5 * N + c1];
-        a[0] += w * r0.x; a[1] += w * r0.y; a[2] += w * r0.z; a[3] += w * r0.w;
-        a[4] += w * r1.x; a[5] += w * r1.y; a[6] += w * r1.z; a[7] += w * r1.w;
-        a[8] += w * r2.x; a[9] += w * r2.y; A += w * r2.z;
-        aH[0] += wH * r3.x; aH[1] += wH * r3.y; aH[2] += wH * r3.z; aH[3] += wH * r3.w;
-        aH[4] += wH * r4.x; aH[5] += wH * r4.y; aH[6] += wH * r4.z; aH[7] += wH * r4.w;
-        aH[8] += wH * r5.x; aH[9] += wH * r5.y; AH += wH * r5.z;
-      }
+  for (int j = 0; j < 8; j++) {
+    const int jx = j >> 2, jy = (j >> 1) & 1, jz = j & 1;
+    const float w = wx[jx] * wy[jy] * wz[jz], wH = wxH[jx] * wyH[jy] * wzH[jz];
+    const float4 r0 = R[j][0], r1 = R[j][1], r2 = R[j][2], r3 = R[j][3], r4 = R[j][4],
+                 r5 = R[j][5];
+    a[0] += w * r0.x; a[1] += w * r0.y; a[2] += w * r0.z; a[3] += w * r0.w;
+    a[4] += w * r1.x; a[5] += w * r1.y; a[6] += w * r1.z; a[7] += w * r1.w;
+    a[8] += w * r2.x; a[9] += w * r2.y; A += w * r2.z;
+    aH[0] += wH * r3.x; aH[1] += wH * r3.y; aH[2] += wH * r3.z; aH[3] += wH * r3.w;
+    aH[4] += wH * r4.x; aH[5] += wH * r4.y; aH[6] += wH * r4.z; aH[7] += wH * r4.w;
+    aH[8] += wH * r5.x; aH[9] += wH * r5.y; AH += wH * r5.z;
+  }
   const bool live = e > 0.f;
   const float maxT0 = fmax_((e + p) * g2 - p, (e + p) * v2 * g2 + p);
   float maxpi = 0.f;
