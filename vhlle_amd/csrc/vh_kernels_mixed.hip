@@ -1355,15 +1355,11 @@ k_visc2(Grids G, const double *__restrict__ ph, const float4 *__restrict__ rec,
 }
 
 // viscous interface flux in FP32 (Hydro::visc_flux, src/hdo.cpp:966-1014)
+// between cells l and r along axis D
 template <int D>
-__device__ __forceinline__ bool visc_face(const Geom &g, long l, long r,
-                                          const double *__restrict__ ph,
+__device__ __forceinline__ void visc_face(long N, long l, long r, const double *__restrict__ ph,
                                           const float *__restrict__ piH,
-                                          const float *__restrict__ PiH,
-                                          const double *__restrict__ m, int m_full, float pre,
-                                          float F[4]) {
-  const long N = g.P;
-  if (!m_full && m[(D - 1) * N + l] < 1. && m[(D - 1) * N + r] < 1.) return false;
+                                          const float *__restrict__ PiH, float pre, float F[4]) {
   const double dvx = 0.5 * (ph[5 * N + l] + ph[5 * N + r]);
   const double dvy = 0.5 * (ph[6 * N + l] + ph[6 * N + r]);
   const double dvz = 0.5 * (ph[7 * N + l] + ph[7 * N + r]);
@@ -1378,9 +1374,13 @@ __device__ __forceinline__ bool visc_face(const Geom &g, long l, long r,
     f += sP * u[mu] * u[D];
     F[mu] = f * pre;
   }
-  return true;
 }
 
+// The six faces are formed without branches: a face outside the grid takes
+// the cell itself as its neighbour (valid memory) and is masked off, like a
+// face whose two cells both have m < 1 (the reference's skip), so that all
+// the loads are issued before the flux arithmetic instead of one face (and
+// one m test) at a time.
 __global__ void __launch_bounds__(256) VH_OCC(VH_W_MX_VISC3)
 k_visc3(Grids G, const double *__restrict__ ph, const float *__restrict__ piH,
         const float *__restrict__ PiH, const float *__restrict__ vf,
@@ -1398,26 +1398,42 @@ k_visc3(Grids G, const double *__restrict__ ph, const float *__restrict__ piH,
   const double pre0 = (tau - 0.5 * dt) * dt;
   const float px = (float)(pre0 / g.dx), py = (float)(pre0 / g.dy),
               pz = (float)(pre0 / (g.dz * (tau + 0.5 * dt)));
-  float F[4];
   const long sy = g.nx, sz = (long)g.nx * g.ny;
-  if (ix > 0 && visc_face<1>(g, c - 1, c, ph, piH, PiH, m, m_full, px, F))
+  // face k: axis 1 + k / 2, lower (even k: neighbour, c) or upper (c, neighbour)
+  const bool in[6] = {ix > 0, ix < g.nx - 1, iy > 0, iy < g.ny - 1, iz > 0, iz < g.nz - 1};
+  const long step[6] = {-1, 1, -sy, sy, -sz, sz};
+  bool use[6];
+  long nbr[6];
 #pragma unroll
-    for (int k = 0; k < 4; k++) acc[k] += F[k];
-  if (ix < g.nx - 1 && visc_face<1>(g, c, c + 1, ph, piH, PiH, m, m_full, px, F))
+  for (int k = 0; k < 6; k++) {
+    nbr[k] = in[k] ? c + step[k] : c;
+    use[k] = in[k];
+  }
+  if (!m_full) {
 #pragma unroll
-    for (int k = 0; k < 4; k++) acc[k] -= F[k];
-  if (iy > 0 && visc_face<2>(g, c - sy, c, ph, piH, PiH, m, m_full, py, F))
+    for (int k = 0; k < 6; k++) {
+      const double *md = m + (k / 2) * N;
+      const double ma = md[nbr[k]], mb = md[c];
+      use[k] = use[k] & !((ma < 1.) & (mb < 1.));
+    }
+  }
+  float F[6][4];
+  visc_face<1>(N, nbr[0], c, ph, piH, PiH, px, F[0]);
+  visc_face<1>(N, c, nbr[1], ph, piH, PiH, px, F[1]);
+  visc_face<2>(N, nbr[2], c, ph, piH, PiH, py, F[2]);
+  visc_face<2>(N, c, nbr[3], ph, piH, PiH, py, F[3]);
+  visc_face<3>(N, nbr[4], c, ph, piH, PiH, pz, F[4]);
+  visc_face<3>(N, c, nbr[5], ph, piH, PiH, pz, F[5]);
 #pragma unroll
-    for (int k = 0; k < 4; k++) acc[k] += F[k];
-  if (iy < g.ny - 1 && visc_face<2>(g, c, c + sy, ph, piH, PiH, m, m_full, py, F))
+  for (int k = 0; k < 6; k++)
 #pragma unroll
-    for (int k = 0; k < 4; k++) acc[k] -= F[k];
-  if (iz > 0 && visc_face<3>(g, c - sz, c, ph, piH, PiH, m, m_full, pz, F))
-#pragma unroll
-    for (int k = 0; k < 4; k++) acc[k] += F[k];
-  if (iz < g.nz - 1 && visc_face<3>(g, c, c + sz, ph, piH, PiH, m, m_full, pz, F))
-#pragma unroll
-    for (int k = 0; k < 4; k++) acc[k] -= F[k];
+    for (int q = 0; q < 4; q++) {
+      const float f = use[k] ? F[k][q] : 0.f;
+      if (k & 1)
+        acc[q] -= f;
+      else
+        acc[q] += f;
+    }
   if (ph[c] > 0.) {
     const double dvx = ph[5 * N + c], dvy = ph[6 * N + c], dvz = ph[7 * N + c];
     const float u0 = (float)(1. / sqrt(1. - dvx * dvx - dvy * dvy - dvz * dvz)),
