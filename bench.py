@@ -98,36 +98,36 @@ ALG_BYTES = {
 # shape, con2prim); None where not profiled.
 KIB = 1024
 TRAFFIC = {
-    ("fp64", "visc_relax", None, "reference"): (2 * 536501 + 843248) * KIB,
-    ("fp64", "visc_advect", None, "reference"): (2 * 550075 + 843750) * KIB,
-    ("fp64", "visc_flux_update", None, "reference"): (2 * 544709 + 218781) * KIB,
-    ("mixed", "stage_pred", "cells", "closed"): (2 * 93176 + 130859) * KIB,
-    ("mixed", "stage_corr", "cells", "closed"): (2 * 155869 + 130859) * KIB,
-    ("mixed", "stage_pred", "cells", "newton"): (2 * 93179 + 130859) * KIB,
-    ("mixed", "stage_corr", "cells", "newton"): (2 * 155871 + 130859) * KIB,
-    ("mixed", "stage_pred", "cells", "learned"): (2 * 101129 + 130859) * KIB,
-    ("mixed", "stage_corr", "cells", "learned"): (2 * 165840 + 130859) * KIB,
-    ("mixed", "stage_pred", "rows", "closed"): (2 * 104874 + 137782) * KIB,
-    ("mixed", "stage_corr", "rows", "closed"): (2 * 209914 + 136216) * KIB,
-    ("mixed", "stage_pred", "rows", "newton"): (2 * 104757 + 137837) * KIB,
-    ("mixed", "stage_corr", "rows", "newton"): (2 * 209103 + 136391) * KIB,
-    ("mixed", "stage_pred", "rows", "learned"): (2 * 104818 + 137815) * KIB,
-    ("mixed", "stage_corr", "rows", "learned"): (2 * 235516 + 136097) * KIB,
-    ("mixed", "stage_pred", "bands", "closed"): (2 * 423035 + 135078) * KIB,
-    ("mixed", "stage_corr", "bands", "closed"): (2 * 517782 + 134676) * KIB,
-    ("mixed", "stage_pred", "bands", "newton"): (2 * 427191 + 135047) * KIB,
-    ("mixed", "stage_corr", "bands", "newton"): (2 * 517296 + 134701) * KIB,
-    ("mixed", "stage_pred", "bands", "learned"): (2 * 480048 + 135445) * KIB,
-    ("mixed", "stage_corr", "bands", "learned"): (2 * 582918 + 134369) * KIB,
-    ("mixed", "visc_relax", None, "closed"): (2 * 352326 + 375000) * KIB,
-    ("mixed", "visc_relax", None, "newton"): (2 * 352267 + 375000) * KIB,
-    ("mixed", "visc_relax", None, "learned"): (2 * 360090 + 375000) * KIB,
-    ("mixed", "visc_advect", None, None): (2 * 330191 + 437500) * KIB,
-    ("mixed", "visc_flux_update", None, None): (2 * 263348 + 125000) * KIB,
-    ("mixed", "visc_prim", None, "closed"): (2 * 62530 + 281250) * KIB,
-    ("mixed", "visc_prim", None, "newton"): (2 * 62561 + 281250) * KIB,
-    ("mixed", "visc_prim", None, "learned"): (2 * 70372 + 281250) * KIB,
-    ("mixed", "c2p_guess", None, "learned"): (2 * 62617 + 15625) * KIB,
+    ("fp64", "visc_relax", None, "reference"): (2 * 536633 + 843226) * KIB,
+    ("fp64", "visc_advect", None, "reference"): (2 * 550046 + 843750) * KIB,
+    ("fp64", "visc_flux_update", None, "reference"): (2 * 544937 + 218782) * KIB,
+    ("mixed", "stage_pred", "cells", "closed"): (2 * 93173 + 130859) * KIB,
+    ("mixed", "stage_corr", "cells", "closed"): (2 * 155871 + 130859) * KIB,
+    ("mixed", "stage_pred", "cells", "newton"): (2 * 93182 + 130859) * KIB,
+    ("mixed", "stage_corr", "cells", "newton"): (2 * 155881 + 130859) * KIB,
+    ("mixed", "stage_pred", "cells", "learned"): (2 * 101135 + 130859) * KIB,
+    ("mixed", "stage_corr", "cells", "learned"): (2 * 165839 + 130859) * KIB,
+    ("mixed", "stage_pred", "rows", "closed"): (2 * 104965 + 137743) * KIB,
+    ("mixed", "stage_corr", "rows", "closed"): (2 * 209804 + 136193) * KIB,
+    ("mixed", "stage_pred", "rows", "newton"): (2 * 104800 + 137863) * KIB,
+    ("mixed", "stage_corr", "rows", "newton"): (2 * 209354 + 136380) * KIB,
+    ("mixed", "stage_pred", "rows", "learned"): (2 * 104832 + 137756) * KIB,
+    ("mixed", "stage_corr", "rows", "learned"): (2 * 235317 + 136074) * KIB,
+    ("mixed", "stage_pred", "bands", "closed"): (2 * 423052 + 135446) * KIB,
+    ("mixed", "stage_corr", "bands", "closed"): (2 * 517915 + 135043) * KIB,
+    ("mixed", "stage_pred", "bands", "newton"): (2 * 426944 + 134767) * KIB,
+    ("mixed", "stage_corr", "bands", "newton"): (2 * 516415 + 134492) * KIB,
+    ("mixed", "stage_pred", "bands", "learned"): (2 * 479110 + 135489) * KIB,
+    ("mixed", "stage_corr", "bands", "learned"): (2 * 582564 + 134406) * KIB,
+    ("mixed", "visc_relax", None, "closed"): (2 * 352331 + 375000) * KIB,
+    ("mixed", "visc_relax", None, "newton"): (2 * 352360 + 375001) * KIB,
+    ("mixed", "visc_relax", None, "learned"): (2 * 360216 + 375003) * KIB,
+    ("mixed", "visc_advect", None, None): (2 * 330176 + 437500) * KIB,
+    ("mixed", "visc_flux_update", None, None): (2 * 257136 + 125000) * KIB,
+    ("mixed", "visc_prim", None, "closed"): (2 * 62522 + 281250) * KIB,
+    ("mixed", "visc_prim", None, "newton"): (2 * 62560 + 281250) * KIB,
+    ("mixed", "visc_prim", None, "learned"): (2 * 70365 + 281250) * KIB,
+    ("mixed", "c2p_guess", None, "learned"): (2 * 62619 + 15625) * KIB,
 }
 
 
