@@ -11,8 +11,6 @@ construction and Fluid::outputSurface's element records).
 * Mirroring a hypercube mirrors its element.
 * Two diagonally opposite corners above the level give two elements.
 * An element record of a uniform state carries that state (u, T, dsigma).
-* The record's pi and Pi columns reproduce fields linear in (x, y, eta) at
-  the centroid, pi boosted from Milne to Cartesian components.
 * The corona (Fluid::outputCorona) of a uniform static state below e_crit is
   every interior cube, each in closed form (centre, dsigma, u, T, dV_eff,
   EtotSurf); centres below e = 0.01 and cubes touching a hot cell are left out.
@@ -159,63 +157,6 @@ def test_record_of_a_uniform_flow():
         if abs(r[3]) < 1e-12:  # eta = 0: lab frame = Milne frame
             g = 1 / np.sqrt(1 - vx * vx)
             assert np.allclose(r[8:12], [g, g * vx, 0, 0], atol=1e-12)
-
-
-def _sym(i, j):
-    """Packed index of a symmetric 4x4 tensor (src/cll.cpp:22-31)."""
-    i, j = max(i, j), min(i, j)
-    return i * (i + 1) // 2 + j
-
-
-def test_record_pi_columns_reproduce_a_linear_field():
-    """Trilinear interpolation reproduces a linear field exactly.  With
-    pi^{mu nu} and Pi linear in (x, y, eta) (random coefficients), the record's
-    Pi column is the field at the element's centroid (columns 1-3), and its pi
-    columns are the Milne components there boosted to Cartesian ones at the
-    centroid's eta: Lambda pi Lambda^T with the boost by eta in the (tau, eta)
-    block, formed here as a matrix product (src/fld.cpp:700-745).  A swapped
-    weight, axis or corner, or a wrong term of the boost, breaks the
-    equality."""
-    nx = ny = nz = 6
-    N = nx * ny * nz
-    tau, dt = 1.2, 0.05
-    dx, dy, dz = 0.1, 0.12, 0.3
-    xmin, ymin, zmin = -0.25, -0.3, -0.75
-    X = xmin + np.arange(nx) * dx
-    Y = ymin + np.arange(ny) * dy
-    Z = zmin + np.arange(nz) * dz
-    Zg, Yg, Xg = np.meshgrid(Z, Y, X, indexing="ij")  # cell order x fastest
-    xs, ys, zs = Xg.ravel(), Yg.ravel(), Zg.ravel()
-    # a tilted level plane off the cell midpoints, so that the centroids'
-    # offsets differ from axis to axis
-    e = 0.5 + 0.5 * (xs - 0.013) + 0.2 * (ys - 0.021) + 0.1 * (zs - 0.057)
-    prim = np.zeros((N, 8))
-    prim[:, 0] = e
-    prim[:, 1] = e / 3
-    prim[:, 5] = 0.2
-    Q = (po.p2c(prim) * tau).T.copy()
-    Qlow = (po.p2c(prim) * (tau - dt)).T.copy()
-    rng = np.random.default_rng(17)
-    A, B = rng.normal(size=10), rng.normal(size=(10, 3))
-    a, b = rng.normal(), rng.normal(size=3)
-    pos = np.stack([xs, ys, zs])
-    pi = A[:, None] + B @ pos
-    Pi = a + b @ pos
-    rows = po.freezeout((nx, ny, nz), (dx, dy, dz), (xmin, ymin, zmin), Q, Qlow, pi, Pi,
-                        tau, dt, 0.5)
-    assert len(rows) >= 4 and np.ptp(rows[:, 3]) > 0.2  # several eta, boosts != 1
-    off = (rows[:, 1:4] - np.array([xmin, ymin, zmin])) / np.array([dx, dy, dz]) % 1.0
-    assert np.max(np.abs(off[:, 0] - off[:, 1])) > 0.05  # x and y weights tell apart
-    for r in rows:
-        c = r[1:4]
-        assert abs(r[26] - (a + b @ c)) < 1e-12
-        pm = A + B @ c
-        M = np.array([[pm[_sym(i, j)] for j in range(4)] for i in range(4)])
-        ch, sh = np.cosh(r[3]), np.sinh(r[3])
-        L = np.array([[ch, 0, 0, sh], [0, 1, 0, 0], [0, 0, 1, 0], [sh, 0, 0, ch]])
-        C = L @ M @ L.T
-        got = np.array([[r[16 + _sym(i, j)] for j in range(4)] for i in range(4)])
-        assert np.allclose(got, C, rtol=0, atol=1e-11 * np.max(np.abs(C)))
 
 
 def _uniform_Q(shape, e, tau):
