@@ -13,6 +13,13 @@ forms:
   theta = 1/tau, zeta = (zeta/s) s / 5.068, and relaxation time
   tau_Pi = 6 zeta / (5.068 s T) (src/trancoeff.cpp:96).
 
+* the charge-dependent eta/s (etaSparam 3, src/trancoeff.cpp:51-62) with the
+  chiral EoS: the same shear flow at uniform n_b drives pi^{xy} to -eta d_y u^x
+  with eta/s evaluated at the cell's T, mu_B, s and p, taken here from the EoS
+  lookup at the cell's (e, n_b); with etaSScaleMuB = 3 mu_B raises eta/s by
+  22 %, so NSquant's eta/s call with a mu_B of 0, or with mu_Q, fails
+  (planted in a copy of the oracle on the CPU).
+
 The residual differences are the second-order terms and the lag behind a
 slowly moving target, O(tau_relax / tau); the tolerances leave room for them
 and no more (a wrong sign or a factor 2 fails by far).
@@ -81,3 +88,37 @@ def test_bulk_navier_stokes_limit_in_bjorken_flow():
           flush=True)
     assert Pi / want == pytest.approx(1.0, abs=0.06)
     assert taubulk < 0.03 * tau
+
+
+def test_charged_eta_over_s_navier_stokes_limit():
+    import charged_eos as ce
+    tab = ce.chiral("smooth")
+    nx, ny = 9, 21
+    xl, yl = (-0.8, 0.8), (-2.0, 2.0)
+    X, Y, _ = setups.axes(nx, ny, 1, xl, yl, (-1, 1))
+    kappa = 0.02
+    e = np.full(X.shape, 200.0)
+    z = np.zeros_like(e)
+    trc = po.make_trc(etaSparam=3, etaSMin=0.08, ah=0.0, al=0.0, T0=0.15, etaSShiftMuB=0.0,
+                      etaSScaleMuB=3.0)
+    s = po.Solver(nx, ny, 1, xl, yl, (-1, 1), trc=trc, table=tab)
+    s.set_prim(2.0, e, kappa * Y, z, z, nb=np.full(X.shape, 5.0))
+    rc, tau = s.run(2.0, 0.02, 100)
+    assert rc == 0
+    st = s.get_state()
+    pr = po.c2p((st["Q"] / tau).T.copy(), tab)[0]
+    ee, nb = pr[:, 0], pr[:, 2]
+    ux = (pr[:, 5] / np.sqrt(1.0 - pr[:, 5] ** 2)).reshape(ny, nx)
+    dux = np.gradient(ux, (yl[1] - yl[0]) / (ny - 1), axis=0)
+    ev = po.eos_eval(tab, ee, nb)
+    p, T, mub = ev[:, 0], ev[:, 1], ev[:, 2]
+    sden = (ee + p - mub * nb) / T  # EoS::s with n_q = n_s = 0
+    etaS = po.trcoeff(trc, np.stack([ee, nb, T, mub, sden, p], axis=1), tab)[:, 0]
+    etaS0 = po.trcoeff(trc, np.stack([ee, nb, T, 0 * mub, sden, p], axis=1), tab)[:, 0]
+    assert np.min(etaS / etaS0) > 1.15  # mu_B matters here
+    eta = (etaS * sden / 5.068).reshape(ny, nx)
+    pixy = st["pi"][4].reshape(ny, nx)  # index44(1, 2) = 4
+    rows = slice(ny // 2 - 5, ny // 2 + 6)
+    rel = np.abs(pixy[rows, 2:-2] / (-eta * dux)[rows, 2:-2] - 1.0)
+    print(f"charged: pi^xy / (-eta d_y u^x) - 1: max {rel.max():.3e}", flush=True)
+    assert rel.max() < 0.05
