@@ -528,3 +528,41 @@ def test_freezeout_pi_columns_reproduce_linear_fields():
         want = np.array([C[i, j] for i in range(4) for j in range(i + 1)])
         np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
         assert abs(r[26] - lin(10, xc, yc, ec)) <= 1e-12 * (1 + abs(r[26]))
+
+
+def _uniform_milne_run(vx, vy, vz, n=6, steps=40):
+    N = n ** 3
+    s = po.Solver(n, n, n, (-0.5, 0.5), (-0.5, 0.5), (-0.5, 0.5), trc=po.make_trc(etaS=0.2))
+    s.set_prim(0.6, np.full(N, 10.0), np.full(N, vx), np.full(N, vy), np.full(N, vz))
+    rc, tau = s.run(0.6, 0.01, steps)
+    assert rc == 0
+    st = s.get_state()
+    c = N // 2
+    P = np.zeros((4, 4))
+    for i in range(4):
+        for j in range(i + 1):
+            P[i, j] = P[j, i] = st["pi"][i * (i + 1) // 2 + j, c]
+    pr = po.c2p((st["Q"][:, c] / tau)[None, :])[0][0]
+    v = pr[5:8]
+    g = 1.0 / np.sqrt(1.0 - v @ v)
+    return P, np.array([g, g * v[0], g * v[1], g * v[2]])
+
+
+def test_single_christoffel_terms_by_mirror():
+    """The single off-diagonal Christoffel terms of the relaxation
+    (src/hdo.cpp:785-788, 838-841: pi^{x tau} and pi^{y tau} fed by
+    v_eta pi^{x eta}, pi^{y eta}; pi^{eta x}, pi^{eta y} by v_eta pi^{tau x},
+    pi^{tau y}).  A uniform flow and its x <-> y mirror give mirrored pi and u
+    to 1e-12, which fixes the x terms against the y terms.  Planted in copies
+    of the oracle on the CPU: any one of the four terms dropped, or pi^{tau x}
+    read in place of pi^{tau y}, fails it (pi off by 2e-3 and 9e-3 of
+    max |pi|); the eta-x and eta-y pair dropped or flipped together fails the
+    transversality test above; the tau-x and tau-y pair dropped or flipped
+    together passes both (unpinned)."""
+    Pa, ua = _uniform_milne_run(0.4, 0.15, 0.5)
+    Pb, ub = _uniform_milne_run(0.15, 0.4, 0.5)
+    perm = [0, 2, 1, 3]
+    scale = np.max(np.abs(Pa))
+    assert scale > 0.3 and abs(Pa[1, 3]) > 0.02 * scale and abs(Pa[0, 2]) > 0.02 * scale
+    assert np.max(np.abs(Pa - Pb[np.ix_(perm, perm)])) <= 1e-12 * scale
+    assert np.max(np.abs(ua - ub[perm])) <= 1e-12
