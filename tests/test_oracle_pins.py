@@ -558,7 +558,7 @@ def test_single_christoffel_terms_by_mirror():
     read in place of pi^{tau y}, fails it (pi off by 2e-3 and 9e-3 of
     max |pi|); the eta-x and eta-y pair dropped or flipped together fails the
     transversality test above; the tau-x and tau-y pair dropped or flipped
-    together passes both (unpinned)."""
+    together passes both and fails the boosted rest frame below."""
     Pa, ua = _uniform_milne_run(0.4, 0.15, 0.5)
     Pb, ub = _uniform_milne_run(0.15, 0.4, 0.5)
     perm = [0, 2, 1, 3]
@@ -566,3 +566,51 @@ def test_single_christoffel_terms_by_mirror():
     assert scale > 0.3 and abs(Pa[1, 3]) > 0.02 * scale and abs(Pa[0, 2]) > 0.02 * scale
     assert np.max(np.abs(Pa - Pb[np.ix_(perm, perm)])) <= 1e-12 * scale
     assert np.max(np.abs(ua - ub[perm])) <= 1e-12
+
+
+def _boosted_rest_frame_pi(nz, dt, t_run=0.3, tau0=1.0, a=0.01, etas=0.2, e0=10.0):
+    """A fluid at rest in Cartesian coordinates, seen on an eta grid
+    (v_eta = -tanh eta, u^tau = cosh eta), carrying a homogeneous Cartesian
+    pi^{xz} that relaxes as A exp(-t/tau_pi) along the proper time
+    t = tau cosh eta.  Returns the largest deviations of pi^{tau x} from
+    -A(t) sinh eta and of tau pi^{eta x} from A(t) cosh eta at |eta| <= 0.4,
+    relative to max A."""
+    eta = np.linspace(-1.0, 1.0, nz)
+    cnst = (16 + 0.5 * 21.0 * 2.5) * np.pi ** 2 / 30.0 / 0.197326968 ** 3  # T = (e/cnst)^(1/4)
+    taupi = 5 * etas / 5.068 / (e0 / cnst) ** 0.25  # 5 eta/(e + p), p = e/3
+    s = po.Solver(1, 1, nz, (-1, 1), (-1, 1), (-1.0, 1.0), trc=po.make_trc(etaS=etas))
+    z = np.zeros(nz)
+    s.set_prim(tau0, np.full(nz, e0), z, z, -np.tanh(eta))
+    A = a * (e0 / 3) * np.exp(-tau0 * np.cosh(eta) / taupi)
+    pi = np.zeros((10, nz))
+    pi[1], pi[7] = -A * np.sinh(eta), A * np.cosh(eta)  # index44(1,0), (3,1)
+    s.set_state({"pi": pi, "piH": pi.copy()})
+    rc, tau = s.run(tau0, dt, int(round(t_run / dt)))
+    assert rc == 0
+    st = s.get_state()
+    A = a * (e0 / 3) * np.exp(-tau * np.cosh(eta) / taupi)
+    inner = np.abs(eta) <= 0.41
+    d1 = np.max(np.abs(st["pi"][1] + A * np.sinh(eta))[inner]) / np.max(A)
+    d7 = np.max(np.abs(st["pi"][7] - A * np.cosh(eta))[inner]) / np.max(A)
+    return d1, d7
+
+
+def test_christoffel_terms_by_a_boosted_rest_frame():
+    """The tau-x and eta-x Christoffel terms of the relaxation
+    (src/hdo.cpp:779-788, 832-841) against the exact solution of a boosted
+    rest frame: at a fixed eta the Cartesian pi^{xz} appears as
+    pi^{tau x} = -A sinh eta and tau pi^{eta x} = A cosh eta, whose changes
+    along tau come from the relaxation at gamma = cosh eta, the upwind
+    advection with v_eta = -tanh eta and the v_eta / tau terms together.  The
+    oracle lies 3.7e-3 and 5.0e-3 of max A away at deta = 0.05, dtau = 0.01,
+    and 2.0e-3 at half of both (first order).  The tau-x and tau-y pair of
+    terms dropped or flipped together (planted in copies of the oracle on the
+    CPU, invisible to the mirror and transversality pins) ends 0.10 and 0.21
+    away.  Q holds the static fluid's ideal T^{tau mu} only, so the fluid
+    stays at rest (the reference's con2prim would read a pi^{tau x} in Q as
+    flow)."""
+    d1, d7 = _boosted_rest_frame_pi(41, 0.01)
+    print(f"boosted rest frame: pi^tx {d1:.3e}, pi^etax {d7:.3e}", flush=True)
+    assert d1 <= 0.01 and d7 <= 0.01
+    h1, h7 = _boosted_rest_frame_pi(81, 0.005)
+    assert h1 <= 0.7 * d1 and h7 <= 0.7 * d7
